@@ -1,0 +1,298 @@
+"""Synthetic fiber scenes for the implicit mechanics step.
+
+A scene is the input one `Simul::solve` hands to Meca: the Mecables (fibers)
+in registration order, their points, the interactions the emitters register
+(sphere confinement from `Fiber::setInteractions`, crosslinker couples from
+`Couple::setInteractions`), the simulation properties and, per step, the
+N(0,1) Brownian draws.  Generation follows the reference's own placement rules:
+
+* fiber length -> number of points by `FiberNaked::bestNbPoints`
+  (fiber_naked.cc:22-30) and segment length fnCut = length/(points-1)
+  (fiber_naked.cc:79-94);
+* straight fibers centred on a random position inside the space with a random
+  orientation (`setStraight(pos, dir, CENTER)`, fiber_naked.cc:57-76);
+* variable lengths uniform in [min, max] (fiber_prop.cc:110-118 style);
+* crosslinkers bound at uniformly random abscissas on pairs of fibers whose
+  distance is within the binding range, attached to the nearest point of the
+  partner fiber (couple_set.cc:185-235 attach semantics; an equilibrium
+  binding state without running the Monte-Carlo stepping, which is out of
+  scope).
+
+Everything here is host-side NumPy; nothing in this module touches the GPU.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+def best_nb_points(ratio):
+    """FiberNaked::bestNbPoints (fiber_naked.cc:22-30)."""
+    n = int(ratio)
+    if (2 * n + 1) * ratio > 2 * n * (n + 1):
+        return n + 2
+    return n + 1
+
+
+@dataclass
+class Scene:
+    dim: int
+    time_step: float
+    kT: float
+    viscosity: float
+    hydro_radius0: float
+    hydro_radius1: float
+    nb_points: np.ndarray          # uint32 per fiber, registration order
+    segmentation: np.ndarray       # fnCut per fiber
+    rigidity: np.ndarray           # FiberProp::rigidity per fiber
+    points: np.ndarray             # float64, DIM*sum(nb_points), matIndex order
+    confine_radius: float = 0.0    # SpaceSphere radius (0: no confinement)
+    confine_stiffness: float = 0.0
+    # couples (Couple::setInteractions -> interLink(PI, PI)), call order
+    link_fiber_a: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    link_abs_a: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    link_fiber_b: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    link_abs_b: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    link_stiffness: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    seed: int = 0
+    name: str = ""
+
+    @property
+    def nfib(self):
+        return int(self.nb_points.size)
+
+    @property
+    def npts(self):
+        return int(self.nb_points.sum())
+
+    @property
+    def size(self):
+        return self.dim * self.npts
+
+    @property
+    def mat_index(self):
+        idx = np.zeros(self.nfib, dtype=np.uint32)
+        np.cumsum(self.nb_points[:-1], out=idx[1:])
+        return idx
+
+    @property
+    def nlinks(self):
+        return int(self.link_fiber_a.size)
+
+    def params(self):
+        return dict(time_step=self.time_step, kT=self.kT, viscosity=self.viscosity,
+                    hydro_radius0=self.hydro_radius0, hydro_radius1=self.hydro_radius1)
+
+    def interpolate(self, fiber, abscissa):
+        """FiberNaked::interpolate (fiber_naked.cc:1050-1065), vectorised;
+        returns (mat1, mat2, coef)."""
+        fiber = np.asarray(fiber, dtype=np.int64)
+        ab = np.asarray(abscissa, dtype=np.float64)
+        cut = self.segmentation[fiber]
+        idx = self.mat_index.astype(np.int64)[fiber]
+        last = self.nb_points.astype(np.int64)[fiber] - 1
+        q = ab / cut
+        n = np.floor(q)
+        co = q - n
+        rd = n.astype(np.int64)
+        m1 = idx + rd
+        coef = co.copy()
+        beyond = rd >= last
+        m1[beyond] = idx[beyond] + last[beyond] - 1
+        coef[beyond] = 1.0
+        neg = ab <= 0
+        m1[neg] = idx[neg]
+        coef[neg] = 0.0
+        return m1.astype(np.uint32), (m1 + 1).astype(np.uint32), coef
+
+    def gauss(self, step):
+        """N(0,1) draws for one step (one per degree of freedom, object order)."""
+        rng = np.random.default_rng([self.seed, 7919, step])
+        return rng.standard_normal(self.size)
+
+
+def _random_directions(rng, n, dim):
+    if dim == 2:
+        a = rng.uniform(0, 2 * np.pi, n)
+        return np.stack([np.cos(a), np.sin(a)], axis=1)
+    v = rng.standard_normal((n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def _random_in_ball(rng, n, dim, radius):
+    out = np.empty((0, dim))
+    while out.shape[0] < n:
+        c = rng.uniform(-radius, radius, (2 * n + 16, dim))
+        c = c[(c * c).sum(axis=1) <= radius * radius]
+        out = np.concatenate([out, c])
+    return out[:n]
+
+
+def _straight_fibers(rng, dim, lengths, seg, radius):
+    nfib = lengths.size
+    nbp = np.array([best_nb_points(L / seg) for L in lengths], dtype=np.uint32)
+    cut = lengths / (nbp - 1)
+    centers = _random_in_ball(rng, nfib, dim, radius)
+    dirs = _random_directions(rng, nfib, dim)
+    pts = []
+    for f in range(nfib):
+        start = centers[f] - 0.5 * dirs[f] * lengths[f]
+        dp = dirs[f] * cut[f]
+        p = np.arange(nbp[f])[:, None] * dp[None, :] + start[None, :]
+        pts.append(p.ravel())
+    return nbp, cut, np.concatenate(pts)
+
+
+def _segment_segment(p0, d0, p1, d1):
+    """Closest points between segments p0+s*d0 and p1+t*d1 (s,t in [0,1]),
+    vectorised; returns s, t, distance."""
+    r = p0 - p1
+    a = (d0 * d0).sum(1)
+    e = (d1 * d1).sum(1)
+    b = (d0 * d1).sum(1)
+    c = (d0 * r).sum(1)
+    f = (d1 * r).sum(1)
+    den = a * e - b * b
+    s = np.where(den > 1e-14, np.clip((b * f - c * e) / np.where(den > 1e-14, den, 1), 0, 1), 0.0)
+    t = (b * s + f) / e
+    t0 = t < 0
+    t1 = t > 1
+    t = np.clip(t, 0, 1)
+    s = np.where(t0, np.clip(-c / a, 0, 1), s)
+    s = np.where(t1, np.clip((b - c) / a, 0, 1), s)
+    q0 = p0 + s[:, None] * d0
+    q1 = p1 + t[:, None] * d1
+    return s, t, np.linalg.norm(q0 - q1, axis=1)
+
+
+def _close_segment_pairs(dim, nbp, points, rng_range):
+    """All pairs of segments of different fibers closer than rng_range."""
+    P = points.reshape(-1, dim)
+    nbp = nbp.astype(np.int64)
+    fib_of_pt = np.repeat(np.arange(nbp.size), nbp)
+    starts = np.concatenate([[0], np.cumsum(nbp)[:-1]]).astype(np.int64)
+    seg_pt = np.concatenate([s + np.arange(n - 1) for s, n in zip(starts, nbp)])
+    seg_fib = fib_of_pt[seg_pt]
+    A = P[seg_pt]
+    B = P[seg_pt + 1]
+    mid = 0.5 * (A + B)
+    half = 0.5 * np.linalg.norm(B - A, axis=1).max()
+    cell = 2 * half + rng_range
+    key = np.floor(mid / cell).astype(np.int64)
+    key -= key.min(axis=0)
+    dims = key.max(axis=0) + 3
+    def lin(k):
+        out = np.zeros(k.shape[0], dtype=np.int64)
+        for d in range(dim):
+            out = out * dims[d] + (k[:, d] + 1)
+        return out
+    lk = lin(key)
+    order = np.argsort(lk, kind="stable")
+    slk = lk[order]
+    pairs_i, pairs_j = [], []
+    offs = np.array(np.meshgrid(*[[-1, 0, 1]] * dim, indexing="ij")).reshape(dim, -1).T
+    for off in offs:
+        nk = lin(key + off)
+        lo = np.searchsorted(slk, nk, side="left")
+        hi = np.searchsorted(slk, nk, side="right")
+        cnt = hi - lo
+        ii = np.repeat(np.arange(lk.size), cnt)
+        jj = order[np.concatenate([np.arange(l, h) for l, h in zip(lo, hi)])] if cnt.sum() else np.zeros(0, np.int64)
+        keep = (ii < jj) & (seg_fib[ii] != seg_fib[jj])
+        pairs_i.append(ii[keep])
+        pairs_j.append(jj[keep])
+    ii = np.concatenate(pairs_i)
+    jj = np.concatenate(pairs_j)
+    s, t, dist = _segment_segment(A[ii], B[ii] - A[ii], A[jj], B[jj] - A[jj])
+    keep = dist <= rng_range
+    return seg_pt, seg_fib, ii[keep], jj[keep]
+
+
+def _bind_crosslinkers(rng, dim, nbp, cut, points, nlinks, binding_range):
+    """Bind `nlinks` couples: hand 1 at a uniform abscissa on a fiber, hand 2 at
+    the nearest point of another fiber within `binding_range`; attempts that
+    find no partner are rejected (conditioned on segments with a partner)."""
+    P = points.reshape(-1, dim)
+    seg_pt, seg_fib, ci, cj = _close_segment_pairs(dim, nbp, points, binding_range)
+    nbp = nbp.astype(np.int64)
+    starts = np.concatenate([[0], np.cumsum(nbp)[:-1]]).astype(np.int64)
+    if ci.size == 0:
+        raise ValueError("no fiber crossings within binding range")
+    # each close pair seen from both sides
+    a_seg = np.concatenate([ci, cj])
+    b_seg = np.concatenate([cj, ci])
+    cand = np.unique(a_seg)
+    fa, aa, fb, ab = [], [], [], []
+    while len(fa) < nlinks:
+        m = 4 * (nlinks - len(fa)) + 64
+        sa = cand[rng.integers(0, cand.size, m)]
+        u = rng.random(m)
+        pa = P[seg_pt[sa]] + u[:, None] * (P[seg_pt[sa] + 1] - P[seg_pt[sa]])
+        for k in range(m):
+            partners = b_seg[a_seg == sa[k]]
+            q0 = P[seg_pt[partners]]
+            d = P[seg_pt[partners] + 1] - q0
+            t = np.clip(((pa[k] - q0) * d).sum(1) / (d * d).sum(1), 0, 1)
+            dist = np.linalg.norm(q0 + t[:, None] * d - pa[k], axis=1)
+            j = int(np.argmin(dist))
+            if dist[j] <= binding_range:
+                f1 = int(seg_fib[sa[k]])
+                f2 = int(seg_fib[partners[j]])
+                s1 = seg_pt[sa[k]] - starts[f1]
+                s2 = seg_pt[partners[j]] - starts[f2]
+                fa.append(f1)
+                aa.append((s1 + u[k]) * cut[f1])
+                fb.append(f2)
+                ab.append((s2 + t[j]) * cut[f2])
+                if len(fa) == nlinks:
+                    break
+    return (np.array(fa, np.uint32), np.array(aa), np.array(fb, np.uint32), np.array(ab))
+
+
+def config1(seed=1):
+    """2D, 20 fibers of length 11 (23 points), rigidity 20, inside a sphere of
+    radius 5 (stiffness 200), viscosity 0.02, time_step 0.005, kT 0.0042."""
+    rng = np.random.default_rng(seed)
+    lengths = np.full(20, 11.0)
+    nbp, cut, pts = _straight_fibers(rng, 2, lengths, 0.5, 5.0)
+    return Scene(dim=2, time_step=0.005, kT=0.0042, viscosity=0.02, hydro_radius0=0.0125,
+                 hydro_radius1=10.0, nb_points=nbp, segmentation=cut,
+                 rigidity=np.full(20, 20.0), points=pts, confine_radius=5.0,
+                 confine_stiffness=200.0, seed=seed, name="fibers-2d-20x23pt-sphere5")
+
+
+def config2(seed=2, nfib=1000, nlinks=20000, radius=10.0, lmin=2.0, lmax=15.0):
+    """3D, `nfib` fibers with lengths uniform in [lmin, lmax] (segmentation 0.5),
+    rigidity 20, inside a sphere (stiffness 200), `nlinks` passive crosslinkers
+    (stiffness 100, binding range 0.05), viscosity 0.02, time_step 0.005,
+    kT 0.0042."""
+    rng = np.random.default_rng(seed)
+    lengths = rng.uniform(lmin, lmax, nfib)
+    nbp, cut, pts = _straight_fibers(rng, 3, lengths, 0.5, radius)
+    sc = Scene(dim=3, time_step=0.005, kT=0.0042, viscosity=0.02, hydro_radius0=0.0125,
+               hydro_radius1=10.0, nb_points=nbp, segmentation=cut,
+               rigidity=np.full(nfib, 20.0), points=pts, confine_radius=radius,
+               confine_stiffness=200.0, seed=seed,
+               name=f"fibers-3d-{nfib}-links-{nlinks}-sphere{radius:g}")
+    if nlinks:
+        fa, aa, fb, ab = _bind_crosslinkers(rng, 3, nbp, cut, pts, nlinks, 0.05)
+        sc.link_fiber_a, sc.link_abs_a = fa, aa
+        sc.link_fiber_b, sc.link_abs_b = fb, ab
+        sc.link_stiffness = np.full(nlinks, 100.0)
+    return sc
+
+
+def mini(dim=3, seed=3, nfib=8, nlinks=20, radius=1.5, lmin=2.0, lmax=6.0, binding_range=0.5):
+    """Small crosslinked scene for operator-level parity checks."""
+    rng = np.random.default_rng(seed)
+    lengths = rng.uniform(lmin, lmax, nfib)
+    nbp, cut, pts = _straight_fibers(rng, dim, lengths, 0.5, radius)
+    sc = Scene(dim=dim, time_step=0.005, kT=0.0042, viscosity=0.02, hydro_radius0=0.0125,
+               hydro_radius1=10.0, nb_points=nbp, segmentation=cut,
+               rigidity=np.full(nfib, 20.0), points=pts, confine_radius=radius,
+               confine_stiffness=200.0, seed=seed, name=f"mini-{dim}d-{nfib}")
+    if nlinks:
+        fa, aa, fb, ab = _bind_crosslinkers(rng, dim, nbp, cut, pts, nlinks, binding_range)
+        sc.link_fiber_a, sc.link_abs_a = fa, aa
+        sc.link_fiber_b, sc.link_abs_b = fb, ab
+        sc.link_stiffness = np.full(nlinks, 100.0)
+    return sc

@@ -1,0 +1,1477 @@
+/*
+ * meca_oracle.c -- CPU restatement of Cytosim's implicit mechanics step (Meca).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path and the
+ * single-threaded CPU baseline ("port") timed beside it.  Nothing in the
+ * product links or calls this file.
+ *
+ * Written from the reference's behaviour, function by function; each block
+ * cites the reference file:line it restates (paths relative to the reference
+ * tree).  FP64 throughout, as the reference (`real` = double, src/math/real.h).
+ * The reference calls LAPACK/BLAS (Fortran ABI, not vendored); the routines it
+ * uses are restated here from LAPACK's documented algorithms (dgetrf/dgetf2,
+ * dgetri/dtrtri, dpttrf, dptts2, dgtsv) and pinned in tests against scipy's
+ * LAPACK.
+ *
+ * Only RigidFiber Mecables are modelled (the Mecable kind of the fiber
+ * configurations).  Brownian draws are an input (N(0,1) per degree of freedom,
+ * in object order), so the SFMT stream of the reference is not restated.
+ */
+#include "meca_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define REAL_EPSILON (128.0 * DBL_EPSILON) /* src/math/real.h:32 */
+
+/* ------------------------------------------------------------------------ */
+/* symmetric sparse matrix, lower storage per column (matsparsesym1.{h,cc})  */
+
+typedef struct { double val; uint32_t line; } sp_elem;
+
+typedef struct {
+    uint32_t  size, alloc;
+    sp_elem** col;
+    uint32_t* colSize;
+    uint32_t* colMax;
+    /* Numerical-Recipes layout built by prepareForMultiply */
+    uint32_t  nmax;
+    uint32_t* ija;
+    double*   sa;
+    uint32_t* colF;
+} spsym;
+
+/* MatrixSparseSymmetric1::allocate (matsparsesym1.cc:35-74) */
+static void sp_allocate(spsym* m, uint32_t sz)
+{
+    m->size = sz;
+    if (sz > m->alloc) {
+        m->col     = (sp_elem**)realloc(m->col, sizeof(sp_elem*) * sz);
+        m->colSize = (uint32_t*)realloc(m->colSize, sizeof(uint32_t) * sz);
+        m->colMax  = (uint32_t*)realloc(m->colMax, sizeof(uint32_t) * sz);
+        for (uint32_t i = m->alloc; i < sz; ++i) {
+            m->col[i] = NULL;
+            m->colSize[i] = 0;
+            m->colMax[i] = 0;
+        }
+        m->colF = (uint32_t*)realloc(m->colF, sizeof(uint32_t) * (sz + 1));
+        m->alloc = sz;
+    }
+}
+
+static void sp_free(spsym* m)
+{
+    for (uint32_t i = 0; i < m->alloc; ++i) free(m->col[i]);
+    free(m->col); free(m->colSize); free(m->colMax);
+    free(m->ija); free(m->sa); free(m->colF);
+    memset(m, 0, sizeof(*m));
+}
+
+/* makeZero (matsparsesym1.cc:246-250) */
+static void sp_zero(spsym* m)
+{
+    for (uint32_t i = 0; i < m->size; ++i) m->colSize[i] = 0;
+}
+
+/* allocateColumn: capacity grows to a multiple of 4 (matsparsesym1.cc:97-124) */
+static sp_elem* sp_col_reserve(spsym* m, uint32_t j, uint32_t need)
+{
+    if (need > m->colMax[j]) {
+        uint32_t cap = (need + 3u) & ~3u;
+        sp_elem* c = (sp_elem*)malloc(sizeof(sp_elem) * cap);
+        if (m->col[j]) {
+            memcpy(c, m->col[j], sizeof(sp_elem) * m->colMax[j]);
+            free(m->col[j]);
+        }
+        m->col[j] = c;
+        m->colMax[j] = cap;
+    }
+    return m->col[j];
+}
+
+/* operator()(i,j): element address in the lower triangle, created on demand;
+   the diagonal is the first element of a column, rows are kept ascending
+   (matsparsesym1.cc:137-224) */
+static double* sp_ref(spsym* m, uint32_t i, uint32_t j)
+{
+    if (j > i) { uint32_t t = i; i = j; j = t; }
+    if (m->colSize[j] == 0) {
+        sp_elem* c = sp_col_reserve(m, j, 2);
+        c[0].line = j; c[0].val = 0.0;
+        m->colSize[j] = 1;
+        if (i != j) {
+            c[1].line = i; c[1].val = 0.0;
+            m->colSize[j] = 2;
+            return &c[1].val;
+        }
+        return &c[0].val;
+    }
+    sp_elem* c = m->col[j];
+    if (i == j) return &c[0].val;
+    uint32_t n = m->colSize[j];
+    uint32_t k = 1;
+    while (k < n) {
+        if (c[k].line == i) return &c[k].val;
+        if (c[k].line > i) break;
+        ++k;
+    }
+    c = sp_col_reserve(m, j, n + 1);
+    memmove(c + k + 1, c + k, sizeof(sp_elem) * (n - k));
+    c[k].line = i;
+    c[k].val = 0.0;
+    m->colSize[j] = n + 1;
+    return &c[k].val;
+}
+
+/* nonZero (matsparsesym1.cc:342-352) */
+static int sp_nonzero(const spsym* m)
+{
+    for (uint32_t j = 0; j < m->size; ++j)
+        for (uint32_t k = 0; k < m->colSize[j]; ++k)
+            if (m->col[j][k].val != 0.0) return 1;
+    return 0;
+}
+
+/* setColF(false) + prepareForMultiply (matsparsesym1.cc:445-523) */
+static void sp_prepare(spsym* m)
+{
+    uint32_t n = m->size;
+    uint32_t inx = n;
+    m->colF[n] = n;
+    for (int64_t j = (int64_t)n - 1; j >= 0; --j) {
+        if (m->colSize[j] > 0) inx = (uint32_t)j;
+        m->colF[j] = inx;
+    }
+    uint32_t nbe = 1;
+    for (uint32_t j = 0; j < n; ++j) nbe += m->colSize[j] > 0 ? m->colSize[j] : 1;
+    if (nbe > m->nmax) {
+        free(m->ija); free(m->sa);
+        m->nmax = nbe + n;
+        m->ija = (uint32_t*)malloc(sizeof(uint32_t) * m->nmax);
+        m->sa = (double*)malloc(sizeof(double) * m->nmax);
+    }
+    m->ija[0] = n + 1;
+    uint32_t kk = n;
+    for (uint32_t j = 0; j < n; ++j) {
+        if (m->colSize[j] > 0) {
+            m->sa[j] = m->col[j][0].val;
+            for (uint32_t c = 1; c < m->colSize[j]; ++c) {
+                ++kk;
+                m->sa[kk] = m->col[j][c].val;
+                m->ija[kk] = m->col[j][c].line;
+            }
+        } else {
+            m->sa[j] = 0.0;
+        }
+        m->ija[j + 1] = kk + 1;
+    }
+}
+
+/* vecMulAdd: Y += M X from the lower storage (matsparsesym1.cc:526-542) */
+static void sp_mul_add(const spsym* m, const double* X, double* Y)
+{
+    const uint32_t n = m->size;
+    for (uint32_t j = m->colF[0]; j < n; j = m->colF[j + 1]) {
+        double x0 = X[j];
+        double y0 = Y[j] + m->sa[j] * x0;
+        for (uint32_t k = m->ija[j]; k < m->ija[j + 1]; ++k) {
+            double a = m->sa[k];
+            uint32_t i = m->ija[k];
+            Y[i] += a * x0;
+            y0 += a * X[i];
+        }
+        Y[j] = y0;
+    }
+}
+
+/* vecMulAddIso2D / vecMulAddIso3D (matsparsesym1.cc:575-631) */
+static void sp_mul_add_iso(const spsym* m, int dim, const double* X, double* Y)
+{
+    const uint32_t n = m->size;
+    for (uint32_t j = m->colF[0]; j < n; j = m->colF[j + 1]) {
+        const uint32_t Dj = dim * j;
+        double xj[3], yj[3];
+        for (int d = 0; d < dim; ++d) {
+            xj[d] = X[Dj + d];
+            yj[d] = Y[Dj + d] + m->sa[j] * xj[d];
+        }
+        for (uint32_t k = m->ija[j]; k < m->ija[j + 1]; ++k) {
+            const uint32_t Di = dim * m->ija[k];
+            const double a = m->sa[k];
+            for (int d = 0; d < dim; ++d) {
+                yj[d] += a * X[Di + d];
+                Y[Di + d] += a * xj[d];
+            }
+        }
+        for (int d = 0; d < dim; ++d) Y[Dj + d] = yj[d];
+    }
+}
+
+/* addTriangularBlock: upper triangle of the diagonal block into M (sx*sx,
+   column-major) (matsparsesym1.cc:262-285) */
+static void sp_add_triangular_block(const spsym* m, double* M, uint32_t x, uint32_t sx)
+{
+    for (uint32_t j = 0; j < sx; ++j) {
+        for (uint32_t k = 0; k < m->colSize[j + x]; ++k) {
+            uint32_t i = m->col[j + x][k].line;
+            if (x <= i) {
+                i -= x;
+                if (i < sx) {
+                    if (i < j) M[i + sx * j] += m->col[j + x][k].val;
+                    else       M[j + sx * i] += m->col[j + x][k].val;
+                }
+            }
+        }
+    }
+}
+
+/* addDiagonalBlock: full diagonal block into M (matsparsesym1.cc:288-310) */
+static void sp_add_diagonal_block(const spsym* m, double* M, uint32_t x, uint32_t sx)
+{
+    for (uint32_t j = 0; j < sx; ++j) {
+        for (uint32_t k = 0; k < m->colSize[j + x]; ++k) {
+            uint32_t i = m->col[j + x][k].line;
+            if (x <= i) {
+                i -= x;
+                if (i < sx) {
+                    M[i + sx * j] += m->col[j + x][k].val;
+                    if (i != j) M[j + sx * i] += m->col[j + x][k].val;
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* LAPACK restatements                                                       */
+
+/* dgetrf with the unblocked (dgetf2) right-looking algorithm: partial
+   pivoting on the first entry of largest magnitude (idamax), ipiv 1-based.
+   Returns info (0, or index of the first exactly-zero pivot). */
+int orc_dgetrf(int n, double* a, int lda, int* ipiv)
+{
+    int info = 0;
+    for (int j = 0; j < n; ++j) {
+        int p = j;
+        double amax = fabs(a[j + lda * j]);
+        for (int i = j + 1; i < n; ++i) {
+            double v = fabs(a[i + lda * j]);
+            if (v > amax) { amax = v; p = i; }
+        }
+        ipiv[j] = p + 1;
+        if (a[p + lda * j] != 0.0) {
+            if (p != j)
+                for (int k = 0; k < n; ++k) {
+                    double t = a[j + lda * k];
+                    a[j + lda * k] = a[p + lda * k];
+                    a[p + lda * k] = t;
+                }
+            double piv = a[j + lda * j];
+            if (fabs(piv) >= DBL_MIN) {
+                double r = 1.0 / piv;
+                for (int i = j + 1; i < n; ++i) a[i + lda * j] *= r;
+            } else {
+                for (int i = j + 1; i < n; ++i) a[i + lda * j] /= piv;
+            }
+        } else if (info == 0) {
+            info = j + 1;
+        }
+        for (int k = j + 1; k < n; ++k) {
+            double u = a[j + lda * k];
+            if (u != 0.0)
+                for (int i = j + 1; i < n; ++i) a[i + lda * k] -= a[i + lda * j] * u;
+        }
+    }
+    return info;
+}
+
+/* dgetri: inv(U) in place (dtrti2, upper, non-unit), then solve
+   inv(A) * L = inv(U), then undo the column interchanges. */
+int orc_dgetri(int n, double* a, int lda, const int* ipiv)
+{
+    for (int j = 0; j < n; ++j)
+        if (a[j + lda * j] == 0.0) return j + 1;
+    /* dtrti2 upper non-unit */
+    for (int j = 0; j < n; ++j) {
+        a[j + lda * j] = 1.0 / a[j + lda * j];
+        double ajj = -a[j + lda * j];
+        /* x(0:j-1) = triu(A(0:j-1,0:j-1)) * x  (dtrmv upper, no-trans, non-unit) */
+        for (int k = 0; k < j; ++k) {
+            double t = a[k + lda * j];
+            if (t != 0.0) {
+                for (int i = 0; i < k; ++i) a[i + lda * j] += t * a[i + lda * k];
+                a[k + lda * j] = t * a[k + lda * k];
+            }
+        }
+        for (int i = 0; i < j; ++i) a[i + lda * j] *= ajj;
+    }
+    double* work = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+    for (int j = n - 1; j >= 0; --j) {
+        for (int i = j + 1; i < n; ++i) {
+            work[i] = a[i + lda * j];
+            a[i + lda * j] = 0.0;
+        }
+        /* A(:,j) -= A(:,j+1:n) * work(j+1:n)  (dgemv) */
+        for (int k = j + 1; k < n; ++k) {
+            double w = work[k];
+            if (w != 0.0)
+                for (int i = 0; i < n; ++i) a[i + lda * j] -= a[i + lda * k] * w;
+        }
+    }
+    free(work);
+    for (int j = n - 2; j >= 0; --j) {
+        int jp = ipiv[j] - 1;
+        if (jp != j)
+            for (int i = 0; i < n; ++i) {
+                double t = a[i + lda * j];
+                a[i + lda * j] = a[i + lda * jp];
+                a[i + lda * jp] = t;
+            }
+    }
+    return 0;
+}
+
+/* dpttrf: L*D*L' of a symmetric positive definite tridiagonal matrix;
+   d (n) becomes D, e (n-1) becomes the subdiagonal of L. */
+int orc_dpttrf(int n, double* d, double* e)
+{
+    for (int i = 0; i < n - 1; ++i) {
+        if (d[i] <= 0.0) return i + 1;
+        double ei = e[i];
+        e[i] = ei / d[i];
+        d[i + 1] -= e[i] * ei;
+    }
+    if (n > 0 && d[n - 1] <= 0.0) return n;
+    return 0;
+}
+
+/* dptts2 with one right-hand side: solve L*D*L' x = b in place. */
+void orc_dptts2(int n, const double* d, const double* e, double* b)
+{
+    if (n <= 1) {
+        if (n == 1) b[0] *= 1.0 / d[0];
+        return;
+    }
+    for (int i = 1; i < n; ++i) b[i] -= b[i - 1] * e[i - 1];
+    b[n - 1] /= d[n - 1];
+    for (int i = n - 2; i >= 0; --i) b[i] = b[i] / d[i] - b[i + 1] * e[i];
+}
+
+/* dgtsv with one right-hand side: Gaussian elimination with partial pivoting
+   on a general tridiagonal matrix (dl: n-1 sub, d: n diag, du: n-1 super). */
+int orc_dgtsv(int n, double* dl, double* d, double* du, double* b)
+{
+    if (n <= 0) return 0;
+    for (int i = 0; i + 2 < n; ++i) {
+        if (fabs(d[i]) >= fabs(dl[i])) {
+            if (d[i] == 0.0) return i + 1;
+            double f = dl[i] / d[i];
+            d[i + 1] -= f * du[i];
+            b[i + 1] -= f * b[i];
+            dl[i] = 0.0;
+        } else {
+            double f = d[i] / dl[i];
+            d[i] = dl[i];
+            double t = d[i + 1];
+            d[i + 1] = du[i] - f * t;
+            dl[i] = du[i + 1];
+            du[i + 1] = -f * dl[i];
+            du[i] = t;
+            t = b[i];
+            b[i] = b[i + 1];
+            b[i + 1] = t - f * b[i + 1];
+        }
+    }
+    if (n > 1) {
+        int i = n - 2;
+        if (fabs(d[i]) >= fabs(dl[i])) {
+            if (d[i] == 0.0) return i + 1;
+            double f = dl[i] / d[i];
+            d[i + 1] -= f * du[i];
+            b[i + 1] -= f * b[i];
+        } else {
+            double f = d[i] / dl[i];
+            d[i] = dl[i];
+            double t = d[i + 1];
+            d[i + 1] = du[i] - f * t;
+            du[i] = t;
+            t = b[i];
+            b[i] = b[i + 1];
+            b[i + 1] = t - f * b[i + 1];
+        }
+    }
+    if (d[n - 1] == 0.0) return n;
+    b[n - 1] /= d[n - 1];
+    if (n > 1) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / d[n - 2];
+    for (int i = n - 3; i >= 0; --i)
+        b[i] = (b[i] - du[i] * b[i + 1] - dl[i] * b[i + 2]) / d[i];
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Meca state                                                                */
+
+typedef struct {
+    uint32_t np, idx;       /* nbPoints, matIndex */
+    uint32_t seg;           /* offset of the first segment in the segment arrays */
+    double   cut;           /* fnCut = segmentation() */
+    double   rigidity;      /* FiberProp::rigidity */
+    double   rf_rigidity;   /* rigidity / cut^3 (fiber.cc:459) */
+    double   mobility;      /* rfMobility (fiber.cc:446) */
+    double*  blk;           /* preconditioner block (mecable.cc:15-34) */
+    uint32_t blk_alloc;
+    int      use_blk;
+} orc_fiber;
+
+struct orc_meca {
+    int        dim;
+    orc_params prm;
+    double     time_step;
+    orc_fiber* fib;
+    int        nfib, fib_alloc;
+    uint32_t   npts, nseg, largest;
+    uint32_t   vec_alloc;
+    double    *vPTS, *vSOL, *vBAS, *vRHS, *vFOR, *vTMP;
+    double    *pos;         /* Mecable-held points (psPos) */
+    double    *diff;        /* rfDiff, DIM per segment */
+    double    *jd, *je;     /* mtJJt, mtJJt2 after pttrf */
+    double    *lag, *ltmp;  /* rfLag, rfTMP */
+    double    *pdiff;       /* mtJJtiJforce */
+    uint32_t   seg_alloc;
+    spsym      mB, mC;
+    int        use_mB, use_mC;
+    double     noise_level;
+    int        prepared, setup_done;
+};
+
+static void* xrealloc(void* p, size_t n) { return realloc(p, n ? n : 1); }
+
+orc_meca* orc_create(int dim)
+{
+    if (dim != 2 && dim != 3) return NULL;
+    orc_meca* m = (orc_meca*)calloc(1, sizeof(orc_meca));
+    m->dim = dim;
+    m->prm.time_step = 0.005;
+    m->prm.kT = 0.0042;
+    m->prm.viscosity = 1.0;
+    m->prm.hydro_radius0 = 0.0125;
+    m->prm.hydro_radius1 = 10.0;
+    return m;
+}
+
+void orc_destroy(orc_meca* m)
+{
+    if (!m) return;
+    for (int f = 0; f < m->fib_alloc; ++f) free(m->fib[f].blk);
+    free(m->fib);
+    free(m->vPTS); free(m->vSOL); free(m->vBAS); free(m->vRHS); free(m->vFOR); free(m->vTMP);
+    free(m->pos); free(m->diff); free(m->jd); free(m->je); free(m->lag); free(m->ltmp); free(m->pdiff);
+    sp_free(&m->mB); sp_free(&m->mC);
+    free(m);
+}
+
+void orc_set_params(orc_meca* m, const orc_params* p) { m->prm = *p; }
+
+/* Meca::clear (meca.cc:55-60) */
+void orc_clear(orc_meca* m)
+{
+    for (int f = 0; f < m->fib_alloc; ++f) m->fib[f].np = 0;
+    m->nfib = 0;
+    m->npts = 0;
+    m->nseg = 0;
+    m->largest = 0;
+    m->prepared = 0;
+    m->setup_done = 0;
+}
+
+/* Meca::add (meca.cc:67-79): matIndex is the running prefix sum */
+int orc_add_fiber(orc_meca* m, uint32_t np, double segmentation, double rigidity)
+{
+    if (np < 2) return -1;
+    if (m->nfib == m->fib_alloc) {
+        int na = m->fib_alloc ? 2 * m->fib_alloc : 64;
+        m->fib = (orc_fiber*)realloc(m->fib, sizeof(orc_fiber) * na);
+        memset(m->fib + m->fib_alloc, 0, sizeof(orc_fiber) * (na - m->fib_alloc));
+        m->fib_alloc = na;
+    }
+    orc_fiber* f = &m->fib[m->nfib];
+    f->np = np;
+    f->idx = m->npts;
+    f->seg = m->nseg;
+    f->cut = segmentation;
+    f->rigidity = rigidity;
+    m->npts += np;
+    m->nseg += np - 1;
+    if (m->largest < np) m->largest = np;
+    return m->nfib++;
+}
+
+uint32_t orc_nb_points(const orc_meca* m) { return m->npts; }
+uint32_t orc_mat_index(const orc_meca* m, int f) { return m->fib[f].idx; }
+
+/* FiberNaked::bestNbPoints (fiber_naked.cc:22-30) */
+uint32_t orc_best_nb_points(double ratio)
+{
+    unsigned n = (unsigned)(int)ratio;
+    if ((2.0 * n + 1.0) * ratio > 2.0 * n * (n + 1.0)) return n + 2;
+    return n + 1;
+}
+
+/* Fiber::dragCoefficientVolume (fiber.cc:315-363) and setDragCoefficient (428-447) */
+static double fiber_mobility(const orc_meca* m, const orc_fiber* f)
+{
+    double len = f->cut * (double)(f->np - 1);      /* length() = nbSegments()*fnCut */
+    double lenc = len;
+    if (lenc > m->prm.hydro_radius1) lenc = m->prm.hydro_radius1;
+    double drag_sphere = 6.0 * m->prm.hydro_radius0;
+    double drag_cyl = 3.0 * len / (log(0.5 * lenc / m->prm.hydro_radius0) + 0.312);
+    double drag = M_PI * m->prm.viscosity * (drag_cyl > drag_sphere ? drag_cyl : drag_sphere);
+    return (double)f->np / drag;
+}
+
+static int ensure_vectors(orc_meca* m)
+{
+    if (m->npts > m->vec_alloc) {
+        uint32_t a = (m->npts + 3u) & ~3u;    /* meca.cc:511-522 */
+        size_t nb = sizeof(double) * (size_t)m->dim * a;
+        m->vPTS = (double*)xrealloc(m->vPTS, nb);
+        m->vSOL = (double*)xrealloc(m->vSOL, nb);
+        m->vBAS = (double*)xrealloc(m->vBAS, nb);
+        m->vRHS = (double*)xrealloc(m->vRHS, nb);
+        m->vFOR = (double*)xrealloc(m->vFOR, nb);
+        m->vTMP = (double*)xrealloc(m->vTMP, nb);
+        m->pos  = (double*)xrealloc(m->pos, nb);
+        memset(m->vSOL, 0, nb); memset(m->vRHS, 0, nb); memset(m->vFOR, 0, nb);
+        m->vec_alloc = a;
+    }
+    if (m->nseg > m->seg_alloc) {
+        uint32_t a = m->nseg;
+        m->diff  = (double*)xrealloc(m->diff, sizeof(double) * m->dim * a);
+        m->jd    = (double*)xrealloc(m->jd, sizeof(double) * a);
+        m->je    = (double*)xrealloc(m->je, sizeof(double) * a);
+        m->lag   = (double*)xrealloc(m->lag, sizeof(double) * a);
+        m->ltmp  = (double*)xrealloc(m->ltmp, sizeof(double) * a);
+        m->pdiff = (double*)xrealloc(m->pdiff, sizeof(double) * a);
+        memset(m->lag, 0, sizeof(double) * a);
+        m->seg_alloc = a;
+    }
+    return 0;
+}
+
+/* RigidFiber::storeDifferences (rigid_fiber.cc:111-117) and
+   RigidFiber::makeProjection (rigid_fiber_project.cc:50-87) */
+static int fiber_make_projection(orc_meca* m, orc_fiber* f)
+{
+    const int D = m->dim;
+    const double* P = m->pos + D * f->idx;
+    double* dif = m->diff + D * f->seg;
+    const double sc = 1.0 / f->cut;
+    const uint32_t S = f->np - 1;
+    for (uint32_t p = 0; p < D * S; ++p) dif[p] = sc * (P[p + D] - P[p]);
+    double* d = m->jd + f->seg;
+    double* e = m->je + f->seg;
+    for (uint32_t s = 0; s < S; ++s) {
+        const double* X = dif + D * s;
+        double dd = 0.0;
+        for (int k = 0; k < D; ++k) dd += X[k] * X[k];
+        d[s] = 2.0 * dd;
+        if (s + 1 < S) {
+            double od = 0.0;
+            for (int k = 0; k < D; ++k) od += X[k] * X[k + D];
+            e[s] = -od;
+        }
+    }
+    return orc_dpttrf((int)S, d, e);
+}
+
+/* Meca::prepare (meca.cc:488-537) with Fiber::prepareMecable (fiber.cc:450-460) */
+int orc_prepare(orc_meca* m, const double* points)
+{
+    ensure_vectors(m);
+    const int D = m->dim;
+    sp_allocate(&m->mB, m->npts);
+    sp_allocate(&m->mC, D * m->npts);
+    sp_zero(&m->mB);
+    sp_zero(&m->mC);
+    memset(m->vBAS, 0, sizeof(double) * D * m->npts);
+    m->time_step = m->prm.time_step;
+    if (points) memcpy(m->pos, points, sizeof(double) * D * m->npts);
+    /* putPoints: vPTS <- Mecable points */
+    memcpy(m->vPTS, m->pos, sizeof(double) * D * m->npts);
+    for (int i = 0; i < m->nfib; ++i) {
+        orc_fiber* f = &m->fib[i];
+        f->mobility = fiber_mobility(m, f);
+        if (fiber_make_projection(m, f)) return -2;   /* "could not build Fiber projection" */
+        f->rf_rigidity = f->rigidity / (f->cut * f->cut * f->cut);
+    }
+    m->prepared = 1;
+    m->setup_done = 0;
+    return 0;
+}
+
+static int mecable_of(const orc_meca* m, uint32_t idx)
+{
+    int lo = 0, hi = m->nfib - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (m->fib[mid].idx <= idx) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+/* FiberNaked::interpolate (fiber_naked.cc:1050-1065), fnAbscissa = 0 */
+void orc_interpolate(const orc_meca* m, int fi, double ab, uint32_t* mat1, uint32_t* mat2, double* coef)
+{
+    const orc_fiber* f = &m->fib[fi];
+    if (ab <= 0) { *mat1 = f->idx; *mat2 = f->idx + 1; *coef = 0.0; return; }
+    double n;
+    double co = modf(ab / f->cut, &n);
+    unsigned rd = (unsigned)n;
+    if (rd < f->np - 1) { *mat1 = f->idx + rd; *mat2 = f->idx + rd + 1; *coef = co; }
+    else { *mat1 = f->idx + f->np - 2; *mat2 = f->idx + f->np - 1; *coef = 1.0; }
+}
+
+/* ------------------------------------------------------------------------ */
+/* assembly (meca_inter.cc)                                                  */
+
+/* addPureForce(PointExact) (meca_inter.cc:123-134) */
+void orc_add_pure_force(orc_meca* m, uint32_t a, const double* force)
+{
+    for (int d = 0; d < m->dim; ++d) m->vBAS[m->dim * a + d] += force[d];
+}
+
+/* interLink(PointExact, PointExact) (meca_inter.cc:196-230), no periodic space.
+   neighbours: same Mecable and adjacent or equal points (point_exact.cc:43-47). */
+void orc_inter_link_ee(orc_meca* m, uint32_t a, uint32_t b, double w)
+{
+    int fa = mecable_of(m, a), fb = mecable_of(m, b);
+    if (fa == fb && (a == b || a == b + 1 || a + 1 == b)) return;
+    *sp_ref(&m->mB, a, a) -= w;
+    *sp_ref(&m->mB, a, b) += w;
+    *sp_ref(&m->mB, b, b) -= w;
+}
+
+/* interLink(PointInterpolated, PointExact) (meca_inter.cc:240-275) */
+void orc_inter_link_ie(orc_meca* m, uint32_t a1, uint32_t a2, double ca, uint32_t b, double w)
+{
+    if (mecable_of(m, a1) == mecable_of(m, b) && (a1 == b || a2 == b)) return;
+    const double c[3] = { 1.0 - ca, ca, -1.0 };
+    const double cw[3] = { w * c[0], w * c[1], -w };
+    const uint32_t inx[3] = { a1, a2, b };
+    for (int k = 0; k < 3; ++k)
+        for (int l = k; l < 3; ++l)
+            *sp_ref(&m->mB, inx[k], inx[l]) -= c[k] * cw[l];
+}
+
+/* interLink(PointInterpolated, PointInterpolated) (meca_inter.cc:285-321) */
+void orc_inter_link_ii(orc_meca* m, uint32_t a1, uint32_t a2, double ca,
+                       uint32_t b1, uint32_t b2, double cb, double w)
+{
+    if (mecable_of(m, a1) == mecable_of(m, b1) &&
+        (a1 == b1 || a1 == b2 || a2 == b1 || a2 == b2)) return;
+    const double c[4] = { 1.0 - ca, ca, -(1.0 - cb), -cb };
+    const double cw[4] = { w * c[0], w * c[1], w * c[2], w * c[3] };
+    const uint32_t inx[4] = { a1, a2, b1, b2 };
+    for (int j = 0; j < 4; ++j)
+        for (int i = j; i < 4; ++i)
+            *sp_ref(&m->mB, inx[i], inx[j]) -= c[j] * cw[i];
+}
+
+/* interLongLink(PointExact, PointExact) (meca_inter.cc:334-415), no periodic space */
+void orc_inter_long_link_ee(orc_meca* m, uint32_t a, uint32_t b, double len, double w)
+{
+    const int D = m->dim;
+    int fa = mecable_of(m, a), fb = mecable_of(m, b);
+    if (fa == fb && (a == b || a == b + 1 || a + 1 == b)) return;
+    const uint32_t iA = D * a, iB = D * b;
+    double ab[3] = { 0, 0, 0 };
+    for (int d = 0; d < D; ++d) ab[d] = m->vPTS[iB + d] - m->vPTS[iA + d];
+    double abn = 0.0;
+    for (int d = 0; d < D; ++d) abn += ab[d] * ab[d];
+    abn = sqrt(abn);
+    if (abn < REAL_EPSILON) return;
+    for (int d = 0; d < D; ++d) ab[d] /= abn;
+    for (int d = 0; d < D; ++d) {
+        m->vBAS[iA + d] -= w * ab[d] * len;
+        m->vBAS[iB + d] += w * ab[d] * len;
+    }
+    len /= abn;
+    const int cooked = (len > 1.0);
+    for (int i = 0; i < D; ++i) {
+        double v = cooked ? w * ab[i] * ab[i] : w * (1.0 + len * (ab[i] * ab[i] - 1.0));
+        *sp_ref(&m->mC, iA + i, iA + i) -= v;
+        *sp_ref(&m->mC, iA + i, iB + i) += v;
+        *sp_ref(&m->mC, iB + i, iB + i) -= v;
+        for (int j = i + 1; j < D; ++j) {
+            v = cooked ? w * ab[i] * ab[j] : w * len * ab[i] * ab[j];
+            *sp_ref(&m->mC, iA + i, iA + j) -= v;
+            *sp_ref(&m->mC, iA + i, iB + j) += v;
+            *sp_ref(&m->mC, iA + j, iB + i) += v;
+            *sp_ref(&m->mC, iB + i, iB + j) -= v;
+        }
+    }
+}
+
+/* interClamp(PointExact) (meca_inter.cc:1883-1907), no periodic space */
+void orc_inter_clamp_e(orc_meca* m, uint32_t a, const double* g, double w)
+{
+    *sp_ref(&m->mB, a, a) -= w;
+    for (int d = 0; d < m->dim; ++d) m->vBAS[m->dim * a + d] += w * g[d];
+}
+
+/* interClamp(PointInterpolated) (meca_inter.cc:1918-1963) */
+void orc_inter_clamp_i(orc_meca* m, uint32_t a1, uint32_t a2, double ca, const double* g, double w)
+{
+    const double c2 = ca, c2w = w * c2;
+    const double c1 = 1.0 - ca, c1w = w * c1;
+    *sp_ref(&m->mB, a1, a1) -= c1w * c1;
+    *sp_ref(&m->mB, a1, a2) -= c2w * c1;
+    *sp_ref(&m->mB, a2, a2) -= c2w * c2;
+    for (int d = 0; d < m->dim; ++d) {
+        m->vBAS[m->dim * a1 + d] += c1w * g[d];
+        m->vBAS[m->dim * a2 + d] += c2w * g[d];
+    }
+}
+
+/* interLongClamp (meca_inter.cc:1977-2022) */
+void orc_inter_long_clamp(orc_meca* m, uint32_t a, const double* center, double len, double w)
+{
+    const int D = m->dim;
+    const uint32_t inx = D * a;
+    double axis[3] = { 0, 0, 0 };
+    for (int d = 0; d < D; ++d) axis[d] = m->vPTS[inx + d] - center[d];
+    double an = 0.0;
+    for (int d = 0; d < D; ++d) an += axis[d] * axis[d];
+    an = sqrt(an);
+    if (an < REAL_EPSILON) return;
+    for (int d = 0; d < D; ++d) axis[d] /= an;
+    double ac = 0.0;
+    for (int d = 0; d < D; ++d) ac += axis[d] * center[d];
+    if (len < an) {
+        len /= an;
+        const double facX = w * len * (an + ac);
+        const double facC = w * (1.0 - len);
+        for (int i = 0; i < D; ++i) {
+            *sp_ref(&m->mC, inx + i, inx + i) += w * (len * (1.0 - axis[i] * axis[i]) - 1.0);
+            for (int j = i + 1; j < D; ++j)
+                *sp_ref(&m->mC, inx + i, inx + j) -= w * len * axis[i] * axis[j];
+            m->vBAS[inx + i] += facX * axis[i] + facC * center[i];
+        }
+    } else {
+        const double facX = w * (len + ac);
+        for (int i = 0; i < D; ++i) {
+            for (int j = i; j < D; ++j)
+                *sp_ref(&m->mC, inx + i, inx + j) -= w * axis[i] * axis[j];
+            m->vBAS[inx + i] += facX * axis[i];
+        }
+    }
+}
+
+/* interPlane(PointExact) (meca_inter.cc:2301-2326) */
+void orc_inter_plane_e(orc_meca* m, uint32_t a, const double* dir, const double* g, double w)
+{
+    const int D = m->dim;
+    const uint32_t inx = D * a;
+    double gd = 0.0;
+    for (int d = 0; d < D; ++d) gd += g[d] * dir[d];
+    const double pr = gd * w;
+    for (int i = 0; i < D; ++i) {
+        for (int j = i; j < D; ++j)
+            *sp_ref(&m->mC, inx + i, inx + j) -= dir[i] * dir[j] * w;
+        m->vBAS[inx + i] += pr * dir[i];
+    }
+}
+
+/* Fiber::setInteractions, CONFINE_INSIDE (fiber.cc:471-481), with
+   SpaceSphere::inside (space_sphere.cc:113-116) and setInteraction (148-151) */
+void orc_confine_sphere(orc_meca* m, double radius, double stiffness)
+{
+    const int D = m->dim;
+    const double zero[3] = { 0, 0, 0 };
+    for (int i = 0; i < m->nfib; ++i) {
+        const orc_fiber* f = &m->fib[i];
+        for (uint32_t p = 0; p < f->np; ++p) {
+            const double* x = m->pos + D * (f->idx + p);
+            double r2 = 0.0;
+            for (int d = 0; d < D; ++d) r2 += x[d] * x[d];
+            if (!(r2 <= radius * radius))
+                orc_inter_long_clamp(m, f->idx + p, zero, radius, stiffness);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* per-fiber operators                                                       */
+
+/* projectForcesA / dptts2 / projectForcesB (rigid_fiber_project.cc:95-149):
+   Y = sca * P X  with  P = I - J'(JJ')^-1 J */
+static void fiber_project(const orc_meca* m, const orc_fiber* f, const double* X, double* Y,
+                          double sca, double* tmp)
+{
+    const int D = m->dim;
+    const uint32_t S = f->np - 1;
+    const double* dif = m->diff + D * f->seg;
+    for (uint32_t s = 0; s < S; ++s) {
+        double t = 0.0;
+        for (int d = 0; d < D; ++d) t += dif[D * s + d] * (X[D * s + D + d] - X[D * s + d]);
+        tmp[s] = t;
+    }
+    orc_dptts2((int)S, m->jd + f->seg, m->je + f->seg, tmp);
+    for (int d = 0; d < D; ++d) {
+        const uint32_t kk = D * S;
+        Y[d] = sca * (X[d] + dif[d] * tmp[0]);
+        Y[kk + d] = sca * (X[kk + d] - dif[kk - D + d] * tmp[S - 1]);
+    }
+    for (uint32_t s = 1; s < S; ++s) {
+        const uint32_t kk = D * s;
+        for (int d = 0; d < D; ++d)
+            Y[kk + d] = sca * (X[kk + d] + dif[kk + d] * tmp[s] - dif[kk - D + d] * tmp[s - 1]);
+    }
+}
+
+/* RigidFiber::setSpeedsFromForces (rigid_fiber.cc:124-132) */
+static void fiber_speeds(const orc_meca* m, const orc_fiber* f, const double* X, double* Y,
+                         double sc, int rhs)
+{
+    double* tmp = (rhs ? m->lag : m->ltmp) + f->seg;
+    fiber_project(m, f, X, Y, sc * f->mobility, tmp);
+}
+
+/* RigidFiber::addRigidity with add_rigidity1 (rigid_fiber.cc:220-229, 294-303) */
+static void fiber_add_rigidity(const orc_meca* m, const orc_fiber* f, const double* X, double* Y)
+{
+    const int D = m->dim;
+    if (f->np <= 2) return;
+    const uint32_t nbs = D * (f->np - 2);
+    const double k = f->rf_rigidity;
+    for (uint32_t j = 0; j < nbs; ++j) {
+        double v = k * (X[j] - 2 * X[j + D] + X[j + 2 * D]);
+        Y[j] -= v;
+        Y[j + D] += v + v;
+        Y[j + 2 * D] -= v;
+    }
+}
+
+/* RigidFiber::addRigidityMatUp (rigid_fiber.cc:191-215) into an upper
+   MatrixSymmetric of size ps (val[min + ps*max], matsym.cc:51-59) */
+static void fiber_rigidity_mat_up(const orc_fiber* f, double* M, int ps)
+{
+    int sz = (int)f->np;
+    if (sz < 3) return;
+    const double r = f->rf_rigidity;
+#define UP(i, j) M[((i) < (j) ? (i) + ps * (j) : (j) + ps * (i))]
+    int s = 0, e = sz;
+    for (int i = s + 2; i < e - 2; ++i) UP(i, i) -= 6 * r;
+    for (int i = s + 1; i < e - 2; ++i) UP(i, i + 1) += 4 * r;
+    for (int i = s; i < e - 2; ++i) UP(i, i + 2) -= r;
+    UP(s, s) -= r;
+    UP(e - 1, e - 1) -= r;
+    if (sz == 3) {
+        UP(s + 1, s + 1) -= 4 * r;
+    } else {
+        UP(s + 1, s + 1) -= 5 * r;
+        UP(e - 2, e - 2) -= 5 * r;
+    }
+    UP(s, s + 1) += 2 * r;
+    UP(e - 2, e - 1) += 2 * r;
+#undef UP
+}
+
+/* makeProjectionDiff (rigid_fiber_project.cc:168-219): compressive
+   multipliers are dropped */
+static void fiber_make_projection_diff(orc_meca* m, const orc_fiber* f)
+{
+    const double sc = 1.0 / f->cut;
+    const uint32_t S = f->np - 1;
+    for (uint32_t s = 0; s < S; ++s) {
+        double l = m->lag[f->seg + s];
+        m->pdiff[f->seg + s] = (l < 0) ? 0.0 : l * sc;
+    }
+}
+
+/* add_projection / addProjectionDiff (rigid_fiber_project.cc:225-248) */
+static void fiber_add_projection_diff(const orc_meca* m, const orc_fiber* f, const double* X, double* Y)
+{
+    const int D = m->dim;
+    const uint32_t S = f->np - 1;
+    const double* c = m->pdiff + f->seg;
+    for (uint32_t s = 0; s < S; ++s) {
+        if (c[s] != 0.0) {
+            const uint32_t l = D * s, k = l + D;
+            for (int d = 0; d < D; ++d) {
+                double v = c[s] * (X[k + d] - X[l + d]);
+                Y[l + d] += v;
+                Y[k + d] -= v;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Meca operators                                                            */
+
+/* addLinearForces (meca.cc:99-127) */
+static void add_linear_forces(const orc_meca* m, const double* X, double* Y, int with_rigidity)
+{
+    const int D = m->dim;
+    if (with_rigidity)
+        for (int i = 0; i < m->nfib; ++i) {
+            const orc_fiber* f = &m->fib[i];
+            fiber_add_rigidity(m, f, X + D * f->idx, Y + D * f->idx);
+        }
+    if (m->use_mB) sp_mul_add_iso(&m->mB, D, X, Y);
+    if (m->use_mC) sp_mul_add(&m->mC, X, Y);
+}
+
+/* computeForces(X, Y, rig) (meca.cc:143-147) */
+void orc_forces_linear(const orc_meca* m, const double* X, double* Y, int with_rigidity)
+{
+    memcpy(Y, m->vBAS, sizeof(double) * m->dim * m->npts);
+    add_linear_forces(m, X, Y, with_rigidity);
+}
+
+/* Meca::multiply (meca.cc:157-181): Y = X - dt * P (mB + mC + R + P') X */
+void orc_multiply(const orc_meca* m, const double* X, double* Y)
+{
+    const int D = m->dim;
+    const uint32_t n = D * m->npts;
+    double* T = m->vTMP;
+    memset(T, 0, sizeof(double) * n);
+    add_linear_forces(m, X, T, 1);
+    for (int i = 0; i < m->nfib; ++i) {
+        const orc_fiber* f = &m->fib[i];
+        const uint32_t o = D * f->idx;
+        fiber_add_projection_diff(m, f, X + o, T + o);
+        fiber_speeds(m, f, T + o, Y + o, -m->time_step, 0);
+    }
+    for (uint32_t k = 0; k < n; ++k) Y[k] += X[k];
+}
+
+/* Meca::duplicateMat (meca.cc:188-223) */
+static void duplicate_mat(int D, int ps, const double* X, double* Y)
+{
+    const int bs = D * ps;
+    memset(Y, 0, sizeof(double) * bs * bs);
+    for (int i = 0; i < ps; ++i) {
+        double xx = X[i + ps * i];
+        int kk = (bs + 1) * D * i;
+        for (int d = 0; d < D; ++d, kk += bs + 1) Y[kk] = xx;
+        for (int j = i + 1; j < ps; ++j) {
+            xx = X[i + ps * j];
+            kk = D * (i + bs * j);
+            int ll = D * (j + bs * i);
+            for (int d = 0; d < D; ++d, kk += bs + 1, ll += bs + 1) {
+                Y[kk] = xx;
+                Y[ll] = xx;
+            }
+        }
+    }
+}
+
+/* Meca::getBlock (meca.cc:231-283): I - dt P (B + C + R + P'), column-major */
+static void get_block(const orc_meca* m, const orc_fiber* f, double* blk)
+{
+    const int D = m->dim;
+    const int ps = (int)f->np, bs = D * ps;
+    size_t n1 = (size_t)ps * ps > (size_t)bs ? (size_t)ps * ps : (size_t)bs;
+    double* t1 = (double*)calloc(n1, sizeof(double));
+    double* t2 = (double*)calloc((size_t)bs * bs, sizeof(double));
+    fiber_rigidity_mat_up(f, t1, ps);
+    if (m->use_mB) sp_add_triangular_block(&m->mB, t1, f->idx, ps);
+    duplicate_mat(D, ps, t1, t2);
+    if (m->use_mC) sp_add_diagonal_block(&m->mC, t2, D * f->idx, bs);
+    memset(t1, 0, sizeof(double) * bs);
+    for (int i = 0; i < bs; ++i) {
+        t1[i] = 1;
+        fiber_add_projection_diff(m, f, t1, t2 + bs * i);
+        t1[i] = 0;
+    }
+    for (int i = 0; i < bs; ++i) {
+        fiber_speeds(m, f, t2 + (size_t)bs * i, blk + (size_t)bs * i, -m->time_step, 0);
+        blk[i + (size_t)bs * i] += 1.0;
+    }
+    free(t1);
+    free(t2);
+}
+
+void orc_get_block(const orc_meca* m, int i, double* blk) { get_block(m, &m->fib[i], blk); }
+
+int orc_get_inverse_block(const orc_meca* m, int i, double* blk)
+{
+    const orc_fiber* f = &m->fib[i];
+    const uint32_t bs = m->dim * f->np;
+    if (!f->use_blk || !f->blk) return 0;
+    memcpy(blk, f->blk, sizeof(double) * bs * bs);
+    return 1;
+}
+
+/* Meca::computePreconditionner (meca.cc:323-444) */
+int orc_compute_preconditioner(orc_meca* m)
+{
+    const int D = m->dim;
+    int* ipiv = (int*)malloc(sizeof(int) * D * (m->largest ? m->largest : 1));
+    for (int i = 0; i < m->nfib; ++i) {
+        orc_fiber* f = &m->fib[i];
+        const uint32_t bs = D * f->np;
+        if (f->blk_alloc < bs * bs) {
+            free(f->blk);
+            f->blk = (double*)malloc(sizeof(double) * bs * bs);
+            f->blk_alloc = bs * bs;
+        }
+        get_block(m, f, f->blk);
+        int res = 0;
+        if (orc_dgetrf((int)bs, f->blk, (int)bs, ipiv)) res = 2;
+        else if (orc_dgetri((int)bs, f->blk, (int)bs, ipiv)) res = 3;
+        f->use_blk = (res == 0);
+    }
+    free(ipiv);
+    return 0;
+}
+
+/* Meca::precondition (meca.cc:448-466): per-block dgemv, or copy */
+void orc_precondition(const orc_meca* m, const double* X, double* Y)
+{
+    const int D = m->dim;
+    for (int i = 0; i < m->nfib; ++i) {
+        const orc_fiber* f = &m->fib[i];
+        const uint32_t bs = D * f->np, o = D * f->idx;
+        if (f->use_blk) {
+            for (uint32_t r = 0; r < bs; ++r) Y[o + r] = 0.0;
+            for (uint32_t c = 0; c < bs; ++c) {
+                double x = X[o + c];
+                const double* col = f->blk + (size_t)bs * c;
+                for (uint32_t r = 0; r < bs; ++r) Y[o + r] += col[r] * x;
+            }
+        } else {
+            memcpy(Y + o, X + o, sizeof(double) * bs);
+        }
+    }
+}
+
+void orc_project(const orc_meca* m, int i, const double* X, double* Y, double sc)
+{
+    const orc_fiber* f = &m->fib[i];
+    fiber_project(m, f, X, Y, sc, m->ltmp + f->seg);
+}
+
+void orc_add_rigidity(const orc_meca* m, int i, const double* X, double* Y)
+{
+    fiber_add_rigidity(m, &m->fib[i], X, Y);
+}
+
+void orc_add_projection_diff(const orc_meca* m, int i, const double* X, double* Y)
+{
+    fiber_add_projection_diff(m, &m->fib[i], X, Y);
+}
+
+/* ------------------------------------------------------------------------ */
+/* solver (bicgstab.{h,cc})                                                  */
+
+typedef struct {
+    int flag;
+    unsigned iter, iter_max;
+    double resid, resid_max;
+} monitor;
+
+static void mon_reset(monitor* mo) { mo->flag = 0; mo->iter = 0; mo->resid = INFINITY; }
+
+/* infinity norm with NaN propagation (blas_xnrm8, cblas.h:283-300) */
+static double nrm_inf(uint32_t n, const double* x)
+{
+    double r = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        double a = fabs(x[i]);
+        if (a > r || a != a) r = a;
+        if (r != r) return r;
+    }
+    return r;
+}
+
+/* Monitor::finished (bicgstab.cc:9-49) */
+static int mon_finished(monitor* mo, uint32_t n, const double* x)
+{
+    mo->resid = nrm_inf(n, x);
+    if (mo->resid != mo->resid) { mo->resid = INFINITY; return 1; }
+    if (mo->iter > mo->iter_max) return 1;
+    return mo->resid < mo->resid_max;
+}
+
+static int mon_finished_flag(monitor* mo, int f, uint32_t n, const double* x)
+{
+    mo->flag = f;
+    return mon_finished(mo, n, x);
+}
+
+static double dot(uint32_t n, const double* a, const double* b)
+{
+    double s = 0.0;
+    for (uint32_t i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+static void axpy(uint32_t n, double a, const double* x, double* y)
+{
+    for (uint32_t i = 0; i < n; ++i) y[i] += a * x[i];
+}
+
+/* Solver::BCGSP (bicgstab.h:247-338); precond=0 gives Solver::BCGS (163-243) */
+static void bicgstab(const orc_meca* m, int precond, const double* rhs, double* x, monitor* mo)
+{
+    const uint32_t n = m->dim * m->npts;
+    double rho1 = 1, rho2, alpha = 0, beta = 0, omega = 1.0, delta;
+    double* mem = (double*)malloc(sizeof(double) * 7 * (n ? n : 1));
+    double *r = mem, *rt = mem + n, *p = mem + 2 * n, *t = mem + 3 * n, *v = mem + 4 * n;
+    double *ph = mem + 5 * n, *sh = mem + 6 * n;
+    memcpy(r, rhs, sizeof(double) * n);
+    orc_multiply(m, x, rt);
+    axpy(n, -1.0, rt, r);
+    memcpy(rt, r, sizeof(double) * n);
+    while (!mon_finished(mo, n, r)) {
+        rho2 = rho1;
+        rho1 = dot(n, rt, r);
+        if (rho1 == 0.0) { mon_finished_flag(mo, 2, n, r); break; }
+        beta = (rho1 / rho2) * (alpha / omega);
+        if (beta == 0.0) {
+            memcpy(p, r, sizeof(double) * n);
+        } else {
+            axpy(n, -omega, v, p);
+            for (uint32_t i = 0; i < n; ++i) p[i] *= beta;
+            axpy(n, 1.0, r, p);
+        }
+        const double* pp = p;
+        if (precond) { orc_precondition(m, p, ph); pp = ph; }
+        orc_multiply(m, pp, v);
+        delta = dot(n, rt, v);
+        if (precond && delta == 0.0) { mon_finished_flag(mo, 4, n, r); break; }
+        alpha = rho1 / delta;
+        axpy(n, -alpha, v, r);
+        axpy(n, alpha, pp, x);
+        const double* ss = r;
+        if (precond) { orc_precondition(m, r, sh); ss = sh; }
+        orc_multiply(m, ss, t);
+        double tdt = dot(n, t, t);
+        if (tdt == 0.0) { mon_finished_flag(mo, 0, n, r); break; }
+        omega = dot(n, t, r) / tdt;
+        if (omega == 0.0) { mon_finished_flag(mo, 3, n, r); break; }
+        axpy(n, omega, ss, x);
+        axpy(n, -omega, t, r);
+        ++mo->iter;
+    }
+    free(mem);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Meca::solve phases                                                        */
+
+int orc_setup(orc_meca* m, const double* gauss)
+{
+    if (!m->prepared) return -1;
+    const int D = m->dim;
+    const uint32_t n = D * m->npts;
+    /* meca.cc:605-619 */
+    m->use_mB = sp_nonzero(&m->mB);
+    if (m->use_mB) sp_prepare(&m->mB);
+    m->use_mC = sp_nonzero(&m->mC);
+    if (m->use_mC) sp_prepare(&m->mC);
+    /* meca.cc:622 */
+    orc_forces_linear(m, m->vPTS, m->vFOR, 1);
+    /* Brownian terms, RigidFiber::addBrownianForces (rigid_fiber.cc:90-98), meca.cc:633-643 */
+    double noise = INFINITY;
+    const double sc = m->prm.kT / m->time_step;
+    for (int i = 0; i < m->nfib; ++i) {
+        const orc_fiber* f = &m->fib[i];
+        const double b = sqrt(2 * sc / f->mobility);
+        const uint32_t o = D * f->idx;
+        if (gauss)
+            for (uint32_t j = 0; j < D * f->np; ++j) m->vFOR[o + j] += b * gauss[o + j];
+        double th = f->mobility * b;
+        if (th < noise) noise = th;
+    }
+    m->noise_level = noise * m->time_step;
+    /* meca.cc:646-651 */
+    for (int i = 0; i < m->nfib; ++i) {
+        const orc_fiber* f = &m->fib[i];
+        const uint32_t o = D * f->idx;
+        fiber_speeds(m, f, m->vFOR + o, m->vRHS + o, m->time_step, 1);
+    }
+    /* meca.cc:687-693 */
+    for (int i = 0; i < m->nfib; ++i) fiber_make_projection_diff(m, &m->fib[i]);
+    (void)n;
+    m->setup_done = 1;
+    return 0;
+}
+
+int orc_iterate(orc_meca* m, const orc_solve_params* sp, const double* guess, orc_report* rep)
+{
+    if (!m->setup_done) return -1;
+    const uint32_t n = m->dim * m->npts;
+    memset(rep, 0, sizeof(*rep));
+    if (guess) memcpy(m->vSOL, guess, sizeof(double) * n);
+    else memset(m->vSOL, 0, sizeof(double) * n);              /* meca.cc:704 */
+    monitor mo;
+    mon_reset(&mo);
+    mo.iter_max = sp->max_iter > 0 ? (unsigned)sp->max_iter : n;   /* meca.cc:731 */
+    mo.resid_max = sp->tol_mode == 1 ? sp->tolerance * nrm_inf(n, m->vRHS)
+                                     : sp->tolerance * m->noise_level;
+    int precond = sp->precondition;
+    int pc_ok = 0;
+    if (precond) { orc_compute_preconditioner(m); pc_ok = 1; }
+    bicgstab(m, precond && pc_ok, m->vRHS, m->vSOL, &mo);          /* meca.cc:737-740 */
+    rep->total_iterations = (int)mo.iter;
+    rep->precond_used = precond && pc_ok;
+    if (!(mo.resid < mo.resid_max)) {                               /* meca.cc:749-799 */
+        memcpy(m->vSOL, m->vRHS, sizeof(double) * n);
+        mon_reset(&mo);
+        bicgstab(m, precond, m->vRHS, m->vSOL, &mo);
+        rep->total_iterations += (int)mo.iter;
+        if (mo.resid < mo.resid_max) {
+            rep->fallback = 1;
+        } else {
+            memset(m->vSOL, 0, sizeof(double) * n);
+            mon_reset(&mo);
+            if (precond) {
+                bicgstab(m, 0, m->vRHS, m->vSOL, &mo);
+                rep->precond_used = 0;
+            } else {
+                orc_compute_preconditioner(m);
+                bicgstab(m, 1, m->vRHS, m->vSOL, &mo);
+                rep->precond_used = 1;
+            }
+            rep->total_iterations += (int)mo.iter;
+            rep->fallback = (mo.resid < mo.resid_max) ? 2 : 3;
+        }
+    }
+    rep->iterations = (int)mo.iter;
+    rep->flag = mo.flag;
+    rep->residual = mo.resid;
+    rep->threshold = mo.resid_max;
+    rep->converged = mo.resid < mo.resid_max;
+    rep->noise_level = m->noise_level;
+    return rep->fallback == 3 ? 1 : 0;   /* 1: "convergence failure in solver" */
+}
+
+/* FiberNaked::reshape_two (fiber_naked.cc:172-193) */
+static void reshape_two(int D, const double* src, double* dst, double cut)
+{
+    double dif[3] = { 0, 0, 0 }, nn = 0.0;
+    for (int d = 0; d < D; ++d) { dif[d] = src[D + d] - src[d]; nn += dif[d] * dif[d]; }
+    double s = 0.5 * (1 - cut / sqrt(nn));
+    for (int d = 0; d < D; ++d) {
+        double v = dif[d] * s;
+        dst[d] = src[d] + v;
+        dst[D + d] = src[D + d] - v;
+    }
+}
+
+/* FiberNaked::reshape_it (fiber_naked.cc:221-354): Newton on the segment
+   lengths with a tridiagonal (dgtsv) linear solve; returns info != 0 on failure. */
+static int reshape_it(int D, uint32_t ns, const double* src, double* dst, double cut)
+{
+    const double a2 = cut * cut;
+    double* mem = (double*)malloc(sizeof(double) * ns * (2 * 3 + 5));
+    double *dif = mem, *vec = mem + 3 * ns, *sca = mem + 6 * ns, *val = sca + ns;
+    double *dia = val + ns, *low = dia + ns, *upe = low + ns;
+    int info = 0;
+    for (uint32_t p = 0; p < ns; ++p) {
+        for (int d = 0; d < 3; ++d) dif[3 * p + d] = d < D ? src[D * p + D + d] - src[D * p + d] : 0.0;
+        sca[p] = 0;
+    }
+    double err = 0;
+    unsigned it = 0;
+#define DF(p, d) dif[3 * (p) + (d)]
+#define VC(p, d) vec[3 * (p) + (d)]
+    do {
+        for (int d = 0; d < 3; ++d) {
+            VC(0, d) = (1 - 2 * sca[0]) * DF(0, d) + sca[1] * DF(1, d);
+            for (uint32_t p = 1; p + 1 < ns; ++p)
+                VC(p, d) = sca[p - 1] * DF(p - 1, d) + (1 - 2 * sca[p]) * DF(p, d) + sca[p + 1] * DF(p + 1, d);
+            VC(ns - 1, d) = sca[ns - 2] * DF(ns - 2, d) + (1 - 2 * sca[ns - 1]) * DF(ns - 1, d);
+        }
+        {
+            double vv = 0, vd = 0;
+            for (int d = 0; d < 3; ++d) { vv += VC(0, d) * VC(0, d); vd += VC(0, d) * DF(0, d); }
+            val[0] = vv - a2;
+            dia[0] = -2 * vd;
+        }
+        for (uint32_t p = 1; p < ns; ++p) {
+            double vv = 0, vl = 0, vd = 0, vu = 0;
+            for (int d = 0; d < 3; ++d) {
+                vv += VC(p, d) * VC(p, d);
+                vl += VC(p, d) * DF(p - 1, d);
+                vd += VC(p, d) * DF(p, d);
+                vu += VC(p - 1, d) * DF(p, d);
+            }
+            val[p] = vv - a2;
+            low[p] = vl;
+            dia[p] = -2 * vd;
+            upe[p - 1] = vu;
+        }
+        info = orc_dgtsv((int)ns, low + 1, dia, upe, val);
+        if (info) goto finish;
+        err = 0;
+        for (uint32_t p = 0; p < ns; ++p) {
+            sca[p] += -0.5 * val[p];
+            err += fabs(val[p]);
+        }
+        if (++it > 32) { info = 1; goto finish; }
+    } while (err > 0.001);
+    {
+        double e[3], dd[3] = { 0, 0, 0 };
+        for (int d = 0; d < 3; ++d) e[d] = sca[0] * DF(0, d);
+        for (int d = 0; d < D; ++d) dst[d] = src[d] + e[d];
+        for (uint32_t p = 1; p < ns; ++p) {
+            for (int d = 0; d < 3; ++d) dd[d] = sca[p] * DF(p, d);
+            for (int d = 0; d < D; ++d) dst[D * p + d] = src[D * p + d] + dd[d] - e[d];
+            for (int d = 0; d < 3; ++d) e[d] = dd[d];
+        }
+        for (int d = 0; d < D; ++d) dst[D * ns + d] = src[D * ns + d] - dd[d];
+    }
+#undef DF
+#undef VC
+finish:
+    free(mem);
+    return info;
+}
+
+/* FiberNaked::reshape_sure (fiber_naked.cc:367-401) */
+static void reshape_sure(int D, uint32_t ns, double* x, double cut)
+{
+    double dp[3] = { 0, 0, 0 }, sum[3] = { 0, 0, 0 }, seg[3];
+    double dis = 0;
+    for (int d = 0; d < D; ++d) { seg[d] = x[D + d] - x[d]; dis += seg[d] * seg[d]; }
+    dis = sqrt(dis);
+    if (dis > REAL_EPSILON)
+        for (int d = 0; d < D; ++d) dp[d] = (cut / dis - 1.0) * seg[d];
+    for (uint32_t p = 1; p < ns; ++p) {
+        dis = 0;
+        for (int d = 0; d < D; ++d) { seg[d] = x[D * p + D + d] - x[D * p + d]; dis += seg[d] * seg[d]; }
+        dis = sqrt(dis);
+        for (int d = 0; d < D; ++d) { x[D * p + d] += dp[d]; sum[d] += dp[d]; }
+        if (dis > REAL_EPSILON)
+            for (int d = 0; d < D; ++d) dp[d] += (cut / dis - 1.0) * seg[d];
+    }
+    for (int d = 0; d < D; ++d) x[D * ns + d] += dp[d];
+    for (int d = 0; d < D; ++d) sum[d] = (sum[d] + dp[d]) * (-1.0 / (ns + 1));
+    for (uint32_t p = 0; p <= ns; ++p)
+        for (int d = 0; d < D; ++d) x[D * p + d] += sum[d];
+}
+
+/* FiberNaked::getPoints (fiber_naked.cc:414-425) */
+int orc_reshape(int D, uint32_t ns, const double* src, double* dst, double cut)
+{
+    if (ns == 1) { reshape_two(D, src, dst, cut); return 0; }
+    if (reshape_it(D, ns, src, dst, cut)) {
+        memcpy(dst, src, sizeof(double) * D * (ns + 1));
+        reshape_sure(D, ns, dst, cut);
+        return 1;
+    }
+    return 0;
+}
+
+int orc_apply(orc_meca* m)
+{
+    const int D = m->dim;
+    const uint32_t n = D * m->npts;
+    axpy(n, 1.0, m->vSOL, m->vPTS);                 /* meca.cc:802 */
+    orc_forces_linear(m, m->vPTS, m->vFOR, 0);      /* meca.cc:826 */
+    for (int i = 0; i < m->nfib; ++i) {             /* meca.cc:830-835 */
+        const orc_fiber* f = &m->fib[i];
+        orc_reshape(D, f->np - 1, m->vPTS + D * f->idx, m->pos + D * f->idx, f->cut);
+    }
+    return 0;
+}
+
+int orc_solve(orc_meca* m, const orc_solve_params* sp, const double* gauss, orc_report* rep)
+{
+    int r = orc_setup(m, gauss);
+    if (r) return r;
+    r = orc_iterate(m, sp, NULL, rep);
+    if (r) return r;
+    return orc_apply(m);
+}
+
+/* Meca::computeForces() (meca.cc:550-580) with RigidFiber::computeTensions
+   (rigid_fiber_project.cc:153-161) */
+void orc_compute_forces(orc_meca* m)
+{
+    const int D = m->dim;
+    m->use_mB = sp_nonzero(&m->mB);
+    if (m->use_mB) sp_prepare(&m->mB);
+    m->use_mC = sp_nonzero(&m->mC);
+    if (m->use_mC) sp_prepare(&m->mC);
+    orc_forces_linear(m, m->vPTS, m->vFOR, 0);
+    for (int i = 0; i < m->nfib; ++i) {
+        const orc_fiber* f = &m->fib[i];
+        const uint32_t S = f->np - 1;
+        const double* dif = m->diff + D * f->seg;
+        const double* F = m->vFOR + D * f->idx;
+        double* lag = m->lag + f->seg;
+        for (uint32_t s = 0; s < S; ++s) {
+            double t = 0.0;
+            for (int d = 0; d < D; ++d) t += dif[D * s + d] * (F[D * s + D + d] - F[D * s + d]);
+            lag[s] = t;
+        }
+        orc_dptts2((int)S, m->jd + f->seg, m->je + f->seg, lag);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* export                                                                    */
+
+void orc_get_vector(const orc_meca* m, int which, double* out)
+{
+    const double* src = NULL;
+    switch (which) {
+        case ORC_PTS: src = m->vPTS; break;
+        case ORC_SOL: src = m->vSOL; break;
+        case ORC_BAS: src = m->vBAS; break;
+        case ORC_RHS: src = m->vRHS; break;
+        case ORC_FOR: src = m->vFOR; break;
+        default: return;
+    }
+    memcpy(out, src, sizeof(double) * m->dim * m->npts);
+}
+
+void orc_get_fiber_points(const orc_meca* m, double* out)
+{
+    memcpy(out, m->pos, sizeof(double) * m->dim * m->npts);
+}
+
+void orc_get_tensions(const orc_meca* m, double* out)
+{
+    memcpy(out, m->lag, sizeof(double) * m->nseg);
+}
+
+double orc_get_mobility(const orc_meca* m, int i) { return m->fib[i].mobility; }
+
+void orc_get_jjt(const orc_meca* m, int i, double* diag, double* offd)
+{
+    const orc_fiber* f = &m->fib[i];
+    memcpy(diag, m->jd + f->seg, sizeof(double) * (f->np - 1));
+    if (f->np > 2) memcpy(offd, m->je + f->seg, sizeof(double) * (f->np - 2));
+}
+
+uint32_t orc_get_matrix_nr(orc_meca* m, int which, uint32_t* ija, double* sa)
+{
+    spsym* s = which ? &m->mC : &m->mB;
+    sp_prepare(s);
+    uint32_t nbe = s->ija[s->size];
+    if (ija) memcpy(ija, s->ija, sizeof(uint32_t) * nbe);
+    if (sa) memcpy(sa, s->sa, sizeof(double) * nbe);
+    return nbe;
+}
+
+void orc_get_matrix_dense(const orc_meca* m, int which, double* out)
+{
+    const spsym* s = which ? &m->mC : &m->mB;
+    const size_t n = s->size;
+    memset(out, 0, sizeof(double) * n * n);
+    for (uint32_t j = 0; j < s->size; ++j)
+        for (uint32_t k = 0; k < s->colSize[j]; ++k) {
+            uint32_t i = s->col[j][k].line;
+            out[i + n * j] = s->col[j][k].val;
+            out[j + n * i] = s->col[j][k].val;
+        }
+}

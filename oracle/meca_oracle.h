@@ -1,0 +1,130 @@
+/*
+ * meca_oracle.h -- CPU restatement of Cytosim's implicit mechanics step (Meca).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path and the single-threaded CPU baseline timed beside it.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product library never links or calls it.
+ *
+ * Semantics follow the 2017 reference (FP64, `real` = double, DIM = 2 or 3 at
+ * run time).  Every function cites the reference file:line it restates.
+ */
+#ifndef MECA_ORACLE_H
+#define MECA_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_meca orc_meca;
+
+typedef struct {
+    double time_step;     /* SimulProp::time_step */
+    double kT;            /* SimulProp::kT */
+    double viscosity;     /* SimulProp::viscosity */
+    double hydro_radius0; /* FiberProp::hydrodynamic_radius[0] (default 0.0125) */
+    double hydro_radius1; /* FiberProp::hydrodynamic_radius[1] (default 10) */
+} orc_params;
+
+typedef struct {
+    double tolerance;     /* SimulProp::tolerance */
+    int    tol_mode;      /* 0: |r|_inf < tolerance*noiseLevel (meca.cc:731); 1: |r|_inf < tolerance*|b|_inf */
+    int    precondition;  /* SimulProp::precondition */
+    int    max_iter;      /* <=0: DIM*nbPts (meca.cc:731) */
+} orc_solve_params;
+
+typedef struct {
+    int    iterations;    /* Monitor::iterations of the final attempt */
+    int    flag;          /* Monitor::flag */
+    int    converged;
+    int    fallback;      /* 0 none, 1 rescued by seed vRHS, 2 rescued by method switch, 3 dead */
+    int    precond_used;  /* 1 if the final attempt used BCGSP */
+    double residual;      /* Monitor::residual */
+    double threshold;     /* residual threshold of the monitor */
+    double noise_level;   /* noiseLevel (meca.cc:633-643) */
+    int    total_iterations; /* sum over all attempts */
+} orc_report;
+
+orc_meca* orc_create(int dim);
+void      orc_destroy(orc_meca*);
+void      orc_set_params(orc_meca*, const orc_params*);
+
+/* Meca::clear / Meca::add (meca.cc:55-79) for a RigidFiber Mecable */
+void      orc_clear(orc_meca*);
+int       orc_add_fiber(orc_meca*, uint32_t nb_points, double segmentation, double rigidity);
+uint32_t  orc_nb_points(const orc_meca*);
+uint32_t  orc_mat_index(const orc_meca*, int mecable);
+
+/* FiberNaked::bestNbPoints (fiber_naked.cc:22-30) */
+uint32_t  orc_best_nb_points(double ratio);
+
+/* Meca::prepare (meca.cc:488-537): points of all Mecables, DIM*nbPts, in matIndex order */
+int       orc_prepare(orc_meca*, const double* points);
+
+/* FiberNaked::interpolate (fiber_naked.cc:1050-1065) with fnAbscissa = 0 */
+void      orc_interpolate(const orc_meca*, int fiber, double abscissa,
+                          uint32_t* mat1, uint32_t* mat2, double* coef);
+
+/* assembly (meca_inter.cc) -- indices are matrix indices (matIndex) */
+void      orc_add_pure_force(orc_meca*, uint32_t a, const double* force);
+void      orc_inter_link_ee(orc_meca*, uint32_t a, uint32_t b, double w);
+void      orc_inter_link_ie(orc_meca*, uint32_t a1, uint32_t a2, double ca, uint32_t b, double w);
+void      orc_inter_link_ii(orc_meca*, uint32_t a1, uint32_t a2, double ca,
+                            uint32_t b1, uint32_t b2, double cb, double w);
+void      orc_inter_long_link_ee(orc_meca*, uint32_t a, uint32_t b, double len, double w);
+void      orc_inter_clamp_e(orc_meca*, uint32_t a, const double* g, double w);
+void      orc_inter_clamp_i(orc_meca*, uint32_t a1, uint32_t a2, double ca, const double* g, double w);
+void      orc_inter_long_clamp(orc_meca*, uint32_t a, const double* center, double len, double w);
+void      orc_inter_plane_e(orc_meca*, uint32_t a, const double* dir, const double* g, double w);
+/* Fiber::setInteractions CONFINE_INSIDE with SpaceSphere (fiber.cc:471-481; space_sphere.cc:148-151) */
+void      orc_confine_sphere(orc_meca*, double radius, double stiffness);
+
+/* Meca::solve (meca.cc:598-845), split in phases:
+   setup   = matrices ready, forces, Brownian, RHS, P' (meca.cc:605-693)
+   iterate = preconditioner + BCGSP/BCGS + fallback chain (meca.cc:704-799)
+   apply   = vPTS += vSOL, force recompute, export (meca.cc:802-835)
+   gauss: N(0,1) draws, one per degree of freedom, in object order.        */
+int       orc_setup(orc_meca*, const double* gauss);
+int       orc_iterate(orc_meca*, const orc_solve_params*, const double* guess, orc_report*);
+int       orc_apply(orc_meca*);
+int       orc_solve(orc_meca*, const orc_solve_params*, const double* gauss, orc_report*);
+
+/* operator-level entry points, valid after orc_setup */
+void      orc_multiply(const orc_meca*, const double* X, double* Y);      /* meca.cc:157-181 */
+int       orc_compute_preconditioner(orc_meca*);                          /* meca.cc:411-444 */
+void      orc_precondition(const orc_meca*, const double* X, double* Y);  /* meca.cc:448-466 */
+void      orc_get_block(const orc_meca*, int mecable, double* blk);       /* meca.cc:231-283 */
+int       orc_get_inverse_block(const orc_meca*, int mecable, double* blk);
+void      orc_project(const orc_meca*, int mecable, const double* X, double* Y, double sc); /* rigid_fiber_project.cc:137-149 */
+void      orc_add_rigidity(const orc_meca*, int mecable, const double* X, double* Y);      /* rigid_fiber.cc:294-303 */
+void      orc_add_projection_diff(const orc_meca*, int mecable, const double* X, double* Y);
+void      orc_compute_forces(orc_meca*);                                  /* meca.cc:550-580 */
+void      orc_forces_linear(const orc_meca*, const double* X, double* Y, int with_rigidity); /* meca.cc:143-147 */
+
+/* state export */
+enum { ORC_PTS = 0, ORC_SOL, ORC_BAS, ORC_RHS, ORC_FOR };
+void      orc_get_vector(const orc_meca*, int which, double* out);
+void      orc_get_fiber_points(const orc_meca*, double* out);   /* points held by the Mecables */
+void      orc_get_tensions(const orc_meca*, double* out);       /* rfLag, one per segment */
+double    orc_get_mobility(const orc_meca*, int mecable);
+void      orc_get_jjt(const orc_meca*, int mecable, double* diag, double* offd); /* factored pttrf */
+/* mB (which=0) or mC (which=1) in the Numerical-Recipes layout of prepareForMultiply
+   (matsparsesym1.cc:472-523); returns nbe; arrays may be NULL to query the size */
+uint32_t  orc_get_matrix_nr(orc_meca*, int which, uint32_t* ija, double* sa);
+/* dense (size x size) symmetric matrix of mB or mC, column-major */
+void      orc_get_matrix_dense(const orc_meca*, int which, double* out);
+
+/* LAPACK restatements (documented algorithms), exported for pinning tests */
+int       orc_dgetrf(int n, double* a, int lda, int* ipiv);
+int       orc_dgetri(int n, double* a, int lda, const int* ipiv);
+int       orc_dpttrf(int n, double* d, double* e);
+void      orc_dptts2(int n, const double* d, const double* e, double* b);
+int       orc_dgtsv(int n, double* dl, double* d, double* du, double* b);
+int       orc_reshape(int dim, uint32_t ns, const double* src, double* dst, double cut);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

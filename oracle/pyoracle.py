@@ -1,0 +1,385 @@
+"""ctypes binding of the CPU oracle (oracle/meca_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker and as the timed CPU restatement.
+The product path never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libmeca_oracle.so")
+
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class Params(C.Structure):
+    _fields_ = [("time_step", C.c_double), ("kT", C.c_double), ("viscosity", C.c_double),
+                ("hydro_radius0", C.c_double), ("hydro_radius1", C.c_double)]
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("tol_mode", C.c_int),
+                ("precondition", C.c_int), ("max_iter", C.c_int)]
+
+
+class Report(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("flag", C.c_int), ("converged", C.c_int),
+                ("fallback", C.c_int), ("precond_used", C.c_int), ("residual", C.c_double),
+                ("threshold", C.c_double), ("noise_level", C.c_double),
+                ("total_iterations", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc only)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "orc_create": (vp, [C.c_int]),
+        "orc_destroy": (None, [vp]),
+        "orc_set_params": (None, [vp, C.POINTER(Params)]),
+        "orc_clear": (None, [vp]),
+        "orc_add_fiber": (C.c_int, [vp, C.c_uint32, C.c_double, C.c_double]),
+        "orc_nb_points": (C.c_uint32, [vp]),
+        "orc_mat_index": (C.c_uint32, [vp, C.c_int]),
+        "orc_best_nb_points": (C.c_uint32, [C.c_double]),
+        "orc_prepare": (C.c_int, [vp, _f64p]),
+        "orc_interpolate": (None, [vp, C.c_int, C.c_double, C.POINTER(C.c_uint32),
+                                   C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
+        "orc_add_pure_force": (None, [vp, C.c_uint32, _f64p]),
+        "orc_inter_link_ee": (None, [vp, C.c_uint32, C.c_uint32, C.c_double]),
+        "orc_inter_link_ie": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_double]),
+        "orc_inter_link_ii": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32,
+                                     C.c_uint32, C.c_double, C.c_double]),
+        "orc_inter_long_link_ee": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_double]),
+        "orc_inter_clamp_e": (None, [vp, C.c_uint32, _f64p, C.c_double]),
+        "orc_inter_clamp_i": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, _f64p, C.c_double]),
+        "orc_inter_long_clamp": (None, [vp, C.c_uint32, _f64p, C.c_double, C.c_double]),
+        "orc_inter_plane_e": (None, [vp, C.c_uint32, _f64p, _f64p, C.c_double]),
+        "orc_confine_sphere": (None, [vp, C.c_double, C.c_double]),
+        "orc_setup": (C.c_int, [vp, vp]),
+        "orc_iterate": (C.c_int, [vp, C.POINTER(SolveParams), vp, C.POINTER(Report)]),
+        "orc_apply": (C.c_int, [vp]),
+        "orc_solve": (C.c_int, [vp, C.POINTER(SolveParams), vp, C.POINTER(Report)]),
+        "orc_multiply": (None, [vp, _f64p, _f64p]),
+        "orc_compute_preconditioner": (C.c_int, [vp]),
+        "orc_precondition": (None, [vp, _f64p, _f64p]),
+        "orc_get_block": (None, [vp, C.c_int, _f64p]),
+        "orc_get_inverse_block": (C.c_int, [vp, C.c_int, _f64p]),
+        "orc_project": (None, [vp, C.c_int, _f64p, _f64p, C.c_double]),
+        "orc_add_rigidity": (None, [vp, C.c_int, _f64p, _f64p]),
+        "orc_add_projection_diff": (None, [vp, C.c_int, _f64p, _f64p]),
+        "orc_compute_forces": (None, [vp]),
+        "orc_forces_linear": (None, [vp, _f64p, _f64p, C.c_int]),
+        "orc_get_vector": (None, [vp, C.c_int, _f64p]),
+        "orc_get_fiber_points": (None, [vp, _f64p]),
+        "orc_get_tensions": (None, [vp, _f64p]),
+        "orc_get_mobility": (C.c_double, [vp, C.c_int]),
+        "orc_get_jjt": (None, [vp, C.c_int, _f64p, _f64p]),
+        "orc_get_matrix_nr": (C.c_uint32, [vp, C.c_int, vp, vp]),
+        "orc_get_matrix_dense": (None, [vp, C.c_int, _f64p]),
+        "orc_dgetrf": (C.c_int, [C.c_int, _f64p, C.c_int, _i32p]),
+        "orc_dgetri": (C.c_int, [C.c_int, _f64p, C.c_int, _i32p]),
+        "orc_dpttrf": (C.c_int, [C.c_int, _f64p, _f64p]),
+        "orc_dptts2": (None, [C.c_int, _f64p, _f64p, _f64p]),
+        "orc_dgtsv": (C.c_int, [C.c_int, _f64p, _f64p, _f64p, _f64p]),
+        "orc_reshape": (C.c_int, [C.c_int, C.c_uint32, _f64p, _f64p, C.c_double]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+PTS, SOL, BAS, RHS, FOR = range(5)
+
+
+class OracleMeca:
+    """Thin object wrapper over one oracle Meca instance."""
+
+    def __init__(self, dim, params=None):
+        self.L = lib()
+        self.dim = dim
+        self.h = self.L.orc_create(dim)
+        if not self.h:
+            raise ValueError("dim must be 2 or 3")
+        self.nfib = 0
+        self.np = []
+        if params is not None:
+            self.set_params(**params)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.orc_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_params(self, time_step, kT, viscosity, hydro_radius0=0.0125, hydro_radius1=10.0):
+        p = Params(time_step, kT, viscosity, hydro_radius0, hydro_radius1)
+        self.L.orc_set_params(self.h, C.byref(p))
+
+    def clear(self):
+        self.L.orc_clear(self.h)
+        self.nfib = 0
+        self.np = []
+
+    def add_fiber(self, nb_points, segmentation, rigidity):
+        r = self.L.orc_add_fiber(self.h, int(nb_points), float(segmentation), float(rigidity))
+        if r < 0:
+            raise ValueError("fiber needs at least 2 points")
+        self.nfib += 1
+        self.np.append(int(nb_points))
+        return r
+
+    @property
+    def nb_points(self):
+        return int(self.L.orc_nb_points(self.h))
+
+    @property
+    def size(self):
+        return self.dim * self.nb_points
+
+    def mat_index(self, f):
+        return int(self.L.orc_mat_index(self.h, f))
+
+    def prepare(self, points):
+        pts = np.ascontiguousarray(points, dtype=np.float64).ravel()
+        assert pts.size == self.size
+        r = self.L.orc_prepare(self.h, pts)
+        if r:
+            raise RuntimeError("could not build Fiber projection")
+
+    def interpolate(self, fiber, abscissa):
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_double()
+        self.L.orc_interpolate(self.h, fiber, abscissa, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def inter_link_ii(self, a1, a2, ca, b1, b2, cb, w):
+        self.L.orc_inter_link_ii(self.h, a1, a2, ca, b1, b2, cb, w)
+
+    def inter_link_ie(self, a1, a2, ca, b, w):
+        self.L.orc_inter_link_ie(self.h, a1, a2, ca, b, w)
+
+    def inter_link_ee(self, a, b, w):
+        self.L.orc_inter_link_ee(self.h, a, b, w)
+
+    def inter_long_link_ee(self, a, b, length, w):
+        self.L.orc_inter_long_link_ee(self.h, a, b, length, w)
+
+    def inter_clamp_e(self, a, g, w):
+        self.L.orc_inter_clamp_e(self.h, a, np.ascontiguousarray(g, dtype=np.float64), w)
+
+    def inter_clamp_i(self, a1, a2, ca, g, w):
+        self.L.orc_inter_clamp_i(self.h, a1, a2, ca, np.ascontiguousarray(g, dtype=np.float64), w)
+
+    def inter_long_clamp(self, a, center, length, w):
+        self.L.orc_inter_long_clamp(self.h, a, np.ascontiguousarray(center, dtype=np.float64), length, w)
+
+    def inter_plane_e(self, a, direction, g, w):
+        self.L.orc_inter_plane_e(self.h, a, np.ascontiguousarray(direction, dtype=np.float64),
+                                 np.ascontiguousarray(g, dtype=np.float64), w)
+
+    def add_pure_force(self, a, force):
+        self.L.orc_add_pure_force(self.h, a, np.ascontiguousarray(force, dtype=np.float64))
+
+    def confine_sphere(self, radius, stiffness):
+        self.L.orc_confine_sphere(self.h, radius, stiffness)
+
+    def setup(self, gauss=None):
+        g = None if gauss is None else np.ascontiguousarray(gauss, dtype=np.float64)
+        r = self.L.orc_setup(self.h, _ptr(g))
+        if r:
+            raise RuntimeError("setup before prepare")
+
+    def iterate(self, tolerance, tol_mode=0, precondition=1, max_iter=0, guess=None):
+        sp = SolveParams(tolerance, tol_mode, precondition, max_iter)
+        rep = Report()
+        g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64)
+        r = self.L.orc_iterate(self.h, C.byref(sp), _ptr(g), C.byref(rep))
+        out = rep.as_dict()
+        out["failed"] = bool(r)
+        return out
+
+    def apply(self):
+        self.L.orc_apply(self.h)
+
+    def solve(self, tolerance, gauss=None, tol_mode=0, precondition=1, max_iter=0):
+        self.setup(gauss)
+        rep = self.iterate(tolerance, tol_mode, precondition, max_iter)
+        if rep["failed"]:
+            raise RuntimeError("convergence failure in solver")
+        self.apply()
+        return rep
+
+    def multiply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self.L.orc_multiply(self.h, x, y)
+        return y
+
+    def compute_preconditioner(self):
+        self.L.orc_compute_preconditioner(self.h)
+
+    def precondition(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self.L.orc_precondition(self.h, x, y)
+        return y
+
+    def get_block(self, f):
+        bs = self.dim * self.np[f]
+        out = np.zeros(bs * bs)
+        self.L.orc_get_block(self.h, f, out)
+        return out.reshape(bs, bs).T.copy()     # column-major -> row-major matrix
+
+    def get_inverse_block(self, f):
+        bs = self.dim * self.np[f]
+        out = np.zeros(bs * bs)
+        ok = self.L.orc_get_inverse_block(self.h, f, out)
+        return out.reshape(bs, bs).T.copy() if ok else None
+
+    def project(self, f, x, sc=1.0):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self.L.orc_project(self.h, f, x, y, sc)
+        return y
+
+    def add_rigidity(self, f, x, y=None):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x) if y is None else np.ascontiguousarray(y, dtype=np.float64).copy()
+        self.L.orc_add_rigidity(self.h, f, x, y)
+        return y
+
+    def add_projection_diff(self, f, x, y=None):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x) if y is None else np.ascontiguousarray(y, dtype=np.float64).copy()
+        self.L.orc_add_projection_diff(self.h, f, x, y)
+        return y
+
+    def forces_linear(self, x, with_rigidity=True):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self.L.orc_forces_linear(self.h, x, y, int(with_rigidity))
+        return y
+
+    def compute_forces(self):
+        self.L.orc_compute_forces(self.h)
+
+    def vector(self, which):
+        out = np.empty(self.size)
+        self.L.orc_get_vector(self.h, which, out)
+        return out
+
+    def fiber_points(self):
+        out = np.empty(self.size)
+        self.L.orc_get_fiber_points(self.h, out)
+        return out
+
+    def tensions(self):
+        out = np.empty(self.nb_points - self.nfib)
+        self.L.orc_get_tensions(self.h, out)
+        return out
+
+    def mobility(self, f):
+        return self.L.orc_get_mobility(self.h, f)
+
+    def jjt(self, f):
+        S = self.np[f] - 1
+        d = np.empty(S)
+        e = np.empty(max(S - 1, 1))
+        self.L.orc_get_jjt(self.h, f, d, e)
+        return d, e[:S - 1]
+
+    def matrix_nr(self, which):
+        nbe = self.L.orc_get_matrix_nr(self.h, which, None, None)
+        ija = np.empty(nbe, dtype=np.uint32)
+        sa = np.empty(nbe)
+        self.L.orc_get_matrix_nr(self.h, which, ija.ctypes.data_as(C.c_void_p),
+                                 sa.ctypes.data_as(C.c_void_p))
+        return ija, sa
+
+    def matrix_dense(self, which):
+        n = self.nb_points if which == 0 else self.size
+        out = np.empty(n * n)
+        self.L.orc_get_matrix_dense(self.h, which, out)
+        return out.reshape(n, n)
+
+
+# LAPACK restatements -------------------------------------------------------
+
+def dgetrf(a):
+    a = np.asfortranarray(a, dtype=np.float64).copy(order="F")
+    n = a.shape[0]
+    buf = a.ravel(order="F").copy()
+    ipiv = np.zeros(n, dtype=np.int32)
+    info = lib().orc_dgetrf(n, buf, n, ipiv)
+    return buf.reshape(n, n, order="F"), ipiv, info
+
+
+def dgetri(lu, ipiv):
+    n = lu.shape[0]
+    buf = np.asfortranarray(lu, dtype=np.float64).ravel(order="F").copy()
+    info = lib().orc_dgetri(n, buf, n, np.ascontiguousarray(ipiv, dtype=np.int32))
+    return buf.reshape(n, n, order="F"), info
+
+
+def dpttrf(d, e):
+    d = np.array(d, dtype=np.float64)
+    e = np.array(e, dtype=np.float64) if len(e) else np.zeros(1)
+    info = lib().orc_dpttrf(len(d), d, e)
+    return d, e[:len(d) - 1], info
+
+
+def dptts2(d, e, b):
+    b = np.array(b, dtype=np.float64)
+    e = np.array(e, dtype=np.float64) if len(e) else np.zeros(1)
+    lib().orc_dptts2(len(d), np.ascontiguousarray(d, dtype=np.float64), e, b)
+    return b
+
+
+def dgtsv(dl, d, du, b):
+    dl = np.array(dl, dtype=np.float64) if len(dl) else np.zeros(1)
+    du = np.array(du, dtype=np.float64) if len(du) else np.zeros(1)
+    d = np.array(d, dtype=np.float64)
+    b = np.array(b, dtype=np.float64)
+    info = lib().orc_dgtsv(len(d), dl, d, du, b)
+    return b, info
+
+
+def reshape(dim, src, cut):
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    dst = np.empty_like(src)
+    ns = src.size // dim - 1
+    failed = lib().orc_reshape(dim, ns, src, dst, cut)
+    return dst, failed
+
+
+def best_nb_points(ratio):
+    return int(lib().orc_best_nb_points(float(ratio)))
