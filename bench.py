@@ -1,0 +1,168 @@
+"""Benchmark of the MI355X implicit mechanics step (one GPU).
+
+Workload: 3D, 1000 fibers with lengths uniform in [2, 15] um (segmentation 0.5,
+5-31 points), rigidity 20, confined in a sphere of radius 10 (stiffness 200),
+20000 crosslinkers (stiffness 100) bound at crossings within 0.05 um,
+viscosity 0.02, time_step 0.005, kT 0.0042.  A step is one implicit step:
+Meca::prepare + device assembly (confinement, couples) + Meca::solve at
+relative residual |r|_inf/|b|_inf < 1e-6 + apply (update, forces, reshape).
+Fiber points and the per-step N(0,1) Brownian draws are resident in HBM before
+the timed region.
+
+Prints one JSON line.  --gpus N with N > 1 exits: every box has one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import __graft_entry__  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_matvec(sc, kb_full, kc_full):
+    """Compulsory HBM bytes of one fused operator launch (Meca::multiply):
+    x in and y out (16n), CSR value+column of every stored entry of the full
+    symmetric mB/mC rows (12 per entry), the gathered coupled x (8n upper
+    bound: each point's coordinates once), per segment rfDiff (8*DIM), pttrf
+    factors (16) and P' coefficient (8)."""
+    n = sc.size
+    S = sc.npts - sc.nfib
+    return 16 * n + 12 * (kb_full + kc_full) + 8 * n + (8 * sc.dim + 24) * S
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nfib", type=int, default=1000)
+    ap.add_argument("--nlinks", type=int, default=20000)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps timed for cpu_baseline (0: skip)")
+    ap.add_argument("--modes", default="fp64,fp32")
+    return ap.parse_args()
+
+
+def run_mode(pkg, sc, precond, steps, warmup, tol, gauss_dev, torch):
+    m = pkg.Meca(sc.dim, 0)
+    pkg.load_scene(m, sc)
+    iters = []
+
+    def one(step):
+        if step > 0:
+            m.prepare(None)
+            pkg.assemble(m, sc, upload_couples=False)
+        rep = m.solve(tol, tol_mode=pkg.TOL_RELATIVE, precondition=precond,
+                      gauss_device_ptr=gauss_dev[step % len(gauss_dev)].data_ptr())
+        iters.append(rep["iterations"])
+        return rep
+
+    for s in range(warmup):
+        one(s)
+    torch.cuda.synchronize()
+    iters.clear()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        one(warmup + s)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return m, dt, iters
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box here has one GPU, so --gpus {args.gpus} "
+              "is not run (the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    pkg = __graft_entry__.load_package()
+    import importlib
+    scenes = importlib.import_module(__graft_entry__.PKG_NAME + ".scene")
+    print(f"bench: generating scene nfib={args.nfib} nlinks={args.nlinks}", file=sys.stderr, flush=True)
+    sc = scenes.config2(seed=2, nfib=args.nfib, nlinks=args.nlinks)
+    nsteps_total = args.warmup + args.steps
+    gauss_dev = [torch.from_numpy(sc.gauss(s)).to("cuda:0") for s in range(min(nsteps_total, 8))]
+    torch.cuda.synchronize()
+
+    results = {}
+    for mode in args.modes.split(","):
+        precond = {"fp64": pkg.PRECOND_FP64, "fp32": pkg.PRECOND_FP32}[mode]
+        print(f"bench: mode {mode} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
+        m, dt, iters = run_mode(pkg, sc, precond, args.steps, args.warmup, args.tol, gauss_dev, torch)
+        results[mode] = dict(seconds=dt, steps_per_s=args.steps / dt, ms_per_step=1e3 * dt / args.steps,
+                             mean_iters=float(np.mean(iters)), meca=m)
+        print(f"bench: {mode} {args.steps / dt:.2f} steps/s, mean iterations {np.mean(iters):.1f}",
+              file=sys.stderr, flush=True)
+
+    # dominant-kernel timing on the final state of the FP64 run
+    m = results["fp64"]["meca"]
+    m.setup(gauss_device_ptr=gauss_dev[0].data_ptr())
+    mv_ms = m.time_kernel(0, reps=200)
+    m.compute_preconditioner(pkg.PRECOND_FP64)
+    pc_ms = m.time_kernel(1, reps=200)
+    build_ms = m.time_kernel(2, reps=5)
+    rB, cB, vB = m.matrix_entries(0)
+    rC, cC, vC = m.matrix_entries(1)
+    kb_full = 2 * int((rB != cB).sum()) + int((rB == cB).sum())
+    kc_full = 2 * int((rC != cC).sum()) + int((rC == cC).sum())
+    mv_bytes = algorithmic_bytes_matvec(sc, kb_full, kc_full)
+    achieved = mv_bytes / (mv_ms * 1e-3) / 1e9
+
+    cpu = None
+    if args.cpu_steps > 0:
+        from oracle import pyoracle
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from conftest import oracle_from_scene, oracle_assemble
+        print(f"bench: cpu baseline, {args.cpu_steps} oracle steps", file=sys.stderr, flush=True)
+        orc = oracle_from_scene(pyoracle, sc)
+        t0 = time.perf_counter()
+        its = []
+        for s in range(args.cpu_steps):
+            rep = orc.solve(args.tol, sc.gauss(s), tol_mode=1)
+            its.append(rep["iterations"])
+            orc.prepare(orc.fiber_points())
+            oracle_assemble(orc, sc)
+        cdt = time.perf_counter() - t0
+        cpu = {"value": args.cpu_steps / cdt, "unit": "implicit steps/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same 1000-fiber workload, FP64 block-LU BCGSP, "
+                         f"mean {np.mean(its):.1f} iterations, single thread"}
+
+    best = results["fp64"]
+    line = {
+        "metric": "implicit steps/s @ rel.res 1e-6: plain FP64 GPU vs learned+mixed; BiCGStab iters",
+        "value": round(best["steps_per_s"], 3),
+        "unit": "implicit steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(best["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"fibers-3d-{args.nfib}-crosslinks-{args.nlinks}-sphere10",
+                   "fibers": args.nfib, "points": sc.npts, "unknowns": sc.size, "links": args.nlinks,
+                   "tolerance": f"|r|inf/|b|inf<{args.tol:g}"},
+        "modes": {k: {"steps_per_s": round(v["steps_per_s"], 3), "ms_per_step": round(v["ms_per_step"], 3),
+                      "mean_bicgstab_iters": round(v["mean_iters"], 2)} for k, v in results.items()},
+        "roofline": {"bound": "hbm", "kernel": "k_matvec (fused Meca::multiply)", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_launch": int(mv_bytes), "avg_launch_ms": mv_ms},
+        "kernels_ms": {"matvec": mv_ms, "precond_apply": pc_ms, "precond_build_fp64": build_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,413 @@
+"""MI355X implicit mechanics step of Cytosim (a drop-in for `Meca`).
+
+Python binding of the C ABI in include/cymeca.h (libcymeca.so, HIP for
+gfx950).  The product path is the HIP library: importing this module never
+falls back to any CPU implementation; a missing library raises.
+
+The directory name is not a Python identifier; load it with `load()` from
+`__graft_entry__` or with importlib.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "_build", "libcymeca.so")
+
+OK, ERR_INVALID, ERR_HIP, ERR_PROJECTION, ERR_CONVERGENCE, ERR_UNSUPPORTED = range(6)
+MAX_FIBER_POINTS = 128
+TOL_REFERENCE, TOL_RELATIVE = 0, 1
+PRECOND_NONE, PRECOND_FP64, PRECOND_FP32 = 0, 1, 2
+GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED = 0, 1, 2
+VEC_PTS, VEC_SOL, VEC_BAS, VEC_RHS, VEC_FOR = range(5)
+
+EXPORTED = [
+    "cymg_create", "cymg_destroy", "cymg_last_error", "cymg_set_params", "cymg_clear",
+    "cymg_add_fiber", "cymg_nb_points", "cymg_prepare", "cymg_add_pure_force",
+    "cymg_inter_link_ee", "cymg_inter_link_ie", "cymg_inter_link_ii", "cymg_inter_long_link_ee",
+    "cymg_inter_clamp_e", "cymg_inter_clamp_i", "cymg_inter_long_clamp", "cymg_inter_plane_e",
+    "cymg_confine_sphere", "cymg_set_couples", "cymg_couples_interactions", "cymg_solve",
+    "cymg_setup", "cymg_iterate", "cymg_apply", "cymg_multiply", "cymg_compute_preconditioner",
+    "cymg_precondition", "cymg_compute_forces", "cymg_multiply_device", "cymg_precondition_device",
+    "cymg_get_vector", "cymg_get_points", "cymg_get_tensions", "cymg_get_block", "cymg_get_matrix",
+    "cymg_get_timings", "cymg_enable_timing", "cymg_time_kernel",
+]
+
+
+class SimParams(C.Structure):
+    _fields_ = [("time_step", C.c_double), ("kT", C.c_double), ("viscosity", C.c_double),
+                ("hydro_radius0", C.c_double), ("hydro_radius1", C.c_double)]
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("tol_mode", C.c_int), ("precondition", C.c_int),
+                ("max_iter", C.c_int), ("guess", C.c_int), ("fallback", C.c_int),
+                ("gauss_on_device", C.c_int)]
+
+
+class SolveReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("flag", C.c_int), ("converged", C.c_int),
+                ("fallback", C.c_int), ("precond_used", C.c_int), ("total_iterations", C.c_int),
+                ("residual", C.c_double), ("threshold", C.c_double), ("noise_level", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Timings(C.Structure):
+    _fields_ = [("assemble_ms", C.c_double), ("setup_ms", C.c_double), ("precond_ms", C.c_double),
+                ("iterate_ms", C.c_double), ("apply_ms", C.c_double), ("matvec_avg_ms", C.c_double),
+                ("pcapply_avg_ms", C.c_double), ("matvec_count", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CymgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class ConvergenceFailure(CymgError):
+    """Exception("convergence failure in solver") of the reference (meca.cc:795)."""
+
+
+class ProjectionFailure(CymgError):
+    """Exception("could not build Fiber projection") (rigid_fiber_project.cc:84-85)."""
+
+
+def build(verbose=False):
+    """Compile the HIP library for gfx950 in-tree (hipcc via the package Makefile)."""
+    jobs = os.environ.get("MAX_JOBS", "8")
+    out = subprocess.run(["make", "-j", jobs, "-C", HERE], capture_output=not verbose, text=True)
+    if out.returncode:
+        raise RuntimeError("HIP build failed:\n" + (out.stdout or "") + (out.stderr or ""))
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libcymeca.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    u32 = C.c_uint32
+    sig = {
+        "cymg_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(vp)]),
+        "cymg_destroy": (C.c_int, [vp]),
+        "cymg_last_error": (C.c_char_p, [vp]),
+        "cymg_set_params": (C.c_int, [vp, C.POINTER(SimParams)]),
+        "cymg_clear": (C.c_int, [vp]),
+        "cymg_add_fiber": (C.c_int, [vp, u32, C.c_double, C.c_double, C.POINTER(u32)]),
+        "cymg_nb_points": (C.c_int, [vp, C.POINTER(u32)]),
+        "cymg_prepare": (C.c_int, [vp, vp]),
+        "cymg_add_pure_force": (C.c_int, [vp, u32, dp]),
+        "cymg_inter_link_ee": (C.c_int, [vp, u32, u32, C.c_double]),
+        "cymg_inter_link_ie": (C.c_int, [vp, u32, u32, C.c_double, u32, C.c_double]),
+        "cymg_inter_link_ii": (C.c_int, [vp, u32, u32, C.c_double, u32, u32, C.c_double, C.c_double]),
+        "cymg_inter_long_link_ee": (C.c_int, [vp, u32, u32, C.c_double, C.c_double]),
+        "cymg_inter_clamp_e": (C.c_int, [vp, u32, dp, C.c_double]),
+        "cymg_inter_clamp_i": (C.c_int, [vp, u32, u32, C.c_double, dp, C.c_double]),
+        "cymg_inter_long_clamp": (C.c_int, [vp, u32, dp, C.c_double, C.c_double]),
+        "cymg_inter_plane_e": (C.c_int, [vp, u32, dp, dp, C.c_double]),
+        "cymg_confine_sphere": (C.c_int, [vp, C.c_double, C.c_double]),
+        "cymg_set_couples": (C.c_int, [vp, u32, vp, vp, vp, vp, vp]),
+        "cymg_couples_interactions": (C.c_int, [vp]),
+        "cymg_solve": (C.c_int, [vp, C.POINTER(SolveParams), vp, C.POINTER(SolveReport)]),
+        "cymg_setup": (C.c_int, [vp, vp, C.c_int]),
+        "cymg_iterate": (C.c_int, [vp, C.POINTER(SolveParams), C.POINTER(SolveReport)]),
+        "cymg_apply": (C.c_int, [vp]),
+        "cymg_multiply": (C.c_int, [vp, vp, vp]),
+        "cymg_compute_preconditioner": (C.c_int, [vp, C.c_int]),
+        "cymg_precondition": (C.c_int, [vp, vp, vp]),
+        "cymg_compute_forces": (C.c_int, [vp]),
+        "cymg_multiply_device": (C.c_int, [vp, vp, vp]),
+        "cymg_precondition_device": (C.c_int, [vp, vp, vp]),
+        "cymg_get_vector": (C.c_int, [vp, C.c_int, vp]),
+        "cymg_get_points": (C.c_int, [vp, vp]),
+        "cymg_get_tensions": (C.c_int, [vp, vp]),
+        "cymg_get_block": (C.c_int, [vp, u32, C.c_int, vp]),
+        "cymg_get_matrix": (C.c_int, [vp, C.c_int, C.POINTER(u32), vp, vp, vp]),
+        "cymg_get_timings": (C.c_int, [vp, C.POINTER(Timings)]),
+        "cymg_enable_timing": (C.c_int, [vp, C.c_int]),
+        "cymg_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _vec3(v, dim):
+    a = np.zeros(3)
+    a[:dim] = np.asarray(v, dtype=np.float64)[:dim]
+    return a.ctypes.data_as(C.POINTER(C.c_double)), a
+
+
+class Meca:
+    """One GPU Meca instance: the reference's `Meca` interface (meca.h:160-327)
+    over the C ABI.  Errors raise CymgError; the solver's failure raises
+    ConvergenceFailure as the reference throws."""
+
+    def __init__(self, dim, device=0, params=None):
+        self.L = lib()
+        self.dim = dim
+        h = C.c_void_p()
+        rc = self.L.cymg_create(dim, device, C.byref(h))
+        if rc:
+            raise CymgError(rc, "cymg_create failed (dim must be 2 or 3, and a GPU must be visible)")
+        self.h = h
+        self.np = []
+        if params is not None:
+            self.set_params(**params)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cymg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc:
+            msg = self.L.cymg_last_error(self.h).decode()
+            if rc == ERR_CONVERGENCE:
+                raise ConvergenceFailure(rc, msg)
+            if rc == ERR_PROJECTION:
+                raise ProjectionFailure(rc, msg)
+            raise CymgError(rc, msg)
+
+    # registration ------------------------------------------------------------
+    def set_params(self, time_step, kT, viscosity, hydro_radius0=0.0125, hydro_radius1=10.0):
+        p = SimParams(time_step, kT, viscosity, hydro_radius0, hydro_radius1)
+        self._chk(self.L.cymg_set_params(self.h, C.byref(p)))
+
+    def clear(self):
+        self._chk(self.L.cymg_clear(self.h))
+        self.np = []
+
+    def add_fiber(self, nb_points, segmentation, rigidity):
+        mi = C.c_uint32()
+        self._chk(self.L.cymg_add_fiber(self.h, int(nb_points), float(segmentation), float(rigidity),
+                                        C.byref(mi)))
+        self.np.append(int(nb_points))
+        return mi.value
+
+    @property
+    def nb_points(self):
+        return int(sum(self.np))
+
+    @property
+    def size(self):
+        return self.dim * self.nb_points
+
+    def prepare(self, points=None):
+        if points is None:
+            self._chk(self.L.cymg_prepare(self.h, None))
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).ravel()
+            assert pts.size == self.size
+            self._chk(self.L.cymg_prepare(self.h, _p(pts)))
+
+    # assembly ------------------------------------------------------------------
+    def inter_link_ii(self, a1, a2, ca, b1, b2, cb, w):
+        self._chk(self.L.cymg_inter_link_ii(self.h, a1, a2, ca, b1, b2, cb, w))
+
+    def inter_link_ie(self, a1, a2, ca, b, w):
+        self._chk(self.L.cymg_inter_link_ie(self.h, a1, a2, ca, b, w))
+
+    def inter_link_ee(self, a, b, w):
+        self._chk(self.L.cymg_inter_link_ee(self.h, a, b, w))
+
+    def inter_long_link_ee(self, a, b, length, w):
+        self._chk(self.L.cymg_inter_long_link_ee(self.h, a, b, length, w))
+
+    def inter_clamp_e(self, a, g, w):
+        gp, keep = _vec3(g, self.dim)
+        self._chk(self.L.cymg_inter_clamp_e(self.h, a, gp, w))
+
+    def inter_clamp_i(self, a1, a2, ca, g, w):
+        gp, keep = _vec3(g, self.dim)
+        self._chk(self.L.cymg_inter_clamp_i(self.h, a1, a2, ca, gp, w))
+
+    def inter_long_clamp(self, a, center, length, w):
+        cp, keep = _vec3(center, self.dim)
+        self._chk(self.L.cymg_inter_long_clamp(self.h, a, cp, length, w))
+
+    def inter_plane_e(self, a, direction, g, w):
+        dp, k1 = _vec3(direction, self.dim)
+        gp, k2 = _vec3(g, self.dim)
+        self._chk(self.L.cymg_inter_plane_e(self.h, a, dp, gp, w))
+
+    def add_pure_force(self, a, force):
+        fp, keep = _vec3(force, self.dim)
+        self._chk(self.L.cymg_add_pure_force(self.h, a, fp))
+
+    def confine_sphere(self, radius, stiffness):
+        self._chk(self.L.cymg_confine_sphere(self.h, radius, stiffness))
+
+    def set_couples(self, fiber_a, abs_a, fiber_b, abs_b, stiffness):
+        fa = np.ascontiguousarray(fiber_a, dtype=np.uint32)
+        fb = np.ascontiguousarray(fiber_b, dtype=np.uint32)
+        aa = np.ascontiguousarray(abs_a, dtype=np.float64)
+        ab = np.ascontiguousarray(abs_b, dtype=np.float64)
+        w = np.ascontiguousarray(stiffness, dtype=np.float64)
+        self._chk(self.L.cymg_set_couples(self.h, fa.size, _p(fa), _p(aa), _p(fb), _p(ab), _p(w)))
+
+    def couples_interactions(self):
+        self._chk(self.L.cymg_couples_interactions(self.h))
+
+    # solve ------------------------------------------------------------------------
+    @staticmethod
+    def _params(tolerance, tol_mode, precondition, max_iter, guess, fallback, gauss_on_device=0):
+        return SolveParams(tolerance, tol_mode, precondition, max_iter, guess, fallback, gauss_on_device)
+
+    def setup(self, gauss=None, gauss_device_ptr=None):
+        if gauss_device_ptr is not None:
+            self._chk(self.L.cymg_setup(self.h, C.c_void_p(gauss_device_ptr), 1))
+        else:
+            g = None if gauss is None else np.ascontiguousarray(gauss, dtype=np.float64)
+            self._chk(self.L.cymg_setup(self.h, _p(g), 0))
+
+    def iterate(self, tolerance, tol_mode=TOL_REFERENCE, precondition=PRECOND_FP64, max_iter=0,
+                guess=GUESS_ZERO, fallback=1):
+        sp = self._params(tolerance, tol_mode, precondition, max_iter, guess, fallback)
+        rep = SolveReport()
+        self._chk(self.L.cymg_iterate(self.h, C.byref(sp), C.byref(rep)))
+        return rep.as_dict()
+
+    def apply(self):
+        self._chk(self.L.cymg_apply(self.h))
+
+    def solve(self, tolerance, gauss=None, tol_mode=TOL_REFERENCE, precondition=PRECOND_FP64,
+              max_iter=0, guess=GUESS_ZERO, fallback=1, gauss_device_ptr=None):
+        on_dev = 1 if gauss_device_ptr is not None else 0
+        sp = self._params(tolerance, tol_mode, precondition, max_iter, guess, fallback, on_dev)
+        rep = SolveReport()
+        if on_dev:
+            g = C.c_void_p(gauss_device_ptr)
+        else:
+            garr = None if gauss is None else np.ascontiguousarray(gauss, dtype=np.float64)
+            g = _p(garr)
+        self._chk(self.L.cymg_solve(self.h, C.byref(sp), g, C.byref(rep)))
+        return rep.as_dict()
+
+    # operators ----------------------------------------------------------------------
+    def multiply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self._chk(self.L.cymg_multiply(self.h, _p(x), _p(y)))
+        return y
+
+    def multiply_device(self, x_ptr, y_ptr):
+        self._chk(self.L.cymg_multiply_device(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr)))
+
+    def precondition_device(self, x_ptr, y_ptr):
+        self._chk(self.L.cymg_precondition_device(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr)))
+
+    def compute_preconditioner(self, precision=PRECOND_FP64):
+        self._chk(self.L.cymg_compute_preconditioner(self.h, precision))
+
+    def precondition(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self._chk(self.L.cymg_precondition(self.h, _p(x), _p(y)))
+        return y
+
+    def compute_forces(self):
+        self._chk(self.L.cymg_compute_forces(self.h))
+
+    # export --------------------------------------------------------------------------
+    def vector(self, which):
+        out = np.empty(self.size)
+        self._chk(self.L.cymg_get_vector(self.h, which, _p(out)))
+        return out
+
+    def fiber_points(self):
+        out = np.empty(self.size)
+        self._chk(self.L.cymg_get_points(self.h, _p(out)))
+        return out
+
+    def tensions(self):
+        out = np.empty(self.nb_points - len(self.np))
+        self._chk(self.L.cymg_get_tensions(self.h, _p(out)))
+        return out
+
+    def get_block(self, f, inverse=False):
+        bs = self.dim * self.np[f]
+        out = np.empty(bs * bs)
+        self._chk(self.L.cymg_get_block(self.h, f, int(bool(inverse)), _p(out)))
+        return out.reshape(bs, bs).T.copy()
+
+    def matrix_entries(self, which):
+        n = C.c_uint32()
+        self._chk(self.L.cymg_get_matrix(self.h, which, C.byref(n), None, None, None))
+        rows = np.empty(n.value, np.uint32)
+        cols = np.empty(n.value, np.uint32)
+        vals = np.empty(n.value)
+        self._chk(self.L.cymg_get_matrix(self.h, which, C.byref(n), _p(rows), _p(cols), _p(vals)))
+        return rows, cols, vals
+
+    def enable_timing(self, on=True):
+        self._chk(self.L.cymg_enable_timing(self.h, int(on)))
+
+    def time_kernel(self, which, reps=50):
+        ms = C.c_double()
+        self._chk(self.L.cymg_time_kernel(self.h, which, reps, C.byref(ms)))
+        return ms.value
+
+    def timings(self):
+        t = Timings()
+        self._chk(self.L.cymg_get_timings(self.h, C.byref(t)))
+        return t.as_dict()
+
+
+def load_scene(meca, scene, couples_on_device=True, upload_couples=True):
+    """Register the scene's fibers and run one step's assembly in the order of
+    Simul::setInteractions (simul_solve.cc:199-245): Meca::clear/add/prepare,
+    then Fiber::setInteractions (sphere confinement), then Couple::setInteractions.
+    Couples are emitted on the device from the uploaded couple list, or, with
+    couples_on_device=False, registered one by one through the host interLink."""
+    meca.clear()
+    for f in range(scene.nfib):
+        meca.add_fiber(scene.nb_points[f], scene.segmentation[f], scene.rigidity[f])
+    meca.set_params(**scene.params())
+    meca.prepare(scene.points)
+    assemble(meca, scene, couples_on_device, upload_couples)
+
+
+def assemble(meca, scene, couples_on_device=True, upload_couples=True):
+    """One step's interactions, after prepare()."""
+    if scene.confine_radius > 0:
+        meca.confine_sphere(scene.confine_radius, scene.confine_stiffness)
+    if scene.nlinks:
+        if couples_on_device:
+            if upload_couples:
+                meca.set_couples(scene.link_fiber_a, scene.link_abs_a, scene.link_fiber_b,
+                                 scene.link_abs_b, scene.link_stiffness)
+            meca.couples_interactions()
+        else:
+            a1, a2, ca = scene.interpolate(scene.link_fiber_a, scene.link_abs_a)
+            b1, b2, cb = scene.interpolate(scene.link_fiber_b, scene.link_abs_b)
+            for k in range(scene.nlinks):
+                meca.inter_link_ii(int(a1[k]), int(a2[k]), float(ca[k]), int(b1[k]), int(b2[k]),
+                                   float(cb[k]), float(scene.link_stiffness[k]))

@@ -1,0 +1,797 @@
+// C ABI of the MI355X Meca library (include/cymeca.h): context, registration,
+// host-side assembly recording in call order, and the solve orchestration with
+// the reference's rescue chain.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "context.h"
+
+using namespace cymg;
+
+struct cymg_context {
+    Context c;
+};
+
+namespace {
+
+template <typename F>
+int guarded(cymg_context* ctx, F&& fn)
+{
+    if (!ctx) return CYMG_ERR_INVALID;
+    try {
+        CYMG_HIP(hipSetDevice(ctx->c.device));
+        fn(ctx->c);
+        ctx->c.err.clear();
+        return CYMG_OK;
+    } catch (const Error& e) {
+        ctx->c.err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->c.err = e.what();
+        return CYMG_ERR_INVALID;
+    }
+}
+
+inline uint64_t lower_key_h(uint32_t i, uint32_t j)
+{
+    const uint32_t r = std::max(i, j), c = std::min(i, j);
+    return ((uint64_t)c << 32) | r;
+}
+
+void addB(Context& c, uint32_t i, uint32_t j, double delta)
+{
+    c.hB.key.push_back(lower_key_h(i, j));
+    c.hB.val.push_back(delta);
+}
+
+void addC(Context& c, uint32_t i, uint32_t j, double delta)
+{
+    c.hC.key.push_back(lower_key_h(i, j));
+    c.hC.val.push_back(delta);
+}
+
+void addBas(Context& c, uint32_t i, double delta)
+{
+    c.hBas.key.push_back(i);
+    c.hBas.val.push_back(delta);
+}
+
+void need_prepared(const Context& c)
+{
+    if (!c.prepared) throw Error(CYMG_ERR_INVALID, "call cymg_prepare before assembling interactions");
+}
+
+void check_point(const Context& c, uint32_t a)
+{
+    if (a >= c.npts) throw Error(CYMG_ERR_INVALID, "point index out of range");
+}
+
+int mecable_of(const Context& c, uint32_t a)
+{
+    auto it = std::upper_bound(c.h_idx.begin(), c.h_idx.end(), a);
+    return (int)(it - c.h_idx.begin()) - 1;
+}
+
+void upload_layout(Context& c)
+{
+    const uint32_t F = c.nfib;
+    c.d_np.resize(F); c.d_idx.resize(F); c.d_seg.resize(F); c.d_cut.resize(F); c.d_rig.resize(F);
+    c.d_mob.resize(F); c.d_rfrig.resize(F); c.d_blkoff.resize(F + 1); c.d_fib_sys.resize(F);
+    c.d_sys_begin.resize(2);
+    std::vector<uint32_t> sysv(F, 0);
+    uint32_t sb[2] = { 0, F };
+    hipStream_t st = c.stream;
+    CYMG_HIP(hipMemcpyAsync(c.d_np.get(), c.h_np.data(), F * 4, hipMemcpyHostToDevice, st));
+    CYMG_HIP(hipMemcpyAsync(c.d_idx.get(), c.h_idx.data(), F * 4, hipMemcpyHostToDevice, st));
+    CYMG_HIP(hipMemcpyAsync(c.d_seg.get(), c.h_seg.data(), F * 4, hipMemcpyHostToDevice, st));
+    CYMG_HIP(hipMemcpyAsync(c.d_cut.get(), c.h_cut.data(), F * 8, hipMemcpyHostToDevice, st));
+    CYMG_HIP(hipMemcpyAsync(c.d_rig.get(), c.h_rig.data(), F * 8, hipMemcpyHostToDevice, st));
+    CYMG_HIP(hipMemcpyAsync(c.d_blkoff.get(), c.h_blkoff.data(), (F + 1) * 8, hipMemcpyHostToDevice, st));
+    CYMG_HIP(hipMemcpyAsync(c.d_fib_sys.get(), sysv.data(), F * 4, hipMemcpyHostToDevice, st));
+    CYMG_HIP(hipMemcpyAsync(c.d_sys_begin.get(), sb, 8, hipMemcpyHostToDevice, st));
+    const size_t n = c.n();
+    for (DevBuf<double>* v : { &c.d_pts, &c.d_pos, &c.d_sol, &c.d_bas, &c.d_rhs, &c.d_for, &c.d_tmp, &c.d_prev_sol })
+        v->resize(n ? n : 1);
+    const size_t S = c.nseg ? c.nseg : 1;
+    for (DevBuf<double>* v : { &c.d_diff }) v->resize(S * c.dim);
+    for (DevBuf<double>* v : { &c.d_jd, &c.d_je, &c.d_lag, &c.d_pdiff }) v->resize(S);
+    CYMG_HIP(hipMemsetAsync(c.d_lag.get(), 0, S * 8, st));
+    CYMG_HIP(hipMemsetAsync(c.d_prev_sol.get(), 0, (n ? n : 1) * 8, st));
+    c.d_part.resize(8 * (size_t)(F ? F : 1));
+    c.d_scal.resize(c.nsys);
+    c.d_counter.resize(c.nsys);
+    CYMG_HIP(hipMemsetAsync(c.d_counter.get(), 0, c.nsys * sizeof(unsigned), st));
+    CYMG_HIP(hipMemsetAsync(c.d_scal.get(), 0, c.nsys * sizeof(SysScalars), st));
+    CYMG_HIP(hipStreamSynchronize(st));   // the sources above are pageable host arrays
+    c.layout_dirty = false;
+}
+
+struct Timer {
+    Context& c;
+    int slot;
+    Timer(Context& cc, int s) : c(cc), slot(s) {
+        if (c.timing) CYMG_HIP(hipEventRecord(c.ev[2 * slot], c.stream));
+    }
+    ~Timer() {
+        if (c.timing) (void)hipEventRecord(c.ev[2 * slot + 1], c.stream);
+    }
+};
+
+float elapsed(Context& c, int slot)
+{
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c.ev[2 * slot], c.ev[2 * slot + 1]) != hipSuccess) return 0;
+    return ms;
+}
+
+void do_setup(Context& c, const double* gauss, int on_device)
+{
+    if (!c.prepared) throw Error(CYMG_ERR_INVALID, "cymg_prepare must precede the solve");
+    const size_t n = c.n();
+    {
+        Timer t(c, 0);
+        build_streams(c);
+    }
+    const double* g = nullptr;
+    if (gauss) {
+        if (on_device) g = gauss;
+        else {
+            c.d_gauss.resize(n);
+            CYMG_HIP(hipMemcpyAsync(c.d_gauss.get(), gauss, n * 8, hipMemcpyHostToDevice, c.stream));
+            g = c.d_gauss.get();
+        }
+    }
+    {
+        Timer t(c, 1);
+        launch_setup(c, g);
+    }
+    c.setup_done = true;
+    c.pc_ready = false;
+}
+
+void check_status(Context& c)
+{
+    int st = 0;
+    CYMG_HIP(hipMemcpyAsync(&st, c.d_status.get(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CYMG_HIP(hipStreamSynchronize(c.stream));
+    if (st) throw Error(CYMG_ERR_PROJECTION, "could not build Fiber projection");
+}
+
+void do_iterate(Context& c, const cymg_solve_params* p, cymg_solve_report* rep)
+{
+    if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_iterate");
+    check_status(c);
+    cymg_solve_report r{};
+    const int maxit = p->max_iter > 0 ? p->max_iter : (int)c.n();
+    const int pc = p->precondition;
+    if (pc < 0 || pc > 2) throw Error(CYMG_ERR_INVALID, "precondition must be 0, 1 or 2");
+    const double* guess = nullptr;
+    if (p->guess == CYMG_GUESS_PREVIOUS) guess = c.d_prev_sol.get();
+    else if (p->guess == CYMG_GUESS_LEARNED) throw Error(CYMG_ERR_UNSUPPORTED, "learned guess not loaded");
+    if (pc) {
+        Timer t(c, 2);
+        launch_precond_build(c, pc);
+    }
+    {
+        Timer t(c, 3);
+        krylov_solve(c, pc, guess, maxit, p->tolerance, p->tol_mode, &r);
+        int total = r.iterations;
+        if (!r.converged && p->fallback) {
+            // meca.cc:749-799: retry with vRHS as the seed, then switch method
+            cymg_solve_report r2{};
+            krylov_solve(c, pc, c.d_rhs.get(), maxit, p->tolerance, p->tol_mode, &r2);
+            total += r2.iterations;
+            if (r2.converged) {
+                r = r2;
+                r.fallback = 1;
+            } else {
+                cymg_solve_report r3{};
+                int other = CYMG_PRECOND_NONE;
+                if (!pc) {
+                    launch_precond_build(c, CYMG_PRECOND_FP64);
+                    other = CYMG_PRECOND_FP64;
+                }
+                krylov_solve(c, other, nullptr, maxit, p->tolerance, p->tol_mode, &r3);
+                total += r3.iterations;
+                r = r3;
+                r.fallback = r3.converged ? 2 : 3;
+            }
+        }
+        r.total_iterations = total;
+    }
+    if (rep) *rep = r;
+    if (r.fallback == 3) throw Error(CYMG_ERR_CONVERGENCE, "convergence failure in solver");
+}
+
+void do_apply(Context& c)
+{
+    Timer t(c, 4);
+    launch_apply(c);
+    CYMG_HIP(hipMemcpyAsync(c.d_prev_sol.get(), c.d_sol.get(), (size_t)c.n() * 8, hipMemcpyDeviceToDevice,
+                            c.stream));
+    c.have_solution = true;
+}
+
+void collect_timings(Context& c)
+{
+    if (!c.timing) return;
+    CYMG_HIP(hipStreamSynchronize(c.stream));
+    c.tm.assemble_ms = elapsed(c, 0);
+    c.tm.setup_ms = elapsed(c, 1);
+    c.tm.precond_ms = elapsed(c, 2);
+    c.tm.iterate_ms = elapsed(c, 3);
+    c.tm.apply_ms = elapsed(c, 4);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cymg_create(int dim, int device, cymg_context** out)
+{
+    if (!out) return CYMG_ERR_INVALID;
+    *out = nullptr;
+    if (dim != 2 && dim != 3) return CYMG_ERR_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return CYMG_ERR_HIP;
+    cymg_context* ctx = new cymg_context();
+    ctx->c.dim = dim;
+    ctx->c.device = device;
+    int rc = guarded(ctx, [&](Context& c) {
+        CYMG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        CYMG_HIP(hipHostMalloc((void**)&c.h_scal, sizeof(SysScalars) * 64, hipHostMallocDefault));
+        for (int i = 0; i < 10; ++i) CYMG_HIP(hipEventCreate(&c.ev[i]));
+    });
+    if (rc) {
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return CYMG_OK;
+}
+
+int cymg_destroy(cymg_context* ctx)
+{
+    if (!ctx) return CYMG_ERR_INVALID;
+    Context& c = ctx->c;
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    for (int i = 0; i < 10; ++i)
+        if (c.ev[i]) (void)hipEventDestroy(c.ev[i]);
+    if (c.h_scal) (void)hipHostFree(c.h_scal);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    delete ctx;
+    return CYMG_OK;
+}
+
+const char* cymg_last_error(const cymg_context* ctx)
+{
+    return ctx ? ctx->c.err.c_str() : "null context";
+}
+
+int cymg_set_params(cymg_context* ctx, const cymg_sim_params* p)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!p || !(p->time_step > 0) || !(p->viscosity > 0) || !(p->hydro_radius0 > 0) || !(p->hydro_radius1 > 0) ||
+            p->kT < 0)
+            throw Error(CYMG_ERR_INVALID, "invalid simulation parameters");
+        c.prm = *p;
+    });
+}
+
+int cymg_clear(cymg_context* ctx)
+{
+    return guarded(ctx, [&](Context& c) {
+        c.h_np.clear(); c.h_idx.clear(); c.h_seg.clear(); c.h_cut.clear(); c.h_rig.clear();
+        c.h_blkoff.assign(1, 0);
+        c.nfib = c.npts = c.nseg = c.largest = 0;
+        c.blk_total = 0;
+        c.layout_dirty = true;
+        c.prepared = c.setup_done = c.pc_ready = false;
+    });
+}
+
+int cymg_add_fiber(cymg_context* ctx, uint32_t np, double segmentation, double rigidity, uint32_t* mat_index)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (np < 2) throw Error(CYMG_ERR_INVALID, "a fiber needs at least 2 points");
+        if (np > (uint32_t)kMaxFiberPoints)
+            throw Error(CYMG_ERR_UNSUPPORTED, "fiber has more than CYMG_MAX_FIBER_POINTS points");
+        if (!(segmentation > 0)) throw Error(CYMG_ERR_INVALID, "segmentation must be > 0");
+        if (!(rigidity >= 0)) throw Error(CYMG_ERR_INVALID, "fiber:rigidity must be specified and >= 0");
+        if (c.h_blkoff.empty()) c.h_blkoff.assign(1, 0);
+        if (mat_index) *mat_index = c.npts;
+        c.h_np.push_back(np);
+        c.h_idx.push_back(c.npts);
+        c.h_seg.push_back(c.nseg);
+        c.h_cut.push_back(segmentation);
+        c.h_rig.push_back(rigidity);
+        const uint64_t bs = (uint64_t)c.dim * np;
+        c.blk_total += bs * bs;
+        c.h_blkoff.push_back(c.blk_total);
+        c.npts += np;
+        c.nseg += np - 1;
+        c.largest = std::max(c.largest, np);
+        c.nfib += 1;
+        c.layout_dirty = true;
+        c.prepared = c.setup_done = c.pc_ready = false;
+    });
+}
+
+int cymg_nb_points(const cymg_context* ctx, uint32_t* nb)
+{
+    if (!ctx || !nb) return CYMG_ERR_INVALID;
+    *nb = ctx->c.npts;
+    return CYMG_OK;
+}
+
+int cymg_prepare(cymg_context* ctx, const double* points)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.nfib) throw Error(CYMG_ERR_INVALID, "no Mecable registered");
+        const bool fresh = c.layout_dirty;
+        if (c.layout_dirty) upload_layout(c);
+        if (points) {
+            CYMG_HIP(hipMemcpyAsync(c.d_pos.get(), points, (size_t)c.n() * 8, hipMemcpyHostToDevice, c.stream));
+            c.have_solution = true;
+        } else if (fresh && !c.have_solution) {
+            throw Error(CYMG_ERR_INVALID, "no points on the device: pass the Mecable points");
+        }
+        c.hB.clear(); c.hC.clear(); c.hBas.clear();
+        c.marks.clear();
+        launch_prepare(c);
+        c.prepared = true;
+        c.setup_done = false;
+        c.pc_ready = false;
+    });
+}
+
+// ---- assembly, in call order -------------------------------------------------
+// Products and sums are rounded as written (no FMA contraction), the
+// arithmetic of the reference source.
+#pragma clang fp contract(off)
+
+int cymg_add_pure_force(cymg_context* ctx, uint32_t a, const double* force)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_point(c, a);
+        for (int d = 0; d < c.dim; ++d) addBas(c, c.dim * a + d, force[d]);
+    });
+}
+
+int cymg_inter_link_ee(cymg_context* ctx, uint32_t a, uint32_t b, double w)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_point(c, a); check_point(c, b);
+        if (!(w >= 0)) throw Error(CYMG_ERR_INVALID, "weight must be >= 0");
+        if (mecable_of(c, a) == mecable_of(c, b) && (a == b || a == b + 1 || a + 1 == b)) return;
+        addB(c, a, a, -w);
+        addB(c, a, b, +w);
+        addB(c, b, b, -w);
+    });
+}
+
+int cymg_inter_link_ie(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b, double w)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_point(c, a1); check_point(c, a2); check_point(c, b);
+        if (!(w >= 0)) throw Error(CYMG_ERR_INVALID, "weight must be >= 0");
+        if (mecable_of(c, a1) == mecable_of(c, b) && (a1 == b || a2 == b)) return;
+        const double cc[3] = { 1.0 - ca, ca, -1.0 };
+        const double cw[3] = { w * cc[0], w * cc[1], -w };
+        const uint32_t inx[3] = { a1, a2, b };
+        for (int k = 0; k < 3; ++k)
+            for (int l = k; l < 3; ++l) addB(c, inx[k], inx[l], -(cc[k] * cw[l]));
+    });
+}
+
+int cymg_inter_link_ii(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2, double cb,
+                       double w)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_point(c, a1); check_point(c, a2); check_point(c, b1); check_point(c, b2);
+        if (!(w >= 0)) throw Error(CYMG_ERR_INVALID, "weight must be >= 0");
+        if (mecable_of(c, a1) == mecable_of(c, b1) && (a1 == b1 || a1 == b2 || a2 == b1 || a2 == b2)) return;
+        const double cc[4] = { 1.0 - ca, ca, -(1.0 - cb), -cb };
+        const double cw[4] = { w * cc[0], w * cc[1], w * cc[2], w * cc[3] };
+        const uint32_t inx[4] = { a1, a2, b1, b2 };
+        for (int j = 0; j < 4; ++j)
+            for (int i = j; i < 4; ++i) addB(c, inx[i], inx[j], -(cc[j] * cw[i]));
+    });
+}
+
+static void fetch_points(Context& c, uint32_t a, double* x)
+{
+    CYMG_HIP(hipMemcpyAsync(x, c.d_pos.get() + (size_t)c.dim * a, c.dim * 8, hipMemcpyDeviceToHost, c.stream));
+    CYMG_HIP(hipStreamSynchronize(c.stream));
+}
+
+int cymg_inter_long_link_ee(cymg_context* ctx, uint32_t a, uint32_t b, double len, double w)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_point(c, a); check_point(c, b);
+        if (!(w >= 0)) throw Error(CYMG_ERR_INVALID, "weight must be >= 0");
+        if (mecable_of(c, a) == mecable_of(c, b) && (a == b || a == b + 1 || a + 1 == b)) return;
+        const int D = c.dim;
+        double pa[3], pb[3], ab[3] = { 0, 0, 0 };
+        fetch_points(c, a, pa);
+        fetch_points(c, b, pb);
+        double abn = 0;
+        for (int d = 0; d < D; ++d) { ab[d] = pb[d] - pa[d]; abn += ab[d] * ab[d]; }
+        abn = std::sqrt(abn);
+        if (abn < 128.0 * 2.220446049250313e-16) return;
+        for (int d = 0; d < D; ++d) ab[d] /= abn;
+        const uint32_t iA = D * a, iB = D * b;
+        for (int d = 0; d < D; ++d) {
+            addBas(c, iA + d, -(w * ab[d] * len));
+            addBas(c, iB + d, w * ab[d] * len);
+        }
+        len /= abn;
+        const bool cooked = len > 1.0;
+        for (int i = 0; i < D; ++i) {
+            double v = cooked ? w * ab[i] * ab[i] : w * (1.0 + len * (ab[i] * ab[i] - 1.0));
+            addC(c, iA + i, iA + i, -v);
+            addC(c, iA + i, iB + i, v);
+            addC(c, iB + i, iB + i, -v);
+            for (int j = i + 1; j < D; ++j) {
+                v = cooked ? w * ab[i] * ab[j] : w * len * ab[i] * ab[j];
+                addC(c, iA + i, iA + j, -v);
+                addC(c, iA + i, iB + j, v);
+                addC(c, iA + j, iB + i, v);
+                addC(c, iB + i, iB + j, -v);
+            }
+        }
+    });
+}
+
+int cymg_inter_clamp_e(cymg_context* ctx, uint32_t a, const double* g, double w)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_point(c, a);
+        addB(c, a, a, -w);
+        for (int d = 0; d < c.dim; ++d) addBas(c, c.dim * a + d, w * g[d]);
+    });
+}
+
+int cymg_inter_clamp_i(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, const double* g, double w)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_point(c, a1); check_point(c, a2);
+        const double c2 = ca, c2w = w * c2;
+        const double c1 = 1.0 - ca, c1w = w * c1;
+        addB(c, a1, a1, -(c1w * c1));
+        addB(c, a1, a2, -(c2w * c1));
+        addB(c, a2, a2, -(c2w * c2));
+        for (int d = 0; d < c.dim; ++d) {
+            addBas(c, c.dim * a1 + d, c1w * g[d]);
+            addBas(c, c.dim * a2 + d, c2w * g[d]);
+        }
+    });
+}
+
+int cymg_inter_long_clamp(cymg_context* ctx, uint32_t a, const double* center, double len, double w)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_point(c, a);
+        const int D = c.dim;
+        const uint32_t inx = D * a;
+        double x[3], axis[3] = { 0, 0, 0 };
+        fetch_points(c, a, x);
+        double an = 0;
+        for (int d = 0; d < D; ++d) { axis[d] = x[d] - center[d]; an += axis[d] * axis[d]; }
+        an = std::sqrt(an);
+        if (an < 128.0 * 2.220446049250313e-16) return;
+        for (int d = 0; d < D; ++d) axis[d] /= an;
+        double ac = 0;
+        for (int d = 0; d < D; ++d) ac += axis[d] * center[d];
+        if (len < an) {
+            len /= an;
+            const double facX = w * len * (an + ac);
+            const double facC = w * (1.0 - len);
+            for (int i = 0; i < D; ++i) {
+                addC(c, inx + i, inx + i, w * (len * (1.0 - axis[i] * axis[i]) - 1.0));
+                for (int j = i + 1; j < D; ++j) addC(c, inx + i, inx + j, -(w * len * axis[i] * axis[j]));
+                addBas(c, inx + i, facX * axis[i] + facC * center[i]);
+            }
+        } else {
+            const double facX = w * (len + ac);
+            for (int i = 0; i < D; ++i) {
+                for (int j = i; j < D; ++j) addC(c, inx + i, inx + j, -(w * axis[i] * axis[j]));
+                addBas(c, inx + i, facX * axis[i]);
+            }
+        }
+    });
+}
+
+int cymg_inter_plane_e(cymg_context* ctx, uint32_t a, const double* dir, const double* g, double w)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_point(c, a);
+        const int D = c.dim;
+        const uint32_t inx = D * a;
+        double gd = 0;
+        for (int d = 0; d < D; ++d) gd += g[d] * dir[d];
+        const double pr = gd * w;
+        for (int i = 0; i < D; ++i) {
+            for (int j = i; j < D; ++j) addC(c, inx + i, inx + j, -(dir[i] * dir[j] * w));
+            addBas(c, inx + i, pr * dir[i]);
+        }
+    });
+}
+
+int cymg_confine_sphere(cymg_context* ctx, double radius, double stiffness)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c);
+        if (!(radius > 0) || !(stiffness >= 0)) throw Error(CYMG_ERR_INVALID, "invalid confinement");
+        c.marks.push_back({ 1, radius, stiffness, c.hB.key.size(), c.hC.key.size(), c.hBas.key.size() });
+    });
+}
+
+int cymg_set_couples(cymg_context* ctx, uint32_t n, const uint32_t* fa, const double* aa, const uint32_t* fb,
+                     const double* ab, const double* w)
+{
+    return guarded(ctx, [&](Context& c) {
+        for (uint32_t k = 0; k < n; ++k) {
+            if (fa[k] >= c.nfib || fb[k] >= c.nfib) throw Error(CYMG_ERR_INVALID, "couple on an unknown fiber");
+            if (!(w[k] >= 0)) throw Error(CYMG_ERR_INVALID, "couple stiffness must be >= 0");
+        }
+        c.ncouples = n;
+        const size_t m = n ? n : 1;
+        c.cpl_fa.resize(m); c.cpl_fb.resize(m); c.cpl_aa.resize(m); c.cpl_ab.resize(m); c.cpl_w.resize(m);
+        if (n) {
+            CYMG_HIP(hipMemcpyAsync(c.cpl_fa.get(), fa, n * 4, hipMemcpyHostToDevice, c.stream));
+            CYMG_HIP(hipMemcpyAsync(c.cpl_fb.get(), fb, n * 4, hipMemcpyHostToDevice, c.stream));
+            CYMG_HIP(hipMemcpyAsync(c.cpl_aa.get(), aa, n * 8, hipMemcpyHostToDevice, c.stream));
+            CYMG_HIP(hipMemcpyAsync(c.cpl_ab.get(), ab, n * 8, hipMemcpyHostToDevice, c.stream));
+            CYMG_HIP(hipMemcpyAsync(c.cpl_w.get(), w, n * 8, hipMemcpyHostToDevice, c.stream));
+            CYMG_HIP(hipStreamSynchronize(c.stream));
+        }
+    });
+}
+
+int cymg_couples_interactions(cymg_context* ctx)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c);
+        c.marks.push_back({ 2, 0.0, 0.0, c.hB.key.size(), c.hC.key.size(), c.hBas.key.size() });
+    });
+}
+
+// ---- solve ---------------------------------------------------------------------
+
+int cymg_setup(cymg_context* ctx, const double* gauss, int on_device)
+{
+    return guarded(ctx, [&](Context& c) { do_setup(c, gauss, on_device); });
+}
+
+int cymg_iterate(cymg_context* ctx, const cymg_solve_params* p, cymg_solve_report* rep)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!p) throw Error(CYMG_ERR_INVALID, "null parameters");
+        do_iterate(c, p, rep);
+        collect_timings(c);
+    });
+}
+
+int cymg_apply(cymg_context* ctx)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "nothing solved");
+        do_apply(c);
+    });
+}
+
+int cymg_solve(cymg_context* ctx, const cymg_solve_params* p, const double* gauss, cymg_solve_report* rep)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!p) throw Error(CYMG_ERR_INVALID, "null parameters");
+        do_setup(c, gauss, p->gauss_on_device);
+        do_iterate(c, p, rep);
+        do_apply(c);
+        collect_timings(c);
+    });
+}
+
+// ---- operators -------------------------------------------------------------------
+
+int cymg_multiply(cymg_context* ctx, const double* X, double* Y)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_multiply");
+        const size_t n = c.n();
+        c.k_v.resize(n); c.k_t.resize(n);
+        CYMG_HIP(hipMemcpyAsync(c.k_v.get(), X, n * 8, hipMemcpyHostToDevice, c.stream));
+        launch_matvec(c, c.k_v.get(), c.k_t.get());
+        CYMG_HIP(hipMemcpyAsync(Y, c.k_t.get(), n * 8, hipMemcpyDeviceToHost, c.stream));
+        CYMG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int cymg_multiply_device(cymg_context* ctx, const double* dX, double* dY)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_multiply");
+        launch_matvec(c, dX, dY);
+    });
+}
+
+int cymg_compute_preconditioner(cymg_context* ctx, int precision)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede the preconditioner");
+        if (precision != CYMG_PRECOND_FP64 && precision != CYMG_PRECOND_FP32)
+            throw Error(CYMG_ERR_INVALID, "precision must be 1 (FP64) or 2 (FP32)");
+        launch_precond_build(c, precision);
+        CYMG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int cymg_precondition(cymg_context* ctx, const double* X, double* Y)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.pc_ready) throw Error(CYMG_ERR_INVALID, "no preconditioner computed");
+        const size_t n = c.n();
+        c.k_v.resize(n); c.k_t.resize(n);
+        CYMG_HIP(hipMemcpyAsync(c.k_v.get(), X, n * 8, hipMemcpyHostToDevice, c.stream));
+        launch_precond_apply(c, c.k_v.get(), c.k_t.get());
+        CYMG_HIP(hipMemcpyAsync(Y, c.k_t.get(), n * 8, hipMemcpyDeviceToHost, c.stream));
+        CYMG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int cymg_precondition_device(cymg_context* ctx, const double* dX, double* dY)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.pc_ready) throw Error(CYMG_ERR_INVALID, "no preconditioner computed");
+        launch_precond_apply(c, dX, dY);
+    });
+}
+
+int cymg_compute_forces(cymg_context* ctx)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.prepared) throw Error(CYMG_ERR_INVALID, "cymg_prepare must precede cymg_compute_forces");
+        build_streams(c);
+        launch_forces(c, c.d_pts.get(), c.d_for.get(), false);
+        launch_tensions(c);
+        CYMG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+// ---- export ------------------------------------------------------------------------
+
+static void d2h(Context& c, void* dst, const void* src, size_t bytes)
+{
+    CYMG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+    CYMG_HIP(hipStreamSynchronize(c.stream));
+}
+
+int cymg_get_vector(cymg_context* ctx, int which, double* out)
+{
+    return guarded(ctx, [&](Context& c) {
+        const double* src = nullptr;
+        switch (which) {
+            case CYMG_VEC_PTS: src = c.d_pts.get(); break;
+            case CYMG_VEC_SOL: src = c.d_sol.get(); break;
+            case CYMG_VEC_BAS: src = c.d_bas.get(); break;
+            case CYMG_VEC_RHS: src = c.d_rhs.get(); break;
+            case CYMG_VEC_FOR: src = c.d_for.get(); break;
+            default: throw Error(CYMG_ERR_INVALID, "unknown vector");
+        }
+        if (!src) throw Error(CYMG_ERR_INVALID, "vector not allocated");
+        d2h(c, out, src, (size_t)c.n() * 8);
+    });
+}
+
+int cymg_get_points(cymg_context* ctx, double* out)
+{
+    return guarded(ctx, [&](Context& c) { d2h(c, out, c.d_pos.get(), (size_t)c.n() * 8); });
+}
+
+int cymg_get_tensions(cymg_context* ctx, double* out)
+{
+    return guarded(ctx, [&](Context& c) { d2h(c, out, c.d_lag.get(), (size_t)c.nseg * 8); });
+}
+
+int cymg_get_block(cymg_context* ctx, uint32_t m, int inverse, double* out)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (m >= c.nfib) throw Error(CYMG_ERR_INVALID, "no such Mecable");
+        const uint64_t bs = (uint64_t)c.dim * c.h_np[m];
+        const uint64_t off = c.h_blkoff[m];
+        if (inverse) {
+            if (!c.pc_ready) throw Error(CYMG_ERR_INVALID, "no preconditioner computed");
+            int use = 0;
+            d2h(c, &use, c.d_useblk.get() + m, sizeof(int));
+            if (!use) throw Error(CYMG_ERR_INVALID, "block not factorised (identity used)");
+            if (c.pc_precision == CYMG_PRECOND_FP32) {
+                std::vector<float> tmp(bs * bs);
+                d2h(c, tmp.data(), c.d_binv32.get() + off, bs * bs * 4);
+                for (uint64_t i = 0; i < bs * bs; ++i) out[i] = tmp[i];
+            } else {
+                d2h(c, out, c.d_binv.get() + off, bs * bs * 8);
+            }
+        } else {
+            if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_get_block");
+            DevBuf<double> tmp;
+            tmp.resize(c.blk_total);
+            launch_precond_build(c, CYMG_PRECOND_FP64, tmp.get());
+            d2h(c, out, tmp.get() + off, bs * bs * 8);
+        }
+    });
+}
+
+int cymg_get_matrix(cymg_context* ctx, int which, uint32_t* nnz, uint32_t* rows, uint32_t* cols, double* vals)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!nnz) throw Error(CYMG_ERR_INVALID, "null nnz");
+        DevSym& m = which ? c.mC : c.mB;
+        *nnz = 0;
+        if (!m.used) return;
+        uint32_t nr = 0;
+        d2h(c, &nr, m.nruns.get(), 4);
+        std::vector<uint64_t> k(nr);
+        std::vector<double> v(nr);
+        d2h(c, k.data(), m.low_key.get(), nr * 8);
+        d2h(c, v.data(), m.low_val.get(), nr * 8);
+        // the reference creates the diagonal element of every column it touches
+        // (matsparsesym1.cc:154-174); the device CSR leaves such zeros implicit,
+        // the export restores them
+        uint32_t cnt = 0;
+        uint32_t prev_col = kInvalid32;
+        for (uint32_t i = 0; i < nr; ++i) {
+            if (k[i] == kInvalidKey) continue;
+            const uint32_t col = (uint32_t)(k[i] >> 32), row = (uint32_t)k[i];
+            if (col != prev_col && row != col) {
+                if (rows) { rows[cnt] = col; cols[cnt] = col; vals[cnt] = 0.0; }
+                ++cnt;
+            }
+            prev_col = col;
+            if (rows) { rows[cnt] = row; cols[cnt] = col; vals[cnt] = v[i]; }
+            ++cnt;
+        }
+        *nnz = cnt;
+    });
+}
+
+int cymg_enable_timing(cymg_context* ctx, int enable)
+{
+    return guarded(ctx, [&](Context& c) { c.timing = enable != 0; });
+}
+
+int cymg_time_kernel(cymg_context* ctx, int which, int reps, double* avg_ms)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!avg_ms || reps <= 0 || which < 0 || which > 3) throw Error(CYMG_ERR_INVALID, "invalid arguments");
+        if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_time_kernel");
+        if (which == 1 && !c.pc_ready) throw Error(CYMG_ERR_INVALID, "no preconditioner computed");
+        const size_t n = c.n();
+        c.k_v.resize(n); c.k_t.resize(n);
+        CYMG_HIP(hipMemcpyAsync(c.k_v.get(), c.d_rhs.get(), n * 8, hipMemcpyDeviceToDevice, c.stream));
+        auto once = [&]() {
+            if (which == 0) launch_matvec(c, c.k_v.get(), c.k_t.get());
+            else if (which == 1) launch_precond_apply(c, c.k_v.get(), c.k_t.get());
+            else launch_precond_build(c, which == 2 ? CYMG_PRECOND_FP64 : CYMG_PRECOND_FP32);
+        };
+        once();
+        CYMG_HIP(hipEventRecord(c.ev[8], c.stream));
+        for (int i = 0; i < reps; ++i) once();
+        CYMG_HIP(hipEventRecord(c.ev[9], c.stream));
+        CYMG_HIP(hipEventSynchronize(c.ev[9]));
+        float ms = 0;
+        CYMG_HIP(hipEventElapsedTime(&ms, c.ev[8], c.ev[9]));
+        *avg_ms = ms / reps;
+    });
+}
+
+int cymg_get_timings(cymg_context* ctx, cymg_timings* out)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!out) throw Error(CYMG_ERR_INVALID, "null output");
+        *out = c.tm;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,36 @@
+// Builds the by-value kernel arguments from a context.
+#pragma once
+
+#include "context.h"
+#include "fiber_ops.h"
+
+namespace cymg {
+
+inline FiberArgs make_fiber_args(const Context& c)
+{
+    FiberArgs a;
+    a.np = c.d_np.get();
+    a.idx = c.d_idx.get();
+    a.seg = c.d_seg.get();
+    a.fib_sys = c.d_fib_sys.get();
+    a.sys_begin = c.d_sys_begin.get();
+    a.mob = c.d_mob.get();
+    a.rfrig = c.d_rfrig.get();
+    a.cut = c.d_cut.get();
+    a.diff = c.d_diff.get();
+    a.jd = c.d_jd.get();
+    a.je = c.d_je.get();
+    a.pdiff = c.d_pdiff.get();
+    a.Brow = c.mB.rowptr.get();
+    a.Bcol = c.mB.col.get();
+    a.Bval = c.mB.val.get();
+    a.Buse = c.mB.used ? 1 : 0;
+    a.Crow = c.mC.rowptr.get();
+    a.Ccol = c.mC.col.get();
+    a.Cval = c.mC.val.get();
+    a.Cuse = c.mC.used ? 1 : 0;
+    a.dt = c.prm.time_step;
+    return a;
+}
+
+}  // namespace cymg

@@ -1,0 +1,246 @@
+// Device-side assembly of mB, mC and vBAS.
+//
+// The reference writes every interaction into MatrixSparseSymmetric1 columns
+// as it is registered (matsparsesym1.cc:137-224): the index structure is the
+// set of touched lower-triangle (row >= col) pairs, and each value is the sum
+// of its contributions in call order.  Here every contribution is an entry
+// (key, delta) of an accumulation stream kept in call order; a stable radix
+// sort on the key groups equal keys while keeping call order inside a group,
+// and one thread per unique key adds its deltas sequentially from 0.0.  This
+// reproduces both the reference's index set and its values bit for bit
+// (v - x == v + (-x) in IEEE arithmetic).  The unique lower entries are then
+// mirrored into a full symmetric CSR (rows sorted by column) for the
+// gather-only matrix-vector products.
+#include <hipcub/hipcub.hpp>
+
+#include "context.h"
+
+namespace cymg {
+
+static int bits_for(uint64_t v)
+{
+    int b = 0;
+    while (v) { ++b; v >>= 1; }
+    return b;
+}
+
+__global__ void k_iota(uint32_t n, uint32_t* __restrict__ out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = i;
+}
+
+// sequential in-order sum of each run of equal keys
+__global__ void k_run_sum(uint32_t total, const uint32_t* __restrict__ nruns, const uint64_t* __restrict__ ukey,
+                          const uint32_t* __restrict__ off, const uint32_t* __restrict__ len,
+                          const uint32_t* __restrict__ perm, const double* __restrict__ val,
+                          uint64_t* __restrict__ low_key, double* __restrict__ low_val)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= total || u >= nruns[0]) return;
+    const uint64_t k = ukey[u];
+    double s = 0.0;
+    if (k != kInvalidKey) {
+        const uint32_t b = off[u], e = b + len[u];
+        for (uint32_t i = b; i < e; ++i) s = s + val[perm[i]];
+    }
+    low_key[u] = k;
+    low_val[u] = s;
+}
+
+// vBAS: dense scatter of the summed entries
+__global__ void k_bas_scatter(uint32_t total, const uint32_t* __restrict__ nruns, const uint64_t* __restrict__ low_key,
+                              const double* __restrict__ low_val, double* __restrict__ bas)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= total || u >= nruns[0]) return;
+    const uint64_t k = low_key[u];
+    if (k != kInvalidKey) bas[(uint32_t)k] = low_val[u];
+}
+
+// mirror unique lower entries (key col<<32|row) into row-major keys row<<32|col
+__global__ void k_mirror(uint32_t total, const uint32_t* __restrict__ nruns, const uint64_t* __restrict__ low_key,
+                         uint64_t* __restrict__ key2, uint32_t* __restrict__ u2)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= total) return;
+    uint64_t a = kInvalidKey, b = kInvalidKey;
+    if (u < nruns[0]) {
+        const uint64_t k = low_key[u];
+        if (k != kInvalidKey) {
+            const uint32_t c = (uint32_t)(k >> 32), r = (uint32_t)k;
+            a = ((uint64_t)r << 32) | c;
+            if (r != c) b = ((uint64_t)c << 32) | r;
+        }
+    }
+    key2[2 * u] = a;
+    key2[2 * u + 1] = b;
+    u2[2 * u] = u;
+    u2[2 * u + 1] = u;
+}
+
+// CSR rows by binary search of the row-sorted keys; invalid keys sort last
+__global__ void k_rowptr(uint32_t size, uint32_t n2, const uint64_t* __restrict__ key2, uint32_t* __restrict__ rowptr)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > size) return;
+    const uint64_t t = (uint64_t)r << 32;
+    uint32_t lo = 0, hi = n2;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (key2[mid] < t) lo = mid + 1; else hi = mid;
+    }
+    rowptr[r] = lo;
+}
+
+__global__ void k_csr_fill(uint32_t n2, const uint64_t* __restrict__ key2, const uint32_t* __restrict__ u2,
+                           const double* __restrict__ low_val, uint32_t* __restrict__ col, uint32_t* __restrict__ map,
+                           double* __restrict__ val)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n2) return;
+    const uint64_t k = key2[i];
+    if (k == kInvalidKey) return;
+    const uint32_t u = u2[i];
+    col[i] = (uint32_t)k;
+    map[i] = u;
+    val[i] = low_val[u];
+}
+
+static inline dim3 grid_for(size_t n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
+
+// sort + run-length + in-order sums of one stream; returns the unique entries in m.low_*
+static void reduce_stream(Context& c, int which, DevSym& m, int end_bit)
+{
+    const size_t total = c.s_total[which];
+    hipStream_t st = c.stream;
+    c.s_pos.resize(total);
+    c.s_pos_sorted.resize(total);
+    c.s_key_sorted.resize(total);
+    c.s_ukey.resize(total);
+    c.s_runlen.resize(total);
+    c.s_runoff.resize(total);
+    m.nruns.resize(1);
+    m.low_key.resize(total);
+    m.low_val.resize(total);
+    hipLaunchKernelGGL(k_iota, grid_for(total), dim3(256), 0, st, (uint32_t)total, c.s_pos.get());
+    size_t tb1 = 0, tb2 = 0, tb3 = 0;
+    CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, c.s_key[which].get(), c.s_key_sorted.get(),
+                                                c.s_pos.get(), c.s_pos_sorted.get(), (int)total, 0, end_bit, st));
+    CYMG_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb2, c.s_key_sorted.get(), c.s_ukey.get(),
+                                                   c.s_runlen.get(), m.nruns.get(), (int)total, st));
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, c.s_runlen.get(), c.s_runoff.get(), (int)total, st));
+    size_t tb = std::max(tb1, std::max(tb2, tb3));
+    c.s_temp.resize(tb);
+    CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(c.s_temp.get(), tb1, c.s_key[which].get(), c.s_key_sorted.get(),
+                                                c.s_pos.get(), c.s_pos_sorted.get(), (int)total, 0, end_bit, st));
+    CYMG_HIP(hipcub::DeviceRunLengthEncode::Encode(c.s_temp.get(), tb2, c.s_key_sorted.get(), c.s_ukey.get(),
+                                                   c.s_runlen.get(), m.nruns.get(), (int)total, st));
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.s_temp.get(), tb3, c.s_runlen.get(), c.s_runoff.get(), (int)total,
+                                              st));
+    hipLaunchKernelGGL(k_run_sum, grid_for(total), dim3(256), 0, st, (uint32_t)total, m.nruns.get(), c.s_ukey.get(),
+                       c.s_runoff.get(), c.s_runlen.get(), c.s_pos_sorted.get(), c.s_val[which].get(),
+                       m.low_key.get(), m.low_val.get());
+    CYMG_HIP(hipGetLastError());
+}
+
+static void build_csr(Context& c, DevSym& m)
+{
+    const size_t total = m.low_key.n;
+    const size_t n2 = 2 * total;
+    hipStream_t st = c.stream;
+    c.c_key.resize(n2);
+    c.c_key_sorted.resize(n2);
+    c.c_u.resize(n2);
+    c.c_u_sorted.resize(n2);
+    m.rowptr.resize(m.size + 1);
+    m.col.resize(n2);
+    m.map.resize(n2);
+    m.val.resize(n2);
+    hipLaunchKernelGGL(k_mirror, grid_for(total), dim3(256), 0, st, (uint32_t)total, m.nruns.get(), m.low_key.get(),
+                       c.c_key.get(), c.c_u.get());
+    const int end_bit = 32 + bits_for(m.size);
+    size_t tb = 0;
+    CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, c.c_key.get(), c.c_key_sorted.get(), c.c_u.get(),
+                                                c.c_u_sorted.get(), (int)n2, 0, end_bit, st));
+    c.s_temp.resize(std::max(tb, c.s_temp.n));
+    CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(c.s_temp.get(), tb, c.c_key.get(), c.c_key_sorted.get(), c.c_u.get(),
+                                                c.c_u_sorted.get(), (int)n2, 0, end_bit, st));
+    hipLaunchKernelGGL(k_rowptr, grid_for(m.size + 1), dim3(256), 0, st, m.size, (uint32_t)n2, c.c_key_sorted.get(),
+                       m.rowptr.get());
+    hipLaunchKernelGGL(k_csr_fill, grid_for(n2), dim3(256), 0, st, (uint32_t)n2, c.c_key_sorted.get(),
+                       c.c_u_sorted.get(), m.low_val.get(), m.col.get(), m.map.get(), m.val.get());
+    CYMG_HIP(hipGetLastError());
+}
+
+// Lay out the three streams in call order (host ranges and emitter outputs),
+// emit on the device, then reduce.
+void build_streams(Context& c)
+{
+    const int D = c.dim;
+    const size_t NC = (size_t)D * (D + 1) / 2;
+    HostStream* hs[3] = { &c.hB, &c.hC, &c.hBas };
+    // sizes
+    size_t total[3] = { 0, 0, 0 };
+    for (int w = 0; w < 3; ++w) total[w] = hs[w]->key.size();
+    for (const EmitterMark& mk : c.marks) {
+        if (mk.kind == 1) { total[1] += NC * c.npts; total[2] += (size_t)D * c.npts; }
+        if (mk.kind == 2) { total[0] += (size_t)10 * c.ncouples; }
+    }
+    for (int w = 0; w < 3; ++w) {
+        c.s_total[w] = total[w];
+        c.s_key[w].resize(total[w] ? total[w] : 1);
+        c.s_val[w].resize(total[w] ? total[w] : 1);
+    }
+    // place host ranges and emitters
+    size_t hpos[3] = { 0, 0, 0 }, dpos[3] = { 0, 0, 0 };
+    auto copy_host = [&](int w, size_t upto) {
+        const size_t cnt = upto - hpos[w];
+        if (cnt) {
+            CYMG_HIP(hipMemcpyAsync(c.s_key[w].get() + dpos[w], hs[w]->key.data() + hpos[w], cnt * sizeof(uint64_t),
+                                    hipMemcpyHostToDevice, c.stream));
+            CYMG_HIP(hipMemcpyAsync(c.s_val[w].get() + dpos[w], hs[w]->val.data() + hpos[w], cnt * sizeof(double),
+                                    hipMemcpyHostToDevice, c.stream));
+        }
+        dpos[w] += cnt;
+        hpos[w] = upto;
+    };
+    for (const EmitterMark& mk : c.marks) {
+        copy_host(0, mk.posB);
+        copy_host(1, mk.posC);
+        copy_host(2, mk.posBas);
+        if (mk.kind == 1) {
+            launch_emit_confine(c, c.s_key[1].get() + dpos[1], c.s_val[1].get() + dpos[1], c.s_key[2].get() + dpos[2],
+                                c.s_val[2].get() + dpos[2], mk.a, mk.b);
+            dpos[1] += NC * c.npts;
+            dpos[2] += (size_t)D * c.npts;
+        } else if (mk.kind == 2) {
+            launch_emit_couples(c, c.s_key[0].get() + dpos[0], c.s_val[0].get() + dpos[0]);
+            dpos[0] += (size_t)10 * c.ncouples;
+        }
+    }
+    for (int w = 0; w < 3; ++w) copy_host(w, hs[w]->key.size());
+
+    // mB over points, mC over degrees of freedom
+    c.mB.size = c.npts;
+    c.mC.size = c.n();
+    DevSym* mats[2] = { &c.mB, &c.mC };
+    for (int w = 0; w < 2; ++w) {
+        DevSym& m = *mats[w];
+        m.used = total[w] > 0;
+        if (!m.used) continue;
+        reduce_stream(c, w, m, 32 + bits_for(m.size));
+        build_csr(c, m);
+    }
+    // vBAS
+    CYMG_HIP(hipMemsetAsync(c.d_bas.get(), 0, (size_t)c.n() * sizeof(double), c.stream));
+    if (total[2]) {
+        DevSym& m = c.mBas;
+        m.size = c.n();
+        reduce_stream(c, 2, m, bits_for(m.size));
+        hipLaunchKernelGGL(k_bas_scatter, grid_for(total[2]), dim3(256), 0, c.stream, (uint32_t)total[2],
+                           m.nruns.get(), m.low_key.get(), m.low_val.get(), c.d_bas.get());
+        CYMG_HIP(hipGetLastError());
+    }
+}
+
+}  // namespace cymg

@@ -1,0 +1,123 @@
+// Shared definitions of the MI355X Meca library: device buffers, the context
+// layout in HBM, and the launchers each translation unit provides.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cymeca.h"
+
+namespace cymg {
+
+constexpr int kWave = 64;               // CDNA wavefront
+constexpr int kMaxFiberPoints = CYMG_MAX_FIBER_POINTS;
+constexpr uint32_t kInvalid32 = 0xFFFFFFFFu;
+constexpr uint64_t kInvalidKey = 0xFFFFFFFFFFFFFFFFull;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define CYMG_HIP(call)                                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            throw ::cymg::Error(CYMG_ERR_HIP, std::string(#call) + ": " +               \
+                                                  hipGetErrorString(e_));               \
+    } while (0)
+
+// grow-only device buffer
+template <typename T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t cap = 0;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { if (ptr) (void)hipFree(ptr); }
+    void resize(size_t count) {
+        n = count;
+        if (count > cap) {
+            if (ptr) CYMG_HIP(hipFree(ptr));
+            ptr = nullptr;
+            size_t c = count + count / 4 + 64;
+            CYMG_HIP(hipMalloc(&ptr, c * sizeof(T)));
+            cap = c;
+        }
+    }
+    T* get() const { return ptr; }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Per-Krylov-system scalar state, one record per independent system.
+struct alignas(16) SysScalars {
+    double rho1, rho2, alpha, omega, beta;
+    double tol;         // residual threshold
+    double resid;       // last |r|_inf
+    double resid_s;     // |s|_inf after the alpha update (breakdown reporting)
+    double rts, rtt;    // <rt,s>, <rt,t>
+    double bnorm;       // |b|_inf
+    double noise;       // min over Mecables of mobility*b (noiseLevel / time_step)
+    int    iter, iter_max;
+    int    done, flag, pending, pad;
+};
+
+// accumulation stream: (key, delta) pairs in call order
+struct HostStream {
+    std::vector<uint64_t> key;
+    std::vector<double> val;
+    void clear() { key.clear(); val.clear(); }
+};
+
+// device emitter placed in call order: the host-recorded entries before it are
+// those below its recorded positions in each host stream
+struct EmitterMark {
+    int kind;                     // 1 sphere confinement, 2 couples
+    double a, b;                  // emitter parameters (radius, stiffness)
+    size_t posB, posC, posBas;    // host stream sizes when the emitter was called
+};
+
+// symmetric sparse matrix on the device, built from a stream
+struct DevSym {
+    uint32_t size = 0;      // rows
+    uint32_t nlow = 0;      // unique lower entries
+    uint32_t nnz = 0;       // full CSR entries
+    bool used = false;      // Meca::use_mB / use_mC
+    DevBuf<uint64_t> low_key;   // sorted unique lower keys (col<<32 | row)
+    DevBuf<double> low_val;
+    DevBuf<uint32_t> rowptr;    // full symmetric CSR
+    DevBuf<uint32_t> col;
+    DevBuf<double> val;
+    DevBuf<uint32_t> map;       // CSR slot -> lower entry
+    DevBuf<uint32_t> nruns;     // device count of runs (unique keys incl. invalid)
+};
+
+struct Context;
+
+// launchers (assembly.hip)
+void build_streams(Context& c);
+// launchers (fiber.hip)
+void launch_prepare(Context& c);
+void launch_emit_confine(Context& c, uint64_t* keyC, double* valC, uint64_t* keyBas,
+                         double* valBas, double radius, double stiffness);
+void launch_emit_couples(Context& c, uint64_t* keyB, double* valB);
+void launch_setup(Context& c, const double* gauss);
+void launch_matvec(Context& c, const double* X, double* Y);
+void launch_forces(Context& c, const double* X, double* Y, bool with_rigidity);
+void launch_apply(Context& c);
+void launch_tensions(Context& c);
+// launchers (precond.hip)
+void launch_precond_build(Context& c, int precision, double* block_out = nullptr);
+void launch_precond_apply(Context& c, const double* X, double* Y);
+// launchers (krylov.hip)
+void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter, double tol,
+                  int tol_mode, cymg_solve_report* rep);
+
+}  // namespace cymg

@@ -1,0 +1,78 @@
+// Host-side context of one Meca instance: registration, assembly streams and
+// the device arrays of the step.
+#pragma once
+
+#include "common.h"
+
+namespace cymg {
+
+struct Context {
+    int dim = 3;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    cymg_sim_params prm{0.005, 0.0042, 1.0, 0.0125, 10.0};
+
+    // registration (Meca::add), host copy
+    std::vector<uint32_t> h_np, h_idx, h_seg;
+    std::vector<uint64_t> h_blkoff;
+    std::vector<double> h_cut, h_rig;
+    uint32_t nfib = 0, npts = 0, nseg = 0, largest = 0;
+    uint64_t blk_total = 0;
+    bool layout_dirty = true;
+    uint32_t nsys = 1;
+
+    // per-fiber device arrays
+    DevBuf<uint32_t> d_np, d_idx, d_seg, d_fib_sys, d_sys_begin;
+    DevBuf<uint64_t> d_blkoff;
+    DevBuf<double> d_cut, d_rig, d_mob, d_rfrig;
+
+    // vectors of DIM*nbPts (meca.h:101-106) and per-segment arrays
+    DevBuf<double> d_pts, d_pos, d_sol, d_bas, d_rhs, d_for, d_tmp, d_prev_sol, d_gauss;
+    DevBuf<double> d_diff, d_jd, d_je, d_lag, d_pdiff;
+    DevBuf<int> d_status;              // [0]: projection failures
+
+    // assembly streams (mB, mC, vBAS) in call order
+    HostStream hB, hC, hBas;
+    std::vector<EmitterMark> marks;
+    DevBuf<uint64_t> s_key[3];         // stream inputs (B, C, vBAS) in call order
+    DevBuf<double> s_val[3];
+    size_t s_total[3] = {0, 0, 0};
+    DevBuf<uint64_t> s_key_sorted, s_ukey;
+    DevBuf<uint32_t> s_pos, s_pos_sorted, s_runlen, s_runoff;
+    DevBuf<uint64_t> c_key, c_key_sorted;
+    DevBuf<uint32_t> c_u, c_u_sorted;
+    DevBuf<unsigned char> s_temp;
+    DevSym mB, mC, mBas;
+
+    // couples (device emitter)
+    uint32_t ncouples = 0;
+    DevBuf<uint32_t> cpl_fa, cpl_fb;
+    DevBuf<double> cpl_aa, cpl_ab, cpl_w;
+
+    // preconditioner
+    int pc_precision = 0;              // CYMG_PRECOND_*
+    DevBuf<double> d_binv;
+    DevBuf<float> d_binv32;
+    DevBuf<int> d_useblk;
+
+    // Krylov work vectors and scalars
+    DevBuf<double> k_r, k_rt, k_p, k_v, k_t, k_ph, k_sh;
+    DevBuf<double> d_part;             // partial reductions [slot][nfib]
+    DevBuf<SysScalars> d_scal;
+    DevBuf<unsigned> d_counter;
+    SysScalars* h_scal = nullptr;      // pinned mirror
+
+    bool prepared = false, setup_done = false, pc_ready = false, have_solution = false;
+
+    // timing
+    bool timing = false;
+    cymg_timings tm{};
+    hipEvent_t ev[10]{};
+    double mv_ms_sum = 0, pc_ms_sum = 0;
+    int mv_count = 0, pc_count = 0;
+
+    uint32_t n() const { return (uint32_t)dim * npts; }
+};
+
+}  // namespace cymg

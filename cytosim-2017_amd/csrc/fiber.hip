@@ -1,0 +1,622 @@
+// Per-fiber kernels of the implicit step: Meca::prepare, the device-side
+// emitters, the force/RHS setup, the standalone operator, and the post-solve
+// update (vPTS += vSOL, forces, getPoints reshape).  One wavefront per fiber.
+#include "args.h"
+
+namespace cymg {
+
+constexpr double kRealEpsilon = 128.0 * 2.220446049250313e-16;  // src/math/real.h:32
+
+// ---------------------------------------------------------------------------
+// Meca::prepare (meca.cc:488-537): putPoints, then Fiber::prepareMecable
+// (fiber.cc:450-460): setDragCoefficient (fiber.cc:315-363, 428-447),
+// storeDifferences (rigid_fiber.cc:111-117), makeProjection
+// (rigid_fiber_project.cc:50-87), rfRigidity = rigidity / fnCut^3.
+template <int D>
+__global__ __launch_bounds__(kWave) void k_prepare(const uint32_t* __restrict__ npv, const uint32_t* __restrict__ idxv,
+                                                   const uint32_t* __restrict__ segv, const double* __restrict__ cutv,
+                                                   const double* __restrict__ rigv, const double* __restrict__ pos,
+                                                   double* __restrict__ pts, double* __restrict__ mob,
+                                                   double* __restrict__ rfrig, double* __restrict__ diff,
+                                                   double* __restrict__ jd, double* __restrict__ je,
+                                                   int* __restrict__ status, double visc, double r0, double r1)
+{
+    __shared__ double sdif[kMaxFiberPoints * D];
+    __shared__ double sd[kMaxFiberPoints], se[kMaxFiberPoints];
+    const uint32_t f = blockIdx.x;
+    const uint32_t np = npv[f], idx = idxv[f], seg = segv[f];
+    const int S = (int)np - 1;
+    const double cut = cutv[f];
+    for (int i = threadIdx.x; i < (int)(D * np); i += kWave) pts[D * idx + i] = pos[D * idx + i];
+    const double sc = 1.0 / cut;
+    for (int i = threadIdx.x; i < D * S; i += kWave) {
+        const double v = sc * (pos[D * idx + i + D] - pos[D * idx + i]);
+        sdif[i] = v;
+        diff[D * seg + i] = v;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < S; q += kWave) {
+        double dd = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) dd += sdif[D * q + d] * sdif[D * q + d];
+        sd[q] = 2 * dd;
+        if (q + 1 < S) {
+            double od = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) od += sdif[D * q + d] * sdif[D * q + D + d];
+            se[q] = -od;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // dpttrf
+        int info = 0;
+        for (int i = 0; i < S - 1; ++i) {
+            if (sd[i] <= 0.0) { info = i + 1; break; }
+            const double ei = se[i];
+            se[i] = ei / sd[i];
+            sd[i + 1] -= se[i] * ei;
+        }
+        if (!info && S > 0 && sd[S - 1] <= 0.0) info = S;
+        if (info) atomicAdd(status, 1);
+        // drag and mobility
+        const double len = (double)S * cut;
+        double lenc = len > r1 ? r1 : len;
+        const double drag_sphere = 6 * r0;
+        const double drag_cyl = 3.0 * len / (log(0.5 * lenc / r0) + 0.312);
+        const double drag = M_PI * visc * (drag_cyl > drag_sphere ? drag_cyl : drag_sphere);
+        mob[f] = (double)np / drag;
+        rfrig[f] = rigv[f] / (cut * cut * cut);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < S; q += kWave) {
+        jd[seg + q] = sd[q];
+        if (q + 1 < S) je[seg + q] = se[q];
+    }
+}
+
+void launch_prepare(Context& c)
+{
+    if (!c.nfib) return;
+    c.d_status.resize(4);
+    CYMG_HIP(hipMemsetAsync(c.d_status.get(), 0, 4 * sizeof(int), c.stream));
+    const dim3 g(c.nfib), b(kWave);
+    if (c.dim == 3)
+        hipLaunchKernelGGL(k_prepare<3>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_seg.get(), c.d_cut.get(),
+                           c.d_rig.get(), c.d_pos.get(), c.d_pts.get(), c.d_mob.get(), c.d_rfrig.get(),
+                           c.d_diff.get(), c.d_jd.get(), c.d_je.get(), c.d_status.get(), c.prm.viscosity,
+                           c.prm.hydro_radius0, c.prm.hydro_radius1);
+    else
+        hipLaunchKernelGGL(k_prepare<2>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_seg.get(), c.d_cut.get(),
+                           c.d_rig.get(), c.d_pos.get(), c.d_pts.get(), c.d_mob.get(), c.d_rfrig.get(),
+                           c.d_diff.get(), c.d_jd.get(), c.d_je.get(), c.d_status.get(), c.prm.viscosity,
+                           c.prm.hydro_radius0, c.prm.hydro_radius1);
+    CYMG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Fiber::setInteractions, CONFINE_INSIDE (fiber.cc:471-481) with
+// SpaceSphere::inside (space_sphere.cc:113-116) and SpaceSphere::setInteraction
+// -> Meca::interLongClamp(pe, 0, radius, stiffness) (space_sphere.cc:148-151,
+// meca_inter.cc:1977-2022).  Fixed slots per point: D(D+1)/2 mC entries and D
+// vBAS entries, invalid keys for points inside.
+__device__ __forceinline__ uint64_t lower_key(uint32_t i, uint32_t j)
+{
+    const uint32_t r = i > j ? i : j, c = i > j ? j : i;
+    return ((uint64_t)c << 32) | r;
+}
+
+template <int D>
+__global__ void k_emit_confine(uint32_t npts, const double* __restrict__ pos, double radius, double w,
+                               uint64_t* __restrict__ keyC, double* __restrict__ valC,
+                               uint64_t* __restrict__ keyBas, double* __restrict__ valBas)
+{
+#pragma clang fp contract(off)
+    constexpr int NC = D * (D + 1) / 2;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    double x[D];
+    double r2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        x[d] = pos[D * p + d];
+        r2 += x[d] * x[d];
+    }
+    uint64_t* kc = keyC + (size_t)NC * p;
+    double* vc = valC + (size_t)NC * p;
+    uint64_t* kb = keyBas + (size_t)D * p;
+    double* vb = valBas + (size_t)D * p;
+    bool emit = !(r2 <= radius * radius);
+    double axis[D], an = 0.0;
+    if (emit) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            axis[d] = x[d] - 0.0;
+            an += axis[d] * axis[d];
+        }
+        an = sqrt(an);
+        if (an < kRealEpsilon) emit = false;
+    }
+    if (!emit) {
+        for (int k = 0; k < NC; ++k) { kc[k] = kInvalidKey; vc[k] = 0.0; }
+        for (int k = 0; k < D; ++k) { kb[k] = kInvalidKey; vb[k] = 0.0; }
+        return;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) axis[d] /= an;
+    double ac = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) ac += axis[d] * 0.0;
+    const uint32_t inx = D * p;
+    int k = 0;
+    double len = radius;
+    if (len < an) {
+        len /= an;
+        const double facX = w * len * (an + ac);
+        const double facC = w * (1.0 - len);
+        for (int i = 0; i < D; ++i) {
+            kc[k] = lower_key(inx + i, inx + i);
+            vc[k] = w * (len * (1.0 - axis[i] * axis[i]) - 1.0);
+            ++k;
+            for (int j = i + 1; j < D; ++j) {
+                kc[k] = lower_key(inx + i, inx + j);
+                vc[k] = -(w * len * axis[i] * axis[j]);
+                ++k;
+            }
+            kb[i] = inx + i;
+            vb[i] = facX * axis[i] + facC * 0.0;
+        }
+    } else {
+        const double facX = w * (len + ac);
+        for (int i = 0; i < D; ++i) {
+            for (int j = i; j < D; ++j) {
+                kc[k] = lower_key(inx + i, inx + j);
+                vc[k] = -(w * axis[i] * axis[j]);
+                ++k;
+            }
+            kb[i] = inx + i;
+            vb[i] = facX * axis[i];
+        }
+    }
+}
+
+void launch_emit_confine(Context& c, uint64_t* keyC, double* valC, uint64_t* keyBas, double* valBas, double radius,
+                         double stiffness)
+{
+    if (!c.npts) return;
+    const dim3 b(256), g((c.npts + 255) / 256);
+    if (c.dim == 3)
+        hipLaunchKernelGGL(k_emit_confine<3>, g, b, 0, c.stream, c.npts, c.d_pos.get(), radius, stiffness, keyC, valC,
+                           keyBas, valBas);
+    else
+        hipLaunchKernelGGL(k_emit_confine<2>, g, b, 0, c.stream, c.npts, c.d_pos.get(), radius, stiffness, keyC, valC,
+                           keyBas, valBas);
+    CYMG_HIP(hipGetLastError());
+}
+
+// The emitters evaluate every product and sum rounded as written
+// (contraction off), the arithmetic of the reference source.
+// Couple::setInteractions (couple.cc:87-92) -> interLink(PI, PI)
+// (meca_inter.cc:285-321) with FiberNaked::interpolate (fiber_naked.cc:1050-1065).
+// 10 slots per couple; overlapping hands give invalid keys.
+__device__ __forceinline__ void interp(uint32_t np, uint32_t idx, double cut, double ab, uint32_t& m1, double& co)
+{
+    if (ab <= 0) { m1 = idx; co = 0.0; return; }
+    double n;
+    const double fr = modf(ab / cut, &n);
+    const uint32_t rd = (uint32_t)n;
+    if (rd < np - 1) { m1 = idx + rd; co = fr; }
+    else { m1 = idx + np - 2; co = 1.0; }
+}
+
+__global__ void k_emit_couples(uint32_t n, const uint32_t* __restrict__ fa, const double* __restrict__ aa,
+                               const uint32_t* __restrict__ fb, const double* __restrict__ ab,
+                               const double* __restrict__ wv, const uint32_t* __restrict__ npv,
+                               const uint32_t* __restrict__ idxv, const double* __restrict__ cutv,
+                               uint64_t* __restrict__ keyB, double* __restrict__ valB)
+{
+#pragma clang fp contract(off)
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t f1 = fa[k], f2 = fb[k];
+    uint32_t a1, b1;
+    double ca, cb;
+    interp(npv[f1], idxv[f1], cutv[f1], aa[k], a1, ca);
+    interp(npv[f2], idxv[f2], cutv[f2], ab[k], b1, cb);
+    const uint32_t a2 = a1 + 1, b2 = b1 + 1;
+    uint64_t* kk = keyB + (size_t)10 * k;
+    double* vv = valB + (size_t)10 * k;
+    if (f1 == f2 && (a1 == b1 || a1 == b2 || a2 == b1 || a2 == b2)) {
+        for (int i = 0; i < 10; ++i) { kk[i] = kInvalidKey; vv[i] = 0.0; }
+        return;
+    }
+    const double w = wv[k];
+    const double cc[4] = { 1.0 - ca, ca, -(1.0 - cb), -cb };
+    const double cw[4] = { w * cc[0], w * cc[1], w * cc[2], w * cc[3] };
+    const uint32_t inx[4] = { a1, a2, b1, b2 };
+    int s = 0;
+    for (int j = 0; j < 4; ++j)
+        for (int i = j; i < 4; ++i) {
+            kk[s] = lower_key(inx[i], inx[j]);
+            vv[s] = -(cc[j] * cw[i]);
+            ++s;
+        }
+}
+
+void launch_emit_couples(Context& c, uint64_t* keyB, double* valB)
+{
+    if (!c.ncouples) return;
+    const dim3 b(256), g((c.ncouples + 255) / 256);
+    hipLaunchKernelGGL(k_emit_couples, g, b, 0, c.stream, c.ncouples, c.cpl_fa.get(), c.cpl_aa.get(), c.cpl_fb.get(),
+                       c.cpl_ab.get(), c.cpl_w.get(), c.d_np.get(), c.d_idx.get(), c.d_cut.get(), keyB, valB);
+    CYMG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Meca::solve, first part (meca.cc:621-693): vFOR = vBAS + (R + B + C) vPTS,
+// Brownian terms b*N(0,1) with b = sqrt(2 kT/dt / mobility) (rigid_fiber.cc:90-98),
+// vRHS = dt * mobility * P vFOR keeping the multipliers in rfLag
+// (rigid_fiber.cc:124-132), P' coefficients (rigid_fiber_project.cc:168-219).
+// Per-system reductions: noise = min(mobility*b), bnorm = |vRHS|_inf.
+template <int D>
+__global__ __launch_bounds__(kWave) void k_setup(FiberArgs a, const double* __restrict__ pts,
+                                                 const double* __restrict__ bas, const double* __restrict__ gauss,
+                                                 double kT_dt, double* __restrict__ vfor, double* __restrict__ rhs,
+                                                 double* __restrict__ lag, double* __restrict__ pdiff,
+                                                 double* __restrict__ part, unsigned* __restrict__ counter,
+                                                 SysScalars* __restrict__ scal, uint32_t nfib)
+{
+    __shared__ FiberLds<D> s;
+    const uint32_t f = blockIdx.x;
+    const uint32_t np = a.np[f], idx = a.idx[f], seg = a.seg[f];
+    const int S = (int)np - 1;
+    const uint32_t o = D * idx;
+    for (int i = threadIdx.x; i < (int)(D * np); i += kWave) s.x[i] = pts[o + i];
+    load_factors<D>(a, s, seg, S);
+    __syncthreads();
+    fiber_forces<D>(a, s, pts, np, idx, seg, a.rfrig[f], true, false);
+    __syncthreads();
+    const double mobf = a.mob[f];
+    const double b = sqrt(2 * kT_dt / mobf);
+    for (int i = threadIdx.x; i < (int)(D * np); i += kWave) {
+        double v = bas[o + i] + s.f[i];
+        if (gauss) v += b * gauss[o + i];
+        s.f[i] = v;
+        vfor[o + i] = v;
+    }
+    __syncthreads();
+    fiber_project<D>(a, s, np, seg, a.dt * mobf);
+    __syncthreads();
+    double bmax = 0.0;
+    for (int i = threadIdx.x; i < (int)(D * np); i += kWave) {
+        rhs[o + i] = s.y[i];
+        bmax = nan_max(bmax, fabs(s.y[i]));
+    }
+    const double isc = 1.0 / a.cut[f];
+    for (int q = threadIdx.x; q < S; q += kWave) {
+        const double l = s.t[q];
+        lag[seg + q] = l;
+        pdiff[seg + q] = (l < 0) ? 0.0 : l * isc;
+    }
+    bmax = wave_max(bmax);
+    if (threadIdx.x == 0) {
+        part[f] = mobf * b;
+        part[nfib + f] = bmax;
+    }
+    const uint32_t sys = a.fib_sys[f];
+    const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
+    if (last_of_system(counter, sys, fe - fb)) {
+        const double noise = system_min(part, fb, fe);
+        const double bn = system_max(part + nfib, fb, fe);
+        if (threadIdx.x == 0) {
+            scal[sys].noise = noise;
+            scal[sys].bnorm = bn;
+        }
+    }
+}
+
+void launch_setup(Context& c, const double* gauss)
+{
+    if (!c.nfib) return;
+    FiberArgs a = make_fiber_args(c);
+    const dim3 g(c.nfib), b(kWave);
+    const double kT_dt = c.prm.kT / c.prm.time_step;
+    if (c.dim == 3)
+        hipLaunchKernelGGL(k_setup<3>, g, b, 0, c.stream, a, c.d_pts.get(), c.d_bas.get(), gauss, kT_dt,
+                           c.d_for.get(), c.d_rhs.get(), c.d_lag.get(), c.d_pdiff.get(), c.d_part.get(),
+                           c.d_counter.get(), c.d_scal.get(), c.nfib);
+    else
+        hipLaunchKernelGGL(k_setup<2>, g, b, 0, c.stream, a, c.d_pts.get(), c.d_bas.get(), gauss, kT_dt,
+                           c.d_for.get(), c.d_rhs.get(), c.d_lag.get(), c.d_pdiff.get(), c.d_part.get(),
+                           c.d_counter.get(), c.d_scal.get(), c.nfib);
+    CYMG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Meca::multiply (meca.cc:157-181) as a standalone launch
+template <int D>
+__global__ __launch_bounds__(kWave) void k_matvec(FiberArgs a, const double* __restrict__ X, double* __restrict__ Y)
+{
+    __shared__ FiberLds<D> s;
+    const uint32_t f = blockIdx.x;
+    const uint32_t np = a.np[f], o = D * a.idx[f];
+    for (int i = threadIdx.x; i < (int)(D * np); i += kWave) s.x[i] = X[o + i];
+    load_factors<D>(a, s, a.seg[f], (int)np - 1);
+    __syncthreads();
+    fiber_operator<D>(a, s, X, f);
+    for (int i = threadIdx.x; i < (int)(D * np); i += kWave) Y[o + i] = s.y[i];
+}
+
+void launch_matvec(Context& c, const double* X, double* Y)
+{
+    if (!c.nfib) return;
+    FiberArgs a = make_fiber_args(c);
+    const dim3 g(c.nfib), b(kWave);
+    if (c.dim == 3)
+        hipLaunchKernelGGL(k_matvec<3>, g, b, 0, c.stream, a, X, Y);
+    else
+        hipLaunchKernelGGL(k_matvec<2>, g, b, 0, c.stream, a, X, Y);
+    CYMG_HIP(hipGetLastError());
+}
+
+// Meca::computeForces(X, Y, with_rigidity) (meca.cc:143-147): Y = vBAS + (R) + (B + C) X
+template <int D>
+__global__ __launch_bounds__(kWave) void k_forces(FiberArgs a, const double* __restrict__ X,
+                                                  const double* __restrict__ bas, double* __restrict__ Y, int rig)
+{
+    __shared__ FiberLds<D> s;
+    const uint32_t f = blockIdx.x;
+    const uint32_t np = a.np[f], idx = a.idx[f], o = D * idx;
+    for (int i = threadIdx.x; i < (int)(D * np); i += kWave) s.x[i] = X[o + i];
+    __syncthreads();
+    fiber_forces<D>(a, s, X, np, idx, a.seg[f], a.rfrig[f], rig != 0, false);
+    __syncthreads();
+    for (int i = threadIdx.x; i < (int)(D * np); i += kWave) Y[o + i] = bas[o + i] + s.f[i];
+}
+
+void launch_forces(Context& c, const double* X, double* Y, bool with_rigidity)
+{
+    if (!c.nfib) return;
+    FiberArgs a = make_fiber_args(c);
+    const dim3 g(c.nfib), b(kWave);
+    if (c.dim == 3)
+        hipLaunchKernelGGL(k_forces<3>, g, b, 0, c.stream, a, X, c.d_bas.get(), Y, with_rigidity ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_forces<2>, g, b, 0, c.stream, a, X, c.d_bas.get(), Y, with_rigidity ? 1 : 0);
+    CYMG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// post-solve: vPTS += vSOL (meca.cc:802)
+__global__ void k_axpy(uint32_t n, const double* __restrict__ x, double* __restrict__ y)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = y[i] + x[i];
+}
+
+// FiberNaked::getPoints (fiber_naked.cc:414-425): reshape_two (172-193),
+// reshape_it (221-354: Newton on the segment lengths, dgtsv solve, stop at
+// sum|dsca| <= 0.001 or after 32 iterations), reshape_sure (367-401) on failure.
+template <int D>
+__global__ __launch_bounds__(kWave) void k_reshape(const uint32_t* __restrict__ npv, const uint32_t* __restrict__ idxv,
+                                                   const double* __restrict__ cutv, const double* __restrict__ src,
+                                                   double* __restrict__ dst)
+{
+    __shared__ double dif[3 * kMaxFiberPoints], vec[3 * kMaxFiberPoints];
+    __shared__ double sca[kMaxFiberPoints], val[kMaxFiberPoints], dia[kMaxFiberPoints];
+    __shared__ double low[kMaxFiberPoints], upe[kMaxFiberPoints];
+    __shared__ int fail;
+    const uint32_t f = blockIdx.x;
+    const uint32_t np = npv[f], o = D * idxv[f];
+    const int ns = (int)np - 1;
+    const double cut = cutv[f];
+    const double* X = src + o;
+    double* Y = dst + o;
+    if (ns == 1) {
+        if (threadIdx.x == 0) {
+            double df[D], nn = 0.0;
+            for (int d = 0; d < D; ++d) { df[d] = X[D + d] - X[d]; nn += df[d] * df[d]; }
+            const double sc = 0.5 * (1 - cut / sqrt(nn));
+            for (int d = 0; d < D; ++d) {
+                const double v = df[d] * sc;
+                Y[d] = X[d] + v;
+                Y[D + d] = X[D + d] - v;
+            }
+        }
+        return;
+    }
+    for (int p = threadIdx.x; p < ns; p += kWave) {
+        for (int d = 0; d < 3; ++d) dif[3 * p + d] = d < D ? X[D * p + D + d] - X[D * p + d] : 0.0;
+        sca[p] = 0.0;
+    }
+    if (threadIdx.x == 0) fail = 0;
+    __syncthreads();
+    const double a2 = cut * cut;
+    int it = 0;
+    while (true) {
+        for (int p = threadIdx.x; p < ns; p += kWave) {
+            for (int d = 0; d < 3; ++d) {
+                double v;
+                if (p == 0) v = (1 - 2 * sca[0]) * dif[d] + sca[1] * dif[3 + d];
+                else if (p == ns - 1) v = sca[ns - 2] * dif[3 * (ns - 2) + d] + (1 - 2 * sca[ns - 1]) * dif[3 * (ns - 1) + d];
+                else v = sca[p - 1] * dif[3 * (p - 1) + d] + (1 - 2 * sca[p]) * dif[3 * p + d] + sca[p + 1] * dif[3 * (p + 1) + d];
+                vec[3 * p + d] = v;
+            }
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < ns; p += kWave) {
+            double vv = 0, vd = 0;
+            for (int d = 0; d < 3; ++d) {
+                vv += vec[3 * p + d] * vec[3 * p + d];
+                vd += vec[3 * p + d] * dif[3 * p + d];
+            }
+            val[p] = vv - a2;
+            dia[p] = -2 * vd;
+            if (p >= 1) {
+                double vl = 0, vu = 0;
+                for (int d = 0; d < 3; ++d) {
+                    vl += vec[3 * p + d] * dif[3 * (p - 1) + d];
+                    vu += vec[3 * (p - 1) + d] * dif[3 * p + d];
+                }
+                low[p] = vl;
+                upe[p - 1] = vu;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // dgtsv(ns, low+1, dia, upe, val): partial pivoting on the tridiagonal
+            double* dl = low + 1;
+            double* dd = dia;
+            double* du = upe;
+            double* bb = val;
+            int info = 0;
+            for (int i = 0; i + 2 < ns && !info; ++i) {
+                if (fabs(dd[i]) >= fabs(dl[i])) {
+                    if (dd[i] == 0.0) { info = i + 1; break; }
+                    const double fct = dl[i] / dd[i];
+                    dd[i + 1] -= fct * du[i];
+                    bb[i + 1] -= fct * bb[i];
+                    dl[i] = 0.0;
+                } else {
+                    const double fct = dd[i] / dl[i];
+                    dd[i] = dl[i];
+                    double t = dd[i + 1];
+                    dd[i + 1] = du[i] - fct * t;
+                    dl[i] = du[i + 1];
+                    du[i + 1] = -fct * dl[i];
+                    du[i] = t;
+                    t = bb[i];
+                    bb[i] = bb[i + 1];
+                    bb[i + 1] = t - fct * bb[i + 1];
+                }
+            }
+            if (!info) {
+                const int i = ns - 2;
+                if (fabs(dd[i]) >= fabs(dl[i])) {
+                    if (dd[i] == 0.0) info = i + 1;
+                    else {
+                        const double fct = dl[i] / dd[i];
+                        dd[i + 1] -= fct * du[i];
+                        bb[i + 1] -= fct * bb[i];
+                    }
+                } else {
+                    const double fct = dd[i] / dl[i];
+                    dd[i] = dl[i];
+                    double t = dd[i + 1];
+                    dd[i + 1] = du[i] - fct * t;
+                    du[i] = t;
+                    t = bb[i];
+                    bb[i] = bb[i + 1];
+                    bb[i + 1] = t - fct * bb[i + 1];
+                }
+            }
+            if (!info && dd[ns - 1] == 0.0) info = ns;
+            if (!info) {
+                bb[ns - 1] /= dd[ns - 1];
+                bb[ns - 2] = (bb[ns - 2] - du[ns - 2] * bb[ns - 1]) / dd[ns - 2];
+                for (int i = ns - 3; i >= 0; --i) bb[i] = (bb[i] - du[i] * bb[i + 1] - dl[i] * bb[i + 2]) / dd[i];
+                double err = 0;
+                for (int p = 0; p < ns; ++p) {
+                    sca[p] += -0.5 * bb[p];
+                    err += fabs(bb[p]);
+                }
+                ++it;
+                if (it > 32) fail = 1;
+                else if (!(err > 0.001)) fail = -1;   // converged
+            } else {
+                fail = 1;
+            }
+        }
+        __syncthreads();
+        if (fail) break;
+    }
+    if (fail < 0) {
+        if (threadIdx.x == 0) {
+            double e[3], dd[3] = { 0, 0, 0 };
+            for (int d = 0; d < 3; ++d) e[d] = sca[0] * dif[d];
+            for (int d = 0; d < D; ++d) Y[d] = X[d] + e[d];
+            for (int p = 1; p < ns; ++p) {
+                for (int d = 0; d < 3; ++d) dd[d] = sca[p] * dif[3 * p + d];
+                for (int d = 0; d < D; ++d) Y[D * p + d] = X[D * p + d] + dd[d] - e[d];
+                for (int d = 0; d < 3; ++d) e[d] = dd[d];
+            }
+            for (int d = 0; d < D; ++d) Y[D * ns + d] = X[D * ns + d] - dd[d];
+        }
+        return;
+    }
+    // reshape_sure on the unmodified points
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < D * (ns + 1); ++i) Y[i] = X[i];
+        double dp[3] = { 0, 0, 0 }, sum[3] = { 0, 0, 0 }, sg[3];
+        double dis = 0;
+        for (int d = 0; d < D; ++d) { sg[d] = Y[D + d] - Y[d]; dis += sg[d] * sg[d]; }
+        dis = sqrt(dis);
+        if (dis > kRealEpsilon)
+            for (int d = 0; d < D; ++d) dp[d] = (cut / dis - 1.0) * sg[d];
+        for (int p = 1; p < ns; ++p) {
+            dis = 0;
+            for (int d = 0; d < D; ++d) { sg[d] = Y[D * p + D + d] - Y[D * p + d]; dis += sg[d] * sg[d]; }
+            dis = sqrt(dis);
+            for (int d = 0; d < D; ++d) { Y[D * p + d] += dp[d]; sum[d] += dp[d]; }
+            if (dis > kRealEpsilon)
+                for (int d = 0; d < D; ++d) dp[d] += (cut / dis - 1.0) * sg[d];
+        }
+        for (int d = 0; d < D; ++d) Y[D * ns + d] += dp[d];
+        for (int d = 0; d < D; ++d) sum[d] = (sum[d] + dp[d]) * (-1.0 / (ns + 1));
+        for (int p = 0; p <= ns; ++p)
+            for (int d = 0; d < D; ++d) Y[D * p + d] += sum[d];
+    }
+}
+
+void launch_apply(Context& c)
+{
+    if (!c.nfib) return;
+    const uint32_t n = c.n();
+    hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.d_sol.get(), c.d_pts.get());
+    CYMG_HIP(hipGetLastError());
+    launch_forces(c, c.d_pts.get(), c.d_for.get(), false);   // meca.cc:826
+    const dim3 g(c.nfib), b(kWave);
+    if (c.dim == 3)
+        hipLaunchKernelGGL(k_reshape<3>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_cut.get(), c.d_pts.get(),
+                           c.d_pos.get());
+    else
+        hipLaunchKernelGGL(k_reshape<2>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_cut.get(), c.d_pts.get(),
+                           c.d_pos.get());
+    CYMG_HIP(hipGetLastError());
+}
+
+// RigidFiber::computeTensions (rigid_fiber_project.cc:153-161) on vFOR
+template <int D>
+__global__ __launch_bounds__(kWave) void k_tensions(FiberArgs a, const double* __restrict__ F, double* __restrict__ lag)
+{
+    __shared__ FiberLds<D> s;
+    const uint32_t f = blockIdx.x;
+    const uint32_t np = a.np[f], o = D * a.idx[f], seg = a.seg[f];
+    const int S = (int)np - 1;
+    for (int i = threadIdx.x; i < (int)(D * np); i += kWave) s.f[i] = F[o + i];
+    load_factors<D>(a, s, seg, S);
+    __syncthreads();
+    for (int q = threadIdx.x; q < S; q += kWave) {
+        double t = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) t += a.diff[D * (seg + q) + d] * (s.f[D * (q + 1) + d] - s.f[D * q + d]);
+        s.t[q] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) ptts2_serial(S, s.d, s.e, s.t);
+    __syncthreads();
+    for (int q = threadIdx.x; q < S; q += kWave) lag[seg + q] = s.t[q];
+}
+
+void launch_tensions(Context& c)
+{
+    if (!c.nfib) return;
+    FiberArgs a = make_fiber_args(c);
+    const dim3 g(c.nfib), b(kWave);
+    if (c.dim == 3)
+        hipLaunchKernelGGL(k_tensions<3>, g, b, 0, c.stream, a, c.d_for.get(), c.d_lag.get());
+    else
+        hipLaunchKernelGGL(k_tensions<2>, g, b, 0, c.stream, a, c.d_for.get(), c.d_lag.get());
+    CYMG_HIP(hipGetLastError());
+}
+
+}  // namespace cymg

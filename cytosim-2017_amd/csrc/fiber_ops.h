@@ -1,0 +1,253 @@
+// Device-side per-fiber operators of the implicit step.  One workgroup of one
+// wavefront handles one fiber; lane p owns point p (p, p+64 for fibers longer
+// than a wavefront).  The fiber's coordinates, forces and Lagrange multipliers
+// are staged in LDS; the coupling rows of mB/mC are gathered from HBM.
+#pragma once
+
+#include "common.h"
+
+namespace cymg {
+
+// Everything a per-fiber kernel needs, passed by value.
+struct FiberArgs {
+    const uint32_t* np;       // points per fiber
+    const uint32_t* idx;      // matIndex
+    const uint32_t* seg;      // first segment
+    const uint32_t* fib_sys;  // system of the fiber
+    const uint32_t* sys_begin;// first fiber of each system (nsys+1)
+    const double* mob;        // rfMobility
+    const double* rfrig;      // rfRigidity = rigidity / fnCut^3
+    const double* cut;        // fnCut
+    const double* diff;       // rfDiff, DIM per segment
+    const double* jd;         // pttrf diagonal of JJ'
+    const double* je;         // pttrf subdiagonal of L
+    const double* pdiff;      // mtJJtiJforce (P' coefficients)
+    // full symmetric CSR of mB (rows = points) and mC (rows = dofs)
+    const uint32_t* Brow; const uint32_t* Bcol; const double* Bval; int Buse;
+    const uint32_t* Crow; const uint32_t* Ccol; const double* Cval; int Cuse;
+    double dt;
+};
+
+template <int D>
+struct FiberLds {
+    double x[kMaxFiberPoints * D];     // fiber coordinates of the operand
+    double f[kMaxFiberPoints * D];     // forces
+    double y[kMaxFiberPoints * D];     // result
+    double t[kMaxFiberPoints];         // J*f, then multipliers
+    double d[kMaxFiberPoints];         // pttrf D
+    double e[kMaxFiberPoints];         // pttrf L subdiagonal
+    double red[kWave];
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ __forceinline__ double nan_max(double a, double b)
+{
+    // maximum that propagates NaN
+    if (a != a || b != b) return __builtin_nan("");
+    return a > b ? a : b;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = nan_max(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// dptts2 with one right-hand side (LAPACK): solve L*D*L' x = b, sequential
+__device__ __forceinline__ void ptts2_serial(int n, const double* d, const double* e, double* b)
+{
+    if (n <= 1) {
+        if (n == 1) b[0] *= 1.0 / d[0];
+        return;
+    }
+    double prev = b[0];
+    for (int i = 1; i < n; ++i) {
+        prev = b[i] - prev * e[i - 1];
+        b[i] = prev;
+    }
+    double nxt = b[n - 1] / d[n - 1];
+    b[n - 1] = nxt;
+    for (int i = n - 2; i >= 0; --i) {
+        nxt = b[i] / d[i] - nxt * e[i];
+        b[i] = nxt;
+    }
+}
+
+// Load the fiber's pttrf factors into LDS.
+template <int D>
+__device__ __forceinline__ void load_factors(const FiberArgs& a, FiberLds<D>& s, uint32_t seg, int S)
+{
+    for (int k = threadIdx.x; k < S; k += kWave) {
+        s.d[k] = a.jd[seg + k];
+        s.e[k] = (k + 1 < S) ? a.je[seg + k] : 0.0;
+    }
+}
+
+// f_p = [R X]_p + [B X]_p + [C X]_p + [P' X]_p for the points of the fiber.
+// X: global operand (coupling gather); s.x must hold the fiber's slice of X.
+template <int D>
+__device__ __forceinline__ void fiber_forces(const FiberArgs& a, FiberLds<D>& s, const double* __restrict__ X,
+                                             uint32_t np, uint32_t idx, uint32_t seg, double k,
+                                             bool rigidity, bool with_pdiff)
+{
+    const int S = (int)np - 1;
+    for (int p = threadIdx.x; p < (int)np; p += kWave) {
+        double fp[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) fp[d] = 0.0;
+        const uint32_t r = idx + p;
+        if (a.Buse) {
+            const uint32_t e1 = a.Brow[r + 1];
+            for (uint32_t q = a.Brow[r]; q < e1; ++q) {
+                const double v = a.Bval[q];
+                const uint32_t c = a.Bcol[q];
+#pragma unroll
+                for (int d = 0; d < D; ++d) fp[d] += v * X[D * c + d];
+            }
+        }
+        if (a.Cuse) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const uint32_t rr = D * r + d;
+                const uint32_t e1 = a.Crow[rr + 1];
+                double acc = 0.0;
+                for (uint32_t q = a.Crow[rr]; q < e1; ++q) acc += a.Cval[q] * X[a.Ccol[q]];
+                fp[d] += acc;
+            }
+        }
+        if (rigidity && np > 2) {
+            // bending stencil v_j = k (x_j - 2 x_{j+1} + x_{j+2}); Y_j -= v, Y_{j+1} += 2v, Y_{j+2} -= v
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                double acc = 0.0;
+                if (p <= (int)np - 3) {
+                    const double v = k * (s.x[D * p + d] - 2 * s.x[D * (p + 1) + d] + s.x[D * (p + 2) + d]);
+                    acc -= v;
+                }
+                if (p >= 1 && p <= (int)np - 2) {
+                    const double v = k * (s.x[D * (p - 1) + d] - 2 * s.x[D * p + d] + s.x[D * (p + 1) + d]);
+                    acc += v + v;
+                }
+                if (p >= 2) {
+                    const double v = k * (s.x[D * (p - 2) + d] - 2 * s.x[D * (p - 1) + d] + s.x[D * p + d]);
+                    acc -= v;
+                }
+                fp[d] += acc;
+            }
+        }
+        if (with_pdiff) {
+            // P': w = c_s (X_{s+1} - X_s); Y_s += w; Y_{s+1} -= w
+            if (p < S) {
+                const double c = a.pdiff[seg + p];
+                if (c != 0.0)
+#pragma unroll
+                    for (int d = 0; d < D; ++d) fp[d] += c * (s.x[D * (p + 1) + d] - s.x[D * p + d]);
+            }
+            if (p >= 1) {
+                const double c = a.pdiff[seg + p - 1];
+                if (c != 0.0)
+#pragma unroll
+                    for (int d = 0; d < D; ++d) fp[d] -= c * (s.x[D * p + d] - s.x[D * (p - 1) + d]);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) s.f[D * p + d] = fp[d];
+    }
+}
+
+// s.y = sca * P s.f   with P = I - J'(JJ')^-1 J; the multipliers (JJ')^-1 J f
+// are left in s.t (rigid_fiber_project.cc:95-149).  s.d/s.e hold the factors.
+template <int D>
+__device__ __forceinline__ void fiber_project(const FiberArgs& a, FiberLds<D>& s, uint32_t np, uint32_t seg,
+                                              double sca)
+{
+    const int S = (int)np - 1;
+    for (int q = threadIdx.x; q < S; q += kWave) {
+        double t = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) t += a.diff[D * (seg + q) + d] * (s.f[D * (q + 1) + d] - s.f[D * q + d]);
+        s.t[q] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) ptts2_serial(S, s.d, s.e, s.t);
+    __syncthreads();
+    for (int p = threadIdx.x; p < (int)np; p += kWave) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            double v = s.f[D * p + d];
+            if (p < S) v = v + a.diff[D * (seg + p) + d] * s.t[p];
+            if (p >= 1) v = v - a.diff[D * (seg + p - 1) + d] * s.t[p - 1];
+            s.y[D * p + d] = sca * v;
+        }
+    }
+}
+
+// s.y = X - dt*mob*P (R + B + C + P') X  (Meca::multiply, meca.cc:157-181),
+// for one fiber; s.x must hold the fiber's slice of X.
+template <int D>
+__device__ __forceinline__ void fiber_operator(const FiberArgs& a, FiberLds<D>& s, const double* __restrict__ X,
+                                               uint32_t f)
+{
+    const uint32_t np = a.np[f], idx = a.idx[f], seg = a.seg[f];
+    fiber_forces<D>(a, s, X, np, idx, seg, a.rfrig[f], true, true);
+    __syncthreads();
+    fiber_project<D>(a, s, np, seg, -a.dt * a.mob[f]);
+    __syncthreads();
+    for (int i = threadIdx.x; i < (int)(D * np); i += kWave) s.y[i] = s.y[i] + s.x[i];
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// per-system reductions: partials per fiber, the last workgroup of a system
+// reduces them in fiber order (deterministic)
+
+__device__ __forceinline__ bool last_of_system(unsigned* counter, uint32_t sys, uint32_t nblocks)
+{
+    __shared__ int last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned prev = atomicAdd(&counter[sys], 1u);
+        last = (prev == nblocks - 1) ? 1 : 0;
+        if (last) counter[sys] = 0;
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last != 0;
+}
+
+// partials written by other workgroups are read with device-scope atomic
+// loads, so no XCD-local L2 line can serve a stale value
+__device__ __forceinline__ double load_partial(const double* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double system_sum(const double* part, uint32_t b, uint32_t e)
+{
+    double v = 0.0;
+    for (uint32_t i = b + threadIdx.x; i < e; i += kWave) v += load_partial(part + i);
+    return wave_sum(v);
+}
+
+__device__ __forceinline__ double system_max(const double* part, uint32_t b, uint32_t e)
+{
+    double v = 0.0;
+    for (uint32_t i = b + threadIdx.x; i < e; i += kWave) v = nan_max(v, load_partial(part + i));
+    return wave_max(v);
+}
+
+__device__ __forceinline__ double system_min(const double* part, uint32_t b, uint32_t e)
+{
+    double v = __builtin_inf();
+    for (uint32_t i = b + threadIdx.x; i < e; i += kWave) v = fmin(v, load_partial(part + i));
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+}  // namespace cymg

@@ -1,0 +1,192 @@
+/*
+ * cymeca.h -- C ABI of the MI355X implicit mechanics step (a drop-in for the
+ * reference's `Meca`).
+ *
+ * One context per host thread; all functions return 0 (CYMG_OK) or an error
+ * code and never throw; cymg_last_error() gives the message.  Host pointers
+ * are borrowed for the duration of a call; device buffers belong to the
+ * context.  Indices are matrix indices (`Mecable::matIndex() + point`, the
+ * reference's `PointExact::matIndex`).  Vectors are interleaved x,y(,z) per
+ * point, DIM*nbPts doubles, as in the reference (mecable.h:86-90).
+ *
+ * Each declaration names the reference interface it replaces
+ * (paths relative to the reference tree).
+ */
+#ifndef CYMECA_H
+#define CYMECA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CYMG_OK              0
+#define CYMG_ERR_INVALID     1   /* bad argument or call out of sequence */
+#define CYMG_ERR_HIP         2   /* HIP runtime error */
+#define CYMG_ERR_PROJECTION  3   /* "could not build Fiber projection" (rigid_fiber_project.cc:84-85) */
+#define CYMG_ERR_CONVERGENCE 4   /* "convergence failure in solver" (meca.cc:795) */
+#define CYMG_ERR_UNSUPPORTED 5   /* size beyond the kernels' stated limits */
+
+/* largest fiber handled by the per-fiber kernels (points) */
+#define CYMG_MAX_FIBER_POINTS 128
+
+typedef struct cymg_context cymg_context;
+
+/* SimulProp::time_step, kT, viscosity (simul_prop.h) and
+   FiberProp::hydrodynamic_radius[2] (fiber_prop.cc:198-199) */
+typedef struct {
+    double time_step;
+    double kT;
+    double viscosity;
+    double hydro_radius0;
+    double hydro_radius1;
+} cymg_sim_params;
+
+enum {
+    CYMG_TOL_REFERENCE = 0,   /* |r|_inf < tolerance*noiseLevel (meca.cc:731, bicgstab.cc:16,48) */
+    CYMG_TOL_RELATIVE  = 1    /* |r|_inf < tolerance*|b|_inf */
+};
+
+enum {
+    CYMG_PRECOND_NONE = 0,    /* Solver::BCGS (bicgstab.h:163-243) */
+    CYMG_PRECOND_FP64 = 1,    /* Solver::BCGSP with FP64 getrf+getri blocks (meca.cc:323-466) */
+    CYMG_PRECOND_FP32 = 2     /* Solver::BCGSP with blocks factorised and applied in FP32 */
+};
+
+enum {
+    CYMG_GUESS_ZERO     = 0,  /* vSOL = 0 (meca.cc:704) */
+    CYMG_GUESS_PREVIOUS = 1,  /* previous step's solution, kept per fiber */
+    CYMG_GUESS_LEARNED  = 2   /* initial guess from the trained per-fiber network */
+};
+
+typedef struct {
+    double tolerance;     /* SimulProp::tolerance */
+    int    tol_mode;      /* CYMG_TOL_* */
+    int    precondition;  /* CYMG_PRECOND_* */
+    int    max_iter;      /* <= 0: DIM*nbPts (meca.cc:731) */
+    int    guess;         /* CYMG_GUESS_* */
+    int    fallback;      /* nonzero: the reference's rescue chain on failure (meca.cc:749-799) */
+    int    gauss_on_device; /* nonzero: the gauss pointer given to setup/solve is device memory */
+} cymg_solve_params;
+
+typedef struct {
+    int    iterations;        /* Monitor::iterations() of the final attempt */
+    int    flag;              /* Monitor::flag(): 0, 2 (rho=0), 3 (omega=0), 4 (delta=0) */
+    int    converged;
+    int    fallback;          /* 0 none, 1 rescued by seed vRHS, 2 by switching method, 3 dead */
+    int    precond_used;      /* CYMG_PRECOND_* of the final attempt */
+    int    total_iterations;  /* over all attempts */
+    double residual;          /* Monitor::residual() */
+    double threshold;
+    double noise_level;       /* noiseLevel (meca.cc:633-643) */
+} cymg_solve_report;
+
+/* context ---------------------------------------------------------------- */
+int         cymg_create(int dim, int device, cymg_context** out);
+int         cymg_destroy(cymg_context* ctx);
+const char* cymg_last_error(const cymg_context* ctx);
+int         cymg_set_params(cymg_context* ctx, const cymg_sim_params* p);
+
+/* registration: Meca::clear (meca.cc:55-60), Meca::add (meca.cc:67-79) of a
+   RigidFiber (rigid_fiber.h) with its segmentation fnCut and FiberProp::rigidity */
+int         cymg_clear(cymg_context* ctx);
+int         cymg_add_fiber(cymg_context* ctx, uint32_t nb_points, double segmentation,
+                           double rigidity, uint32_t* mat_index);
+int         cymg_nb_points(const cymg_context* ctx, uint32_t* nb_points);
+
+/* Meca::prepare (meca.cc:488-537) with Fiber::prepareMecable (fiber.cc:450-460).
+   points: host array of the Mecables' points (PointSet::putPoints), or NULL to
+   reuse the points the device holds from the previous step's export. */
+int         cymg_prepare(cymg_context* ctx, const double* points);
+
+/* assembly, recorded in call order (meca_inter.cc); PointInterpolated is given
+   as (mat1, mat2, coef) with pos = (1-coef)*P1 + coef*P2 (point_interpolated.h:26-106) */
+int         cymg_add_pure_force(cymg_context* ctx, uint32_t a, const double* force);           /* meca_inter.cc:123-134 */
+int         cymg_inter_link_ee(cymg_context* ctx, uint32_t a, uint32_t b, double weight);      /* meca_inter.cc:196-230 */
+int         cymg_inter_link_ie(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca,
+                               uint32_t b, double weight);                                     /* meca_inter.cc:240-275 */
+int         cymg_inter_link_ii(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca,
+                               uint32_t b1, uint32_t b2, double cb, double weight);           /* meca_inter.cc:285-321 */
+int         cymg_inter_long_link_ee(cymg_context* ctx, uint32_t a, uint32_t b, double len,
+                                    double weight);                                            /* meca_inter.cc:334-415 */
+int         cymg_inter_clamp_e(cymg_context* ctx, uint32_t a, const double* g, double weight); /* meca_inter.cc:1883-1907 */
+int         cymg_inter_clamp_i(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca,
+                               const double* g, double weight);                                /* meca_inter.cc:1918-1963 */
+int         cymg_inter_long_clamp(cymg_context* ctx, uint32_t a, const double* center,
+                                  double len, double weight);                                 /* meca_inter.cc:1977-2022 */
+int         cymg_inter_plane_e(cymg_context* ctx, uint32_t a, const double* dir,
+                               const double* g, double weight);                               /* meca_inter.cc:2301-2326 */
+
+/* device-side emitters, placed in the call order like the calls above.
+   Fiber::setInteractions with CONFINE_INSIDE in a SpaceSphere centred at the
+   origin, for every registered fiber (fiber.cc:464-481, space_sphere.cc:113-116,148-151) */
+int         cymg_confine_sphere(cymg_context* ctx, double radius, double stiffness);
+/* Couple::setInteractions -> interLink(PI, PI) (couple.cc:87-92) for a list of
+   doubly-attached couples; hands are (fiber index in registration order,
+   abscissa from the minus end), interpolated as FiberNaked::interpolate
+   (fiber_naked.cc:1050-1065).  The list is uploaded by set_couples and
+   emitted on the device by couples_interactions. */
+int         cymg_set_couples(cymg_context* ctx, uint32_t n, const uint32_t* fiber_a,
+                             const double* abscissa_a, const uint32_t* fiber_b,
+                             const double* abscissa_b, const double* stiffness);
+int         cymg_couples_interactions(cymg_context* ctx);
+
+/* Meca::solve (meca.cc:598-845) and its phases:
+   setup   (meca.cc:605-693): matrices, forces, Brownian terms, RHS, P'
+   iterate (meca.cc:704-799): preconditioner, BCGSP/BCGS, rescue chain
+   apply   (meca.cc:802-835): vPTS += vSOL, forces, getPoints (reshape)
+   gauss: N(0,1) per degree of freedom in object order (the draws of
+   RigidFiber::addBrownianForces, rigid_fiber.cc:90-98), host or device memory
+   per params->gauss_on_device; NULL means no Brownian term. */
+int         cymg_solve(cymg_context* ctx, const cymg_solve_params* params, const double* gauss,
+                       cymg_solve_report* report);
+int         cymg_setup(cymg_context* ctx, const double* gauss, int gauss_on_device);
+int         cymg_iterate(cymg_context* ctx, const cymg_solve_params* params,
+                         cymg_solve_report* report);
+int         cymg_apply(cymg_context* ctx);
+
+/* operators, valid after setup (host arrays of DIM*nbPts) */
+int         cymg_multiply(cymg_context* ctx, const double* X, double* Y);       /* Meca::multiply, meca.cc:157-181 */
+int         cymg_compute_preconditioner(cymg_context* ctx, int precision);     /* Meca::computePreconditionner, meca.cc:411-444 */
+int         cymg_precondition(cymg_context* ctx, const double* X, double* Y);   /* Meca::precondition, meca.cc:448-466 */
+int         cymg_compute_forces(cymg_context* ctx);                             /* Meca::computeForces(), meca.cc:550-580 */
+/* same on device pointers, on the context's stream */
+int         cymg_multiply_device(cymg_context* ctx, const double* dX, double* dY);
+int         cymg_precondition_device(cymg_context* ctx, const double* dX, double* dY);
+
+/* export */
+enum { CYMG_VEC_PTS = 0, CYMG_VEC_SOL, CYMG_VEC_BAS, CYMG_VEC_RHS, CYMG_VEC_FOR };
+int         cymg_get_vector(cymg_context* ctx, int which, double* out);  /* vPTS..vFOR (meca.h:101-106) */
+int         cymg_get_points(cymg_context* ctx, double* out);   /* Mecable points after getPoints (fiber_naked.cc:414-425) */
+int         cymg_get_tensions(cymg_context* ctx, double* out); /* RigidFiber::tension, one per segment (rigid_fiber.h:90) */
+int         cymg_get_block(cymg_context* ctx, uint32_t mecable, int inverse, double* out); /* column-major bs*bs */
+/* mB (which=0) or mC (which=1) as sorted unique lower-triangle entries (row>=col),
+   column-major order as MatrixSparseSymmetric1 stores them (matsparsesym1.cc:137-224).
+   Returns the count in *nnz; arrays may be NULL to query it. */
+int         cymg_get_matrix(cymg_context* ctx, int which, uint32_t* nnz, uint32_t* rows,
+                            uint32_t* cols, double* vals);
+
+/* device timings of the last solve, milliseconds */
+typedef struct {
+    double assemble_ms;     /* stream sort/reduce, CSR build */
+    double setup_ms;        /* forces, Brownian, RHS, P' */
+    double precond_ms;      /* block build + factorisation */
+    double iterate_ms;      /* BiCGStab iterations */
+    double apply_ms;        /* update + reshape */
+    double matvec_avg_ms;   /* average duration of one fused operator launch */
+    double pcapply_avg_ms;  /* average duration of one preconditioner apply launch */
+    int    matvec_count;
+} cymg_timings;
+int         cymg_get_timings(cymg_context* ctx, cymg_timings* out);
+int         cymg_enable_timing(cymg_context* ctx, int enable);
+/* average duration (ms) of one launch of a hot kernel, timed with HIP events
+   around `reps` back-to-back launches on the context's stream, on the current
+   system state: which = 0 fused operator (Meca::multiply), 1 block apply
+   (Meca::precondition), 2 block build + factorisation (FP64), 3 the same in FP32 */
+int         cymg_time_kernel(cymg_context* ctx, int which, int reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,181 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Bars: assembly indices and values bit-exact; operators within 1e-12 relative
+(FP64, different summation order); fiber positions after one implicit step
+within 1e-9 relative in FP64 mode at a tight solver tolerance; the FP32
+preconditioner within 1e-6 relative of the FP64 oracle at relative residual 1e-8.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_from_scene, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def nr_entries(orc, which):
+    """Unique lower entries (row, col, val) of the oracle's NR layout
+    (matsparsesym1.cc:472-523), in column-major order."""
+    ija, sa = orc.matrix_nr(which)
+    n = int(ija[0]) - 1
+    rows, cols, vals = [], [], []
+    for j in range(n):
+        b, e = int(ija[j]), int(ija[j + 1])
+        touched = (e > b) or sa[j] != 0.0
+        # diagonal of a touched column always exists in the reference
+        rows.append(j); cols.append(j); vals.append(sa[j])
+        for k in range(b, e):
+            rows.append(int(ija[k])); cols.append(j); vals.append(sa[k])
+    return np.array(rows), np.array(cols), np.array(vals)
+
+
+def gpu_from_scene(pkg, sc, couples_on_device=True):
+    m = pkg.Meca(sc.dim, 0)
+    pkg.load_scene(m, sc, couples_on_device=couples_on_device)
+    return m
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("device_couples", [True, False])
+def test_assembly_bit_exact(pkg, scenes, oracle, dim, device_couples):
+    sc = scenes.mini(dim=dim, seed=11, nfib=8, nlinks=30)
+    gpu = gpu_from_scene(pkg, sc, device_couples)
+    orc = oracle_from_scene(oracle, sc)
+    gauss = sc.gauss(0)
+    gpu.setup(gauss)
+    orc.setup(gauss)
+    for which in (0, 1):
+        r, c, v = gpu.matrix_entries(which)
+        rr, cc, vv = nr_entries(orc, which)
+        # columns the reference never touched carry a zero diagonal in sa: drop them
+        keep = np.ones(rr.size, bool)
+        touched_cols = set(c.tolist())
+        for i in range(rr.size):
+            if rr[i] == cc[i] and cc[i] not in touched_cols and vv[i] == 0.0:
+                keep[i] = False
+        rr, cc, vv = rr[keep], cc[keep], vv[keep]
+        o1 = np.lexsort((r, c))
+        o2 = np.lexsort((rr, cc))
+        assert np.array_equal(r[o1], rr[o2]) and np.array_equal(c[o1], cc[o2])
+        assert np.array_equal(v[o1], vv[o2]), "matrix values must be bit-exact"
+    assert np.array_equal(gpu.vector(pkg.VEC_BAS), orc.vector(oracle.BAS))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_setup_vectors(pkg, scenes, oracle, dim):
+    sc = scenes.mini(dim=dim, seed=12, nfib=8, nlinks=25)
+    gpu = gpu_from_scene(pkg, sc)
+    orc = oracle_from_scene(oracle, sc)
+    g = sc.gauss(0)
+    gpu.setup(g)
+    orc.setup(g)
+    assert rel_err(gpu.vector(pkg.VEC_FOR), orc.vector(oracle.FOR)) < 1e-12
+    assert rel_err(gpu.vector(pkg.VEC_RHS), orc.vector(oracle.RHS)) < 1e-12
+    assert rel_err(gpu.vector(pkg.VEC_PTS), orc.vector(oracle.PTS)) == 0.0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_multiply(pkg, scenes, oracle, dim):
+    sc = scenes.mini(dim=dim, seed=13, nfib=10, nlinks=40)
+    gpu = gpu_from_scene(pkg, sc)
+    orc = oracle_from_scene(oracle, sc)
+    g = sc.gauss(0)
+    gpu.setup(g)
+    orc.setup(g)
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        x = rng.standard_normal(sc.size)
+        assert rel_err(gpu.multiply(x), orc.multiply(x)) < 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_blocks_and_precondition(pkg, scenes, oracle, dim):
+    sc = scenes.mini(dim=dim, seed=14, nfib=6, nlinks=20)
+    gpu = gpu_from_scene(pkg, sc)
+    orc = oracle_from_scene(oracle, sc)
+    g = sc.gauss(0)
+    gpu.setup(g)
+    orc.setup(g)
+    orc.compute_preconditioner()
+    gpu.compute_preconditioner(pkg.PRECOND_FP64)
+    for f in range(sc.nfib):
+        B = gpu.get_block(f)
+        assert rel_err(B, orc.get_block(f)) < 1e-12
+        Bi = gpu.get_block(f, inverse=True)
+        assert rel_err(Bi, orc.get_inverse_block(f)) < 1e-10
+        assert np.abs(Bi @ B - np.eye(B.shape[0])).max() < 1e-10
+    x = np.random.default_rng(4).standard_normal(sc.size)
+    assert rel_err(gpu.precondition(x), orc.precondition(x)) < 1e-10
+
+
+def _step_pair(pkg, oracle, sc, steps, tol, tol_mode, precondition=1):
+    gpu = gpu_from_scene(pkg, sc)
+    orc = oracle_from_scene(oracle, sc)
+    reps = []
+    for s in range(steps):
+        g = sc.gauss(s)
+        r1 = gpu.solve(tol, g, tol_mode=tol_mode, precondition=precondition)
+        r2 = orc.solve(tol, g, tol_mode=tol_mode)
+        reps.append((r1, r2))
+        if s + 1 < steps:
+            gpu.prepare(None)
+            pkg.assemble(gpu, sc, upload_couples=False)
+            orc.prepare(orc.fiber_points())
+            from conftest import oracle_assemble
+            oracle_assemble(orc, sc)
+    return gpu, orc, reps
+
+
+def test_config1_one_iteration_and_positions(pkg, scenes, oracle):
+    """Config 1 (2D, 20 uncoupled fibers): block preconditioner is exact, so
+    BCGSP takes one iteration per step as the reference does; positions after
+    3 steps match the oracle to 1e-9 relative."""
+    sc = scenes.config1(seed=1)
+    gpu, orc, reps = _step_pair(pkg, oracle, sc, 3, 0.05, 0)
+    for r1, r2 in reps:
+        assert r1["iterations"] == 1 and r2["iterations"] == 1
+    assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-9
+    assert rel_err(gpu.tensions(), orc.tensions()) < 1e-7
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_crosslinked_step_fp64(pkg, scenes, oracle, dim):
+    sc = scenes.mini(dim=dim, seed=15, nfib=12, nlinks=40, radius=2.0)
+    gpu, orc, reps = _step_pair(pkg, oracle, sc, 2, 1e-11, 1)
+    for r1, r2 in reps:
+        assert r1["converged"] and r2["converged"]
+    assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-9
+    assert rel_err(gpu.vector(pkg.VEC_FOR), orc.vector(oracle.FOR)) < 1e-7
+
+
+def test_config2_reduced_fp64(pkg, scenes, oracle):
+    """Config 2 shape (3D, lengths 2-15, crosslinks) at 150 fibers."""
+    sc = scenes.config2(seed=21, nfib=150, nlinks=600, radius=5.0)
+    gpu, orc, reps = _step_pair(pkg, oracle, sc, 1, 1e-11, 1)
+    assert reps[0][0]["converged"]
+    assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-9
+
+
+def test_fp32_preconditioner(pkg, scenes, oracle):
+    sc = scenes.config2(seed=22, nfib=150, nlinks=600, radius=5.0)
+    gpu, orc, reps = _step_pair(pkg, oracle, sc, 1, 1e-8, 1, precondition=pkg.PRECOND_FP32)
+    assert reps[0][0]["converged"] and reps[0][0]["precond_used"] == pkg.PRECOND_FP32
+    assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-6
+
+
+def test_unpreconditioned_bcgs(pkg, scenes, oracle):
+    sc = scenes.mini(dim=3, seed=16, nfib=6, nlinks=10)
+    gpu, orc, reps = _step_pair(pkg, oracle, sc, 1, 1e-10, 1, precondition=pkg.PRECOND_NONE)
+    assert reps[0][0]["converged"]
+    assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-8
+
+
+def test_convergence_failure_raises(pkg, scenes, oracle):
+    """Rescue chain (meca.cc:749-799) ending in the reference's exception."""
+    sc = scenes.config2(seed=23, nfib=60, nlinks=200, radius=4.0)
+    gpu = gpu_from_scene(pkg, sc)
+    with pytest.raises(pkg.ConvergenceFailure):
+        gpu.solve(1e-14, sc.gauss(0), tol_mode=1, max_iter=1)
+    orc = oracle_from_scene(oracle, sc)
+    with pytest.raises(RuntimeError):
+        orc.solve(1e-14, sc.gauss(0), tol_mode=1, max_iter=1)
