@@ -110,6 +110,8 @@ def main():
     m.compute_preconditioner(pkg.PRECOND_FP64)
     pc_ms = m.time_kernel(1, reps=200)
     build_ms = m.time_kernel(2, reps=5)
+    build_only_ms = m.time_kernel(4, reps=5)
+    build_lu_ms = m.time_kernel(5, reps=5)
     rB, cB, vB = m.matrix_entries(0)
     rC, cC, vC = m.matrix_entries(1)
     kb_full = 2 * int((rB != cB).sum()) + int((rB == cB).sum())
@@ -158,7 +160,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_matvec (fused Meca::multiply)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_launch": int(mv_bytes), "avg_launch_ms": mv_ms},
-        "kernels_ms": {"matvec": mv_ms, "precond_apply": pc_ms, "precond_build_fp64": build_ms},
+        "kernels_ms": {"matvec": mv_ms, "precond_apply": pc_ms, "precond_build_fp64": build_ms,
+                       "precond_assemble_only": build_only_ms, "precond_build_getrf_getri": build_lu_ms},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

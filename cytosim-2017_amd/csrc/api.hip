@@ -95,14 +95,12 @@ void upload_layout(Context& c)
         v->resize(n ? n : 1);
     const size_t S = c.nseg ? c.nseg : 1;
     for (DevBuf<double>* v : { &c.d_diff }) v->resize(S * c.dim);
-    for (DevBuf<double>* v : { &c.d_jd, &c.d_je, &c.d_lag, &c.d_pdiff }) v->resize(S);
+    for (DevBuf<double>* v : { &c.d_jd, &c.d_je, &c.d_jd0, &c.d_je0, &c.d_lag, &c.d_pdiff }) v->resize(S);
     CYMG_HIP(hipMemsetAsync(c.d_lag.get(), 0, S * 8, st));
     CYMG_HIP(hipMemsetAsync(c.d_prev_sol.get(), 0, (n ? n : 1) * 8, st));
-    c.d_part.resize(8 * (size_t)(F ? F : 1));
-    c.d_scal.resize(c.nsys);
-    c.d_counter.resize(c.nsys);
-    CYMG_HIP(hipMemsetAsync(c.d_counter.get(), 0, c.nsys * sizeof(unsigned), st));
-    CYMG_HIP(hipMemsetAsync(c.d_scal.get(), 0, c.nsys * sizeof(SysScalars), st));
+    c.d_part.resize(P_SLOTS * (size_t)(F ? F : 1));
+    c.d_scal.resize(2 * c.nsys);
+    CYMG_HIP(hipMemsetAsync(c.d_scal.get(), 0, 2 * c.nsys * sizeof(SysScalars), st));
     CYMG_HIP(hipStreamSynchronize(st));   // the sources above are pageable host arrays
     c.layout_dirty = false;
 }
@@ -240,7 +238,7 @@ int cymg_create(int dim, int device, cymg_context** out)
     ctx->c.device = device;
     int rc = guarded(ctx, [&](Context& c) {
         CYMG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-        CYMG_HIP(hipHostMalloc((void**)&c.h_scal, sizeof(SysScalars) * 64, hipHostMallocDefault));
+        CYMG_HIP(hipHostMalloc((void**)&c.h_scal, sizeof(SysScalars) * 4096, hipHostMallocDefault));
         for (int i = 0; i < 10; ++i) CYMG_HIP(hipEventCreate(&c.ev[i]));
     });
     if (rc) {
@@ -764,16 +762,24 @@ int cymg_enable_timing(cymg_context* ctx, int enable)
 int cymg_time_kernel(cymg_context* ctx, int which, int reps, double* avg_ms)
 {
     return guarded(ctx, [&](Context& c) {
-        if (!avg_ms || reps <= 0 || which < 0 || which > 3) throw Error(CYMG_ERR_INVALID, "invalid arguments");
+        if (!avg_ms || reps <= 0 || which < 0 || which > 5) throw Error(CYMG_ERR_INVALID, "invalid arguments");
         if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_time_kernel");
         if (which == 1 && !c.pc_ready) throw Error(CYMG_ERR_INVALID, "no preconditioner computed");
         const size_t n = c.n();
         c.k_v.resize(n); c.k_t.resize(n);
         CYMG_HIP(hipMemcpyAsync(c.k_v.get(), c.d_rhs.get(), n * 8, hipMemcpyDeviceToDevice, c.stream));
+        DevBuf<double> tmp;
+        if (which == 4) tmp.resize(c.blk_total);
+        const int method = c.pc_method;
         auto once = [&]() {
             if (which == 0) launch_matvec(c, c.k_v.get(), c.k_t.get());
             else if (which == 1) launch_precond_apply(c, c.k_v.get(), c.k_t.get());
-            else launch_precond_build(c, which == 2 ? CYMG_PRECOND_FP64 : CYMG_PRECOND_FP32);
+            else if (which == 4) launch_precond_build(c, CYMG_PRECOND_FP64, tmp.get());
+            else if (which == 5) {
+                c.pc_method = 1;
+                launch_precond_build(c, CYMG_PRECOND_FP64);
+                c.pc_method = method;
+            } else launch_precond_build(c, which == 2 ? CYMG_PRECOND_FP64 : CYMG_PRECOND_FP32);
         };
         once();
         CYMG_HIP(hipEventRecord(c.ev[8], c.stream));

@@ -20,6 +20,8 @@ inline FiberArgs make_fiber_args(const Context& c)
     a.diff = c.d_diff.get();
     a.jd = c.d_jd.get();
     a.je = c.d_je.get();
+    a.jd0 = c.d_jd0.get();
+    a.je0 = c.d_je0.get();
     a.pdiff = c.d_pdiff.get();
     a.Brow = c.mB.rowptr.get();
     a.Bcol = c.mB.col.get();

@@ -56,18 +56,21 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
-// Per-Krylov-system scalar state, one record per independent system.
+// Per-Krylov-system scalar state, one record per independent system; two
+// copies alternate between consecutive kernels (read one, write the other).
 struct alignas(16) SysScalars {
     double rho1, rho2, alpha, omega, beta;
     double tol;         // residual threshold
     double resid;       // last |r|_inf
     double resid_s;     // |s|_inf after the alpha update (breakdown reporting)
-    double rts, rtt;    // <rt,s>, <rt,t>
     double bnorm;       // |b|_inf
     double noise;       // min over Mecables of mobility*b (noiseLevel / time_step)
     int    iter, iter_max;
-    int    done, flag, pending, pad;
+    int    done, flag, pending, stage;
 };
+
+// slots of per-fiber partial reductions ([slot][fiber])
+enum { P_RR = 0, P_RMAX, P_DELTA, P_SMAX, P_TT, P_TS, P_RTS, P_RTT, P_NOISE, P_BNORM, P_SLOTS };
 
 // accumulation stream: (key, delta) pairs in call order
 struct HostStream {

@@ -29,7 +29,7 @@ struct Context {
 
     // vectors of DIM*nbPts (meca.h:101-106) and per-segment arrays
     DevBuf<double> d_pts, d_pos, d_sol, d_bas, d_rhs, d_for, d_tmp, d_prev_sol, d_gauss;
-    DevBuf<double> d_diff, d_jd, d_je, d_lag, d_pdiff;
+    DevBuf<double> d_diff, d_jd, d_je, d_jd0, d_je0, d_lag, d_pdiff;
     DevBuf<int> d_status;              // [0]: projection failures
 
     // assembly streams (mB, mC, vBAS) in call order
@@ -52,6 +52,7 @@ struct Context {
 
     // preconditioner
     int pc_precision = 0;              // CYMG_PRECOND_*
+    int pc_method = 0;                 // 0 Gauss-Jordan inverse, 1 getrf + getri
     DevBuf<double> d_binv;
     DevBuf<float> d_binv32;
     DevBuf<int> d_useblk;
@@ -60,8 +61,8 @@ struct Context {
     DevBuf<double> k_r, k_rt, k_p, k_v, k_t, k_ph, k_sh;
     DevBuf<double> d_part;             // partial reductions [slot][nfib]
     DevBuf<SysScalars> d_scal;
-    DevBuf<unsigned> d_counter;
     SysScalars* h_scal = nullptr;      // pinned mirror
+    int last_iters[3] = {0, 0, 0};     // iterations of the previous solve, per preconditioner
 
     bool prepared = false, setup_done = false, pc_ready = false, have_solution = false;
 

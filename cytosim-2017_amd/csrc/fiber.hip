@@ -19,6 +19,7 @@ __global__ __launch_bounds__(kWave) void k_prepare(const uint32_t* __restrict__ 
                                                    double* __restrict__ pts, double* __restrict__ mob,
                                                    double* __restrict__ rfrig, double* __restrict__ diff,
                                                    double* __restrict__ jd, double* __restrict__ je,
+                                                   double* __restrict__ jd0, double* __restrict__ je0,
                                                    int* __restrict__ status, double visc, double r0, double r1)
 {
     __shared__ double sdif[kMaxFiberPoints * D];
@@ -40,11 +41,13 @@ __global__ __launch_bounds__(kWave) void k_prepare(const uint32_t* __restrict__ 
 #pragma unroll
         for (int d = 0; d < D; ++d) dd += sdif[D * q + d] * sdif[D * q + d];
         sd[q] = 2 * dd;
+        jd0[seg + q] = 2 * dd;
         if (q + 1 < S) {
             double od = 0.0;
 #pragma unroll
             for (int d = 0; d < D; ++d) od += sdif[D * q + d] * sdif[D * q + D + d];
             se[q] = -od;
+            je0[seg + q] = -od;
         }
     }
     __syncthreads();
@@ -84,12 +87,12 @@ void launch_prepare(Context& c)
     if (c.dim == 3)
         hipLaunchKernelGGL(k_prepare<3>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_seg.get(), c.d_cut.get(),
                            c.d_rig.get(), c.d_pos.get(), c.d_pts.get(), c.d_mob.get(), c.d_rfrig.get(),
-                           c.d_diff.get(), c.d_jd.get(), c.d_je.get(), c.d_status.get(), c.prm.viscosity,
+                           c.d_diff.get(), c.d_jd.get(), c.d_je.get(), c.d_jd0.get(), c.d_je0.get(), c.d_status.get(), c.prm.viscosity,
                            c.prm.hydro_radius0, c.prm.hydro_radius1);
     else
         hipLaunchKernelGGL(k_prepare<2>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_seg.get(), c.d_cut.get(),
                            c.d_rig.get(), c.d_pos.get(), c.d_pts.get(), c.d_mob.get(), c.d_rfrig.get(),
-                           c.d_diff.get(), c.d_jd.get(), c.d_je.get(), c.d_status.get(), c.prm.viscosity,
+                           c.d_diff.get(), c.d_jd.get(), c.d_je.get(), c.d_jd0.get(), c.d_je0.get(), c.d_status.get(), c.prm.viscosity,
                            c.prm.hydro_radius0, c.prm.hydro_radius1);
     CYMG_HIP(hipGetLastError());
 }
@@ -257,14 +260,14 @@ void launch_emit_couples(Context& c, uint64_t* keyB, double* valB)
 // Brownian terms b*N(0,1) with b = sqrt(2 kT/dt / mobility) (rigid_fiber.cc:90-98),
 // vRHS = dt * mobility * P vFOR keeping the multipliers in rfLag
 // (rigid_fiber.cc:124-132), P' coefficients (rigid_fiber_project.cc:168-219).
-// Per-system reductions: noise = min(mobility*b), bnorm = |vRHS|_inf.
+// Per-fiber partials of noise = min(mobility*b) and bnorm = |vRHS|_inf; the
+// solver's first kernel reduces them per system.
 template <int D>
 __global__ __launch_bounds__(kWave) void k_setup(FiberArgs a, const double* __restrict__ pts,
                                                  const double* __restrict__ bas, const double* __restrict__ gauss,
                                                  double kT_dt, double* __restrict__ vfor, double* __restrict__ rhs,
                                                  double* __restrict__ lag, double* __restrict__ pdiff,
-                                                 double* __restrict__ part, unsigned* __restrict__ counter,
-                                                 SysScalars* __restrict__ scal, uint32_t nfib)
+                                                 double* __restrict__ part, uint32_t nfib)
 {
     __shared__ FiberLds<D> s;
     const uint32_t f = blockIdx.x;
@@ -300,18 +303,8 @@ __global__ __launch_bounds__(kWave) void k_setup(FiberArgs a, const double* __re
     }
     bmax = wave_max(bmax);
     if (threadIdx.x == 0) {
-        part[f] = mobf * b;
-        part[nfib + f] = bmax;
-    }
-    const uint32_t sys = a.fib_sys[f];
-    const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
-    if (last_of_system(counter, sys, fe - fb)) {
-        const double noise = system_min(part, fb, fe);
-        const double bn = system_max(part + nfib, fb, fe);
-        if (threadIdx.x == 0) {
-            scal[sys].noise = noise;
-            scal[sys].bnorm = bn;
-        }
+        part[P_NOISE * nfib + f] = mobf * b;
+        part[P_BNORM * nfib + f] = bmax;
     }
 }
 
@@ -324,11 +317,11 @@ void launch_setup(Context& c, const double* gauss)
     if (c.dim == 3)
         hipLaunchKernelGGL(k_setup<3>, g, b, 0, c.stream, a, c.d_pts.get(), c.d_bas.get(), gauss, kT_dt,
                            c.d_for.get(), c.d_rhs.get(), c.d_lag.get(), c.d_pdiff.get(), c.d_part.get(),
-                           c.d_counter.get(), c.d_scal.get(), c.nfib);
+                           c.nfib);
     else
         hipLaunchKernelGGL(k_setup<2>, g, b, 0, c.stream, a, c.d_pts.get(), c.d_bas.get(), gauss, kT_dt,
                            c.d_for.get(), c.d_rhs.get(), c.d_lag.get(), c.d_pdiff.get(), c.d_part.get(),
-                           c.d_counter.get(), c.d_scal.get(), c.nfib);
+                           c.nfib);
     CYMG_HIP(hipGetLastError());
 }
 
@@ -593,7 +586,7 @@ __global__ __launch_bounds__(kWave) void k_tensions(FiberArgs a, const double* _
     const uint32_t np = a.np[f], o = D * a.idx[f], seg = a.seg[f];
     const int S = (int)np - 1;
     for (int i = threadIdx.x; i < (int)(D * np); i += kWave) s.f[i] = F[o + i];
-    load_factors<D>(a, s, seg, S);
+    load_factors<D>(a, s, seg, S, true);
     __syncthreads();
     for (int q = threadIdx.x; q < S; q += kWave) {
         double t = 0.0;

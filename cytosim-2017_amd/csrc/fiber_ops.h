@@ -21,6 +21,8 @@ struct FiberArgs {
     const double* diff;       // rfDiff, DIM per segment
     const double* jd;         // pttrf diagonal of JJ'
     const double* je;         // pttrf subdiagonal of L
+    const double* jd0;        // JJ' diagonal 2|d_s|^2 (unfactored)
+    const double* je0;        // JJ' off-diagonal -d_s.d_{s+1} (unfactored)
     const double* pdiff;      // mtJJtiJforce (P' coefficients)
     // full symmetric CSR of mB (rows = points) and mC (rows = dofs)
     const uint32_t* Brow; const uint32_t* Bcol; const double* Bval; int Buse;
@@ -78,10 +80,38 @@ __device__ __forceinline__ void ptts2_serial(int n, const double* d, const doubl
     }
 }
 
-// Load the fiber's pttrf factors into LDS.
-template <int D>
-__device__ __forceinline__ void load_factors(const FiberArgs& a, FiberLds<D>& s, uint32_t seg, int S)
+// Parallel cyclic reduction of the symmetric tridiagonal system JJ' x = t
+// across one wavefront (lane = row, S <= 64): log2(S) steps of 8 shuffles.
+// The same system as dptts2 solves with the pttrf factors; rounding differs
+// at the ulp level.
+__device__ __forceinline__ double pcr_solve(int S, double a, double b, double c, double d)
 {
+    const int lane = threadIdx.x & (kWave - 1);
+    if (lane >= S) { a = 0.0; b = 1.0; c = 0.0; d = 0.0; }
+    for (int k = 1; k < S; k <<= 1) {
+        double al = __shfl_up(a, k, kWave), bl = __shfl_up(b, k, kWave);
+        double cl = __shfl_up(c, k, kWave), dl = __shfl_up(d, k, kWave);
+        double au = __shfl_down(a, k, kWave), bu = __shfl_down(b, k, kWave);
+        double cu = __shfl_down(c, k, kWave), du = __shfl_down(d, k, kWave);
+        if (lane < k) { al = 0.0; bl = 1.0; cl = 0.0; dl = 0.0; }
+        if (lane + k >= kWave) { au = 0.0; bu = 1.0; cu = 0.0; du = 0.0; }
+        const double alpha = -a / bl;
+        const double gamma = -c / bu;
+        b = b + alpha * cl + gamma * au;
+        d = d + alpha * dl + gamma * du;
+        a = alpha * al;
+        c = gamma * cu;
+    }
+    return d / b;
+}
+
+// Load the fiber's pttrf factors into LDS.
+// (only the serial path for fibers longer than a wavefront needs them, unless forced)
+template <int D>
+__device__ __forceinline__ void load_factors(const FiberArgs& a, FiberLds<D>& s, uint32_t seg, int S,
+                                             bool force = false)
+{
+    if (!force && S <= kWave) return;
     for (int k = threadIdx.x; k < S; k += kWave) {
         s.d[k] = a.jd[seg + k];
         s.e[k] = (k + 1 < S) ? a.je[seg + k] : 0.0;
@@ -101,6 +131,11 @@ __device__ __forceinline__ void fiber_forces(const FiberArgs& a, FiberLds<D>& s,
 #pragma unroll
         for (int d = 0; d < D; ++d) fp[d] = 0.0;
         const uint32_t r = idx + p;
+        uint32_t cb[D + 1];
+        if (a.Cuse) {
+#pragma unroll
+            for (int d = 0; d <= D; ++d) cb[d] = a.Crow[D * r + d];
+        }
         if (a.Buse) {
             const uint32_t e1 = a.Brow[r + 1];
             for (uint32_t q = a.Brow[r]; q < e1; ++q) {
@@ -113,10 +148,8 @@ __device__ __forceinline__ void fiber_forces(const FiberArgs& a, FiberLds<D>& s,
         if (a.Cuse) {
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                const uint32_t rr = D * r + d;
-                const uint32_t e1 = a.Crow[rr + 1];
                 double acc = 0.0;
-                for (uint32_t q = a.Crow[rr]; q < e1; ++q) acc += a.Cval[q] * X[a.Ccol[q]];
+                for (uint32_t q = cb[d]; q < cb[d + 1]; ++q) acc += a.Cval[q] * X[a.Ccol[q]];
                 fp[d] += acc;
             }
         }
@@ -167,14 +200,28 @@ __device__ __forceinline__ void fiber_project(const FiberArgs& a, FiberLds<D>& s
                                               double sca)
 {
     const int S = (int)np - 1;
-    for (int q = threadIdx.x; q < S; q += kWave) {
-        double t = 0.0;
+    if (S <= kWave) {
+        const int q = threadIdx.x;
+        double t = 0.0, sa = 0.0, sb = 1.0, sc = 0.0;
+        if (q < S) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) t += a.diff[D * (seg + q) + d] * (s.f[D * (q + 1) + d] - s.f[D * q + d]);
-        s.t[q] = t;
+            for (int d = 0; d < D; ++d) t += a.diff[D * (seg + q) + d] * (s.f[D * (q + 1) + d] - s.f[D * q + d]);
+            sb = a.jd0[seg + q];
+            if (q > 0) sa = a.je0[seg + q - 1];
+            if (q + 1 < S) sc = a.je0[seg + q];
+        }
+        const double lam = pcr_solve(S, sa, sb, sc, t);
+        if (q < S) s.t[q] = lam;
+    } else {
+        for (int q = threadIdx.x; q < S; q += kWave) {
+            double t = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) t += a.diff[D * (seg + q) + d] * (s.f[D * (q + 1) + d] - s.f[D * q + d]);
+            s.t[q] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) ptts2_serial(S, s.d, s.e, s.t);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) ptts2_serial(S, s.d, s.e, s.t);
     __syncthreads();
     for (int p = threadIdx.x; p < (int)np; p += kWave) {
 #pragma unroll
@@ -200,54 +247,6 @@ __device__ __forceinline__ void fiber_operator(const FiberArgs& a, FiberLds<D>& 
     __syncthreads();
     for (int i = threadIdx.x; i < (int)(D * np); i += kWave) s.y[i] = s.y[i] + s.x[i];
     __syncthreads();
-}
-
-// ---------------------------------------------------------------------------
-// per-system reductions: partials per fiber, the last workgroup of a system
-// reduces them in fiber order (deterministic)
-
-__device__ __forceinline__ bool last_of_system(unsigned* counter, uint32_t sys, uint32_t nblocks)
-{
-    __shared__ int last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned prev = atomicAdd(&counter[sys], 1u);
-        last = (prev == nblocks - 1) ? 1 : 0;
-        if (last) counter[sys] = 0;
-    }
-    __syncthreads();
-    if (last) __threadfence();
-    return last != 0;
-}
-
-// partials written by other workgroups are read with device-scope atomic
-// loads, so no XCD-local L2 line can serve a stale value
-__device__ __forceinline__ double load_partial(const double* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ double system_sum(const double* part, uint32_t b, uint32_t e)
-{
-    double v = 0.0;
-    for (uint32_t i = b + threadIdx.x; i < e; i += kWave) v += load_partial(part + i);
-    return wave_sum(v);
-}
-
-__device__ __forceinline__ double system_max(const double* part, uint32_t b, uint32_t e)
-{
-    double v = 0.0;
-    for (uint32_t i = b + threadIdx.x; i < e; i += kWave) v = nan_max(v, load_partial(part + i));
-    return wave_max(v);
-}
-
-__device__ __forceinline__ double system_min(const double* part, uint32_t b, uint32_t e)
-{
-    double v = __builtin_inf();
-    for (uint32_t i = b + threadIdx.x; i < e; i += kWave) v = fmin(v, load_partial(part + i));
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
-    return v;
 }
 
 }  // namespace cymg

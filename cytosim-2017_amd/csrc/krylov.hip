@@ -2,21 +2,32 @@
 // bicgstab.h:247-338; Solver::BCGS, bicgstab.h:163-243; Monitor,
 // bicgstab.h:51-92 and bicgstab.cc:9-49), entirely on the device.
 //
-// One iteration is four fused per-fiber launches:
-//   A: [r = s - w t; x += w shat] (previous iteration), |r|_inf, convergence
-//      test, p = r + beta (p - w v), phat = Binv p
-//   B: v = M phat, <rt, v>  -> alpha = rho / <rt, v>
-//   C: s = r - alpha v, x += alpha phat, |s|_inf, shat = Binv s
-//   D: t = M shat, <t,t>, <t,s>, <rt,s>, <rt,t>  -> omega, next rho, beta
-// The next rho is <rt, r_new> = <rt, s> - omega <rt, t>, which needs no extra
-// pass over r.  Every reduction is per system: partials per fiber, summed in
-// fiber order by the last workgroup of the system, so results do not depend on
-// workgroup scheduling.  Scalars and the stop flag stay in HBM; the host only
-// reads the flag between chunks of iterations.
+// After an init launch, one iteration is four fused per-fiber launches:
+//   A: omega, next rho, beta from D's partials; [x += w shat; r = s - w t];
+//      |r|_inf partials; p = r + beta (p - w v); phat = Binv p
+//   B: convergence test on A's partials; v = M phat; <rt, v> partials
+//   C: alpha = rho / <rt, v>; s = r - alpha v; x += alpha phat; |s|_inf
+//      partials; shat = Binv s
+//   D: t = M shat; <t,t>, <t,s>, <rt,s>, <rt,t> partials
+// The next rho is <rt, r_new> = <rt, s> - omega <rt, t>, so no extra pass over
+// r is needed.  No kernel synchronises with other workgroups: each workgroup
+// writes the partial sums of its fiber, and every workgroup of the next launch
+// re-reduces the partials of its system in fiber order, so all of them derive
+// bit-identical scalars without atomics or device-scope fences (kernel
+// boundaries order the memory).  The first workgroup of each system writes the
+// scalar state into the other half of a double buffer.  The host reads the
+// stop flag between chunks of iterations.
+#include <algorithm>
+
 #include "args.h"
 #include "pc_ops.h"
 
 namespace cymg {
+
+constexpr int kApplyWaves = 4;
+constexpr int kApplyThreads = kApplyWaves * kWave;
+
+enum { ST_INIT = 0, ST_A = 1, ST_B = 2, ST_C = 3, ST_D = 4 };
 
 struct KArgs {
     FiberArgs fa;
@@ -34,23 +45,80 @@ struct KArgs {
     double* ph;
     double* sh;
     double* part;
-    unsigned* counter;
-    SysScalars* scal;
+    double* part4;      // [fiber][4] partials of D
+    SysScalars* st_in;
+    SysScalars* st_out;
     uint32_t nfib;
 };
 
-enum { P_RR = 0, P_RMAX = 1, P_DELTA = 2, P_SMAX = 3, P_TT = 4, P_TS = 5, P_RTS = 6, P_RTT = 7, P_SLOTS = 8 };
-
-template <int PC>
-__device__ __forceinline__ void apply_block(const KArgs& k, uint32_t f, int bs, const double* xs, double* y)
+// Reductions over the partials of one system (fibers [fb, fe)), repeated by
+// every workgroup.  In a one-wave workgroup the xor-shuffle reduction leaves
+// the total in every lane; wider workgroups let wave 0 reduce and broadcast
+// through LDS behind one barrier (bc: a slot used once per launch).
+__device__ __forceinline__ double wsum(const double* part, uint32_t fb, uint32_t fe)
 {
-    if constexpr (PC == CYMG_PRECOND_FP64)
-        block_apply<double>(k.B64 + k.blkoff[f], bs, k.useblk[f] != 0, xs, xs, y);
-    else if constexpr (PC == CYMG_PRECOND_FP32)
-        block_apply<float>(k.B32 + k.blkoff[f], bs, k.useblk[f] != 0, xs, xs, y);
+    double v = 0.0;
+    for (uint32_t i = fb + (threadIdx.x & (kWave - 1)); i < fe; i += kWave) v += part[i];
+    return wave_sum(v);
 }
 
-// r = b - M x0 (or b), rt = r, x = x0 (or 0); rho = <rt, r>
+__device__ __forceinline__ double wmax(const double* part, uint32_t fb, uint32_t fe)
+{
+    double v = 0.0;
+    for (uint32_t i = fb + (threadIdx.x & (kWave - 1)); i < fe; i += kWave) v = nan_max(v, part[i]);
+    return wave_max(v);
+}
+
+__device__ __forceinline__ double wmin(const double* part, uint32_t fb, uint32_t fe)
+{
+    double v = __builtin_inf();
+    for (uint32_t i = fb + (threadIdx.x & (kWave - 1)); i < fe; i += kWave) v = fmin(v, part[i]);
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// the 4 partials of D, stored together per fiber ([fiber][4])
+__device__ __forceinline__ void wsum4(const double* part4, uint32_t fb, uint32_t fe, double out[4])
+{
+    double v[4] = { 0, 0, 0, 0 };
+    for (uint32_t i = fb + (threadIdx.x & (kWave - 1)); i < fe; i += kWave) {
+        const double4 q = *reinterpret_cast<const double4*>(part4 + 4 * (size_t)i);
+        v[0] += q.x; v[1] += q.y; v[2] += q.z; v[3] += q.w;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[q] = wave_sum(v[q]);
+}
+
+// Monitor::finished on |r|_inf (bicgstab.cc:38-48), then the rho = 0 breakdown
+__device__ __forceinline__ void decide(SysScalars& s, double res)
+{
+    if (res != res) {
+        s.resid = __builtin_inf();
+        s.done = 1;
+        return;
+    }
+    s.resid = res;
+    if (s.iter > s.iter_max) s.done = 1;
+    else if (res < s.tol) s.done = 1;
+    else if (s.rho1 == 0.0) { s.done = 1; s.flag = 2; }
+}
+
+__device__ __forceinline__ bool first_of_system(const FiberArgs& a, uint32_t sys)
+{
+    return blockIdx.x == a.sys_begin[sys];
+}
+
+template <int PC>
+__device__ __forceinline__ void apply_block(const KArgs& k, uint32_t f, int bs, const double* xs, double* y,
+                                            double* red)
+{
+    if constexpr (PC == CYMG_PRECOND_FP64)
+        block_apply_mw<double, kApplyWaves>(k.B64 + k.blkoff[f], bs, k.useblk[f] != 0, xs, y, red);
+    else if constexpr (PC == CYMG_PRECOND_FP32)
+        block_apply_mw<float, kApplyWaves>(k.B32 + k.blkoff[f], bs, k.useblk[f] != 0, xs, y, red);
+}
+
+// r = b - M x0 (or b), rt = r, x = x0 (or 0); <rt, r> partials; threshold
 template <int D, int PC>
 __global__ __launch_bounds__(kWave) void k_init(KArgs k, const double* __restrict__ x0, double tolerance,
                                                 int tol_mode, int iter_max, double dt)
@@ -78,41 +146,87 @@ __global__ __launch_bounds__(kWave) void k_init(KArgs k, const double* __restric
     rr = wave_sum(rr);
     if (threadIdx.x == 0) k.part[P_RR * k.nfib + f] = rr;
     const uint32_t sys = a.fib_sys[f];
-    const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
-    if (last_of_system(k.counter, sys, fe - fb)) {
-        const double dot = system_sum(k.part + P_RR * k.nfib, fb, fe);
+    if (first_of_system(a, sys)) {
+        const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
+        const double noise = wmin(k.part + P_NOISE * k.nfib, fb, fe);
+        const double bn = wmax(k.part + P_BNORM * k.nfib, fb, fe);
         if (threadIdx.x == 0) {
-            SysScalars& sc = k.scal[sys];
-            sc.rho2 = 1.0;
-            sc.rho1 = dot;
-            sc.alpha = 0.0;
-            sc.omega = 1.0;
-            sc.beta = (sc.rho1 / sc.rho2) * (sc.alpha / sc.omega);
-            sc.tol = (tol_mode == CYMG_TOL_RELATIVE) ? tolerance * sc.bnorm : tolerance * (sc.noise * dt);
-            sc.resid = __builtin_inf();
-            sc.resid_s = __builtin_inf();
-            sc.iter = 0;
-            sc.iter_max = iter_max;
-            sc.done = 0;
-            sc.flag = 0;
-            sc.pending = 0;
+            SysScalars w{};
+            w.noise = noise;
+            w.bnorm = bn;
+            w.rho1 = 1.0;
+            w.alpha = 0.0;
+            w.omega = 1.0;
+            w.tol = (tol_mode == CYMG_TOL_RELATIVE) ? tolerance * bn : tolerance * (noise * dt);
+            w.resid = __builtin_inf();
+            w.resid_s = __builtin_inf();
+            w.iter = 0;
+            w.iter_max = iter_max;
+            w.stage = ST_INIT;
+            k.st_out[sys] = w;
         }
     }
 }
 
 template <int D, int PC>
-__global__ __launch_bounds__(kWave) void k_iter_a(KArgs k)
+__global__ __launch_bounds__(kApplyThreads) void k_iter_a(KArgs k)
 {
-    __shared__ FiberLds<D> s;
+    __shared__ double xs[kMaxFiberPoints * D];
+    __shared__ double red[kApplyWaves * kMaxFiberPoints * D];
+    __shared__ double wr[kApplyWaves];
+    __shared__ double bc[4];
     const uint32_t f = blockIdx.x;
     const FiberArgs& a = k.fa;
     const uint32_t sys = a.fib_sys[f];
-    const SysScalars sc = k.scal[sys];
+    SysScalars sc = k.st_in[sys];
+    const bool first = first_of_system(a, sys);
+    if (sc.done) {
+        if (first && threadIdx.x == 0) k.st_out[sys] = sc;
+        return;
+    }
+    const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
+    sc.pending = 0;
+    if (threadIdx.x < kWave) {
+        if (sc.stage == ST_INIT) {
+            const double rr = wsum(k.part + P_RR * k.nfib, fb, fe);
+            if (threadIdx.x == 0) bc[0] = rr;
+        } else {
+            double S4[4];
+            wsum4(k.part4, fb, fe, S4);
+            if (threadIdx.x == 0) { bc[0] = S4[0]; bc[1] = S4[1]; bc[2] = S4[2]; bc[3] = S4[3]; }
+        }
+    }
+    __syncthreads();
+    if (sc.stage == ST_INIT) {
+        // first iteration: rho2 = 1, rho1 = <rt, r0>
+        sc.rho2 = sc.rho1;
+        sc.rho1 = bc[0];
+        sc.beta = (sc.rho1 / sc.rho2) * (sc.alpha / sc.omega);
+    } else {
+        const double tt = bc[0], ts = bc[1], rts = bc[2], rtt = bc[3];
+        if (tt == 0.0) {
+            sc.done = 1; sc.flag = 0; sc.resid = sc.resid_s;
+        } else {
+            const double om = ts / tt;
+            if (om == 0.0) {
+                sc.done = 1; sc.flag = 3; sc.resid = sc.resid_s;
+            } else {
+                sc.omega = om;
+                sc.pending = 1;
+                sc.rho2 = sc.rho1;
+                sc.rho1 = rts - om * rtt;
+                sc.beta = (sc.rho1 / sc.rho2) * (sc.alpha / sc.omega);
+                sc.iter += 1;
+            }
+        }
+    }
+    sc.stage = ST_A;
+    if (first && threadIdx.x == 0) k.st_out[sys] = sc;
     if (sc.done) return;
     const uint32_t np = a.np[f], o = D * a.idx[f];
     const int n = D * (int)np;
     double rmax = 0.0;
-    for (int i = threadIdx.x; i < n; i += kWave) {
+    for (int i = threadIdx.x; i < n; i += kApplyThreads) {
         double ri = k.r[o + i];
         if (sc.pending) {
             // x += omega shat (BCGS: shat = s), r = s - omega t
@@ -126,30 +240,31 @@ __global__ __launch_bounds__(kWave) void k_iter_a(KArgs k)
         if (sc.beta == 0.0) pi = ri;
         else pi = sc.beta * (k.p[o + i] + (-sc.omega) * k.v[o + i]) + ri;
         k.p[o + i] = pi;
-        s.x[i] = pi;
-    }
-    if constexpr (PC != CYMG_PRECOND_NONE) {
-        __syncthreads();
-        apply_block<PC>(k, f, n, s.x, k.ph + o);
+        xs[i] = pi;
     }
     rmax = wave_max(rmax);
-    if (threadIdx.x == 0) k.part[P_RMAX * k.nfib + f] = rmax;
-    const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
-    if (last_of_system(k.counter, sys, fe - fb)) {
-        double res = system_max(k.part + P_RMAX * k.nfib, fb, fe);
-        if (threadIdx.x == 0) {
-            SysScalars& w = k.scal[sys];
-            w.pending = 0;
-            if (res != res) {
-                w.resid = __builtin_inf();
-                w.done = 1;
-            } else {
-                w.resid = res;
-                if (w.iter > w.iter_max) w.done = 1;
-                else if (res < w.tol) w.done = 1;
-                else if (w.rho1 == 0.0) { w.done = 1; w.flag = 2; }
-            }
-        }
+    if ((threadIdx.x & (kWave - 1)) == 0) wr[threadIdx.x / kWave] = rmax;
+    __syncthreads();
+    if constexpr (PC != CYMG_PRECOND_NONE) apply_block<PC>(k, f, n, xs, k.ph + o, red);
+    if (threadIdx.x == 0) {
+        double m = wr[0];
+        for (int q = 1; q < kApplyWaves; ++q) m = nan_max(m, wr[q]);
+        k.part[P_RMAX * k.nfib + f] = m;
+    }
+}
+
+// convergence test of the last A, one workgroup per system, in place
+__global__ __launch_bounds__(kWave) void k_decide(KArgs k, uint32_t nsys)
+{
+    const uint32_t sys = blockIdx.x;
+    if (sys >= nsys) return;
+    SysScalars sc = k.st_in[sys];
+    if (sc.done || sc.stage != ST_A) return;
+    const uint32_t fb = k.fa.sys_begin[sys], fe = k.fa.sys_begin[sys + 1];
+    const double res = wmax(k.part + P_RMAX * k.nfib, fb, fe);
+    if (threadIdx.x == 0) {
+        decide(sc, res);
+        k.st_in[sys] = sc;
     }
 }
 
@@ -157,10 +272,18 @@ template <int D, int PC>
 __global__ __launch_bounds__(kWave) void k_iter_b(KArgs k)
 {
     __shared__ FiberLds<D> s;
-    const uint32_t f = blockIdx.x;
+        const uint32_t f = blockIdx.x;
     const FiberArgs& a = k.fa;
     const uint32_t sys = a.fib_sys[f];
-    if (k.scal[sys].done) return;
+    SysScalars sc = k.st_in[sys];
+    const bool first = first_of_system(a, sys);
+    if (!sc.done) {
+        const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
+        decide(sc, wmax(k.part + P_RMAX * k.nfib, fb, fe));
+        sc.stage = ST_B;
+    }
+    if (first && threadIdx.x == 0) k.st_out[sys] = sc;
+    if (sc.done) return;
     const double* X = PC ? k.ph : k.p;
     const uint32_t np = a.np[f], o = D * a.idx[f];
     const int n = D * (int)np;
@@ -175,47 +298,53 @@ __global__ __launch_bounds__(kWave) void k_iter_b(KArgs k)
     }
     dl = wave_sum(dl);
     if (threadIdx.x == 0) k.part[P_DELTA * k.nfib + f] = dl;
-    const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
-    if (last_of_system(k.counter, sys, fe - fb)) {
-        const double delta = system_sum(k.part + P_DELTA * k.nfib, fb, fe);
-        if (threadIdx.x == 0) {
-            SysScalars& w = k.scal[sys];
-            if (PC != CYMG_PRECOND_NONE && delta == 0.0) { w.done = 1; w.flag = 4; }
-            else w.alpha = w.rho1 / delta;
-        }
-    }
 }
 
 template <int D, int PC>
-__global__ __launch_bounds__(kWave) void k_iter_c(KArgs k)
+__global__ __launch_bounds__(kApplyThreads) void k_iter_c(KArgs k)
 {
-    __shared__ FiberLds<D> s;
+    __shared__ double xs[kMaxFiberPoints * D];
+    __shared__ double red[kApplyWaves * kMaxFiberPoints * D];
+    __shared__ double wr[kApplyWaves];
+    __shared__ double bc[4];
     const uint32_t f = blockIdx.x;
     const FiberArgs& a = k.fa;
     const uint32_t sys = a.fib_sys[f];
-    const SysScalars sc = k.scal[sys];
+    SysScalars sc = k.st_in[sys];
+    const bool first = first_of_system(a, sys);
+    if (!sc.done) {
+        const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
+        if (threadIdx.x < kWave) {
+            const double dsum = wsum(k.part + P_DELTA * k.nfib, fb, fe);
+            if (threadIdx.x == 0) bc[0] = dsum;
+        }
+        __syncthreads();
+        const double delta = bc[0];
+        if (PC != CYMG_PRECOND_NONE && delta == 0.0) { sc.done = 1; sc.flag = 4; }
+        else sc.alpha = sc.rho1 / delta;
+        sc.stage = ST_C;
+    }
+    if (first && threadIdx.x == 0) k.st_out[sys] = sc;
     if (sc.done) return;
     const double* X = PC ? k.ph : k.p;
     const uint32_t np = a.np[f], o = D * a.idx[f];
     const int n = D * (int)np;
     double smax = 0.0;
-    for (int i = threadIdx.x; i < n; i += kWave) {
+    for (int i = threadIdx.x; i < n; i += kApplyThreads) {
         const double si = k.r[o + i] + (-sc.alpha) * k.v[o + i];
         k.r[o + i] = si;
         k.x[o + i] = k.x[o + i] + sc.alpha * X[o + i];
         smax = nan_max(smax, fabs(si));
-        s.x[i] = si;
-    }
-    if constexpr (PC != CYMG_PRECOND_NONE) {
-        __syncthreads();
-        apply_block<PC>(k, f, n, s.x, k.sh + o);
+        xs[i] = si;
     }
     smax = wave_max(smax);
-    if (threadIdx.x == 0) k.part[P_SMAX * k.nfib + f] = smax;
-    const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
-    if (last_of_system(k.counter, sys, fe - fb)) {
-        const double m = system_max(k.part + P_SMAX * k.nfib, fb, fe);
-        if (threadIdx.x == 0) k.scal[sys].resid_s = (m != m) ? __builtin_inf() : m;
+    if ((threadIdx.x & (kWave - 1)) == 0) wr[threadIdx.x / kWave] = smax;
+    __syncthreads();
+    if constexpr (PC != CYMG_PRECOND_NONE) apply_block<PC>(k, f, n, xs, k.sh + o, red);
+    if (threadIdx.x == 0) {
+        double m = wr[0];
+        for (int q = 1; q < kApplyWaves; ++q) m = nan_max(m, wr[q]);
+        k.part[P_SMAX * k.nfib + f] = m;
     }
 }
 
@@ -223,10 +352,19 @@ template <int D, int PC>
 __global__ __launch_bounds__(kWave) void k_iter_d(KArgs k)
 {
     __shared__ FiberLds<D> s;
-    const uint32_t f = blockIdx.x;
+        const uint32_t f = blockIdx.x;
     const FiberArgs& a = k.fa;
     const uint32_t sys = a.fib_sys[f];
-    if (k.scal[sys].done) return;
+    SysScalars sc = k.st_in[sys];
+    const bool first = first_of_system(a, sys);
+    if (!sc.done) {
+        const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
+        const double m = wmax(k.part + P_SMAX * k.nfib, fb, fe);
+        sc.resid_s = (m != m) ? __builtin_inf() : m;
+        sc.stage = ST_D;
+    }
+    if (first && threadIdx.x == 0) k.st_out[sys] = sc;
+    if (sc.done) return;
     const double* X = PC ? k.sh : k.r;
     const uint32_t np = a.np[f], o = D * a.idx[f];
     const int n = D * (int)np;
@@ -249,37 +387,8 @@ __global__ __launch_bounds__(kWave) void k_iter_d(KArgs k)
     ts = wave_sum(ts);
     rts = wave_sum(rts);
     rtt = wave_sum(rtt);
-    if (threadIdx.x == 0) {
-        k.part[P_TT * k.nfib + f] = tt;
-        k.part[P_TS * k.nfib + f] = ts;
-        k.part[P_RTS * k.nfib + f] = rts;
-        k.part[P_RTT * k.nfib + f] = rtt;
-    }
-    const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
-    if (last_of_system(k.counter, sys, fe - fb)) {
-        const double Stt = system_sum(k.part + P_TT * k.nfib, fb, fe);
-        const double Sts = system_sum(k.part + P_TS * k.nfib, fb, fe);
-        const double Srts = system_sum(k.part + P_RTS * k.nfib, fb, fe);
-        const double Srtt = system_sum(k.part + P_RTT * k.nfib, fb, fe);
-        if (threadIdx.x == 0) {
-            SysScalars& w = k.scal[sys];
-            if (Stt == 0.0) {
-                w.done = 1; w.flag = 0; w.resid = w.resid_s;
-            } else {
-                const double om = Sts / Stt;
-                if (om == 0.0) {
-                    w.done = 1; w.flag = 3; w.resid = w.resid_s;
-                } else {
-                    w.omega = om;
-                    w.pending = 1;
-                    w.rho2 = w.rho1;
-                    w.rho1 = Srts - om * Srtt;
-                    w.beta = (w.rho1 / w.rho2) * (w.alpha / w.omega);
-                    w.iter += 1;
-                }
-            }
-        }
-    }
+    if (threadIdx.x == 0)
+        *reinterpret_cast<double4*>(k.part4 + 4 * (size_t)f) = make_double4(tt, ts, rts, rtt);
 }
 
 static KArgs make_kargs(Context& c)
@@ -300,45 +409,57 @@ static KArgs make_kargs(Context& c)
     k.ph = c.k_ph.get();
     k.sh = c.k_sh.get();
     k.part = c.d_part.get();
-    k.counter = c.d_counter.get();
-    k.scal = c.d_scal.get();
+    k.part4 = c.d_part.get() + (size_t)P_TT * c.nfib;
+    k.st_in = nullptr;
+    k.st_out = nullptr;
     k.nfib = c.nfib;
     return k;
 }
 
+// State buffers: S0 = d_scal[0:nsys], S1 = d_scal[nsys:2nsys].
+// init -> S1; A: S1 -> S0; decide: S0 in place; B: S0 -> S1; C: S1 -> S0; D: S0 -> S1.
 template <int D, int PC>
 static void run(Context& c, const double* guess, int max_iter, double tol, int tol_mode, cymg_solve_report* rep)
 {
     KArgs k = make_kargs(c);
-    const dim3 g(c.nfib), b(kWave);
-    hipLaunchKernelGGL((k_init<D, PC>), g, b, 0, c.stream, k, guess, tol, tol_mode, max_iter, c.prm.time_step);
+    SysScalars* S0 = c.d_scal.get();
+    SysScalars* S1 = S0 + c.nsys;
+    auto with = [&](SysScalars* in, SysScalars* out) {
+        KArgs q = k;
+        q.st_in = in;
+        q.st_out = out;
+        return q;
+    };
+    const KArgs kI = with(S0, S1), kA = with(S1, S0), kB = with(S0, S1), kC = with(S1, S0), kD = with(S0, S1);
+    const KArgs kDec = with(S0, S0);
+    const dim3 g(c.nfib), b(kWave), b4(kApplyThreads), gs(c.nsys);
+    hipLaunchKernelGGL((k_init<D, PC>), g, b, 0, c.stream, kI, guess, tol, tol_mode, max_iter, c.prm.time_step);
+    hipLaunchKernelGGL((k_iter_a<D, PC>), g, b4, 0, c.stream, kA);
+    hipLaunchKernelGGL(k_decide, gs, b, 0, c.stream, kDec, c.nsys);
     CYMG_HIP(hipGetLastError());
+    // the first chunk covers the iteration count of the previous solve in this
+    // mode, so a typical solve needs one or two host reads of the stop flag;
+    // iterations launched after convergence exit at once
     int launched = 0;
-    int chunk = 4;
+    int& predicted = c.last_iters[PC];
+    int chunk = predicted > 0 ? predicted + 1 : 4;
     const size_t sbytes = sizeof(SysScalars) * c.nsys;
     while (true) {
-        for (int it = 0; it < chunk; ++it) {
-            hipLaunchKernelGGL((k_iter_a<D, PC>), g, b, 0, c.stream, k);
-            hipLaunchKernelGGL((k_iter_b<D, PC>), g, b, 0, c.stream, k);
-            hipLaunchKernelGGL((k_iter_c<D, PC>), g, b, 0, c.stream, k);
-            hipLaunchKernelGGL((k_iter_d<D, PC>), g, b, 0, c.stream, k);
-        }
-        CYMG_HIP(hipGetLastError());
-        launched += chunk;
-        // one more A pass decides convergence of the last iterate
-        CYMG_HIP(hipMemcpyAsync(c.h_scal, c.d_scal.get(), sbytes, hipMemcpyDeviceToHost, c.stream));
+        CYMG_HIP(hipMemcpyAsync(c.h_scal, S0, sbytes, hipMemcpyDeviceToHost, c.stream));
         CYMG_HIP(hipStreamSynchronize(c.stream));
         bool all = true;
         for (uint32_t s = 0; s < c.nsys; ++s) all = all && c.h_scal[s].done;
-        if (all) break;
-        if (launched > max_iter + 2) {
-            // iteration cap reached: a final A pass sets the flags
-            hipLaunchKernelGGL((k_iter_a<D, PC>), g, b, 0, c.stream, k);
-            CYMG_HIP(hipMemcpyAsync(c.h_scal, c.d_scal.get(), sbytes, hipMemcpyDeviceToHost, c.stream));
-            CYMG_HIP(hipStreamSynchronize(c.stream));
-            break;
+        if (all || launched > max_iter + 2) break;
+        for (int it = 0; it < chunk; ++it) {
+            hipLaunchKernelGGL((k_iter_b<D, PC>), g, b, 0, c.stream, kB);
+            hipLaunchKernelGGL((k_iter_c<D, PC>), g, b4, 0, c.stream, kC);
+            hipLaunchKernelGGL((k_iter_d<D, PC>), g, b, 0, c.stream, kD);
+            hipLaunchKernelGGL((k_iter_a<D, PC>), g, b4, 0, c.stream, kA);
         }
-        if (chunk < 16) chunk *= 2;
+        hipLaunchKernelGGL(k_decide, gs, b, 0, c.stream, kDec, c.nsys);
+        CYMG_HIP(hipGetLastError());
+        launched += chunk;
+        chunk = predicted > 0 ? std::max(4, predicted / 8) : std::min(2 * chunk, 16);
     }
     // report system 0 (single-system contexts); batched callers read per system
     const SysScalars& w = c.h_scal[0];
@@ -349,6 +470,7 @@ static void run(Context& c, const double* guess, int max_iter, double tol, int t
     rep->converged = w.resid < w.tol;
     rep->noise_level = w.noise * c.prm.time_step;
     rep->precond_used = PC;
+    predicted = w.iter;
 }
 
 void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter, double tol, int tol_mode,

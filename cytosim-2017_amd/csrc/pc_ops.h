@@ -40,4 +40,57 @@ __device__ __forceinline__ void block_apply(const T* __restrict__ B, int bs, boo
     }
 }
 
+// Same product with NW wavefronts per fiber: wave w takes the columns
+// j = w, w+NW, ..., lane i the rows i, i+64, ...; the NW partial sums are
+// added in wave order through LDS (red: NW*bs doubles), so the result does not
+// depend on scheduling.  More wavefronts per block keep more loads in flight.
+template <typename T, int NW>
+__device__ __forceinline__ void block_apply_mw(const T* __restrict__ B, int bs, bool use, const double* xs,
+                                               double* y, double* red)
+{
+    constexpr int R = (kMaxFiberPoints * 3 + kWave - 1) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    if (!use) {
+        for (int i = threadIdx.x; i < bs; i += NW * kWave) y[i] = xs[i];
+        return;
+    }
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    const int nr = (bs + kWave - 1) / kWave;
+    int j = w;
+    for (; j + NW < bs; j += 2 * NW) {
+        const double x0 = xs[j], x1 = xs[j + NW];
+        const T* c0 = B + (size_t)bs * j;
+        const T* c1 = B + (size_t)bs * (j + NW);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = lane + kWave * r;
+            if (r < nr && i < bs) acc[r] += (double)c0[i] * x0 + (double)c1[i] * x1;
+        }
+    }
+    for (; j < bs; j += NW) {
+        const double x0 = xs[j];
+        const T* c0 = B + (size_t)bs * j;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = lane + kWave * r;
+            if (r < nr && i < bs) acc[r] += (double)c0[i] * x0;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + kWave * r;
+        if (r < nr && i < bs) red[w * bs + i] = acc[r];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < bs; i += NW * kWave) {
+        double v = red[i];
+#pragma unroll
+        for (int q = 1; q < NW; ++q) v += red[q * bs + i];
+        y[i] = v;
+    }
+}
+
 }  // namespace cymg

@@ -31,6 +31,7 @@ struct PcArgs {
     double dt;
     int bs_max;
     int factor;   // 0: store the assembled block M itself (for export)
+    int method;   // 0: Gauss-Jordan (default), 1: getrf + getri
 };
 
 // LU with partial pivoting (dgetf2 order), in place, column-major, lda = bs.
@@ -131,6 +132,81 @@ __device__ void lds_getri(T* A, int bs, const int* ipiv, T* work)
     __syncthreads();
 }
 
+// In-place inverse by Gauss-Jordan elimination with partial pivoting (the
+// pivot of column k is the first entry of largest magnitude at or below the
+// diagonal, as dgetf2 chooses it).  One pass: 3 barriers per column instead of
+// the ~9 of getrf + getri, and n^3 multiply-adds instead of ~n^3 spread over
+// two triangular phases.  Row interchanges are undone as column
+// interchanges at the end (the last step of dgetri).  info != 0: exactly zero
+// pivot, the block is not used (meca.cc:437-438).
+template <typename T>
+__device__ void lds_gauss_jordan(T* A, int bs, int* ipiv, int* info, int* pivrow, T* colk, T* sc)
+{
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    for (int k = 0; k < bs; ++k) {
+        if (tid < kWave) {
+            T best = -1;
+            int bi = bs;
+            for (int i = k + lane; i < bs; i += kWave) {
+                const T v = fabs(A[i + bs * k]);
+                if (v > best) { best = v; bi = i; }
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const T ob = __shfl_xor(best, o, kWave);
+                const int oi = __shfl_xor(bi, o, kWave);
+                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            }
+            if (lane == 0) {
+                if (bi >= bs) bi = k;
+                *pivrow = bi;
+                ipiv[k] = bi + 1;
+                sc[0] = A[bi + bs * k];   // pivot
+                sc[1] = A[k + bs * k];    // entry that moves to row bi
+                if (sc[0] == T(0)) *info = k + 1;
+            }
+        }
+        __syncthreads();
+        if (*info) return;
+        const int p = *pivrow;
+        const T r = T(1) / sc[0];
+        // multipliers of column k after the interchange (0 on the pivot row)
+        for (int i = tid; i < bs; i += kPcThreads) {
+            if (i == k) colk[i] = 0;
+            else if (i == p) colk[i] = sc[1];
+            else colk[i] = A[i + bs * k];
+        }
+        // interchange rows k and p, scale the pivot row
+        for (int j = tid; j < bs; j += kPcThreads) {
+            T vk = A[k + bs * j], vp = A[p + bs * j];
+            if (p != k) { const T t = vk; vk = vp; vp = t; }
+            A[p + bs * j] = vp;
+            A[k + bs * j] = (j == k) ? r : vk * r;
+        }
+        __syncthreads();
+        // eliminate column k from every other row
+        for (int t = tid; t < bs * bs; t += kPcThreads) {
+            const int i = t % bs, j = t / bs;
+            if (i == k) continue;
+            const T m = colk[i];
+            if (j == k) A[i + bs * k] = -m * r;
+            else A[i + bs * j] -= m * A[k + bs * j];
+        }
+        __syncthreads();
+    }
+    // undo the row interchanges as column interchanges
+    for (int i = tid; i < bs; i += kPcThreads)
+        for (int j = bs - 1; j >= 0; --j) {
+            const int jp = ipiv[j] - 1;
+            if (jp != j) {
+                const T t = A[i + bs * j];
+                A[i + bs * j] = A[i + bs * jp];
+                A[i + bs * jp] = t;
+            }
+        }
+    __syncthreads();
+}
+
 template <int D, typename T>
 __global__ __launch_bounds__(kPcThreads) void k_precond_build(PcArgs a, T* __restrict__ out, int* __restrict__ useblk)
 {
@@ -150,8 +226,8 @@ __global__ __launch_bounds__(kPcThreads) void k_precond_build(PcArgs a, T* __res
     double* spd = se + S;                          // S
     int* ipiv = (int*)(spd + S);                   // bs
     int* misc = ipiv + bs;                         // info, pivot row
-    T* work = (T*)(((uintptr_t)(misc + 4) + 15) & ~(uintptr_t)15);   // bs
-    T* AT = (T*)(((uintptr_t)(work + bs) + 15) & ~(uintptr_t)15);   // bs*bs when T != double
+    T* work = (T*)(((uintptr_t)(misc + 4) + 15) & ~(uintptr_t)15);   // bs + 2
+    T* AT = (T*)(((uintptr_t)(work + bs + 2) + 15) & ~(uintptr_t)15);   // bs*bs when T != double
 
     for (int i = tid; i < bs * bs; i += kPcThreads) A[i] = 0.0;
     for (int i = tid; i < D * S; i += kPcThreads) sdif[i] = a.diff[D * seg + i];
@@ -257,9 +333,13 @@ __global__ __launch_bounds__(kPcThreads) void k_precond_build(PcArgs a, T* __res
         for (int i = tid; i < bs * bs; i += kPcThreads) dst[i] = M[i];
         return;
     }
-    lds_getrf<T>(M, bs, ipiv, &misc[0], &misc[1]);
+    if (a.method == 0) {
+        lds_gauss_jordan<T>(M, bs, ipiv, &misc[0], &misc[1], work, work + bs);
+    } else {
+        lds_getrf<T>(M, bs, ipiv, &misc[0], &misc[1]);
+        if (misc[0] == 0) lds_getri<T>(M, bs, ipiv, work);
+    }
     const int ok = (misc[0] == 0);
-    if (ok) lds_getri<T>(M, bs, ipiv, work);
     T* dst = out + a.blkoff[f];
     if (ok)
         for (int i = tid; i < bs * bs; i += kPcThreads) dst[i] = M[i];
@@ -293,6 +373,7 @@ void launch_precond_build(Context& c, int precision, double* block_out)
     a.dt = c.prm.time_step;
     a.bs_max = bs_max;
     a.factor = block_out ? 0 : 1;
+    a.method = c.pc_method;
     c.d_useblk.resize(c.nfib);
     const dim3 g(c.nfib), b(kPcThreads);
     if (precision == CYMG_PRECOND_FP32) {
@@ -331,25 +412,26 @@ void launch_precond_build(Context& c, int precision, double* block_out)
 
 // standalone Meca::precondition
 template <typename T>
-__global__ __launch_bounds__(kWave) void k_precond_apply(int D, const uint32_t* __restrict__ npv,
-                                                         const uint32_t* __restrict__ idxv,
-                                                         const uint64_t* __restrict__ blkoff,
-                                                         const int* __restrict__ useblk, const T* __restrict__ B,
-                                                         const double* __restrict__ X, double* __restrict__ Y)
+__global__ __launch_bounds__(4 * kWave) void k_precond_apply(int D, const uint32_t* __restrict__ npv,
+                                                             const uint32_t* __restrict__ idxv,
+                                                             const uint64_t* __restrict__ blkoff,
+                                                             const int* __restrict__ useblk, const T* __restrict__ B,
+                                                             const double* __restrict__ X, double* __restrict__ Y)
 {
     __shared__ double xs[kMaxFiberPoints * 3];
+    __shared__ double red[4 * kMaxFiberPoints * 3];
     const uint32_t f = blockIdx.x;
     const int bs = D * (int)npv[f];
     const uint32_t o = D * idxv[f];
-    for (int j = threadIdx.x; j < bs; j += kWave) xs[j] = X[o + j];
+    for (int j = threadIdx.x; j < bs; j += 4 * kWave) xs[j] = X[o + j];
     __syncthreads();
-    block_apply<T>(B + blkoff[f], bs, useblk[f] != 0, xs, X + o, Y + o);
+    block_apply_mw<T, 4>(B + blkoff[f], bs, useblk[f] != 0, xs, Y + o, red);
 }
 
 void launch_precond_apply(Context& c, const double* X, double* Y)
 {
     if (!c.nfib) return;
-    const dim3 g(c.nfib), b(kWave);
+    const dim3 g(c.nfib), b(4 * kWave);
     if (c.pc_precision == CYMG_PRECOND_FP32)
         hipLaunchKernelGGL(k_precond_apply<float>, g, b, 0, c.stream, c.dim, c.d_np.get(), c.d_idx.get(),
                            c.d_blkoff.get(), c.d_useblk.get(), c.d_binv32.get(), X, Y);

@@ -1,0 +1,157 @@
+// C++ host interface of the MI355X implicit step (see meca.hpp).
+#include "meca.hpp"
+
+namespace cymeca {
+
+Meca::Meca(int dim, int device, double hr0, double hr1) : dim_(dim), hr0_(hr0), hr1_(hr1)
+{
+    const int rc = cymg_create(dim, device, &ctx_);
+    if (rc) throw Exception(rc, "cymg_create failed (DIM must be 2 or 3 and a GPU visible)");
+}
+
+Meca::~Meca()
+{
+    if (ctx_) cymg_destroy(ctx_);
+}
+
+void Meca::check(int rc) const
+{
+    if (rc) throw Exception(rc, cymg_last_error(ctx_));
+}
+
+void Meca::clear()
+{
+    check(cymg_clear(ctx_));
+    objs_.clear();
+    nbPts_ = 0;
+}
+
+void Meca::add(Fiber* f)
+{
+    uint32_t mi = 0;
+    check(cymg_add_fiber(ctx_, f->nbPoints(dim_), f->segmentation, f->rigidity, &mi));
+    f->matIndex = mi;
+    objs_.push_back(f);
+    nbPts_ += f->nbPoints(dim_);
+}
+
+void Meca::prepare(const SimulProp* prop)
+{
+    cymg_sim_params p{ prop->time_step, prop->kT, prop->viscosity, hr0_, hr1_ };
+    check(cymg_set_params(ctx_, &p));
+    // putPoints of every Mecable, in matIndex order
+    std::vector<double> pts(size());
+    for (Fiber* f : objs_)
+        std::copy(f->points.begin(), f->points.end(), pts.begin() + (size_t)dim_ * f->matIndex);
+    check(cymg_prepare(ctx_, pts.data()));
+}
+
+void Meca::interLink(const PointExact& a, const PointExact& b, double w)
+{
+    check(cymg_inter_link_ee(ctx_, a.mat, b.mat, w));
+}
+
+void Meca::interLink(const PointInterpolated& a, const PointExact& b, double w)
+{
+    check(cymg_inter_link_ie(ctx_, a.mat1, a.mat2, a.coef, b.mat, w));
+}
+
+void Meca::interLink(const PointInterpolated& a, const PointInterpolated& b, double w)
+{
+    check(cymg_inter_link_ii(ctx_, a.mat1, a.mat2, a.coef, b.mat1, b.mat2, b.coef, w));
+}
+
+void Meca::interLongLink(const PointExact& a, const PointExact& b, double len, double w)
+{
+    check(cymg_inter_long_link_ee(ctx_, a.mat, b.mat, len, w));
+}
+
+void Meca::interClamp(const PointExact& a, const double g[], double w)
+{
+    check(cymg_inter_clamp_e(ctx_, a.mat, g, w));
+}
+
+void Meca::interClamp(const PointInterpolated& a, const double g[], double w)
+{
+    check(cymg_inter_clamp_i(ctx_, a.mat1, a.mat2, a.coef, g, w));
+}
+
+void Meca::interLongClamp(const PointExact& a, const double center[], double len, double w)
+{
+    check(cymg_inter_long_clamp(ctx_, a.mat, center, len, w));
+}
+
+void Meca::interPlane(const PointExact& a, const double dir[], const double g[], double w)
+{
+    check(cymg_inter_plane_e(ctx_, a.mat, dir, g, w));
+}
+
+void Meca::addPureForce(const PointExact& a, const double force[])
+{
+    check(cymg_add_pure_force(ctx_, a.mat, force));
+}
+
+void Meca::confineSphere(double radius, double stiffness)
+{
+    check(cymg_confine_sphere(ctx_, radius, stiffness));
+}
+
+void Meca::setCouples(const std::vector<uint32_t>& fa, const std::vector<double>& aa,
+                      const std::vector<uint32_t>& fb, const std::vector<double>& ab,
+                      const std::vector<double>& w)
+{
+    check(cymg_set_couples(ctx_, (uint32_t)fa.size(), fa.data(), aa.data(), fb.data(), ab.data(), w.data()));
+}
+
+void Meca::couplesInteractions()
+{
+    check(cymg_couples_interactions(ctx_));
+}
+
+cymg_solve_report Meca::solve(const SimulProp* prop, bool precondition, const double* gauss, int tol_mode,
+                              int precision)
+{
+    cymg_solve_params p{};
+    p.tolerance = prop->tolerance;
+    p.tol_mode = tol_mode;
+    p.precondition = precondition ? precision : CYMG_PRECOND_NONE;
+    p.max_iter = 0;
+    p.guess = CYMG_GUESS_ZERO;
+    p.fallback = 1;
+    p.gauss_on_device = 0;
+    cymg_solve_report rep{};
+    check(cymg_solve(ctx_, &p, gauss, &rep));
+    getPoints();
+    return rep;
+}
+
+void Meca::computeForces()
+{
+    check(cymg_compute_forces(ctx_));
+}
+
+void Meca::multiply(const double* X, double* Y) const
+{
+    check(cymg_multiply(ctx_, X, Y));
+}
+
+void Meca::precondition(const double* X, double* Y) const
+{
+    check(cymg_precondition(ctx_, X, Y));
+}
+
+void Meca::getVector(int which, double* out) const
+{
+    check(cymg_get_vector(ctx_, which, out));
+}
+
+void Meca::getPoints()
+{
+    std::vector<double> pts(size());
+    check(cymg_get_points(ctx_, pts.data()));
+    for (Fiber* f : objs_)
+        std::copy(pts.begin() + (size_t)dim_ * f->matIndex,
+                  pts.begin() + (size_t)dim_ * (f->matIndex + f->nbPoints(dim_)), f->points.begin());
+}
+
+}  // namespace cymeca

@@ -1,0 +1,121 @@
+// C++ host interface of the MI355X implicit step, shaped like the reference's
+// `Meca` (src/sim/meca.h:76-327): Mecables are registered with add(), the
+// step is prepare() + the inter*() assembly calls + solve(), and multiply() /
+// precondition() expose the operator.  Everything runs on the GPU through the
+// C ABI of include/cymeca.h; this class owns one cymg_context.
+//
+// A Mecable here is a fiber described by its points, segmentation and
+// rigidity (the RigidFiber state Meca reads through putPoints/prepareMecable);
+// points come back through getPoints() after the solve, reshaped to equal
+// segment lengths as FiberNaked::getPoints does.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cymeca.h"
+
+namespace cymeca {
+
+// SimulProp fields the step reads (simul_prop.h)
+struct SimulProp {
+    double time_step = 0.005;
+    double viscosity = 1.0;
+    double kT = 0.0042;
+    double tolerance = 0.05;   // SimulProp::tolerance
+    int precondition = 1;      // SimulProp::precondition
+};
+
+// PointExact / PointInterpolated (point_exact.h:10-67, point_interpolated.h:26-137)
+struct PointExact {
+    uint32_t mat;              // matIndex() of the point
+};
+struct PointInterpolated {
+    uint32_t mat1, mat2;       // matIndex1/2
+    double coef;               // pos = (1-coef)*P1 + coef*P2
+};
+
+// the reference's Exception("convergence failure in solver") etc.
+struct Exception : std::runtime_error {
+    int code;
+    Exception(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+struct Fiber {
+    std::vector<double> points;   // DIM * nbPoints
+    double segmentation = 0.5;    // fnCut
+    double rigidity = 0.0;        // FiberProp::rigidity
+    uint32_t matIndex = 0;        // set by Meca::add
+    uint32_t nbPoints(int dim) const { return (uint32_t)(points.size() / dim); }
+};
+
+// FiberNaked::interpolate (fiber_naked.cc:1050-1065), abscissa from the minus end
+inline PointInterpolated interpolate(const Fiber& f, int dim, double ab)
+{
+    const uint32_t np = f.nbPoints(dim);
+    if (ab <= 0) return { f.matIndex, f.matIndex + 1, 0.0 };
+    const double q = ab / f.segmentation;
+    const double n = (double)(uint64_t)q;
+    const uint32_t rd = (uint32_t)n;
+    if (rd < np - 1) return { f.matIndex + rd, f.matIndex + rd + 1, q - n };
+    return { f.matIndex + np - 2, f.matIndex + np - 1, 1.0 };
+}
+
+class Meca {
+public:
+    explicit Meca(int dim, int device = 0, double hydro_radius0 = 0.0125, double hydro_radius1 = 10.0);
+    ~Meca();
+    Meca(const Meca&) = delete;
+    Meca& operator=(const Meca&) = delete;
+
+    void clear();                           // meca.cc:55-60
+    void add(Fiber* f);                     // meca.cc:67-79
+    unsigned nbMecables() const { return (unsigned)objs_.size(); }
+    unsigned nbPoints() const { return nbPts_; }
+    unsigned size() const { return dim_ * nbPts_; }
+
+    void prepare(const SimulProp* prop);    // meca.cc:488-537
+
+    // assembly (meca_inter.cc)
+    void interLink(const PointExact& a, const PointExact& b, double weight);
+    void interLink(const PointInterpolated& a, const PointExact& b, double weight);
+    void interLink(const PointInterpolated& a, const PointInterpolated& b, double weight);
+    void interLongLink(const PointExact& a, const PointExact& b, double len, double weight);
+    void interClamp(const PointExact& a, const double g[], double weight);
+    void interClamp(const PointInterpolated& a, const double g[], double weight);
+    void interLongClamp(const PointExact& a, const double center[], double len, double weight);
+    void interPlane(const PointExact& a, const double dir[], const double g[], double weight);
+    void addPureForce(const PointExact& a, const double force[]);
+    // device-side emitters for the common cases
+    void confineSphere(double radius, double stiffness);          // Fiber::setInteractions + SpaceSphere
+    void setCouples(const std::vector<uint32_t>& fa, const std::vector<double>& aa,
+                    const std::vector<uint32_t>& fb, const std::vector<double>& ab,
+                    const std::vector<double>& stiffness);
+    void couplesInteractions();                                   // Couple::setInteractions
+
+    // meca.cc:598-845; gauss: N(0,1) per degree of freedom (nullptr: no Brownian term)
+    cymg_solve_report solve(const SimulProp* prop, bool precondition, const double* gauss = nullptr,
+                            int tol_mode = CYMG_TOL_REFERENCE, int precond_precision = CYMG_PRECOND_FP64);
+
+    void computeForces();                                         // meca.cc:550-580
+    void multiply(const double* X, double* Y) const;              // meca.cc:157-181
+    void precondition(const double* X, double* Y) const;          // meca.cc:448-466
+
+    // export after solve: vPTS/vFOR and the Mecable points (getPoints)
+    void getVector(int which, double* out) const;
+    void getPoints();                                             // fills Fiber::points
+
+    cymg_context* context() const { return ctx_; }
+
+private:
+    void check(int rc) const;
+    int dim_;
+    cymg_context* ctx_ = nullptr;
+    std::vector<Fiber*> objs_;
+    unsigned nbPts_ = 0;
+    double hr0_, hr1_;
+};
+
+}  // namespace cymeca

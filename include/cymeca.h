@@ -183,7 +183,8 @@ int         cymg_enable_timing(cymg_context* ctx, int enable);
 /* average duration (ms) of one launch of a hot kernel, timed with HIP events
    around `reps` back-to-back launches on the context's stream, on the current
    system state: which = 0 fused operator (Meca::multiply), 1 block apply
-   (Meca::precondition), 2 block build + factorisation (FP64), 3 the same in FP32 */
+   (Meca::precondition), 2 block build + factorisation (FP64), 3 the same in FP32,
+   4 block build without factorisation, 5 build with getrf + getri instead of Gauss-Jordan */
 int         cymg_time_kernel(cymg_context* ctx, int which, int reps, double* avg_ms);
 
 #ifdef __cplusplus

@@ -1,0 +1,51 @@
+"""The C++ Meca interface (cytosim-2017_amd/host/meca.hpp), driven by a
+compiled C++ program the way Simul::solve drives the reference's Meca, gives
+the oracle's fiber positions."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, oracle_assemble, oracle_from_scene, rel_err
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(ROOT, "cytosim-2017_amd", "_build", "drive_meca")
+
+
+def write_scene(path, sc, nsteps, tolerance, tol_mode):
+    with open(path, "w") as f:
+        f.write(f"{sc.dim} {sc.nfib} {sc.nlinks} {nsteps}\n")
+        f.write(f"{float(sc.time_step)!r} {float(sc.kT)!r} {float(sc.viscosity)!r} {float(tolerance)!r} "
+                f"{tol_mode} {float(sc.confine_radius)!r} {float(sc.confine_stiffness)!r}\n")
+        P = sc.points
+        for fi in range(sc.nfib):
+            o, n = sc.dim * int(sc.mat_index[fi]), sc.dim * int(sc.nb_points[fi])
+            f.write(f"{int(sc.nb_points[fi])} {float(sc.segmentation[fi])!r} {float(sc.rigidity[fi])!r}\n")
+            f.write(" ".join(repr(float(x)) for x in P[o:o + n]) + "\n")
+        for k in range(sc.nlinks):
+            f.write(f"{int(sc.link_fiber_a[k])} {float(sc.link_abs_a[k])!r} {int(sc.link_fiber_b[k])} "
+                    f"{float(sc.link_abs_b[k])!r} {float(sc.link_stiffness[k])!r}\n")
+        for s in range(nsteps):
+            f.write(" ".join(repr(float(x)) for x in sc.gauss(s)) + "\n")
+
+
+def test_cpp_driver_matches_oracle(scenes, oracle, tmp_path):
+    assert os.path.exists(DRIVER), "build the package first"
+    sc = scenes.mini(dim=3, seed=41, nfib=10, nlinks=30, radius=2.0)
+    nsteps, tol = 2, 1e-11
+    write_scene(tmp_path / "scene.txt", sc, nsteps, tol, 1)
+    out = tmp_path / "out.txt"
+    r = subprocess.run([DRIVER, str(tmp_path / "scene.txt"), str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    vals = open(out).read().split()
+    pts = np.array([float(v) for v in vals[2 * nsteps:]])
+    orc = oracle_from_scene(oracle, sc)
+    for s in range(nsteps):
+        if s:
+            orc.prepare(orc.fiber_points())
+            oracle_assemble(orc, sc)
+        orc.solve(tol, sc.gauss(s), tol_mode=1)
+    assert rel_err(pts, orc.fiber_points()) < 1e-9
