@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps timed for cpu_baseline (0: skip)")
     ap.add_argument("--modes", default="fp64,fp32")
+    ap.add_argument("--batched-steps", type=int, default=3,
+                    help="steps of the 64-system batched workload (0: skip)")
     return ap.parse_args()
 
 
@@ -119,6 +121,36 @@ def main():
     mv_bytes = algorithmic_bytes_matvec(sc, kb_full, kc_full)
     achieved = mv_bytes / (mv_ms * 1e-3) / 1e9
 
+    batched = None
+    if args.batched_steps > 0:
+        print("bench: generating 64 independent systems", file=sys.stderr, flush=True)
+        batch = scenes.config5(seed=5)
+        big = batch.scene
+        gb = [torch.from_numpy(big.gauss(s)).to("cuda:0") for s in range(2)]
+        mb = pkg.Meca(3, 0)
+        pkg.load_batch(mb, batch)
+        its = []
+        for s in range(1 + args.batched_steps):
+            if s == 1:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+            if s > 0:
+                mb.prepare(None)
+                pkg.assemble(mb, big, upload_couples=False)
+            rep = mb.solve(args.tol, tol_mode=pkg.TOL_RELATIVE, precondition=pkg.PRECOND_FP64,
+                           gauss_device_ptr=gb[s % 2].data_ptr())
+            if s > 0:
+                its.append(rep["iterations"])
+        torch.cuda.synchronize()
+        bdt = time.perf_counter() - t0
+        batched = {"workload": f"batch-64-systems-{big.nfib}-fibers-{big.npts}-points-{big.nlinks}-links",
+                   "systems": batch.nsys, "fibers": big.nfib, "unknowns": big.size,
+                   "steps_per_s": round(args.batched_steps / bdt, 3),
+                   "system_steps_per_s": round(batch.nsys * args.batched_steps / bdt, 2),
+                   "max_bicgstab_iters": float(np.mean(its))}
+        print(f"bench: batched {batched}", file=sys.stderr, flush=True)
+        mb.close()
+
     cpu = None
     if args.cpu_steps > 0:
         from oracle import pyoracle
@@ -163,6 +195,7 @@ def main():
         "kernels_ms": {"matvec": mv_ms, "precond_apply": pc_ms, "precond_build_fp64": build_ms,
                        "precond_assemble_only": build_only_ms, "precond_build_getrf_getri": build_lu_ms},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 

@@ -26,7 +26,8 @@ VEC_PTS, VEC_SOL, VEC_BAS, VEC_RHS, VEC_FOR = range(5)
 
 EXPORTED = [
     "cymg_create", "cymg_destroy", "cymg_last_error", "cymg_set_params", "cymg_clear",
-    "cymg_add_fiber", "cymg_nb_points", "cymg_prepare", "cymg_add_pure_force",
+    "cymg_add_fiber", "cymg_nb_points", "cymg_set_systems", "cymg_get_system_reports",
+    "cymg_prepare", "cymg_add_pure_force",
     "cymg_inter_link_ee", "cymg_inter_link_ie", "cymg_inter_link_ii", "cymg_inter_long_link_ee",
     "cymg_inter_clamp_e", "cymg_inter_clamp_i", "cymg_inter_long_clamp", "cymg_inter_plane_e",
     "cymg_confine_sphere", "cymg_set_couples", "cymg_couples_interactions", "cymg_solve",
@@ -109,6 +110,8 @@ def lib():
         "cymg_clear": (C.c_int, [vp]),
         "cymg_add_fiber": (C.c_int, [vp, u32, C.c_double, C.c_double, C.POINTER(u32)]),
         "cymg_nb_points": (C.c_int, [vp, C.POINTER(u32)]),
+        "cymg_set_systems": (C.c_int, [vp, u32, vp]),
+        "cymg_get_system_reports": (C.c_int, [vp, u32, vp]),
         "cymg_prepare": (C.c_int, [vp, vp]),
         "cymg_add_pure_force": (C.c_int, [vp, u32, dp]),
         "cymg_inter_link_ee": (C.c_int, [vp, u32, u32, C.c_double]),
@@ -211,6 +214,17 @@ class Meca:
                                         C.byref(mi)))
         self.np.append(int(nb_points))
         return mi.value
+
+    def set_systems(self, first_fiber):
+        ff = np.ascontiguousarray(first_fiber, dtype=np.uint32)
+        self._chk(self.L.cymg_set_systems(self.h, ff.size, _p(ff)))
+        self.nsys = int(ff.size)
+
+    def system_reports(self):
+        n = getattr(self, "nsys", 1)
+        arr = (SolveReport * n)()
+        self._chk(self.L.cymg_get_system_reports(self.h, n, C.cast(arr, C.c_void_p)))
+        return [arr[i].as_dict() for i in range(n)]
 
     @property
     def nb_points(self):
@@ -379,6 +393,18 @@ class Meca:
         t = Timings()
         self._chk(self.L.cymg_get_timings(self.h, C.byref(t)))
         return t.as_dict()
+
+
+def load_batch(meca, batch, couples_on_device=True):
+    """Register a Batch of independent systems and run one step's assembly."""
+    sc = batch.scene
+    meca.clear()
+    for f in range(sc.nfib):
+        meca.add_fiber(sc.nb_points[f], sc.segmentation[f], sc.rigidity[f])
+    meca.set_systems(batch.first_fiber)
+    meca.set_params(**sc.params())
+    meca.prepare(sc.points)
+    assemble(meca, sc, couples_on_device, True)
 
 
 def load_scene(meca, scene, couples_on_device=True, upload_couples=True):

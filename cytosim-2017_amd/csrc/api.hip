@@ -73,14 +73,37 @@ int mecable_of(const Context& c, uint32_t a)
     return (int)(it - c.h_idx.begin()) - 1;
 }
 
+int system_of_fiber(const Context& c, uint32_t f)
+{
+    if (c.h_sys_begin.size() < 2) return 0;
+    auto it = std::upper_bound(c.h_sys_begin.begin(), c.h_sys_begin.end(), f);
+    return (int)(it - c.h_sys_begin.begin()) - 1;
+}
+
+// interactions may not couple two independent systems
+void same_system(const Context& c, std::initializer_list<uint32_t> pts)
+{
+    if (c.h_sys_begin.size() < 3) return;
+    int s0 = -1;
+    for (uint32_t p : pts) {
+        const int s = system_of_fiber(c, (uint32_t)mecable_of(c, p));
+        if (s0 < 0) s0 = s;
+        else if (s != s0) throw Error(CYMG_ERR_INVALID, "interaction couples two independent systems");
+    }
+}
+
 void upload_layout(Context& c)
 {
     const uint32_t F = c.nfib;
     c.d_np.resize(F); c.d_idx.resize(F); c.d_seg.resize(F); c.d_cut.resize(F); c.d_rig.resize(F);
     c.d_mob.resize(F); c.d_rfrig.resize(F); c.d_blkoff.resize(F + 1); c.d_fib_sys.resize(F);
-    c.d_sys_begin.resize(2);
+    std::vector<uint32_t> sb = c.h_sys_begin;
+    if (sb.size() < 2 || sb.back() != F) sb = { 0, F };
+    c.nsys = (uint32_t)sb.size() - 1;
+    c.d_sys_begin.resize(sb.size());
     std::vector<uint32_t> sysv(F, 0);
-    uint32_t sb[2] = { 0, F };
+    for (uint32_t s = 0; s < c.nsys; ++s)
+        for (uint32_t f = sb[s]; f < sb[s + 1]; ++f) sysv[f] = s;
     hipStream_t st = c.stream;
     CYMG_HIP(hipMemcpyAsync(c.d_np.get(), c.h_np.data(), F * 4, hipMemcpyHostToDevice, st));
     CYMG_HIP(hipMemcpyAsync(c.d_idx.get(), c.h_idx.data(), F * 4, hipMemcpyHostToDevice, st));
@@ -89,7 +112,7 @@ void upload_layout(Context& c)
     CYMG_HIP(hipMemcpyAsync(c.d_rig.get(), c.h_rig.data(), F * 8, hipMemcpyHostToDevice, st));
     CYMG_HIP(hipMemcpyAsync(c.d_blkoff.get(), c.h_blkoff.data(), (F + 1) * 8, hipMemcpyHostToDevice, st));
     CYMG_HIP(hipMemcpyAsync(c.d_fib_sys.get(), sysv.data(), F * 4, hipMemcpyHostToDevice, st));
-    CYMG_HIP(hipMemcpyAsync(c.d_sys_begin.get(), sb, 8, hipMemcpyHostToDevice, st));
+    CYMG_HIP(hipMemcpyAsync(c.d_sys_begin.get(), sb.data(), sb.size() * 4, hipMemcpyHostToDevice, st));
     const size_t n = c.n();
     for (DevBuf<double>* v : { &c.d_pts, &c.d_pos, &c.d_sol, &c.d_bas, &c.d_rhs, &c.d_for, &c.d_tmp, &c.d_prev_sol })
         v->resize(n ? n : 1);
@@ -285,6 +308,7 @@ int cymg_clear(cymg_context* ctx)
         c.h_blkoff.assign(1, 0);
         c.nfib = c.npts = c.nseg = c.largest = 0;
         c.blk_total = 0;
+        c.h_sys_begin.clear();
         c.layout_dirty = true;
         c.prepared = c.setup_done = c.pc_ready = false;
     });
@@ -314,6 +338,29 @@ int cymg_add_fiber(cymg_context* ctx, uint32_t np, double segmentation, double r
         c.nfib += 1;
         c.layout_dirty = true;
         c.prepared = c.setup_done = c.pc_ready = false;
+    });
+}
+
+int cymg_set_systems(cymg_context* ctx, uint32_t nsys, const uint32_t* first_fiber)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (nsys < 1 || nsys > 4096) throw Error(CYMG_ERR_UNSUPPORTED, "between 1 and 4096 systems");
+        std::vector<uint32_t> sb(first_fiber, first_fiber + nsys);
+        sb.push_back(c.nfib);
+        if (sb[0] != 0) throw Error(CYMG_ERR_INVALID, "the first system must start at fiber 0");
+        for (uint32_t s = 0; s < nsys; ++s)
+            if (sb[s + 1] <= sb[s]) throw Error(CYMG_ERR_INVALID, "systems must be non-empty and in order");
+        c.h_sys_begin = sb;
+        c.layout_dirty = true;
+        c.prepared = c.setup_done = c.pc_ready = false;
+    });
+}
+
+int cymg_get_system_reports(cymg_context* ctx, uint32_t nsys, cymg_solve_report* out)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!out || nsys != c.sys_reports.size()) throw Error(CYMG_ERR_INVALID, "nsys must match the last solve");
+        for (uint32_t s = 0; s < nsys; ++s) out[s] = c.sys_reports[s];
     });
 }
 
@@ -363,6 +410,7 @@ int cymg_inter_link_ee(cymg_context* ctx, uint32_t a, uint32_t b, double w)
     return guarded(ctx, [&](Context& c) {
         need_prepared(c); check_point(c, a); check_point(c, b);
         if (!(w >= 0)) throw Error(CYMG_ERR_INVALID, "weight must be >= 0");
+        same_system(c, { a, b });
         if (mecable_of(c, a) == mecable_of(c, b) && (a == b || a == b + 1 || a + 1 == b)) return;
         addB(c, a, a, -w);
         addB(c, a, b, +w);
@@ -375,6 +423,7 @@ int cymg_inter_link_ie(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, u
     return guarded(ctx, [&](Context& c) {
         need_prepared(c); check_point(c, a1); check_point(c, a2); check_point(c, b);
         if (!(w >= 0)) throw Error(CYMG_ERR_INVALID, "weight must be >= 0");
+        same_system(c, { a1, a2, b });
         if (mecable_of(c, a1) == mecable_of(c, b) && (a1 == b || a2 == b)) return;
         const double cc[3] = { 1.0 - ca, ca, -1.0 };
         const double cw[3] = { w * cc[0], w * cc[1], -w };
@@ -390,6 +439,7 @@ int cymg_inter_link_ii(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, u
     return guarded(ctx, [&](Context& c) {
         need_prepared(c); check_point(c, a1); check_point(c, a2); check_point(c, b1); check_point(c, b2);
         if (!(w >= 0)) throw Error(CYMG_ERR_INVALID, "weight must be >= 0");
+        same_system(c, { a1, a2, b1, b2 });
         if (mecable_of(c, a1) == mecable_of(c, b1) && (a1 == b1 || a1 == b2 || a2 == b1 || a2 == b2)) return;
         const double cc[4] = { 1.0 - ca, ca, -(1.0 - cb), -cb };
         const double cw[4] = { w * cc[0], w * cc[1], w * cc[2], w * cc[3] };
@@ -410,6 +460,7 @@ int cymg_inter_long_link_ee(cymg_context* ctx, uint32_t a, uint32_t b, double le
     return guarded(ctx, [&](Context& c) {
         need_prepared(c); check_point(c, a); check_point(c, b);
         if (!(w >= 0)) throw Error(CYMG_ERR_INVALID, "weight must be >= 0");
+        same_system(c, { a, b });
         if (mecable_of(c, a) == mecable_of(c, b) && (a == b || a == b + 1 || a + 1 == b)) return;
         const int D = c.dim;
         double pa[3], pb[3], ab[3] = { 0, 0, 0 };
@@ -534,6 +585,8 @@ int cymg_set_couples(cymg_context* ctx, uint32_t n, const uint32_t* fa, const do
         for (uint32_t k = 0; k < n; ++k) {
             if (fa[k] >= c.nfib || fb[k] >= c.nfib) throw Error(CYMG_ERR_INVALID, "couple on an unknown fiber");
             if (!(w[k] >= 0)) throw Error(CYMG_ERR_INVALID, "couple stiffness must be >= 0");
+            if (system_of_fiber(c, fa[k]) != system_of_fiber(c, fb[k]))
+                throw Error(CYMG_ERR_INVALID, "couple links two independent systems");
         }
         c.ncouples = n;
         const size_t m = n ? n : 1;

@@ -21,6 +21,8 @@ struct Context {
     uint64_t blk_total = 0;
     bool layout_dirty = true;
     uint32_t nsys = 1;
+    std::vector<uint32_t> h_sys_begin;   // first fiber of each system, nsys+1 entries (empty: one system)
+    std::vector<cymg_solve_report> sys_reports;
 
     // per-fiber device arrays
     DevBuf<uint32_t> d_np, d_idx, d_seg, d_fib_sys, d_sys_begin;

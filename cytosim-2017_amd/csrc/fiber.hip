@@ -417,13 +417,82 @@ __global__ __launch_bounds__(kWave) void k_reshape(const uint32_t* __restrict__ 
         }
         return;
     }
+    const double a2 = cut * cut;
+    if (ns <= kWave) {
+        // Newton on the wave: lane p owns segment p, neighbours through shuffles,
+        // the tridiagonal Jacobian solved by cyclic reduction (the dgtsv system;
+        // no row interchanges are needed, the Jacobian is diagonally dominant
+        // near the constraint)
+        const int p = threadIdx.x;
+        const bool on = p < ns;
+        double df[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) df[d] = on ? X[D * p + D + d] - X[D * p + d] : 0.0;
+        double sc = 0.0;
+        int ok = 0;
+        for (int it = 0; it <= 32; ++it) {
+            const double scm = __shfl_up(sc, 1, kWave), scp = __shfl_down(sc, 1, kWave);
+            double dm[D], dp[D], vec[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                dm[d] = __shfl_up(df[d], 1, kWave);
+                dp[d] = __shfl_down(df[d], 1, kWave);
+            }
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                double v = (1 - 2 * sc) * df[d];
+                if (p > 0) v = scm * dm[d] + v;
+                if (p + 1 < ns) v = v + scp * dp[d];
+                vec[d] = v;
+            }
+            double vv = 0, vd = 0, vl = 0, vu = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                vv += vec[d] * vec[d];
+                vd += vec[d] * df[d];
+                vl += vec[d] * dm[d];
+                vu += vec[d] * dp[d];
+            }
+            const double val = vv - a2;
+            const double dia = -2 * vd;
+            const double low = (p > 0) ? vl : 0.0;
+            const double upe = (p + 1 < ns) ? vu : 0.0;
+            const double x = pcr_solve(ns, low, on ? dia : 1.0, upe, on ? val : 0.0);
+            sc += on ? -0.5 * x : 0.0;
+            double err = wave_sum(on ? fabs(x) : 0.0);
+            if (!(err == err) || err > 1e300) break;        // singular system: reshape_sure
+            if (it + 1 > 32) break;                           // iteration cap of reshape_it
+            if (!(err > 0.001)) { ok = 1; break; }
+        }
+        if (ok) {
+            double e[D], dd[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                dd[d] = sc * df[d];
+                e[d] = __shfl_up(dd[d], 1, kWave);
+            }
+            if (on) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    double v = X[D * p + d] + dd[d];
+                    if (p > 0) v = v - e[d];
+                    Y[D * p + d] = v;
+                }
+                if (p == ns - 1)
+#pragma unroll
+                    for (int d = 0; d < D; ++d) Y[D * ns + d] = X[D * ns + d] - dd[d];
+            }
+            return;
+        }
+        if (threadIdx.x == 0) fail = 1;
+        __syncthreads();
+    } else {
     for (int p = threadIdx.x; p < ns; p += kWave) {
         for (int d = 0; d < 3; ++d) dif[3 * p + d] = d < D ? X[D * p + D + d] - X[D * p + d] : 0.0;
         sca[p] = 0.0;
     }
     if (threadIdx.x == 0) fail = 0;
     __syncthreads();
-    const double a2 = cut * cut;
     int it = 0;
     while (true) {
         for (int p = threadIdx.x; p < ns; p += kWave) {
@@ -535,6 +604,7 @@ __global__ __launch_bounds__(kWave) void k_reshape(const uint32_t* __restrict__ 
             for (int d = 0; d < D; ++d) Y[D * ns + d] = X[D * ns + d] - dd[d];
         }
         return;
+    }
     }
     // reshape_sure on the unmodified points
     if (threadIdx.x == 0) {

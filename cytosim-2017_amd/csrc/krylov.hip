@@ -49,6 +49,7 @@ struct KArgs {
     SysScalars* st_in;
     SysScalars* st_out;
     uint32_t nfib;
+    uint32_t nsys;
 };
 
 // Reductions over the partials of one system (fibers [fb, fe)), repeated by
@@ -145,7 +146,7 @@ __global__ __launch_bounds__(kWave) void k_init(KArgs k, const double* __restric
     }
     rr = wave_sum(rr);
     if (threadIdx.x == 0) k.part[P_RR * k.nfib + f] = rr;
-    const uint32_t sys = a.fib_sys[f];
+    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
     if (first_of_system(a, sys)) {
         const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
         const double noise = wmin(k.part + P_NOISE * k.nfib, fb, fe);
@@ -171,13 +172,29 @@ __global__ __launch_bounds__(kWave) void k_init(KArgs k, const double* __restric
 template <int D, int PC>
 __global__ __launch_bounds__(kApplyThreads) void k_iter_a(KArgs k)
 {
+    constexpr int R = (kMaxFiberPoints * D + kApplyThreads - 1) / kApplyThreads;
     __shared__ double xs[kMaxFiberPoints * D];
     __shared__ double red[kApplyWaves * kMaxFiberPoints * D];
     __shared__ double wr[kApplyWaves];
     __shared__ double bc[4];
     const uint32_t f = blockIdx.x;
     const FiberArgs& a = k.fa;
-    const uint32_t sys = a.fib_sys[f];
+    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
+    const uint32_t np = a.np[f], o = D * a.idx[f];
+    const int n = D * (int)np;
+    // the fiber's vectors are loaded while the scalar state is derived
+    double vr[R], vt[R], vs[R], vp[R], vv[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int i = threadIdx.x + q * kApplyThreads;
+        if (i < n) {
+            vr[q] = k.r[o + i];
+            vt[q] = k.t[o + i];
+            vs[q] = PC ? k.sh[o + i] : vr[q];
+            vp[q] = k.p[o + i];
+            vv[q] = k.v[o + i];
+        }
+    }
     SysScalars sc = k.st_in[sys];
     const bool first = first_of_system(a, sys);
     if (sc.done) {
@@ -223,22 +240,22 @@ __global__ __launch_bounds__(kApplyThreads) void k_iter_a(KArgs k)
     sc.stage = ST_A;
     if (first && threadIdx.x == 0) k.st_out[sys] = sc;
     if (sc.done) return;
-    const uint32_t np = a.np[f], o = D * a.idx[f];
-    const int n = D * (int)np;
     double rmax = 0.0;
-    for (int i = threadIdx.x; i < n; i += kApplyThreads) {
-        double ri = k.r[o + i];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int i = threadIdx.x + q * kApplyThreads;
+        if (i >= n) continue;
+        double ri = vr[q];
         if (sc.pending) {
             // x += omega shat (BCGS: shat = s), r = s - omega t
-            const double sh = PC ? k.sh[o + i] : ri;
-            k.x[o + i] = k.x[o + i] + sc.omega * sh;
-            ri = ri + (-sc.omega) * k.t[o + i];
+            k.x[o + i] = k.x[o + i] + sc.omega * vs[q];
+            ri = ri + (-sc.omega) * vt[q];
             k.r[o + i] = ri;
         }
         rmax = nan_max(rmax, fabs(ri));
         double pi;
         if (sc.beta == 0.0) pi = ri;
-        else pi = sc.beta * (k.p[o + i] + (-sc.omega) * k.v[o + i]) + ri;
+        else pi = sc.beta * (vp[q] + (-sc.omega) * vv[q]) + ri;
         k.p[o + i] = pi;
         xs[i] = pi;
     }
@@ -274,7 +291,12 @@ __global__ __launch_bounds__(kWave) void k_iter_b(KArgs k)
     __shared__ FiberLds<D> s;
         const uint32_t f = blockIdx.x;
     const FiberArgs& a = k.fa;
-    const uint32_t sys = a.fib_sys[f];
+    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
+    // the operand slice is loaded while the scalar state is derived
+    const double* X = PC ? k.ph : k.p;
+    const uint32_t np = a.np[f], o = D * a.idx[f];
+    const int n = D * (int)np;
+    for (int i = threadIdx.x; i < n; i += kWave) s.x[i] = X[o + i];
     SysScalars sc = k.st_in[sys];
     const bool first = first_of_system(a, sys);
     if (!sc.done) {
@@ -284,10 +306,6 @@ __global__ __launch_bounds__(kWave) void k_iter_b(KArgs k)
     }
     if (first && threadIdx.x == 0) k.st_out[sys] = sc;
     if (sc.done) return;
-    const double* X = PC ? k.ph : k.p;
-    const uint32_t np = a.np[f], o = D * a.idx[f];
-    const int n = D * (int)np;
-    for (int i = threadIdx.x; i < n; i += kWave) s.x[i] = X[o + i];
     load_factors<D>(a, s, a.seg[f], (int)np - 1);
     __syncthreads();
     fiber_operator<D>(a, s, X, f);
@@ -303,13 +321,28 @@ __global__ __launch_bounds__(kWave) void k_iter_b(KArgs k)
 template <int D, int PC>
 __global__ __launch_bounds__(kApplyThreads) void k_iter_c(KArgs k)
 {
+    constexpr int R = (kMaxFiberPoints * D + kApplyThreads - 1) / kApplyThreads;
     __shared__ double xs[kMaxFiberPoints * D];
     __shared__ double red[kApplyWaves * kMaxFiberPoints * D];
     __shared__ double wr[kApplyWaves];
     __shared__ double bc[4];
     const uint32_t f = blockIdx.x;
     const FiberArgs& a = k.fa;
-    const uint32_t sys = a.fib_sys[f];
+    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
+    const double* X = PC ? k.ph : k.p;
+    const uint32_t np = a.np[f], o = D * a.idx[f];
+    const int n = D * (int)np;
+    double vr[R], vv[R], vx[R], vX[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int i = threadIdx.x + q * kApplyThreads;
+        if (i < n) {
+            vr[q] = k.r[o + i];
+            vv[q] = k.v[o + i];
+            vx[q] = k.x[o + i];
+            vX[q] = X[o + i];
+        }
+    }
     SysScalars sc = k.st_in[sys];
     const bool first = first_of_system(a, sys);
     if (!sc.done) {
@@ -326,14 +359,14 @@ __global__ __launch_bounds__(kApplyThreads) void k_iter_c(KArgs k)
     }
     if (first && threadIdx.x == 0) k.st_out[sys] = sc;
     if (sc.done) return;
-    const double* X = PC ? k.ph : k.p;
-    const uint32_t np = a.np[f], o = D * a.idx[f];
-    const int n = D * (int)np;
     double smax = 0.0;
-    for (int i = threadIdx.x; i < n; i += kApplyThreads) {
-        const double si = k.r[o + i] + (-sc.alpha) * k.v[o + i];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int i = threadIdx.x + q * kApplyThreads;
+        if (i >= n) continue;
+        const double si = vr[q] + (-sc.alpha) * vv[q];
         k.r[o + i] = si;
-        k.x[o + i] = k.x[o + i] + sc.alpha * X[o + i];
+        k.x[o + i] = vx[q] + sc.alpha * vX[q];
         smax = nan_max(smax, fabs(si));
         xs[i] = si;
     }
@@ -354,7 +387,11 @@ __global__ __launch_bounds__(kWave) void k_iter_d(KArgs k)
     __shared__ FiberLds<D> s;
         const uint32_t f = blockIdx.x;
     const FiberArgs& a = k.fa;
-    const uint32_t sys = a.fib_sys[f];
+    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
+    const double* X = PC ? k.sh : k.r;
+    const uint32_t np = a.np[f], o = D * a.idx[f];
+    const int n = D * (int)np;
+    for (int i = threadIdx.x; i < n; i += kWave) s.x[i] = X[o + i];
     SysScalars sc = k.st_in[sys];
     const bool first = first_of_system(a, sys);
     if (!sc.done) {
@@ -365,10 +402,6 @@ __global__ __launch_bounds__(kWave) void k_iter_d(KArgs k)
     }
     if (first && threadIdx.x == 0) k.st_out[sys] = sc;
     if (sc.done) return;
-    const double* X = PC ? k.sh : k.r;
-    const uint32_t np = a.np[f], o = D * a.idx[f];
-    const int n = D * (int)np;
-    for (int i = threadIdx.x; i < n; i += kWave) s.x[i] = X[o + i];
     load_factors<D>(a, s, a.seg[f], (int)np - 1);
     __syncthreads();
     fiber_operator<D>(a, s, X, f);
@@ -413,6 +446,7 @@ static KArgs make_kargs(Context& c)
     k.st_in = nullptr;
     k.st_out = nullptr;
     k.nfib = c.nfib;
+    k.nsys = c.nsys;
     return k;
 }
 
@@ -461,16 +495,27 @@ static void run(Context& c, const double* guess, int max_iter, double tol, int t
         launched += chunk;
         chunk = predicted > 0 ? std::max(4, predicted / 8) : std::min(2 * chunk, 16);
     }
-    // report system 0 (single-system contexts); batched callers read per system
-    const SysScalars& w = c.h_scal[0];
-    rep->iterations = w.iter;
-    rep->flag = w.flag;
-    rep->residual = w.resid;
-    rep->threshold = w.tol;
-    rep->converged = w.resid < w.tol;
-    rep->noise_level = w.noise * c.prm.time_step;
-    rep->precond_used = PC;
-    predicted = w.iter;
+    // per-system reports; the summary is the worst system (most iterations,
+    // any failure), system 0 when there is one system
+    c.sys_reports.assign(c.nsys, cymg_solve_report{});
+    int worst = 0, all_conv = 1;
+    for (uint32_t s = 0; s < c.nsys; ++s) {
+        const SysScalars& w = c.h_scal[s];
+        cymg_solve_report& r = c.sys_reports[s];
+        r.iterations = w.iter;
+        r.total_iterations = w.iter;
+        r.flag = w.flag;
+        r.residual = w.resid;
+        r.threshold = w.tol;
+        r.converged = w.resid < w.tol;
+        r.noise_level = w.noise * c.prm.time_step;
+        r.precond_used = PC;
+        all_conv = all_conv && r.converged;
+        if (w.iter > c.h_scal[worst].iter || (!r.converged && c.sys_reports[worst].converged)) worst = (int)s;
+    }
+    *rep = c.sys_reports[worst];
+    rep->converged = all_conv;
+    predicted = c.h_scal[worst].iter;
 }
 
 void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter, double tol, int tol_mode,

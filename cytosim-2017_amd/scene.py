@@ -164,7 +164,7 @@ def _segment_segment(p0, d0, p1, d1):
     return s, t, np.linalg.norm(q0 - q1, axis=1)
 
 
-def _close_segment_pairs(dim, nbp, points, rng_range):
+def _close_segment_pairs(dim, nbp, points, rng_range, return_params=False):
     """All pairs of segments of different fibers closer than rng_range."""
     P = points.reshape(-1, dim)
     nbp = nbp.astype(np.int64)
@@ -196,7 +196,8 @@ def _close_segment_pairs(dim, nbp, points, rng_range):
         hi = np.searchsorted(slk, nk, side="right")
         cnt = hi - lo
         ii = np.repeat(np.arange(lk.size), cnt)
-        jj = order[np.concatenate([np.arange(l, h) for l, h in zip(lo, hi)])] if cnt.sum() else np.zeros(0, np.int64)
+        within = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+        jj = order[np.repeat(lo, cnt) + within]
         keep = (ii < jj) & (seg_fib[ii] != seg_fib[jj])
         pairs_i.append(ii[keep])
         pairs_j.append(jj[keep])
@@ -204,6 +205,8 @@ def _close_segment_pairs(dim, nbp, points, rng_range):
     jj = np.concatenate(pairs_j)
     s, t, dist = _segment_segment(A[ii], B[ii] - A[ii], A[jj], B[jj] - A[jj])
     keep = dist <= rng_range
+    if return_params:
+        return seg_pt, seg_fib, ii[keep], jj[keep], s[keep], t[keep]
     return seg_pt, seg_fib, ii[keep], jj[keep]
 
 
@@ -296,3 +299,104 @@ def mini(dim=3, seed=3, nfib=8, nlinks=20, radius=1.5, lmin=2.0, lmax=6.0, bindi
         sc.link_fiber_b, sc.link_abs_b = fb, ab
         sc.link_stiffness = np.full(nlinks, 100.0)
     return sc
+
+
+def _bind_crosslinkers_fast(rng, dim, nbp, cut, points, nlinks, binding_range):
+    """Vectorised variant of _bind_crosslinkers for large scenes: hand 1 at a
+    uniform abscissa within binding range of the closest approach of a random
+    close segment pair, hand 2 at the nearest point of the partner segment;
+    kept when the hands are within binding range."""
+    P = points.reshape(-1, dim)
+    seg_pt, seg_fib, ci, cj, cs, ct = _close_segment_pairs(dim, nbp, points, binding_range, True)
+    if ci.size == 0:
+        raise ValueError("no fiber crossings within binding range")
+    nbp = nbp.astype(np.int64)
+    starts = np.concatenate([[0], np.cumsum(nbp)[:-1]]).astype(np.int64)
+    out = [[], [], [], []]
+    got = 0
+    while got < nlinks:
+        m = 3 * (nlinks - got) + 64
+        k = rng.integers(0, ci.size, m)
+        flip = rng.random(m) < 0.5
+        sa = np.where(flip, cj[k], ci[k])
+        sb = np.where(flip, ci[k], cj[k])
+        s0 = np.where(flip, ct[k], cs[k])
+        la = np.linalg.norm(P[seg_pt[sa] + 1] - P[seg_pt[sa]], axis=1)
+        u = np.clip(s0 + rng.uniform(-1, 1, m) * binding_range / la, 0.0, 1.0)
+        pa = P[seg_pt[sa]] + u[:, None] * (P[seg_pt[sa] + 1] - P[seg_pt[sa]])
+        q0 = P[seg_pt[sb]]
+        d = P[seg_pt[sb] + 1] - q0
+        t = np.clip(((pa - q0) * d).sum(1) / (d * d).sum(1), 0.0, 1.0)
+        dist = np.linalg.norm(q0 + t[:, None] * d - pa, axis=1)
+        ok = np.nonzero(dist <= binding_range)[0][: nlinks - got]
+        f1 = seg_fib[sa[ok]]
+        f2 = seg_fib[sb[ok]]
+        out[0].append(f1)
+        out[1].append((seg_pt[sa[ok]] - starts[f1] + u[ok]) * cut[f1])
+        out[2].append(f2)
+        out[3].append((seg_pt[sb[ok]] - starts[f2] + t[ok]) * cut[f2])
+        got += ok.size
+    return (np.concatenate(out[0]).astype(np.uint32), np.concatenate(out[1]),
+            np.concatenate(out[2]).astype(np.uint32), np.concatenate(out[3]))
+
+
+@dataclass
+class Batch:
+    """Independent systems solved together: one Scene holding every fiber, and
+    the first fiber of each system."""
+    scene: Scene
+    first_fiber: np.ndarray
+    systems: list
+
+    @property
+    def nsys(self):
+        return int(self.first_fiber.size)
+
+
+def concatenate(systems, name="batch"):
+    """Stack independent scenes (same dim and properties) into one Batch."""
+    s0 = systems[0]
+    nfib = np.array([sc.nfib for sc in systems])
+    first = np.concatenate([[0], np.cumsum(nfib)[:-1]]).astype(np.uint32)
+    cat = lambda attr: np.concatenate([getattr(sc, attr) for sc in systems])
+    big = Scene(dim=s0.dim, time_step=s0.time_step, kT=s0.kT, viscosity=s0.viscosity,
+                hydro_radius0=s0.hydro_radius0, hydro_radius1=s0.hydro_radius1,
+                nb_points=cat("nb_points").astype(np.uint32), segmentation=cat("segmentation"),
+                rigidity=cat("rigidity"), points=cat("points"),
+                confine_radius=s0.confine_radius, confine_stiffness=s0.confine_stiffness,
+                link_fiber_a=np.concatenate([sc.link_fiber_a + f for sc, f in zip(systems, first)]).astype(np.uint32),
+                link_abs_a=cat("link_abs_a"),
+                link_fiber_b=np.concatenate([sc.link_fiber_b + f for sc, f in zip(systems, first)]).astype(np.uint32),
+                link_abs_b=cat("link_abs_b"), link_stiffness=cat("link_stiffness"),
+                seed=s0.seed, name=name)
+    big.gauss = lambda step: np.concatenate([sc.gauss(step) for sc in systems])
+    return Batch(scene=big, first_fiber=first, systems=systems)
+
+
+def config5(seed=5, nsys=64, fmin=50, fmax=2000, links_min=5, links_max=30, radius=10.0):
+    """64 independent 3D systems: fiber counts uniform in [fmin, fmax], lengths
+    uniform in [2, 15] um, 5-30 crosslinkers per fiber, other properties as the
+    1000-fiber crosslinked scene."""
+    rng = np.random.default_rng(seed)
+    systems = []
+    for k in range(nsys):
+        nf = int(rng.integers(fmin, fmax + 1))
+        srng = np.random.default_rng([seed, k])
+        lengths = srng.uniform(2.0, 15.0, nf)
+        nbp, cut, pts = _straight_fibers(srng, 3, lengths, 0.5, radius)
+        nl = int(srng.integers(links_min, links_max + 1)) * nf // 2
+        sc = Scene(dim=3, time_step=0.005, kT=0.0042, viscosity=0.02, hydro_radius0=0.0125,
+                   hydro_radius1=10.0, nb_points=nbp, segmentation=cut, rigidity=np.full(nf, 20.0),
+                   points=pts, confine_radius=radius, confine_stiffness=200.0, seed=seed * 1000 + k,
+                   name=f"system-{k}")
+        fa, aa, fb, ab = _bind_crosslinkers_fast(srng, 3, nbp, cut, pts, nl, 0.05)
+        sc.link_fiber_a, sc.link_abs_a, sc.link_fiber_b, sc.link_abs_b = fa, aa, fb, ab
+        sc.link_stiffness = np.full(nl, 100.0)
+        systems.append(sc)
+    return concatenate(systems, name=f"batch-{nsys}-systems")
+
+
+def mini_batch(seed=7, nsys=4, dim=3):
+    """Small batch of independent crosslinked scenes for parity checks."""
+    systems = [mini(dim=dim, seed=seed + 13 * k, nfib=5 + 3 * k, nlinks=10 + 5 * k) for k in range(nsys)]
+    return concatenate(systems, name=f"mini-batch-{nsys}")

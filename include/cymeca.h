@@ -94,6 +94,14 @@ int         cymg_clear(cymg_context* ctx);
 int         cymg_add_fiber(cymg_context* ctx, uint32_t nb_points, double segmentation,
                            double rigidity, uint32_t* mat_index);
 int         cymg_nb_points(const cymg_context* ctx, uint32_t* nb_points);
+/* independent systems solved together (no counterpart in the reference, which
+   solves one system per Meca): fibers [first_fiber[s], first_fiber[s+1]) form
+   system s, with its own Krylov scalars, threshold (noise level or |b|_inf) and
+   stop test; interactions may not couple two systems.  Call after the add()s. */
+int         cymg_set_systems(cymg_context* ctx, uint32_t nsys, const uint32_t* first_fiber);
+/* per-system reports of the last solve (the solve's own report is the system
+   with the most iterations, converged only if every system converged) */
+int         cymg_get_system_reports(cymg_context* ctx, uint32_t nsys, cymg_solve_report* out);
 
 /* Meca::prepare (meca.cc:488-537) with Fiber::prepareMecable (fiber.cc:450-460).
    points: host array of the Mecables' points (PointSet::putPoints), or NULL to
