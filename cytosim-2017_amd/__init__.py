@@ -34,8 +34,9 @@ EXPORTED = [
     "cymg_setup", "cymg_iterate", "cymg_apply", "cymg_multiply", "cymg_compute_preconditioner",
     "cymg_precondition", "cymg_compute_forces", "cymg_multiply_device", "cymg_precondition_device",
     "cymg_get_vector", "cymg_get_points", "cymg_get_tensions", "cymg_get_block", "cymg_get_matrix",
-    "cymg_get_timings", "cymg_enable_timing", "cymg_time_kernel",
+    "cymg_get_timings", "cymg_enable_timing", "cymg_time_kernel", "cymg_set_option",
 ]
+OPT_KRYLOV_LAUNCHES, OPT_BLOCK_INVERSE = 1, 2
 
 
 class SimParams(C.Structure):
@@ -143,6 +144,7 @@ def lib():
         "cymg_get_timings": (C.c_int, [vp, C.POINTER(Timings)]),
         "cymg_enable_timing": (C.c_int, [vp, C.c_int]),
         "cymg_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+        "cymg_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -383,6 +385,9 @@ class Meca:
 
     def enable_timing(self, on=True):
         self._chk(self.L.cymg_enable_timing(self.h, int(on)))
+
+    def set_option(self, option, value):
+        self._chk(self.L.cymg_set_option(self.h, option, value))
 
     def time_kernel(self, which, reps=50):
         ms = C.c_double()

@@ -807,6 +807,16 @@ int cymg_get_matrix(cymg_context* ctx, int which, uint32_t* nnz, uint32_t* rows,
     });
 }
 
+int cymg_set_option(cymg_context* ctx, int option, int value)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (option == CYMG_OPT_KRYLOV_LAUNCHES && (value == 2 || value == 4)) c.krylov_fused = value == 2;
+        else if (option == CYMG_OPT_BLOCK_INVERSE && (value == 0 || value == 1)) c.pc_method = value;
+        else throw Error(CYMG_ERR_INVALID, "unknown option or value");
+        c.last_iters[0] = c.last_iters[1] = c.last_iters[2] = 0;
+    });
+}
+
 int cymg_enable_timing(cymg_context* ctx, int enable)
 {
     return guarded(ctx, [&](Context& c) { c.timing = enable != 0; });

@@ -61,6 +61,8 @@ struct Context {
 
     // Krylov work vectors and scalars
     DevBuf<double> k_r, k_rt, k_p, k_v, k_t, k_ph, k_sh;
+    DevBuf<double> k_y, k_z, k_z2, k_w, k_ph2;   // fused variant: Binv r, Binv v (2), Binv t, phat (2)
+    int krylov_fused = 1;              // 1: two launches per iteration, 0: four
     DevBuf<double> d_part;             // partial reductions [slot][nfib]
     DevBuf<SysScalars> d_scal;
     SysScalars* h_scal = nullptr;      // pinned mirror

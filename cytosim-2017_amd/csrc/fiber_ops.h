@@ -112,7 +112,7 @@ __device__ __forceinline__ void load_factors(const FiberArgs& a, FiberLds<D>& s,
                                              bool force = false)
 {
     if (!force && S <= kWave) return;
-    for (int k = threadIdx.x; k < S; k += kWave) {
+    for (int k = threadIdx.x; k < S; k += blockDim.x) {
         s.d[k] = a.jd[seg + k];
         s.e[k] = (k + 1 < S) ? a.je[seg + k] : 0.0;
     }
@@ -120,13 +120,19 @@ __device__ __forceinline__ void load_factors(const FiberArgs& a, FiberLds<D>& s,
 
 // f_p = [R X]_p + [B X]_p + [C X]_p + [P' X]_p for the points of the fiber.
 // X: global operand (coupling gather); s.x must hold the fiber's slice of X.
-template <int D>
-__device__ __forceinline__ void fiber_forces(const FiberArgs& a, FiberLds<D>& s, const double* __restrict__ X,
-                                             uint32_t np, uint32_t idx, uint32_t seg, double k,
-                                             bool rigidity, bool with_pdiff)
+// operand read through a plain array
+struct Plain {
+    const double* __restrict__ X;
+    __device__ __forceinline__ double operator()(uint32_t j) const { return X[j]; }
+};
+
+template <int D, typename Op>
+__device__ __forceinline__ void fiber_forces_op(const FiberArgs& a, FiberLds<D>& s, const Op& X,
+                                                uint32_t np, uint32_t idx, uint32_t seg, double k,
+                                                bool rigidity, bool with_pdiff)
 {
     const int S = (int)np - 1;
-    for (int p = threadIdx.x; p < (int)np; p += kWave) {
+    for (int p = threadIdx.x; p < (int)np; p += blockDim.x) {
         double fp[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) fp[d] = 0.0;
@@ -142,14 +148,14 @@ __device__ __forceinline__ void fiber_forces(const FiberArgs& a, FiberLds<D>& s,
                 const double v = a.Bval[q];
                 const uint32_t c = a.Bcol[q];
 #pragma unroll
-                for (int d = 0; d < D; ++d) fp[d] += v * X[D * c + d];
+                for (int d = 0; d < D; ++d) fp[d] += v * X(D * c + d);
             }
         }
         if (a.Cuse) {
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 double acc = 0.0;
-                for (uint32_t q = cb[d]; q < cb[d + 1]; ++q) acc += a.Cval[q] * X[a.Ccol[q]];
+                for (uint32_t q = cb[d]; q < cb[d + 1]; ++q) acc += a.Cval[q] * X(a.Ccol[q]);
                 fp[d] += acc;
             }
         }
@@ -193,6 +199,14 @@ __device__ __forceinline__ void fiber_forces(const FiberArgs& a, FiberLds<D>& s,
     }
 }
 
+template <int D>
+__device__ __forceinline__ void fiber_forces(const FiberArgs& a, FiberLds<D>& s, const double* __restrict__ X,
+                                             uint32_t np, uint32_t idx, uint32_t seg, double k,
+                                             bool rigidity, bool with_pdiff)
+{
+    fiber_forces_op<D>(a, s, Plain{ X }, np, idx, seg, k, rigidity, with_pdiff);
+}
+
 // s.y = sca * P s.f   with P = I - J'(JJ')^-1 J; the multipliers (JJ')^-1 J f
 // are left in s.t (rigid_fiber_project.cc:95-149).  s.d/s.e hold the factors.
 template <int D>
@@ -213,7 +227,7 @@ __device__ __forceinline__ void fiber_project(const FiberArgs& a, FiberLds<D>& s
         const double lam = pcr_solve(S, sa, sb, sc, t);
         if (q < S) s.t[q] = lam;
     } else {
-        for (int q = threadIdx.x; q < S; q += kWave) {
+        for (int q = threadIdx.x; q < S; q += blockDim.x) {
             double t = 0.0;
 #pragma unroll
             for (int d = 0; d < D; ++d) t += a.diff[D * (seg + q) + d] * (s.f[D * (q + 1) + d] - s.f[D * q + d]);
@@ -223,7 +237,7 @@ __device__ __forceinline__ void fiber_project(const FiberArgs& a, FiberLds<D>& s
         if (threadIdx.x == 0) ptts2_serial(S, s.d, s.e, s.t);
     }
     __syncthreads();
-    for (int p = threadIdx.x; p < (int)np; p += kWave) {
+    for (int p = threadIdx.x; p < (int)np; p += blockDim.x) {
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             double v = s.f[D * p + d];
@@ -235,18 +249,24 @@ __device__ __forceinline__ void fiber_project(const FiberArgs& a, FiberLds<D>& s
 }
 
 // s.y = X - dt*mob*P (R + B + C + P') X  (Meca::multiply, meca.cc:157-181),
-// for one fiber; s.x must hold the fiber's slice of X.
+// for one fiber; s.x must hold the fiber's slice of X, X(j) gives any entry.
+template <int D, typename Op>
+__device__ __forceinline__ void fiber_operator_op(const FiberArgs& a, FiberLds<D>& s, const Op& X, uint32_t f)
+{
+    const uint32_t np = a.np[f], idx = a.idx[f], seg = a.seg[f];
+    fiber_forces_op<D>(a, s, X, np, idx, seg, a.rfrig[f], true, true);
+    __syncthreads();
+    fiber_project<D>(a, s, np, seg, -a.dt * a.mob[f]);
+    __syncthreads();
+    for (int i = threadIdx.x; i < (int)(D * np); i += blockDim.x) s.y[i] = s.y[i] + s.x[i];
+    __syncthreads();
+}
+
 template <int D>
 __device__ __forceinline__ void fiber_operator(const FiberArgs& a, FiberLds<D>& s, const double* __restrict__ X,
                                                uint32_t f)
 {
-    const uint32_t np = a.np[f], idx = a.idx[f], seg = a.seg[f];
-    fiber_forces<D>(a, s, X, np, idx, seg, a.rfrig[f], true, true);
-    __syncthreads();
-    fiber_project<D>(a, s, np, seg, -a.dt * a.mob[f]);
-    __syncthreads();
-    for (int i = threadIdx.x; i < (int)(D * np); i += kWave) s.y[i] = s.y[i] + s.x[i];
-    __syncthreads();
+    fiber_operator_op<D>(a, s, Plain{ X }, f);
 }
 
 }  // namespace cymg

@@ -44,6 +44,13 @@ struct KArgs {
     double* t;
     double* ph;
     double* sh;
+    // fused variant
+    double* y;          // Binv r
+    double* w;          // Binv t
+    double* z_in;       // Binv v of the previous iteration
+    double* z_out;
+    double* ph_in;      // phat of the previous iteration
+    double* ph_out;
     double* part;
     double* part4;      // [fiber][4] partials of D
     SysScalars* st_in;
@@ -424,6 +431,337 @@ __global__ __launch_bounds__(kWave) void k_iter_d(KArgs k)
         *reinterpret_cast<double4*>(k.part4 + 4 * (size_t)f) = make_double4(tt, ts, rts, rtt);
 }
 
+
+// ===========================================================================
+// Fused variant: two launches per iteration.  The preconditioned vectors are
+// carried by recurrences of the linear block inverse M^-1:
+//   y = M^-1 r, z = M^-1 v, w = M^-1 t,
+//   phat = M^-1 p = y + beta (phat_old - omega z_old),
+//   shat = M^-1 s = y - alpha z,   y_new = M^-1 r_new = shat - omega w,
+// so every preconditioned vector a coupling term needs is computed on the fly
+// from vectors the current launch does not modify, and the two fiber-local
+// phases of the four-launch scheme disappear.  Each launch applies the block
+// inverse once, to the operator result it has just computed.
+//   P: omega/rho/beta from Q's partials; [x += w shat; r = s - w t; y]; phat;
+//      v = M phat; z = M^-1 v; |r|_inf and <rt, v> partials
+//   Q: stop test on |r|_inf; alpha; s = r - alpha v; x += alpha phat;
+//      shat = y - alpha z; t = M shat; w = M^-1 t; the partials of D
+
+template <int PC>
+__device__ __forceinline__ void apply_or_copy(const KArgs& k, uint32_t f, int bs, const double* xs, double* y,
+                                              double* red)
+{
+    if constexpr (PC == CYMG_PRECOND_NONE)
+        block_apply_mw<double, kApplyWaves>(nullptr, bs, false, xs, y, red);
+    else
+        apply_block<PC>(k, f, bs, xs, y, red);
+}
+
+// phat(j) for any j, from vectors P does not write
+struct PhatOp {
+    const double* __restrict__ y;
+    const double* __restrict__ sh;
+    const double* __restrict__ w;
+    const double* __restrict__ ph;
+    const double* __restrict__ z;
+    double beta, omega;
+    int pending;
+    __device__ __forceinline__ double yv(uint32_t j) const { return pending ? sh[j] + (-omega) * w[j] : y[j]; }
+    __device__ __forceinline__ double operator()(uint32_t j) const
+    {
+        const double yj = yv(j);
+        return beta == 0.0 ? yj : yj + beta * (ph[j] + (-omega) * z[j]);
+    }
+};
+
+// shat(j) = y(j) - alpha z(j)
+struct ShatOp {
+    const double* __restrict__ y;
+    const double* __restrict__ z;
+    double alpha;
+    __device__ __forceinline__ double operator()(uint32_t j) const { return y[j] + (-alpha) * z[j]; }
+};
+
+template <int D, int PC>
+__global__ __launch_bounds__(kApplyThreads) void k_finit(KArgs k, const double* __restrict__ x0, double tolerance,
+                                                         int tol_mode, int iter_max, double dt)
+{
+    __shared__ FiberLds<D> s;
+    __shared__ double red[kApplyWaves * kMaxFiberPoints * D];
+    __shared__ double wr[kApplyWaves];
+    const uint32_t f = blockIdx.x;
+    const FiberArgs& a = k.fa;
+    const uint32_t np = a.np[f], o = D * a.idx[f];
+    const int n = D * (int)np;
+    if (x0) {
+        for (int i = threadIdx.x; i < n; i += kApplyThreads) s.x[i] = x0[o + i];
+        load_factors<D>(a, s, a.seg[f], (int)np - 1);
+        __syncthreads();
+        fiber_operator<D>(a, s, x0, f);
+    }
+    double rr = 0.0;
+    for (int i = threadIdx.x; i < n; i += kApplyThreads) {
+        double ri = k.b[o + i];
+        if (x0) ri = ri + (-1.0) * s.y[i];
+        k.r[o + i] = ri;
+        k.rt[o + i] = ri;
+        k.x[o + i] = x0 ? s.x[i] : 0.0;
+        s.f[i] = ri;
+        rr += ri * ri;
+    }
+    rr = wave_sum(rr);
+    if ((threadIdx.x & (kWave - 1)) == 0) wr[threadIdx.x / kWave] = rr;
+    __syncthreads();
+    apply_or_copy<PC>(k, f, n, s.f, k.y + o, red);
+    if (threadIdx.x == 0) {
+        double t = wr[0];
+        for (int q = 1; q < kApplyWaves; ++q) t += wr[q];
+        k.part[P_RR * k.nfib + f] = t;
+    }
+    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
+    if (first_of_system(a, sys) && threadIdx.x < kWave) {
+        const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
+        const double noise = wmin(k.part + P_NOISE * k.nfib, fb, fe);
+        const double bn = wmax(k.part + P_BNORM * k.nfib, fb, fe);
+        if (threadIdx.x == 0) {
+            SysScalars st{};
+            st.noise = noise;
+            st.bnorm = bn;
+            st.rho1 = 1.0;
+            st.alpha = 0.0;
+            st.omega = 1.0;
+            st.tol = (tol_mode == CYMG_TOL_RELATIVE) ? tolerance * bn : tolerance * (noise * dt);
+            st.resid = __builtin_inf();
+            st.resid_s = __builtin_inf();
+            st.iter = 0;
+            st.iter_max = iter_max;
+            st.stage = ST_INIT;
+            k.st_out[sys] = st;
+        }
+    }
+}
+
+template <int D, int PC>
+__global__ __launch_bounds__(kApplyThreads) void k_fp(KArgs k)
+{
+    constexpr int R = (kMaxFiberPoints * D + kApplyThreads - 1) / kApplyThreads;
+    __shared__ FiberLds<D> s;
+    __shared__ double red[kApplyWaves * kMaxFiberPoints * D];
+    __shared__ double wr[2 * kApplyWaves];
+    __shared__ double bc[5];
+    const uint32_t f = blockIdx.x;
+    const FiberArgs& a = k.fa;
+    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
+    const uint32_t np = a.np[f], o = D * a.idx[f];
+    const int n = D * (int)np;
+    double vr[R], vt[R], vs[R], vw[R], vy[R], vph[R], vz[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int i = threadIdx.x + q * kApplyThreads;
+        if (i < n) {
+            vr[q] = k.r[o + i];
+            vt[q] = k.t[o + i];
+            vs[q] = k.sh[o + i];
+            vw[q] = k.w[o + i];
+            vy[q] = k.y[o + i];
+            vph[q] = k.ph_in[o + i];
+            vz[q] = k.z_in[o + i];
+        }
+    }
+    SysScalars sc = k.st_in[sys];
+    const bool first = first_of_system(a, sys);
+    if (sc.done) {
+        if (first && threadIdx.x == 0) k.st_out[sys] = sc;
+        return;
+    }
+    const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
+    sc.pending = 0;
+    if (threadIdx.x < kWave) {
+        if (sc.stage == ST_INIT) {
+            const double rr = wsum(k.part + P_RR * k.nfib, fb, fe);
+            if (threadIdx.x == 0) bc[0] = rr;
+        } else {
+            double S4[4];
+            wsum4(k.part4, fb, fe, S4);
+            const double m = wmax(k.part + P_SMAX * k.nfib, fb, fe);
+            if (threadIdx.x == 0) { bc[0] = S4[0]; bc[1] = S4[1]; bc[2] = S4[2]; bc[3] = S4[3]; bc[4] = m; }
+        }
+    }
+    __syncthreads();
+    if (sc.stage == ST_INIT) {
+        sc.rho2 = sc.rho1;
+        sc.rho1 = bc[0];
+        sc.beta = (sc.rho1 / sc.rho2) * (sc.alpha / sc.omega);
+    } else {
+        sc.resid_s = (bc[4] != bc[4]) ? __builtin_inf() : bc[4];
+        const double tt = bc[0], ts = bc[1], rts = bc[2], rtt = bc[3];
+        if (tt == 0.0) {
+            sc.done = 1; sc.flag = 0; sc.resid = sc.resid_s;
+        } else {
+            const double om = ts / tt;
+            if (om == 0.0) {
+                sc.done = 1; sc.flag = 3; sc.resid = sc.resid_s;
+            } else {
+                sc.omega = om;
+                sc.pending = 1;
+                sc.rho2 = sc.rho1;
+                sc.rho1 = rts - om * rtt;
+                sc.beta = (sc.rho1 / sc.rho2) * (sc.alpha / sc.omega);
+                sc.iter += 1;
+            }
+        }
+    }
+    sc.stage = ST_A;
+    if (first && threadIdx.x == 0) k.st_out[sys] = sc;
+    if (sc.done) return;
+    const PhatOp op{ k.y, k.sh, k.w, k.ph_in, k.z_in, sc.beta, sc.omega, sc.pending };
+    double rmax = 0.0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int i = threadIdx.x + q * kApplyThreads;
+        if (i >= n) continue;
+        double ri = vr[q];
+        double yi = vy[q];
+        if (sc.pending) {
+            k.x[o + i] = k.x[o + i] + sc.omega * vs[q];
+            ri = ri + (-sc.omega) * vt[q];
+            k.r[o + i] = ri;
+            yi = vs[q] + (-sc.omega) * vw[q];
+            k.y[o + i] = yi;
+        }
+        rmax = nan_max(rmax, fabs(ri));
+        const double phi = sc.beta == 0.0 ? yi : yi + sc.beta * (vph[q] + (-sc.omega) * vz[q]);
+        k.ph_out[o + i] = phi;
+        s.x[i] = phi;
+    }
+    load_factors<D>(a, s, a.seg[f], (int)np - 1);
+    __syncthreads();
+    fiber_operator_op<D>(a, s, op, f);
+    double dl = 0.0;
+    for (int i = threadIdx.x; i < n; i += kApplyThreads) {
+        k.v[o + i] = s.y[i];
+        dl += k.rt[o + i] * s.y[i];
+    }
+    rmax = wave_max(rmax);
+    dl = wave_sum(dl);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        wr[threadIdx.x / kWave] = rmax;
+        wr[kApplyWaves + threadIdx.x / kWave] = dl;
+    }
+    apply_or_copy<PC>(k, f, n, s.y, k.z_out + o, red);
+    if (threadIdx.x == 0) {
+        double m = wr[0], d = wr[kApplyWaves];
+        for (int q = 1; q < kApplyWaves; ++q) {
+            m = nan_max(m, wr[q]);
+            d += wr[kApplyWaves + q];
+        }
+        k.part[P_RMAX * k.nfib + f] = m;
+        k.part[P_DELTA * k.nfib + f] = d;
+    }
+}
+
+template <int D, int PC>
+__global__ __launch_bounds__(kApplyThreads) void k_fq(KArgs k)
+{
+    constexpr int R = (kMaxFiberPoints * D + kApplyThreads - 1) / kApplyThreads;
+    __shared__ FiberLds<D> s;
+    __shared__ double red[kApplyWaves * kMaxFiberPoints * D];
+    __shared__ double wr[5 * kApplyWaves];
+    __shared__ double bc[4];
+    const uint32_t f = blockIdx.x;
+    const FiberArgs& a = k.fa;
+    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
+    const uint32_t np = a.np[f], o = D * a.idx[f];
+    const int n = D * (int)np;
+    double vr[R], vv[R], vx[R], vph[R], vy[R], vz[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int i = threadIdx.x + q * kApplyThreads;
+        if (i < n) {
+            vr[q] = k.r[o + i];
+            vv[q] = k.v[o + i];
+            vx[q] = k.x[o + i];
+            vph[q] = k.ph_in[o + i];
+            vy[q] = k.y[o + i];
+            vz[q] = k.z_in[o + i];
+        }
+    }
+    SysScalars sc = k.st_in[sys];
+    const bool first = first_of_system(a, sys);
+    if (!sc.done) {
+        const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
+        if (threadIdx.x < kWave) {
+            const double res = wmax(k.part + P_RMAX * k.nfib, fb, fe);
+            const double dsum = wsum(k.part + P_DELTA * k.nfib, fb, fe);
+            if (threadIdx.x == 0) { bc[0] = res; bc[1] = dsum; }
+        }
+        __syncthreads();
+        decide(sc, bc[0]);
+        if (!sc.done) {
+            const double delta = bc[1];
+            if (PC != CYMG_PRECOND_NONE && delta == 0.0) { sc.done = 1; sc.flag = 4; }
+            else sc.alpha = sc.rho1 / delta;
+        }
+        sc.stage = ST_C;
+    }
+    if (first && threadIdx.x == 0) k.st_out[sys] = sc;
+    if (sc.done) return;
+    const ShatOp op{ k.y, k.z_in, sc.alpha };
+    double smax = 0.0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int i = threadIdx.x + q * kApplyThreads;
+        if (i >= n) continue;
+        const double si = vr[q] + (-sc.alpha) * vv[q];
+        k.r[o + i] = si;
+        k.x[o + i] = vx[q] + sc.alpha * vph[q];
+        smax = nan_max(smax, fabs(si));
+        const double shi = vy[q] + (-sc.alpha) * vz[q];
+        k.sh[o + i] = shi;
+        s.x[i] = shi;
+    }
+    load_factors<D>(a, s, a.seg[f], (int)np - 1);
+    __syncthreads();
+    fiber_operator_op<D>(a, s, op, f);
+    double tt = 0, ts = 0, rts = 0, rtt = 0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int i = threadIdx.x + q * kApplyThreads;
+        if (i >= n) continue;
+        const double ti = s.y[i];
+        const double si = vr[q] + (-sc.alpha) * vv[q];
+        const double rti = k.rt[o + i];
+        k.t[o + i] = ti;
+        tt += ti * ti;
+        ts += ti * si;
+        rts += rti * si;
+        rtt += rti * ti;
+    }
+    smax = wave_max(smax);
+    tt = wave_sum(tt);
+    ts = wave_sum(ts);
+    rts = wave_sum(rts);
+    rtt = wave_sum(rtt);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        const int wv = threadIdx.x / kWave;
+        wr[wv] = smax;
+        wr[kApplyWaves + wv] = tt;
+        wr[2 * kApplyWaves + wv] = ts;
+        wr[3 * kApplyWaves + wv] = rts;
+        wr[4 * kApplyWaves + wv] = rtt;
+    }
+    apply_or_copy<PC>(k, f, n, s.y, k.w + o, red);
+    if (threadIdx.x == 0) {
+        double m = wr[0], S[4] = { wr[kApplyWaves], wr[2 * kApplyWaves], wr[3 * kApplyWaves], wr[4 * kApplyWaves] };
+        for (int q = 1; q < kApplyWaves; ++q) {
+            m = nan_max(m, wr[q]);
+            for (int j = 0; j < 4; ++j) S[j] += wr[(j + 1) * kApplyWaves + q];
+        }
+        k.part[P_SMAX * k.nfib + f] = m;
+        *reinterpret_cast<double4*>(k.part4 + 4 * (size_t)f) = make_double4(S[0], S[1], S[2], S[3]);
+    }
+}
+
 static KArgs make_kargs(Context& c)
 {
     KArgs k;
@@ -441,6 +779,9 @@ static KArgs make_kargs(Context& c)
     k.t = c.k_t.get();
     k.ph = c.k_ph.get();
     k.sh = c.k_sh.get();
+    k.y = c.k_y.get();
+    k.w = c.k_w.get();
+    k.z_in = k.z_out = k.ph_in = k.ph_out = nullptr;
     k.part = c.d_part.get();
     k.part4 = c.d_part.get() + (size_t)P_TT * c.nfib;
     k.st_in = nullptr;
@@ -518,9 +859,100 @@ static void run(Context& c, const double* guess, int max_iter, double tol, int t
     predicted = c.h_scal[worst].iter;
 }
 
+template <int D, int PC>
+static void run_fused(Context& c, const double* guess, int max_iter, double tol, int tol_mode,
+                      cymg_solve_report* rep)
+{
+    KArgs k = make_kargs(c);
+    SysScalars* S0 = c.d_scal.get();
+    SysScalars* S1 = S0 + c.nsys;
+    double* ph[2] = { c.k_ph.get(), c.k_ph2.get() };
+    double* zz[2] = { c.k_z.get(), c.k_z2.get() };
+    auto with = [&](SysScalars* in, SysScalars* out, int par) {
+        KArgs q = k;
+        q.st_in = in;
+        q.st_out = out;
+        q.ph_in = ph[par];
+        q.ph_out = ph[par ^ 1];
+        q.z_in = zz[par];
+        q.z_out = zz[par ^ 1];
+        return q;
+    };
+    // P reads phat/z of its parity and writes the other; Q reads what P wrote
+    const KArgs kI = with(S0, S1, 0);
+    const KArgs kP0 = with(S1, S0, 0), kQ0 = with(S0, S1, 1);
+    const KArgs kP1 = with(S1, S0, 1), kQ1 = with(S0, S1, 0);
+    const dim3 g(c.nfib), b(kApplyThreads);
+    hipLaunchKernelGGL((k_finit<D, PC>), g, b, 0, c.stream, kI, guess, tol, tol_mode, max_iter, c.prm.time_step);
+    CYMG_HIP(hipGetLastError());
+    int launched = 0;
+    int& predicted = c.last_iters[PC];
+    int chunk = predicted > 0 ? predicted + 2 : 4;
+    const size_t sbytes = sizeof(SysScalars) * c.nsys;
+    bool first = true;
+    while (true) {
+        if (!first) {
+            CYMG_HIP(hipMemcpyAsync(c.h_scal, S1, sbytes, hipMemcpyDeviceToHost, c.stream));
+            CYMG_HIP(hipStreamSynchronize(c.stream));
+            bool all = true;
+            for (uint32_t s = 0; s < c.nsys; ++s) all = all && c.h_scal[s].done;
+            if (all || launched > max_iter + 4) break;
+        }
+        first = false;
+        for (int it = 0; it < chunk; it += 2) {
+            hipLaunchKernelGGL((k_fp<D, PC>), g, b, 0, c.stream, kP0);
+            hipLaunchKernelGGL((k_fq<D, PC>), g, b, 0, c.stream, kQ0);
+            hipLaunchKernelGGL((k_fp<D, PC>), g, b, 0, c.stream, kP1);
+            hipLaunchKernelGGL((k_fq<D, PC>), g, b, 0, c.stream, kQ1);
+        }
+        CYMG_HIP(hipGetLastError());
+        launched += chunk;
+        chunk = predicted > 0 ? std::max(4, predicted / 8) : std::min(2 * chunk, 16);
+        chunk += chunk & 1;
+    }
+    c.sys_reports.assign(c.nsys, cymg_solve_report{});
+    int worst = 0, all_conv = 1;
+    for (uint32_t s = 0; s < c.nsys; ++s) {
+        const SysScalars& w = c.h_scal[s];
+        cymg_solve_report& r = c.sys_reports[s];
+        r.iterations = w.iter;
+        r.total_iterations = w.iter;
+        r.flag = w.flag;
+        r.residual = w.resid;
+        r.threshold = w.tol;
+        r.converged = w.resid < w.tol;
+        r.noise_level = w.noise * c.prm.time_step;
+        r.precond_used = PC;
+        all_conv = all_conv && r.converged;
+        if (w.iter > c.h_scal[worst].iter || (!r.converged && c.sys_reports[worst].converged)) worst = (int)s;
+    }
+    *rep = c.sys_reports[worst];
+    rep->converged = all_conv;
+    predicted = c.h_scal[worst].iter;
+}
+
 void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter, double tol, int tol_mode,
                   cymg_solve_report* rep)
 {
+    if (c.krylov_fused) {
+        const size_t n = c.n();
+        const bool fresh = c.k_y.cap < n;
+        c.k_r.resize(n); c.k_rt.resize(n); c.k_v.resize(n); c.k_t.resize(n); c.k_sh.resize(n);
+        c.k_ph.resize(n); c.k_ph2.resize(n); c.k_y.resize(n); c.k_z.resize(n); c.k_z2.resize(n); c.k_w.resize(n);
+        if (fresh)
+            for (DevBuf<double>* v : { &c.k_ph, &c.k_ph2, &c.k_z, &c.k_z2, &c.k_w, &c.k_sh, &c.k_t })
+                CYMG_HIP(hipMemsetAsync(v->get(), 0, v->cap * sizeof(double), c.stream));
+        if (c.dim == 3) {
+            if (precond == CYMG_PRECOND_FP64) run_fused<3, CYMG_PRECOND_FP64>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            else if (precond == CYMG_PRECOND_FP32) run_fused<3, CYMG_PRECOND_FP32>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            else run_fused<3, CYMG_PRECOND_NONE>(c, guess_dev, max_iter, tol, tol_mode, rep);
+        } else {
+            if (precond == CYMG_PRECOND_FP64) run_fused<2, CYMG_PRECOND_FP64>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            else if (precond == CYMG_PRECOND_FP32) run_fused<2, CYMG_PRECOND_FP32>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            else run_fused<2, CYMG_PRECOND_NONE>(c, guess_dev, max_iter, tol, tol_mode, rep);
+        }
+        return;
+    }
     const size_t n = c.n();
     c.k_r.resize(n); c.k_rt.resize(n); c.k_p.resize(n); c.k_v.resize(n); c.k_t.resize(n);
     c.k_ph.resize(n); c.k_sh.resize(n);

@@ -384,14 +384,18 @@ def config5(seed=5, nsys=64, fmin=50, fmax=2000, links_min=5, links_max=30, radi
         srng = np.random.default_rng([seed, k])
         lengths = srng.uniform(2.0, 15.0, nf)
         nbp, cut, pts = _straight_fibers(srng, 3, lengths, 0.5, radius)
-        nl = int(srng.integers(links_min, links_max + 1)) * nf // 2
+        nl = int(srng.integers(links_min, links_max + 1)) * nf
         sc = Scene(dim=3, time_step=0.005, kT=0.0042, viscosity=0.02, hydro_radius0=0.0125,
                    hydro_radius1=10.0, nb_points=nbp, segmentation=cut, rigidity=np.full(nf, 20.0),
                    points=pts, confine_radius=radius, confine_stiffness=200.0, seed=seed * 1000 + k,
                    name=f"system-{k}")
-        fa, aa, fb, ab = _bind_crosslinkers_fast(srng, 3, nbp, cut, pts, nl, 0.05)
+        try:
+            fa, aa, fb, ab = _bind_crosslinkers_fast(srng, 3, nbp, cut, pts, nl, 0.05)
+        except ValueError:
+            # a sparse system may have no fiber crossing within binding range
+            fa, aa, fb, ab = (np.zeros(0, np.uint32), np.zeros(0), np.zeros(0, np.uint32), np.zeros(0))
         sc.link_fiber_a, sc.link_abs_a, sc.link_fiber_b, sc.link_abs_b = fa, aa, fb, ab
-        sc.link_stiffness = np.full(nl, 100.0)
+        sc.link_stiffness = np.full(fa.size, 100.0)
         systems.append(sc)
     return concatenate(systems, name=f"batch-{nsys}-systems")
 

@@ -175,6 +175,14 @@ int         cymg_get_block(cymg_context* ctx, uint32_t mecable, int inverse, dou
 int         cymg_get_matrix(cymg_context* ctx, int which, uint32_t* nnz, uint32_t* rows,
                             uint32_t* cols, double* vals);
 
+/* implementation choices (no counterpart in the reference; results agree to
+   the solver tolerance, see the parity tests):
+   CYMG_OPT_KRYLOV_LAUNCHES: 2 (default, recurrences for the preconditioned
+     vectors) or 4 launches per BiCGStab iteration;
+   CYMG_OPT_BLOCK_INVERSE: 0 Gauss-Jordan (default) or 1 getrf + getri order */
+enum { CYMG_OPT_KRYLOV_LAUNCHES = 1, CYMG_OPT_BLOCK_INVERSE = 2 };
+int         cymg_set_option(cymg_context* ctx, int option, int value);
+
 /* device timings of the last solve, milliseconds */
 typedef struct {
     double assemble_ms;     /* stream sort/reduce, CSR build */

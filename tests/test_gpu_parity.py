@@ -108,8 +108,10 @@ def test_blocks_and_precondition(pkg, scenes, oracle, dim):
     assert rel_err(gpu.precondition(x), orc.precondition(x)) < 1e-10
 
 
-def _step_pair(pkg, oracle, sc, steps, tol, tol_mode, precondition=1):
+def _step_pair(pkg, oracle, sc, steps, tol, tol_mode, precondition=1, launches=2, inverse=0):
     gpu = gpu_from_scene(pkg, sc)
+    gpu.set_option(pkg.OPT_KRYLOV_LAUNCHES, launches)
+    gpu.set_option(pkg.OPT_BLOCK_INVERSE, inverse)
     orc = oracle_from_scene(oracle, sc)
     reps = []
     for s in range(steps):
@@ -139,9 +141,10 @@ def test_config1_one_iteration_and_positions(pkg, scenes, oracle):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-def test_crosslinked_step_fp64(pkg, scenes, oracle, dim):
+@pytest.mark.parametrize("launches,inverse", [(2, 0), (4, 0), (2, 1)])
+def test_crosslinked_step_fp64(pkg, scenes, oracle, dim, launches, inverse):
     sc = scenes.mini(dim=dim, seed=15, nfib=12, nlinks=40, radius=2.0)
-    gpu, orc, reps = _step_pair(pkg, oracle, sc, 2, 1e-11, 1)
+    gpu, orc, reps = _step_pair(pkg, oracle, sc, 2, 1e-11, 1, launches=launches, inverse=inverse)
     for r1, r2 in reps:
         assert r1["converged"] and r2["converged"]
     assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-9
@@ -163,9 +166,10 @@ def test_fp32_preconditioner(pkg, scenes, oracle):
     assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-6
 
 
-def test_unpreconditioned_bcgs(pkg, scenes, oracle):
+@pytest.mark.parametrize("launches", [2, 4])
+def test_unpreconditioned_bcgs(pkg, scenes, oracle, launches):
     sc = scenes.mini(dim=3, seed=16, nfib=6, nlinks=10)
-    gpu, orc, reps = _step_pair(pkg, oracle, sc, 1, 1e-10, 1, precondition=pkg.PRECOND_NONE)
+    gpu, orc, reps = _step_pair(pkg, oracle, sc, 1, 1e-10, 1, precondition=pkg.PRECOND_NONE, launches=launches)
     assert reps[0][0]["converged"]
     assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-8
 
