@@ -38,6 +38,22 @@ def algorithmic_bytes_matvec(sc, kb_full, kc_full):
     return 16 * n + 12 * (kb_full + kc_full) + 8 * n + (8 * sc.dim + 24) * S
 
 
+def algorithmic_bytes_iteration_launch(sc, kb_full, kc_full, block_bytes):
+    """Compulsory HBM bytes of one launch of the two-launch BiCGStab
+    iteration, the mean of its two launches P and Q:
+      vectors: P reads r,t,shat,w,y,phat,z,x,rt and writes x,r,y,phat',v,z'
+      (15 n doubles), Q reads r,v,x,phat,y,z,rt and writes r,x,shat,t,w (12 n);
+      operator: CSR column+value of the full mB/mC rows (12 per entry), row
+      extents (4 per point row and per dof row), per segment rfDiff (8*DIM),
+      JJ' diagonal and off-diagonal (16) and P' coefficient (8);
+      block inverse: sum over fibers of bs^2 entries of the stored type."""
+    n = sc.size
+    S = sc.npts - sc.nfib
+    vectors = 8 * n * (15 + 12) / 2
+    operator = 12 * (kb_full + kc_full) + 4 * (sc.npts + n) + (8 * sc.dim + 24) * S
+    return vectors + operator + block_bytes
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -48,6 +64,12 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps timed for cpu_baseline (0: skip)")
     ap.add_argument("--modes", default="fp64,fp32")
+    ap.add_argument("--timed-launches", type=int, default=40,
+                    help="iteration launches timed with HIP events for the roofline")
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per iteration launch measured with rocprofv3 counters (reported as is)")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run a plain device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--batched-steps", type=int, default=3,
                     help="steps of the 64-system batched workload (0: skip)")
     return ap.parse_args()
@@ -119,7 +141,26 @@ def main():
     kb_full = 2 * int((rB != cB).sum()) + int((rB == cB).sum())
     kc_full = 2 * int((rC != cC).sum()) + int((rC == cC).sum())
     mv_bytes = algorithmic_bytes_matvec(sc, kb_full, kc_full)
-    achieved = mv_bytes / (mv_ms * 1e-3) / 1e9
+    mv_achieved = mv_bytes / (mv_ms * 1e-3) / 1e9
+    nblk = int(((sc.dim * sc.nb_points.astype(np.int64)) ** 2).sum())
+    m.compute_preconditioner(pkg.PRECOND_FP64)
+    it_ms = m.time_kernel(6, reps=args.timed_launches)
+    it_bytes = algorithmic_bytes_iteration_launch(sc, kb_full, kc_full, 8 * nblk)
+    achieved = it_bytes / (it_ms * 1e-3) / 1e9
+    it32_ms = it32_achieved = None
+    if "fp32" in results:
+        m.compute_preconditioner(pkg.PRECOND_FP32)
+        it32_ms = m.time_kernel(6, reps=args.timed_launches)
+        it32_achieved = algorithmic_bytes_iteration_launch(sc, kb_full, kc_full, 4 * nblk) / (it32_ms * 1e-3) / 1e9
+
+    if args.copy_calibration_mb > 0:
+        nb = args.copy_calibration_mb << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     batched = None
     if args.batched_steps > 0:
@@ -154,17 +195,15 @@ def main():
     cpu = None
     if args.cpu_steps > 0:
         from oracle import pyoracle
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from conftest import oracle_from_scene, oracle_assemble
         print(f"bench: cpu baseline, {args.cpu_steps} oracle steps", file=sys.stderr, flush=True)
-        orc = oracle_from_scene(pyoracle, sc)
+        orc = pyoracle.from_scene(sc)
         t0 = time.perf_counter()
         its = []
         for s in range(args.cpu_steps):
             rep = orc.solve(args.tol, sc.gauss(s), tol_mode=1)
             its.append(rep["iterations"])
             orc.prepare(orc.fiber_points())
-            oracle_assemble(orc, sc)
+            pyoracle.assemble_scene(orc, sc)
         cdt = time.perf_counter() - t0
         cpu = {"value": args.cpu_steps / cdt, "unit": "implicit steps/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_steps} steps of the same 1000-fiber workload, FP64 block-LU BCGSP, "
@@ -189,11 +228,15 @@ def main():
                    "tolerance": f"|r|inf/|b|inf<{args.tol:g}"},
         "modes": {k: {"steps_per_s": round(v["steps_per_s"], 3), "ms_per_step": round(v["ms_per_step"], 3),
                       "mean_bicgstab_iters": round(v["mean_iters"], 2)} for k, v in results.items()},
-        "roofline": {"bound": "hbm", "kernel": "k_matvec (fused Meca::multiply)", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_launch": int(mv_bytes), "avg_launch_ms": mv_ms},
-        "kernels_ms": {"matvec": mv_ms, "precond_apply": pc_ms, "precond_build_fp64": build_ms,
-                       "precond_assemble_only": build_only_ms, "precond_build_getrf_getri": build_lu_ms},
+        "roofline": {"bound": "hbm", "kernel": "k_tp/k_tq (one launch of the FP64 BiCGStab iteration)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": args.traffic,
+                     "bytes_per_launch": int(it_bytes), "avg_launch_ms": round(it_ms, 5)},
+        "kernels_ms": {"iteration_launch_fp64": it_ms, "iteration_launch_fp32": it32_ms,
+                       "iteration_launch_fp32_gbs": it32_achieved and round(it32_achieved, 1),
+                       "matvec": mv_ms, "matvec_gbs": round(mv_achieved, 1), "precond_apply": pc_ms,
+                       "precond_build_fp64": build_ms, "precond_assemble_only": build_only_ms,
+                       "precond_build_getrf_getri": build_lu_ms},
         "cpu_baseline": cpu,
         "batched": batched,
     }

@@ -36,7 +36,7 @@ EXPORTED = [
     "cymg_get_vector", "cymg_get_points", "cymg_get_tensions", "cymg_get_block", "cymg_get_matrix",
     "cymg_get_timings", "cymg_enable_timing", "cymg_time_kernel", "cymg_set_option",
 ]
-OPT_KRYLOV_LAUNCHES, OPT_BLOCK_INVERSE = 1, 2
+OPT_KRYLOV_LAUNCHES, OPT_BLOCK_INVERSE, OPT_TILE_POINTS = 1, 2, 3
 
 
 class SimParams(C.Structure):

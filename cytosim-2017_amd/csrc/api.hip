@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "context.h"
+#include "tile_ops.h"
 
 using namespace cymg;
 
@@ -92,6 +93,66 @@ void same_system(const Context& c, std::initializer_list<uint32_t> pts)
     }
 }
 
+// Tiles of consecutive whole fibers of one system, at most `target` points
+// (a longer fiber gets a tile of its own) and kTileMaxFibers fibers.  The
+// default target keeps the largest system at about 256 tiles, which bounds the
+// partial sums every tile re-reduces, and at least 32 points per tile.
+void build_tiles(Context& c)
+{
+    const std::vector<uint32_t>& sb = c.h_sys_begin_used;
+    uint32_t target = c.tile_points;
+    if (!target) {
+        uint32_t most = 0;
+        for (uint32_t s = 0; s < c.nsys; ++s) {
+            const uint32_t f0 = sb[s], f1 = sb[s + 1];
+            const uint32_t pts = c.h_idx[f1 - 1] + c.h_np[f1 - 1] - c.h_idx[f0];
+            most = std::max(most, pts);
+        }
+        target = std::min<uint32_t>(std::max<uint32_t>((most + 255) / 256, 32), kMaxFiberPoints);
+    }
+    c.h_tile_fib.clear();
+    c.h_tile_sys.clear();
+    c.h_sys_tile.assign(c.nsys + 1, 0);
+    for (uint32_t s = 0; s < c.nsys; ++s) {
+        c.h_sys_tile[s] = (uint32_t)c.h_tile_fib.size();
+        uint32_t pts = 0, nf = 0;
+        for (uint32_t f = sb[s]; f < sb[s + 1]; ++f) {
+            if (nf && (pts + c.h_np[f] > target || nf == (uint32_t)kTileMaxFibers)) nf = 0;
+            if (!nf) {
+                c.h_tile_fib.push_back(f);
+                c.h_tile_sys.push_back(s);
+                pts = 0;
+            }
+            pts += c.h_np[f];
+            ++nf;
+        }
+    }
+    c.ntiles = (uint32_t)c.h_tile_fib.size();
+    c.h_sys_tile[c.nsys] = c.ntiles;
+    c.h_tile_fib.push_back(c.nfib);
+    std::vector<TileDesc> desc(c.ntiles);
+    for (uint32_t t = 0; t < c.ntiles; ++t) {
+        TileDesc& d = desc[t];
+        const uint32_t fb = c.h_tile_fib[t], fe = c.h_tile_fib[t + 1];
+        d.fb = fb;
+        d.nft = fe - fb;
+        d.P0 = c.h_idx[fb];
+        d.tp = c.h_idx[fe - 1] + c.h_np[fe - 1] - d.P0;
+        d.sys = c.h_tile_sys[t];
+        d.tb = c.h_sys_tile[d.sys];
+        d.te = c.h_sys_tile[d.sys + 1];
+        d.pad = 0;
+    }
+    c.d_tile_desc.resize(c.ntiles ? c.ntiles : 1);
+    c.d_tpart.resize((size_t)P_SLOTS * (c.ntiles ? c.ntiles : 1));
+    hipStream_t st = c.stream;
+    if (c.ntiles)
+        CYMG_HIP(hipMemcpyAsync(c.d_tile_desc.get(), desc.data(), c.ntiles * sizeof(TileDesc), hipMemcpyHostToDevice,
+                                st));
+    CYMG_HIP(hipStreamSynchronize(st));   // pageable source
+    c.last_iters[0] = c.last_iters[1] = c.last_iters[2] = 0;
+}
+
 void upload_layout(Context& c)
 {
     const uint32_t F = c.nfib;
@@ -113,6 +174,7 @@ void upload_layout(Context& c)
     CYMG_HIP(hipMemcpyAsync(c.d_blkoff.get(), c.h_blkoff.data(), (F + 1) * 8, hipMemcpyHostToDevice, st));
     CYMG_HIP(hipMemcpyAsync(c.d_fib_sys.get(), sysv.data(), F * 4, hipMemcpyHostToDevice, st));
     CYMG_HIP(hipMemcpyAsync(c.d_sys_begin.get(), sb.data(), sb.size() * 4, hipMemcpyHostToDevice, st));
+    c.h_sys_begin_used = sb;
     const size_t n = c.n();
     for (DevBuf<double>* v : { &c.d_pts, &c.d_pos, &c.d_sol, &c.d_bas, &c.d_rhs, &c.d_for, &c.d_tmp, &c.d_prev_sol })
         v->resize(n ? n : 1);
@@ -125,6 +187,7 @@ void upload_layout(Context& c)
     c.d_scal.resize(2 * c.nsys);
     CYMG_HIP(hipMemsetAsync(c.d_scal.get(), 0, 2 * c.nsys * sizeof(SysScalars), st));
     CYMG_HIP(hipStreamSynchronize(st));   // the sources above are pageable host arrays
+    build_tiles(c);
     c.layout_dirty = false;
 }
 
@@ -811,7 +874,11 @@ int cymg_set_option(cymg_context* ctx, int option, int value)
 {
     return guarded(ctx, [&](Context& c) {
         if (option == CYMG_OPT_KRYLOV_LAUNCHES && (value == 2 || value == 4)) c.krylov_fused = value == 2;
-        else if (option == CYMG_OPT_BLOCK_INVERSE && (value == 0 || value == 1)) c.pc_method = value;
+        else if (option == CYMG_OPT_BLOCK_INVERSE && value >= 0 && value <= 2) c.pc_method = value;
+        else if (option == CYMG_OPT_TILE_POINTS && value >= 0 && value <= kMaxFiberPoints) {
+            c.tile_points = (uint32_t)value;
+            if (!c.layout_dirty && c.nfib) build_tiles(c);
+        }
         else throw Error(CYMG_ERR_INVALID, "unknown option or value");
         c.last_iters[0] = c.last_iters[1] = c.last_iters[2] = 0;
     });
@@ -825,8 +892,13 @@ int cymg_enable_timing(cymg_context* ctx, int enable)
 int cymg_time_kernel(cymg_context* ctx, int which, int reps, double* avg_ms)
 {
     return guarded(ctx, [&](Context& c) {
-        if (!avg_ms || reps <= 0 || which < 0 || which > 5) throw Error(CYMG_ERR_INVALID, "invalid arguments");
+        if (!avg_ms || reps <= 0 || which < 0 || which > 6) throw Error(CYMG_ERR_INVALID, "invalid arguments");
         if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_time_kernel");
+        if (which == 6) {
+            if (!c.pc_ready) throw Error(CYMG_ERR_INVALID, "no preconditioner computed");
+            *avg_ms = krylov_time_launch(c, c.pc_precision, reps);
+            return;
+        }
         if (which == 1 && !c.pc_ready) throw Error(CYMG_ERR_INVALID, "no preconditioner computed");
         const size_t n = c.n();
         c.k_v.resize(n); c.k_t.resize(n);

@@ -70,6 +70,18 @@ struct alignas(16) SysScalars {
 };
 
 // slots of per-fiber partial reductions ([slot][fiber])
+// A tile of the two-launch iteration: a run of consecutive whole fibers of one
+// system.  Tiles partition the fibers in order and never span two systems.
+struct TileDesc {
+    uint32_t fb;        // first fiber
+    uint32_t nft;       // fibers
+    uint32_t P0;        // first point
+    uint32_t tp;        // points
+    uint32_t sys;       // system
+    uint32_t tb, te;    // the tiles of that system
+    uint32_t pad;
+};
+
 enum { P_RR = 0, P_RMAX, P_DELTA, P_SMAX, P_TT, P_TS, P_RTS, P_RTT, P_NOISE, P_BNORM, P_SLOTS };
 
 // accumulation stream: (key, delta) pairs in call order
@@ -120,6 +132,7 @@ void launch_tensions(Context& c);
 void launch_precond_build(Context& c, int precision, double* block_out = nullptr);
 void launch_precond_apply(Context& c, const double* X, double* Y);
 // launchers (krylov.hip)
+double krylov_time_launch(Context& c, int precond, int launches);
 void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter, double tol,
                   int tol_mode, cymg_solve_report* rep);
 

@@ -22,10 +22,16 @@ struct Context {
     bool layout_dirty = true;
     uint32_t nsys = 1;
     std::vector<uint32_t> h_sys_begin;   // first fiber of each system, nsys+1 entries (empty: one system)
+    std::vector<uint32_t> h_sys_begin_used;   // the same, as uploaded (always nsys+1 entries)
     std::vector<cymg_solve_report> sys_reports;
+    // tiles of whole fibers for the two-launch iteration (never across systems)
+    std::vector<uint32_t> h_tile_fib, h_tile_sys, h_sys_tile;
+    uint32_t ntiles = 0;
+    uint32_t tile_points = 0;          // target points per tile, 0: chosen from the layout
 
     // per-fiber device arrays
     DevBuf<uint32_t> d_np, d_idx, d_seg, d_fib_sys, d_sys_begin;
+    DevBuf<TileDesc> d_tile_desc;
     DevBuf<uint64_t> d_blkoff;
     DevBuf<double> d_cut, d_rig, d_mob, d_rfrig;
 
@@ -54,7 +60,10 @@ struct Context {
 
     // preconditioner
     int pc_precision = 0;              // CYMG_PRECOND_*
-    int pc_method = 0;                 // 0 Gauss-Jordan inverse, 1 getrf + getri
+    int pc_method = 0;                 // 0 Gauss-Jordan in registers, 1 getrf + getri, 2 Gauss-Jordan in LDS
+    DevBuf<uint32_t> d_gj_list;        // fibers by block-size class
+    std::vector<uint32_t> gj_start;    // class boundaries in d_gj_list
+    uint32_t gj_nfib = 0, gj_largest = 0;
     DevBuf<double> d_binv;
     DevBuf<float> d_binv32;
     DevBuf<int> d_useblk;
@@ -64,6 +73,7 @@ struct Context {
     DevBuf<double> k_y, k_z, k_z2, k_w, k_ph2;   // fused variant: Binv r, Binv v (2), Binv t, phat (2)
     int krylov_fused = 1;              // 1: two launches per iteration, 0: four
     DevBuf<double> d_part;             // partial reductions [slot][nfib]
+    DevBuf<double> d_tpart;            // partial reductions [slot][tile]
     DevBuf<SysScalars> d_scal;
     SysScalars* h_scal = nullptr;      // pinned mirror
     int last_iters[3] = {0, 0, 0};     // iterations of the previous solve, per preconditioner

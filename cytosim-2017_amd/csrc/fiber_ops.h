@@ -83,12 +83,14 @@ __device__ __forceinline__ void ptts2_serial(int n, const double* d, const doubl
 // Parallel cyclic reduction of the symmetric tridiagonal system JJ' x = t
 // across one wavefront (lane = row, S <= 64): log2(S) steps of 8 shuffles.
 // The same system as dptts2 solves with the pttrf factors; rounding differs
-// at the ulp level.
-__device__ __forceinline__ double pcr_solve(int S, double a, double b, double c, double d)
+// at the ulp level.  Lanes >= S hold identity rows.  With span < S the system
+// must be block diagonal with blocks of at most span rows (zero couplings
+// between blocks): span bounds the number of steps.
+__device__ __forceinline__ double pcr_solve_span(int S, int span, double a, double b, double c, double d)
 {
     const int lane = threadIdx.x & (kWave - 1);
     if (lane >= S) { a = 0.0; b = 1.0; c = 0.0; d = 0.0; }
-    for (int k = 1; k < S; k <<= 1) {
+    for (int k = 1; k < span; k <<= 1) {
         double al = __shfl_up(a, k, kWave), bl = __shfl_up(b, k, kWave);
         double cl = __shfl_up(c, k, kWave), dl = __shfl_up(d, k, kWave);
         double au = __shfl_down(a, k, kWave), bu = __shfl_down(b, k, kWave);
@@ -103,6 +105,11 @@ __device__ __forceinline__ double pcr_solve(int S, double a, double b, double c,
         c = gamma * cu;
     }
     return d / b;
+}
+
+__device__ __forceinline__ double pcr_solve(int S, double a, double b, double c, double d)
+{
+    return pcr_solve_span(S, S, a, b, c, d);
 }
 
 // Load the fiber's pttrf factors into LDS.

@@ -21,6 +21,7 @@
 
 #include "args.h"
 #include "pc_ops.h"
+#include "tile_ops.h"
 
 namespace cymg {
 
@@ -51,47 +52,91 @@ struct KArgs {
     double* z_out;
     double* ph_in;      // phat of the previous iteration
     double* ph_out;
-    double* part;
-    double* part4;      // [fiber][4] partials of D
+    double* part;       // partial sums [slot][fiber] (four launches) or [slot][tile] (two)
+    double* part4;      // [fiber or tile][4] partials of D
+    const double* fpart;// per-fiber partials of the setup (noise, |b|_inf)
+    TileArgs ta;
     SysScalars* st_in;
     SysScalars* st_out;
     uint32_t nfib;
     uint32_t nsys;
 };
 
-// Reductions over the partials of one system (fibers [fb, fe)), repeated by
-// every workgroup.  In a one-wave workgroup the xor-shuffle reduction leaves
-// the total in every lane; wider workgroups let wave 0 reduce and broadcast
-// through LDS behind one barrier (bc: a slot used once per launch).
+// Reductions over the partials of one system (entries [fb, fe)), repeated by
+// every workgroup in the same order, so all of them derive the same value.
+// Each lane issues up to kRed loads before using any (one memory round trip
+// for up to kRed * 64 entries), then the wavefront reduces with xor shuffles.
+constexpr int kRed = 8;
+
 __device__ __forceinline__ double wsum(const double* part, uint32_t fb, uint32_t fe)
 {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
     double v = 0.0;
-    for (uint32_t i = fb + (threadIdx.x & (kWave - 1)); i < fe; i += kWave) v += part[i];
+    for (uint32_t base = fb; base < fe; base += kRed * kWave) {
+        double t[kRed];
+#pragma unroll
+        for (int u = 0; u < kRed; ++u) {
+            const uint32_t i = base + u * kWave + lane;
+            t[u] = i < fe ? part[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kRed; ++u) v += t[u];
+    }
     return wave_sum(v);
 }
 
 __device__ __forceinline__ double wmax(const double* part, uint32_t fb, uint32_t fe)
 {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
     double v = 0.0;
-    for (uint32_t i = fb + (threadIdx.x & (kWave - 1)); i < fe; i += kWave) v = nan_max(v, part[i]);
+    for (uint32_t base = fb; base < fe; base += kRed * kWave) {
+        double t[kRed];
+#pragma unroll
+        for (int u = 0; u < kRed; ++u) {
+            const uint32_t i = base + u * kWave + lane;
+            t[u] = i < fe ? part[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kRed; ++u) v = nan_max(v, t[u]);
+    }
     return wave_max(v);
 }
 
 __device__ __forceinline__ double wmin(const double* part, uint32_t fb, uint32_t fe)
 {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
     double v = __builtin_inf();
-    for (uint32_t i = fb + (threadIdx.x & (kWave - 1)); i < fe; i += kWave) v = fmin(v, part[i]);
+    for (uint32_t base = fb; base < fe; base += kRed * kWave) {
+        double t[kRed];
+#pragma unroll
+        for (int u = 0; u < kRed; ++u) {
+            const uint32_t i = base + u * kWave + lane;
+            t[u] = i < fe ? part[i] : __builtin_inf();
+        }
+#pragma unroll
+        for (int u = 0; u < kRed; ++u) v = fmin(v, t[u]);
+    }
     for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
     return v;
 }
 
-// the 4 partials of D, stored together per fiber ([fiber][4])
+// the 4 partials of D, stored together per entry ([entry][4])
 __device__ __forceinline__ void wsum4(const double* part4, uint32_t fb, uint32_t fe, double out[4])
 {
+    constexpr int U = kRed / 2;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
     double v[4] = { 0, 0, 0, 0 };
-    for (uint32_t i = fb + (threadIdx.x & (kWave - 1)); i < fe; i += kWave) {
-        const double4 q = *reinterpret_cast<const double4*>(part4 + 4 * (size_t)i);
-        v[0] += q.x; v[1] += q.y; v[2] += q.z; v[3] += q.w;
+    for (uint32_t base = fb; base < fe; base += U * kWave) {
+        double4 t[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = base + u * kWave + lane;
+            t[u] = i < fe ? *reinterpret_cast<const double4*>(part4 + 4 * (size_t)i) : make_double4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            v[0] += t[u].x; v[1] += t[u].y; v[2] += t[u].z; v[3] += t[u].w;
+        }
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) out[q] = wave_sum(v[q]);
@@ -433,8 +478,9 @@ __global__ __launch_bounds__(kWave) void k_iter_d(KArgs k)
 
 
 // ===========================================================================
-// Fused variant: two launches per iteration.  The preconditioned vectors are
-// carried by recurrences of the linear block inverse M^-1:
+// Fused variant: two launches per iteration over tiles of whole fibers.  The
+// preconditioned vectors are carried by recurrences of the linear block
+// inverse M^-1:
 //   y = M^-1 r, z = M^-1 v, w = M^-1 t,
 //   phat = M^-1 p = y + beta (phat_old - omega z_old),
 //   shat = M^-1 s = y - alpha z,   y_new = M^-1 r_new = shat - omega w,
@@ -446,16 +492,20 @@ __global__ __launch_bounds__(kWave) void k_iter_d(KArgs k)
 //      v = M phat; z = M^-1 v; |r|_inf and <rt, v> partials
 //   Q: stop test on |r|_inf; alpha; s = r - alpha v; x += alpha phat;
 //      shat = y - alpha z; t = M shat; w = M^-1 t; the partials of D
+// Partial sums are written per tile and re-reduced, in tile order, by every
+// tile of the same system in the next launch.
 
+// storage type and array of the block inverses of a preconditioner mode
 template <int PC>
-__device__ __forceinline__ void apply_or_copy(const KArgs& k, uint32_t f, int bs, const double* xs, double* y,
-                                              double* red)
-{
-    if constexpr (PC == CYMG_PRECOND_NONE)
-        block_apply_mw<double, kApplyWaves>(nullptr, bs, false, xs, y, red);
-    else
-        apply_block<PC>(k, f, bs, xs, y, red);
-}
+struct PcBlocks {
+    using T = double;
+    __device__ static const double* get(const KArgs& k) { return PC == CYMG_PRECOND_FP64 ? k.B64 : nullptr; }
+};
+template <>
+struct PcBlocks<CYMG_PRECOND_FP32> {
+    using T = float;
+    __device__ static const float* get(const KArgs& k) { return k.B32; }
+};
 
 // phat(j) for any j, from vectors P does not write
 struct PhatOp {
@@ -482,47 +532,58 @@ struct ShatOp {
     __device__ __forceinline__ double operator()(uint32_t j) const { return y[j] + (-alpha) * z[j]; }
 };
 
+__device__ __forceinline__ bool first_tile_of_system(const Tile& T)
+{
+    return T.t == T.tb;
+}
+
+// r = b - M x0 (or b), rt = r, x = x0 (or 0), y = M^-1 r; <rt, r> partials;
+// the first tile of each system sets the threshold from the setup partials
 template <int D, int PC>
-__global__ __launch_bounds__(kApplyThreads) void k_finit(KArgs k, const double* __restrict__ x0, double tolerance,
+__global__ __launch_bounds__(kTileThreads) void k_tinit(KArgs k, const double* __restrict__ x0, double tolerance,
                                                          int tol_mode, int iter_max, double dt)
 {
-    __shared__ FiberLds<D> s;
-    __shared__ double red[kApplyWaves * kMaxFiberPoints * D];
-    __shared__ double wr[kApplyWaves];
-    const uint32_t f = blockIdx.x;
+    constexpr int R = TileRows<D>::R;
+    __shared__ TileLds<D> L;
+    __shared__ double wr[kTileWaves];
     const FiberArgs& a = k.fa;
-    const uint32_t np = a.np[f], o = D * a.idx[f];
-    const int n = D * (int)np;
-    if (x0) {
-        for (int i = threadIdx.x; i < n; i += kApplyThreads) s.x[i] = x0[o + i];
-        load_factors<D>(a, s, a.seg[f], (int)np - 1);
-        __syncthreads();
-        fiber_operator<D>(a, s, x0, f);
-    }
+    const Tile T = tile_begin<D>(k.ta, blockIdx.x);
+    tile_tables<D, PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
+    for (int u = threadIdx.x; u < (int)T.nu; u += kTileThreads) L.x[u] = x0 ? x0[T.U0 + u] : 0.0;
+    __syncthreads();
+    using TB = typename PcBlocks<PC>::T;
+    const TB* Bk = PcBlocks<PC>::get(k);
+    TileRows<D> rw;
+    tile_rows<D>(L, T, rw);
+    TilePre<D, TB> pre;
+    tile_prefetch<D, TB>(a, L, T, rw, Bk, pre);
+    if (x0) tile_operator<D>(a, L, Plain{ x0 }, T, rw, pre);
     double rr = 0.0;
-    for (int i = threadIdx.x; i < n; i += kApplyThreads) {
-        double ri = k.b[o + i];
-        if (x0) ri = ri + (-1.0) * s.y[i];
-        k.r[o + i] = ri;
-        k.rt[o + i] = ri;
-        k.x[o + i] = x0 ? s.x[i] : 0.0;
-        s.f[i] = ri;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        if (rw.k[q] < 0) continue;
+        const int u = threadIdx.x + q * kTileThreads;
+        double ri = k.b[T.U0 + u];
+        if (x0) ri = ri + (-1.0) * L.y[u];
+        k.r[T.U0 + u] = ri;
+        k.rt[T.U0 + u] = ri;
+        k.x[T.U0 + u] = L.x[u];
+        L.f[u] = ri;
         rr += ri * ri;
     }
     rr = wave_sum(rr);
     if ((threadIdx.x & (kWave - 1)) == 0) wr[threadIdx.x / kWave] = rr;
     __syncthreads();
-    apply_or_copy<PC>(k, f, n, s.f, k.y + o, red);
+    tile_block_apply<TB, D>(Bk, L, rw, pre, L.f, k.y + T.U0);
     if (threadIdx.x == 0) {
         double t = wr[0];
-        for (int q = 1; q < kApplyWaves; ++q) t += wr[q];
-        k.part[P_RR * k.nfib + f] = t;
+        for (int q = 1; q < kTileWaves; ++q) t += wr[q];
+        k.part[P_RR * k.ta.ntiles + T.t] = t;
     }
-    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
-    if (first_of_system(a, sys) && threadIdx.x < kWave) {
-        const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
-        const double noise = wmin(k.part + P_NOISE * k.nfib, fb, fe);
-        const double bn = wmax(k.part + P_BNORM * k.nfib, fb, fe);
+    if (first_tile_of_system(T) && threadIdx.x < kWave) {
+        const uint32_t fb = a.sys_begin[T.sys], fe = a.sys_begin[T.sys + 1];
+        const double noise = wmin(k.fpart + P_NOISE * k.nfib, fb, fe);
+        const double bn = wmax(k.fpart + P_BNORM * k.nfib, fb, fe);
         if (threadIdx.x == 0) {
             SysScalars st{};
             st.noise = noise;
@@ -536,55 +597,56 @@ __global__ __launch_bounds__(kApplyThreads) void k_finit(KArgs k, const double* 
             st.iter = 0;
             st.iter_max = iter_max;
             st.stage = ST_INIT;
-            k.st_out[sys] = st;
+            k.st_out[T.sys] = st;
         }
     }
 }
 
 template <int D, int PC>
-__global__ __launch_bounds__(kApplyThreads) void k_fp(KArgs k)
+__global__ __launch_bounds__(kTileThreads) void k_tp(KArgs k)
 {
-    constexpr int R = (kMaxFiberPoints * D + kApplyThreads - 1) / kApplyThreads;
-    __shared__ FiberLds<D> s;
-    __shared__ double red[kApplyWaves * kMaxFiberPoints * D];
-    __shared__ double wr[2 * kApplyWaves];
+    constexpr int R = TileRows<D>::R;
+    __shared__ TileLds<D> L;
+    __shared__ double wr[2 * kTileWaves];
     __shared__ double bc[5];
-    const uint32_t f = blockIdx.x;
     const FiberArgs& a = k.fa;
-    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
-    const uint32_t np = a.np[f], o = D * a.idx[f];
-    const int n = D * (int)np;
-    double vr[R], vt[R], vs[R], vw[R], vy[R], vph[R], vz[R];
+    const Tile T = tile_begin<D>(k.ta, blockIdx.x);
+    double vr[R], vt[R], vs[R], vw[R], vy[R], vph[R], vz[R], vx[R], vrt[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-        const int i = threadIdx.x + q * kApplyThreads;
-        if (i < n) {
-            vr[q] = k.r[o + i];
-            vt[q] = k.t[o + i];
-            vs[q] = k.sh[o + i];
-            vw[q] = k.w[o + i];
-            vy[q] = k.y[o + i];
-            vph[q] = k.ph_in[o + i];
-            vz[q] = k.z_in[o + i];
+        const int u = threadIdx.x + q * kTileThreads;
+        if (u < (int)T.nu) {
+            const uint32_t g = T.U0 + u;
+            vr[q] = k.r[g];
+            vt[q] = k.t[g];
+            vs[q] = k.sh[g];
+            vw[q] = k.w[g];
+            vy[q] = k.y[g];
+            vph[q] = k.ph_in[g];
+            vz[q] = k.z_in[g];
+            vx[q] = k.x[g];
+            vrt[q] = k.rt[g];
         }
     }
-    SysScalars sc = k.st_in[sys];
-    const bool first = first_of_system(a, sys);
+    SysScalars sc = k.st_in[T.sys];
+    const bool first = first_tile_of_system(T);
     if (sc.done) {
-        if (first && threadIdx.x == 0) k.st_out[sys] = sc;
+        if (first && threadIdx.x == 0) k.st_out[T.sys] = sc;
         return;
     }
-    const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
+    tile_tables<D, PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
+    const uint32_t nt = k.ta.ntiles;
+    const uint32_t tb = T.tb, te = T.te;
     sc.pending = 0;
-    if (threadIdx.x < kWave) {
+    if (threadIdx.x >= kWave && threadIdx.x < 2 * kWave) {
         if (sc.stage == ST_INIT) {
-            const double rr = wsum(k.part + P_RR * k.nfib, fb, fe);
-            if (threadIdx.x == 0) bc[0] = rr;
+            const double rr = wsum(k.part + P_RR * nt, tb, te);
+            if (threadIdx.x == kWave) bc[0] = rr;
         } else {
             double S4[4];
-            wsum4(k.part4, fb, fe, S4);
-            const double m = wmax(k.part + P_SMAX * k.nfib, fb, fe);
-            if (threadIdx.x == 0) { bc[0] = S4[0]; bc[1] = S4[1]; bc[2] = S4[2]; bc[3] = S4[3]; bc[4] = m; }
+            wsum4(k.part4, tb, te, S4);
+            const double m = wmax(k.part + P_SMAX * nt, tb, te);
+            if (threadIdx.x == kWave) { bc[0] = S4[0]; bc[1] = S4[1]; bc[2] = S4[2]; bc[3] = S4[3]; bc[4] = m; }
         }
     }
     __syncthreads();
@@ -612,88 +674,99 @@ __global__ __launch_bounds__(kApplyThreads) void k_fp(KArgs k)
         }
     }
     sc.stage = ST_A;
-    if (first && threadIdx.x == 0) k.st_out[sys] = sc;
+    if (first && threadIdx.x == 0) k.st_out[T.sys] = sc;
     if (sc.done) return;
+    using TB = typename PcBlocks<PC>::T;
+    const TB* Bk = PcBlocks<PC>::get(k);
+    TileRows<D> rw;
+    tile_rows<D>(L, T, rw);
+    TilePre<D, TB> pre;
+    tile_prefetch<D, TB>(a, L, T, rw, Bk, pre);
     const PhatOp op{ k.y, k.sh, k.w, k.ph_in, k.z_in, sc.beta, sc.omega, sc.pending };
     double rmax = 0.0;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-        const int i = threadIdx.x + q * kApplyThreads;
-        if (i >= n) continue;
+        if (rw.k[q] < 0) continue;
+        const int u = threadIdx.x + q * kTileThreads;
+        const uint32_t g = T.U0 + u;
         double ri = vr[q];
         double yi = vy[q];
         if (sc.pending) {
-            k.x[o + i] = k.x[o + i] + sc.omega * vs[q];
+            k.x[g] = vx[q] + sc.omega * vs[q];
             ri = ri + (-sc.omega) * vt[q];
-            k.r[o + i] = ri;
+            k.r[g] = ri;
             yi = vs[q] + (-sc.omega) * vw[q];
-            k.y[o + i] = yi;
+            k.y[g] = yi;
         }
         rmax = nan_max(rmax, fabs(ri));
         const double phi = sc.beta == 0.0 ? yi : yi + sc.beta * (vph[q] + (-sc.omega) * vz[q]);
-        k.ph_out[o + i] = phi;
-        s.x[i] = phi;
+        k.ph_out[g] = phi;
+        L.x[u] = phi;
     }
-    load_factors<D>(a, s, a.seg[f], (int)np - 1);
     __syncthreads();
-    fiber_operator_op<D>(a, s, op, f);
+    tile_operator<D>(a, L, op, T, rw, pre);
     double dl = 0.0;
-    for (int i = threadIdx.x; i < n; i += kApplyThreads) {
-        k.v[o + i] = s.y[i];
-        dl += k.rt[o + i] * s.y[i];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        if (rw.k[q] < 0) continue;
+        const int u = threadIdx.x + q * kTileThreads;
+        const uint32_t g = T.U0 + u;
+        k.v[g] = L.y[u];
+        dl += vrt[q] * L.y[u];
     }
     rmax = wave_max(rmax);
     dl = wave_sum(dl);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         wr[threadIdx.x / kWave] = rmax;
-        wr[kApplyWaves + threadIdx.x / kWave] = dl;
+        wr[kTileWaves + threadIdx.x / kWave] = dl;
     }
-    apply_or_copy<PC>(k, f, n, s.y, k.z_out + o, red);
+    tile_block_apply<TB, D>(Bk, L, rw, pre, L.y, k.z_out + T.U0);
+    __syncthreads();
     if (threadIdx.x == 0) {
-        double m = wr[0], d = wr[kApplyWaves];
-        for (int q = 1; q < kApplyWaves; ++q) {
+        double m = wr[0], d = wr[kTileWaves];
+        for (int q = 1; q < kTileWaves; ++q) {
             m = nan_max(m, wr[q]);
-            d += wr[kApplyWaves + q];
+            d += wr[kTileWaves + q];
         }
-        k.part[P_RMAX * k.nfib + f] = m;
-        k.part[P_DELTA * k.nfib + f] = d;
+        k.part[P_RMAX * nt + T.t] = m;
+        k.part[P_DELTA * nt + T.t] = d;
     }
 }
 
 template <int D, int PC>
-__global__ __launch_bounds__(kApplyThreads) void k_fq(KArgs k)
+__global__ __launch_bounds__(kTileThreads) void k_tq(KArgs k)
 {
-    constexpr int R = (kMaxFiberPoints * D + kApplyThreads - 1) / kApplyThreads;
-    __shared__ FiberLds<D> s;
-    __shared__ double red[kApplyWaves * kMaxFiberPoints * D];
-    __shared__ double wr[5 * kApplyWaves];
-    __shared__ double bc[4];
-    const uint32_t f = blockIdx.x;
+    constexpr int R = TileRows<D>::R;
+    __shared__ TileLds<D> L;
+    __shared__ double wr[5 * kTileWaves];
+    __shared__ double bc[2];
     const FiberArgs& a = k.fa;
-    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
-    const uint32_t np = a.np[f], o = D * a.idx[f];
-    const int n = D * (int)np;
-    double vr[R], vv[R], vx[R], vph[R], vy[R], vz[R];
+    const Tile T = tile_begin<D>(k.ta, blockIdx.x);
+    double vr[R], vv[R], vx[R], vph[R], vy[R], vz[R], vrt[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-        const int i = threadIdx.x + q * kApplyThreads;
-        if (i < n) {
-            vr[q] = k.r[o + i];
-            vv[q] = k.v[o + i];
-            vx[q] = k.x[o + i];
-            vph[q] = k.ph_in[o + i];
-            vy[q] = k.y[o + i];
-            vz[q] = k.z_in[o + i];
+        const int u = threadIdx.x + q * kTileThreads;
+        if (u < (int)T.nu) {
+            const uint32_t g = T.U0 + u;
+            vr[q] = k.r[g];
+            vv[q] = k.v[g];
+            vx[q] = k.x[g];
+            vph[q] = k.ph_in[g];
+            vy[q] = k.y[g];
+            vz[q] = k.z_in[g];
+            vrt[q] = k.rt[g];
         }
     }
-    SysScalars sc = k.st_in[sys];
-    const bool first = first_of_system(a, sys);
+    SysScalars sc = k.st_in[T.sys];
+    const bool first = first_tile_of_system(T);
+    const uint32_t nt = k.ta.ntiles;
     if (!sc.done) {
-        const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
-        if (threadIdx.x < kWave) {
-            const double res = wmax(k.part + P_RMAX * k.nfib, fb, fe);
-            const double dsum = wsum(k.part + P_DELTA * k.nfib, fb, fe);
-            if (threadIdx.x == 0) { bc[0] = res; bc[1] = dsum; }
+        tile_tables<D, PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
+        const uint32_t tb = T.tb, te = T.te;
+        if (threadIdx.x >= kWave && threadIdx.x < 2 * kWave) {
+            const double res = wmax(k.part + P_RMAX * nt, tb, te);
+            const double dsum = wsum(k.part + P_DELTA * nt, tb, te);
+            if (threadIdx.x == kWave) { bc[0] = res; bc[1] = dsum; }
         }
         __syncthreads();
         decide(sc, bc[0]);
@@ -704,34 +777,41 @@ __global__ __launch_bounds__(kApplyThreads) void k_fq(KArgs k)
         }
         sc.stage = ST_C;
     }
-    if (first && threadIdx.x == 0) k.st_out[sys] = sc;
+    if (first && threadIdx.x == 0) k.st_out[T.sys] = sc;
     if (sc.done) return;
+    using TB = typename PcBlocks<PC>::T;
+    const TB* Bk = PcBlocks<PC>::get(k);
+    TileRows<D> rw;
+    tile_rows<D>(L, T, rw);
+    TilePre<D, TB> pre;
+    tile_prefetch<D, TB>(a, L, T, rw, Bk, pre);
     const ShatOp op{ k.y, k.z_in, sc.alpha };
     double smax = 0.0;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-        const int i = threadIdx.x + q * kApplyThreads;
-        if (i >= n) continue;
+        if (rw.k[q] < 0) continue;
+        const int u = threadIdx.x + q * kTileThreads;
+        const uint32_t g = T.U0 + u;
         const double si = vr[q] + (-sc.alpha) * vv[q];
-        k.r[o + i] = si;
-        k.x[o + i] = vx[q] + sc.alpha * vph[q];
+        k.r[g] = si;
+        k.x[g] = vx[q] + sc.alpha * vph[q];
         smax = nan_max(smax, fabs(si));
         const double shi = vy[q] + (-sc.alpha) * vz[q];
-        k.sh[o + i] = shi;
-        s.x[i] = shi;
+        k.sh[g] = shi;
+        L.x[u] = shi;
     }
-    load_factors<D>(a, s, a.seg[f], (int)np - 1);
     __syncthreads();
-    fiber_operator_op<D>(a, s, op, f);
+    tile_operator<D>(a, L, op, T, rw, pre);
     double tt = 0, ts = 0, rts = 0, rtt = 0;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-        const int i = threadIdx.x + q * kApplyThreads;
-        if (i >= n) continue;
-        const double ti = s.y[i];
+        if (rw.k[q] < 0) continue;
+        const int u = threadIdx.x + q * kTileThreads;
+        const uint32_t g = T.U0 + u;
+        const double ti = L.y[u];
         const double si = vr[q] + (-sc.alpha) * vv[q];
-        const double rti = k.rt[o + i];
-        k.t[o + i] = ti;
+        const double rti = vrt[q];
+        k.t[g] = ti;
         tt += ti * ti;
         ts += ti * si;
         rts += rti * si;
@@ -745,20 +825,26 @@ __global__ __launch_bounds__(kApplyThreads) void k_fq(KArgs k)
     if ((threadIdx.x & (kWave - 1)) == 0) {
         const int wv = threadIdx.x / kWave;
         wr[wv] = smax;
-        wr[kApplyWaves + wv] = tt;
-        wr[2 * kApplyWaves + wv] = ts;
-        wr[3 * kApplyWaves + wv] = rts;
-        wr[4 * kApplyWaves + wv] = rtt;
+        wr[kTileWaves + wv] = tt;
+        wr[2 * kTileWaves + wv] = ts;
+        wr[3 * kTileWaves + wv] = rts;
+        wr[4 * kTileWaves + wv] = rtt;
     }
-    apply_or_copy<PC>(k, f, n, s.y, k.w + o, red);
+    tile_block_apply<TB, D>(Bk, L, rw, pre, L.y, k.w + T.U0);
+    __syncthreads();
     if (threadIdx.x == 0) {
-        double m = wr[0], S[4] = { wr[kApplyWaves], wr[2 * kApplyWaves], wr[3 * kApplyWaves], wr[4 * kApplyWaves] };
-        for (int q = 1; q < kApplyWaves; ++q) {
+        double m = wr[0], s0 = wr[kTileWaves], s1 = wr[2 * kTileWaves], s2 = wr[3 * kTileWaves],
+               s3 = wr[4 * kTileWaves];
+#pragma unroll
+        for (int q = 1; q < kTileWaves; ++q) {
             m = nan_max(m, wr[q]);
-            for (int j = 0; j < 4; ++j) S[j] += wr[(j + 1) * kApplyWaves + q];
+            s0 += wr[kTileWaves + q];
+            s1 += wr[2 * kTileWaves + q];
+            s2 += wr[3 * kTileWaves + q];
+            s3 += wr[4 * kTileWaves + q];
         }
-        k.part[P_SMAX * k.nfib + f] = m;
-        *reinterpret_cast<double4*>(k.part4 + 4 * (size_t)f) = make_double4(S[0], S[1], S[2], S[3]);
+        k.part[P_SMAX * nt + T.t] = m;
+        *reinterpret_cast<double4*>(k.part4 + 4 * (size_t)T.t) = make_double4(s0, s1, s2, s3);
     }
 }
 
@@ -784,6 +870,9 @@ static KArgs make_kargs(Context& c)
     k.z_in = k.z_out = k.ph_in = k.ph_out = nullptr;
     k.part = c.d_part.get();
     k.part4 = c.d_part.get() + (size_t)P_TT * c.nfib;
+    k.fpart = c.d_part.get();
+    k.ta.desc = c.d_tile_desc.get();
+    k.ta.ntiles = c.ntiles;
     k.st_in = nullptr;
     k.st_out = nullptr;
     k.nfib = c.nfib;
@@ -860,10 +949,12 @@ static void run(Context& c, const double* guess, int max_iter, double tol, int t
 }
 
 template <int D, int PC>
-static void run_fused(Context& c, const double* guess, int max_iter, double tol, int tol_mode,
+static void run_tiled(Context& c, const double* guess, int max_iter, double tol, int tol_mode,
                       cymg_solve_report* rep)
 {
     KArgs k = make_kargs(c);
+    k.part = c.d_tpart.get();
+    k.part4 = c.d_tpart.get() + (size_t)P_TT * c.ntiles;
     SysScalars* S0 = c.d_scal.get();
     SysScalars* S1 = S0 + c.nsys;
     double* ph[2] = { c.k_ph.get(), c.k_ph2.get() };
@@ -882,8 +973,8 @@ static void run_fused(Context& c, const double* guess, int max_iter, double tol,
     const KArgs kI = with(S0, S1, 0);
     const KArgs kP0 = with(S1, S0, 0), kQ0 = with(S0, S1, 1);
     const KArgs kP1 = with(S1, S0, 1), kQ1 = with(S0, S1, 0);
-    const dim3 g(c.nfib), b(kApplyThreads);
-    hipLaunchKernelGGL((k_finit<D, PC>), g, b, 0, c.stream, kI, guess, tol, tol_mode, max_iter, c.prm.time_step);
+    const dim3 g(c.ntiles), b(kTileThreads);
+    hipLaunchKernelGGL((k_tinit<D, PC>), g, b, 0, c.stream, kI, guess, tol, tol_mode, max_iter, c.prm.time_step);
     CYMG_HIP(hipGetLastError());
     int launched = 0;
     int& predicted = c.last_iters[PC];
@@ -900,10 +991,10 @@ static void run_fused(Context& c, const double* guess, int max_iter, double tol,
         }
         first = false;
         for (int it = 0; it < chunk; it += 2) {
-            hipLaunchKernelGGL((k_fp<D, PC>), g, b, 0, c.stream, kP0);
-            hipLaunchKernelGGL((k_fq<D, PC>), g, b, 0, c.stream, kQ0);
-            hipLaunchKernelGGL((k_fp<D, PC>), g, b, 0, c.stream, kP1);
-            hipLaunchKernelGGL((k_fq<D, PC>), g, b, 0, c.stream, kQ1);
+            hipLaunchKernelGGL((k_tp<D, PC>), g, b, 0, c.stream, kP0);
+            hipLaunchKernelGGL((k_tq<D, PC>), g, b, 0, c.stream, kQ0);
+            hipLaunchKernelGGL((k_tp<D, PC>), g, b, 0, c.stream, kP1);
+            hipLaunchKernelGGL((k_tq<D, PC>), g, b, 0, c.stream, kQ1);
         }
         CYMG_HIP(hipGetLastError());
         launched += chunk;
@@ -931,25 +1022,95 @@ static void run_fused(Context& c, const double* guess, int max_iter, double tol,
     predicted = c.h_scal[worst].iter;
 }
 
+// Average duration of one launch of the two-launch iteration (P and Q
+// alternately), from events around `pairs` iterations on the context's
+// stream.  The iteration runs with a zero threshold so that every launch does
+// its full work; an iteration that stops early (breakdown) is an error.
+template <int D, int PC>
+static double time_tiled(Context& c, int pairs)
+{
+    KArgs k = make_kargs(c);
+    k.part = c.d_tpart.get();
+    k.part4 = c.d_tpart.get() + (size_t)P_TT * c.ntiles;
+    SysScalars* S0 = c.d_scal.get();
+    SysScalars* S1 = S0 + c.nsys;
+    double* ph[2] = { c.k_ph.get(), c.k_ph2.get() };
+    double* zz[2] = { c.k_z.get(), c.k_z2.get() };
+    auto with = [&](SysScalars* in, SysScalars* out, int par) {
+        KArgs q = k;
+        q.st_in = in;
+        q.st_out = out;
+        q.ph_in = ph[par];
+        q.ph_out = ph[par ^ 1];
+        q.z_in = zz[par];
+        q.z_out = zz[par ^ 1];
+        return q;
+    };
+    const KArgs kI = with(S0, S1, 0);
+    const KArgs kP0 = with(S1, S0, 0), kQ0 = with(S0, S1, 1);
+    const KArgs kP1 = with(S1, S0, 1), kQ1 = with(S0, S1, 0);
+    const dim3 g(c.ntiles), b(kTileThreads);
+    hipLaunchKernelGGL((k_tinit<D, PC>), g, b, 0, c.stream, kI, nullptr, 0.0, CYMG_TOL_RELATIVE, 1 << 30,
+                       c.prm.time_step);
+    pairs += pairs & 1;
+    CYMG_HIP(hipEventRecord(c.ev[8], c.stream));
+    for (int it = 0; it < pairs; it += 2) {
+        hipLaunchKernelGGL((k_tp<D, PC>), g, b, 0, c.stream, kP0);
+        hipLaunchKernelGGL((k_tq<D, PC>), g, b, 0, c.stream, kQ0);
+        hipLaunchKernelGGL((k_tp<D, PC>), g, b, 0, c.stream, kP1);
+        hipLaunchKernelGGL((k_tq<D, PC>), g, b, 0, c.stream, kQ1);
+    }
+    CYMG_HIP(hipGetLastError());
+    CYMG_HIP(hipEventRecord(c.ev[9], c.stream));
+    CYMG_HIP(hipEventSynchronize(c.ev[9]));
+    float ms = 0;
+    CYMG_HIP(hipEventElapsedTime(&ms, c.ev[8], c.ev[9]));
+    CYMG_HIP(hipMemcpyAsync(c.h_scal, S1, sizeof(SysScalars) * c.nsys, hipMemcpyDeviceToHost, c.stream));
+    CYMG_HIP(hipStreamSynchronize(c.stream));
+    for (uint32_t s = 0; s < c.nsys; ++s)
+        if (c.h_scal[s].done) throw Error(CYMG_ERR_INVALID, "the timed iteration stopped early; use fewer launches");
+    return ms / (2.0 * pairs);
+}
+
+static void tiled_buffers(Context& c)
+{
+    const size_t n = c.n();
+    const bool fresh = c.k_y.cap < n;
+    c.d_tpart.resize((size_t)P_SLOTS * (c.ntiles ? c.ntiles : 1));
+    c.k_r.resize(n); c.k_rt.resize(n); c.k_v.resize(n); c.k_t.resize(n); c.k_sh.resize(n);
+    c.k_ph.resize(n); c.k_ph2.resize(n); c.k_y.resize(n); c.k_z.resize(n); c.k_z2.resize(n); c.k_w.resize(n);
+    if (fresh)
+        for (DevBuf<double>* v : { &c.k_ph, &c.k_ph2, &c.k_z, &c.k_z2, &c.k_w, &c.k_sh, &c.k_t })
+            CYMG_HIP(hipMemsetAsync(v->get(), 0, v->cap * sizeof(double), c.stream));
+}
+
+double krylov_time_launch(Context& c, int precond, int launches)
+{
+    tiled_buffers(c);
+    const int pairs = std::max(1, launches / 2);
+    if (c.dim == 3) {
+        if (precond == CYMG_PRECOND_FP64) return time_tiled<3, CYMG_PRECOND_FP64>(c, pairs);
+        if (precond == CYMG_PRECOND_FP32) return time_tiled<3, CYMG_PRECOND_FP32>(c, pairs);
+        return time_tiled<3, CYMG_PRECOND_NONE>(c, pairs);
+    }
+    if (precond == CYMG_PRECOND_FP64) return time_tiled<2, CYMG_PRECOND_FP64>(c, pairs);
+    if (precond == CYMG_PRECOND_FP32) return time_tiled<2, CYMG_PRECOND_FP32>(c, pairs);
+    return time_tiled<2, CYMG_PRECOND_NONE>(c, pairs);
+}
+
 void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter, double tol, int tol_mode,
                   cymg_solve_report* rep)
 {
     if (c.krylov_fused) {
-        const size_t n = c.n();
-        const bool fresh = c.k_y.cap < n;
-        c.k_r.resize(n); c.k_rt.resize(n); c.k_v.resize(n); c.k_t.resize(n); c.k_sh.resize(n);
-        c.k_ph.resize(n); c.k_ph2.resize(n); c.k_y.resize(n); c.k_z.resize(n); c.k_z2.resize(n); c.k_w.resize(n);
-        if (fresh)
-            for (DevBuf<double>* v : { &c.k_ph, &c.k_ph2, &c.k_z, &c.k_z2, &c.k_w, &c.k_sh, &c.k_t })
-                CYMG_HIP(hipMemsetAsync(v->get(), 0, v->cap * sizeof(double), c.stream));
+        tiled_buffers(c);
         if (c.dim == 3) {
-            if (precond == CYMG_PRECOND_FP64) run_fused<3, CYMG_PRECOND_FP64>(c, guess_dev, max_iter, tol, tol_mode, rep);
-            else if (precond == CYMG_PRECOND_FP32) run_fused<3, CYMG_PRECOND_FP32>(c, guess_dev, max_iter, tol, tol_mode, rep);
-            else run_fused<3, CYMG_PRECOND_NONE>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            if (precond == CYMG_PRECOND_FP64) run_tiled<3, CYMG_PRECOND_FP64>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            else if (precond == CYMG_PRECOND_FP32) run_tiled<3, CYMG_PRECOND_FP32>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            else run_tiled<3, CYMG_PRECOND_NONE>(c, guess_dev, max_iter, tol, tol_mode, rep);
         } else {
-            if (precond == CYMG_PRECOND_FP64) run_fused<2, CYMG_PRECOND_FP64>(c, guess_dev, max_iter, tol, tol_mode, rep);
-            else if (precond == CYMG_PRECOND_FP32) run_fused<2, CYMG_PRECOND_FP32>(c, guess_dev, max_iter, tol, tol_mode, rep);
-            else run_fused<2, CYMG_PRECOND_NONE>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            if (precond == CYMG_PRECOND_FP64) run_tiled<2, CYMG_PRECOND_FP64>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            else if (precond == CYMG_PRECOND_FP32) run_tiled<2, CYMG_PRECOND_FP32>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            else run_tiled<2, CYMG_PRECOND_NONE>(c, guess_dev, max_iter, tol, tol_mode, rep);
         }
         return;
     }

@@ -31,7 +31,8 @@ struct PcArgs {
     double dt;
     int bs_max;
     int factor;   // 0: store the assembled block M itself (for export)
-    int method;   // 0: Gauss-Jordan (default), 1: getrf + getri
+    int method;   // 0 or 2: Gauss-Jordan, 1: getrf + getri
+    const uint32_t* list;   // fibers of the launch (null: all, blockIdx.x = fiber)
 };
 
 // LU with partial pivoting (dgetf2 order), in place, column-major, lda = bs.
@@ -211,7 +212,7 @@ template <int D, typename T>
 __global__ __launch_bounds__(kPcThreads) void k_precond_build(PcArgs a, T* __restrict__ out, int* __restrict__ useblk)
 {
     extern __shared__ double lds[];
-    const uint32_t f = blockIdx.x;
+    const uint32_t f = a.list ? a.list[blockIdx.x] : blockIdx.x;
     const int np = (int)a.np[f];
     const int S = np - 1;
     const int bs = D * np;
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(kPcThreads) void k_precond_build(PcArgs a, T* __res
         for (int i = tid; i < bs * bs; i += kPcThreads) dst[i] = M[i];
         return;
     }
-    if (a.method == 0) {
+    if (a.method != 1) {
         lds_gauss_jordan<T>(M, bs, ipiv, &misc[0], &misc[1], work, work + bs);
     } else {
         lds_getrf<T>(M, bs, ipiv, &misc[0], &misc[1]);
@@ -355,6 +356,171 @@ size_t precond_lds_bytes(int D, int bs_max, int precision)
     return b;
 }
 
+// ---------------------------------------------------------------------------
+// Gauss-Jordan inverse with the block in registers: one workgroup of NW
+// wavefronts per block, lane i owns row i (bs <= 64 * NW, bs <= BSMAX).  At
+// step k the candidate pivots are the rows not yet used as pivot (partial
+// pivoting by row search, largest magnitude, lowest row on ties: the pivot
+// dgetf2 takes, meca.cc:352-353); the pivot row is scaled and broadcast
+// through LDS, every other row eliminates column k.  The row array rotates
+// left by one column per step, so column k is always a[0] and the inverse
+// column of the step is appended at a[BSMAX-1]: all register indices are
+// compile-time constants.  After bs steps, the row that was pivot of step k
+// holds row k of inv(P A) with its columns in step order, i.e. row k of
+// inv(A) with column j stored at perm[j] (the pivot row of step j).  A zero
+// pivot marks the block unusable (meca.cc:437-438).  The assembled FP64 block
+// is read from M; the inverse (type T) is written to out at the same offset.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T, int BSMAX, int NW>
+__global__ __launch_bounds__(NW * kWave) void k_block_gj(const uint32_t* __restrict__ list,
+                                                         const uint32_t* __restrict__ npv,
+                                                         const uint64_t* __restrict__ blkoff, int D,
+                                                         const double* __restrict__ M, T* __restrict__ out,
+                                                         int* __restrict__ useblk)
+{
+    static_assert(BSMAX <= NW * kWave, "one row per lane");
+    __shared__ T row[BSMAX];
+    __shared__ int perm[BSMAX];
+    __shared__ T wbest[NW], wval[NW];
+    __shared__ int widx[NW];
+    const uint32_t f = list[blockIdx.x];
+    const int bs = D * (int)npv[f];
+    const uint64_t off = blkoff[f];
+    const int i = threadIdx.x;
+    const int lane = i & (kWave - 1);
+    const int w = i / kWave;
+    T a[BSMAX];
+#pragma unroll
+    for (int j = 0; j < BSMAX; ++j) a[j] = (j < bs && i < bs) ? (T)M[off + i + (size_t)bs * j] : T(0);
+    bool used = false;
+    int kst = 0, info = 0;
+    for (int k = 0; k < bs; ++k) {
+        const T ak = a[0];
+        T best = (i < bs && !used) ? (T)fabs(ak) : T(-1);
+        int bi = i;
+        for (int o = 32; o > 0; o >>= 1) {
+            const T ob = __shfl_xor(best, o, kWave);
+            const int oi = __shfl_xor(bi, o, kWave);
+            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        int p;
+        T piv;
+        if constexpr (NW == 1) {
+            p = __builtin_amdgcn_readfirstlane(bi);
+            piv = __shfl(ak, p, kWave);
+        } else {
+            if (lane == 0) { wbest[w] = best; widx[w] = bi; }
+            if (i == bi) wval[w] = ak;
+            __syncthreads();
+            T bb = wbest[0];
+            int bj = widx[0], ww = 0;
+#pragma unroll
+            for (int q = 1; q < NW; ++q)
+                if (wbest[q] > bb || (wbest[q] == bb && widx[q] < bj)) { bb = wbest[q]; bj = widx[q]; ww = q; }
+            best = bb;
+            p = bj;
+            piv = wval[ww];
+        }
+        if (!(best > T(0)) || piv == T(0)) {
+            info = k + 1;
+            break;
+        }
+        const T r = T(1) / piv;
+        const bool isp = (i == p);
+        if (isp) {
+#pragma unroll
+            for (int j = 1; j < BSMAX; ++j) {
+                a[j] = a[j] * r;
+                row[j - 1] = a[j];
+            }
+            used = true;
+            kst = k;
+        }
+        if (i == 0) perm[k] = p;
+        if constexpr (NW == 1) wave_sync();
+        else __syncthreads();
+        const T m = isp ? T(0) : ak;
+#pragma unroll
+        for (int j = 1; j < BSMAX; ++j) a[j - 1] = a[j] - m * row[j - 1];
+        a[BSMAX - 1] = isp ? r : -m * r;
+    }
+    if constexpr (NW == 1) wave_sync();
+    else __syncthreads();
+    if (info == 0 && i < bs) {
+        T* dst = out + off;
+#pragma unroll
+        for (int t = 0; t < BSMAX; ++t) {
+            const int j = t - (BSMAX - bs);
+            if (j >= 0) dst[kst + (size_t)bs * perm[j]] = a[t];
+        }
+    }
+    if (i == 0) useblk[f] = (info == 0);
+}
+
+// Fibers grouped by block size for the register kernels: [<=32 | <=64 | <=96 |
+// <=128 | larger], uploaded once per layout.
+static void gj_lists(Context& c)
+{
+    if (c.gj_nfib == c.nfib && c.gj_largest == c.largest && !c.gj_start.empty()) return;
+    const int edges[4] = { 32, 64, 96, 128 };
+    std::vector<uint32_t> lists[5];
+    for (uint32_t f = 0; f < c.nfib; ++f) {
+        const int bs = c.dim * (int)c.h_np[f];
+        int b = 0;
+        while (b < 4 && bs > edges[b]) ++b;
+        lists[b].push_back(f);
+    }
+    std::vector<uint32_t> all;
+    c.gj_start.assign(6, 0);
+    for (int b = 0; b < 5; ++b) {
+        c.gj_start[b] = (uint32_t)all.size();
+        all.insert(all.end(), lists[b].begin(), lists[b].end());
+    }
+    c.gj_start[5] = (uint32_t)all.size();
+    c.d_gj_list.resize(all.size() ? all.size() : 1);
+    if (!all.empty()) {
+        CYMG_HIP(hipMemcpyAsync(c.d_gj_list.get(), all.data(), all.size() * 4, hipMemcpyHostToDevice, c.stream));
+        CYMG_HIP(hipStreamSynchronize(c.stream));   // pageable source
+    }
+    c.gj_nfib = c.nfib;
+    c.gj_largest = c.largest;
+}
+
+template <typename T, int BSMAX, int NW>
+static void launch_gj(Context& c, uint32_t begin, uint32_t end, T* out)
+{
+    if (end <= begin) return;
+    hipLaunchKernelGGL((k_block_gj<T, BSMAX, NW>), dim3(end - begin), dim3(NW * kWave), 0, c.stream,
+                       c.d_gj_list.get() + begin, c.d_np.get(), c.d_blkoff.get(), c.dim, c.d_binv.get(), out,
+                       c.d_useblk.get());
+}
+
+static void launch_lds_build(Context& c, const PcArgs& a0, int precision, void* dst, const uint32_t* list,
+                             uint32_t count)
+{
+    if (!count) return;
+    PcArgs a = a0;
+    a.list = list;
+    const int D = c.dim;
+    const size_t lds = precond_lds_bytes(D, a.bs_max, precision);
+    const dim3 g(count), b(kPcThreads);
+    if (precision == CYMG_PRECOND_FP32) {
+        auto kern = D == 3 ? k_precond_build<3, float> : k_precond_build<2, float>;
+        CYMG_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kern, g, b, lds, c.stream, a, (float*)dst, c.d_useblk.get());
+    } else {
+        auto kern = D == 3 ? k_precond_build<3, double> : k_precond_build<2, double>;
+        CYMG_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kern, g, b, lds, c.stream, a, (double*)dst, c.d_useblk.get());
+    }
+}
+
 void launch_precond_build(Context& c, int precision, double* block_out)
 {
     if (!c.nfib) return;
@@ -372,42 +538,49 @@ void launch_precond_build(Context& c, int precision, double* block_out)
     a.Crow = c.mC.rowptr.get(); a.Ccol = c.mC.col.get(); a.Cval = c.mC.val.get(); a.Cuse = c.mC.used;
     a.dt = c.prm.time_step;
     a.bs_max = bs_max;
-    a.factor = block_out ? 0 : 1;
+    a.factor = 1;
     a.method = c.pc_method;
+    a.list = nullptr;
     c.d_useblk.resize(c.nfib);
-    const dim3 g(c.nfib), b(kPcThreads);
-    if (precision == CYMG_PRECOND_FP32) {
-        c.d_binv32.resize(c.blk_total);
-        if (D == 3) {
-            CYMG_HIP(hipFuncSetAttribute((const void*)k_precond_build<3, float>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL((k_precond_build<3, float>), g, b, lds, c.stream, a, c.d_binv32.get(), c.d_useblk.get());
-        } else {
-            CYMG_HIP(hipFuncSetAttribute((const void*)k_precond_build<2, float>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL((k_precond_build<2, float>), g, b, lds, c.stream, a, c.d_binv32.get(), c.d_useblk.get());
-        }
+    if (block_out) {
+        // the assembled blocks themselves
+        a.factor = 0;
+        launch_lds_build(c, a, CYMG_PRECOND_FP64, block_out, nullptr, c.nfib);
+        CYMG_HIP(hipGetLastError());
+        return;
+    }
+    c.d_binv.resize(c.blk_total);
+    if (precision == CYMG_PRECOND_FP32) c.d_binv32.resize(c.blk_total);
+    void* dst = precision == CYMG_PRECOND_FP32 ? (void*)c.d_binv32.get() : (void*)c.d_binv.get();
+    if (c.pc_method != 0) {
+        // LDS kernels for every block: 1 getrf + getri, 2 Gauss-Jordan
+        launch_lds_build(c, a, precision, dst, nullptr, c.nfib);
     } else {
-        double* dst = block_out;
-        if (!dst) {
-            c.d_binv.resize(c.blk_total);
-            dst = c.d_binv.get();
-        }
-        if (D == 3) {
-            CYMG_HIP(hipFuncSetAttribute((const void*)k_precond_build<3, double>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL((k_precond_build<3, double>), g, b, lds, c.stream, a, dst, c.d_useblk.get());
+        // assemble every block into d_binv, invert in registers by size class;
+        // blocks beyond the register classes are rebuilt and inverted in LDS
+        gj_lists(c);
+        PcArgs e = a;
+        e.factor = 0;
+        launch_lds_build(c, e, CYMG_PRECOND_FP64, c.d_binv.get(), nullptr, c.nfib);
+        const std::vector<uint32_t>& st = c.gj_start;
+        if (precision == CYMG_PRECOND_FP32) {
+            float* o = c.d_binv32.get();
+            launch_gj<float, 32, 1>(c, st[0], st[1], o);
+            launch_gj<float, 64, 1>(c, st[1], st[2], o);
+            launch_gj<float, 96, 2>(c, st[2], st[3], o);
+            launch_gj<float, 128, 2>(c, st[3], st[4], o);
+            launch_lds_build(c, a, precision, dst, c.d_gj_list.get() + st[4], st[5] - st[4]);
         } else {
-            CYMG_HIP(hipFuncSetAttribute((const void*)k_precond_build<2, double>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL((k_precond_build<2, double>), g, b, lds, c.stream, a, dst, c.d_useblk.get());
+            double* o = c.d_binv.get();
+            launch_gj<double, 32, 1>(c, st[0], st[1], o);
+            launch_gj<double, 64, 1>(c, st[1], st[2], o);
+            launch_gj<double, 96, 2>(c, st[2], st[3], o);
+            launch_lds_build(c, a, precision, dst, c.d_gj_list.get() + st[3], st[5] - st[3]);
         }
     }
     CYMG_HIP(hipGetLastError());
-    if (!block_out) {
-        c.pc_precision = precision;
-        c.pc_ready = true;
-    }
+    c.pc_precision = precision;
+    c.pc_ready = true;
 }
 
 // standalone Meca::precondition

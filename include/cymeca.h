@@ -179,8 +179,13 @@ int         cymg_get_matrix(cymg_context* ctx, int which, uint32_t* nnz, uint32_
    the solver tolerance, see the parity tests):
    CYMG_OPT_KRYLOV_LAUNCHES: 2 (default, recurrences for the preconditioned
      vectors) or 4 launches per BiCGStab iteration;
-   CYMG_OPT_BLOCK_INVERSE: 0 Gauss-Jordan (default) or 1 getrf + getri order */
-enum { CYMG_OPT_KRYLOV_LAUNCHES = 1, CYMG_OPT_BLOCK_INVERSE = 2 };
+   CYMG_OPT_BLOCK_INVERSE: 0 Gauss-Jordan with the block in registers
+     (default; blocks beyond 96 unknowns in FP64, 128 in FP32, use 2),
+     1 getrf + getri order in LDS, 2 Gauss-Jordan in LDS;
+   CYMG_OPT_TILE_POINTS: target points per tile of whole fibers in the
+     two-launch iteration, 1..CYMG_MAX_FIBER_POINTS (1: one fiber per tile),
+     0 (default): chosen from the layout */
+enum { CYMG_OPT_KRYLOV_LAUNCHES = 1, CYMG_OPT_BLOCK_INVERSE = 2, CYMG_OPT_TILE_POINTS = 3 };
 int         cymg_set_option(cymg_context* ctx, int option, int value);
 
 /* device timings of the last solve, milliseconds */
@@ -200,7 +205,9 @@ int         cymg_enable_timing(cymg_context* ctx, int enable);
    around `reps` back-to-back launches on the context's stream, on the current
    system state: which = 0 fused operator (Meca::multiply), 1 block apply
    (Meca::precondition), 2 block build + factorisation (FP64), 3 the same in FP32,
-   4 block build without factorisation, 5 build with getrf + getri instead of Gauss-Jordan */
+   4 block build without factorisation, 5 build with getrf + getri instead of Gauss-Jordan,
+   6 one launch of the two-launch BiCGStab iteration with the current preconditioner
+     (`reps` launches after an init launch, zero threshold; no counterpart in the reference) */
 int         cymg_time_kernel(cymg_context* ctx, int which, int reps, double* avg_ms);
 
 #ifdef __cplusplus

@@ -383,3 +383,28 @@ def reshape(dim, src, cut):
 
 def best_nb_points(ratio):
     return int(lib().orc_best_nb_points(float(ratio)))
+
+
+def assemble_scene(m, sc):
+    """Records the scene's interactions on an OracleMeca in the call order of
+    Simul::setInteractions (simul_solve.cc): fibers' sphere confinement, then
+    the crosslinkers (Couple::setInteractions -> Meca::interLink)."""
+    if sc.confine_radius > 0:
+        m.confine_sphere(sc.confine_radius, sc.confine_stiffness)
+    if sc.nlinks:
+        a1, a2, ca = sc.interpolate(sc.link_fiber_a, sc.link_abs_a)
+        b1, b2, cb = sc.interpolate(sc.link_fiber_b, sc.link_abs_b)
+        for k in range(sc.nlinks):
+            m.inter_link_ii(int(a1[k]), int(a2[k]), float(ca[k]), int(b1[k]), int(b2[k]),
+                            float(cb[k]), float(sc.link_stiffness[k]))
+
+
+def from_scene(sc):
+    """OracleMeca loaded with one step of a scene: registration (Meca::add),
+    prepare, then the interactions."""
+    m = OracleMeca(sc.dim, sc.params())
+    for f in range(sc.nfib):
+        m.add_fiber(sc.nb_points[f], sc.segmentation[f], sc.rigidity[f])
+    m.prepare(sc.points)
+    assemble_scene(m, sc)
+    return m

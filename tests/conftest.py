@@ -33,25 +33,13 @@ def oracle():
 
 
 def oracle_from_scene(pyoracle, sc):
-    """Oracle Meca loaded with one step of the scene, in the call order of
-    Simul::setInteractions: fibers' confinement, then couples."""
-    m = pyoracle.OracleMeca(sc.dim, sc.params())
-    for f in range(sc.nfib):
-        m.add_fiber(sc.nb_points[f], sc.segmentation[f], sc.rigidity[f])
-    m.prepare(sc.points)
-    oracle_assemble(m, sc)
-    return m
+    """Oracle Meca loaded with one step of the scene."""
+    return pyoracle.from_scene(sc)
 
 
 def oracle_assemble(m, sc):
-    if sc.confine_radius > 0:
-        m.confine_sphere(sc.confine_radius, sc.confine_stiffness)
-    if sc.nlinks:
-        a1, a2, ca = sc.interpolate(sc.link_fiber_a, sc.link_abs_a)
-        b1, b2, cb = sc.interpolate(sc.link_fiber_b, sc.link_abs_b)
-        for k in range(sc.nlinks):
-            m.inter_link_ii(int(a1[k]), int(a2[k]), float(ca[k]), int(b1[k]), int(b2[k]),
-                            float(cb[k]), float(sc.link_stiffness[k]))
+    from oracle import pyoracle
+    pyoracle.assemble_scene(m, sc)
 
 
 def rel_err(a, b):

@@ -108,10 +108,11 @@ def test_blocks_and_precondition(pkg, scenes, oracle, dim):
     assert rel_err(gpu.precondition(x), orc.precondition(x)) < 1e-10
 
 
-def _step_pair(pkg, oracle, sc, steps, tol, tol_mode, precondition=1, launches=2, inverse=0):
+def _step_pair(pkg, oracle, sc, steps, tol, tol_mode, precondition=1, launches=2, inverse=0, tile_points=0):
     gpu = gpu_from_scene(pkg, sc)
     gpu.set_option(pkg.OPT_KRYLOV_LAUNCHES, launches)
     gpu.set_option(pkg.OPT_BLOCK_INVERSE, inverse)
+    gpu.set_option(pkg.OPT_TILE_POINTS, tile_points)
     orc = oracle_from_scene(oracle, sc)
     reps = []
     for s in range(steps):
@@ -141,7 +142,7 @@ def test_config1_one_iteration_and_positions(pkg, scenes, oracle):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-@pytest.mark.parametrize("launches,inverse", [(2, 0), (4, 0), (2, 1)])
+@pytest.mark.parametrize("launches,inverse", [(2, 0), (4, 0), (2, 1), (2, 2)])
 def test_crosslinked_step_fp64(pkg, scenes, oracle, dim, launches, inverse):
     sc = scenes.mini(dim=dim, seed=15, nfib=12, nlinks=40, radius=2.0)
     gpu, orc, reps = _step_pair(pkg, oracle, sc, 2, 1e-11, 1, launches=launches, inverse=inverse)
@@ -149,6 +150,28 @@ def test_crosslinked_step_fp64(pkg, scenes, oracle, dim, launches, inverse):
         assert r1["converged"] and r2["converged"]
     assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-9
     assert rel_err(gpu.vector(pkg.VEC_FOR), orc.vector(oracle.FOR)) < 1e-7
+
+
+@pytest.mark.parametrize("tile_points", [1, 32, 128])
+def test_tiles_of_fibers(pkg, scenes, oracle, tile_points):
+    """Two-launch iteration with one fiber per tile, tiles of <= 32 points
+    (several fibers, one wavefront solves their multipliers) and of <= 128
+    points (more than 64 segments: multipliers solved through LDS)."""
+    sc = scenes.config2(seed=24, nfib=80, nlinks=400, radius=4.0, lmin=2.0, lmax=20.0)
+    gpu, orc, reps = _step_pair(pkg, oracle, sc, 2, 1e-11, 1, tile_points=tile_points)
+    for r1, r2 in reps:
+        assert r1["converged"] and r2["converged"]
+    assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-9
+
+
+def test_long_fibers_unpreconditioned(pkg, scenes, oracle):
+    """Fibers of up to 101 points (100 segments in one tile) without the block
+    preconditioner."""
+    sc = scenes.config2(seed=25, nfib=30, nlinks=150, radius=4.0, lmin=20.0, lmax=50.0)
+    assert sc.nb_points.max() > 65
+    gpu, orc, reps = _step_pair(pkg, oracle, sc, 1, 1e-10, 1, precondition=pkg.PRECOND_NONE)
+    assert reps[0][0]["converged"]
+    assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-8
 
 
 def test_config2_reduced_fp64(pkg, scenes, oracle):
