@@ -95,21 +95,14 @@ void same_system(const Context& c, std::initializer_list<uint32_t> pts)
 
 // Tiles of consecutive whole fibers of one system, at most `target` points
 // (a longer fiber gets a tile of its own) and kTileMaxFibers fibers.  The
-// default target keeps the largest system at about 256 tiles, which bounds the
-// partial sums every tile re-reduces, and at least 32 points per tile.
+// default target is the largest tile whose unknowns get two threads each in
+// the iteration launches (256 / DIM points): few enough tiles that one round
+// of workgroups covers the 1000-fiber system on 256 compute units.
 void build_tiles(Context& c)
 {
     const std::vector<uint32_t>& sb = c.h_sys_begin_used;
     uint32_t target = c.tile_points;
-    if (!target) {
-        uint32_t most = 0;
-        for (uint32_t s = 0; s < c.nsys; ++s) {
-            const uint32_t f0 = sb[s], f1 = sb[s + 1];
-            const uint32_t pts = c.h_idx[f1 - 1] + c.h_np[f1 - 1] - c.h_idx[f0];
-            most = std::max(most, pts);
-        }
-        target = std::min<uint32_t>(std::max<uint32_t>((most + 255) / 256, 32), kMaxFiberPoints);
-    }
+    if (!target) target = std::min<uint32_t>(256 / (uint32_t)c.dim, kMaxFiberPoints);
     c.h_tile_fib.clear();
     c.h_tile_sys.clear();
     c.h_sys_tile.assign(c.nsys + 1, 0);
@@ -175,6 +168,11 @@ void upload_layout(Context& c)
     CYMG_HIP(hipMemcpyAsync(c.d_fib_sys.get(), sysv.data(), F * 4, hipMemcpyHostToDevice, st));
     CYMG_HIP(hipMemcpyAsync(c.d_sys_begin.get(), sb.data(), sb.size() * 4, hipMemcpyHostToDevice, st));
     c.h_sys_begin_used = sb;
+    std::vector<uint32_t> pf(c.npts);
+    for (uint32_t f = 0; f < F; ++f)
+        for (uint32_t p = 0; p < c.h_np[f]; ++p) pf[c.h_idx[f] + p] = f;
+    c.d_pfib.resize(c.npts ? c.npts : 1);
+    if (c.npts) CYMG_HIP(hipMemcpyAsync(c.d_pfib.get(), pf.data(), c.npts * 4, hipMemcpyHostToDevice, st));
     const size_t n = c.n();
     for (DevBuf<double>* v : { &c.d_pts, &c.d_pos, &c.d_sol, &c.d_bas, &c.d_rhs, &c.d_for, &c.d_tmp, &c.d_prev_sol })
         v->resize(n ? n : 1);
@@ -892,11 +890,15 @@ int cymg_enable_timing(cymg_context* ctx, int enable)
 int cymg_time_kernel(cymg_context* ctx, int which, int reps, double* avg_ms)
 {
     return guarded(ctx, [&](Context& c) {
-        if (!avg_ms || reps <= 0 || which < 0 || which > 6) throw Error(CYMG_ERR_INVALID, "invalid arguments");
+        if (!avg_ms || reps <= 0 || which < 0 || which > 7) throw Error(CYMG_ERR_INVALID, "invalid arguments");
         if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_time_kernel");
         if (which == 6) {
             if (!c.pc_ready) throw Error(CYMG_ERR_INVALID, "no preconditioner computed");
             *avg_ms = krylov_time_launch(c, c.pc_precision, reps);
+            return;
+        }
+        if (which == 7) {
+            *avg_ms = krylov_time_launch(c, CYMG_PRECOND_NONE, reps);
             return;
         }
         if (which == 1 && !c.pc_ready) throw Error(CYMG_ERR_INVALID, "no preconditioner computed");

@@ -31,6 +31,7 @@ struct Context {
 
     // per-fiber device arrays
     DevBuf<uint32_t> d_np, d_idx, d_seg, d_fib_sys, d_sys_begin;
+    DevBuf<uint32_t> d_pfib;           // fiber of each point
     DevBuf<TileDesc> d_tile_desc;
     DevBuf<uint64_t> d_blkoff;
     DevBuf<double> d_cut, d_rig, d_mob, d_rfrig;
@@ -70,7 +71,8 @@ struct Context {
 
     // Krylov work vectors and scalars
     DevBuf<double> k_r, k_rt, k_p, k_v, k_t, k_ph, k_sh;
-    DevBuf<double> k_y, k_z, k_z2, k_w, k_ph2;   // fused variant: Binv r, Binv v (2), Binv t, phat (2)
+    DevBuf<double4> k_g1[2];           // two-launch variant: (shat, w, phat, z) records, by parity
+    DevBuf<double2> k_g2;              // two-launch variant: (y, z) records
     int krylov_fused = 1;              // 1: two launches per iteration, 0: four
     DevBuf<double> d_part;             // partial reductions [slot][nfib]
     DevBuf<double> d_tpart;            // partial reductions [slot][tile]

@@ -24,6 +24,8 @@ struct FiberArgs {
     const double* jd0;        // JJ' diagonal 2|d_s|^2 (unfactored)
     const double* je0;        // JJ' off-diagonal -d_s.d_{s+1} (unfactored)
     const double* pdiff;      // mtJJtiJforce (P' coefficients)
+    const uint32_t* pfib;     // fiber of each point
+    uint32_t nseg;            // segments in total
     // full symmetric CSR of mB (rows = points) and mC (rows = dofs)
     const uint32_t* Brow; const uint32_t* Bcol; const double* Bval; int Buse;
     const uint32_t* Crow; const uint32_t* Ccol; const double* Cval; int Cuse;

@@ -18,6 +18,7 @@
 // scalar state into the other half of a double buffer.  The host reads the
 // stop flag between chunks of iterations.
 #include <algorithm>
+#include <type_traits>
 
 #include "args.h"
 #include "pc_ops.h"
@@ -45,13 +46,10 @@ struct KArgs {
     double* t;
     double* ph;
     double* sh;
-    // fused variant
-    double* y;          // Binv r
-    double* w;          // Binv t
-    double* z_in;       // Binv v of the previous iteration
-    double* z_out;
-    double* ph_in;      // phat of the previous iteration
-    double* ph_out;
+    // two-launch iteration: records gathered at coupled points
+    double4* g1a;       // (shat, w, phat, z) of the previous iteration: read by P
+    double4* g1b;       // the same for this iteration: P writes (phat, z), Q writes (shat, w)
+    double2* g2;        // (y, z) of this iteration: P writes, Q reads
     double* part;       // partial sums [slot][fiber] (four launches) or [slot][tile] (two)
     double* part4;      // [fiber or tile][4] partials of D
     const double* fpart;// per-fiber partials of the setup (noise, |b|_inf)
@@ -142,6 +140,28 @@ __device__ __forceinline__ void wsum4(const double* part4, uint32_t fb, uint32_t
     for (int q = 0; q < 4; ++q) out[q] = wave_sum(v[q]);
 }
 
+// The scalar state is read and written field by field: a whole-struct copy
+// lets the compiler keep part of it in a per-thread LDS array indexed by the
+// flat work-item id, which makes every wavefront read the workgroup size from
+// the dispatch packet in host memory at the start of the kernel.
+__device__ __forceinline__ SysScalars state_load(const SysScalars* __restrict__ p)
+{
+    SysScalars s;
+    s.rho1 = p->rho1; s.rho2 = p->rho2; s.alpha = p->alpha; s.omega = p->omega; s.beta = p->beta;
+    s.tol = p->tol; s.resid = p->resid; s.resid_s = p->resid_s; s.bnorm = p->bnorm; s.noise = p->noise;
+    s.iter = p->iter; s.iter_max = p->iter_max; s.done = p->done; s.flag = p->flag;
+    s.pending = p->pending; s.stage = p->stage;
+    return s;
+}
+
+__device__ __forceinline__ void state_store(SysScalars* __restrict__ p, const SysScalars& s)
+{
+    p->rho1 = s.rho1; p->rho2 = s.rho2; p->alpha = s.alpha; p->omega = s.omega; p->beta = s.beta;
+    p->tol = s.tol; p->resid = s.resid; p->resid_s = s.resid_s; p->bnorm = s.bnorm; p->noise = s.noise;
+    p->iter = s.iter; p->iter_max = s.iter_max; p->done = s.done; p->flag = s.flag;
+    p->pending = s.pending; p->stage = s.stage;
+}
+
 // Monitor::finished on |r|_inf (bicgstab.cc:38-48), then the rho = 0 breakdown
 __device__ __forceinline__ void decide(SysScalars& s, double res)
 {
@@ -216,7 +236,7 @@ __global__ __launch_bounds__(kWave) void k_init(KArgs k, const double* __restric
             w.iter = 0;
             w.iter_max = iter_max;
             w.stage = ST_INIT;
-            k.st_out[sys] = w;
+            state_store(k.st_out + sys, w);
         }
     }
 }
@@ -247,10 +267,10 @@ __global__ __launch_bounds__(kApplyThreads) void k_iter_a(KArgs k)
             vv[q] = k.v[o + i];
         }
     }
-    SysScalars sc = k.st_in[sys];
+    SysScalars sc = state_load(k.st_in + sys);
     const bool first = first_of_system(a, sys);
     if (sc.done) {
-        if (first && threadIdx.x == 0) k.st_out[sys] = sc;
+        if (first && threadIdx.x == 0) state_store(k.st_out + sys, sc);
         return;
     }
     const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
@@ -290,7 +310,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_iter_a(KArgs k)
         }
     }
     sc.stage = ST_A;
-    if (first && threadIdx.x == 0) k.st_out[sys] = sc;
+    if (first && threadIdx.x == 0) state_store(k.st_out + sys, sc);
     if (sc.done) return;
     double rmax = 0.0;
 #pragma unroll
@@ -327,13 +347,13 @@ __global__ __launch_bounds__(kWave) void k_decide(KArgs k, uint32_t nsys)
 {
     const uint32_t sys = blockIdx.x;
     if (sys >= nsys) return;
-    SysScalars sc = k.st_in[sys];
+    SysScalars sc = state_load(k.st_in + sys);
     if (sc.done || sc.stage != ST_A) return;
     const uint32_t fb = k.fa.sys_begin[sys], fe = k.fa.sys_begin[sys + 1];
     const double res = wmax(k.part + P_RMAX * k.nfib, fb, fe);
     if (threadIdx.x == 0) {
         decide(sc, res);
-        k.st_in[sys] = sc;
+        state_store(k.st_in + sys, sc);
     }
 }
 
@@ -349,14 +369,14 @@ __global__ __launch_bounds__(kWave) void k_iter_b(KArgs k)
     const uint32_t np = a.np[f], o = D * a.idx[f];
     const int n = D * (int)np;
     for (int i = threadIdx.x; i < n; i += kWave) s.x[i] = X[o + i];
-    SysScalars sc = k.st_in[sys];
+    SysScalars sc = state_load(k.st_in + sys);
     const bool first = first_of_system(a, sys);
     if (!sc.done) {
         const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
         decide(sc, wmax(k.part + P_RMAX * k.nfib, fb, fe));
         sc.stage = ST_B;
     }
-    if (first && threadIdx.x == 0) k.st_out[sys] = sc;
+    if (first && threadIdx.x == 0) state_store(k.st_out + sys, sc);
     if (sc.done) return;
     load_factors<D>(a, s, a.seg[f], (int)np - 1);
     __syncthreads();
@@ -395,7 +415,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_iter_c(KArgs k)
             vX[q] = X[o + i];
         }
     }
-    SysScalars sc = k.st_in[sys];
+    SysScalars sc = state_load(k.st_in + sys);
     const bool first = first_of_system(a, sys);
     if (!sc.done) {
         const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
@@ -409,7 +429,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_iter_c(KArgs k)
         else sc.alpha = sc.rho1 / delta;
         sc.stage = ST_C;
     }
-    if (first && threadIdx.x == 0) k.st_out[sys] = sc;
+    if (first && threadIdx.x == 0) state_store(k.st_out + sys, sc);
     if (sc.done) return;
     double smax = 0.0;
 #pragma unroll
@@ -444,7 +464,7 @@ __global__ __launch_bounds__(kWave) void k_iter_d(KArgs k)
     const uint32_t np = a.np[f], o = D * a.idx[f];
     const int n = D * (int)np;
     for (int i = threadIdx.x; i < n; i += kWave) s.x[i] = X[o + i];
-    SysScalars sc = k.st_in[sys];
+    SysScalars sc = state_load(k.st_in + sys);
     const bool first = first_of_system(a, sys);
     if (!sc.done) {
         const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
@@ -452,7 +472,7 @@ __global__ __launch_bounds__(kWave) void k_iter_d(KArgs k)
         sc.resid_s = (m != m) ? __builtin_inf() : m;
         sc.stage = ST_D;
     }
-    if (first && threadIdx.x == 0) k.st_out[sys] = sc;
+    if (first && threadIdx.x == 0) state_store(k.st_out + sys, sc);
     if (sc.done) return;
     load_factors<D>(a, s, a.seg[f], (int)np - 1);
     __syncthreads();
@@ -548,7 +568,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tinit(KArgs k, const double* _
     __shared__ double wr[kTileWaves];
     const FiberArgs& a = k.fa;
     const Tile T = tile_begin<D>(k.ta, blockIdx.x);
-    tile_tables<D, PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
+    tile_tables<PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
     for (int u = threadIdx.x; u < (int)T.nu; u += kTileThreads) L.x[u] = x0 ? x0[T.U0 + u] : 0.0;
     __syncthreads();
     using TB = typename PcBlocks<PC>::T;
@@ -574,7 +594,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tinit(KArgs k, const double* _
     rr = wave_sum(rr);
     if ((threadIdx.x & (kWave - 1)) == 0) wr[threadIdx.x / kWave] = rr;
     __syncthreads();
-    tile_block_apply<TB, D>(Bk, L, rw, pre, L.f, k.y + T.U0);
+    tile_block_apply<TB, D>(Bk, L, rw, pre, L.f, reinterpret_cast<double*>(k.g2 + T.U0), 2);
     if (threadIdx.x == 0) {
         double t = wr[0];
         for (int q = 1; q < kTileWaves; ++q) t += wr[q];
@@ -597,66 +617,406 @@ __global__ __launch_bounds__(kTileThreads) void k_tinit(KArgs k, const double* _
             st.iter = 0;
             st.iter_max = iter_max;
             st.stage = ST_INIT;
-            k.st_out[T.sys] = st;
+            state_store(k.st_out + T.sys, st);
         }
     }
 }
 
-template <int D, int PC>
-__global__ __launch_bounds__(kTileThreads) void k_tp(KArgs k)
+// ---------------------------------------------------------------------------
+// P and Q run 512 threads per tile.  A tile of at most kTpSplit unknowns
+// gives each row two threads (h = 0 owns the row, h = 1 helps): each takes
+// half of the row's mB and mC entries and half of the columns of its row of
+// the block inverse.  Everything that does not depend on the iteration
+// scalars is requested in the prologue: the tile descriptor, then the row's
+// vectors, CSR extents and entries, the raw vectors at the coupled points
+// (the operand of another tile is a combination of vectors the launch does
+// not write), P' and rfDiff constants; after the first barrier the block
+// inverse columns and the segment constants.  A longer tile (one fiber of
+// more than kTpSplit / DIM points) gives each row one thread.
+constexpr int kTpThreads = 512;
+constexpr int kTpWaves = kTpThreads / kWave;
+constexpr int kTpSplit = 256;
+constexpr int kTpPreB = 6;      // mB entries per thread in registers
+constexpr int kTpPreC = 3;      // mC entries per thread in registers
+constexpr int kTpBlk = 24;      // block-inverse columns per thread in registers
+
+template <int D>
+struct TpLds {
+    double x[kTileMaxPoints * D];      // operand slice
+    double f[kTileMaxPoints * D];      // forces
+    double y[kTileMaxPoints * D];      // operator result
+    double hb[kTileMaxPoints * D];     // helper's mB partial sum, then its block partial sum
+    double hc[kTileMaxPoints * D];     // helper's mC partial sum
+    double lam[kTileMaxPoints];
+    double pa[kTileMaxPoints], pb[kTileMaxPoints], pc[kTileMaxPoints];
+    double fk[kTileMaxFibers];
+    double fsca[kTileMaxFibers];
+    uint64_t fblk[kTileMaxFibers];
+    uint32_t fo[kTileMaxFibers + 1];
+    uint32_t fseg[kTileMaxFibers];
+    int fuse[kTileMaxFibers];
+    int maxS;
+    double bc[5];
+    double wr[5 * kTpWaves];
+};
+
+// the thread's share of a row: entries/columns [b, e) of a range of n,
+// the first half to h = 0, the second to h = 1 (all of it when not split)
+__device__ __forceinline__ void share(uint32_t b0, uint32_t b1, bool split, int h, uint32_t& b, uint32_t& e)
 {
-    constexpr int R = TileRows<D>::R;
-    __shared__ TileLds<D> L;
-    __shared__ double wr[2 * kTileWaves];
-    __shared__ double bc[5];
-    const FiberArgs& a = k.fa;
-    const Tile T = tile_begin<D>(k.ta, blockIdx.x);
-    double vr[R], vt[R], vs[R], vw[R], vy[R], vph[R], vz[R], vx[R], vrt[R];
+    if (!split) { b = b0; e = b1; return; }
+    const uint32_t mid = b0 + (b1 - b0 + 1) / 2;
+    b = h ? mid : b0;
+    e = h ? b1 : mid;
+}
+
+// Coupled records of a row share: the record (Rec) at col*mul + add for the
+// first PRE entries of [e, e1); the rest is gathered when consumed.  One
+// record holds every vector the operand needs at that unknown, so a coupled
+// point costs one cache line rather than one per vector.
+template <int PRE, typename Rec>
+struct Gath {
+    uint32_t e, e1;
+    uint32_t col[PRE];
+    double val[PRE];
+    Rec g[PRE];
+};
+
+template <int PRE, typename Rec>
+__device__ __forceinline__ void gath_entries(Gath<PRE, Rec>& G, const uint32_t* __restrict__ col,
+                                             const double* __restrict__ val)
+{
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-        const int u = threadIdx.x + q * kTileThreads;
-        if (u < (int)T.nu) {
-            const uint32_t g = T.U0 + u;
-            vr[q] = k.r[g];
-            vt[q] = k.t[g];
-            vs[q] = k.sh[g];
-            vw[q] = k.w[g];
-            vy[q] = k.y[g];
-            vph[q] = k.ph_in[g];
-            vz[q] = k.z_in[g];
-            vx[q] = k.x[g];
-            vrt[q] = k.rt[g];
+    for (int v = 0; v < PRE; ++v) {
+        G.col[v] = 0;
+        G.val[v] = 0.0;
+        if (G.e + v < G.e1) {
+            G.col[v] = col[G.e + v];
+            G.val[v] = val[G.e + v];
         }
     }
-    SysScalars sc = k.st_in[T.sys];
-    const bool first = first_tile_of_system(T);
-    if (sc.done) {
-        if (first && threadIdx.x == 0) k.st_out[T.sys] = sc;
-        return;
+}
+
+template <int PRE, typename Rec>
+__device__ __forceinline__ void gath_values(Gath<PRE, Rec>& G, const Rec* __restrict__ rec, uint32_t mul,
+                                            uint32_t add)
+{
+#pragma unroll
+    for (int v = 0; v < PRE; ++v)
+        if (G.e + v < G.e1) G.g[v] = rec[G.col[v] * mul + add];
+}
+
+// sum of val * X(record) over the share, in entry order
+template <int PRE, typename Rec, typename Comb>
+__device__ __forceinline__ double gath_sum(const Gath<PRE, Rec>& G, const Rec* __restrict__ rec,
+                                           const uint32_t* __restrict__ col, const double* __restrict__ val,
+                                           uint32_t mul, uint32_t add, const Comb& X)
+{
+    double acc = 0.0;
+#pragma unroll
+    for (int v = 0; v < PRE; ++v)
+        if (G.e + v < G.e1) acc += G.val[v] * X(G.g[v]);
+    for (uint32_t e = G.e + PRE; e < G.e1; ++e) acc += val[e] * X(rec[col[e] * mul + add]);
+    return acc;
+}
+
+// phat from the record (shat, w, phat_old, z_old), or from (y, z) in the first
+// launch: the expression the owner of the entry evaluates for its own row
+struct PhatComb {
+    double beta, omega;
+    __device__ __forceinline__ double operator()(const double4& r) const
+    {
+        const double yj = r.x + (-omega) * r.y;
+        return beta == 0.0 ? yj : yj + beta * (r.z + (-omega) * r.w);
     }
-    tile_tables<D, PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
+};
+struct PhatFirstComb {
+    double beta;
+    __device__ __forceinline__ double operator()(const double2& r) const
+    {
+        return beta == 0.0 ? r.x : r.x + beta * 0.0;
+    }
+};
+// shat from (y, z)
+struct ShatComb {
+    double alpha;
+    __device__ __forceinline__ double operator()(const double2& r) const { return r.x + (-alpha) * r.y; }
+};
+
+// Row state of one thread for the operator and the block apply.
+template <int D, typename TB>
+struct TpRow {
+    int u;             // row in the tile
+    bool act, own, split;
+    int h;
+    // constants of the owner
+    double pd0, pd1, df0, df1;
+    uint32_t pt, f;
+    // block inverse share
+    uint32_t j0, j1;
+    TB blk[kTpBlk];
+    // segment constants (thread s < ts)
+    double sdf[D], sjd, sja, sjc;
+    int sk, sp;
+};
+
+template <int D>
+__device__ __forceinline__ void tp_roles(const Tile& T, int& u, bool& act, bool& own, bool& split, int& h)
+{
+    split = T.nu <= (uint32_t)kTpSplit;
+    const int tid = threadIdx.x;
+    u = split ? (tid & (kTpSplit - 1)) : tid;
+    h = split ? (tid >> 8) : 0;
+    act = u < (int)T.nu;
+    own = act && h == 0;
+}
+
+// P' and rfDiff of the segments after and before the owner's point, by
+// global index (masked later with the fiber's extent)
+template <int D, typename TB>
+__device__ __forceinline__ void tp_row_constants(const FiberArgs& a, const Tile& T, TpRow<D, TB>& R)
+{
+    R.pd0 = R.pd1 = R.df0 = R.df1 = 0.0;
+    R.pt = 0;
+    R.f = 0;
+    if (!R.own) return;
+    R.pt = (uint32_t)R.u / D;
+    const int d = R.u - D * (int)R.pt;
+    const uint32_t P = T.P0 + R.pt;
+    R.f = a.pfib[P];
+    const uint32_t g = P - R.f;     // segment starting at P, if P is not the fiber's last point
+    if (g < a.nseg) {
+        R.pd0 = a.pdiff[g];
+        R.df0 = a.diff[D * g + d];
+    }
+    if (g >= 1) {
+        R.pd1 = a.pdiff[g - 1];
+        R.df1 = a.diff[D * (g - 1) + d];
+    }
+}
+
+// after the tables: the block-inverse share and the segment constants
+template <int D, typename TB, typename LDS>
+__device__ __forceinline__ void tp_after_tables(const FiberArgs& a, const LDS& L, const Tile& T, const TB* B,
+                                                TpRow<D, TB>& R)
+{
+    R.j0 = R.j1 = 0;
+#pragma unroll
+    for (int v = 0; v < kTpBlk; ++v) R.blk[v] = TB(0);
+    if (R.act && B != nullptr) {
+        const int k = tile_fiber_of(L.fo, (int)T.nft, (uint32_t)R.u / D);
+        if (L.fuse[k]) {
+            const int fo = (int)L.fo[k];
+            const int bs = D * ((int)L.fo[k + 1] - fo);
+            share(0, (uint32_t)bs, R.split, R.h, R.j0, R.j1);
+            const TB* col = B + L.fblk[k] + (R.u - D * fo);
+#pragma unroll
+            for (int v = 0; v < kTpBlk; ++v)
+                if (R.j0 + v < R.j1) R.blk[v] = __builtin_nontemporal_load(col + (size_t)bs * (R.j0 + v));
+        }
+    }
+    const int s = threadIdx.x;
+#pragma unroll
+    for (int d = 0; d < D; ++d) R.sdf[d] = 0.0;
+    R.sjd = 1.0;
+    R.sja = R.sjc = 0.0;
+    R.sk = R.sp = 0;
+    if (s < (int)T.ts) {
+        const int ks = tile_fiber_of_seg(L.fo, (int)T.nft, (uint32_t)s);
+        const int p = s - ((int)L.fo[ks] - ks);
+        const int S = (int)L.fo[ks + 1] - (int)L.fo[ks] - 1;
+        const uint32_t g = L.fseg[ks] + p;
+#pragma unroll
+        for (int d = 0; d < D; ++d) R.sdf[d] = a.diff[D * g + d];
+        R.sjd = a.jd0[g];
+        if (p > 0) R.sja = a.je0[g - 1];
+        if (p + 1 < S) R.sjc = a.je0[g];
+        R.sk = ks;
+        R.sp = p;
+    }
+}
+
+// The operator over the tile once L.x holds the operand and the helpers'
+// coupling sums are in L.hb/L.hc: forces, multipliers, L.y.  bsum/csum: the
+// owner's own shares.  Ends synchronised.
+template <int D, typename TB, typename LDS>
+__device__ __forceinline__ void tp_operator(LDS& L, const Tile& T, const TpRow<D, TB>& R, double bsum, double csum)
+{
+    if (R.own) {
+        const int k = tile_fiber_of(L.fo, (int)T.nft, R.pt);
+        const int fo = (int)L.fo[k];
+        const int np = (int)L.fo[k + 1] - fo;
+        const int p = (int)R.pt - fo;
+        const int d = R.u - D * (int)R.pt;
+        const int S = np - 1;
+        double fp = bsum;
+        double cs = csum;
+        if (R.split) {
+            fp = fp + L.hb[R.u];
+            cs = cs + L.hc[R.u];
+        }
+        fp += cs;
+        const double* xf = L.x + D * fo;
+        if (np > 2) {
+            const double kr = L.fk[k];
+            double acc = 0.0;
+            if (p <= np - 3) {
+                const double v = kr * (xf[D * p + d] - 2 * xf[D * (p + 1) + d] + xf[D * (p + 2) + d]);
+                acc -= v;
+            }
+            if (p >= 1 && p <= np - 2) {
+                const double v = kr * (xf[D * (p - 1) + d] - 2 * xf[D * p + d] + xf[D * (p + 1) + d]);
+                acc += v + v;
+            }
+            if (p >= 2) {
+                const double v = kr * (xf[D * (p - 2) + d] - 2 * xf[D * (p - 1) + d] + xf[D * p + d]);
+                acc -= v;
+            }
+            fp += acc;
+        }
+        if (p < S && R.pd0 != 0.0) fp += R.pd0 * (xf[D * (p + 1) + d] - xf[D * p + d]);
+        if (p >= 1 && R.pd1 != 0.0) fp -= R.pd1 * (xf[D * p + d] - xf[D * (p - 1) + d]);
+        L.f[R.u] = fp;
+    }
+    __syncthreads();
+    const int ts = (int)T.ts;
+    const int s = threadIdx.x;
+    double t = 0.0;
+    if (s < ts) {
+        const int pt = (int)L.fo[R.sk] + R.sp;
+#pragma unroll
+        for (int d = 0; d < D; ++d) t += R.sdf[d] * (L.f[D * (pt + 1) + d] - L.f[D * pt + d]);
+    }
+    if (ts <= kWave) {
+        if (s < kWave) {
+            const double lam = pcr_solve_span(ts, L.maxS, R.sja, R.sjd, R.sjc, t);
+            if (s < ts) L.lam[s] = lam;
+        }
+    } else {
+        const double lam = pcr_lds(L.pa, L.pb, L.pc, L.lam, ts, L.maxS, R.sja, R.sjd, R.sjc, t);
+        __syncthreads();
+        if (s < ts) L.lam[s] = lam;
+    }
+    __syncthreads();
+    if (R.own) {
+        const int k = tile_fiber_of(L.fo, (int)T.nft, R.pt);
+        const int fo = (int)L.fo[k];
+        const int S = (int)L.fo[k + 1] - fo - 1;
+        const int p = (int)R.pt - fo;
+        const int sl = fo - k + p;
+        double v = L.f[R.u];
+        if (p < S) v = v + R.df0 * L.lam[sl];
+        if (p >= 1) v = v - R.df1 * L.lam[sl - 1];
+        const double yv = L.fsca[k] * v;
+        L.y[R.u] = yv + L.x[R.u];
+    }
+    __syncthreads();
+}
+
+// out[u] = Binv xin over the tile: each thread its column share, the helper's
+// partial through LDS; copies where there is no usable block.  Ends synchronised.
+template <int D, typename TB, typename LDS>
+__device__ __forceinline__ double tp_block_apply(LDS& L, const Tile& T, const TpRow<D, TB>& R, const TB* B,
+                                                 const double* xin)
+{
+    double acc[4] = { 0, 0, 0, 0 };
+    int k = 0, fo = 0, bs = 0;
+    bool use = false;
+    if (R.act) {
+        k = tile_fiber_of(L.fo, (int)T.nft, (uint32_t)R.u / D);
+        fo = (int)L.fo[k];
+        bs = D * ((int)L.fo[k + 1] - fo);
+        use = B != nullptr && L.fuse[k];
+    }
+    if (use) {
+        const double* xs = xin + D * fo;
+#pragma unroll
+        for (int v = 0; v < kTpBlk; ++v)
+            if (R.j0 + v < R.j1) acc[v & 3] += (double)R.blk[v] * xs[R.j0 + v];
+        const TB* col = B + L.fblk[k] + (R.u - D * fo);
+        for (uint32_t j = R.j0 + kTpBlk; j < R.j1; j += kApplyChunk) {
+            TB c[kApplyChunk];
+#pragma unroll
+            for (int v = 0; v < kApplyChunk; ++v)
+                c[v] = (j + v < R.j1) ? __builtin_nontemporal_load(col + (size_t)bs * (j + v)) : TB(0);
+#pragma unroll
+            for (int v = 0; v < kApplyChunk; ++v)
+                if (j + v < R.j1) acc[v & 3] += (double)c[v] * xs[j + v];
+        }
+    }
+    const double part = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    if (R.split && R.act && R.h == 1) L.hb[R.u] = part;
+    __syncthreads();
+    if (!R.own) return 0.0;
+    if (!use) return xin[R.u];
+    return R.split ? part + L.hb[R.u] : part;
+}
+
+template <int D, int PC, bool FIRST>
+__global__ __launch_bounds__(kTpThreads) void k_tp(KArgs k)
+{
+    using TB = typename PcBlocks<PC>::T;
+    using Rec = typename std::conditional<FIRST, double2, double4>::type;
+    __shared__ TpLds<D> L;
+    const FiberArgs& a = k.fa;
+    const Tile T = tile_begin<D>(k.ta, blockIdx.x);
+    TpRow<D, TB> R;
+    tp_roles<D>(T, R.u, R.act, R.own, R.split, R.h);
+    // prologue: everything independent of the iteration scalars
+    const uint32_t g = T.U0 + R.u;
+    double vr = 0, vt = 0, vx = 0, vrt = 0;
+    double4 q1 = make_double4(0, 0, 0, 0);
+    double vy = 0;
+    if (R.own) {
+        vr = k.r[g]; vt = k.t[g]; vx = k.x[g]; vrt = k.rt[g];
+        q1 = k.g1a[g];
+        if (FIRST) vy = k.g2[g].x;
+    }
+    const Rec* rec;
+    if constexpr (FIRST) rec = k.g2;
+    else rec = k.g1a;
+    Gath<kTpPreB, Rec> GB;
+    Gath<kTpPreC, Rec> GC;
+    GB.e = GB.e1 = GC.e = GC.e1 = 0;
+    const int dd = R.u - D * (R.u / D);
+    if (R.act && a.Buse) {
+        const uint32_t r = T.P0 + (uint32_t)R.u / D;
+        share(a.Brow[r], a.Brow[r + 1], R.split, R.h, GB.e, GB.e1);
+    }
+    if (R.act && a.Cuse) share(a.Crow[g], a.Crow[g + 1], R.split, R.h, GC.e, GC.e1);
+    gath_entries(GB, a.Bcol, a.Bval);
+    gath_entries(GC, a.Ccol, a.Cval);
+    gath_values(GB, rec, (uint32_t)D, (uint32_t)dd);
+    gath_values(GC, rec, 1u, 0u);
+    tp_row_constants<D, TB>(a, T, R);
+    SysScalars sc = state_load(k.st_in + T.sys);
+    tile_tables<PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
     const uint32_t nt = k.ta.ntiles;
-    const uint32_t tb = T.tb, te = T.te;
-    sc.pending = 0;
     if (threadIdx.x >= kWave && threadIdx.x < 2 * kWave) {
-        if (sc.stage == ST_INIT) {
-            const double rr = wsum(k.part + P_RR * nt, tb, te);
-            if (threadIdx.x == kWave) bc[0] = rr;
+        if (FIRST) {
+            const double rr = wsum(k.part + P_RR * nt, T.tb, T.te);
+            if (threadIdx.x == kWave) L.bc[0] = rr;
         } else {
             double S4[4];
-            wsum4(k.part4, tb, te, S4);
-            const double m = wmax(k.part + P_SMAX * nt, tb, te);
-            if (threadIdx.x == kWave) { bc[0] = S4[0]; bc[1] = S4[1]; bc[2] = S4[2]; bc[3] = S4[3]; bc[4] = m; }
+            wsum4(k.part4, T.tb, T.te, S4);
+            const double m = wmax(k.part + P_SMAX * nt, T.tb, T.te);
+            if (threadIdx.x == kWave) { L.bc[0] = S4[0]; L.bc[1] = S4[1]; L.bc[2] = S4[2]; L.bc[3] = S4[3]; L.bc[4] = m; }
         }
     }
     __syncthreads();
+    const bool first = T.t == T.tb;
+    if (sc.done) {
+        if (first && threadIdx.x == 0) state_store(k.st_out + T.sys, sc);
+        return;
+    }
+    sc.pending = 0;
     if (sc.stage == ST_INIT) {
         sc.rho2 = sc.rho1;
-        sc.rho1 = bc[0];
+        sc.rho1 = L.bc[0];
         sc.beta = (sc.rho1 / sc.rho2) * (sc.alpha / sc.omega);
     } else {
-        sc.resid_s = (bc[4] != bc[4]) ? __builtin_inf() : bc[4];
-        const double tt = bc[0], ts = bc[1], rts = bc[2], rtt = bc[3];
+        sc.resid_s = (L.bc[4] != L.bc[4]) ? __builtin_inf() : L.bc[4];
+        const double tt = L.bc[0], ts = L.bc[1], rts = L.bc[2], rtt = L.bc[3];
         if (tt == 0.0) {
             sc.done = 1; sc.flag = 0; sc.resid = sc.resid_s;
         } else {
@@ -674,59 +1034,61 @@ __global__ __launch_bounds__(kTileThreads) void k_tp(KArgs k)
         }
     }
     sc.stage = ST_A;
-    if (first && threadIdx.x == 0) k.st_out[T.sys] = sc;
+    if (first && threadIdx.x == 0) state_store(k.st_out + T.sys, sc);
     if (sc.done) return;
-    using TB = typename PcBlocks<PC>::T;
-    const TB* Bk = PcBlocks<PC>::get(k);
-    TileRows<D> rw;
-    tile_rows<D>(L, T, rw);
-    TilePre<D, TB> pre;
-    tile_prefetch<D, TB>(a, L, T, rw, Bk, pre);
-    const PhatOp op{ k.y, k.sh, k.w, k.ph_in, k.z_in, sc.beta, sc.omega, sc.pending };
-    double rmax = 0.0;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-        if (rw.k[q] < 0) continue;
-        const int u = threadIdx.x + q * kTileThreads;
-        const uint32_t g = T.U0 + u;
-        double ri = vr[q];
-        double yi = vy[q];
+    tp_after_tables<D, TB>(a, L, T, PcBlocks<PC>::get(k), R);
+    // vector updates and phat; q1 = (shat, w, phat_old, z_old) of the row
+    double rmax = 0.0, yi = vy, phi = 0.0;
+    if (R.own) {
+        double ri = vr;
         if (sc.pending) {
-            k.x[g] = vx[q] + sc.omega * vs[q];
-            ri = ri + (-sc.omega) * vt[q];
+            k.x[g] = vx + sc.omega * q1.x;
+            ri = ri + (-sc.omega) * vt;
             k.r[g] = ri;
-            yi = vs[q] + (-sc.omega) * vw[q];
-            k.y[g] = yi;
+            yi = q1.x + (-sc.omega) * q1.y;
         }
-        rmax = nan_max(rmax, fabs(ri));
-        const double phi = sc.beta == 0.0 ? yi : yi + sc.beta * (vph[q] + (-sc.omega) * vz[q]);
-        k.ph_out[g] = phi;
-        L.x[u] = phi;
+        rmax = fabs(ri);
+        phi = sc.beta == 0.0 ? yi : yi + sc.beta * (q1.z + (-sc.omega) * q1.w);
+        L.x[R.u] = phi;
+    }
+    double bsum = 0.0, csum = 0.0;
+    if constexpr (FIRST) {
+        const PhatFirstComb X{ sc.beta };
+        bsum = gath_sum(GB, rec, a.Bcol, a.Bval, (uint32_t)D, (uint32_t)dd, X);
+        csum = gath_sum(GC, rec, a.Ccol, a.Cval, 1u, 0u, X);
+    } else {
+        const PhatComb X{ sc.beta, sc.omega };
+        bsum = gath_sum(GB, rec, a.Bcol, a.Bval, (uint32_t)D, (uint32_t)dd, X);
+        csum = gath_sum(GC, rec, a.Ccol, a.Cval, 1u, 0u, X);
+    }
+    if (R.split && R.act && R.h == 1) {
+        L.hb[R.u] = bsum;
+        L.hc[R.u] = csum;
     }
     __syncthreads();
-    tile_operator<D>(a, L, op, T, rw, pre);
+    tp_operator<D, TB>(L, T, R, bsum, csum);
     double dl = 0.0;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-        if (rw.k[q] < 0) continue;
-        const int u = threadIdx.x + q * kTileThreads;
-        const uint32_t g = T.U0 + u;
-        k.v[g] = L.y[u];
-        dl += vrt[q] * L.y[u];
+    if (R.own) {
+        k.v[g] = L.y[R.u];
+        dl = vrt * L.y[R.u];
     }
     rmax = wave_max(rmax);
     dl = wave_sum(dl);
     if ((threadIdx.x & (kWave - 1)) == 0) {
-        wr[threadIdx.x / kWave] = rmax;
-        wr[kTileWaves + threadIdx.x / kWave] = dl;
+        L.wr[threadIdx.x / kWave] = rmax;
+        L.wr[kTpWaves + threadIdx.x / kWave] = dl;
     }
-    tile_block_apply<TB, D>(Bk, L, rw, pre, L.y, k.z_out + T.U0);
-    __syncthreads();
+    const double zi = tp_block_apply<D, TB>(L, T, R, PcBlocks<PC>::get(k), L.y);
+    if (R.own) {
+        k.g2[g] = make_double2(yi, zi);
+        reinterpret_cast<double2*>(k.g1b + g)[1] = make_double2(phi, zi);
+    }
     if (threadIdx.x == 0) {
-        double m = wr[0], d = wr[kTileWaves];
-        for (int q = 1; q < kTileWaves; ++q) {
-            m = nan_max(m, wr[q]);
-            d += wr[kTileWaves + q];
+        double m = L.wr[0], d = L.wr[kTpWaves];
+#pragma unroll
+        for (int q = 1; q < kTpWaves; ++q) {
+            m = nan_max(m, L.wr[q]);
+            d += L.wr[kTpWaves + q];
         }
         k.part[P_RMAX * nt + T.t] = m;
         k.part[P_DELTA * nt + T.t] = d;
@@ -734,88 +1096,85 @@ __global__ __launch_bounds__(kTileThreads) void k_tp(KArgs k)
 }
 
 template <int D, int PC>
-__global__ __launch_bounds__(kTileThreads) void k_tq(KArgs k)
+__global__ __launch_bounds__(kTpThreads) void k_tq(KArgs k)
 {
-    constexpr int R = TileRows<D>::R;
-    __shared__ TileLds<D> L;
-    __shared__ double wr[5 * kTileWaves];
-    __shared__ double bc[2];
+    using TB = typename PcBlocks<PC>::T;
+    __shared__ TpLds<D> L;
     const FiberArgs& a = k.fa;
     const Tile T = tile_begin<D>(k.ta, blockIdx.x);
-    double vr[R], vv[R], vx[R], vph[R], vy[R], vz[R], vrt[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-        const int u = threadIdx.x + q * kTileThreads;
-        if (u < (int)T.nu) {
-            const uint32_t g = T.U0 + u;
-            vr[q] = k.r[g];
-            vv[q] = k.v[g];
-            vx[q] = k.x[g];
-            vph[q] = k.ph_in[g];
-            vy[q] = k.y[g];
-            vz[q] = k.z_in[g];
-            vrt[q] = k.rt[g];
-        }
+    TpRow<D, TB> R;
+    tp_roles<D>(T, R.u, R.act, R.own, R.split, R.h);
+    const uint32_t g = T.U0 + R.u;
+    double vr = 0, vv = 0, vx = 0, vph = 0, vrt = 0;
+    double2 q2 = make_double2(0, 0);
+    if (R.own) {
+        vr = k.r[g]; vv = k.v[g]; vx = k.x[g]; vrt = k.rt[g];
+        q2 = k.g2[g];
+        vph = k.g1b[g].z;
     }
-    SysScalars sc = k.st_in[T.sys];
-    const bool first = first_tile_of_system(T);
+    const double2* rec = k.g2;
+    Gath<kTpPreB, double2> GB;
+    Gath<kTpPreC, double2> GC;
+    GB.e = GB.e1 = GC.e = GC.e1 = 0;
+    const int dd = R.u - D * (R.u / D);
+    if (R.act && a.Buse) {
+        const uint32_t r = T.P0 + (uint32_t)R.u / D;
+        share(a.Brow[r], a.Brow[r + 1], R.split, R.h, GB.e, GB.e1);
+    }
+    if (R.act && a.Cuse) share(a.Crow[g], a.Crow[g + 1], R.split, R.h, GC.e, GC.e1);
+    gath_entries(GB, a.Bcol, a.Bval);
+    gath_entries(GC, a.Ccol, a.Cval);
+    gath_values(GB, rec, (uint32_t)D, (uint32_t)dd);
+    gath_values(GC, rec, 1u, 0u);
+    tp_row_constants<D, TB>(a, T, R);
+    SysScalars sc = state_load(k.st_in + T.sys);
+    tile_tables<PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
     const uint32_t nt = k.ta.ntiles;
+    if (threadIdx.x >= kWave && threadIdx.x < 2 * kWave) {
+        const double res = wmax(k.part + P_RMAX * nt, T.tb, T.te);
+        const double dsum = wsum(k.part + P_DELTA * nt, T.tb, T.te);
+        if (threadIdx.x == kWave) { L.bc[0] = res; L.bc[1] = dsum; }
+    }
+    __syncthreads();
+    const bool first = T.t == T.tb;
     if (!sc.done) {
-        tile_tables<D, PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
-        const uint32_t tb = T.tb, te = T.te;
-        if (threadIdx.x >= kWave && threadIdx.x < 2 * kWave) {
-            const double res = wmax(k.part + P_RMAX * nt, tb, te);
-            const double dsum = wsum(k.part + P_DELTA * nt, tb, te);
-            if (threadIdx.x == kWave) { bc[0] = res; bc[1] = dsum; }
-        }
-        __syncthreads();
-        decide(sc, bc[0]);
+        decide(sc, L.bc[0]);
         if (!sc.done) {
-            const double delta = bc[1];
+            const double delta = L.bc[1];
             if (PC != CYMG_PRECOND_NONE && delta == 0.0) { sc.done = 1; sc.flag = 4; }
             else sc.alpha = sc.rho1 / delta;
         }
         sc.stage = ST_C;
     }
-    if (first && threadIdx.x == 0) k.st_out[T.sys] = sc;
+    if (first && threadIdx.x == 0) state_store(k.st_out + T.sys, sc);
     if (sc.done) return;
-    using TB = typename PcBlocks<PC>::T;
-    const TB* Bk = PcBlocks<PC>::get(k);
-    TileRows<D> rw;
-    tile_rows<D>(L, T, rw);
-    TilePre<D, TB> pre;
-    tile_prefetch<D, TB>(a, L, T, rw, Bk, pre);
-    const ShatOp op{ k.y, k.z_in, sc.alpha };
-    double smax = 0.0;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-        if (rw.k[q] < 0) continue;
-        const int u = threadIdx.x + q * kTileThreads;
-        const uint32_t g = T.U0 + u;
-        const double si = vr[q] + (-sc.alpha) * vv[q];
+    tp_after_tables<D, TB>(a, L, T, PcBlocks<PC>::get(k), R);
+    double smax = 0.0, si = 0.0, shi = 0.0;
+    if (R.own) {
+        si = vr + (-sc.alpha) * vv;
         k.r[g] = si;
-        k.x[g] = vx[q] + sc.alpha * vph[q];
-        smax = nan_max(smax, fabs(si));
-        const double shi = vy[q] + (-sc.alpha) * vz[q];
-        k.sh[g] = shi;
-        L.x[u] = shi;
+        k.x[g] = vx + sc.alpha * vph;
+        smax = fabs(si);
+        shi = q2.x + (-sc.alpha) * q2.y;
+        L.x[R.u] = shi;
+    }
+    const ShatComb X{ sc.alpha };
+    const double bsum = gath_sum(GB, rec, a.Bcol, a.Bval, (uint32_t)D, (uint32_t)dd, X);
+    const double csum = gath_sum(GC, rec, a.Ccol, a.Cval, 1u, 0u, X);
+    if (R.split && R.act && R.h == 1) {
+        L.hb[R.u] = bsum;
+        L.hc[R.u] = csum;
     }
     __syncthreads();
-    tile_operator<D>(a, L, op, T, rw, pre);
+    tp_operator<D, TB>(L, T, R, bsum, csum);
     double tt = 0, ts = 0, rts = 0, rtt = 0;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-        if (rw.k[q] < 0) continue;
-        const int u = threadIdx.x + q * kTileThreads;
-        const uint32_t g = T.U0 + u;
-        const double ti = L.y[u];
-        const double si = vr[q] + (-sc.alpha) * vv[q];
-        const double rti = vrt[q];
+    if (R.own) {
+        const double ti = L.y[R.u];
         k.t[g] = ti;
-        tt += ti * ti;
-        ts += ti * si;
-        rts += rti * si;
-        rtt += rti * ti;
+        tt = ti * ti;
+        ts = ti * si;
+        rts = vrt * si;
+        rtt = vrt * ti;
     }
     smax = wave_max(smax);
     tt = wave_sum(tt);
@@ -824,24 +1183,24 @@ __global__ __launch_bounds__(kTileThreads) void k_tq(KArgs k)
     rtt = wave_sum(rtt);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         const int wv = threadIdx.x / kWave;
-        wr[wv] = smax;
-        wr[kTileWaves + wv] = tt;
-        wr[2 * kTileWaves + wv] = ts;
-        wr[3 * kTileWaves + wv] = rts;
-        wr[4 * kTileWaves + wv] = rtt;
+        L.wr[wv] = smax;
+        L.wr[kTpWaves + wv] = tt;
+        L.wr[2 * kTpWaves + wv] = ts;
+        L.wr[3 * kTpWaves + wv] = rts;
+        L.wr[4 * kTpWaves + wv] = rtt;
     }
-    tile_block_apply<TB, D>(Bk, L, rw, pre, L.y, k.w + T.U0);
-    __syncthreads();
+    const double wi = tp_block_apply<D, TB>(L, T, R, PcBlocks<PC>::get(k), L.y);
+    if (R.own) reinterpret_cast<double2*>(k.g1b + g)[0] = make_double2(shi, wi);
     if (threadIdx.x == 0) {
-        double m = wr[0], s0 = wr[kTileWaves], s1 = wr[2 * kTileWaves], s2 = wr[3 * kTileWaves],
-               s3 = wr[4 * kTileWaves];
+        double m = L.wr[0], s0 = L.wr[kTpWaves], s1 = L.wr[2 * kTpWaves], s2 = L.wr[3 * kTpWaves],
+               s3 = L.wr[4 * kTpWaves];
 #pragma unroll
-        for (int q = 1; q < kTileWaves; ++q) {
-            m = nan_max(m, wr[q]);
-            s0 += wr[kTileWaves + q];
-            s1 += wr[2 * kTileWaves + q];
-            s2 += wr[3 * kTileWaves + q];
-            s3 += wr[4 * kTileWaves + q];
+        for (int q = 1; q < kTpWaves; ++q) {
+            m = nan_max(m, L.wr[q]);
+            s0 += L.wr[kTpWaves + q];
+            s1 += L.wr[2 * kTpWaves + q];
+            s2 += L.wr[3 * kTpWaves + q];
+            s3 += L.wr[4 * kTpWaves + q];
         }
         k.part[P_SMAX * nt + T.t] = m;
         *reinterpret_cast<double4*>(k.part4 + 4 * (size_t)T.t) = make_double4(s0, s1, s2, s3);
@@ -865,9 +1224,8 @@ static KArgs make_kargs(Context& c)
     k.t = c.k_t.get();
     k.ph = c.k_ph.get();
     k.sh = c.k_sh.get();
-    k.y = c.k_y.get();
-    k.w = c.k_w.get();
-    k.z_in = k.z_out = k.ph_in = k.ph_out = nullptr;
+    k.g1a = k.g1b = nullptr;
+    k.g2 = nullptr;
     k.part = c.d_part.get();
     k.part4 = c.d_part.get() + (size_t)P_TT * c.nfib;
     k.fpart = c.d_part.get();
@@ -957,24 +1315,23 @@ static void run_tiled(Context& c, const double* guess, int max_iter, double tol,
     k.part4 = c.d_tpart.get() + (size_t)P_TT * c.ntiles;
     SysScalars* S0 = c.d_scal.get();
     SysScalars* S1 = S0 + c.nsys;
-    double* ph[2] = { c.k_ph.get(), c.k_ph2.get() };
-    double* zz[2] = { c.k_z.get(), c.k_z2.get() };
+    double4* g1[2] = { c.k_g1[0].get(), c.k_g1[1].get() };
     auto with = [&](SysScalars* in, SysScalars* out, int par) {
         KArgs q = k;
         q.st_in = in;
         q.st_out = out;
-        q.ph_in = ph[par];
-        q.ph_out = ph[par ^ 1];
-        q.z_in = zz[par];
-        q.z_out = zz[par ^ 1];
+        q.g1a = g1[par];
+        q.g1b = g1[par ^ 1];
+        q.g2 = c.k_g2.get();
         return q;
     };
     // P reads phat/z of its parity and writes the other; Q reads what P wrote
     const KArgs kI = with(S0, S1, 0);
-    const KArgs kP0 = with(S1, S0, 0), kQ0 = with(S0, S1, 1);
-    const KArgs kP1 = with(S1, S0, 1), kQ1 = with(S0, S1, 0);
-    const dim3 g(c.ntiles), b(kTileThreads);
+    const KArgs kP0 = with(S1, S0, 0), kQ0 = with(S0, S1, 0);
+    const KArgs kP1 = with(S1, S0, 1), kQ1 = with(S0, S1, 1);
+    const dim3 g(c.ntiles), b(kTileThreads), bt(kTpThreads);
     hipLaunchKernelGGL((k_tinit<D, PC>), g, b, 0, c.stream, kI, guess, tol, tol_mode, max_iter, c.prm.time_step);
+    bool first_p = true;
     CYMG_HIP(hipGetLastError());
     int launched = 0;
     int& predicted = c.last_iters[PC];
@@ -991,10 +1348,12 @@ static void run_tiled(Context& c, const double* guess, int max_iter, double tol,
         }
         first = false;
         for (int it = 0; it < chunk; it += 2) {
-            hipLaunchKernelGGL((k_tp<D, PC>), g, b, 0, c.stream, kP0);
-            hipLaunchKernelGGL((k_tq<D, PC>), g, b, 0, c.stream, kQ0);
-            hipLaunchKernelGGL((k_tp<D, PC>), g, b, 0, c.stream, kP1);
-            hipLaunchKernelGGL((k_tq<D, PC>), g, b, 0, c.stream, kQ1);
+            if (first_p) hipLaunchKernelGGL((k_tp<D, PC, true>), g, bt, 0, c.stream, kP0);
+            else hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, kP0);
+            first_p = false;
+            hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ0);
+            hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, kP1);
+            hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ1);
         }
         CYMG_HIP(hipGetLastError());
         launched += chunk;
@@ -1034,31 +1393,30 @@ static double time_tiled(Context& c, int pairs)
     k.part4 = c.d_tpart.get() + (size_t)P_TT * c.ntiles;
     SysScalars* S0 = c.d_scal.get();
     SysScalars* S1 = S0 + c.nsys;
-    double* ph[2] = { c.k_ph.get(), c.k_ph2.get() };
-    double* zz[2] = { c.k_z.get(), c.k_z2.get() };
+    double4* g1[2] = { c.k_g1[0].get(), c.k_g1[1].get() };
     auto with = [&](SysScalars* in, SysScalars* out, int par) {
         KArgs q = k;
         q.st_in = in;
         q.st_out = out;
-        q.ph_in = ph[par];
-        q.ph_out = ph[par ^ 1];
-        q.z_in = zz[par];
-        q.z_out = zz[par ^ 1];
+        q.g1a = g1[par];
+        q.g1b = g1[par ^ 1];
+        q.g2 = c.k_g2.get();
         return q;
     };
     const KArgs kI = with(S0, S1, 0);
-    const KArgs kP0 = with(S1, S0, 0), kQ0 = with(S0, S1, 1);
-    const KArgs kP1 = with(S1, S0, 1), kQ1 = with(S0, S1, 0);
-    const dim3 g(c.ntiles), b(kTileThreads);
+    const KArgs kP0 = with(S1, S0, 0), kQ0 = with(S0, S1, 0);
+    const KArgs kP1 = with(S1, S0, 1), kQ1 = with(S0, S1, 1);
+    const dim3 g(c.ntiles), b(kTileThreads), bt(kTpThreads);
     hipLaunchKernelGGL((k_tinit<D, PC>), g, b, 0, c.stream, kI, nullptr, 0.0, CYMG_TOL_RELATIVE, 1 << 30,
                        c.prm.time_step);
     pairs += pairs & 1;
     CYMG_HIP(hipEventRecord(c.ev[8], c.stream));
     for (int it = 0; it < pairs; it += 2) {
-        hipLaunchKernelGGL((k_tp<D, PC>), g, b, 0, c.stream, kP0);
-        hipLaunchKernelGGL((k_tq<D, PC>), g, b, 0, c.stream, kQ0);
-        hipLaunchKernelGGL((k_tp<D, PC>), g, b, 0, c.stream, kP1);
-        hipLaunchKernelGGL((k_tq<D, PC>), g, b, 0, c.stream, kQ1);
+        if (it == 0) hipLaunchKernelGGL((k_tp<D, PC, true>), g, bt, 0, c.stream, kP0);
+        else hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, kP0);
+        hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ0);
+        hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, kP1);
+        hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ1);
     }
     CYMG_HIP(hipGetLastError());
     CYMG_HIP(hipEventRecord(c.ev[9], c.stream));
@@ -1075,13 +1433,18 @@ static double time_tiled(Context& c, int pairs)
 static void tiled_buffers(Context& c)
 {
     const size_t n = c.n();
-    const bool fresh = c.k_y.cap < n;
+    const bool fresh = c.k_g2.cap < n;
     c.d_tpart.resize((size_t)P_SLOTS * (c.ntiles ? c.ntiles : 1));
-    c.k_r.resize(n); c.k_rt.resize(n); c.k_v.resize(n); c.k_t.resize(n); c.k_sh.resize(n);
-    c.k_ph.resize(n); c.k_ph2.resize(n); c.k_y.resize(n); c.k_z.resize(n); c.k_z2.resize(n); c.k_w.resize(n);
-    if (fresh)
-        for (DevBuf<double>* v : { &c.k_ph, &c.k_ph2, &c.k_z, &c.k_z2, &c.k_w, &c.k_sh, &c.k_t })
-            CYMG_HIP(hipMemsetAsync(v->get(), 0, v->cap * sizeof(double), c.stream));
+    c.k_r.resize(n); c.k_rt.resize(n); c.k_v.resize(n); c.k_t.resize(n);
+    c.k_g1[0].resize(n); c.k_g1[1].resize(n); c.k_g2.resize(n);
+    if (fresh) {
+        // the first launch reads phat_old, z_old and t only where beta = 0 or omega
+        // multiplies them by zero: keep them finite
+        CYMG_HIP(hipMemsetAsync(c.k_g1[0].get(), 0, c.k_g1[0].cap * sizeof(double4), c.stream));
+        CYMG_HIP(hipMemsetAsync(c.k_g1[1].get(), 0, c.k_g1[1].cap * sizeof(double4), c.stream));
+        CYMG_HIP(hipMemsetAsync(c.k_g2.get(), 0, c.k_g2.cap * sizeof(double2), c.stream));
+        CYMG_HIP(hipMemsetAsync(c.k_t.get(), 0, c.k_t.cap * sizeof(double), c.stream));
+    }
 }
 
 double krylov_time_launch(Context& c, int precond, int launches)

@@ -64,8 +64,8 @@ __device__ __forceinline__ Tile tile_begin(const TileArgs& ta, uint32_t t)
 }
 
 // Per-fiber tables of the tile (wave 0; nft <= 64).  The caller synchronises.
-template <int D, bool BLK>
-__device__ __forceinline__ void tile_tables(const FiberArgs& a, TileLds<D>& L, const Tile& T,
+template <bool BLK, typename LDS>
+__device__ __forceinline__ void tile_tables(const FiberArgs& a, LDS& L, const Tile& T,
                                             const uint64_t* blkoff, const int* useblk)
 {
     const int k = threadIdx.x;
@@ -432,7 +432,7 @@ __device__ __forceinline__ void tile_operator(const FiberArgs& a, TileLds<D>& L,
 template <typename TB, int D>
 __device__ __forceinline__ void tile_block_apply(const TB* __restrict__ B, const TileLds<D>& L,
                                                  const TileRows<D>& rw, const TilePre<D, TB>& pre,
-                                                 const double* xin, double* __restrict__ out)
+                                                 const double* xin, double* __restrict__ out, int stride = 1)
 {
     constexpr int R = TileRows<D>::R;
 #pragma unroll
@@ -445,7 +445,7 @@ __device__ __forceinline__ void tile_block_apply(const TB* __restrict__ B, const
         const int i = u - D * fo;
         const double* xs = xin + D * fo;
         if (B == nullptr || !L.fuse[k]) {
-            out[u] = xs[i];
+            out[(size_t)u * stride] = xs[i];
             continue;
         }
         const TB* col = B + L.fblk[k] + i;
@@ -465,7 +465,7 @@ __device__ __forceinline__ void tile_block_apply(const TB* __restrict__ B, const
             for (int v = 0; v < kApplyChunk; ++v)
                 if (j + v < bs) acc[v & 3] += (double)c[v] * xs[j + v];
         }
-        out[u] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        out[(size_t)u * stride] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     }
 }
 

@@ -207,7 +207,8 @@ int         cymg_enable_timing(cymg_context* ctx, int enable);
    (Meca::precondition), 2 block build + factorisation (FP64), 3 the same in FP32,
    4 block build without factorisation, 5 build with getrf + getri instead of Gauss-Jordan,
    6 one launch of the two-launch BiCGStab iteration with the current preconditioner
-     (`reps` launches after an init launch, zero threshold; no counterpart in the reference) */
+     (`reps` launches after an init launch, zero threshold; no counterpart in the reference),
+   7 the same without preconditioner */
 int         cymg_time_kernel(cymg_context* ctx, int which, int reps, double* avg_ms);
 
 #ifdef __cplusplus
