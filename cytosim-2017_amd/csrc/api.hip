@@ -134,7 +134,7 @@ void build_tiles(Context& c)
         d.sys = c.h_tile_sys[t];
         d.tb = c.h_sys_tile[d.sys];
         d.te = c.h_sys_tile[d.sys + 1];
-        d.pad = 0;
+        d.S0 = c.h_seg[fb];
     }
     c.d_tile_desc.resize(c.ntiles ? c.ntiles : 1);
     c.d_tpart.resize((size_t)P_SLOTS * (c.ntiles ? c.ntiles : 1));
@@ -323,7 +323,9 @@ int cymg_create(int dim, int device, cymg_context** out)
     int rc = guarded(ctx, [&](Context& c) {
         CYMG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         CYMG_HIP(hipHostMalloc((void**)&c.h_scal, sizeof(SysScalars) * 4096, hipHostMallocDefault));
+        CYMG_HIP(hipHostMalloc((void**)&c.h_snap, sizeof(SysScalars) * 2 * 4096, hipHostMallocDefault));
         for (int i = 0; i < 10; ++i) CYMG_HIP(hipEventCreate(&c.ev[i]));
+        for (int i = 0; i < 2; ++i) CYMG_HIP(hipEventCreateWithFlags(&c.chunk_ev[i], hipEventDisableTiming));
     });
     if (rc) {
         delete ctx;
@@ -342,6 +344,9 @@ int cymg_destroy(cymg_context* ctx)
     for (int i = 0; i < 10; ++i)
         if (c.ev[i]) (void)hipEventDestroy(c.ev[i]);
     if (c.h_scal) (void)hipHostFree(c.h_scal);
+    if (c.h_snap) (void)hipHostFree(c.h_snap);
+    for (int i = 0; i < 2; ++i)
+        if (c.chunk_ev[i]) (void)hipEventDestroy(c.chunk_ev[i]);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     delete ctx;
     return CYMG_OK;

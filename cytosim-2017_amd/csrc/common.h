@@ -79,7 +79,7 @@ struct TileDesc {
     uint32_t tp;        // points
     uint32_t sys;       // system
     uint32_t tb, te;    // the tiles of that system
-    uint32_t pad;
+    uint32_t S0;        // first segment
 };
 
 enum { P_RR = 0, P_RMAX, P_DELTA, P_SMAX, P_TT, P_TS, P_RTS, P_RTT, P_NOISE, P_BNORM, P_SLOTS };

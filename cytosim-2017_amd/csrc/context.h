@@ -78,6 +78,8 @@ struct Context {
     DevBuf<double> d_tpart;            // partial reductions [slot][tile]
     DevBuf<SysScalars> d_scal;
     SysScalars* h_scal = nullptr;      // pinned mirror
+    SysScalars* h_snap = nullptr;      // pinned snapshots of queued iteration chunks (2 x 4096)
+    hipEvent_t chunk_ev[2]{};
     int last_iters[3] = {0, 0, 0};     // iterations of the previous solve, per preconditioner
 
     bool prepared = false, setup_done = false, pc_ready = false, have_solution = false;

@@ -18,6 +18,7 @@
 // scalar state into the other half of a double buffer.  The host reads the
 // stop flag between chunks of iterations.
 #include <algorithm>
+#include <cstring>
 #include <type_traits>
 
 #include "args.h"
@@ -638,7 +639,7 @@ constexpr int kTpWaves = kTpThreads / kWave;
 constexpr int kTpSplit = 256;
 constexpr int kTpPreB = 6;      // mB entries per thread in registers
 constexpr int kTpPreC = 3;      // mC entries per thread in registers
-constexpr int kTpBlk = 24;      // block-inverse columns per thread in registers
+constexpr int kTpBlk = 48;      // block-inverse columns per thread in registers
 
 template <int D>
 struct TpLds {
@@ -749,16 +750,38 @@ struct TpRow {
     int u;             // row in the tile
     bool act, own, split;
     int h;
+    // the row's fiber in the tile: index, first point, points; point in the fiber
+    int k, fo, np, p;
     // constants of the owner
     double pd0, pd1, df0, df1;
     uint32_t pt, f;
     // block inverse share
     uint32_t j0, j1;
     TB blk[kTpBlk];
-    // segment constants (thread s < ts)
+    // segment constants (thread s < ts), masked at the fiber ends after the tables
     double sdf[D], sjd, sja, sjc;
     int sk, sp;
 };
+
+// the thread's segment (s = threadIdx.x < ts): rfDiff and the JJ' entries of
+// global segment S0 + s and its predecessor, loaded before the tables are known
+template <int D, typename TB>
+__device__ __forceinline__ void tp_seg_constants(const FiberArgs& a, const Tile& T, TpRow<D, TB>& R)
+{
+#pragma unroll
+    for (int d = 0; d < D; ++d) R.sdf[d] = 0.0;
+    R.sjd = 1.0;
+    R.sja = R.sjc = 0.0;
+    const int s = threadIdx.x;
+    if (s < (int)T.ts) {
+        const uint32_t g = T.S0 + s;
+#pragma unroll
+        for (int d = 0; d < D; ++d) R.sdf[d] = a.diff[D * g + d];
+        R.sjd = a.jd0[g];
+        R.sjc = a.je0[g];
+        if (g >= 1) R.sja = a.je0[g - 1];
+    }
+}
 
 template <int D>
 __device__ __forceinline__ void tp_roles(const Tile& T, int& u, bool& act, bool& own, bool& split, int& h)
@@ -795,42 +818,39 @@ __device__ __forceinline__ void tp_row_constants(const FiberArgs& a, const Tile&
     }
 }
 
-// after the tables: the block-inverse share and the segment constants
+// after the tables: the row's fiber, the block-inverse share, and the
+// segment's fiber (masking the JJ' couplings at the fiber ends)
 template <int D, typename TB, typename LDS>
 __device__ __forceinline__ void tp_after_tables(const FiberArgs& a, const LDS& L, const Tile& T, const TB* B,
                                                 TpRow<D, TB>& R)
 {
     R.j0 = R.j1 = 0;
+    R.k = R.fo = R.np = R.p = 0;
 #pragma unroll
     for (int v = 0; v < kTpBlk; ++v) R.blk[v] = TB(0);
-    if (R.act && B != nullptr) {
-        const int k = tile_fiber_of(L.fo, (int)T.nft, (uint32_t)R.u / D);
-        if (L.fuse[k]) {
-            const int fo = (int)L.fo[k];
-            const int bs = D * ((int)L.fo[k + 1] - fo);
+    if (R.act) {
+        const uint32_t pt = (uint32_t)R.u / D;
+        R.k = tile_fiber_of(L.fo, (int)T.nft, pt);
+        R.fo = (int)L.fo[R.k];
+        R.np = (int)L.fo[R.k + 1] - R.fo;
+        R.p = (int)pt - R.fo;
+        if (B != nullptr && L.fuse[R.k]) {
+            const int bs = D * R.np;
             share(0, (uint32_t)bs, R.split, R.h, R.j0, R.j1);
-            const TB* col = B + L.fblk[k] + (R.u - D * fo);
+            const TB* col = B + L.fblk[R.k] + (R.u - D * R.fo);
 #pragma unroll
             for (int v = 0; v < kTpBlk; ++v)
                 if (R.j0 + v < R.j1) R.blk[v] = __builtin_nontemporal_load(col + (size_t)bs * (R.j0 + v));
         }
     }
-    const int s = threadIdx.x;
-#pragma unroll
-    for (int d = 0; d < D; ++d) R.sdf[d] = 0.0;
-    R.sjd = 1.0;
-    R.sja = R.sjc = 0.0;
     R.sk = R.sp = 0;
+    const int s = threadIdx.x;
     if (s < (int)T.ts) {
         const int ks = tile_fiber_of_seg(L.fo, (int)T.nft, (uint32_t)s);
         const int p = s - ((int)L.fo[ks] - ks);
         const int S = (int)L.fo[ks + 1] - (int)L.fo[ks] - 1;
-        const uint32_t g = L.fseg[ks] + p;
-#pragma unroll
-        for (int d = 0; d < D; ++d) R.sdf[d] = a.diff[D * g + d];
-        R.sjd = a.jd0[g];
-        if (p > 0) R.sja = a.je0[g - 1];
-        if (p + 1 < S) R.sjc = a.je0[g];
+        if (p == 0) R.sja = 0.0;
+        if (p + 1 >= S) R.sjc = 0.0;
         R.sk = ks;
         R.sp = p;
     }
@@ -843,10 +863,10 @@ template <int D, typename TB, typename LDS>
 __device__ __forceinline__ void tp_operator(LDS& L, const Tile& T, const TpRow<D, TB>& R, double bsum, double csum)
 {
     if (R.own) {
-        const int k = tile_fiber_of(L.fo, (int)T.nft, R.pt);
-        const int fo = (int)L.fo[k];
-        const int np = (int)L.fo[k + 1] - fo;
-        const int p = (int)R.pt - fo;
+        const int k = R.k;
+        const int fo = R.fo;
+        const int np = R.np;
+        const int p = R.p;
         const int d = R.u - D * (int)R.pt;
         const int S = np - 1;
         double fp = bsum;
@@ -899,10 +919,10 @@ __device__ __forceinline__ void tp_operator(LDS& L, const Tile& T, const TpRow<D
     }
     __syncthreads();
     if (R.own) {
-        const int k = tile_fiber_of(L.fo, (int)T.nft, R.pt);
-        const int fo = (int)L.fo[k];
-        const int S = (int)L.fo[k + 1] - fo - 1;
-        const int p = (int)R.pt - fo;
+        const int k = R.k;
+        const int fo = R.fo;
+        const int S = R.np - 1;
+        const int p = R.p;
         const int sl = fo - k + p;
         double v = L.f[R.u];
         if (p < S) v = v + R.df0 * L.lam[sl];
@@ -920,14 +940,8 @@ __device__ __forceinline__ double tp_block_apply(LDS& L, const Tile& T, const Tp
                                                  const double* xin)
 {
     double acc[4] = { 0, 0, 0, 0 };
-    int k = 0, fo = 0, bs = 0;
-    bool use = false;
-    if (R.act) {
-        k = tile_fiber_of(L.fo, (int)T.nft, (uint32_t)R.u / D);
-        fo = (int)L.fo[k];
-        bs = D * ((int)L.fo[k + 1] - fo);
-        use = B != nullptr && L.fuse[k];
-    }
+    const int k = R.k, fo = R.fo, bs = D * R.np;
+    const bool use = R.act && B != nullptr && L.fuse[k];
     if (use) {
         const double* xs = xin + D * fo;
 #pragma unroll
@@ -989,6 +1003,7 @@ __global__ __launch_bounds__(kTpThreads) void k_tp(KArgs k)
     gath_values(GB, rec, (uint32_t)D, (uint32_t)dd);
     gath_values(GC, rec, 1u, 0u);
     tp_row_constants<D, TB>(a, T, R);
+    tp_seg_constants<D, TB>(a, T, R);
     SysScalars sc = state_load(k.st_in + T.sys);
     tile_tables<PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
     const uint32_t nt = k.ta.ntiles;
@@ -1127,6 +1142,7 @@ __global__ __launch_bounds__(kTpThreads) void k_tq(KArgs k)
     gath_values(GB, rec, (uint32_t)D, (uint32_t)dd);
     gath_values(GC, rec, 1u, 0u);
     tp_row_constants<D, TB>(a, T, R);
+    tp_seg_constants<D, TB>(a, T, R);
     SysScalars sc = state_load(k.st_in + T.sys);
     tile_tables<PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
     const uint32_t nt = k.ta.ntiles;
@@ -1333,21 +1349,20 @@ static void run_tiled(Context& c, const double* guess, int max_iter, double tol,
     hipLaunchKernelGGL((k_tinit<D, PC>), g, b, 0, c.stream, kI, guess, tol, tol_mode, max_iter, c.prm.time_step);
     bool first_p = true;
     CYMG_HIP(hipGetLastError());
+    // Chunks of iterations are queued one ahead: after queuing chunk k+1 the
+    // host waits for the state snapshot that ends chunk k, so the device never
+    // idles on the host's convergence check.  The first chunk covers 3/4 of the
+    // previous solve's iterations; launches queued after convergence exit after
+    // their prologue.
     int launched = 0;
     int& predicted = c.last_iters[PC];
-    int chunk = predicted > 0 ? predicted + 2 : 4;
+    constexpr int kStep = 8;
+    int first_chunk = predicted > 0 ? std::max(kStep, (predicted * 3) / 4) : kStep;
+    first_chunk += first_chunk & 1;
     const size_t sbytes = sizeof(SysScalars) * c.nsys;
-    bool first = true;
-    while (true) {
-        if (!first) {
-            CYMG_HIP(hipMemcpyAsync(c.h_scal, S1, sbytes, hipMemcpyDeviceToHost, c.stream));
-            CYMG_HIP(hipStreamSynchronize(c.stream));
-            bool all = true;
-            for (uint32_t s = 0; s < c.nsys; ++s) all = all && c.h_scal[s].done;
-            if (all || launched > max_iter + 4) break;
-        }
-        first = false;
-        for (int it = 0; it < chunk; it += 2) {
+    SysScalars* snap[2] = { c.h_snap, c.h_snap + 4096 };
+    auto enqueue = [&](int n, int slot) {
+        for (int it = 0; it < n; it += 2) {
             if (first_p) hipLaunchKernelGGL((k_tp<D, PC, true>), g, bt, 0, c.stream, kP0);
             else hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, kP0);
             first_p = false;
@@ -1356,10 +1371,22 @@ static void run_tiled(Context& c, const double* guess, int max_iter, double tol,
             hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ1);
         }
         CYMG_HIP(hipGetLastError());
-        launched += chunk;
-        chunk = predicted > 0 ? std::max(4, predicted / 8) : std::min(2 * chunk, 16);
-        chunk += chunk & 1;
+        CYMG_HIP(hipMemcpyAsync(snap[slot], S1, sbytes, hipMemcpyDeviceToHost, c.stream));
+        CYMG_HIP(hipEventRecord(c.chunk_ev[slot], c.stream));
+        launched += n;
+    };
+    enqueue(first_chunk, 0);
+    int cur = 0;
+    while (true) {
+        const bool more = launched <= max_iter + 4;
+        if (more) enqueue(kStep, cur ^ 1);
+        CYMG_HIP(hipEventSynchronize(c.chunk_ev[cur]));
+        bool all = true;
+        for (uint32_t s = 0; s < c.nsys; ++s) all = all && snap[cur][s].done;
+        if (all || !more) break;
+        cur ^= 1;
     }
+    std::memcpy(c.h_scal, snap[cur], sbytes);
     c.sys_reports.assign(c.nsys, cymg_solve_report{});
     int worst = 0, all_conv = 1;
     for (uint32_t s = 0; s < c.nsys; ++s) {

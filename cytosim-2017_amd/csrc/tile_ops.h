@@ -41,7 +41,7 @@ struct TileLds {
 };
 
 struct Tile {
-    uint32_t t, fb, nft, P0, tp, ts, U0, nu, sys, tb, te;
+    uint32_t t, fb, nft, P0, tp, ts, U0, nu, sys, tb, te, S0;
 };
 
 template <int D>
@@ -60,6 +60,7 @@ __device__ __forceinline__ Tile tile_begin(const TileArgs& ta, uint32_t t)
     T.sys = d.sys;
     T.tb = d.tb;
     T.te = d.te;
+    T.S0 = d.S0;
     return T;
 }
 
