@@ -360,8 +360,9 @@ size_t precond_lds_bytes(int D, int bs_max, int precision)
 // Gauss-Jordan inverse with the block in registers: one workgroup of NW
 // wavefronts per block, lane i owns row i (bs <= 64 * NW, bs <= BSMAX).  At
 // step k the candidate pivots are the rows not yet used as pivot (partial
-// pivoting by row search, largest magnitude, lowest row on ties: the pivot
-// dgetf2 takes, meca.cc:352-353); the pivot row is scaled and broadcast
+// pivoting by row search as dgetf2, meca.cc:352-353: largest magnitude, the
+// magnitude compared in FP32 with its 7 low mantissa bits dropped, lowest row
+// on ties within that resolution); the pivot row is scaled and broadcast
 // through LDS, every other row eliminates column k.  The row array rotates
 // left by one column per step, so column k is always a[0] and the inverse
 // column of the step is appended at a[BSMAX-1]: all register indices are
@@ -377,6 +378,33 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Maximum over a wavefront with DPP row operations (quad permutations,
+// half-row and row mirrors, row broadcasts 15 and 31): lane 63 ends with the
+// maximum, read back as a uniform value.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v)
+{
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+__device__ __forceinline__ double readlane_t(double v, int lane)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ float readlane_t(float v, int lane)
+{
+    return __uint_as_float((unsigned)__builtin_amdgcn_readlane((int)__float_as_uint(v), lane));
+}
+
 template <typename T, int BSMAX, int NW>
 __global__ __launch_bounds__(NW * kWave) void k_block_gj(const uint32_t* __restrict__ list,
                                                          const uint32_t* __restrict__ npv,
@@ -387,8 +415,8 @@ __global__ __launch_bounds__(NW * kWave) void k_block_gj(const uint32_t* __restr
     static_assert(BSMAX <= NW * kWave, "one row per lane");
     __shared__ T row[BSMAX];
     __shared__ int perm[BSMAX];
-    __shared__ T wbest[NW], wval[NW];
-    __shared__ int widx[NW];
+    __shared__ T wval[NW];
+    __shared__ unsigned wkey[NW];
     const uint32_t f = list[blockIdx.x];
     const int bs = D * (int)npv[f];
     const uint64_t off = blkoff[f];
@@ -402,52 +430,52 @@ __global__ __launch_bounds__(NW * kWave) void k_block_gj(const uint32_t* __restr
     int kst = 0, info = 0;
     for (int k = 0; k < bs; ++k) {
         const T ak = a[0];
-        T best = (i < bs && !used) ? (T)fabs(ak) : T(-1);
-        int bi = i;
-        for (int o = 32; o > 0; o >>= 1) {
-            const T ob = __shfl_xor(best, o, kWave);
-            const int oi = __shfl_xor(bi, o, kWave);
-            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-        }
+        // candidate key: magnitude (FP32 bits, low 7 bits cleared) | 127 - row
+        unsigned key = 0;
+        if (i < bs && !used && ak == ak) key = (__float_as_uint((float)fabs(ak)) & ~127u) | (unsigned)(127 - i);
+        unsigned best = wave_max_u32(key);
         int p;
         T piv;
         if constexpr (NW == 1) {
-            p = __builtin_amdgcn_readfirstlane(bi);
-            piv = __shfl(ak, p, kWave);
+            p = 127 - (int)(best & 127u);
+            piv = readlane_t(ak, best ? p : 0);
         } else {
-            if (lane == 0) { wbest[w] = best; widx[w] = bi; }
-            if (i == bi) wval[w] = ak;
+            const int pw = 127 - (int)(best & 127u);
+            if (lane == 0) wkey[w] = best;
+            if (best && i == pw) wval[w] = ak;
             __syncthreads();
-            T bb = wbest[0];
-            int bj = widx[0], ww = 0;
+            int ww = 0;
+            best = wkey[0];
 #pragma unroll
             for (int q = 1; q < NW; ++q)
-                if (wbest[q] > bb || (wbest[q] == bb && widx[q] < bj)) { bb = wbest[q]; bj = widx[q]; ww = q; }
-            best = bb;
-            p = bj;
-            piv = wval[ww];
+                if (wkey[q] > best) { best = wkey[q]; ww = q; }
+            p = 127 - (int)(best & 127u);
+            piv = best ? wval[ww] : T(0);
         }
-        if (!(best > T(0)) || piv == T(0)) {
+        if (best == 0u || piv == T(0)) {
             info = k + 1;
             break;
         }
         const T r = T(1) / piv;
         const bool isp = (i == p);
         if (isp) {
+            // the scaled pivot row goes to LDS; the registers are replaced by it
+            // in the update below (a divergent in-place scaling would keep two
+            // copies of the row live)
 #pragma unroll
-            for (int j = 1; j < BSMAX; ++j) {
-                a[j] = a[j] * r;
-                row[j - 1] = a[j];
-            }
+            for (int j = 1; j < BSMAX; ++j) row[j - 1] = a[j] * r;
             used = true;
             kst = k;
         }
         if (i == 0) perm[k] = p;
         if constexpr (NW == 1) wave_sync();
         else __syncthreads();
-        const T m = isp ? T(0) : ak;
+        const T m = ak;
 #pragma unroll
-        for (int j = 1; j < BSMAX; ++j) a[j - 1] = a[j] - m * row[j - 1];
+        for (int j = 1; j < BSMAX; ++j) {
+            const T rj = row[j - 1];
+            a[j - 1] = isp ? rj : a[j] - m * rj;
+        }
         a[BSMAX - 1] = isp ? r : -m * r;
     }
     if constexpr (NW == 1) wave_sync();
