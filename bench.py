@@ -209,6 +209,18 @@ def main():
                "sample": f"{args.cpu_steps} steps of the same 1000-fiber workload, FP64 block-LU BCGSP, "
                          f"mean {np.mean(its):.1f} iterations, single thread"}
 
+    traffic = args.traffic
+    if traffic is None:
+        # HBM bytes per iteration launch measured with rocprofv3 counters on this
+        # workload (FETCH_SIZE and WRITE_SIZE passes, calibrated by a plain copy)
+        tpath = os.path.join(ROOT, "profiles", "iteration_launch_traffic.json")
+        wl = f"fibers-3d-{args.nfib}-crosslinks-{args.nlinks}-sphere10"
+        if os.path.exists(tpath):
+            with open(tpath) as fh:
+                t = json.load(fh)
+            if t.get("workload") == wl:
+                traffic = t.get("traffic_bytes")
+
     best = results["fp64"]
     line = {
         "metric": "implicit steps/s @ rel.res 1e-6: plain FP64 GPU vs learned+mixed; BiCGStab iters",
@@ -230,7 +242,7 @@ def main():
                       "mean_bicgstab_iters": round(v["mean_iters"], 2)} for k, v in results.items()},
         "roofline": {"bound": "hbm", "kernel": "k_tp/k_tq (one launch of the FP64 BiCGStab iteration)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": args.traffic,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": int(it_bytes), "avg_launch_ms": round(it_ms, 5)},
         "kernels_ms": {"iteration_launch_fp64": it_ms, "iteration_launch_fp32": it32_ms,
                        "iteration_launch_fp32_gbs": it32_achieved and round(it32_achieved, 1),

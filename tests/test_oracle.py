@@ -228,6 +228,119 @@ def test_long_clamp_linearises_the_force_law(oracle, dim):
     assert np.abs(C - Jfd).max() < 1e-5 * w
 
 
+def _two_fiber_meca(oracle, dim):
+    pts = np.concatenate([_bent_fiber(dim, 5, 0.5, seed=1), _bent_fiber(dim, 4, 0.5, seed=2) + 1.0])
+    m = oracle.OracleMeca(dim, dict(time_step=0.005, kT=0.0042, viscosity=0.02))
+    m.add_fiber(5, 0.5, 20.0)
+    m.add_fiber(4, 0.5, 20.0)
+    m.prepare(pts)
+    return m, pts
+
+
+def _linear_force(oracle, m, dim, x):
+    """vBAS + (mB (x) I + mC) x: the force law an emitter recorded."""
+    B = m.matrix_dense(0)
+    C = m.matrix_dense(1)
+    m.setup(None)
+    bas = m.vector(oracle.BAS)
+    return np.kron(B, np.eye(dim)) @ x + C @ x + bas
+
+
+def _pt(x, dim, i):
+    return x[dim * i:dim * i + dim]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("kind", ["link_ee", "link_ie", "clamp_e", "clamp_i", "plane_e", "pure_force"])
+def test_linear_emitters_follow_their_force_law(oracle, dim, kind):
+    """Each Hookean emitter (meca_inter.cc: interLink PE-PE 196-230, PI-PE
+    240-275, interClamp PE 1883-1907 and PI, interPlane PE 2301-2327,
+    addPureForce) records a force that is exactly linear in the positions:
+    vBAS + (mB (x) I + mC) x equals the spring force at any x."""
+    rng = np.random.default_rng(7 + dim)
+    m, pts = _two_fiber_meca(oracle, dim)
+    n = pts.size
+    w = 37.5
+    g = rng.standard_normal(dim)
+    nrm = rng.standard_normal(dim)
+    nrm /= np.linalg.norm(nrm)
+    a1, a2, ca, b = 1, 2, 0.3, 6
+    if kind == "link_ee":
+        m.inter_link_ee(a1, b, w)
+    elif kind == "link_ie":
+        m.inter_link_ie(a1, a2, ca, b, w)
+    elif kind == "clamp_e":
+        m.inter_clamp_e(b, g, w)
+    elif kind == "clamp_i":
+        m.inter_clamp_i(a1, a2, ca, g, w)
+    elif kind == "plane_e":
+        m.inter_plane_e(b, nrm, g, w)
+    else:
+        m.add_pure_force(b, g)
+    for trial in range(3):
+        x = pts + 0.3 * rng.standard_normal(n)
+        F = np.zeros(n)
+        if kind == "link_ee":
+            f = w * (_pt(x, dim, b) - _pt(x, dim, a1))
+            F[dim * a1:dim * a1 + dim] += f
+            F[dim * b:dim * b + dim] -= f
+        elif kind == "link_ie":
+            pa = (1 - ca) * _pt(x, dim, a1) + ca * _pt(x, dim, a2)
+            f = w * (_pt(x, dim, b) - pa)
+            F[dim * a1:dim * a1 + dim] += (1 - ca) * f
+            F[dim * a2:dim * a2 + dim] += ca * f
+            F[dim * b:dim * b + dim] -= f
+        elif kind == "clamp_e":
+            F[dim * b:dim * b + dim] += w * (g - _pt(x, dim, b))
+        elif kind == "clamp_i":
+            f = w * (g - ((1 - ca) * _pt(x, dim, a1) + ca * _pt(x, dim, a2)))
+            F[dim * a1:dim * a1 + dim] += (1 - ca) * f
+            F[dim * a2:dim * a2 + dim] += ca * f
+        elif kind == "plane_e":
+            F[dim * b:dim * b + dim] += w * nrm * (nrm @ (g - _pt(x, dim, b)))
+        else:
+            F[dim * b:dim * b + dim] += g
+        assert np.abs(_linear_force(oracle, m, dim, x) - F).max() < 1e-11 * max(w, 1.0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("stretch", [1.6, 0.6])
+def test_long_link_linearisation(oracle, dim, stretch):
+    """interLongLink PE-PE (meca_inter.cc:334-405): F = w (|ab| - len) ab/|ab|.
+    The recorded terms give the exact force at the current positions; when the
+    link is stretched the matrix is the exact Jacobian (finite differences),
+    when compressed it is the rank-1 negative semidefinite -w u u' block that
+    removes the negative eigenvalues (meca_inter.cc:372-374)."""
+    rng = np.random.default_rng(3)
+    m, pts = _two_fiber_meca(oracle, dim)
+    a, b, w = 1, 6, 50.0
+    d0 = np.linalg.norm(_pt(pts, dim, b) - _pt(pts, dim, a))
+    length = d0 / stretch
+    m.inter_long_link_ee(a, b, length, w)
+
+    def force(x):
+        ab = _pt(x, dim, b) - _pt(x, dim, a)
+        nab = np.linalg.norm(ab)
+        f = w * (nab - length) * ab / nab
+        F = np.zeros(x.size)
+        F[dim * a:dim * a + dim] += f
+        F[dim * b:dim * b + dim] -= f
+        return F
+
+    assert np.abs(_linear_force(oracle, m, dim, pts) - force(pts)).max() < 1e-10 * w
+    C = m.matrix_dense(1)
+    ia = slice(dim * a, dim * a + dim)
+    if stretch > 1:
+        h = 1e-6
+        J = np.stack([(force(pts + h * e) - force(pts - h * e)) / (2 * h) for e in np.eye(pts.size)], axis=1)
+        assert np.abs(C - J).max() < 1e-5 * w
+    else:
+        u = _pt(pts, dim, b) - _pt(pts, dim, a)
+        u /= np.linalg.norm(u)
+        assert np.abs(C[ia, ia] + w * np.outer(u, u)).max() < 1e-10 * w
+        assert np.linalg.eigvalsh(C[ia, ia]).max() < 1e-10 * w
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_operator_equals_dense_construction(oracle, scenes, dim):
     """Meca::multiply == I - dt*mob*P (B x I + C + R + P') built from the pieces."""
