@@ -35,8 +35,11 @@ EXPORTED = [
     "cymg_precondition", "cymg_compute_forces", "cymg_multiply_device", "cymg_precondition_device",
     "cymg_get_vector", "cymg_get_points", "cymg_get_tensions", "cymg_get_block", "cymg_get_matrix",
     "cymg_get_timings", "cymg_enable_timing", "cymg_time_kernel", "cymg_set_option",
+    "cymg_guess_init", "cymg_guess_weights", "cymg_guess_record", "cymg_guess_samples",
+    "cymg_guess_train", "cymg_guess_check", "cymg_guess_features",
 ]
 OPT_KRYLOV_LAUNCHES, OPT_BLOCK_INVERSE, OPT_TILE_POINTS = 1, 2, 3
+GUESS_INPUTS, GUESS_OUTPUTS, GUESS_POINTS, GUESS_WEIGHTS = 288, 96, 31, 230240
 
 
 class SimParams(C.Structure):
@@ -53,7 +56,8 @@ class SolveParams(C.Structure):
 class SolveReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("flag", C.c_int), ("converged", C.c_int),
                 ("fallback", C.c_int), ("precond_used", C.c_int), ("total_iterations", C.c_int),
-                ("residual", C.c_double), ("threshold", C.c_double), ("noise_level", C.c_double)]
+                ("residual", C.c_double), ("threshold", C.c_double), ("noise_level", C.c_double),
+                ("guess_used", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -145,6 +149,13 @@ def lib():
         "cymg_enable_timing": (C.c_int, [vp, C.c_int]),
         "cymg_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "cymg_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
+        "cymg_guess_init": (C.c_int, [vp, C.c_uint64]),
+        "cymg_guess_weights": (C.c_int, [vp, C.c_int, vp]),
+        "cymg_guess_record": (C.c_int, [vp, u32]),
+        "cymg_guess_samples": (C.c_int, [vp, C.POINTER(u32)]),
+        "cymg_guess_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_uint64, C.POINTER(C.c_double)]),
+        "cymg_guess_check": (C.c_int, [vp, vp, vp, C.c_int, C.POINTER(C.c_double), vp]),
+        "cymg_guess_features": (C.c_int, [vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -398,6 +409,49 @@ class Meca:
         t = Timings()
         self._chk(self.L.cymg_get_timings(self.h, C.byref(t)))
         return t.as_dict()
+
+    # learned initial guess (no counterpart in the reference) ------------------------
+    def guess_init(self, seed=0):
+        self._chk(self.L.cymg_guess_init(self.h, int(seed)))
+
+    def guess_weights(self):
+        w = np.empty(GUESS_WEIGHTS, np.float32)
+        self._chk(self.L.cymg_guess_weights(self.h, 0, _p(w)))
+        return w
+
+    def set_guess_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == GUESS_WEIGHTS
+        self._chk(self.L.cymg_guess_weights(self.h, 1, _p(w)))
+
+    def guess_record(self, capacity):
+        self._chk(self.L.cymg_guess_record(self.h, int(capacity)))
+
+    def guess_samples(self):
+        n = C.c_uint32()
+        self._chk(self.L.cymg_guess_samples(self.h, C.byref(n)))
+        return n.value
+
+    def guess_train(self, steps, batch=256, lr=1e-3, seed=0):
+        loss = C.c_double()
+        self._chk(self.L.cymg_guess_train(self.h, int(steps), int(batch), float(lr), int(seed), C.byref(loss)))
+        return loss.value
+
+    def guess_check(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        assert X.shape[1] == GUESS_INPUTS and T.shape == (X.shape[0], GUESS_OUTPUTS)
+        g = np.empty(GUESS_WEIGHTS, np.float32)
+        loss = C.c_double()
+        self._chk(self.L.cymg_guess_check(self.h, _p(X), _p(T), X.shape[0], C.byref(loss), _p(g)))
+        return loss.value, g
+
+    def guess_features(self):
+        nf = len(self.np)
+        X = np.empty((nf, GUESS_INPUTS), np.float32)
+        s = np.empty(nf, np.float32)
+        self._chk(self.L.cymg_guess_features(self.h, _p(X), _p(s)))
+        return X, s
 
 
 def load_batch(meca, batch, couples_on_device=True):

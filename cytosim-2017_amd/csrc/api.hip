@@ -248,16 +248,30 @@ void do_iterate(Context& c, const cymg_solve_params* p, cymg_solve_report* rep)
     const int maxit = p->max_iter > 0 ? p->max_iter : (int)c.n();
     const int pc = p->precondition;
     if (pc < 0 || pc > 2) throw Error(CYMG_ERR_INVALID, "precondition must be 0, 1 or 2");
+    if (p->guess < CYMG_GUESS_ZERO || p->guess > CYMG_GUESS_LEARNED) throw Error(CYMG_ERR_INVALID, "unknown guess");
     const double* guess = nullptr;
-    if (p->guess == CYMG_GUESS_PREVIOUS) guess = c.d_prev_sol.get();
-    else if (p->guess == CYMG_GUESS_LEARNED) throw Error(CYMG_ERR_UNSUPPORTED, "learned guess not loaded");
+    int guess_used = CYMG_GUESS_ZERO;
+    if (p->guess == CYMG_GUESS_PREVIOUS) {
+        guess = c.d_prev_sol.get();
+        guess_used = CYMG_GUESS_PREVIOUS;
+    }
     if (pc) {
         Timer t(c, 2);
         launch_precond_build(c, pc);
     }
+    // the learned guess and the recording of training samples need the
+    // features of this step and M^-1 b
+    const bool learned = p->guess == CYMG_GUESS_LEARNED && c.mlp.ready;
+    c.mlp.step_ready = false;
+    if (learned || c.mlp.ds_cap) mlp_prepare_step(c, pc);
+    if (learned) {
+        guess = mlp_guess(c);
+        guess_used = CYMG_GUESS_LEARNED;
+    }
     {
         Timer t(c, 3);
         krylov_solve(c, pc, guess, maxit, p->tolerance, p->tol_mode, &r);
+        if (r.converged && c.mlp.ds_cap) mlp_record(c);
         int total = r.iterations;
         if (!r.converged && p->fallback) {
             // meca.cc:749-799: retry with vRHS as the seed, then switch method
@@ -282,6 +296,7 @@ void do_iterate(Context& c, const cymg_solve_params* p, cymg_solve_report* rep)
         }
         r.total_iterations = total;
     }
+    r.guess_used = guess_used;
     if (rep) *rep = r;
     if (r.fallback == 3) throw Error(CYMG_ERR_CONVERGENCE, "convergence failure in solver");
 }
@@ -931,6 +946,65 @@ int cymg_time_kernel(cymg_context* ctx, int which, int reps, double* avg_ms)
         float ms = 0;
         CYMG_HIP(hipEventElapsedTime(&ms, c.ev[8], c.ev[9]));
         *avg_ms = ms / reps;
+    });
+}
+
+int cymg_guess_init(cymg_context* ctx, uint64_t seed)
+{
+    return guarded(ctx, [&](Context& c) { mlp_init(c, seed); });
+}
+
+int cymg_guess_weights(cymg_context* ctx, int set, float* w)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!w) throw Error(CYMG_ERR_INVALID, "null weights");
+        if (set) mlp_set_weights(c, w);
+        else {
+            if (!c.mlp.ready) throw Error(CYMG_ERR_INVALID, "no weights loaded");
+            mlp_get_weights(c, w);
+        }
+    });
+}
+
+int cymg_guess_record(cymg_context* ctx, uint32_t capacity)
+{
+    return guarded(ctx, [&](Context& c) { mlp_set_record(c, capacity); });
+}
+
+int cymg_guess_samples(cymg_context* ctx, uint32_t* count)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!count) throw Error(CYMG_ERR_INVALID, "null output");
+        *count = c.mlp.ds_count;
+    });
+}
+
+int cymg_guess_train(cymg_context* ctx, int steps, int batch, double lr, uint64_t seed, double* loss)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (steps < 0 || !(lr > 0.0)) throw Error(CYMG_ERR_INVALID, "invalid training arguments");
+        if (!c.mlp.ready) throw Error(CYMG_ERR_INVALID, "no weights loaded");
+        const double l = mlp_train(c, steps, batch, lr, seed);
+        if (loss) *loss = l;
+    });
+}
+
+int cymg_guess_check(cymg_context* ctx, const float* X, const float* T, int batch, double* loss, float* grads)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!X || !T || !grads) throw Error(CYMG_ERR_INVALID, "null array");
+        if (!c.mlp.ready) throw Error(CYMG_ERR_INVALID, "no weights loaded");
+        const double l = mlp_check(c, X, T, batch, grads);
+        if (loss) *loss = l;
+    });
+}
+
+int cymg_guess_features(cymg_context* ctx, float* X, float* scale)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!X || !scale) throw Error(CYMG_ERR_INVALID, "null output");
+        if (!c.mlp.step_ready) throw Error(CYMG_ERR_INVALID, "no step features computed");
+        mlp_features(c, X, scale);
     });
 }
 

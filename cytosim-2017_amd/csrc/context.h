@@ -3,6 +3,7 @@
 #pragma once
 
 #include "common.h"
+#include "mlp.h"
 
 namespace cymg {
 
@@ -83,6 +84,9 @@ struct Context {
     int last_iters[3] = {0, 0, 0};     // iterations of the previous solve, per preconditioner
 
     bool prepared = false, setup_done = false, pc_ready = false, have_solution = false;
+
+    // learned initial guess
+    MlpState mlp;
 
     // timing
     bool timing = false;

@@ -80,6 +80,8 @@ typedef struct {
     double residual;          /* Monitor::residual() */
     double threshold;
     double noise_level;       /* noiseLevel (meca.cc:633-643) */
+    int    guess_used;        /* CYMG_GUESS_* seeding the first attempt (LEARNED falls back to ZERO
+                                 when no network is loaded) */
 } cymg_solve_report;
 
 /* context ---------------------------------------------------------------- */
@@ -210,6 +212,40 @@ int         cymg_enable_timing(cymg_context* ctx, int enable);
      (`reps` launches after an init launch, zero threshold; no counterpart in the reference),
    7 the same without preconditioner */
 int         cymg_time_kernel(cymg_context* ctx, int which, int reps, double* avg_ms);
+
+/* learned initial guess (no counterpart in the reference, which seeds BCGS with
+   vSOL = 0, meca.cc:704).  A per-fiber network (inputs: the fiber's points
+   relative to their centroid / 5, rhs / s and previous solution / s, s = the
+   fiber's |rhs|_inf, 31 points x 3 each, zero-padded to 288; three hidden
+   layers of 256 with GELU; 96 outputs) predicts the correction c of the
+   block-Jacobi solution: x0 = M^-1 b + s*c (M^-1 = 0 without preconditioner).
+   Fibers of more than 31 points are seeded with M^-1 b.  GEMMs run on BF16
+   MFMA with FP32 accumulation, weights are FP32 with BF16 copies.  The solver
+   iterates from x0 to the same stop test as from any other guess.
+   Weights are flat FP32, layer by layer: W1 [288][256], b1 [256], W2 [256][256],
+   b2, W3 [256][256], b3, W4 [256][96], b4 [96]; W[in][out] row-major. */
+#define CYMG_GUESS_INPUTS   288
+#define CYMG_GUESS_OUTPUTS  96
+#define CYMG_GUESS_POINTS   31
+#define CYMG_GUESS_WEIGHTS  230240
+int         cymg_guess_init(cymg_context* ctx, uint64_t seed);             /* uniform(+-1/sqrt(in)) weights, zero biases */
+int         cymg_guess_weights(cymg_context* ctx, int set, float* w);      /* set != 0: load w; else copy the weights out */
+/* record (features, targets) of every fiber of each converged solve into a
+   ring of `capacity` samples (0 stops recording); targets are (x - M^-1 b)/s */
+int         cymg_guess_record(cymg_context* ctx, uint32_t capacity);
+int         cymg_guess_samples(cymg_context* ctx, uint32_t* count);
+/* `steps` Adam steps (beta 0.9/0.999, eps 1e-8) on random minibatches of
+   `batch` recorded samples (batch a multiple of 16); loss of the last step:
+   sum of squared errors over the valid outputs / batch */
+int         cymg_guess_train(cymg_context* ctx, int steps, int batch, double lr, uint64_t seed, double* loss);
+/* forward + backward on host inputs X [batch][288] (rounded to BF16) and
+   targets T [batch][96] (T[.][95] = number of valid outputs); loss and the
+   gradient of every weight (CYMG_GUESS_WEIGHTS floats) */
+int         cymg_guess_check(cymg_context* ctx, const float* X, const float* T, int batch, double* loss,
+                             float* grads);
+/* features [nfib][288] (as FP32 values of the BF16 inputs) and scales [nfib] of
+   the last iterate() that used or recorded the learned guess */
+int         cymg_guess_features(cymg_context* ctx, float* X, float* scale);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,164 @@
+"""Learned initial guess on the device (cymg_guess_*), checked against the
+oracle restatement (oracle/mlp_oracle.py): loss and gradients of the BF16-MFMA
+network, the features of a step, training that lowers the loss, and a trained
+guess that lowers the BiCGStab iteration count while the step still agrees
+with the oracle to the parity bar.
+
+Bars: loss within 1e-3 relative and each layer's weight and bias gradients
+within 1e-2 relative (2-norm) of the oracle evaluated on the same BF16-rounded
+operands (the device accumulates in FP32 and may round a boundary value to the
+other BF16 neighbour); features bit-equal except position slots, which may
+differ by one BF16 ulp or by rounding noise (centroid summation order); positions after a step
+seeded by the network within 1e-9 relative of the oracle's zero-seeded step at
+relative residual 1e-12 (both stop at the residual bar from different seeds, so
+the bar on positions needs the tighter residual).
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_from_scene, rel_err
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(B, seed):
+    rng = np.random.default_rng(seed)
+    w = mo.init(seed)
+    for W, b in mo.unpack(w):
+        b[:] = rng.uniform(-0.1, 0.1, b.size).astype(np.float32)
+    X = rng.standard_normal((B, mo.INPUTS)).astype(np.float32)
+    T = (rng.standard_normal((B, mo.OUTPUTS)) * 0.3).astype(np.float32)
+    T[:, -1] = rng.integers(0, 94, B)
+    T[0, -1] = 93
+    return w, X, T
+
+
+@pytest.mark.parametrize("B,seed", [(16, 1), (64, 2), (256, 3)])
+def test_gradients_match_oracle(pkg, B, seed):
+    w, X, T = _problem(B, seed)
+    m = pkg.Meca(3, 0)
+    m.set_guess_weights(w)
+    assert np.array_equal(m.guess_weights(), w)
+    print(f"guess_check B={B} seed={seed}", flush=True)
+    loss, g = m.guess_check(X, T)
+    lo, go = mo.loss_and_grad(w.astype(np.float64), X, T, rounding=True)
+    assert abs(loss - lo) <= 1e-3 * lo, (loss, lo)
+    off = 0
+    for q, (i, o) in enumerate(zip(mo.LAYER_IN, mo.LAYER_OUT)):
+        for name, n in (("W", i * o), ("b", o)):
+            a, b = g[off:off + n], go[off:off + n]
+            err = np.linalg.norm(a - b) / np.linalg.norm(b)
+            assert err <= 1e-2, (q, name, err)
+            off += n
+    m.close()
+
+
+def test_gradient_rejects_bad_batch(pkg):
+    w, X, T = _problem(16, 4)
+    m = pkg.Meca(3, 0)
+    m.set_guess_weights(w)
+    with pytest.raises(pkg.CymgError):
+        m.guess_check(X[:10], T[:10])
+    m.close()
+
+
+def _step(m, pkg, sc, step, precond=None, tol=1e-6, guess=None, apply=True):
+    if step > 0:
+        m.prepare(None)
+        pkg.assemble(m, sc, upload_couples=False)
+    m.setup(sc.gauss(step))
+    rep = m.iterate(tol, tol_mode=pkg.TOL_RELATIVE,
+                    precondition=pkg.PRECOND_FP64 if precond is None else precond,
+                    guess=pkg.GUESS_ZERO if guess is None else guess)
+    if apply:
+        m.apply()
+    return rep
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_features_match_oracle(pkg, scenes, dim):
+    sc = scenes.config2(seed=4, nfib=60, nlinks=300) if dim == 3 else scenes.mini(dim=2, seed=4, nfib=12,
+                                                                                    nlinks=40)
+    m = pkg.Meca(dim, 0)
+    pkg.load_scene(m, sc)
+    m.guess_record(4096)
+    prev = np.zeros(sc.size)
+    for step in range(2):
+        print(f"features dim={dim} step={step}", flush=True)
+        _step(m, pkg, sc, step, apply=False)
+        X, S = m.guess_features()
+        Xo, So = mo.features(dim, sc.nb_points, m.vector(pkg.VEC_PTS), m.vector(pkg.VEC_RHS), prev)
+        assert np.array_equal(S, So)
+        pos = np.zeros(mo.INPUTS, bool)
+        pos[:93] = True
+        d = X != Xo
+        assert not np.any(d[:, ~pos])
+        if np.any(d):
+            a, b = X[d].astype(np.float64), Xo[d].astype(np.float64)
+            # a centroid equal to a coordinate leaves rounding noise (~1e-17) in that slot
+            assert np.all(np.abs(a - b) <= 2 ** -7 * np.maximum(np.abs(a), np.abs(b)) * 1.01 + 1e-12)
+            assert d.sum() <= 0.01 * d.size
+        prev = m.vector(pkg.VEC_SOL)
+        m.apply()
+        assert m.guess_samples() == min(4096, (step + 1) * sc.nfib)
+    m.close()
+
+
+def test_learned_guess_without_weights_falls_back(pkg, scenes):
+    sc = scenes.mini(dim=3, seed=5, nfib=8, nlinks=20)
+    m = pkg.Meca(3, 0)
+    pkg.load_scene(m, sc)
+    rep = _step(m, pkg, sc, 0, guess=pkg.GUESS_LEARNED)
+    assert rep["converged"] and rep["guess_used"] == pkg.GUESS_ZERO
+    m.close()
+
+
+@pytest.mark.parametrize("precond", [0, 1, 2])
+def test_learned_guess_step_matches_oracle(pkg, scenes, oracle, precond):
+    # an untrained network still only seeds the solve: the step agrees with the
+    # oracle's zero-seeded step at a tight tolerance
+    sc = scenes.mini(dim=3, seed=5, nfib=8, nlinks=20)
+    m = pkg.Meca(3, 0)
+    pkg.load_scene(m, sc)
+    m.guess_init(9)
+    print(f"learned step precond={precond}", flush=True)
+    rep = m.solve(1e-12, sc.gauss(0), tol_mode=pkg.TOL_RELATIVE, precondition=precond, guess=pkg.GUESS_LEARNED)
+    assert rep["converged"] and rep["guess_used"] == pkg.GUESS_LEARNED
+    orc = oracle_from_scene(oracle, sc)
+    orc.solve(1e-12, sc.gauss(0), tol_mode=1, precondition=min(precond, 1))
+    bar = 1e-9 if precond != pkg.PRECOND_FP32 else 1e-6
+    assert rel_err(m.fiber_points(), orc.fiber_points()) <= bar
+    m.close()
+
+
+def test_training_lowers_loss_and_iterations(pkg, scenes):
+    sc = scenes.config2(seed=6, nfib=200, nlinks=3000)
+    m = pkg.Meca(3, 0)
+    pkg.load_scene(m, sc)
+    m.guess_record(1 << 15)
+    nrec = 16
+    for step in range(nrec):
+        _step(m, pkg, sc, step)
+    assert m.guess_samples() == nrec * sc.nfib
+    m.guess_init(3)
+    print("training", flush=True)
+    first = m.guess_train(1, batch=256, lr=1e-3, seed=1)
+    last = m.guess_train(600, batch=256, lr=1e-3, seed=2)
+    print(f"loss {first:.4g} -> {last:.4g}", flush=True)
+    assert last < 0.5 * first
+    m.guess_record(0)
+    zero, learned = [], []
+    for step in range(nrec, nrec + 4):
+        r0 = _step(m, pkg, sc, step, apply=False)
+        x0 = m.vector(pkg.VEC_SOL)
+        r1 = m.iterate(1e-6, tol_mode=pkg.TOL_RELATIVE, precondition=pkg.PRECOND_FP64, guess=pkg.GUESS_LEARNED)
+        assert r0["converged"] and r1["converged"] and r1["guess_used"] == pkg.GUESS_LEARNED
+        x1 = m.vector(pkg.VEC_SOL)
+        assert rel_err(x1, x0) <= 1e-4
+        zero.append(r0["iterations"])
+        learned.append(r1["iterations"])
+        m.apply()
+    print(f"iterations zero {zero} learned {learned}", flush=True)
+    assert sum(learned) < sum(zero)
+    m.close()
