@@ -9,6 +9,11 @@ relative residual |r|_inf/|b|_inf < 1e-6 + apply (update, forces, reshape).
 Fiber points and the per-step N(0,1) Brownian draws are resident in HBM before
 the timed region.
 
+Modes: fp64 (zero seed, FP64 preconditioner blocks: `value`), fp32 (FP32
+blocks) and learned-fp32 (FP32 blocks, seeded by the per-fiber network trained
+on the device from an untimed rollout of FP64 steps; rollout and training time
+are reported beside it, not inside the timed steps).
+
 Prints one JSON line.  --gpus N with N > 1 exits: every box has one GPU.
 """
 import argparse
@@ -63,7 +68,14 @@ def parse():
     ap.add_argument("--nlinks", type=int, default=20000)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps timed for cpu_baseline (0: skip)")
-    ap.add_argument("--modes", default="fp64,fp32")
+    ap.add_argument("--modes", default="fp64,fp32,learned-fp32",
+                    help="fp64 / fp32: zero-seeded BCGSP with FP64 / FP32 preconditioner blocks; "
+                         "learned-fp32: seeded by the trained per-fiber network, FP32 blocks")
+    ap.add_argument("--guess-rollout", type=int, default=32,
+                    help="untimed FP64 steps recorded as training samples for the learned guess")
+    ap.add_argument("--guess-train-steps", type=int, default=1000)
+    ap.add_argument("--guess-batch", type=int, default=512)
+    ap.add_argument("--guess-lr", type=float, default=1e-3)
     ap.add_argument("--timed-launches", type=int, default=40,
                     help="iteration launches timed with HIP events for the roofline")
     ap.add_argument("--traffic", type=float, default=None,
@@ -75,17 +87,46 @@ def parse():
     return ap.parse_args()
 
 
-def run_mode(pkg, sc, precond, steps, warmup, tol, gauss_dev, torch):
+def train_guess(pkg, m, sc, args, tol, torch):
+    """Record a rollout of zero-seeded FP64 steps (Brownian draws of their own,
+    distinct from the timed steps'), then train the network on the device."""
+    t0 = time.perf_counter()
+    m.guess_record(args.guess_rollout * sc.nfib)
+    for s in range(args.guess_rollout):
+        if s > 0:
+            m.prepare(None)
+            pkg.assemble(m, sc, upload_couples=False)
+        m.solve(tol, sc.gauss(1000 + s), tol_mode=pkg.TOL_RELATIVE, precondition=pkg.PRECOND_FP64)
+    m.guess_record(0)
+    t1 = time.perf_counter()
+    m.guess_init(1)
+    first = m.guess_train(1, batch=args.guess_batch, lr=args.guess_lr, seed=1)
+    last = m.guess_train(args.guess_train_steps, batch=args.guess_batch, lr=args.guess_lr, seed=2)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return {"rollout_steps": args.guess_rollout, "samples": args.guess_rollout * sc.nfib,
+            "rollout_s": round(t1 - t0, 3), "train_steps": args.guess_train_steps, "batch": args.guess_batch,
+            "train_s": round(t2 - t1, 3), "loss_first": first, "loss_last": last}
+
+
+def run_mode(pkg, sc, precond, steps, warmup, tol, gauss_dev, torch, guess=0, train=None):
     m = pkg.Meca(sc.dim, 0)
     pkg.load_scene(m, sc)
     iters = []
+    info = train(m) if train else None
+    if train:
+        # the timed steps start from the scene's initial state, like the other modes
+        m.prepare(sc.points)
+        pkg.assemble(m, sc, upload_couples=False)
 
     def one(step):
         if step > 0:
             m.prepare(None)
             pkg.assemble(m, sc, upload_couples=False)
-        rep = m.solve(tol, tol_mode=pkg.TOL_RELATIVE, precondition=precond,
+        rep = m.solve(tol, tol_mode=pkg.TOL_RELATIVE, precondition=precond, guess=guess,
                       gauss_device_ptr=gauss_dev[step % len(gauss_dev)].data_ptr())
+        if guess and rep["guess_used"] != guess:
+            raise RuntimeError("learned guess not used")
         iters.append(rep["iterations"])
         return rep
 
@@ -98,7 +139,7 @@ def run_mode(pkg, sc, precond, steps, warmup, tol, gauss_dev, torch):
         one(warmup + s)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return m, dt, iters
+    return m, dt, iters, info
 
 
 def main():
@@ -119,13 +160,19 @@ def main():
 
     results = {}
     for mode in args.modes.split(","):
-        precond = {"fp64": pkg.PRECOND_FP64, "fp32": pkg.PRECOND_FP32}[mode]
+        precond = {"fp64": pkg.PRECOND_FP64, "fp32": pkg.PRECOND_FP32, "learned-fp32": pkg.PRECOND_FP32}[mode]
+        learned = mode.startswith("learned")
         print(f"bench: mode {mode} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
-        m, dt, iters = run_mode(pkg, sc, precond, args.steps, args.warmup, args.tol, gauss_dev, torch)
+        m, dt, iters, info = run_mode(pkg, sc, precond, args.steps, args.warmup, args.tol, gauss_dev, torch,
+                                      guess=pkg.GUESS_LEARNED if learned else pkg.GUESS_ZERO,
+                                      train=(lambda mm: train_guess(pkg, mm, sc, args, args.tol, torch))
+                                      if learned else None)
         results[mode] = dict(seconds=dt, steps_per_s=args.steps / dt, ms_per_step=1e3 * dt / args.steps,
-                             mean_iters=float(np.mean(iters)), meca=m)
-        print(f"bench: {mode} {args.steps / dt:.2f} steps/s, mean iterations {np.mean(iters):.1f}",
-              file=sys.stderr, flush=True)
+                             mean_iters=float(np.mean(iters)), meca=m, training=info)
+        print(f"bench: {mode} {args.steps / dt:.2f} steps/s, mean iterations {np.mean(iters):.1f}"
+              + (f", training {info}" if info else ""), file=sys.stderr, flush=True)
+        if mode != "fp64":
+            m.close()
 
     # dominant-kernel timing on the final state of the FP64 run
     m = results["fp64"]["meca"]
@@ -239,7 +286,10 @@ def main():
                    "fibers": args.nfib, "points": sc.npts, "unknowns": sc.size, "links": args.nlinks,
                    "tolerance": f"|r|inf/|b|inf<{args.tol:g}"},
         "modes": {k: {"steps_per_s": round(v["steps_per_s"], 3), "ms_per_step": round(v["ms_per_step"], 3),
-                      "mean_bicgstab_iters": round(v["mean_iters"], 2)} for k, v in results.items()},
+                      "mean_bicgstab_iters": round(v["mean_iters"], 2),
+                      **({"training": v["training"]} if v["training"] else {})} for k, v in results.items()},
+        "learned_mixed_vs_plain_fp64": (round(results["learned-fp32"]["steps_per_s"] / best["steps_per_s"], 3)
+                                        if "learned-fp32" in results else None),
         "roofline": {"bound": "hbm", "kernel": "k_tp/k_tq (one launch of the FP64 BiCGStab iteration)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

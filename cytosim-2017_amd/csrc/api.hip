@@ -263,7 +263,7 @@ void do_iterate(Context& c, const cymg_solve_params* p, cymg_solve_report* rep)
     // features of this step and M^-1 b
     const bool learned = p->guess == CYMG_GUESS_LEARNED && c.mlp.ready;
     c.mlp.step_ready = false;
-    if (learned || c.mlp.ds_cap) mlp_prepare_step(c, pc);
+    if (learned || c.mlp.recording) mlp_prepare_step(c, pc);
     if (learned) {
         guess = mlp_guess(c);
         guess_used = CYMG_GUESS_LEARNED;
@@ -271,7 +271,7 @@ void do_iterate(Context& c, const cymg_solve_params* p, cymg_solve_report* rep)
     {
         Timer t(c, 3);
         krylov_solve(c, pc, guess, maxit, p->tolerance, p->tol_mode, &r);
-        if (r.converged && c.mlp.ds_cap) mlp_record(c);
+        if (r.converged && c.mlp.recording) mlp_record(c);
         int total = r.iterations;
         if (!r.converged && p->fallback) {
             // meca.cc:749-799: retry with vRHS as the seed, then switch method

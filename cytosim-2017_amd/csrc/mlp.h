@@ -30,6 +30,7 @@ struct MlpState {
     DevBuf<float> fs;                          // per-fiber scale |rhs|_inf
     DevBuf<double> y, x0;                      // M^-1 b and the guess
     // recorded (features, targets) ring
+    bool recording = false;
     uint32_t ds_cap = 0, ds_count = 0, ds_next = 0;
     DevBuf<uint16_t> ds_x;
     DevBuf<float> ds_t;

@@ -462,7 +462,7 @@ const double* mlp_guess(Context& c)
 void mlp_record(Context& c)
 {
     MlpState& s = c.mlp;
-    if (!s.ds_cap || !s.step_ready) return;
+    if (!s.recording || !s.step_ready) return;
     const uint32_t F = c.nfib;
     if (F > s.ds_cap) return;
     DevBuf<float>& T = s.tgt;
@@ -508,12 +508,12 @@ void mlp_features(Context& c, float* X, float* scale)
 void mlp_set_record(Context& c, uint32_t capacity)
 {
     MlpState& s = c.mlp;
+    s.recording = capacity > 0;
+    if (!capacity) return;   // stop; the samples stay for training
     s.ds_cap = capacity;
     s.ds_count = s.ds_next = 0;
-    if (capacity) {
-        s.ds_x.resize((size_t)capacity * kMlpIn);
-        s.ds_t.resize((size_t)capacity * kMlpOut);
-    }
+    s.ds_x.resize((size_t)capacity * kMlpIn);
+    s.ds_t.resize((size_t)capacity * kMlpOut);
 }
 
 }  // namespace cymg

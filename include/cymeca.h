@@ -231,7 +231,8 @@ int         cymg_time_kernel(cymg_context* ctx, int which, int reps, double* avg
 int         cymg_guess_init(cymg_context* ctx, uint64_t seed);             /* uniform(+-1/sqrt(in)) weights, zero biases */
 int         cymg_guess_weights(cymg_context* ctx, int set, float* w);      /* set != 0: load w; else copy the weights out */
 /* record (features, targets) of every fiber of each converged solve into a
-   ring of `capacity` samples (0 stops recording); targets are (x - M^-1 b)/s */
+   ring of `capacity` samples, discarding earlier samples; 0 stops recording and
+   keeps the samples for training.  Targets are (x - M^-1 b)/s. */
 int         cymg_guess_record(cymg_context* ctx, uint32_t capacity);
 int         cymg_guess_samples(cymg_context* ctx, uint32_t* count);
 /* `steps` Adam steps (beta 0.9/0.999, eps 1e-8) on random minibatches of

@@ -148,6 +148,7 @@ def test_training_lowers_loss_and_iterations(pkg, scenes):
     print(f"loss {first:.4g} -> {last:.4g}", flush=True)
     assert last < 0.5 * first
     m.guess_record(0)
+    assert m.guess_samples() == nrec * sc.nfib      # stopping keeps the samples
     zero, learned = [], []
     for step in range(nrec, nrec + 4):
         r0 = _step(m, pkg, sc, step, apply=False)
