@@ -16,7 +16,12 @@ N(0,1) Brownian draws.  Generation follows the reference's own placement rules:
   distance is within the binding range, attached to the nearest point of the
   partner fiber (couple_set.cc:185-235 attach semantics; an equilibrium
   binding state without running the Monte-Carlo stepping, which is out of
-  scope).
+  scope);
+* asters (Aster::build / buildSolid, aster.cc:121-300): a Solid made of a
+  massive central bead of radius r and one drag-free point per fiber at
+  distance r in a direction spread over the sphere, each fiber straight from
+  the centre through its point (minus end at the centre), held by two links
+  (Aster::setInteractions, aster.cc:53-89).
 
 Everything here is host-side NumPy; nothing in this module touches the GPU.
 """
@@ -53,6 +58,19 @@ class Scene:
     link_fiber_b: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
     link_abs_b: np.ndarray = field(default_factory=lambda: np.zeros(0))
     link_stiffness: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    # Solids, registered after the fibers (simul_solve.cc:204-207): radius of
+    # each point (0: no drag) and the initial reshape timer (solid.cc:28)
+    sol_radius: list = field(default_factory=list)
+    sol_timer: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+    # aster links in call order, after the couples (simul_solve.cc:238-239):
+    # kind 0 interLink(PE a1, PE b) (meca_inter.cc:196-230), kind 1
+    # interLink(PI (a1, a2, ca), PE b) (meca_inter.cc:240-275)
+    aster_kind: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+    aster_a1: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    aster_a2: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    aster_ca: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    aster_b: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    aster_w: np.ndarray = field(default_factory=lambda: np.zeros(0))
     seed: int = 0
     name: str = ""
 
@@ -61,8 +79,34 @@ class Scene:
         return int(self.nb_points.size)
 
     @property
-    def npts(self):
+    def nsol(self):
+        return len(self.sol_radius)
+
+    @property
+    def sol_nb_points(self):
+        return np.array([r.size for r in self.sol_radius], dtype=np.uint32)
+
+    @property
+    def fiber_points_count(self):
         return int(self.nb_points.sum())
+
+    @property
+    def npts(self):
+        return int(self.nb_points.sum()) + int(sum(r.size for r in self.sol_radius))
+
+    @property
+    def sol_index(self):
+        """matIndex of each Solid (after every fiber point)."""
+        n = self.sol_nb_points.astype(np.int64)
+        return (self.fiber_points_count + np.concatenate([[0], np.cumsum(n)[:-1]])).astype(np.uint32)
+
+    def solid_points(self, k):
+        i = int(self.sol_index[k])
+        return self.points[self.dim * i:self.dim * (i + self.sol_radius[k].size)]
+
+    @property
+    def naster_links(self):
+        return int(self.aster_kind.size)
 
     @property
     def size(self):
@@ -338,6 +382,101 @@ def _bind_crosslinkers_fast(rng, dim, nbp, cut, points, nlinks, binding_range):
         got += ok.size
     return (np.concatenate(out[0]).astype(np.uint32), np.concatenate(out[1]),
             np.concatenate(out[2]).astype(np.uint32), np.concatenate(out[3]))
+
+
+def _sphere_directions(n, dim, rng=None):
+    """n directions spread over the circle (2D, equal angles as aster.cc:273-282)
+    or the sphere (3D, a Fibonacci lattice standing in for PointsOnSphere),
+    randomly rotated when rng is given."""
+    if dim == 2:
+        a = 2 * np.pi * np.arange(n) / n + (rng.uniform(0, 2 * np.pi) if rng is not None else 0.0)
+        return np.stack([np.cos(a), np.sin(a)], axis=1)
+    i = np.arange(n)
+    z = 1 - (2 * i + 1) / n
+    r = np.sqrt(1 - z * z)
+    phi = i * np.pi * (3 - np.sqrt(5))
+    d = np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+    if rng is not None:
+        q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        d = d @ q.T
+    return d
+
+
+def asters(dim=3, seed=3, nasters=10, fibers_per_aster=200, lmin=5.0, lmax=10.0, core=0.5, spread=4.0,
+           radius=12.0, nmotors=50000, binding_range=0.05, stiffness=(1000.0, 500.0), motor_stiffness=100.0,
+           name=None):
+    """Asters: each a Solid (central bead of radius `core` + one drag-free point
+    per fiber at distance `core`) with `fibers_per_aster` fibers of lengths
+    uniform in [lmin, lmax] (segmentation 0.5, rigidity 20) starting at the
+    centre; aster centres uniform in a ball of radius `spread`; fibers confined
+    in a sphere of radius `radius` (stiffness 200); `nmotors` couples
+    (stiffness `motor_stiffness`) bound between fibers within `binding_range`
+    (motor stepping frozen); aster stiffness (clamp1 link, clamp2 link)."""
+    rng = np.random.default_rng(seed)
+    centers = _random_in_ball(rng, nasters, dim, spread)
+    nbp_l, cut_l, fpts, sol_r, sol_p, ends = [], [], [], [], [], []
+    for a in range(nasters):
+        dirs = _sphere_directions(fibers_per_aster, dim, rng)
+        c = centers[a]
+        sol_r.append(np.concatenate([[core], np.zeros(fibers_per_aster)]))
+        sol_p.append(np.concatenate([c[None, :], c[None, :] + core * dirs]).ravel())
+        lengths = rng.uniform(lmin, lmax, fibers_per_aster)
+        for i in range(fibers_per_aster):
+            n = best_nb_points(lengths[i] / 0.5)
+            cut = lengths[i] / (n - 1)
+            nbp_l.append(n)
+            cut_l.append(cut)
+            fpts.append((c[None, :] + np.arange(n)[:, None] * (cut * dirs[i])[None, :]).ravel())
+    nbp = np.array(nbp_l, dtype=np.uint32)
+    cut = np.array(cut_l)
+    fiber_pts = np.concatenate(fpts)
+    nfib = nbp.size
+    sc = Scene(dim=dim, time_step=0.005, kT=0.0042, viscosity=0.02, hydro_radius0=0.0125,
+               hydro_radius1=10.0, nb_points=nbp, segmentation=cut, rigidity=np.full(nfib, 20.0),
+               points=np.concatenate([fiber_pts] + sol_p), confine_radius=radius, confine_stiffness=200.0,
+               seed=seed, name=name or f"asters-{dim}d-{nasters}x{fibers_per_aster}-motors-{nmotors}")
+    sc.sol_radius = sol_r
+    sc.sol_timer = rng.integers(0, 7, nasters)
+    if nmotors:
+        fa, aa, fb, ab = _bind_crosslinkers_fast(rng, dim, nbp, cut, fiber_pts, nmotors, binding_range)
+        sc.link_fiber_a, sc.link_abs_a, sc.link_fiber_b, sc.link_abs_b = fa, aa, fb, ab
+        sc.link_stiffness = np.full(fa.size, motor_stiffness)
+    # Aster::setInteractions: per fiber, centre -- minus end, then the fiber at
+    # abscissa `core` -- its Solid point
+    fidx = sc.mat_index
+    sidx = sc.sol_index
+    kind, a1, a2, ca, b, w = [], [], [], [], [], []
+    fib = np.arange(nfib)
+    m1, m2, co = sc.interpolate(fib, np.full(nfib, core))
+    for a in range(nasters):
+        for i in range(fibers_per_aster):
+            f = a * fibers_per_aster + i
+            kind += [0, 1]
+            a1 += [sidx[a], m1[f]]
+            a2 += [sidx[a], m2[f]]
+            ca += [0.0, co[f]]
+            b += [fidx[f], sidx[a] + 1 + i]
+            w += [stiffness[0], stiffness[1]]
+    sc.aster_kind = np.array(kind, np.int64)
+    sc.aster_a1 = np.array(a1, np.uint32)
+    sc.aster_a2 = np.array(a2, np.uint32)
+    sc.aster_ca = np.array(ca)
+    sc.aster_b = np.array(b, np.uint32)
+    sc.aster_w = np.array(w)
+    return sc
+
+
+def config3(seed=3):
+    """10 asters x 200 fibers (lengths 5-10 um), 50000 motors (stiffness 100),
+    aster stiffness (1000, 500), in a sphere of radius 12."""
+    return asters(dim=3, seed=seed)
+
+
+def mini_asters(dim=3, seed=11, nasters=2, fibers_per_aster=6, nmotors=12):
+    """Small aster scene for operator-level parity checks."""
+    return asters(dim=dim, seed=seed, nasters=nasters, fibers_per_aster=fibers_per_aster, lmin=1.5, lmax=3.0,
+                  spread=1.0, radius=2.5, nmotors=nmotors, binding_range=0.3,
+                  name=f"mini-asters-{dim}d-{nasters}x{fibers_per_aster}")
 
 
 @dataclass

@@ -13,8 +13,9 @@
  * dgetri/dtrtri, dpttrf, dptts2, dgtsv) and pinned in tests against scipy's
  * LAPACK.
  *
- * Only RigidFiber Mecables are modelled (the Mecable kind of the fiber
- * configurations).  Brownian draws are an input (N(0,1) per degree of freedom,
+ * Two Mecable kinds are modelled: RigidFiber and Solid (the aster cores of the
+ * aster configuration), registered fibers first as Simul::setInteractions
+ * does (simul_solve.cc:204-207).  Brownian draws are an input (N(0,1) per degree of freedom,
  * in object order), so the SFMT stream of the reference is not restated.
  */
 #include "meca_oracle.h"
@@ -423,6 +424,16 @@ typedef struct {
     double*  blk;           /* preconditioner block (mecable.cc:15-34) */
     uint32_t blk_alloc;
     int      use_blk;
+    int      kind;          /* 0 RigidFiber, 1 Solid */
+    /* Solid (solid.h): soRadius, soShape (centred reference), soShapeSqr,
+       soReshapeTimer, soDrag, soDragRot, soMom2D, soCenter, soMom */
+    double*  radius;
+    double*  shape;
+    double   shape_sqr;
+    int      timer;
+    double   drag, drag_rot, mom2d;
+    double   center[3];
+    double   mom[9];
 } orc_fiber;
 
 struct orc_meca {
@@ -464,7 +475,11 @@ orc_meca* orc_create(int dim)
 void orc_destroy(orc_meca* m)
 {
     if (!m) return;
-    for (int f = 0; f < m->fib_alloc; ++f) free(m->fib[f].blk);
+    for (int f = 0; f < m->fib_alloc; ++f) {
+        free(m->fib[f].blk);
+        free(m->fib[f].radius);
+        free(m->fib[f].shape);
+    }
     free(m->fib);
     free(m->vPTS); free(m->vSOL); free(m->vBAS); free(m->vRHS); free(m->vFOR); free(m->vTMP);
     free(m->pos); free(m->diff); free(m->jd); free(m->je); free(m->lag); free(m->ltmp); free(m->pdiff);
@@ -487,16 +502,22 @@ void orc_clear(orc_meca* m)
 }
 
 /* Meca::add (meca.cc:67-79): matIndex is the running prefix sum */
-int orc_add_fiber(orc_meca* m, uint32_t np, double segmentation, double rigidity)
+static orc_fiber* new_mecable(orc_meca* m)
 {
-    if (np < 2) return -1;
     if (m->nfib == m->fib_alloc) {
         int na = m->fib_alloc ? 2 * m->fib_alloc : 64;
         m->fib = (orc_fiber*)realloc(m->fib, sizeof(orc_fiber) * na);
         memset(m->fib + m->fib_alloc, 0, sizeof(orc_fiber) * (na - m->fib_alloc));
         m->fib_alloc = na;
     }
-    orc_fiber* f = &m->fib[m->nfib];
+    return &m->fib[m->nfib];
+}
+
+int orc_add_fiber(orc_meca* m, uint32_t np, double segmentation, double rigidity)
+{
+    if (np < 2) return -1;
+    orc_fiber* f = new_mecable(m);
+    f->kind = 0;
     f->np = np;
     f->idx = m->npts;
     f->seg = m->nseg;
@@ -504,6 +525,47 @@ int orc_add_fiber(orc_meca* m, uint32_t np, double segmentation, double rigidity
     f->rigidity = rigidity;
     m->npts += np;
     m->nseg += np - 1;
+    if (m->largest < np) m->largest = np;
+    return m->nfib++;
+}
+
+/* A Solid with the given per-point radii (Solid::addSphere, solid.cc:352-362;
+   radius 0: a point without drag), its reference shape fixed from `points`
+   (Solid::fixShape, solid.cc:370-400, with PointSet::calculateMomentum,
+   point_set.cc:288-317) and the initial soReshapeTimer (solid.cc:28 draws it
+   in [0, 7); the caller gives it). */
+int orc_add_solid(orc_meca* m, uint32_t np, const double* radius, const double* points, int timer)
+{
+    const int D = m->dim;
+    if (np < 1) return -1;
+    orc_fiber* f = new_mecable(m);
+    f->kind = 1;
+    f->np = np;
+    f->idx = m->npts;
+    f->seg = m->nseg;
+    f->cut = 0.0;
+    f->rigidity = 0.0;
+    f->radius = (double*)realloc(f->radius, sizeof(double) * np);
+    f->shape = (double*)realloc(f->shape, sizeof(double) * D * np);
+    memcpy(f->radius, radius, sizeof(double) * np);
+    double cc[3] = { 0, 0, 0 }, pp[3] = { 0, 0, 0 };
+    for (int d = 0; d < D; ++d) {
+        cc[d] = points[d];
+        pp[d] = points[d] * points[d];
+    }
+    for (uint32_t p = 1; p < np; ++p)
+        for (int d = 0; d < D; ++d) {
+            cc[d] += points[D * p + d];
+            pp[d] += points[D * p + d] * points[D * p + d];
+        }
+    const double N = (double)np;
+    for (int d = 0; d < D; ++d) cc[d] *= 1.0 / N;
+    f->shape_sqr = 0.0;
+    for (int d = 0; d < D; ++d) f->shape_sqr += pp[d] - N * cc[d] * cc[d];
+    for (uint32_t p = 0; p < np; ++p)
+        for (int d = 0; d < D; ++d) f->shape[D * p + d] = points[D * p + d] - cc[d];
+    f->timer = timer;
+    m->npts += np;
     if (m->largest < np) m->largest = np;
     return m->nfib++;
 }
@@ -586,6 +648,96 @@ static int fiber_make_projection(orc_meca* m, orc_fiber* f)
     return orc_dpttrf((int)S, d, e);
 }
 
+/* LAPACK dpotf2('U') on a 3x3 column-major matrix: A = U'U, U in the upper
+   triangle; returns j+1 if the leading minor j+1 is not positive */
+static int potf2_upper3(double* A)
+{
+    for (int j = 0; j < 3; ++j) {
+        double ajj = A[j + 3 * j];
+        for (int k = 0; k < j; ++k) ajj -= A[k + 3 * j] * A[k + 3 * j];
+        if (!(ajj > 0.0)) return j + 1;
+        ajj = sqrt(ajj);
+        A[j + 3 * j] = ajj;
+        for (int i = j + 1; i < 3; ++i) {
+            double v = A[j + 3 * i];
+            for (int k = 0; k < j; ++k) v -= A[k + 3 * j] * A[k + 3 * i];
+            A[j + 3 * i] = v / ajj;
+        }
+    }
+    return 0;
+}
+
+/* LAPACK dpotrs('U') with one right-hand side: U'U x = b */
+static void potrs_upper3(const double* U, double* b)
+{
+    for (int i = 0; i < 3; ++i) {
+        double v = b[i];
+        for (int k = 0; k < i; ++k) v -= U[k + 3 * i] * b[k];
+        b[i] = v / U[i + 3 * i];
+    }
+    for (int i = 2; i >= 0; --i) {
+        double v = b[i];
+        for (int k = i + 1; k < 3; ++k) v -= U[i + 3 * k] * b[k];
+        b[i] = v / U[i + 3 * i];
+    }
+}
+
+/* Solid::prepareMecable (solid.cc:707-711): setDragCoefficient (665-704) then
+   makeProjection (2D: 757-760, 3D: 789-847); soCenter = centroid() (181-200),
+   the radius-weighted centre of the points */
+static int solid_prepare(orc_meca* m, orc_fiber* f)
+{
+    const int D = m->dim;
+    const double* P = m->pos + D * f->idx;
+    const double eta6 = 6 * M_PI * m->prm.viscosity;
+    double drag = 0.0, rot = 0.0, roti = 0.0, cen[3] = { 0, 0, 0 };
+    for (uint32_t p = 0; p < f->np; ++p) {
+        const double r = f->radius[p];
+        if (r > 0) {
+            drag += r;
+            rot += r * r * r;
+            for (int d = 0; d < D; ++d) cen[d] += r * P[D * p + d];
+            if (D == 2) roti += r * (P[D * p] * P[D * p] + P[D * p + 1] * P[D * p + 1]);
+        }
+    }
+    if (drag < REAL_EPSILON) return -1;                /* "zero drag in Solid" */
+    for (int d = 0; d < D; ++d) cen[d] /= drag;
+    f->drag = drag * eta6;
+    f->drag_rot = rot * (8 * M_PI * m->prm.viscosity);
+    for (int d = 0; d < 3; ++d) f->center[d] = d < D ? cen[d] : 0.0;
+    if (D == 2) {
+        f->mom2d = f->drag_rot + eta6 * roti - f->drag * (cen[0] * cen[0] + cen[1] * cen[1]);
+        if (f->mom2d < REAL_EPSILON) return -1;         /* "zero rotational drag in Solid" */
+        return 0;
+    }
+    double m0 = 0, m3 = 0, m6 = 0, m4 = 0, m7 = 0, m8 = 0;
+    for (uint32_t p = 0; p < f->np; ++p) {
+        const double r = f->radius[p];
+        if (r > 0) {
+            const double* x = P + 3 * p;
+            const double px = r * x[0], py = r * x[1], pz = r * x[2];
+            m0 += px * x[0];
+            m3 += px * x[1];
+            m6 += px * x[2];
+            m4 += py * x[1];
+            m7 += py * x[2];
+            m8 += pz * x[2];
+        }
+    }
+    m0 *= eta6; m3 *= eta6; m6 *= eta6; m4 *= eta6; m7 *= eta6; m8 *= eta6;
+    const double* c = f->center;
+    const double diag = m0 + m4 + m8 + f->drag_rot - f->drag * (c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    double* M = f->mom;
+    memset(M, 0, sizeof(double) * 9);
+    M[0 + 3 * 0] = diag - m0 + f->drag * c[0] * c[0];
+    M[0 + 3 * 1] = -m3 + f->drag * c[0] * c[1];
+    M[0 + 3 * 2] = -m6 + f->drag * c[0] * c[2];
+    M[1 + 3 * 1] = diag - m4 + f->drag * c[1] * c[1];
+    M[1 + 3 * 2] = -m7 + f->drag * c[1] * c[2];
+    M[2 + 3 * 2] = diag - m8 + f->drag * c[2] * c[2];
+    return potf2_upper3(M) ? -1 : 0;                   /* "failed to factorize Solid momentum matrix" */
+}
+
 /* Meca::prepare (meca.cc:488-537) with Fiber::prepareMecable (fiber.cc:450-460) */
 int orc_prepare(orc_meca* m, const double* points)
 {
@@ -602,6 +754,10 @@ int orc_prepare(orc_meca* m, const double* points)
     memcpy(m->vPTS, m->pos, sizeof(double) * D * m->npts);
     for (int i = 0; i < m->nfib; ++i) {
         orc_fiber* f = &m->fib[i];
+        if (f->kind == 1) {
+            if (solid_prepare(m, f)) return -3;        /* zero drag, or momentum not factorisable */
+            continue;
+        }
         f->mobility = fiber_mobility(m, f);
         if (fiber_make_projection(m, f)) return -2;   /* "could not build Fiber projection" */
         f->rf_rigidity = f->rigidity / (f->cut * f->cut * f->cut);
@@ -792,6 +948,7 @@ void orc_confine_sphere(orc_meca* m, double radius, double stiffness)
     const double zero[3] = { 0, 0, 0 };
     for (int i = 0; i < m->nfib; ++i) {
         const orc_fiber* f = &m->fib[i];
+        if (f->kind != 0) continue;
         for (uint32_t p = 0; p < f->np; ++p) {
             const double* x = m->pos + D * (f->idx + p);
             double r2 = 0.0;
@@ -831,10 +988,61 @@ static void fiber_project(const orc_meca* m, const orc_fiber* f, const double* X
     }
 }
 
+/* Solid::setSpeedsFromForces (2D: solid.cc:763-782, 3D: 858-888): the
+   total force T and torque about the origin R, reduced to rigid motion */
+static void solid_speeds(const orc_meca* m, const orc_fiber* f, const double* X, double* Y, double sc)
+{
+    const double* P = m->vPTS + m->dim * f->idx;   /* psPos: the points put into vPTS by prepare */
+    const double* c = f->center;
+    if (m->dim == 2) {
+        double TX = 0, TY = 0, R = 0;
+        for (uint32_t p = 0; p < f->np; ++p) {
+            TX += X[2 * p];
+            TY += X[2 * p + 1];
+            R += P[2 * p] * X[2 * p + 1] - P[2 * p + 1] * X[2 * p];
+        }
+        R = (sc / f->mom2d) * (R + (TX * c[1] - TY * c[0]));
+        const double Tx = (sc / f->drag) * TX + c[1] * R;
+        const double Ty = (sc / f->drag) * TY - c[0] * R;
+        for (uint32_t p = 0; p < f->np; ++p) {
+            Y[2 * p] = Tx - R * P[2 * p + 1];
+            Y[2 * p + 1] = Ty + R * P[2 * p];
+        }
+        return;
+    }
+    double TX = 0, TY = 0, TZ = 0, RX = 0, RY = 0, RZ = 0;
+    for (uint32_t p = 0; p < f->np; ++p) {
+        const double* x = X + 3 * p;
+        const double* q = P + 3 * p;
+        TX += x[0];
+        TY += x[1];
+        TZ += x[2];
+        RX += q[1] * x[2] - q[2] * x[1];
+        RY += q[2] * x[0] - q[0] * x[2];
+        RZ += q[0] * x[1] - q[1] * x[0];
+    }
+    double R[3] = { sc * (RX + (TY * c[2] - TZ * c[1])), sc * (RY + (TZ * c[0] - TX * c[2])),
+                    sc * (RZ + (TX * c[1] - TY * c[0])) };
+    potrs_upper3(f->mom, R);
+    const double s = sc / f->drag;
+    const double T[3] = { s * TX + (c[1] * R[2] - c[2] * R[1]), s * TY + (c[2] * R[0] - c[0] * R[2]),
+                          s * TZ + (c[0] * R[1] - c[1] * R[0]) };
+    for (uint32_t p = 0; p < f->np; ++p) {
+        const double* q = P + 3 * p;
+        Y[3 * p] = T[0] + R[1] * q[2] - R[2] * q[1];
+        Y[3 * p + 1] = T[1] + R[2] * q[0] - R[0] * q[2];
+        Y[3 * p + 2] = T[2] + R[0] * q[1] - R[1] * q[0];
+    }
+}
+
 /* RigidFiber::setSpeedsFromForces (rigid_fiber.cc:124-132) */
 static void fiber_speeds(const orc_meca* m, const orc_fiber* f, const double* X, double* Y,
                          double sc, int rhs)
 {
+    if (f->kind == 1) {
+        solid_speeds(m, f, X, Y, sc);
+        return;
+    }
     double* tmp = (rhs ? m->lag : m->ltmp) + f->seg;
     fiber_project(m, f, X, Y, sc * f->mobility, tmp);
 }
@@ -843,7 +1051,7 @@ static void fiber_speeds(const orc_meca* m, const orc_fiber* f, const double* X,
 static void fiber_add_rigidity(const orc_meca* m, const orc_fiber* f, const double* X, double* Y)
 {
     const int D = m->dim;
-    if (f->np <= 2) return;
+    if (f->kind != 0 || f->np <= 2) return;
     const uint32_t nbs = D * (f->np - 2);
     const double k = f->rf_rigidity;
     for (uint32_t j = 0; j < nbs; ++j) {
@@ -859,7 +1067,7 @@ static void fiber_add_rigidity(const orc_meca* m, const orc_fiber* f, const doub
 static void fiber_rigidity_mat_up(const orc_fiber* f, double* M, int ps)
 {
     int sz = (int)f->np;
-    if (sz < 3) return;
+    if (f->kind != 0 || sz < 3) return;
     const double r = f->rf_rigidity;
 #define UP(i, j) M[((i) < (j) ? (i) + ps * (j) : (j) + ps * (i))]
     int s = 0, e = sz;
@@ -883,6 +1091,7 @@ static void fiber_rigidity_mat_up(const orc_fiber* f, double* M, int ps)
    multipliers are dropped */
 static void fiber_make_projection_diff(orc_meca* m, const orc_fiber* f)
 {
+    if (f->kind != 0) return;
     const double sc = 1.0 / f->cut;
     const uint32_t S = f->np - 1;
     for (uint32_t s = 0; s < S; ++s) {
@@ -895,6 +1104,7 @@ static void fiber_make_projection_diff(orc_meca* m, const orc_fiber* f)
 static void fiber_add_projection_diff(const orc_meca* m, const orc_fiber* f, const double* X, double* Y)
 {
     const int D = m->dim;
+    if (f->kind != 0) return;
     const uint32_t S = f->np - 1;
     const double* c = m->pdiff + f->seg;
     for (uint32_t s = 0; s < S; ++s) {
@@ -1182,11 +1392,12 @@ int orc_setup(orc_meca* m, const double* gauss)
     const double sc = m->prm.kT / m->time_step;
     for (int i = 0; i < m->nfib; ++i) {
         const orc_fiber* f = &m->fib[i];
-        const double b = sqrt(2 * sc / f->mobility);
+        /* Solid::addBrownianForces (solid.cc:714-725) returns b / soDrag */
+        const double b = f->kind == 1 ? sqrt(2 * sc * f->drag / f->np) : sqrt(2 * sc / f->mobility);
         const uint32_t o = D * f->idx;
         if (gauss)
             for (uint32_t j = 0; j < D * f->np; ++j) m->vFOR[o + j] += b * gauss[o + j];
-        double th = f->mobility * b;
+        double th = f->kind == 1 ? b / f->drag : f->mobility * b;
         if (th < noise) noise = th;
     }
     m->noise_level = noise * m->time_step;
@@ -1370,6 +1581,136 @@ int orc_reshape(int D, uint32_t ns, const double* src, double* dst, double cut)
     return 0;
 }
 
+/* Jacobi eigenvalue iteration on a symmetric 4x4 (column-major, full):
+   eigenvector of the largest eigenvalue, unit 2-norm (the result LAPACK
+   dsyevx('V','I') returns for il = iu = 4, up to its sign) */
+static void sym4_top_eigenvector(const double* A0, double* v)
+{
+    double A[16], V[16];
+    memcpy(A, A0, sizeof(A));
+    for (int i = 0; i < 16; ++i) V[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        double off = 0.0;
+        for (int q = 1; q < 4; ++q)
+            for (int p = 0; p < q; ++p) off += A[p + 4 * q] * A[p + 4 * q];
+        if (off == 0.0) break;
+        for (int p = 0; p < 3; ++p)
+            for (int q = p + 1; q < 4; ++q) {
+                const double apq = A[p + 4 * q];
+                if (apq == 0.0) continue;
+                const double th = (A[q + 4 * q] - A[p + 4 * p]) / (2 * apq);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
+                const double c = 1 / sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 4; ++k) {      /* A <- A J */
+                    const double akp = A[k + 4 * p], akq = A[k + 4 * q];
+                    A[k + 4 * p] = c * akp - s * akq;
+                    A[k + 4 * q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 4; ++k) {      /* A <- J' A */
+                    const double apk = A[p + 4 * k], aqk = A[q + 4 * k];
+                    A[p + 4 * k] = c * apk - s * aqk;
+                    A[q + 4 * k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 4; ++k) {      /* V <- V J */
+                    const double vkp = V[k + 4 * p], vkq = V[k + 4 * q];
+                    V[k + 4 * p] = c * vkp - s * vkq;
+                    V[k + 4 * q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    int best = 0;
+    for (int i = 1; i < 4; ++i)
+        if (A[i + 4 * i] > A[best + 4 * best]) best = i;
+    for (int k = 0; k < 4; ++k) v[k] = V[k + 4 * best];
+}
+
+/* Solid::getPoints (solid.cc:635-650): copy, then reshape() every 8th call
+   (solid.cc:547-620 in 3D after Horn 1987; 503-540 in 2D), rescale()
+   (solid.cc:434-467) otherwise */
+static int solid_get_points(orc_meca* m, orc_fiber* f)
+{
+    const int D = m->dim;
+    double* P = m->pos + D * f->idx;
+    memcpy(P, m->vPTS + D * f->idx, sizeof(double) * D * f->np);
+    if (f->np < 2) return 0;
+    const double N = (double)f->np;
+    double cc[3] = { 0, 0, 0 }, pp[3] = { 0, 0, 0 };
+    for (int d = 0; d < D; ++d) {
+        cc[d] = P[d];
+        pp[d] = P[d] * P[d];
+    }
+    for (uint32_t p = 1; p < f->np; ++p)
+        for (int d = 0; d < D; ++d) {
+            cc[d] += P[D * p + d];
+            pp[d] += P[D * p + d] * P[D * p + d];
+        }
+    for (int d = 0; d < D; ++d) cc[d] *= 1.0 / N;
+    if (++f->timer > 7) {
+        f->timer = 0;
+        const double* Sh = f->shape;
+        if (D == 2) {
+            double a = 0, b = 0;
+            for (uint32_t p = 0; p < f->np; ++p) {
+                a += P[2 * p] * Sh[2 * p] + P[2 * p + 1] * Sh[2 * p + 1];
+                b += Sh[2 * p] * P[2 * p + 1] - Sh[2 * p + 1] * P[2 * p];
+            }
+            const double n = sqrt(a * a + b * b);
+            double c = 1, s = 0;
+            if (n > REAL_EPSILON) { c = a / n; s = b / n; }
+            for (uint32_t p = 0; p < f->np; ++p) {
+                P[2 * p] = c * Sh[2 * p] - s * Sh[2 * p + 1] + cc[0];
+                P[2 * p + 1] = s * Sh[2 * p] + c * Sh[2 * p + 1] + cc[1];
+            }
+            return 0;
+        }
+        double S[9] = { 0 };
+        for (uint32_t p = 0; p < f->np; ++p)
+            for (int dd = 0; dd < 3; ++dd)
+                for (int ee = 0; ee < 3; ++ee) S[dd + 3 * ee] += Sh[3 * p + dd] * P[3 * p + ee];
+        double mx = fabs(S[0]);
+        if (fabs(S[4]) > mx) mx = fabs(S[4]);
+        if (fabs(S[8]) > mx) mx = fabs(S[8]);
+        const double sc = 1.0 / mx;
+        double Nm[16];
+        Nm[0 + 4 * 0] = sc * (S[0] + S[4] + S[8]);
+        Nm[0 + 4 * 1] = sc * (S[1 + 3 * 2] - S[2 + 3 * 1]);
+        Nm[0 + 4 * 2] = sc * (S[2 + 3 * 0] - S[0 + 3 * 2]);
+        Nm[0 + 4 * 3] = sc * (S[0 + 3 * 1] - S[1 + 3 * 0]);
+        Nm[1 + 4 * 1] = sc * (S[0] - S[4] - S[8]);
+        Nm[1 + 4 * 2] = sc * (S[0 + 3 * 1] + S[1 + 3 * 0]);
+        Nm[1 + 4 * 3] = sc * (S[2 + 3 * 0] + S[0 + 3 * 2]);
+        Nm[2 + 4 * 2] = sc * (S[4] - S[0] - S[8]);
+        Nm[2 + 4 * 3] = sc * (S[1 + 3 * 2] + S[2 + 3 * 1]);
+        Nm[3 + 4 * 3] = sc * (S[8] - S[4] - S[0]);
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < i; ++j) Nm[i + 4 * j] = Nm[j + 4 * i];
+        double q[4];
+        sym4_top_eigenvector(Nm, q);
+        /* Quaternion::setMatrix3 (quaternion.h:376-400) */
+        const double x2 = q[1] + q[1], y2 = q[2] + q[2], z2 = q[3] + q[3];
+        const double rx = q[0] * x2, ry = q[0] * y2, rz = q[0] * z2;
+        const double xx = q[1] * x2, xy = q[1] * y2, xz = q[1] * z2;
+        const double yy = q[2] * y2, yz = q[2] * z2, zz = q[3] * z2;
+        const double Rm[9] = { 1.0 - (yy + zz), xy + rz, xz - ry,
+                               xy - rz, 1.0 - (xx + zz), yz + rx,
+                               xz + ry, yz - rx, 1.0 - (xx + yy) };
+        for (uint32_t p = 0; p < f->np; ++p) {
+            const double* h = Sh + 3 * p;
+            P[3 * p] = cc[0] + Rm[0] * h[0] + Rm[3] * h[1] + Rm[6] * h[2];
+            P[3 * p + 1] = cc[1] + Rm[1] * h[0] + Rm[4] * h[1] + Rm[7] * h[2];
+            P[3 * p + 2] = cc[2] + Rm[2] * h[0] + Rm[5] * h[1] + Rm[8] * h[2];
+        }
+        return 0;
+    }
+    double sz = 0.0;
+    for (int d = 0; d < D; ++d) sz += pp[d] - N * cc[d] * cc[d];
+    if (sz <= 0) return -1;
+    const double scale = sqrt(f->shape_sqr / sz);
+    for (uint32_t p = 0; p < f->np; ++p)
+        for (int d = 0; d < D; ++d) P[D * p + d] = scale * (P[D * p + d] - cc[d]) + cc[d];
+    return 0;
+}
+
 int orc_apply(orc_meca* m)
 {
     const int D = m->dim;
@@ -1377,7 +1718,11 @@ int orc_apply(orc_meca* m)
     axpy(n, 1.0, m->vSOL, m->vPTS);                 /* meca.cc:802 */
     orc_forces_linear(m, m->vPTS, m->vFOR, 0);      /* meca.cc:826 */
     for (int i = 0; i < m->nfib; ++i) {             /* meca.cc:830-835 */
-        const orc_fiber* f = &m->fib[i];
+        orc_fiber* f = &m->fib[i];
+        if (f->kind == 1) {
+            if (solid_get_points(m, f)) return -4;   /* "Cannot rescale the Solid" */
+            continue;
+        }
         orc_reshape(D, f->np - 1, m->vPTS + D * f->idx, m->pos + D * f->idx, f->cut);
     }
     return 0;
@@ -1404,6 +1749,7 @@ void orc_compute_forces(orc_meca* m)
     orc_forces_linear(m, m->vPTS, m->vFOR, 0);
     for (int i = 0; i < m->nfib; ++i) {
         const orc_fiber* f = &m->fib[i];
+        if (f->kind != 0) continue;
         const uint32_t S = f->np - 1;
         const double* dif = m->diff + D * f->seg;
         const double* F = m->vFOR + D * f->idx;
@@ -1445,6 +1791,19 @@ void orc_get_tensions(const orc_meca* m, double* out)
 }
 
 double orc_get_mobility(const orc_meca* m, int i) { return m->fib[i].mobility; }
+
+/* soDrag, soDragRot, soMom2D, soCenter[3], soMom (upper Cholesky factor, 9) */
+void orc_get_solid(const orc_meca* m, int i, double* out)
+{
+    const orc_fiber* f = &m->fib[i];
+    out[0] = f->drag;
+    out[1] = f->drag_rot;
+    out[2] = f->mom2d;
+    for (int d = 0; d < 3; ++d) out[3 + d] = f->center[d];
+    for (int k = 0; k < 9; ++k) out[6 + k] = f->mom[k];
+}
+
+int orc_solid_timer(const orc_meca* m, int i) { return m->fib[i].timer; }
 
 void orc_get_jjt(const orc_meca* m, int i, double* diag, double* offd)
 {

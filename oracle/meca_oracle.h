@@ -54,6 +54,8 @@ void      orc_set_params(orc_meca*, const orc_params*);
 /* Meca::clear / Meca::add (meca.cc:55-79) for a RigidFiber Mecable */
 void      orc_clear(orc_meca*);
 int       orc_add_fiber(orc_meca*, uint32_t nb_points, double segmentation, double rigidity);
+int       orc_add_solid(orc_meca*, uint32_t nb_points, const double* radius, const double* points,
+                        int reshape_timer);   /* Solid after fixShape (solid.cc:352-400) */
 uint32_t  orc_nb_points(const orc_meca*);
 uint32_t  orc_mat_index(const orc_meca*, int mecable);
 
@@ -109,6 +111,8 @@ void      orc_get_vector(const orc_meca*, int which, double* out);
 void      orc_get_fiber_points(const orc_meca*, double* out);   /* points held by the Mecables */
 void      orc_get_tensions(const orc_meca*, double* out);       /* rfLag, one per segment */
 double    orc_get_mobility(const orc_meca*, int mecable);
+void      orc_get_solid(const orc_meca*, int mecable, double* out); /* drag, drag_rot, mom2d, center[3], mom[9] */
+int       orc_solid_timer(const orc_meca*, int mecable);
 void      orc_get_jjt(const orc_meca*, int mecable, double* diag, double* offd); /* factored pttrf */
 /* mB (which=0) or mC (which=1) in the Numerical-Recipes layout of prepareForMultiply
    (matsparsesym1.cc:472-523); returns nbe; arrays may be NULL to query the size */

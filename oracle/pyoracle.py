@@ -60,6 +60,9 @@ def lib():
         "orc_set_params": (None, [vp, C.POINTER(Params)]),
         "orc_clear": (None, [vp]),
         "orc_add_fiber": (C.c_int, [vp, C.c_uint32, C.c_double, C.c_double]),
+        "orc_add_solid": (C.c_int, [vp, C.c_uint32, _f64p, _f64p, C.c_int]),
+        "orc_get_solid": (None, [vp, C.c_int, _f64p]),
+        "orc_solid_timer": (C.c_int, [vp, C.c_int]),
         "orc_nb_points": (C.c_uint32, [vp]),
         "orc_mat_index": (C.c_uint32, [vp, C.c_int]),
         "orc_best_nb_points": (C.c_uint32, [C.c_double]),
@@ -131,6 +134,7 @@ class OracleMeca:
             raise ValueError("dim must be 2 or 3")
         self.nfib = 0
         self.np = []
+        self.nseg = 0
         if params is not None:
             self.set_params(**params)
 
@@ -149,6 +153,7 @@ class OracleMeca:
         self.L.orc_clear(self.h)
         self.nfib = 0
         self.np = []
+        self.nseg = 0
 
     def add_fiber(self, nb_points, segmentation, rigidity):
         r = self.L.orc_add_fiber(self.h, int(nb_points), float(segmentation), float(rigidity))
@@ -156,7 +161,29 @@ class OracleMeca:
             raise ValueError("fiber needs at least 2 points")
         self.nfib += 1
         self.np.append(int(nb_points))
+        self.nseg += int(nb_points) - 1
         return r
+
+    def add_solid(self, radius, points, reshape_timer=0):
+        """A Solid Mecable (after the fibers) with per-point radii and its
+        reference shape fixed from `points` (DIM per point)."""
+        r = np.ascontiguousarray(radius, dtype=np.float64)
+        pts = np.ascontiguousarray(points, dtype=np.float64).ravel()
+        assert pts.size == self.dim * r.size
+        i = self.L.orc_add_solid(self.h, r.size, r, pts, int(reshape_timer))
+        if i < 0:
+            raise ValueError("invalid solid")
+        self.np.append(int(r.size))
+        return i
+
+    def solid(self, i):
+        out = np.empty(15)
+        self.L.orc_get_solid(self.h, i, out)
+        return {"drag": out[0], "drag_rot": out[1], "mom2d": out[2], "center": out[3:6].copy(),
+                "mom": out[6:15].reshape(3, 3, order="F").copy()}
+
+    def solid_timer(self, i):
+        return int(self.L.orc_solid_timer(self.h, i))
 
     @property
     def nb_points(self):
@@ -173,6 +200,8 @@ class OracleMeca:
         pts = np.ascontiguousarray(points, dtype=np.float64).ravel()
         assert pts.size == self.size
         r = self.L.orc_prepare(self.h, pts)
+        if r == -3:
+            raise RuntimeError("zero drag in Solid, or Solid momentum matrix not factorisable")
         if r:
             raise RuntimeError("could not build Fiber projection")
 
@@ -228,7 +257,8 @@ class OracleMeca:
         return out
 
     def apply(self):
-        self.L.orc_apply(self.h)
+        if self.L.orc_apply(self.h):
+            raise RuntimeError("Cannot rescale the Solid, because it has collapsed!")
 
     def solve(self, tolerance, gauss=None, tol_mode=0, precondition=1, max_iter=0):
         self.setup(gauss)
@@ -303,7 +333,7 @@ class OracleMeca:
         return out
 
     def tensions(self):
-        out = np.empty(self.nb_points - self.nfib)
+        out = np.empty(self.nseg)
         self.L.orc_get_tensions(self.h, out)
         return out
 
@@ -387,8 +417,9 @@ def best_nb_points(ratio):
 
 def assemble_scene(m, sc):
     """Records the scene's interactions on an OracleMeca in the call order of
-    Simul::setInteractions (simul_solve.cc): fibers' sphere confinement, then
-    the crosslinkers (Couple::setInteractions -> Meca::interLink)."""
+    Simul::setInteractions (simul_solve.cc:215-242): fibers' sphere
+    confinement, the crosslinkers (Couple::setInteractions -> Meca::interLink),
+    then the asters (Aster::setInteractions, aster.cc:53-89)."""
     if sc.confine_radius > 0:
         m.confine_sphere(sc.confine_radius, sc.confine_stiffness)
     if sc.nlinks:
@@ -397,14 +428,27 @@ def assemble_scene(m, sc):
         for k in range(sc.nlinks):
             m.inter_link_ii(int(a1[k]), int(a2[k]), float(ca[k]), int(b1[k]), int(b2[k]),
                             float(cb[k]), float(sc.link_stiffness[k]))
+    for k in range(getattr(sc, "naster_links", 0)):
+        if sc.aster_kind[k] == 0:
+            m.inter_link_ee(int(sc.aster_a1[k]), int(sc.aster_b[k]), float(sc.aster_w[k]))
+        else:
+            m.inter_link_ie(int(sc.aster_a1[k]), int(sc.aster_a2[k]), float(sc.aster_ca[k]),
+                            int(sc.aster_b[k]), float(sc.aster_w[k]))
+
+
+def register_scene(m, sc):
+    """Meca::add of the scene's Mecables: fibers, then Solids."""
+    for f in range(sc.nfib):
+        m.add_fiber(sc.nb_points[f], sc.segmentation[f], sc.rigidity[f])
+    for k in range(getattr(sc, "nsol", 0)):
+        m.add_solid(sc.sol_radius[k], sc.solid_points(k), int(sc.sol_timer[k]))
 
 
 def from_scene(sc):
     """OracleMeca loaded with one step of a scene: registration (Meca::add),
     prepare, then the interactions."""
     m = OracleMeca(sc.dim, sc.params())
-    for f in range(sc.nfib):
-        m.add_fiber(sc.nb_points[f], sc.segmentation[f], sc.rigidity[f])
+    register_scene(m, sc)
     m.prepare(sc.points)
     assemble_scene(m, sc)
     return m
