@@ -26,7 +26,7 @@ VEC_PTS, VEC_SOL, VEC_BAS, VEC_RHS, VEC_FOR = range(5)
 
 EXPORTED = [
     "cymg_create", "cymg_destroy", "cymg_last_error", "cymg_set_params", "cymg_clear",
-    "cymg_add_fiber", "cymg_nb_points", "cymg_set_systems", "cymg_get_system_reports",
+    "cymg_add_fiber", "cymg_add_solid", "cymg_nb_points", "cymg_set_systems", "cymg_get_system_reports",
     "cymg_prepare", "cymg_add_pure_force",
     "cymg_inter_link_ee", "cymg_inter_link_ie", "cymg_inter_link_ii", "cymg_inter_long_link_ee",
     "cymg_inter_clamp_e", "cymg_inter_clamp_i", "cymg_inter_long_clamp", "cymg_inter_plane_e",
@@ -114,6 +114,7 @@ def lib():
         "cymg_set_params": (C.c_int, [vp, C.POINTER(SimParams)]),
         "cymg_clear": (C.c_int, [vp]),
         "cymg_add_fiber": (C.c_int, [vp, u32, C.c_double, C.c_double, C.POINTER(u32)]),
+        "cymg_add_solid": (C.c_int, [vp, u32, vp, vp, C.c_int, C.POINTER(u32)]),
         "cymg_nb_points": (C.c_int, [vp, C.POINTER(u32)]),
         "cymg_set_systems": (C.c_int, [vp, u32, vp]),
         "cymg_get_system_reports": (C.c_int, [vp, u32, vp]),
@@ -226,6 +227,17 @@ class Meca:
         self._chk(self.L.cymg_add_fiber(self.h, int(nb_points), float(segmentation), float(rigidity),
                                         C.byref(mi)))
         self.np.append(int(nb_points))
+        return mi.value
+
+    def add_solid(self, radius, points, reshape_timer=0):
+        """A Solid (after every fiber): per-point radii and its reference shape."""
+        r = np.ascontiguousarray(radius, dtype=np.float64)
+        pts = np.ascontiguousarray(points, dtype=np.float64).ravel()
+        assert pts.size == self.dim * r.size
+        mi = C.c_uint32()
+        self._chk(self.L.cymg_add_solid(self.h, r.size, _p(r), _p(pts), int(reshape_timer), C.byref(mi)))
+        self.np.append(int(r.size))
+        self.nsol = getattr(self, "nsol", 0) + 1
         return mi.value
 
     def set_systems(self, first_fiber):
@@ -375,7 +387,8 @@ class Meca:
         return out
 
     def tensions(self):
-        out = np.empty(self.nb_points - len(self.np))
+        nfib = len(self.np) - getattr(self, "nsol", 0)
+        out = np.empty(sum(self.np[:nfib]) - nfib)
         self._chk(self.L.cymg_get_tensions(self.h, _p(out)))
         return out
 
@@ -475,6 +488,8 @@ def load_scene(meca, scene, couples_on_device=True, upload_couples=True):
     meca.clear()
     for f in range(scene.nfib):
         meca.add_fiber(scene.nb_points[f], scene.segmentation[f], scene.rigidity[f])
+    for k in range(getattr(scene, "nsol", 0)):
+        meca.add_solid(scene.sol_radius[k], scene.solid_points(k), int(scene.sol_timer[k]))
     meca.set_params(**scene.params())
     meca.prepare(scene.points)
     assemble(meca, scene, couples_on_device, upload_couples)
@@ -496,3 +511,10 @@ def assemble(meca, scene, couples_on_device=True, upload_couples=True):
             for k in range(scene.nlinks):
                 meca.inter_link_ii(int(a1[k]), int(a2[k]), float(ca[k]), int(b1[k]), int(b2[k]),
                                    float(cb[k]), float(scene.link_stiffness[k]))
+    # Aster::setInteractions (aster.cc:53-89), after the couples (simul_solve.cc:238-239)
+    for k in range(getattr(scene, "naster_links", 0)):
+        if scene.aster_kind[k] == 0:
+            meca.inter_link_ee(int(scene.aster_a1[k]), int(scene.aster_b[k]), float(scene.aster_w[k]))
+        else:
+            meca.inter_link_ie(int(scene.aster_a1[k]), int(scene.aster_a2[k]), float(scene.aster_ca[k]),
+                               int(scene.aster_b[k]), float(scene.aster_w[k]))

@@ -70,6 +70,10 @@ void check_point(const Context& c, uint32_t a)
 
 int mecable_of(const Context& c, uint32_t a)
 {
+    if (a >= c.fib_pts && c.nsol) {
+        auto it = std::upper_bound(c.h_sol_idx.begin(), c.h_sol_idx.end(), a);
+        return (int)c.nfib + (int)(it - c.h_sol_idx.begin()) - 1;
+    }
     auto it = std::upper_bound(c.h_idx.begin(), c.h_idx.end(), a);
     return (int)(it - c.h_idx.begin()) - 1;
 }
@@ -168,6 +172,21 @@ void upload_layout(Context& c)
     CYMG_HIP(hipMemcpyAsync(c.d_fib_sys.get(), sysv.data(), F * 4, hipMemcpyHostToDevice, st));
     CYMG_HIP(hipMemcpyAsync(c.d_sys_begin.get(), sb.data(), sb.size() * 4, hipMemcpyHostToDevice, st));
     c.h_sys_begin_used = sb;
+    if (c.nsol) {
+        // one system: its per-Mecable partial sums cover the Solids after the fibers
+        const std::vector<uint32_t> sbm = { 0, c.nmec() };
+        CYMG_HIP(hipMemcpyAsync(c.d_sys_begin.get(), sbm.data(), 8, hipMemcpyHostToDevice, st));
+        const uint32_t K = c.nsol, SP = c.npts - c.fib_pts;
+        c.d_sol_np.resize(K); c.d_sol_idx.resize(K); c.d_sol_shape_sqr.resize(K); c.d_sol_timer.resize(K);
+        c.d_sol_rad.resize(SP); c.d_sol_shape.resize((size_t)c.dim * SP);
+        CYMG_HIP(hipMemcpyAsync(c.d_sol_np.get(), c.h_sol_np.data(), K * 4, hipMemcpyHostToDevice, st));
+        CYMG_HIP(hipMemcpyAsync(c.d_sol_idx.get(), c.h_sol_idx.data(), K * 4, hipMemcpyHostToDevice, st));
+        CYMG_HIP(hipMemcpyAsync(c.d_sol_shape_sqr.get(), c.h_sol_shape_sqr.data(), K * 8, hipMemcpyHostToDevice, st));
+        CYMG_HIP(hipMemcpyAsync(c.d_sol_timer.get(), c.h_sol_timer.data(), K * 4, hipMemcpyHostToDevice, st));
+        CYMG_HIP(hipMemcpyAsync(c.d_sol_rad.get(), c.h_sol_rad.data(), SP * 8, hipMemcpyHostToDevice, st));
+        CYMG_HIP(hipMemcpyAsync(c.d_sol_shape.get(), c.h_sol_shape.data(), (size_t)c.dim * SP * 8,
+                                hipMemcpyHostToDevice, st));
+    }
     std::vector<uint32_t> pf(c.npts);
     for (uint32_t f = 0; f < F; ++f)
         for (uint32_t p = 0; p < c.h_np[f]; ++p) pf[c.h_idx[f] + p] = f;
@@ -181,7 +200,7 @@ void upload_layout(Context& c)
     for (DevBuf<double>* v : { &c.d_jd, &c.d_je, &c.d_jd0, &c.d_je0, &c.d_lag, &c.d_pdiff }) v->resize(S);
     CYMG_HIP(hipMemsetAsync(c.d_lag.get(), 0, S * 8, st));
     CYMG_HIP(hipMemsetAsync(c.d_prev_sol.get(), 0, (n ? n : 1) * 8, st));
-    c.d_part.resize(P_SLOTS * (size_t)(F ? F : 1));
+    c.d_part.resize(P_SLOTS * (size_t)(c.nmec() ? c.nmec() : 1));
     c.d_scal.resize(2 * c.nsys);
     CYMG_HIP(hipMemsetAsync(c.d_scal.get(), 0, 2 * c.nsys * sizeof(SysScalars), st));
     CYMG_HIP(hipStreamSynchronize(st));   // the sources above are pageable host arrays
@@ -232,12 +251,16 @@ void do_setup(Context& c, const double* gauss, int on_device)
     c.pc_ready = false;
 }
 
+// d_status: [0] fibers whose projection failed, [1] Solids that could not be
+// prepared, [2] Solids that collapsed in the last getPoints
 void check_status(Context& c)
 {
-    int st = 0;
-    CYMG_HIP(hipMemcpyAsync(&st, c.d_status.get(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    int st[3] = { 0, 0, 0 };
+    CYMG_HIP(hipMemcpyAsync(st, c.d_status.get(), sizeof(st), hipMemcpyDeviceToHost, c.stream));
     CYMG_HIP(hipStreamSynchronize(c.stream));
-    if (st) throw Error(CYMG_ERR_PROJECTION, "could not build Fiber projection");
+    if (st[0]) throw Error(CYMG_ERR_PROJECTION, "could not build Fiber projection");
+    // InvalidParameter / ABORT_NOW of the reference (solid.cc:690-703, 843-846)
+    if (st[1]) throw Error(CYMG_ERR_INVALID, "zero drag in Solid, or failed to factorize Solid momentum matrix");
 }
 
 void do_iterate(Context& c, const cymg_solve_params* p, cymg_solve_report* rep)
@@ -388,6 +411,9 @@ int cymg_clear(cymg_context* ctx)
         c.h_np.clear(); c.h_idx.clear(); c.h_seg.clear(); c.h_cut.clear(); c.h_rig.clear();
         c.h_blkoff.assign(1, 0);
         c.nfib = c.npts = c.nseg = c.largest = 0;
+        c.nsol = c.fib_pts = 0;
+        c.h_sol_np.clear(); c.h_sol_idx.clear(); c.h_sol_rad.clear(); c.h_sol_shape.clear();
+        c.h_sol_shape_sqr.clear(); c.h_sol_timer.clear();
         c.blk_total = 0;
         c.h_sys_begin.clear();
         c.layout_dirty = true;
@@ -403,6 +429,7 @@ int cymg_add_fiber(cymg_context* ctx, uint32_t np, double segmentation, double r
             throw Error(CYMG_ERR_UNSUPPORTED, "fiber has more than CYMG_MAX_FIBER_POINTS points");
         if (!(segmentation > 0)) throw Error(CYMG_ERR_INVALID, "segmentation must be > 0");
         if (!(rigidity >= 0)) throw Error(CYMG_ERR_INVALID, "fiber:rigidity must be specified and >= 0");
+        if (c.nsol) throw Error(CYMG_ERR_INVALID, "fibers are registered before Solids (simul_solve.cc:204-207)");
         if (c.h_blkoff.empty()) c.h_blkoff.assign(1, 0);
         if (mat_index) *mat_index = c.npts;
         c.h_np.push_back(np);
@@ -414,9 +441,51 @@ int cymg_add_fiber(cymg_context* ctx, uint32_t np, double segmentation, double r
         c.blk_total += bs * bs;
         c.h_blkoff.push_back(c.blk_total);
         c.npts += np;
+        c.fib_pts = c.npts;
         c.nseg += np - 1;
         c.largest = std::max(c.largest, np);
         c.nfib += 1;
+        c.layout_dirty = true;
+        c.prepared = c.setup_done = c.pc_ready = false;
+    });
+}
+
+int cymg_add_solid(cymg_context* ctx, uint32_t np, const double* radius, const double* points, int reshape_timer,
+                   uint32_t* mat_index)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (np < 1 || !radius || !points) throw Error(CYMG_ERR_INVALID, "a Solid needs points and radii");
+        if (np > (uint32_t)CYMG_MAX_SOLID_POINTS)
+            throw Error(CYMG_ERR_UNSUPPORTED, "Solid has more than CYMG_MAX_SOLID_POINTS points");
+        if (c.h_sys_begin.size() > 2) throw Error(CYMG_ERR_UNSUPPORTED, "Solids in a batch of independent systems");
+        for (uint32_t p = 0; p < np; ++p)
+            if (!(radius[p] >= 0)) throw Error(CYMG_ERR_INVALID, "solid:sphere's radius should be >= 0");
+        const int D = c.dim;
+        if (mat_index) *mat_index = c.npts;
+        c.h_sol_np.push_back(np);
+        c.h_sol_idx.push_back(c.npts);
+        c.h_sol_rad.insert(c.h_sol_rad.end(), radius, radius + np);
+        // Solid::fixShape (solid.cc:370-400) with PointSet::calculateMomentum (point_set.cc:288-317)
+        double cc[3] = { 0, 0, 0 }, pp[3] = { 0, 0, 0 };
+        for (int d = 0; d < D; ++d) {
+            cc[d] = points[d];
+            pp[d] = points[d] * points[d];
+        }
+        for (uint32_t p = 1; p < np; ++p)
+            for (int d = 0; d < D; ++d) {
+                cc[d] += points[D * p + d];
+                pp[d] += points[D * p + d] * points[D * p + d];
+            }
+        const double N = (double)np;
+        for (int d = 0; d < D; ++d) cc[d] *= 1.0 / N;
+        double sq = 0.0;
+        for (int d = 0; d < D; ++d) sq += pp[d] - N * cc[d] * cc[d];
+        c.h_sol_shape_sqr.push_back(sq);
+        for (uint32_t p = 0; p < np; ++p)
+            for (int d = 0; d < D; ++d) c.h_sol_shape.push_back(points[D * p + d] - cc[d]);
+        c.h_sol_timer.push_back(reshape_timer);
+        c.npts += np;
+        c.nsol += 1;
         c.layout_dirty = true;
         c.prepared = c.setup_done = c.pc_ready = false;
     });
@@ -426,6 +495,7 @@ int cymg_set_systems(cymg_context* ctx, uint32_t nsys, const uint32_t* first_fib
 {
     return guarded(ctx, [&](Context& c) {
         if (nsys < 1 || nsys > 4096) throw Error(CYMG_ERR_UNSUPPORTED, "between 1 and 4096 systems");
+        if (nsys > 1 && c.nsol) throw Error(CYMG_ERR_UNSUPPORTED, "Solids in a batch of independent systems");
         std::vector<uint32_t> sb(first_fiber, first_fiber + nsys);
         sb.push_back(c.nfib);
         if (sb[0] != 0) throw Error(CYMG_ERR_INVALID, "the first system must start at fiber 0");
@@ -455,7 +525,14 @@ int cymg_nb_points(const cymg_context* ctx, uint32_t* nb)
 int cymg_prepare(cymg_context* ctx, const double* points)
 {
     return guarded(ctx, [&](Context& c) {
-        if (!c.nfib) throw Error(CYMG_ERR_INVALID, "no Mecable registered");
+        if (!c.nmec()) throw Error(CYMG_ERR_INVALID, "no Mecable registered");
+        if (c.nsol && !c.layout_dirty && c.d_status.get()) {
+            // Solid::rescale's ABORT_NOW (solid.cc:449-453) in the previous getPoints
+            int st = 0;
+            CYMG_HIP(hipMemcpyAsync(&st, c.d_status.get() + 2, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+            CYMG_HIP(hipStreamSynchronize(c.stream));
+            if (st) throw Error(CYMG_ERR_INVALID, "Cannot rescale the Solid, because it has collapsed!");
+        }
         const bool fresh = c.layout_dirty;
         if (c.layout_dirty) upload_layout(c);
         if (points) {
@@ -827,10 +904,54 @@ int cymg_get_tensions(cymg_context* ctx, double* out)
     return guarded(ctx, [&](Context& c) { d2h(c, out, c.d_lag.get(), (size_t)c.nseg * 8); });
 }
 
+// dense block of Solid k from its low-rank form: I + G A, or I - G W
+static void solid_dense_block(Context& c, uint32_t k, int inverse, double* out)
+{
+    const int D = c.dim, NG = D == 3 ? 6 : 3;
+    const uint32_t np = c.h_sol_np[k], idx = c.h_sol_idx[k];
+    const uint64_t bs = (uint64_t)D * np;
+    std::vector<double> x(bs), F(6 * bs);
+    d2h(c, x.data(), c.d_pts.get() + (size_t)D * idx, bs * 8);
+    d2h(c, F.data(), (inverse ? c.d_sol_W.get() : c.d_sol_A.get()) + 6 * (size_t)D * (idx - c.fib_pts), 6 * bs * 8);
+    const double sign = inverse ? -1.0 : 1.0;
+    for (uint64_t j = 0; j < bs; ++j)
+        for (uint64_t i = 0; i < bs; ++i) {
+            const uint64_t p = i / D;
+            const int d = (int)(i - D * p);
+            const double* q = x.data() + D * p;
+            double g[6] = { 0, 0, 0, 0, 0, 0 };
+            g[d] = 1.0;
+            if (D == 3) {
+                g[3] = d == 0 ? 0.0 : (d == 1 ? -q[2] : q[1]);
+                g[4] = d == 0 ? q[2] : (d == 1 ? 0.0 : -q[0]);
+                g[5] = d == 0 ? -q[1] : (d == 1 ? q[0] : 0.0);
+            } else {
+                g[2] = d == 0 ? -q[1] : q[0];
+            }
+            double v = 0.0;
+            for (int e = 0; e < NG; ++e) v += g[e] * F[6 * j + e];
+            out[i + bs * j] = (i == j ? 1.0 : 0.0) + sign * v;
+        }
+}
+
 int cymg_get_block(cymg_context* ctx, uint32_t m, int inverse, double* out)
 {
     return guarded(ctx, [&](Context& c) {
-        if (m >= c.nfib) throw Error(CYMG_ERR_INVALID, "no such Mecable");
+        if (m >= c.nmec()) throw Error(CYMG_ERR_INVALID, "no such Mecable");
+        if (m >= c.nfib) {
+            const uint32_t k = m - c.nfib;
+            if (inverse) {
+                if (!c.pc_ready) throw Error(CYMG_ERR_INVALID, "no preconditioner computed");
+                int use = 0;
+                d2h(c, &use, c.d_sol_useW.get() + k, sizeof(int));
+                if (!use) throw Error(CYMG_ERR_INVALID, "block not factorised (identity used)");
+            } else {
+                if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_get_block");
+                launch_solid_pc_build(c);
+            }
+            solid_dense_block(c, k, inverse, out);
+            return;
+        }
         const uint64_t bs = (uint64_t)c.dim * c.h_np[m];
         const uint64_t off = c.h_blkoff[m];
         if (inverse) {

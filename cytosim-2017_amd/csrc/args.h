@@ -3,6 +3,7 @@
 
 #include "context.h"
 #include "fiber_ops.h"
+#include "solid_ops.h"
 
 namespace cymg {
 
@@ -35,6 +36,22 @@ inline FiberArgs make_fiber_args(const Context& c)
     a.Cuse = c.mC.used ? 1 : 0;
     a.dt = c.prm.time_step;
     return a;
+}
+
+inline SolidArgs make_solid_args(const Context& c)
+{
+    SolidArgs s;
+    s.nsol = c.nsol;
+    s.p0 = c.fib_pts;
+    s.nfib = c.nfib;
+    s.np = c.d_sol_np.get();
+    s.idx = c.d_sol_idx.get();
+    s.rad = c.d_sol_rad.get();
+    s.geo = c.d_sol_geo.get();
+    s.W = c.d_sol_W.get();
+    s.useW = c.d_sol_useW.get();
+    s.pos = c.d_pts.get();
+    return s;
 }
 
 }  // namespace cymg

@@ -183,7 +183,7 @@ void build_streams(Context& c)
     size_t total[3] = { 0, 0, 0 };
     for (int w = 0; w < 3; ++w) total[w] = hs[w]->key.size();
     for (const EmitterMark& mk : c.marks) {
-        if (mk.kind == 1) { total[1] += NC * c.npts; total[2] += (size_t)D * c.npts; }
+        if (mk.kind == 1) { total[1] += NC * c.fib_pts; total[2] += (size_t)D * c.fib_pts; }
         if (mk.kind == 2) { total[0] += (size_t)10 * c.ncouples; }
     }
     for (int w = 0; w < 3; ++w) {
@@ -211,8 +211,8 @@ void build_streams(Context& c)
         if (mk.kind == 1) {
             launch_emit_confine(c, c.s_key[1].get() + dpos[1], c.s_val[1].get() + dpos[1], c.s_key[2].get() + dpos[2],
                                 c.s_val[2].get() + dpos[2], mk.a, mk.b);
-            dpos[1] += NC * c.npts;
-            dpos[2] += (size_t)D * c.npts;
+            dpos[1] += NC * c.fib_pts;
+            dpos[2] += (size_t)D * c.fib_pts;
         } else if (mk.kind == 2) {
             launch_emit_couples(c, c.s_key[0].get() + dpos[0], c.s_val[0].get() + dpos[0]);
             dpos[0] += (size_t)10 * c.ncouples;

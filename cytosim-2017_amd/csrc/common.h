@@ -131,6 +131,14 @@ void launch_tensions(Context& c);
 // launchers (precond.hip)
 void launch_precond_build(Context& c, int precision, double* block_out = nullptr);
 void launch_precond_apply(Context& c, const double* X, double* Y);
+// launchers (solid.hip)
+void launch_solid_prepare(Context& c);
+void launch_solid_setup(Context& c, const double* gauss);
+void launch_solid_matvec(Context& c, const double* X, double* Y);
+void launch_solid_forces(Context& c, const double* X, double* Y);
+void launch_solid_pc_build(Context& c);
+void launch_solid_pc_apply(Context& c, const double* X, double* Y);
+void launch_solid_points(Context& c);
 // launchers (krylov.hip)
 double krylov_time_launch(Context& c, int precond, int launches);
 void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter, double tol,

@@ -30,6 +30,16 @@ struct Context {
     uint32_t ntiles = 0;
     uint32_t tile_points = 0;          // target points per tile, 0: chosen from the layout
 
+    // Solids, registered after every fiber (simul_solve.cc:204-207)
+    uint32_t nsol = 0;
+    uint32_t fib_pts = 0;              // points of the fibers; Solid points follow
+    std::vector<uint32_t> h_sol_np, h_sol_idx;
+    std::vector<double> h_sol_rad, h_sol_shape, h_sol_shape_sqr;   // per point, DIM per point, per Solid
+    std::vector<int> h_sol_timer;
+    DevBuf<uint32_t> d_sol_np, d_sol_idx;
+    DevBuf<double> d_sol_rad, d_sol_shape, d_sol_shape_sqr, d_sol_geo, d_sol_A, d_sol_W;
+    DevBuf<int> d_sol_timer, d_sol_useW;
+
     // per-fiber device arrays
     DevBuf<uint32_t> d_np, d_idx, d_seg, d_fib_sys, d_sys_begin;
     DevBuf<uint32_t> d_pfib;           // fiber of each point
@@ -96,6 +106,7 @@ struct Context {
     int mv_count = 0, pc_count = 0;
 
     uint32_t n() const { return (uint32_t)dim * npts; }
+    uint32_t nmec() const { return nfib + nsol; }   // Mecables: the stride of per-Mecable partial sums
 };
 
 }  // namespace cymg

@@ -80,9 +80,10 @@ __global__ __launch_bounds__(kWave) void k_prepare(const uint32_t* __restrict__ 
 
 void launch_prepare(Context& c)
 {
-    if (!c.nfib) return;
     c.d_status.resize(4);
     CYMG_HIP(hipMemsetAsync(c.d_status.get(), 0, 4 * sizeof(int), c.stream));
+    launch_solid_prepare(c);
+    if (!c.nfib) return;
     const dim3 g(c.nfib), b(kWave);
     if (c.dim == 3)
         hipLaunchKernelGGL(k_prepare<3>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_seg.get(), c.d_cut.get(),
@@ -186,14 +187,15 @@ __global__ void k_emit_confine(uint32_t npts, const double* __restrict__ pos, do
 void launch_emit_confine(Context& c, uint64_t* keyC, double* valC, uint64_t* keyBas, double* valBas, double radius,
                          double stiffness)
 {
-    if (!c.npts) return;
-    const dim3 b(256), g((c.npts + 255) / 256);
+    // Fiber::setInteractions: the fibers' points only (a Solid is not confined, solid.cc:112-114)
+    if (!c.fib_pts) return;
+    const dim3 b(256), g((c.fib_pts + 255) / 256);
     if (c.dim == 3)
-        hipLaunchKernelGGL(k_emit_confine<3>, g, b, 0, c.stream, c.npts, c.d_pos.get(), radius, stiffness, keyC, valC,
-                           keyBas, valBas);
+        hipLaunchKernelGGL(k_emit_confine<3>, g, b, 0, c.stream, c.fib_pts, c.d_pos.get(), radius, stiffness, keyC,
+                           valC, keyBas, valBas);
     else
-        hipLaunchKernelGGL(k_emit_confine<2>, g, b, 0, c.stream, c.npts, c.d_pos.get(), radius, stiffness, keyC, valC,
-                           keyBas, valBas);
+        hipLaunchKernelGGL(k_emit_confine<2>, g, b, 0, c.stream, c.fib_pts, c.d_pos.get(), radius, stiffness, keyC,
+                           valC, keyBas, valBas);
     CYMG_HIP(hipGetLastError());
 }
 
@@ -310,6 +312,7 @@ __global__ __launch_bounds__(kWave) void k_setup(FiberArgs a, const double* __re
 
 void launch_setup(Context& c, const double* gauss)
 {
+    launch_solid_setup(c, gauss);
     if (!c.nfib) return;
     FiberArgs a = make_fiber_args(c);
     const dim3 g(c.nfib), b(kWave);
@@ -317,11 +320,11 @@ void launch_setup(Context& c, const double* gauss)
     if (c.dim == 3)
         hipLaunchKernelGGL(k_setup<3>, g, b, 0, c.stream, a, c.d_pts.get(), c.d_bas.get(), gauss, kT_dt,
                            c.d_for.get(), c.d_rhs.get(), c.d_lag.get(), c.d_pdiff.get(), c.d_part.get(),
-                           c.nfib);
+                           c.nmec());
     else
         hipLaunchKernelGGL(k_setup<2>, g, b, 0, c.stream, a, c.d_pts.get(), c.d_bas.get(), gauss, kT_dt,
                            c.d_for.get(), c.d_rhs.get(), c.d_lag.get(), c.d_pdiff.get(), c.d_part.get(),
-                           c.nfib);
+                           c.nmec());
     CYMG_HIP(hipGetLastError());
 }
 
@@ -342,6 +345,7 @@ __global__ __launch_bounds__(kWave) void k_matvec(FiberArgs a, const double* __r
 
 void launch_matvec(Context& c, const double* X, double* Y)
 {
+    launch_solid_matvec(c, X, Y);
     if (!c.nfib) return;
     FiberArgs a = make_fiber_args(c);
     const dim3 g(c.nfib), b(kWave);
@@ -369,6 +373,7 @@ __global__ __launch_bounds__(kWave) void k_forces(FiberArgs a, const double* __r
 
 void launch_forces(Context& c, const double* X, double* Y, bool with_rigidity)
 {
+    launch_solid_forces(c, X, Y);
     if (!c.nfib) return;
     FiberArgs a = make_fiber_args(c);
     const dim3 g(c.nfib), b(kWave);
@@ -632,11 +637,13 @@ __global__ __launch_bounds__(kWave) void k_reshape(const uint32_t* __restrict__ 
 
 void launch_apply(Context& c)
 {
-    if (!c.nfib) return;
     const uint32_t n = c.n();
+    if (!n) return;
     hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.d_sol.get(), c.d_pts.get());
     CYMG_HIP(hipGetLastError());
     launch_forces(c, c.d_pts.get(), c.d_for.get(), false);   // meca.cc:826
+    launch_solid_points(c);
+    if (!c.nfib) return;
     const dim3 g(c.nfib), b(kWave);
     if (c.dim == 3)
         hipLaunchKernelGGL(k_reshape<3>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_cut.get(), c.d_pts.get(),

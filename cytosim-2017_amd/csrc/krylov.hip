@@ -57,8 +57,10 @@ struct KArgs {
     TileArgs ta;
     SysScalars* st_in;
     SysScalars* st_out;
-    uint32_t nfib;
+    uint32_t nfib;      // Mecables: the stride of the partial sums (fibers, then Solids)
     uint32_t nsys;
+    uint32_t nfiber;    // fibers: workgroups from nfiber on handle Solid blockIdx.x - nfiber
+    SolidArgs sa;
 };
 
 // Reductions over the partials of one system (entries [fb, fe)), repeated by
@@ -192,6 +194,126 @@ __device__ __forceinline__ void apply_block(const KArgs& k, uint32_t f, int bs, 
         block_apply_mw<float, kApplyWaves>(k.B32 + k.blkoff[f], bs, k.useblk[f] != 0, xs, y, red);
 }
 
+// ---------------------------------------------------------------------------
+// The rows of Solid sk in each phase (the same vector updates as the fibers'
+// rows, with the Solid operator and its low-rank block inverse); red: LDS
+// scratch of at least 96 doubles.
+
+template <int D>
+__device__ double solid_init(const KArgs& k, uint32_t sk, const double* __restrict__ x0, double* red)
+{
+    const SolidArgs& s = k.sa;
+    const uint32_t o = D * s.idx[sk], n = D * s.np[sk];
+    if (x0) {
+        double* r = k.r;
+        const double* b = k.b;
+        solid_operator<D>(k.fa, s, sk, Plain{ x0 }, [=](uint32_t i, double y) { r[i] = b[i] + (-1.0) * y; }, red);
+        __syncthreads();
+    }
+    double rr = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const double ri = x0 ? k.r[o + i] : k.b[o + i];
+        k.r[o + i] = ri;
+        k.rt[o + i] = ri;
+        k.x[o + i] = x0 ? x0[o + i] : 0.0;
+        rr += ri * ri;
+    }
+    double v[1] = { rr };
+    wg_sum<1>(v, red);
+    return v[0];
+}
+
+// A: [x += omega shat; r = s - omega t]; p = r + beta (p - omega v); phat = M^-1 p
+template <int D, int PC>
+__device__ void solid_iter_a(const KArgs& k, uint32_t sk, const SysScalars& sc, double* red)
+{
+    const SolidArgs& s = k.sa;
+    const uint32_t o = D * s.idx[sk], n = D * s.np[sk];
+    double rmax = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        double ri = k.r[o + i];
+        if (sc.pending) {
+            const double si = PC ? k.sh[o + i] : ri;
+            k.x[o + i] = k.x[o + i] + sc.omega * si;
+            ri = ri + (-sc.omega) * k.t[o + i];
+            k.r[o + i] = ri;
+        }
+        rmax = nan_max(rmax, fabs(ri));
+        double pi;
+        if (sc.beta == 0.0) pi = ri;
+        else pi = sc.beta * (k.p[o + i] + (-sc.omega) * k.v[o + i]) + ri;
+        k.p[o + i] = pi;
+    }
+    rmax = wg_max(rmax, red);
+    if constexpr (PC != CYMG_PRECOND_NONE) {
+        __syncthreads();
+        double* ph = k.ph;
+        solid_precond<D>(s, sk, Plain{ k.p }, [=](uint32_t i, double y) { ph[i] = y; }, red);
+    }
+    if (threadIdx.x == 0) k.part[P_RMAX * k.nfib + k.nfiber + sk] = rmax;
+}
+
+// B: v = M phat; <rt, v>
+template <int D, int PC>
+__device__ void solid_iter_b(const KArgs& k, uint32_t sk, double* red)
+{
+    const double* X = PC ? k.ph : k.p;
+    double* v = k.v;
+    const double* rt = k.rt;
+    double dl = 0.0;
+    solid_operator<D>(k.fa, k.sa, sk, Plain{ X }, [&](uint32_t i, double y) {
+        v[i] = y;
+        dl += rt[i] * y;
+    }, red);
+    double w[1] = { dl };
+    wg_sum<1>(w, red);
+    if (threadIdx.x == 0) k.part[P_DELTA * k.nfib + k.nfiber + sk] = w[0];
+}
+
+// C: s = r - alpha v; x += alpha phat; shat = M^-1 s
+template <int D, int PC>
+__device__ void solid_iter_c(const KArgs& k, uint32_t sk, const SysScalars& sc, double* red)
+{
+    const SolidArgs& s = k.sa;
+    const uint32_t o = D * s.idx[sk], n = D * s.np[sk];
+    const double* X = PC ? k.ph : k.p;
+    double smax = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const double si = k.r[o + i] + (-sc.alpha) * k.v[o + i];
+        k.r[o + i] = si;
+        k.x[o + i] = k.x[o + i] + sc.alpha * X[o + i];
+        smax = nan_max(smax, fabs(si));
+    }
+    smax = wg_max(smax, red);
+    if constexpr (PC != CYMG_PRECOND_NONE) {
+        __syncthreads();
+        double* sh = k.sh;
+        solid_precond<D>(s, sk, Plain{ k.r }, [=](uint32_t i, double y) { sh[i] = y; }, red);
+    }
+    if (threadIdx.x == 0) k.part[P_SMAX * k.nfib + k.nfiber + sk] = smax;
+}
+
+// D: t = M shat; <t,t>, <t,s>, <rt,s>, <rt,t>
+template <int D, int PC>
+__device__ void solid_iter_d(const KArgs& k, uint32_t sk, double* red)
+{
+    const double* X = PC ? k.sh : k.r;
+    double* t = k.t;
+    const double* sv = k.r;
+    const double* rt = k.rt;
+    double acc[4] = { 0, 0, 0, 0 };
+    solid_operator<D>(k.fa, k.sa, sk, Plain{ X }, [&](uint32_t i, double y) {
+        t[i] = y;
+        acc[0] += y * y;
+        acc[1] += y * sv[i];
+        acc[2] += rt[i] * sv[i];
+        acc[3] += rt[i] * y;
+    }, red);
+    wg_sum<4>(acc, red);
+    if (threadIdx.x == 0)
+        *reinterpret_cast<double4*>(k.part4 + 4 * (size_t)(k.nfiber + sk)) = make_double4(acc[0], acc[1], acc[2], acc[3]);
+}
+
 // r = b - M x0 (or b), rt = r, x = x0 (or 0); <rt, r> partials; threshold
 template <int D, int PC>
 __global__ __launch_bounds__(kWave) void k_init(KArgs k, const double* __restrict__ x0, double tolerance,
@@ -200,26 +322,30 @@ __global__ __launch_bounds__(kWave) void k_init(KArgs k, const double* __restric
     __shared__ FiberLds<D> s;
     const uint32_t f = blockIdx.x;
     const FiberArgs& a = k.fa;
-    const uint32_t np = a.np[f], o = D * a.idx[f];
-    const int n = D * (int)np;
-    if (x0) {
-        for (int i = threadIdx.x; i < n; i += kWave) s.x[i] = x0[o + i];
-        load_factors<D>(a, s, a.seg[f], (int)np - 1);
-        __syncthreads();
-        fiber_operator<D>(a, s, x0, f);
-    }
     double rr = 0.0;
-    for (int i = threadIdx.x; i < n; i += kWave) {
-        double ri = k.b[o + i];
-        if (x0) ri = ri + (-1.0) * s.y[i];
-        k.r[o + i] = ri;
-        k.rt[o + i] = ri;
-        k.x[o + i] = x0 ? s.x[i] : 0.0;
-        rr += ri * ri;
+    if (f >= k.nfiber) {
+        rr = solid_init<D>(k, f - k.nfiber, x0, s.x);
+    } else {
+        const uint32_t np = a.np[f], o = D * a.idx[f];
+        const int n = D * (int)np;
+        if (x0) {
+            for (int i = threadIdx.x; i < n; i += kWave) s.x[i] = x0[o + i];
+            load_factors<D>(a, s, a.seg[f], (int)np - 1);
+            __syncthreads();
+            fiber_operator<D>(a, s, x0, f);
+        }
+        for (int i = threadIdx.x; i < n; i += kWave) {
+            double ri = k.b[o + i];
+            if (x0) ri = ri + (-1.0) * s.y[i];
+            k.r[o + i] = ri;
+            k.rt[o + i] = ri;
+            k.x[o + i] = x0 ? s.x[i] : 0.0;
+            rr += ri * ri;
+        }
+        rr = wave_sum(rr);
     }
-    rr = wave_sum(rr);
     if (threadIdx.x == 0) k.part[P_RR * k.nfib + f] = rr;
-    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
+    const uint32_t sys = (k.nsys == 1 || f >= k.nfiber) ? 0u : a.fib_sys[f];
     if (first_of_system(a, sys)) {
         const uint32_t fb = a.sys_begin[sys], fe = a.sys_begin[sys + 1];
         const double noise = wmin(k.part + P_NOISE * k.nfib, fb, fe);
@@ -252,8 +378,9 @@ __global__ __launch_bounds__(kApplyThreads) void k_iter_a(KArgs k)
     __shared__ double bc[4];
     const uint32_t f = blockIdx.x;
     const FiberArgs& a = k.fa;
-    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
-    const uint32_t np = a.np[f], o = D * a.idx[f];
+    const bool solid = f >= k.nfiber;
+    const uint32_t sys = (k.nsys == 1 || solid) ? 0u : a.fib_sys[f];
+    const uint32_t np = solid ? 0u : a.np[f], o = solid ? 0u : D * a.idx[f];
     const int n = D * (int)np;
     // the fiber's vectors are loaded while the scalar state is derived
     double vr[R], vt[R], vs[R], vp[R], vv[R];
@@ -313,6 +440,10 @@ __global__ __launch_bounds__(kApplyThreads) void k_iter_a(KArgs k)
     sc.stage = ST_A;
     if (first && threadIdx.x == 0) state_store(k.st_out + sys, sc);
     if (sc.done) return;
+    if (solid) {
+        solid_iter_a<D, PC>(k, f - k.nfiber, sc, red);
+        return;
+    }
     double rmax = 0.0;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -364,10 +495,11 @@ __global__ __launch_bounds__(kWave) void k_iter_b(KArgs k)
     __shared__ FiberLds<D> s;
         const uint32_t f = blockIdx.x;
     const FiberArgs& a = k.fa;
-    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
+    const bool solid = f >= k.nfiber;
+    const uint32_t sys = (k.nsys == 1 || solid) ? 0u : a.fib_sys[f];
     // the operand slice is loaded while the scalar state is derived
     const double* X = PC ? k.ph : k.p;
-    const uint32_t np = a.np[f], o = D * a.idx[f];
+    const uint32_t np = solid ? 0u : a.np[f], o = solid ? 0u : D * a.idx[f];
     const int n = D * (int)np;
     for (int i = threadIdx.x; i < n; i += kWave) s.x[i] = X[o + i];
     SysScalars sc = state_load(k.st_in + sys);
@@ -379,6 +511,10 @@ __global__ __launch_bounds__(kWave) void k_iter_b(KArgs k)
     }
     if (first && threadIdx.x == 0) state_store(k.st_out + sys, sc);
     if (sc.done) return;
+    if (solid) {
+        solid_iter_b<D, PC>(k, f - k.nfiber, s.x);
+        return;
+    }
     load_factors<D>(a, s, a.seg[f], (int)np - 1);
     __syncthreads();
     fiber_operator<D>(a, s, X, f);
@@ -401,9 +537,10 @@ __global__ __launch_bounds__(kApplyThreads) void k_iter_c(KArgs k)
     __shared__ double bc[4];
     const uint32_t f = blockIdx.x;
     const FiberArgs& a = k.fa;
-    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
+    const bool solid = f >= k.nfiber;
+    const uint32_t sys = (k.nsys == 1 || solid) ? 0u : a.fib_sys[f];
     const double* X = PC ? k.ph : k.p;
-    const uint32_t np = a.np[f], o = D * a.idx[f];
+    const uint32_t np = solid ? 0u : a.np[f], o = solid ? 0u : D * a.idx[f];
     const int n = D * (int)np;
     double vr[R], vv[R], vx[R], vX[R];
 #pragma unroll
@@ -432,6 +569,10 @@ __global__ __launch_bounds__(kApplyThreads) void k_iter_c(KArgs k)
     }
     if (first && threadIdx.x == 0) state_store(k.st_out + sys, sc);
     if (sc.done) return;
+    if (solid) {
+        solid_iter_c<D, PC>(k, f - k.nfiber, sc, red);
+        return;
+    }
     double smax = 0.0;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -460,9 +601,10 @@ __global__ __launch_bounds__(kWave) void k_iter_d(KArgs k)
     __shared__ FiberLds<D> s;
         const uint32_t f = blockIdx.x;
     const FiberArgs& a = k.fa;
-    const uint32_t sys = k.nsys == 1 ? 0u : a.fib_sys[f];
+    const bool solid = f >= k.nfiber;
+    const uint32_t sys = (k.nsys == 1 || solid) ? 0u : a.fib_sys[f];
     const double* X = PC ? k.sh : k.r;
-    const uint32_t np = a.np[f], o = D * a.idx[f];
+    const uint32_t np = solid ? 0u : a.np[f], o = solid ? 0u : D * a.idx[f];
     const int n = D * (int)np;
     for (int i = threadIdx.x; i < n; i += kWave) s.x[i] = X[o + i];
     SysScalars sc = state_load(k.st_in + sys);
@@ -475,6 +617,10 @@ __global__ __launch_bounds__(kWave) void k_iter_d(KArgs k)
     }
     if (first && threadIdx.x == 0) state_store(k.st_out + sys, sc);
     if (sc.done) return;
+    if (solid) {
+        solid_iter_d<D, PC>(k, f - k.nfiber, s.x);
+        return;
+    }
     load_factors<D>(a, s, a.seg[f], (int)np - 1);
     __syncthreads();
     fiber_operator<D>(a, s, X, f);
@@ -1243,14 +1389,16 @@ static KArgs make_kargs(Context& c)
     k.g1a = k.g1b = nullptr;
     k.g2 = nullptr;
     k.part = c.d_part.get();
-    k.part4 = c.d_part.get() + (size_t)P_TT * c.nfib;
+    k.part4 = c.d_part.get() + (size_t)P_TT * c.nmec();
     k.fpart = c.d_part.get();
     k.ta.desc = c.d_tile_desc.get();
     k.ta.ntiles = c.ntiles;
     k.st_in = nullptr;
     k.st_out = nullptr;
-    k.nfib = c.nfib;
+    k.nfib = c.nmec();
     k.nsys = c.nsys;
+    k.nfiber = c.nfib;
+    k.sa = make_solid_args(c);
     return k;
 }
 
@@ -1270,7 +1418,7 @@ static void run(Context& c, const double* guess, int max_iter, double tol, int t
     };
     const KArgs kI = with(S0, S1), kA = with(S1, S0), kB = with(S0, S1), kC = with(S1, S0), kD = with(S0, S1);
     const KArgs kDec = with(S0, S0);
-    const dim3 g(c.nfib), b(kWave), b4(kApplyThreads), gs(c.nsys);
+    const dim3 g(c.nmec()), b(kWave), b4(kApplyThreads), gs(c.nsys);
     hipLaunchKernelGGL((k_init<D, PC>), g, b, 0, c.stream, kI, guess, tol, tol_mode, max_iter, c.prm.time_step);
     hipLaunchKernelGGL((k_iter_a<D, PC>), g, b4, 0, c.stream, kA);
     hipLaunchKernelGGL(k_decide, gs, b, 0, c.stream, kDec, c.nsys);
@@ -1476,6 +1624,7 @@ static void tiled_buffers(Context& c)
 
 double krylov_time_launch(Context& c, int precond, int launches)
 {
+    if (c.nsol) throw Error(CYMG_ERR_UNSUPPORTED, "the two-launch iteration holds fibers only");
     tiled_buffers(c);
     const int pairs = std::max(1, launches / 2);
     if (c.dim == 3) {
@@ -1491,7 +1640,8 @@ double krylov_time_launch(Context& c, int precond, int launches)
 void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter, double tol, int tol_mode,
                   cymg_solve_report* rep)
 {
-    if (c.krylov_fused) {
+    // tiles hold whole fibers only: a system with Solids runs the four-launch iteration
+    if (c.krylov_fused && !c.nsol) {
         tiled_buffers(c);
         if (c.dim == 3) {
             if (precond == CYMG_PRECOND_FP64) run_tiled<3, CYMG_PRECOND_FP64>(c, guess_dev, max_iter, tol, tol_mode, rep);

@@ -447,6 +447,11 @@ const double* mlp_guess(Context& c)
     const int M = (int)((F + 15) / 16 * 16);
     CYMG_HIP(hipMemcpyAsync(s.h[0].get(), s.fx.get(), (size_t)M * kMlpIn * 2, hipMemcpyDeviceToDevice, c.stream));
     forward(c, M);
+    if (c.nsol) {
+        // Solids are seeded with M^-1 b
+        const size_t o = (size_t)c.dim * c.fib_pts;
+        CYMG_HIP(hipMemcpyAsync(s.x0.get() + o, s.y.get() + o, (c.n() - o) * 8, hipMemcpyDeviceToDevice, c.stream));
+    }
     const FiberArgs a = make_fiber_args(c);
     if (c.dim == 3)
         hipLaunchKernelGGL(k_guess<3>, dim3(F), dim3(kWave), 0, c.stream, a, s.y.get(),

@@ -551,7 +551,12 @@ static void launch_lds_build(Context& c, const PcArgs& a0, int precision, void* 
 
 void launch_precond_build(Context& c, int precision, double* block_out)
 {
-    if (!c.nfib) return;
+    if (!block_out) launch_solid_pc_build(c);
+    if (!c.nfib) {
+        c.pc_precision = precision;
+        c.pc_ready = true;
+        return;
+    }
     const int D = c.dim;
     const int bs_max = D * (int)c.largest;
     const size_t lds = precond_lds_bytes(D, bs_max, precision);
@@ -631,6 +636,7 @@ __global__ __launch_bounds__(4 * kWave) void k_precond_apply(int D, const uint32
 
 void launch_precond_apply(Context& c, const double* X, double* Y)
 {
+    launch_solid_pc_apply(c, X, Y);
     if (!c.nfib) return;
     const dim3 g(c.nfib), b(4 * kWave);
     if (c.pc_precision == CYMG_PRECOND_FP32)

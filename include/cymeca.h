@@ -30,6 +30,7 @@ extern "C" {
 
 /* largest fiber handled by the per-fiber kernels (points) */
 #define CYMG_MAX_FIBER_POINTS 128
+#define CYMG_MAX_SOLID_POINTS 65536
 
 typedef struct cymg_context cymg_context;
 
@@ -96,6 +97,15 @@ int         cymg_clear(cymg_context* ctx);
 int         cymg_add_fiber(cymg_context* ctx, uint32_t nb_points, double segmentation,
                            double rigidity, uint32_t* mat_index);
 int         cymg_nb_points(const cymg_context* ctx, uint32_t* nb_points);
+/* Meca::add of a Solid (solid.h), after every fiber (simul_solve.cc:204-207):
+   per-point radii (Solid::addSphere, solid.cc:352-362; 0 = no drag), the
+   reference shape fixed from `points` (DIM per point, Solid::fixShape,
+   solid.cc:370-400) and the initial soReshapeTimer (solid.cc:28, in [0, 7)).
+   Its preconditioner block is kept as identity + rank 6 (3 in 2D), the exact
+   inverse of the reference's dense block; a system with Solids runs the
+   four-launch iteration and may not be split into independent systems. */
+int         cymg_add_solid(cymg_context* ctx, uint32_t nb_points, const double* radius, const double* points,
+                           int reshape_timer, uint32_t* mat_index);
 /* independent systems solved together (no counterpart in the reference, which
    solves one system per Meca): fibers [first_fiber[s], first_fiber[s+1]) form
    system s, with its own Krylov scalars, threshold (noise level or |b|_inf) and
