@@ -84,6 +84,8 @@ def parse():
                     help="also run a plain device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--batched-steps", type=int, default=3,
                     help="steps of the 64-system batched workload (0: skip)")
+    ap.add_argument("--aster-steps", type=int, default=5,
+                    help="timed steps of the aster workload (10 Solids + 2000 fibers, 50000 motors; 0: skip)")
     return ap.parse_args()
 
 
@@ -239,6 +241,35 @@ def main():
         print(f"bench: batched {batched}", file=sys.stderr, flush=True)
         mb.close()
 
+    asters = None
+    if args.aster_steps > 0:
+        print("bench: generating the aster scene", file=sys.stderr, flush=True)
+        ast = scenes.config3(seed=3)
+        ga = [torch.from_numpy(ast.gauss(s)).to("cuda:0") for s in range(2)]
+        ma = pkg.Meca(3, 0)
+        pkg.load_scene(ma, ast)
+        its = []
+        for s in range(2 + args.aster_steps):
+            if s == 2:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+            if s > 0:
+                ma.prepare(None)
+                pkg.assemble(ma, ast, upload_couples=False)
+            rep = ma.solve(args.tol, tol_mode=pkg.TOL_RELATIVE, precondition=pkg.PRECOND_FP64,
+                           gauss_device_ptr=ga[s % 2].data_ptr())
+            if s >= 2:
+                its.append(rep["iterations"])
+        torch.cuda.synchronize()
+        adt = time.perf_counter() - t0
+        asters = {"workload": f"asters-10x200-fibers-{ast.nfib}-solids-{ast.nsol}-points-{ast.npts}-motors-{ast.nlinks}",
+                  "unknowns": ast.size, "steps_per_s": round(args.aster_steps / adt, 3),
+                  "ms_per_step": round(1e3 * adt / args.aster_steps, 3),
+                  "mean_bicgstab_iters": float(np.mean(its)),
+                  "iteration": "four launches per iteration (Solid rows)"}
+        print(f"bench: asters {asters}", file=sys.stderr, flush=True)
+        ma.close()
+
     cpu = None
     if args.cpu_steps > 0:
         from oracle import pyoracle
@@ -301,6 +332,7 @@ def main():
                        "precond_build_getrf_getri": build_lu_ms},
         "cpu_baseline": cpu,
         "batched": batched,
+        "asters": asters,
     }
     print(json.dumps(line), flush=True)
 

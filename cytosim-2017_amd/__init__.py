@@ -30,7 +30,8 @@ EXPORTED = [
     "cymg_prepare", "cymg_add_pure_force",
     "cymg_inter_link_ee", "cymg_inter_link_ie", "cymg_inter_link_ii", "cymg_inter_long_link_ee",
     "cymg_inter_clamp_e", "cymg_inter_clamp_i", "cymg_inter_long_clamp", "cymg_inter_plane_e",
-    "cymg_confine_sphere", "cymg_set_couples", "cymg_couples_interactions", "cymg_solve",
+    "cymg_confine_sphere", "cymg_set_couples", "cymg_couples_interactions", "cymg_set_aster_links",
+    "cymg_aster_interactions", "cymg_solve",
     "cymg_setup", "cymg_iterate", "cymg_apply", "cymg_multiply", "cymg_compute_preconditioner",
     "cymg_precondition", "cymg_compute_forces", "cymg_multiply_device", "cymg_precondition_device",
     "cymg_get_vector", "cymg_get_points", "cymg_get_tensions", "cymg_get_block", "cymg_get_matrix",
@@ -131,6 +132,8 @@ def lib():
         "cymg_confine_sphere": (C.c_int, [vp, C.c_double, C.c_double]),
         "cymg_set_couples": (C.c_int, [vp, u32, vp, vp, vp, vp, vp]),
         "cymg_couples_interactions": (C.c_int, [vp]),
+        "cymg_set_aster_links": (C.c_int, [vp, u32, vp, vp, vp, vp, vp, vp]),
+        "cymg_aster_interactions": (C.c_int, [vp]),
         "cymg_solve": (C.c_int, [vp, C.POINTER(SolveParams), vp, C.POINTER(SolveReport)]),
         "cymg_setup": (C.c_int, [vp, vp, C.c_int]),
         "cymg_iterate": (C.c_int, [vp, C.POINTER(SolveParams), C.POINTER(SolveReport)]),
@@ -314,6 +317,18 @@ class Meca:
 
     def couples_interactions(self):
         self._chk(self.L.cymg_couples_interactions(self.h))
+
+    def set_aster_links(self, kind, s1, s2, fiber, abscissa, stiffness):
+        kd = np.ascontiguousarray(kind, dtype=np.int32)
+        a = np.ascontiguousarray(s1, dtype=np.uint32)
+        b = np.ascontiguousarray(s2, dtype=np.uint32)
+        f = np.ascontiguousarray(fiber, dtype=np.uint32)
+        ab = np.ascontiguousarray(abscissa, dtype=np.float64)
+        w = np.ascontiguousarray(stiffness, dtype=np.float64)
+        self._chk(self.L.cymg_set_aster_links(self.h, kd.size, _p(kd), _p(a), _p(b), _p(f), _p(ab), _p(w)))
+
+    def aster_interactions(self):
+        self._chk(self.L.cymg_aster_interactions(self.h))
 
     # solve ------------------------------------------------------------------------
     @staticmethod
@@ -512,6 +527,12 @@ def assemble(meca, scene, couples_on_device=True, upload_couples=True):
                 meca.inter_link_ii(int(a1[k]), int(a2[k]), float(ca[k]), int(b1[k]), int(b2[k]),
                                    float(cb[k]), float(scene.link_stiffness[k]))
     # Aster::setInteractions (aster.cc:53-89), after the couples (simul_solve.cc:238-239)
+    if getattr(scene, "naster_links", 0) and couples_on_device:
+        if upload_couples:
+            meca.set_aster_links(scene.aster_kind, scene.aster_s1, scene.aster_s1, scene.aster_fiber,
+                                 scene.aster_abs, scene.aster_w)
+        meca.aster_interactions()
+        return
     for k in range(getattr(scene, "naster_links", 0)):
         if scene.aster_kind[k] == 0:
             meca.inter_link_ee(int(scene.aster_a1[k]), int(scene.aster_b[k]), float(scene.aster_w[k]))

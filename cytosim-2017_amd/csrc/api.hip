@@ -768,6 +768,42 @@ int cymg_couples_interactions(cymg_context* ctx)
     });
 }
 
+int cymg_set_aster_links(cymg_context* ctx, uint32_t n, const int* kind, const uint32_t* s1, const uint32_t* s2,
+                         const uint32_t* fiber, const double* ab, const double* w)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (n && (!kind || !s1 || !s2 || !fiber || !ab || !w)) throw Error(CYMG_ERR_INVALID, "null array");
+        for (uint32_t k = 0; k < n; ++k) {
+            if (kind[k] < 0 || kind[k] > 2) throw Error(CYMG_ERR_INVALID, "aster link kind must be 0, 1 or 2");
+            if (fiber[k] >= c.nfib) throw Error(CYMG_ERR_INVALID, "aster link on an unknown fiber");
+            if (s1[k] < c.fib_pts || s1[k] >= c.npts || (kind[k] == 2 && (s2[k] < c.fib_pts || s2[k] >= c.npts)))
+                throw Error(CYMG_ERR_INVALID, "aster link on a point that is not a Solid point");
+            if (!(w[k] >= 0)) throw Error(CYMG_ERR_INVALID, "aster stiffness must be >= 0");
+        }
+        c.nasters = n;
+        const size_t m = n ? n : 1;
+        c.ast_kind.resize(m); c.ast_s1.resize(m); c.ast_s2.resize(m); c.ast_f.resize(m);
+        c.ast_ab.resize(m); c.ast_w.resize(m);
+        if (n) {
+            CYMG_HIP(hipMemcpyAsync(c.ast_kind.get(), kind, n * 4, hipMemcpyHostToDevice, c.stream));
+            CYMG_HIP(hipMemcpyAsync(c.ast_s1.get(), s1, n * 4, hipMemcpyHostToDevice, c.stream));
+            CYMG_HIP(hipMemcpyAsync(c.ast_s2.get(), s2, n * 4, hipMemcpyHostToDevice, c.stream));
+            CYMG_HIP(hipMemcpyAsync(c.ast_f.get(), fiber, n * 4, hipMemcpyHostToDevice, c.stream));
+            CYMG_HIP(hipMemcpyAsync(c.ast_ab.get(), ab, n * 8, hipMemcpyHostToDevice, c.stream));
+            CYMG_HIP(hipMemcpyAsync(c.ast_w.get(), w, n * 8, hipMemcpyHostToDevice, c.stream));
+            CYMG_HIP(hipStreamSynchronize(c.stream));
+        }
+    });
+}
+
+int cymg_aster_interactions(cymg_context* ctx)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c);
+        c.marks.push_back({ 3, 0.0, 0.0, c.hB.key.size(), c.hC.key.size(), c.hBas.key.size() });
+    });
+}
+
 // ---- solve ---------------------------------------------------------------------
 
 int cymg_setup(cymg_context* ctx, const double* gauss, int on_device)

@@ -185,6 +185,7 @@ void build_streams(Context& c)
     for (const EmitterMark& mk : c.marks) {
         if (mk.kind == 1) { total[1] += NC * c.fib_pts; total[2] += (size_t)D * c.fib_pts; }
         if (mk.kind == 2) { total[0] += (size_t)10 * c.ncouples; }
+        if (mk.kind == 3) { total[0] += (size_t)6 * c.nasters; }
     }
     for (int w = 0; w < 3; ++w) {
         c.s_total[w] = total[w];
@@ -216,6 +217,9 @@ void build_streams(Context& c)
         } else if (mk.kind == 2) {
             launch_emit_couples(c, c.s_key[0].get() + dpos[0], c.s_val[0].get() + dpos[0]);
             dpos[0] += (size_t)10 * c.ncouples;
+        } else if (mk.kind == 3) {
+            launch_emit_asters(c, c.s_key[0].get() + dpos[0], c.s_val[0].get() + dpos[0]);
+            dpos[0] += (size_t)6 * c.nasters;
         }
     }
     for (int w = 0; w < 3; ++w) copy_host(w, hs[w]->key.size());

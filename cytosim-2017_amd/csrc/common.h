@@ -94,7 +94,7 @@ struct HostStream {
 // device emitter placed in call order: the host-recorded entries before it are
 // those below its recorded positions in each host stream
 struct EmitterMark {
-    int kind;                     // 1 sphere confinement, 2 couples
+    int kind;                     // 1 sphere confinement, 2 couples, 3 aster links
     double a, b;                  // emitter parameters (radius, stiffness)
     size_t posB, posC, posBas;    // host stream sizes when the emitter was called
 };
@@ -123,6 +123,7 @@ void launch_prepare(Context& c);
 void launch_emit_confine(Context& c, uint64_t* keyC, double* valC, uint64_t* keyBas,
                          double* valBas, double radius, double stiffness);
 void launch_emit_couples(Context& c, uint64_t* keyB, double* valB);
+void launch_emit_asters(Context& c, uint64_t* keyB, double* valB);
 void launch_setup(Context& c, const double* gauss);
 void launch_matvec(Context& c, const double* X, double* Y);
 void launch_forces(Context& c, const double* X, double* Y, bool with_rigidity);

@@ -70,6 +70,12 @@ struct Context {
     DevBuf<uint32_t> cpl_fa, cpl_fb;
     DevBuf<double> cpl_aa, cpl_ab, cpl_w;
 
+    // aster links (device emitter)
+    uint32_t nasters = 0;
+    DevBuf<int> ast_kind;
+    DevBuf<uint32_t> ast_s1, ast_s2, ast_f;
+    DevBuf<double> ast_ab, ast_w;
+
     // preconditioner
     int pc_precision = 0;              // CYMG_PRECOND_*
     int pc_method = 0;                 // 0 Gauss-Jordan in registers, 1 getrf + getri, 2 Gauss-Jordan in LDS

@@ -248,6 +248,68 @@ __global__ void k_emit_couples(uint32_t n, const uint32_t* __restrict__ fa, cons
         }
 }
 
+// Aster::setInteractions (aster.cc:53-89): links between a Solid point and a
+// fiber, 6 slots per link in the order of the host interLink calls:
+//   kind 0: interLink(PointExact(solid, s1), fiber minus end)       (meca_inter.cc:196-230)
+//   kind 1: interLink(fiber at abscissa ab, PointExact(solid, s1))  (meca_inter.cc:240-275)
+//   kind 2: interLink(PointInterpolated(solid, s1, s2, ab), fiber plus end)
+// The two ends of a link are on different Mecables, so no link is skipped.
+__global__ void k_emit_asters(uint32_t n, const int* __restrict__ kind, const uint32_t* __restrict__ s1v,
+                              const uint32_t* __restrict__ s2v, const uint32_t* __restrict__ fv,
+                              const double* __restrict__ abv, const double* __restrict__ wv,
+                              const uint32_t* __restrict__ npv, const uint32_t* __restrict__ idxv,
+                              const double* __restrict__ cutv, uint64_t* __restrict__ keyB, double* __restrict__ valB)
+{
+#pragma clang fp contract(off)
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint64_t* kk = keyB + (size_t)6 * k;
+    double* vv = valB + (size_t)6 * k;
+    const double w = wv[k];
+    const uint32_t f = fv[k];
+    const int kd = kind[k];
+    if (kd == 0) {
+        const uint32_t a = s1v[k], b = idxv[f];
+        kk[0] = lower_key(a, a); vv[0] = -w;
+        kk[1] = lower_key(a, b); vv[1] = +w;
+        kk[2] = lower_key(b, b); vv[2] = -w;
+        for (int i = 3; i < 6; ++i) { kk[i] = kInvalidKey; vv[i] = 0.0; }
+        return;
+    }
+    uint32_t a1, a2, b;
+    double ca;
+    if (kd == 1) {
+        interp(npv[f], idxv[f], cutv[f], abv[k], a1, ca);
+        a2 = a1 + 1;
+        b = s1v[k];
+    } else {
+        a1 = s1v[k];
+        a2 = s2v[k];
+        ca = abv[k];
+        b = idxv[f] + npv[f] - 1;
+    }
+    const double cc[3] = { 1.0 - ca, ca, -1.0 };
+    const double cw[3] = { w * cc[0], w * cc[1], -w };
+    const uint32_t inx[3] = { a1, a2, b };
+    int s = 0;
+    for (int q = 0; q < 3; ++q)
+        for (int l = q; l < 3; ++l) {
+            kk[s] = lower_key(inx[q], inx[l]);
+            vv[s] = -(cc[q] * cw[l]);
+            ++s;
+        }
+}
+
+void launch_emit_asters(Context& c, uint64_t* keyB, double* valB)
+{
+    if (!c.nasters) return;
+    const dim3 b(256), g((c.nasters + 255) / 256);
+    hipLaunchKernelGGL(k_emit_asters, g, b, 0, c.stream, c.nasters, c.ast_kind.get(), c.ast_s1.get(), c.ast_s2.get(),
+                       c.ast_f.get(), c.ast_ab.get(), c.ast_w.get(), c.d_np.get(), c.d_idx.get(), c.d_cut.get(),
+                       keyB, valB);
+    CYMG_HIP(hipGetLastError());
+}
+
 void launch_emit_couples(Context& c, uint64_t* keyB, double* valB)
 {
     if (!c.ncouples) return;

@@ -41,7 +41,13 @@ struct FiberLds {
     double d[kMaxFiberPoints];         // pttrf D
     double e[kMaxFiberPoints];         // pttrf L subdiagonal
     double red[kWave];
+    uint32_t hub[kMaxFiberPoints];     // points whose mB row is long (many links on one point)
+    uint32_t nhub;
 };
+
+// mB rows longer than this are gathered by the whole workgroup, one entry per
+// thread, instead of by the thread that owns the point
+constexpr uint32_t kHubRow = 24;
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -141,6 +147,8 @@ __device__ __forceinline__ void fiber_forces_op(const FiberArgs& a, FiberLds<D>&
                                                 bool rigidity, bool with_pdiff)
 {
     const int S = (int)np - 1;
+    if (threadIdx.x == 0) s.nhub = 0;
+    __syncthreads();
     for (int p = threadIdx.x; p < (int)np; p += blockDim.x) {
         double fp[D];
 #pragma unroll
@@ -152,8 +160,13 @@ __device__ __forceinline__ void fiber_forces_op(const FiberArgs& a, FiberLds<D>&
             for (int d = 0; d <= D; ++d) cb[d] = a.Crow[D * r + d];
         }
         if (a.Buse) {
-            const uint32_t e1 = a.Brow[r + 1];
-            for (uint32_t q = a.Brow[r]; q < e1; ++q) {
+            uint32_t e1 = a.Brow[r + 1];
+            const uint32_t e0 = a.Brow[r];
+            if (e1 - e0 > kHubRow) {
+                s.hub[atomicAdd(&s.nhub, 1u)] = (uint32_t)p;
+                e1 = e0;
+            }
+            for (uint32_t q = e0; q < e1; ++q) {
                 const double v = a.Bval[q];
                 const uint32_t c = a.Bcol[q];
 #pragma unroll
@@ -205,6 +218,37 @@ __device__ __forceinline__ void fiber_forces_op(const FiberArgs& a, FiberLds<D>&
         }
 #pragma unroll
         for (int d = 0; d < D; ++d) s.f[D * p + d] = fp[d];
+    }
+    __syncthreads();
+    // long rows: entries spread over the workgroup, summed in a fixed order
+    const uint32_t nh = s.nhub;
+    for (uint32_t h = 0; h < nh; ++h) {
+        const uint32_t p = s.hub[h], r = idx + p;
+        double acc[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[d] = 0.0;
+        const uint32_t e1 = a.Brow[r + 1];
+        for (uint32_t q = a.Brow[r] + threadIdx.x; q < e1; q += blockDim.x) {
+            const double v = a.Bval[q];
+            const uint32_t c = a.Bcol[q];
+#pragma unroll
+            for (int d = 0; d < D; ++d) acc[d] += v * X(D * c + d);
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[d] = wave_sum(acc[d]);
+        const int w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+        if ((threadIdx.x & (kWave - 1)) == 0)
+#pragma unroll
+            for (int d = 0; d < D; ++d) s.red[D * w + d] = acc[d];
+        __syncthreads();
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                double t = 0.0;
+                for (int q = 0; q < nw; ++q) t += s.red[D * q + d];
+                s.f[D * p + d] += t;
+            }
+        __syncthreads();
     }
 }
 

@@ -194,20 +194,27 @@ __device__ __forceinline__ void apply_block(const KArgs& k, uint32_t f, int bs, 
         block_apply_mw<float, kApplyWaves>(k.B32 + k.blkoff[f], bs, k.useblk[f] != 0, xs, y, red);
 }
 
+// row extents a Solid workgroup of the operator phases stages in the
+// FiberLds force array it does not otherwise use
+template <int D>
+constexpr uint32_t kLdsRows = 2 * kMaxFiberPoints * D;
+
 // ---------------------------------------------------------------------------
 // The rows of Solid sk in each phase (the same vector updates as the fibers'
 // rows, with the Solid operator and its low-rank block inverse); red: LDS
 // scratch of at least 96 doubles.
 
 template <int D>
-__device__ double solid_init(const KArgs& k, uint32_t sk, const double* __restrict__ x0, double* red)
+__device__ double solid_init(const KArgs& k, uint32_t sk, const double* __restrict__ x0, double* red, uint32_t* lrow,
+                             uint32_t cap)
 {
     const SolidArgs& s = k.sa;
     const uint32_t o = D * s.idx[sk], n = D * s.np[sk];
     if (x0) {
         double* r = k.r;
         const double* b = k.b;
-        solid_operator<D>(k.fa, s, sk, Plain{ x0 }, [=](uint32_t i, double y) { r[i] = b[i] + (-1.0) * y; }, red);
+        solid_operator<D>(k.fa, s, sk, Plain{ x0 }, [=](uint32_t i, double y) { r[i] = b[i] + (-1.0) * y; }, red,
+                          lrow, cap);
         __syncthreads();
     }
     double rr = 0.0;
@@ -255,7 +262,7 @@ __device__ void solid_iter_a(const KArgs& k, uint32_t sk, const SysScalars& sc, 
 
 // B: v = M phat; <rt, v>
 template <int D, int PC>
-__device__ void solid_iter_b(const KArgs& k, uint32_t sk, double* red)
+__device__ void solid_iter_b(const KArgs& k, uint32_t sk, double* red, uint32_t* lrow, uint32_t cap)
 {
     const double* X = PC ? k.ph : k.p;
     double* v = k.v;
@@ -264,7 +271,7 @@ __device__ void solid_iter_b(const KArgs& k, uint32_t sk, double* red)
     solid_operator<D>(k.fa, k.sa, sk, Plain{ X }, [&](uint32_t i, double y) {
         v[i] = y;
         dl += rt[i] * y;
-    }, red);
+    }, red, lrow, cap);
     double w[1] = { dl };
     wg_sum<1>(w, red);
     if (threadIdx.x == 0) k.part[P_DELTA * k.nfib + k.nfiber + sk] = w[0];
@@ -295,7 +302,7 @@ __device__ void solid_iter_c(const KArgs& k, uint32_t sk, const SysScalars& sc, 
 
 // D: t = M shat; <t,t>, <t,s>, <rt,s>, <rt,t>
 template <int D, int PC>
-__device__ void solid_iter_d(const KArgs& k, uint32_t sk, double* red)
+__device__ void solid_iter_d(const KArgs& k, uint32_t sk, double* red, uint32_t* lrow, uint32_t cap)
 {
     const double* X = PC ? k.sh : k.r;
     double* t = k.t;
@@ -308,7 +315,7 @@ __device__ void solid_iter_d(const KArgs& k, uint32_t sk, double* red)
         acc[1] += y * sv[i];
         acc[2] += rt[i] * sv[i];
         acc[3] += rt[i] * y;
-    }, red);
+    }, red, lrow, cap);
     wg_sum<4>(acc, red);
     if (threadIdx.x == 0)
         *reinterpret_cast<double4*>(k.part4 + 4 * (size_t)(k.nfiber + sk)) = make_double4(acc[0], acc[1], acc[2], acc[3]);
@@ -324,7 +331,7 @@ __global__ __launch_bounds__(kWave) void k_init(KArgs k, const double* __restric
     const FiberArgs& a = k.fa;
     double rr = 0.0;
     if (f >= k.nfiber) {
-        rr = solid_init<D>(k, f - k.nfiber, x0, s.x);
+        rr = solid_init<D>(k, f - k.nfiber, x0, s.x, reinterpret_cast<uint32_t*>(s.f), kLdsRows<D>);
     } else {
         const uint32_t np = a.np[f], o = D * a.idx[f];
         const int n = D * (int)np;
@@ -511,10 +518,6 @@ __global__ __launch_bounds__(kWave) void k_iter_b(KArgs k)
     }
     if (first && threadIdx.x == 0) state_store(k.st_out + sys, sc);
     if (sc.done) return;
-    if (solid) {
-        solid_iter_b<D, PC>(k, f - k.nfiber, s.x);
-        return;
-    }
     load_factors<D>(a, s, a.seg[f], (int)np - 1);
     __syncthreads();
     fiber_operator<D>(a, s, X, f);
@@ -617,10 +620,6 @@ __global__ __launch_bounds__(kWave) void k_iter_d(KArgs k)
     }
     if (first && threadIdx.x == 0) state_store(k.st_out + sys, sc);
     if (sc.done) return;
-    if (solid) {
-        solid_iter_d<D, PC>(k, f - k.nfiber, s.x);
-        return;
-    }
     load_factors<D>(a, s, a.seg[f], (int)np - 1);
     __syncthreads();
     fiber_operator<D>(a, s, X, f);
@@ -643,6 +642,38 @@ __global__ __launch_bounds__(kWave) void k_iter_d(KArgs k)
         *reinterpret_cast<double4*>(k.part4 + 4 * (size_t)f) = make_double4(tt, ts, rts, rtt);
 }
 
+
+// The operator phases B and D of the Solids, launched after the fibers' phase
+// (same scalar state in, partial sums in the Solids' slots): a Solid's rows
+// hold many more coupling entries than a fiber's (one link per aster fiber on
+// the central point), so each Solid gets a workgroup of kSolidOpThreads.  The
+// fibers' launch writes the phase's state; a Solid workgroup writes it only
+// when there is no fiber.
+constexpr int kSolidOpThreads = 1024;
+constexpr uint32_t kSolidOpRows = 4096;
+
+template <int D, int PC, int PHASE>
+__global__ __launch_bounds__(kSolidOpThreads) void k_solid_phase(KArgs k)
+{
+    __shared__ double red[16 * 6];
+    __shared__ uint32_t lrow[kSolidOpRows];
+    const uint32_t sk = blockIdx.x;
+    SysScalars sc = state_load(k.st_in);
+    const bool first = k.nfiber == 0 && sk == 0;
+    if (!sc.done) {
+        if constexpr (PHASE == ST_B) {
+            decide(sc, wmax(k.part + P_RMAX * k.nfib, 0, k.nfib));
+        } else {
+            const double m = wmax(k.part + P_SMAX * k.nfib, 0, k.nfib);
+            sc.resid_s = (m != m) ? __builtin_inf() : m;
+        }
+        sc.stage = PHASE;
+    }
+    if (first && threadIdx.x == 0) state_store(k.st_out, sc);
+    if (sc.done) return;
+    if constexpr (PHASE == ST_B) solid_iter_b<D, PC>(k, sk, red, lrow, kSolidOpRows);
+    else solid_iter_d<D, PC>(k, sk, red, lrow, kSolidOpRows);
+}
 
 // ===========================================================================
 // Fused variant: two launches per iteration over tiles of whole fibers.  The
@@ -1418,7 +1449,7 @@ static void run(Context& c, const double* guess, int max_iter, double tol, int t
     };
     const KArgs kI = with(S0, S1), kA = with(S1, S0), kB = with(S0, S1), kC = with(S1, S0), kD = with(S0, S1);
     const KArgs kDec = with(S0, S0);
-    const dim3 g(c.nmec()), b(kWave), b4(kApplyThreads), gs(c.nsys);
+    const dim3 g(c.nmec()), gf(c.nfib), b(kWave), b4(kApplyThreads), gs(c.nsys), gso(c.nsol), bso(kSolidOpThreads);
     hipLaunchKernelGGL((k_init<D, PC>), g, b, 0, c.stream, kI, guess, tol, tol_mode, max_iter, c.prm.time_step);
     hipLaunchKernelGGL((k_iter_a<D, PC>), g, b4, 0, c.stream, kA);
     hipLaunchKernelGGL(k_decide, gs, b, 0, c.stream, kDec, c.nsys);
@@ -1437,9 +1468,11 @@ static void run(Context& c, const double* guess, int max_iter, double tol, int t
         for (uint32_t s = 0; s < c.nsys; ++s) all = all && c.h_scal[s].done;
         if (all || launched > max_iter + 2) break;
         for (int it = 0; it < chunk; ++it) {
-            hipLaunchKernelGGL((k_iter_b<D, PC>), g, b, 0, c.stream, kB);
+            if (c.nfib) hipLaunchKernelGGL((k_iter_b<D, PC>), gf, b, 0, c.stream, kB);
+            if (c.nsol) hipLaunchKernelGGL((k_solid_phase<D, PC, ST_B>), gso, bso, 0, c.stream, kB);
             hipLaunchKernelGGL((k_iter_c<D, PC>), g, b4, 0, c.stream, kC);
-            hipLaunchKernelGGL((k_iter_d<D, PC>), g, b, 0, c.stream, kD);
+            if (c.nfib) hipLaunchKernelGGL((k_iter_d<D, PC>), gf, b, 0, c.stream, kD);
+            if (c.nsol) hipLaunchKernelGGL((k_solid_phase<D, PC, ST_D>), gso, bso, 0, c.stream, kD);
             hipLaunchKernelGGL((k_iter_a<D, PC>), g, b4, 0, c.stream, kA);
         }
         hipLaunchKernelGGL(k_decide, gs, b, 0, c.stream, kDec, c.nsys);

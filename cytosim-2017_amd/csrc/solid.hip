@@ -138,6 +138,8 @@ __global__ __launch_bounds__(kSolidThreads) void k_solid_setup(FiberArgs a, Soli
     }
 }
 
+constexpr uint32_t kSolidRows = 4096;   // row extents staged in LDS by the Solid kernels
+
 struct StoreTo {
     double* __restrict__ Y;
     __device__ __forceinline__ void operator()(uint32_t i, double v) const { Y[i] = v; }
@@ -148,7 +150,8 @@ __global__ __launch_bounds__(kSolidThreads) void k_solid_matvec(FiberArgs a, Sol
                                                                 double* __restrict__ Y)
 {
     __shared__ double red[16 * 6];
-    solid_operator<D>(a, s, blockIdx.x, Plain{ X }, StoreTo{ Y }, red);
+    __shared__ uint32_t lrow[kSolidRows];
+    solid_operator<D>(a, s, blockIdx.x, Plain{ X }, StoreTo{ Y }, red, lrow, kSolidRows);
 }
 
 // Meca::computeForces(X, Y) rows of Solid k: Y = vBAS + (B + C) X (a Solid has no rigidity)

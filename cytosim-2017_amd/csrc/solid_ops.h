@@ -183,21 +183,104 @@ __device__ __forceinline__ void coupling_row(const FiberArgs& a, uint32_t r, con
     }
 }
 
+// last p in [0, n) with rows[p] <= q (rows ascending)
+__device__ __forceinline__ uint32_t row_of(const uint32_t* rows, uint32_t n, uint32_t q)
+{
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (rows[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// total force and torque of (B + C) X over the Solid's rows.  Only these six
+// sums matter to the projection, so the CSR entries of the Solid's rows are
+// spread over the workgroup one entry per thread (a Solid point can carry one
+// link per fiber of its aster: a row-per-thread loop would serialise them),
+// each entry's row found by binary search in the row extents staged in LDS
+// (lrow: cap entries).  Solids too large for the staging use one row per thread.
+template <int D, typename Op>
+__device__ __forceinline__ void solid_coupling_moments(const FiberArgs& a, const SolidArgs& s, uint32_t k,
+                                                       const Op& X, double acc[6], uint32_t* lrow, uint32_t cap)
+{
+    const uint32_t np = s.np[k], idx = s.idx[k];
+    if (a.Buse) {
+        if (np + 1 <= cap) {
+            for (uint32_t i = threadIdx.x; i <= np; i += blockDim.x) lrow[i] = a.Brow[idx + i];
+            __syncthreads();
+            const uint32_t q1 = lrow[np];
+            for (uint32_t q = lrow[0] + threadIdx.x; q < q1; q += blockDim.x) {
+                const uint32_t r = idx + row_of(lrow, np, q);
+                const double v = a.Bval[q];
+                const uint32_t c = a.Bcol[q];
+                double f[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) f[d] = v * X(D * c + d);
+                add_moments<D>(s.pos + D * r, f, acc);
+            }
+            __syncthreads();
+        } else {
+            for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
+                double f[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) f[d] = 0.0;
+                const uint32_t r = idx + p;
+                for (uint32_t q = a.Brow[r]; q < a.Brow[r + 1]; ++q) {
+                    const double v = a.Bval[q];
+                    const uint32_t c = a.Bcol[q];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) f[d] += v * X(D * c + d);
+                }
+                add_moments<D>(s.pos + D * r, f, acc);
+            }
+        }
+    }
+    if (a.Cuse) {
+        const uint32_t nd = D * np, o = D * idx;
+        if (nd + 1 <= cap) {
+            for (uint32_t i = threadIdx.x; i <= nd; i += blockDim.x) lrow[i] = a.Crow[o + i];
+            __syncthreads();
+            const uint32_t q1 = lrow[nd];
+            for (uint32_t q = lrow[0] + threadIdx.x; q < q1; q += blockDim.x) {
+                const uint32_t i = row_of(lrow, nd, q);
+                const uint32_t p = i / D;
+                double f[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) f[d] = 0.0;
+                f[i - D * p] = a.Cval[q] * X(a.Ccol[q]);
+                add_moments<D>(s.pos + D * (idx + p), f, acc);
+            }
+            __syncthreads();
+        } else {
+            for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
+                double f[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    double t = 0.0;
+                    for (uint32_t q = a.Crow[o + D * p + d]; q < a.Crow[o + D * p + d + 1]; ++q)
+                        t += a.Cval[q] * X(a.Ccol[q]);
+                    f[d] = t;
+                }
+                add_moments<D>(s.pos + D * (idx + p), f, acc);
+            }
+        }
+    }
+}
+
 // Meca::multiply on the rows of Solid k: Y = X + setSpeedsFromForces((B+C) X, -dt)
-// (meca.cc:157-181); out(i, value) receives each row
+// (meca.cc:157-181); out(i, value) receives each row.  lrow: LDS staging of
+// cap row extents (see solid_coupling_moments).
 template <int D, typename Op, typename Out>
 __device__ __forceinline__ void solid_operator(const FiberArgs& a, const SolidArgs& s, uint32_t k, const Op& X,
-                                               const Out& out, double* red)
+                                               const Out& out, double* red, uint32_t* lrow, uint32_t cap)
 {
     constexpr int NG = RigidDim<D>::NG;
     const uint32_t np = s.np[k], idx = s.idx[k];
     const double* geo = s.geo + kSolidGeo * k;
     double acc[6] = { 0, 0, 0, 0, 0, 0 };
-    for (uint32_t p = threadIdx.x; p < np; p += blockDim.x) {
-        double fp[D];
-        coupling_row<D>(a, idx + p, X, fp);
-        add_moments<D>(s.pos + D * (idx + p), fp, acc);
-    }
+    solid_coupling_moments<D>(a, s, k, X, acc, lrow, cap);
     wg_sum<NG>(acc, red);
     double q[6];
     solid_q<D>(geo, -a.dt, acc, q);

@@ -71,6 +71,10 @@ class Scene:
     aster_ca: np.ndarray = field(default_factory=lambda: np.zeros(0))
     aster_b: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
     aster_w: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    # the same links as (Solid point, fiber, abscissa) for the device emitter
+    aster_s1: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    aster_fiber: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    aster_abs: np.ndarray = field(default_factory=lambda: np.zeros(0))
     seed: int = 0
     name: str = ""
 
@@ -208,8 +212,9 @@ def _segment_segment(p0, d0, p1, d1):
     return s, t, np.linalg.norm(q0 - q1, axis=1)
 
 
-def _close_segment_pairs(dim, nbp, points, rng_range, return_params=False):
-    """All pairs of segments of different fibers closer than rng_range."""
+def _close_segment_pairs(dim, nbp, points, rng_range, return_params=False, group=None):
+    """All pairs of segments of different fibers closer than rng_range (and
+    of different groups when `group`, one label per fiber, is given)."""
     P = points.reshape(-1, dim)
     nbp = nbp.astype(np.int64)
     fib_of_pt = np.repeat(np.arange(nbp.size), nbp)
@@ -243,6 +248,8 @@ def _close_segment_pairs(dim, nbp, points, rng_range, return_params=False):
         within = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)
         jj = order[np.repeat(lo, cnt) + within]
         keep = (ii < jj) & (seg_fib[ii] != seg_fib[jj])
+        if group is not None:
+            keep &= group[seg_fib[ii]] != group[seg_fib[jj]]
         pairs_i.append(ii[keep])
         pairs_j.append(jj[keep])
     ii = np.concatenate(pairs_i)
@@ -345,13 +352,13 @@ def mini(dim=3, seed=3, nfib=8, nlinks=20, radius=1.5, lmin=2.0, lmax=6.0, bindi
     return sc
 
 
-def _bind_crosslinkers_fast(rng, dim, nbp, cut, points, nlinks, binding_range):
+def _bind_crosslinkers_fast(rng, dim, nbp, cut, points, nlinks, binding_range, group=None):
     """Vectorised variant of _bind_crosslinkers for large scenes: hand 1 at a
     uniform abscissa within binding range of the closest approach of a random
     close segment pair, hand 2 at the nearest point of the partner segment;
     kept when the hands are within binding range."""
     P = points.reshape(-1, dim)
-    seg_pt, seg_fib, ci, cj, cs, ct = _close_segment_pairs(dim, nbp, points, binding_range, True)
+    seg_pt, seg_fib, ci, cj, cs, ct = _close_segment_pairs(dim, nbp, points, binding_range, True, group)
     if ci.size == 0:
         raise ValueError("no fiber crossings within binding range")
     nbp = nbp.astype(np.int64)
@@ -404,16 +411,23 @@ def _sphere_directions(n, dim, rng=None):
 
 def asters(dim=3, seed=3, nasters=10, fibers_per_aster=200, lmin=5.0, lmax=10.0, core=0.5, spread=4.0,
            radius=12.0, nmotors=50000, binding_range=0.05, stiffness=(1000.0, 500.0), motor_stiffness=100.0,
-           name=None):
+           min_separation=0.0, name=None):
     """Asters: each a Solid (central bead of radius `core` + one drag-free point
     per fiber at distance `core`) with `fibers_per_aster` fibers of lengths
     uniform in [lmin, lmax] (segmentation 0.5, rigidity 20) starting at the
     centre; aster centres uniform in a ball of radius `spread`; fibers confined
     in a sphere of radius `radius` (stiffness 200); `nmotors` couples
     (stiffness `motor_stiffness`) bound between fibers within `binding_range`
-    (motor stepping frozen); aster stiffness (clamp1 link, clamp2 link)."""
+    (motor stepping frozen) between fibers of different asters (the fibers of
+    one aster all meet at its centre, where every pair is within range);
+    aster stiffness (clamp1 link, clamp2 link)."""
     rng = np.random.default_rng(seed)
-    centers = _random_in_ball(rng, nasters, dim, spread)
+    # aster centres at least `min_separation` apart (rejection sampling)
+    centers = np.empty((0, dim))
+    while centers.shape[0] < nasters:
+        c = _random_in_ball(rng, 1, dim, spread)
+        if centers.shape[0] == 0 or np.linalg.norm(centers - c, axis=1).min() >= min_separation:
+            centers = np.concatenate([centers, c])
     nbp_l, cut_l, fpts, sol_r, sol_p, ends = [], [], [], [], [], []
     for a in range(nasters):
         dirs = _sphere_directions(fibers_per_aster, dim, rng)
@@ -438,14 +452,15 @@ def asters(dim=3, seed=3, nasters=10, fibers_per_aster=200, lmin=5.0, lmax=10.0,
     sc.sol_radius = sol_r
     sc.sol_timer = rng.integers(0, 7, nasters)
     if nmotors:
-        fa, aa, fb, ab = _bind_crosslinkers_fast(rng, dim, nbp, cut, fiber_pts, nmotors, binding_range)
+        group = np.repeat(np.arange(nasters), fibers_per_aster)
+        fa, aa, fb, ab = _bind_crosslinkers_fast(rng, dim, nbp, cut, fiber_pts, nmotors, binding_range, group)
         sc.link_fiber_a, sc.link_abs_a, sc.link_fiber_b, sc.link_abs_b = fa, aa, fb, ab
         sc.link_stiffness = np.full(fa.size, motor_stiffness)
     # Aster::setInteractions: per fiber, centre -- minus end, then the fiber at
     # abscissa `core` -- its Solid point
     fidx = sc.mat_index
     sidx = sc.sol_index
-    kind, a1, a2, ca, b, w = [], [], [], [], [], []
+    kind, a1, a2, ca, b, w, s1, fl, ab = [], [], [], [], [], [], [], [], []
     fib = np.arange(nfib)
     m1, m2, co = sc.interpolate(fib, np.full(nfib, core))
     for a in range(nasters):
@@ -457,6 +472,12 @@ def asters(dim=3, seed=3, nasters=10, fibers_per_aster=200, lmin=5.0, lmax=10.0,
             ca += [0.0, co[f]]
             b += [fidx[f], sidx[a] + 1 + i]
             w += [stiffness[0], stiffness[1]]
+            s1 += [sidx[a], sidx[a] + 1 + i]
+            fl += [f, f]
+            ab += [0.0, core]
+    sc.aster_s1 = np.array(s1, np.uint32)
+    sc.aster_fiber = np.array(fl, np.uint32)
+    sc.aster_abs = np.array(ab)
     sc.aster_kind = np.array(kind, np.int64)
     sc.aster_a1 = np.array(a1, np.uint32)
     sc.aster_a2 = np.array(a2, np.uint32)
@@ -468,8 +489,9 @@ def asters(dim=3, seed=3, nasters=10, fibers_per_aster=200, lmin=5.0, lmax=10.0,
 
 def config3(seed=3):
     """10 asters x 200 fibers (lengths 5-10 um), 50000 motors (stiffness 100),
-    aster stiffness (1000, 500), in a sphere of radius 12."""
-    return asters(dim=3, seed=seed)
+    aster stiffness (1000, 500), in a sphere of radius 12, aster centres at
+    least 3 um apart within 5 um of the centre."""
+    return asters(dim=3, seed=seed, spread=5.0, min_separation=3.0)
 
 
 def mini_asters(dim=3, seed=11, nasters=2, fibers_per_aster=6, nmotors=12):

@@ -151,6 +151,18 @@ int         cymg_set_couples(cymg_context* ctx, uint32_t n, const uint32_t* fibe
                              const double* abscissa_a, const uint32_t* fiber_b,
                              const double* abscissa_b, const double* stiffness);
 int         cymg_couples_interactions(cymg_context* ctx);
+/* Aster::setInteractions (aster.cc:53-89) for a list of aster links, uploaded
+   by set_aster_links and emitted on the device by aster_interactions, each as
+   the host interLink call it stands for:
+     kind 0: interLink(PointExact(solid point s1), minus end of `fiber`)
+     kind 1: interLink(`fiber` interpolated at abscissa ab from its minus end,
+             PointExact(solid point s1))
+     kind 2: interLink(PointInterpolated(solid points s1, s2, coefficient ab),
+             plus end of `fiber`)  (a fiber shorter than the clamp distance)
+   s1, s2 are matIndex values of Solid points; w the stiffness. */
+int         cymg_set_aster_links(cymg_context* ctx, uint32_t n, const int* kind, const uint32_t* s1,
+                                 const uint32_t* s2, const uint32_t* fiber, const double* ab, const double* w);
+int         cymg_aster_interactions(cymg_context* ctx);
 
 /* Meca::solve (meca.cc:598-845) and its phases:
    setup   (meca.cc:605-693): matrices, forces, Brownian terms, RHS, P'

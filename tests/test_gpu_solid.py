@@ -14,10 +14,11 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-def test_aster_assembly_bit_exact(pkg, scenes, oracle, dim):
+@pytest.mark.parametrize("device_emitters", [True, False])
+def test_aster_assembly_bit_exact(pkg, scenes, oracle, dim, device_emitters):
     sc = scenes.mini_asters(dim=dim)
-    print(f"aster assembly dim={dim}", flush=True)
-    gpu = gpu_from_scene(pkg, sc)
+    print(f"aster assembly dim={dim} device emitters={device_emitters}", flush=True)
+    gpu = gpu_from_scene(pkg, sc, device_emitters)
     orc = oracle_from_scene(oracle, sc)
     g = sc.gauss(0)
     gpu.setup(g)
@@ -113,6 +114,15 @@ def test_aster_step_learned_guess(pkg, scenes, oracle):
     gpu, orc, reps = _aster_steps(pkg, oracle, sc, 2, 1e-12, pkg.PRECOND_FP64, guess=pkg.GUESS_LEARNED,
                                   init_guess=True)
     assert all(r1["guess_used"] == pkg.GUESS_LEARNED for r1, _ in reps)
+    gpu.close()
+
+
+def test_asters_reduced_config3(pkg, scenes, oracle):
+    """The aster scene's shape (Solids of 1 + N points, fibers of 11-21 points,
+    motors) at 3 asters x 40 fibers."""
+    sc = scenes.asters(dim=3, seed=16, nasters=3, fibers_per_aster=40, nmotors=1500, spread=2.0, radius=8.0)
+    gpu, orc, reps = _aster_steps(pkg, oracle, sc, 2, 1e-11, pkg.PRECOND_FP64)
+    assert all(r1["converged"] for r1, _ in reps)
     gpu.close()
 
 
