@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--modes", default="fp64,fp32,learned-fp32",
                     help="fp64 / fp32: zero-seeded BCGSP with FP64 / FP32 preconditioner blocks; "
                          "learned-fp32: seeded by the trained per-fiber network, FP32 blocks")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="run the library's autotuner (tile shape, block-inverse method) before the warmup")
     ap.add_argument("--guess-rollout", type=int, default=32,
                     help="untimed FP64 steps recorded as training samples for the learned guess")
     ap.add_argument("--guess-train-steps", type=int, default=1000)
@@ -111,10 +113,16 @@ def train_guess(pkg, m, sc, args, tol, torch):
             "train_s": round(t2 - t1, 3), "loss_first": first, "loss_last": last}
 
 
-def run_mode(pkg, sc, precond, steps, warmup, tol, gauss_dev, torch, guess=0, train=None):
+def run_mode(pkg, sc, precond, steps, warmup, tol, gauss_dev, torch, guess=0, train=None, autotune=True):
     m = pkg.Meca(sc.dim, 0)
     pkg.load_scene(m, sc)
     iters = []
+    tuned = None
+    if autotune:
+        # untimed: the autotuner picks the tile shape and block-inverse method on this system
+        m.setup(gauss_device_ptr=gauss_dev[0].data_ptr())
+        m.compute_preconditioner(precond if precond else pkg.PRECOND_FP64)
+        tuned = m.autotune(reps=20)
     info = train(m) if train else None
     if train:
         # the timed steps start from the scene's initial state, like the other modes
@@ -141,6 +149,8 @@ def run_mode(pkg, sc, precond, steps, warmup, tol, gauss_dev, torch, guess=0, tr
         one(warmup + s)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    if tuned:
+        info = dict(info or {}, autotune=tuned)
     return m, dt, iters, info
 
 
@@ -168,7 +178,7 @@ def main():
         m, dt, iters, info = run_mode(pkg, sc, precond, args.steps, args.warmup, args.tol, gauss_dev, torch,
                                       guess=pkg.GUESS_LEARNED if learned else pkg.GUESS_ZERO,
                                       train=(lambda mm: train_guess(pkg, mm, sc, args, args.tol, torch))
-                                      if learned else None)
+                                      if learned else None, autotune=bool(args.autotune))
         results[mode] = dict(seconds=dt, steps_per_s=args.steps / dt, ms_per_step=1e3 * dt / args.steps,
                              mean_iters=float(np.mean(iters)), meca=m, training=info)
         print(f"bench: {mode} {args.steps / dt:.2f} steps/s, mean iterations {np.mean(iters):.1f}"
@@ -318,7 +328,7 @@ def main():
                    "tolerance": f"|r|inf/|b|inf<{args.tol:g}"},
         "modes": {k: {"steps_per_s": round(v["steps_per_s"], 3), "ms_per_step": round(v["ms_per_step"], 3),
                       "mean_bicgstab_iters": round(v["mean_iters"], 2),
-                      **({"training": v["training"]} if v["training"] else {})} for k, v in results.items()},
+                      **({"untimed_setup": v["training"]} if v["training"] else {})} for k, v in results.items()},
         "learned_mixed_vs_plain_fp64": (round(results["learned-fp32"]["steps_per_s"] / best["steps_per_s"], 3)
                                         if "learned-fp32" in results else None),
         "roofline": {"bound": "hbm", "kernel": "k_tp/k_tq (one launch of the FP64 BiCGStab iteration)",

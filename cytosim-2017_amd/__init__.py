@@ -35,11 +35,11 @@ EXPORTED = [
     "cymg_setup", "cymg_iterate", "cymg_apply", "cymg_multiply", "cymg_compute_preconditioner",
     "cymg_precondition", "cymg_compute_forces", "cymg_multiply_device", "cymg_precondition_device",
     "cymg_get_vector", "cymg_get_points", "cymg_get_tensions", "cymg_get_block", "cymg_get_matrix",
-    "cymg_get_timings", "cymg_enable_timing", "cymg_time_kernel", "cymg_set_option",
+    "cymg_get_timings", "cymg_enable_timing", "cymg_time_kernel", "cymg_set_option", "cymg_autotune",
     "cymg_guess_init", "cymg_guess_weights", "cymg_guess_record", "cymg_guess_samples",
     "cymg_guess_train", "cymg_guess_check", "cymg_guess_features",
 ]
-OPT_KRYLOV_LAUNCHES, OPT_BLOCK_INVERSE, OPT_TILE_POINTS = 1, 2, 3
+OPT_KRYLOV_LAUNCHES, OPT_BLOCK_INVERSE, OPT_TILE_POINTS, OPT_GRAPHS = 1, 2, 3, 4
 GUESS_INPUTS, GUESS_OUTPUTS, GUESS_POINTS, GUESS_WEIGHTS = 288, 96, 31, 230240
 
 
@@ -153,6 +153,7 @@ def lib():
         "cymg_enable_timing": (C.c_int, [vp, C.c_int]),
         "cymg_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "cymg_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
+        "cymg_autotune": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]),
         "cymg_guess_init": (C.c_int, [vp, C.c_uint64]),
         "cymg_guess_weights": (C.c_int, [vp, C.c_int, vp]),
         "cymg_guess_record": (C.c_int, [vp, u32]),
@@ -427,6 +428,13 @@ class Meca:
 
     def set_option(self, option, value):
         self._chk(self.L.cymg_set_option(self.h, option, value))
+
+    def autotune(self, reps=20):
+        """Pick the block-inverse method and tile shape by timing them on the
+        current system (after setup); returns the choices."""
+        t, b, ms = C.c_int(), C.c_int(), C.c_double()
+        self._chk(self.L.cymg_autotune(self.h, int(reps), C.byref(t), C.byref(b), C.byref(ms)))
+        return {"tile_points": t.value, "block_inverse": b.value, "launch_ms": ms.value}
 
     def time_kernel(self, which, reps=50):
         ms = C.c_double()

@@ -252,12 +252,12 @@ void do_setup(Context& c, const double* gauss, int on_device)
 }
 
 // d_status: [0] fibers whose projection failed, [1] Solids that could not be
-// prepared, [2] Solids that collapsed in the last getPoints
+// prepared, [2] Solids that collapsed in the last getPoints.  prepare queues a
+// copy into pinned memory (h_status); the solve reads it after its first
+// synchronisation with the stream, so no extra round trip stalls the device.
 void check_status(Context& c)
 {
-    int st[3] = { 0, 0, 0 };
-    CYMG_HIP(hipMemcpyAsync(st, c.d_status.get(), sizeof(st), hipMemcpyDeviceToHost, c.stream));
-    CYMG_HIP(hipStreamSynchronize(c.stream));
+    const int* st = c.h_status;
     if (st[0]) throw Error(CYMG_ERR_PROJECTION, "could not build Fiber projection");
     // InvalidParameter / ABORT_NOW of the reference (solid.cc:690-703, 843-846)
     if (st[1]) throw Error(CYMG_ERR_INVALID, "zero drag in Solid, or failed to factorize Solid momentum matrix");
@@ -266,7 +266,6 @@ void check_status(Context& c)
 void do_iterate(Context& c, const cymg_solve_params* p, cymg_solve_report* rep)
 {
     if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_iterate");
-    check_status(c);
     cymg_solve_report r{};
     const int maxit = p->max_iter > 0 ? p->max_iter : (int)c.n();
     const int pc = p->precondition;
@@ -294,6 +293,7 @@ void do_iterate(Context& c, const cymg_solve_params* p, cymg_solve_report* rep)
     {
         Timer t(c, 3);
         krylov_solve(c, pc, guess, maxit, p->tolerance, p->tol_mode, &r);
+        check_status(c);   // the solve has synchronised past the status copy of prepare
         if (r.converged && c.mlp.recording) mlp_record(c);
         int total = r.iterations;
         if (!r.converged && p->fallback) {
@@ -362,6 +362,7 @@ int cymg_create(int dim, int device, cymg_context** out)
         CYMG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         CYMG_HIP(hipHostMalloc((void**)&c.h_scal, sizeof(SysScalars) * 4096, hipHostMallocDefault));
         CYMG_HIP(hipHostMalloc((void**)&c.h_snap, sizeof(SysScalars) * 2 * 4096, hipHostMallocDefault));
+        CYMG_HIP(hipHostMalloc((void**)&c.h_status, 4 * sizeof(int), hipHostMallocDefault));
         for (int i = 0; i < 10; ++i) CYMG_HIP(hipEventCreate(&c.ev[i]));
         for (int i = 0; i < 2; ++i) CYMG_HIP(hipEventCreateWithFlags(&c.chunk_ev[i], hipEventDisableTiming));
     });
@@ -383,8 +384,12 @@ int cymg_destroy(cymg_context* ctx)
         if (c.ev[i]) (void)hipEventDestroy(c.ev[i]);
     if (c.h_scal) (void)hipHostFree(c.h_scal);
     if (c.h_snap) (void)hipHostFree(c.h_snap);
+    if (c.h_status) (void)hipHostFree(c.h_status);
     for (int i = 0; i < 2; ++i)
         if (c.chunk_ev[i]) (void)hipEventDestroy(c.chunk_ev[i]);
+    for (auto& row : c.iter_graph)
+        for (hipGraphExec_t& g : row)
+            if (g) (void)hipGraphExecDestroy(g);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     delete ctx;
     return CYMG_OK;
@@ -1050,12 +1055,60 @@ int cymg_set_option(cymg_context* ctx, int option, int value)
     return guarded(ctx, [&](Context& c) {
         if (option == CYMG_OPT_KRYLOV_LAUNCHES && (value == 2 || value == 4)) c.krylov_fused = value == 2;
         else if (option == CYMG_OPT_BLOCK_INVERSE && value >= 0 && value <= 2) c.pc_method = value;
+        else if (option == CYMG_OPT_GRAPHS && (value == 0 || value == 1)) c.use_graphs = value;
         else if (option == CYMG_OPT_TILE_POINTS && value >= 0 && value <= kMaxFiberPoints) {
             c.tile_points = (uint32_t)value;
             if (!c.layout_dirty && c.nfib) build_tiles(c);
         }
         else throw Error(CYMG_ERR_INVALID, "unknown option or value");
         c.last_iters[0] = c.last_iters[1] = c.last_iters[2] = 0;
+    });
+}
+
+int cymg_autotune(cymg_context* ctx, int reps, int* tile_points, int* block_inverse, double* launch_ms)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (reps <= 0) throw Error(CYMG_ERR_INVALID, "reps must be > 0");
+        if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_autotune");
+        const int precision = c.pc_ready ? c.pc_precision : CYMG_PRECOND_FP64;
+        // block inverse: the faster of Gauss-Jordan in registers and in LDS
+        // (both produce the inverse the iteration reads)
+        const int methods[2] = { 0, 2 };
+        double best_build = 1e300;
+        int best_method = c.pc_method;
+        for (int m : methods) {
+            c.pc_method = m;
+            launch_precond_build(c, precision);
+            CYMG_HIP(hipEventRecord(c.ev[8], c.stream));
+            for (int i = 0; i < 3; ++i) launch_precond_build(c, precision);
+            CYMG_HIP(hipEventRecord(c.ev[9], c.stream));
+            CYMG_HIP(hipEventSynchronize(c.ev[9]));
+            float ms = 0;
+            CYMG_HIP(hipEventElapsedTime(&ms, c.ev[8], c.ev[9]));
+            if (ms < best_build) { best_build = ms; best_method = m; }
+        }
+        c.pc_method = best_method;
+        launch_precond_build(c, precision);
+        // tile shape of the two-launch iteration: the target points per tile
+        // with the shortest launch on the current system
+        int best_tile = (int)c.tile_points;
+        double best = 1e300;
+        if (!c.nsol) {
+            const int cands[] = { 0, 16, 24, 32, 48, 64, 96, 128 };
+            for (int t : cands) {
+                if (t > kMaxFiberPoints) continue;
+                c.tile_points = (uint32_t)t;
+                build_tiles(c);
+                const double ms = krylov_time_launch(c, precision, reps);
+                if (ms < best) { best = ms; best_tile = t; }
+            }
+            c.tile_points = (uint32_t)best_tile;
+            build_tiles(c);
+        }
+        c.last_iters[0] = c.last_iters[1] = c.last_iters[2] = 0;
+        if (tile_points) *tile_points = best_tile;
+        if (block_inverse) *block_inverse = best_method;
+        if (launch_ms) *launch_ms = best < 1e300 ? best : 0.0;
     });
 }
 

@@ -1,15 +1,20 @@
 // Builds the by-value kernel arguments from a context.
 #pragma once
 
+#include <cstring>
+
 #include "context.h"
 #include "fiber_ops.h"
 #include "solid_ops.h"
 
 namespace cymg {
 
+// The argument structs are zeroed first so that their padding bytes are
+// defined: the captured iteration graphs are keyed on the argument bytes.
 inline FiberArgs make_fiber_args(const Context& c)
 {
     FiberArgs a;
+    std::memset(&a, 0, sizeof(a));
     a.np = c.d_np.get();
     a.idx = c.d_idx.get();
     a.seg = c.d_seg.get();
@@ -41,6 +46,7 @@ inline FiberArgs make_fiber_args(const Context& c)
 inline SolidArgs make_solid_args(const Context& c)
 {
     SolidArgs s;
+    std::memset(&s, 0, sizeof(s));
     s.nsol = c.nsol;
     s.p0 = c.fib_pts;
     s.nfib = c.nfib;

@@ -120,6 +120,7 @@ struct Context;
 void build_streams(Context& c);
 // launchers (fiber.hip)
 void launch_prepare(Context& c);
+void launch_fiber_prepare(Context& c);
 void launch_emit_confine(Context& c, uint64_t* keyC, double* valC, uint64_t* keyBas,
                          double* valBas, double radius, double stiffness);
 void launch_emit_couples(Context& c, uint64_t* keyB, double* valB);

@@ -51,6 +51,7 @@ struct Context {
     DevBuf<double> d_pts, d_pos, d_sol, d_bas, d_rhs, d_for, d_tmp, d_prev_sol, d_gauss;
     DevBuf<double> d_diff, d_jd, d_je, d_jd0, d_je0, d_lag, d_pdiff;
     DevBuf<int> d_status;              // [0]: projection failures
+    int* h_status = nullptr;           // pinned copy of d_status, queued at the end of prepare
 
     // assembly streams (mB, mC, vBAS) in call order
     HostStream hB, hC, hBas;
@@ -98,6 +99,11 @@ struct Context {
     SysScalars* h_snap = nullptr;      // pinned snapshots of queued iteration chunks (2 x 4096)
     hipEvent_t chunk_ev[2]{};
     int last_iters[3] = {0, 0, 0};     // iterations of the previous solve, per preconditioner
+    // captured iteration chunks of the two-launch iteration: [preconditioner][2 or 8 iterations],
+    // valid while the launch arguments they were captured with stay the same
+    int use_graphs = 1;
+    hipGraphExec_t iter_graph[3][2]{};
+    std::vector<unsigned char> iter_graph_key[3][2];
 
     bool prepared = false, setup_done = false, pc_ready = false, have_solution = false;
 

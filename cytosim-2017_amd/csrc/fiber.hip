@@ -83,7 +83,12 @@ void launch_prepare(Context& c)
     c.d_status.resize(4);
     CYMG_HIP(hipMemsetAsync(c.d_status.get(), 0, 4 * sizeof(int), c.stream));
     launch_solid_prepare(c);
-    if (!c.nfib) return;
+    if (c.nfib) launch_fiber_prepare(c);
+    CYMG_HIP(hipMemcpyAsync(c.h_status, c.d_status.get(), 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+}
+
+void launch_fiber_prepare(Context& c)
+{
     const dim3 g(c.nfib), b(kWave);
     if (c.dim == 3)
         hipLaunchKernelGGL(k_prepare<3>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_seg.get(), c.d_cut.get(),

@@ -1403,6 +1403,7 @@ __global__ __launch_bounds__(kTpThreads) void k_tq(KArgs k)
 static KArgs make_kargs(Context& c)
 {
     KArgs k;
+    std::memset(&k, 0, sizeof(k));
     k.fa = make_fiber_args(c);
     k.blkoff = c.d_blkoff.get();
     k.useblk = c.d_useblk.get();
@@ -1503,6 +1504,46 @@ static void run(Context& c, const double* guess, int max_iter, double tol, int t
     predicted = c.h_scal[worst].iter;
 }
 
+// Captures (or reuses) the graphs of 2 and 8 iterations of the two-launch
+// iteration for these launch arguments.  Only kernel launches are captured.
+template <int D, int PC>
+static void iteration_graphs(Context& c, const KArgs& P0, const KArgs& Q0, const KArgs& P1, const KArgs& Q1,
+                             dim3 g, dim3 bt)
+{
+    std::vector<unsigned char> key(4 * sizeof(KArgs) + sizeof(dim3));
+    unsigned char* w = key.data();
+    for (const KArgs* a : { &P0, &Q0, &P1, &Q1 }) {
+        std::memcpy(w, a, sizeof(KArgs));
+        w += sizeof(KArgs);
+    }
+    std::memcpy(w, &g, sizeof(dim3));
+    const int iters[2] = { 2, 8 };
+    for (int v = 0; v < 2; ++v) {
+        hipGraphExec_t& exec = c.iter_graph[PC][v];
+        if (exec && c.iter_graph_key[PC][v] == key) continue;
+        if (exec) {
+            CYMG_HIP(hipGraphExecDestroy(exec));
+            exec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        CYMG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        for (int it = 0; it < iters[v]; it += 2) {
+            hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, P0);
+            hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, Q0);
+            hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, P1);
+            hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, Q1);
+        }
+        const hipError_t launch_err = hipGetLastError();
+        const hipError_t end_err = hipStreamEndCapture(c.stream, &graph);
+        CYMG_HIP(launch_err);
+        CYMG_HIP(end_err);
+        const hipError_t inst_err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        CYMG_HIP(inst_err);
+        c.iter_graph_key[PC][v] = key;
+    }
+}
+
 template <int D, int PC>
 static void run_tiled(Context& c, const double* guess, int max_iter, double tol, int tol_mode,
                       cymg_solve_report* rep)
@@ -1542,11 +1583,30 @@ static void run_tiled(Context& c, const double* guess, int max_iter, double tol,
     first_chunk += first_chunk & 1;
     const size_t sbytes = sizeof(SysScalars) * c.nsys;
     SysScalars* snap[2] = { c.h_snap, c.h_snap + 4096 };
+    // every iteration after the first is the same four launches with the same
+    // arguments (the state lives in device memory), so chunks of 2 and 8
+    // iterations are captured once as HIP graphs and replayed
+    hipGraphExec_t* graphs = nullptr;
+    if (c.use_graphs) {
+        graphs = c.iter_graph[PC];
+        iteration_graphs<D, PC>(c, kP0, kQ0, kP1, kQ1, g, bt);
+    }
     auto enqueue = [&](int n, int slot) {
-        for (int it = 0; it < n; it += 2) {
-            if (first_p) hipLaunchKernelGGL((k_tp<D, PC, true>), g, bt, 0, c.stream, kP0);
-            else hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, kP0);
+        int it = 0;
+        if (first_p) {
+            hipLaunchKernelGGL((k_tp<D, PC, true>), g, bt, 0, c.stream, kP0);
+            hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ0);
+            hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, kP1);
+            hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ1);
             first_p = false;
+            it = 2;
+        }
+        if (graphs) {
+            for (; it + 8 <= n; it += 8) CYMG_HIP(hipGraphLaunch(graphs[1], c.stream));
+            for (; it + 2 <= n; it += 2) CYMG_HIP(hipGraphLaunch(graphs[0], c.stream));
+        }
+        for (; it < n; it += 2) {
+            hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, kP0);
             hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ0);
             hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, kP1);
             hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ1);

@@ -262,3 +262,59 @@ def test_convergence_failure_raises(pkg, scenes, oracle):
     orc = oracle_from_scene(oracle, sc)
     with pytest.raises(RuntimeError):
         orc.solve(1e-14, sc.gauss(0), tol_mode=1, max_iter=1)
+
+
+@pytest.mark.parametrize("precondition", [1, 2, 0])
+def test_graph_replay_matches_eager(pkg, scenes, precondition):
+    """Chunks of the two-launch iteration replayed from captured HIP graphs give
+    the same bits as the same launches issued one by one, over several steps
+    (graphs replayed many times, with the eager setup/assembly work between)."""
+    sc = scenes.config2(seed=26, nfib=90, nlinks=500, radius=4.0)
+    out = []
+    for graphs in (1, 0):
+        gpu = gpu_from_scene(pkg, sc)
+        gpu.set_option(pkg.OPT_GRAPHS, graphs)
+        reps = []
+        for s in range(4):
+            print(f"graph replay precond={precondition} graphs={graphs} step={s}", flush=True)
+            if s:
+                gpu.prepare(None)
+                pkg.assemble(gpu, sc, upload_couples=False)
+            reps.append(gpu.solve(1e-8, sc.gauss(s), tol_mode=pkg.TOL_RELATIVE, precondition=precondition))
+        out.append((gpu.fiber_points(), [r["iterations"] for r in reps]))
+        gpu.close()
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
+
+
+def test_autotune_then_step_matches_oracle(pkg, scenes, oracle):
+    sc = scenes.config2(seed=27, nfib=120, nlinks=500, radius=5.0)
+    gpu = gpu_from_scene(pkg, sc)
+    gpu.setup(sc.gauss(0))
+    gpu.compute_preconditioner(pkg.PRECOND_FP64)
+    print("autotune", flush=True)
+    t = gpu.autotune(reps=6)
+    assert t["tile_points"] in (0, 16, 24, 32, 48, 64, 96, 128)
+    assert t["block_inverse"] in (0, 2) and t["launch_ms"] > 0
+    r1 = gpu.solve(1e-11, sc.gauss(0), tol_mode=pkg.TOL_RELATIVE)
+    orc = oracle_from_scene(oracle, sc)
+    orc.solve(1e-11, sc.gauss(0), tol_mode=1)
+    assert r1["converged"]
+    assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-9
+    gpu.close()
+
+
+def test_projection_failure_raises(pkg):
+    """A fiber whose points coincide has a singular JJ' (dpttrf fails):
+    "could not build Fiber projection" (rigid_fiber_project.cc:84-85)."""
+    m = pkg.Meca(3, 0)
+    m.add_fiber(4, 0.5, 20.0)
+    m.add_fiber(3, 0.5, 20.0)
+    m.set_params(time_step=0.005, kT=0.0042, viscosity=0.02)
+    pts = np.zeros(3 * 7)
+    pts[3 * 4:] = np.array([0, 0, 1, 0.5, 0, 1, 1.0, 0, 1])
+    m.prepare(pts)
+    m.setup(None)
+    with pytest.raises(pkg.ProjectionFailure):
+        m.iterate(1e-6, tol_mode=pkg.TOL_RELATIVE)
+    m.close()
