@@ -26,7 +26,7 @@ VEC_PTS, VEC_SOL, VEC_BAS, VEC_RHS, VEC_FOR = range(5)
 
 EXPORTED = [
     "cymg_create", "cymg_destroy", "cymg_last_error", "cymg_set_params", "cymg_clear",
-    "cymg_add_fiber", "cymg_add_solid", "cymg_nb_points", "cymg_set_systems", "cymg_get_system_reports",
+    "cymg_add_fiber", "cymg_add_solid", "cymg_get_solid_timers", "cymg_nb_points", "cymg_set_systems", "cymg_get_system_reports",
     "cymg_prepare", "cymg_add_pure_force",
     "cymg_inter_link_ee", "cymg_inter_link_ie", "cymg_inter_link_ii", "cymg_inter_long_link_ee",
     "cymg_inter_clamp_e", "cymg_inter_clamp_i", "cymg_inter_long_clamp", "cymg_inter_plane_e",
@@ -116,6 +116,7 @@ def lib():
         "cymg_clear": (C.c_int, [vp]),
         "cymg_add_fiber": (C.c_int, [vp, u32, C.c_double, C.c_double, C.POINTER(u32)]),
         "cymg_add_solid": (C.c_int, [vp, u32, vp, vp, C.c_int, C.POINTER(u32)]),
+        "cymg_get_solid_timers": (C.c_int, [vp, u32, vp]),
         "cymg_nb_points": (C.c_int, [vp, C.POINTER(u32)]),
         "cymg_set_systems": (C.c_int, [vp, u32, vp]),
         "cymg_get_system_reports": (C.c_int, [vp, u32, vp]),
@@ -243,6 +244,12 @@ class Meca:
         self.np.append(int(r.size))
         self.nsol = getattr(self, "nsol", 0) + 1
         return mi.value
+
+    def solid_timers(self):
+        n = getattr(self, "nsol", 0)
+        out = np.zeros(n, np.int32)
+        self._chk(self.L.cymg_get_solid_timers(self.h, n, _p(out) if n else None))
+        return out
 
     def set_systems(self, first_fiber):
         ff = np.ascontiguousarray(first_fiber, dtype=np.uint32)

@@ -496,6 +496,20 @@ int cymg_add_solid(cymg_context* ctx, uint32_t np, const double* radius, const d
     });
 }
 
+int cymg_get_solid_timers(cymg_context* ctx, uint32_t nsol, int* timers)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (nsol != c.nsol || (nsol && !timers)) throw Error(CYMG_ERR_INVALID, "nsol must match the Solids registered");
+        if (!nsol) return;
+        if (c.layout_dirty) {
+            std::copy(c.h_sol_timer.begin(), c.h_sol_timer.end(), timers);
+            return;
+        }
+        CYMG_HIP(hipMemcpyAsync(timers, c.d_sol_timer.get(), nsol * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        CYMG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
 int cymg_set_systems(cymg_context* ctx, uint32_t nsys, const uint32_t* first_fiber)
 {
     return guarded(ctx, [&](Context& c) {

@@ -1,11 +1,14 @@
-// Runs implicit steps of a fiber scene through the C++ Meca interface, the way
-// Simul::solve drives the reference's Meca (simul_solve.cc:199-253): clear,
-// add every fiber, prepare, confinement, couples, solve.
+// Runs implicit steps of a fiber (and aster) scene through the C++ Meca
+// interface, the way Simul::solve drives the reference's Meca
+// (simul_solve.cc:199-253): clear, add every fiber then every Solid, prepare,
+// confinement, couples, asters, solve.
 // usage: drive_meca scene.txt out.txt
-// scene.txt: dim nfib nlinks nsteps / time_step kT viscosity tolerance tol_mode
-//            radius stiffness / per fiber: np cut rigidity then DIM*np points /
+// scene.txt: dim nfib nlinks nsteps nsol naster / time_step kT viscosity
+//            tolerance tol_mode radius stiffness / per fiber: np cut rigidity
+//            then DIM*np points / per Solid: np timer, np radii, DIM*np points /
 //            per link: fiber_a abscissa_a fiber_b abscissa_b stiffness /
-//            per step: DIM*nbPts gaussian draws
+//            per aster link: kind s1 fiber abscissa stiffness (s1: matIndex of
+//            the Solid point) / per step: DIM*nbPts gaussian draws
 // out.txt:   per step "iterations residual", then the final points.
 #include <cstdio>
 #include <fstream>
@@ -21,10 +24,10 @@ int main(int argc, char** argv)
         return 2;
     }
     std::ifstream in(argv[1]);
-    int dim, nfib, nlinks, nsteps, tol_mode;
+    int dim, nfib, nlinks, nsteps, nsol, nast, tol_mode;
     cymeca::SimulProp prop;
     double radius, stiff;
-    in >> dim >> nfib >> nlinks >> nsteps;
+    in >> dim >> nfib >> nlinks >> nsteps >> nsol >> nast;
     in >> prop.time_step >> prop.kT >> prop.viscosity >> prop.tolerance >> tol_mode >> radius >> stiff;
     std::vector<cymeca::Fiber> fibers(nfib);
     for (auto& f : fibers) {
@@ -33,9 +36,23 @@ int main(int argc, char** argv)
         f.points.resize((size_t)dim * np);
         for (double& x : f.points) in >> x;
     }
+    std::vector<cymeca::Solid> solids(nsol);
+    for (auto& s : solids) {
+        unsigned np;
+        in >> np >> s.reshapeTimer;
+        s.radius.resize(np);
+        for (double& r : s.radius) in >> r;
+        s.points.resize((size_t)dim * np);
+        for (double& x : s.points) in >> x;
+        s.shape = s.points;   // Solid::fixShape when built
+    }
     std::vector<uint32_t> fa(nlinks), fb(nlinks);
     std::vector<double> aa(nlinks), ab(nlinks), w(nlinks);
     for (int k = 0; k < nlinks; ++k) in >> fa[k] >> aa[k] >> fb[k] >> ab[k] >> w[k];
+    std::vector<int> akind(nast);
+    std::vector<uint32_t> as1(nast), af(nast);
+    std::vector<double> aab(nast), aw(nast);
+    for (int k = 0; k < nast; ++k) in >> akind[k] >> as1[k] >> af[k] >> aab[k] >> aw[k];
     if (!in) {
         std::fprintf(stderr, "bad scene file\n");
         return 2;
@@ -46,21 +63,31 @@ int main(int argc, char** argv)
         cymeca::Meca meca(dim, 0);
         size_t npts = 0;
         for (auto& f : fibers) npts += f.nbPoints(dim);
+        for (auto& s : solids) npts += s.nbPoints();
         std::vector<double> gauss((size_t)dim * npts);
         for (int step = 0; step < nsteps; ++step) {
             for (double& g : gauss) in >> g;
             meca.clear();
             for (auto& f : fibers) meca.add(&f);
+            for (auto& s : solids) meca.add(&s);
             meca.prepare(&prop);
             if (radius > 0) meca.confineSphere(radius, stiff);
             for (int k = 0; k < nlinks; ++k)
                 meca.interLink(cymeca::interpolate(fibers[fa[k]], dim, aa[k]),
                                cymeca::interpolate(fibers[fb[k]], dim, ab[k]), w[k]);
+            // Aster::setInteractions (aster.cc:53-89)
+            for (int k = 0; k < nast; ++k) {
+                const cymeca::Fiber& f = fibers[af[k]];
+                if (akind[k] == 0) meca.interLink(cymeca::PointExact{ as1[k] }, cymeca::PointExact{ f.matIndex }, aw[k]);
+                else meca.interLink(cymeca::interpolate(f, dim, aab[k]), cymeca::PointExact{ as1[k] }, aw[k]);
+            }
             cymg_solve_report rep = meca.solve(&prop, prop.precondition != 0, gauss.data(), tol_mode);
             out << rep.iterations << " " << rep.residual << "\n";
         }
         for (auto& f : fibers)
             for (double x : f.points) out << x << "\n";
+        for (auto& s : solids)
+            for (double x : s.points) out << x << "\n";
     } catch (const cymeca::Exception& e) {
         std::fprintf(stderr, "error %d: %s\n", e.code, e.what());
         return 1;

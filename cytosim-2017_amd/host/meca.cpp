@@ -23,6 +23,7 @@ void Meca::clear()
 {
     check(cymg_clear(ctx_));
     objs_.clear();
+    solids_.clear();
     nbPts_ = 0;
 }
 
@@ -35,6 +36,15 @@ void Meca::add(Fiber* f)
     nbPts_ += f->nbPoints(dim_);
 }
 
+void Meca::add(Solid* s)
+{
+    uint32_t mi = 0;
+    check(cymg_add_solid(ctx_, s->nbPoints(), s->radius.data(), s->shape.data(), s->reshapeTimer, &mi));
+    s->matIndex = mi;
+    solids_.push_back(s);
+    nbPts_ += s->nbPoints();
+}
+
 void Meca::prepare(const SimulProp* prop)
 {
     cymg_sim_params p{ prop->time_step, prop->kT, prop->viscosity, hr0_, hr1_ };
@@ -43,6 +53,8 @@ void Meca::prepare(const SimulProp* prop)
     std::vector<double> pts(size());
     for (Fiber* f : objs_)
         std::copy(f->points.begin(), f->points.end(), pts.begin() + (size_t)dim_ * f->matIndex);
+    for (Solid* s : solids_)
+        std::copy(s->points.begin(), s->points.end(), pts.begin() + (size_t)dim_ * s->matIndex);
     check(cymg_prepare(ctx_, pts.data()));
 }
 
@@ -108,15 +120,33 @@ void Meca::couplesInteractions()
     check(cymg_couples_interactions(ctx_));
 }
 
+void Meca::setAsterLinks(const std::vector<int>& kind, const std::vector<uint32_t>& s1,
+                         const std::vector<uint32_t>& s2, const std::vector<uint32_t>& fiber,
+                         const std::vector<double>& ab, const std::vector<double>& w)
+{
+    check(cymg_set_aster_links(ctx_, (uint32_t)kind.size(), kind.data(), s1.data(), s2.data(), fiber.data(),
+                               ab.data(), w.data()));
+}
+
+void Meca::asterInteractions()
+{
+    check(cymg_aster_interactions(ctx_));
+}
+
+void Meca::autotune(int reps)
+{
+    check(cymg_autotune(ctx_, reps, nullptr, nullptr, nullptr));
+}
+
 cymg_solve_report Meca::solve(const SimulProp* prop, bool precondition, const double* gauss, int tol_mode,
-                              int precision)
+                              int precision, int guess)
 {
     cymg_solve_params p{};
     p.tolerance = prop->tolerance;
     p.tol_mode = tol_mode;
     p.precondition = precondition ? precision : CYMG_PRECOND_NONE;
     p.max_iter = 0;
-    p.guess = CYMG_GUESS_ZERO;
+    p.guess = guess;
     p.fallback = 1;
     p.gauss_on_device = 0;
     cymg_solve_report rep{};
@@ -152,6 +182,15 @@ void Meca::getPoints()
     for (Fiber* f : objs_)
         std::copy(pts.begin() + (size_t)dim_ * f->matIndex,
                   pts.begin() + (size_t)dim_ * (f->matIndex + f->nbPoints(dim_)), f->points.begin());
+    for (Solid* s : solids_)
+        std::copy(pts.begin() + (size_t)dim_ * s->matIndex,
+                  pts.begin() + (size_t)dim_ * (s->matIndex + s->nbPoints()), s->points.begin());
+    // Solid::getPoints advanced each Solid's reshape timer on the device
+    if (!solids_.empty()) {
+        std::vector<int> t(solids_.size());
+        check(cymg_get_solid_timers(ctx_, (uint32_t)t.size(), t.data()));
+        for (size_t k = 0; k < t.size(); ++k) solids_[k]->reshapeTimer = t[k];
+    }
 }
 
 }  // namespace cymeca

@@ -4,10 +4,11 @@
 // precondition() expose the operator.  Everything runs on the GPU through the
 // C ABI of include/cymeca.h; this class owns one cymg_context.
 //
-// A Mecable here is a fiber described by its points, segmentation and
-// rigidity (the RigidFiber state Meca reads through putPoints/prepareMecable);
-// points come back through getPoints() after the solve, reshaped to equal
-// segment lengths as FiberNaked::getPoints does.
+// Mecables here are fibers, described by their points, segmentation and
+// rigidity (the RigidFiber state Meca reads through putPoints/prepareMecable),
+// and Solids (points, per-point radii, the reference shape fixed at
+// construction and the reshape timer).  Points come back through getPoints()
+// after the solve, reshaped as FiberNaked::getPoints / Solid::getPoints do.
 #pragma once
 
 #include <cstdint>
@@ -51,6 +52,17 @@ struct Fiber {
     uint32_t nbPoints(int dim) const { return (uint32_t)(points.size() / dim); }
 };
 
+// Solid (solid.h): soRadius per point, soShape fixed by fixShape() when built,
+// soReshapeTimer; registered after every fiber
+struct Solid {
+    std::vector<double> points;   // DIM * nbPoints, current
+    std::vector<double> shape;    // DIM * nbPoints, the points at fixShape() time
+    std::vector<double> radius;   // per point (0: no drag)
+    int reshapeTimer = 0;
+    uint32_t matIndex = 0;        // set by Meca::add
+    uint32_t nbPoints() const { return (uint32_t)radius.size(); }
+};
+
 // FiberNaked::interpolate (fiber_naked.cc:1050-1065), abscissa from the minus end
 inline PointInterpolated interpolate(const Fiber& f, int dim, double ab)
 {
@@ -72,7 +84,8 @@ public:
 
     void clear();                           // meca.cc:55-60
     void add(Fiber* f);                     // meca.cc:67-79
-    unsigned nbMecables() const { return (unsigned)objs_.size(); }
+    void add(Solid* s);                     // meca.cc:67-79, after every fiber
+    unsigned nbMecables() const { return (unsigned)(objs_.size() + solids_.size()); }
     unsigned nbPoints() const { return nbPts_; }
     unsigned size() const { return dim_ * nbPts_; }
 
@@ -94,10 +107,19 @@ public:
                     const std::vector<uint32_t>& fb, const std::vector<double>& ab,
                     const std::vector<double>& stiffness);
     void couplesInteractions();                                   // Couple::setInteractions
+    // Aster::setInteractions (aster.cc:53-89): kind 0 interLink(solid point s1, fiber minus end),
+    // kind 1 interLink(fiber at abscissa, solid point s1), kind 2 interLink(solid s1-s2 at coef, plus end)
+    void setAsterLinks(const std::vector<int>& kind, const std::vector<uint32_t>& s1,
+                       const std::vector<uint32_t>& s2, const std::vector<uint32_t>& fiber,
+                       const std::vector<double>& abscissa, const std::vector<double>& stiffness);
+    void asterInteractions();
 
     // meca.cc:598-845; gauss: N(0,1) per degree of freedom (nullptr: no Brownian term)
     cymg_solve_report solve(const SimulProp* prop, bool precondition, const double* gauss = nullptr,
-                            int tol_mode = CYMG_TOL_REFERENCE, int precond_precision = CYMG_PRECOND_FP64);
+                            int tol_mode = CYMG_TOL_REFERENCE, int precond_precision = CYMG_PRECOND_FP64,
+                            int guess = CYMG_GUESS_ZERO);
+    // the library's autotuner on the current system (after a solve or setup)
+    void autotune(int reps = 20);
 
     void computeForces();                                         // meca.cc:550-580
     void multiply(const double* X, double* Y) const;              // meca.cc:157-181
@@ -105,7 +127,7 @@ public:
 
     // export after solve: vPTS/vFOR and the Mecable points (getPoints)
     void getVector(int which, double* out) const;
-    void getPoints();                                             // fills Fiber::points
+    void getPoints();                                             // fills the Mecables' points
 
     cymg_context* context() const { return ctx_; }
 
@@ -114,6 +136,7 @@ private:
     int dim_;
     cymg_context* ctx_ = nullptr;
     std::vector<Fiber*> objs_;
+    std::vector<Solid*> solids_;
     unsigned nbPts_ = 0;
     double hr0_, hr1_;
 };

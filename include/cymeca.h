@@ -106,6 +106,9 @@ int         cymg_nb_points(const cymg_context* ctx, uint32_t* nb_points);
    four-launch iteration and may not be split into independent systems. */
 int         cymg_add_solid(cymg_context* ctx, uint32_t nb_points, const double* radius, const double* points,
                            int reshape_timer, uint32_t* mat_index);
+/* soReshapeTimer of every Solid after the last getPoints (the caller's Solid
+   objects keep it across Meca::clear/add, as the reference's Solids do) */
+int         cymg_get_solid_timers(cymg_context* ctx, uint32_t nsol, int* timers);
 /* independent systems solved together (no counterpart in the reference, which
    solves one system per Meca): fibers [first_fiber[s], first_fiber[s+1]) form
    system s, with its own Krylov scalars, threshold (noise level or |b|_inf) and
