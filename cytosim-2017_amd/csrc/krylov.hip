@@ -894,7 +894,27 @@ __device__ __forceinline__ double gath_sum(const Gath<PRE, Rec>& G, const Rec* _
 #pragma unroll
     for (int v = 0; v < PRE; ++v)
         if (G.e + v < G.e1) acc += G.val[v] * X(G.g[v]);
-    for (uint32_t e = G.e + PRE; e < G.e1; ++e) acc += val[e] * X(rec[col[e] * mul + add]);
+    // entries beyond the prefetched ones, a few at a time: the column and value
+    // loads of a batch, then its record loads, are issued together (addresses
+    // clamped to the share's last entry, weights of the clamped ones zero) so
+    // a long row costs one memory round trip per batch instead of per entry
+    constexpr int CH = sizeof(Rec) > 16 ? 2 : 4;   // registers of the batch's records
+    const uint32_t el = G.e1 - 1;
+    for (uint32_t e = G.e + PRE; e < G.e1; e += CH) {
+        uint32_t c[CH];
+        double w[CH];
+#pragma unroll
+        for (int v = 0; v < CH; ++v) {
+            const uint32_t q = e + v < el ? e + v : el;
+            c[v] = col[q];
+            w[v] = e + v < G.e1 ? val[q] : 0.0;
+        }
+        Rec r[CH];
+#pragma unroll
+        for (int v = 0; v < CH; ++v) r[v] = rec[c[v] * mul + add];
+#pragma unroll
+        for (int v = 0; v < CH; ++v) acc += w[v] * X(r[v]);
+    }
     return acc;
 }
 
@@ -1119,11 +1139,26 @@ __device__ __forceinline__ double tp_block_apply(LDS& L, const Tile& T, const Tp
     double acc[4] = { 0, 0, 0, 0 };
     const int k = R.k, fo = R.fo, bs = D * R.np;
     const bool use = R.act && B != nullptr && L.fuse[k];
-    if (use) {
+    if (use && R.j1 > R.j0) {
+        // FP64 entries: branch-free, every operand read stays inside the
+        // fiber's rows (the index is clamped to the last column of this
+        // thread's share) and is multiplied by a zero entry beyond it, so the
+        // LDS reads of the loop are in flight together instead of one per
+        // branch (16.1 instead of 17.5 us per launch on the 1000-fiber system).
         const double* xs = xin + D * fo;
+        const uint32_t jl = R.j1 - 1;
+        if constexpr (std::is_same<TB, double>::value) {
 #pragma unroll
-        for (int v = 0; v < kTpBlk; ++v)
-            if (R.j0 + v < R.j1) acc[v & 3] += (double)R.blk[v] * xs[R.j0 + v];
+            for (int v = 0; v < kTpBlk; ++v) {
+                const uint32_t j = R.j0 + v < jl ? R.j0 + v : jl;
+                acc[v & 3] += R.blk[v] * xs[j];
+            }
+        } else {
+            // FP32 entries: the predicated form measures faster here
+#pragma unroll
+            for (int v = 0; v < kTpBlk; ++v)
+                if (R.j0 + v < R.j1) acc[v & 3] += (double)R.blk[v] * xs[R.j0 + v];
+        }
         const TB* col = B + L.fblk[k] + (R.u - D * fo);
         for (uint32_t j = R.j0 + kTpBlk; j < R.j1; j += kApplyChunk) {
             TB c[kApplyChunk];
@@ -1131,8 +1166,10 @@ __device__ __forceinline__ double tp_block_apply(LDS& L, const Tile& T, const Tp
             for (int v = 0; v < kApplyChunk; ++v)
                 c[v] = (j + v < R.j1) ? __builtin_nontemporal_load(col + (size_t)bs * (j + v)) : TB(0);
 #pragma unroll
-            for (int v = 0; v < kApplyChunk; ++v)
-                if (j + v < R.j1) acc[v & 3] += (double)c[v] * xs[j + v];
+            for (int v = 0; v < kApplyChunk; ++v) {
+                const uint32_t jj = j + v < jl ? j + v : jl;
+                acc[v & 3] += (double)c[v] * xs[jj];
+            }
         }
     }
     const double part = (acc[0] + acc[1]) + (acc[2] + acc[3]);
