@@ -228,11 +228,29 @@ __device__ __forceinline__ void fiber_forces_op(const FiberArgs& a, FiberLds<D>&
 #pragma unroll
         for (int d = 0; d < D; ++d) acc[d] = 0.0;
         const uint32_t e1 = a.Brow[r + 1];
-        for (uint32_t q = a.Brow[r] + threadIdx.x; q < e1; q += blockDim.x) {
-            const double v = a.Bval[q];
-            const uint32_t c = a.Bcol[q];
+        // four entries per thread per round: their loads are issued together
+        constexpr uint32_t U = 4;
+        const uint32_t stride = blockDim.x;
+        const uint32_t el = e1 - 1;
+        for (uint32_t q0 = a.Brow[r] + threadIdx.x; q0 < e1; q0 += U * stride) {
+            double v[U];
+            uint32_t c[U];
 #pragma unroll
-            for (int d = 0; d < D; ++d) acc[d] += v * X(D * c + d);
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t q = q0 + u * stride;
+                const uint32_t qc = q < el ? q : el;
+                v[u] = q < e1 ? a.Bval[qc] : 0.0;
+                c[u] = a.Bcol[qc];
+            }
+            double xv[U][D];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+#pragma unroll
+                for (int d = 0; d < D; ++d) xv[u][d] = X(D * c[u] + d);
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+#pragma unroll
+                for (int d = 0; d < D; ++d) acc[d] += v[u] * xv[u][d];
         }
 #pragma unroll
         for (int d = 0; d < D; ++d) acc[d] = wave_sum(acc[d]);
