@@ -37,7 +37,7 @@ EXPORTED = [
     "cymg_get_vector", "cymg_get_points", "cymg_get_tensions", "cymg_get_block", "cymg_get_matrix",
     "cymg_get_timings", "cymg_enable_timing", "cymg_time_kernel", "cymg_set_option", "cymg_autotune",
     "cymg_guess_init", "cymg_guess_weights", "cymg_guess_record", "cymg_guess_samples",
-    "cymg_guess_train", "cymg_guess_check", "cymg_guess_features",
+    "cymg_guess_train", "cymg_guess_check", "cymg_guess_features", "cymg_guess_predict",
 ]
 OPT_KRYLOV_LAUNCHES, OPT_BLOCK_INVERSE, OPT_TILE_POINTS, OPT_GRAPHS = 1, 2, 3, 4
 GUESS_INPUTS, GUESS_OUTPUTS, GUESS_POINTS, GUESS_WEIGHTS = 288, 96, 31, 230240
@@ -162,6 +162,7 @@ def lib():
         "cymg_guess_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_uint64, C.POINTER(C.c_double)]),
         "cymg_guess_check": (C.c_int, [vp, vp, vp, C.c_int, C.POINTER(C.c_double), vp]),
         "cymg_guess_features": (C.c_int, [vp, vp, vp]),
+        "cymg_guess_predict": (C.c_int, [vp, C.c_int, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -495,6 +496,14 @@ class Meca:
         s = np.empty(nf, np.float32)
         self._chk(self.L.cymg_guess_features(self.h, _p(X), _p(s)))
         return X, s
+
+    def guess_predict(self, precondition=PRECOND_FP64):
+        """(learned guess x0, its base M^-1 b) of the current system."""
+        n = self.size
+        x0 = np.empty(n)
+        base = np.empty(n)
+        self._chk(self.L.cymg_guess_predict(self.h, precondition, _p(x0), _p(base)))
+        return x0, base
 
 
 def load_batch(meca, batch, couples_on_device=True):

@@ -1068,7 +1068,7 @@ int cymg_set_option(cymg_context* ctx, int option, int value)
 {
     return guarded(ctx, [&](Context& c) {
         if (option == CYMG_OPT_KRYLOV_LAUNCHES && (value == 2 || value == 4)) c.krylov_fused = value == 2;
-        else if (option == CYMG_OPT_BLOCK_INVERSE && value >= 0 && value <= 2) c.pc_method = value;
+        else if (option == CYMG_OPT_BLOCK_INVERSE && value >= 0 && value <= 3) c.pc_method = value;
         else if (option == CYMG_OPT_GRAPHS && (value == 0 || value == 1)) c.use_graphs = value;
         else if (option == CYMG_OPT_TILE_POINTS && value >= 0 && value <= kMaxFiberPoints) {
             c.tile_points = (uint32_t)value;
@@ -1085,9 +1085,9 @@ int cymg_autotune(cymg_context* ctx, int reps, int* tile_points, int* block_inve
         if (reps <= 0) throw Error(CYMG_ERR_INVALID, "reps must be > 0");
         if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_autotune");
         const int precision = c.pc_ready ? c.pc_precision : CYMG_PRECOND_FP64;
-        // block inverse: the faster of Gauss-Jordan in registers and in LDS
-        // (both produce the inverse the iteration reads)
-        const int methods[2] = { 0, 2 };
+        // block inverse: the fastest of Gauss-Jordan in registers, in LDS and
+        // blocked on the matrix cores (all produce the inverse the iteration reads)
+        const int methods[3] = { 0, 2, 3 };
         double best_build = 1e300;
         int best_method = c.pc_method;
         for (int m : methods) {
@@ -1229,6 +1229,21 @@ int cymg_guess_features(cymg_context* ctx, float* X, float* scale)
         if (!X || !scale) throw Error(CYMG_ERR_INVALID, "null output");
         if (!c.mlp.step_ready) throw Error(CYMG_ERR_INVALID, "no step features computed");
         mlp_features(c, X, scale);
+    });
+}
+
+int cymg_guess_predict(cymg_context* ctx, int precondition, double* x0, double* base)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!x0) throw Error(CYMG_ERR_INVALID, "null output");
+        if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_guess_predict");
+        if (precondition < 0 || precondition > 2) throw Error(CYMG_ERR_INVALID, "precondition must be 0, 1 or 2");
+        if (!c.mlp.ready) throw Error(CYMG_ERR_UNSUPPORTED, "learned guess not trained or loaded");
+        if (precondition) launch_precond_build(c, precondition);
+        mlp_prepare_step(c, precondition);
+        const double* g = mlp_guess(c);
+        d2h(c, x0, g, c.n() * 8);
+        if (base) d2h(c, base, c.mlp.y.get(), c.n() * 8);
     });
 }
 

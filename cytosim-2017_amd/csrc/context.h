@@ -79,7 +79,8 @@ struct Context {
 
     // preconditioner
     int pc_precision = 0;              // CYMG_PRECOND_*
-    int pc_method = 0;                 // 0 Gauss-Jordan in registers, 1 getrf + getri, 2 Gauss-Jordan in LDS
+    int pc_method = 3;                 // 0 Gauss-Jordan in registers, 1 getrf + getri, 2 Gauss-Jordan in LDS,
+                                       // 3 blocked Gauss-Jordan with matrix-core panel updates
     DevBuf<uint32_t> d_gj_list;        // fibers by block-size class
     std::vector<uint32_t> gj_start;    // class boundaries in d_gj_list
     uint32_t gj_nfib = 0, gj_largest = 0;

@@ -491,8 +491,215 @@ __global__ __launch_bounds__(NW * kWave) void k_block_gj(const uint32_t* __restr
     if (i == 0) useblk[f] = (info == 0);
 }
 
+// ---------------------------------------------------------------------------
+// Blocked Gauss-Jordan inverse on the matrix cores.  One workgroup of 4
+// wavefronts per block; the block lives in LDS, column-major with leading
+// dimension ld (odd), padded to nb = 16*ceil(bs/16) <= 96 with an identity
+// tail.  For each panel K of 16 columns:
+//  * wavefront 0 runs the panel's elimination steps on the panel columns only,
+//    rows in registers (lanes l and l + 64).  Pivot choice as k_block_gj
+//    (largest FP32-truncated magnitude among the rows not yet pivot, lowest
+//    row on ties; no interchanges, the pivot row of each step is recorded);
+//  * the other columns J receive the panel's steps at once as a product on
+//    the matrix cores (v_mfma_{f64,f32}_16x16x4, 16x16 output tiles, K = 16 in
+//    4 steps):  A[:, J] <- Z A[:, J] + X A[P, J],  where X = A[:, K] after the
+//    panel steps, P the panel's pivot rows in step order and Z the identity
+//    with the rows P zeroed (blocked Gauss-Jordan inversion: Quintana-Orti,
+//    Quintana-Orti, Sun and van de Geijn, SIAM J. Sci. Comput. 23 (2001)).
+// After the last panel the row that was pivot of step s holds row s of
+// inv(A), its column t at the pivot row of step t; the result is written in
+// natural order, coalesced.  A zero pivot marks the block unusable
+// (meca.cc:437-438).  Reads the assembled FP64 block from M; writes the
+// inverse (type T, factorised in T) to out at the same offset.
+constexpr int kInvNB = 96;
+constexpr int kInvThreads = 256;
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+// Output layout of the 16x16x4 products (measured on gfx950): lane l holds
+// column l % 16; its register r holds row 4 (l / 16) + r for FP32 and row
+// l / 16 + 4 r for FP64.
+template <typename T> struct MfmaVec;
+template <> struct MfmaVec<double> {
+    typedef f64x4 type;
+    static __device__ __forceinline__ int row(int g, int r) { return g + 4 * r; }
+};
+template <> struct MfmaVec<float> {
+    typedef f32x4v type;
+    static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
+};
+
+__device__ __forceinline__ f64x4 mfma_16x16x4(double a, double b, f64x4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4v mfma_16x16x4(float a, float b, f32x4v c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// The elimination steps of one panel (columns k .. k + steps - 1) in
+// wavefront 0.  TWO: rows 64..nb-1 exist (held in v1).  Returns false on a
+// zero pivot (sets *info).
+template <typename T, bool TWO>
+__device__ __forceinline__ bool inv_panel(T* A, int ld, int nb, int bs, int k, int steps, unsigned& used, int* perm,
+                                          int* stepof, int* info)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool row0 = lane < nb, row1 = TWO && lane + 64 < nb;   // the lane's rows inside the block
+    T v0[16], v1[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        v0[c] = row0 ? A[lane + ld * (k + c)] : T(0);
+        v1[c] = row1 ? A[lane + 64 + ld * (k + c)] : T(0);
+    }
+    // fully unrolled (every register index a constant): no early exit, the
+    // steps past `steps` or after a zero pivot are skipped by a uniform test
+    bool ok = true;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        if (!ok || t >= steps) continue;
+        const T a0 = v0[t], a1 = v1[t];
+        unsigned k0 = 0, k1 = 0;
+        if (lane < bs && !(used & 1u) && a0 == a0)
+            k0 = (__float_as_uint((float)fabs(a0)) & ~127u) | (unsigned)(127 - lane);
+        if (TWO && lane + 64 < bs && !(used & 2u) && a1 == a1)
+            k1 = (__float_as_uint((float)fabs(a1)) & ~127u) | (unsigned)(127 - (lane + 64));
+        const unsigned best = wave_max_u32(max(k0, k1));
+        const int p = 127 - (int)(best & 127u);
+        const int pl = p & (kWave - 1);
+        const bool hi = TWO && p >= 64;
+        const T piv = readlane_t(hi ? v1[t] : v0[t], pl);
+        if (best == 0u || piv == T(0)) {
+            if (lane == 0) *info = k + t + 1;
+            ok = false;
+            continue;
+        }
+        const T r = T(1) / piv;
+        const T m0 = v0[t], m1 = v1[t];
+        const bool own0 = lane == pl && !hi, own1 = lane == pl && hi;
+        // column c of the scaled pivot row is read from lane pl before the
+        // lanes update column c; the pivot lane takes it in place of its row
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            if (c == t) continue;
+            const T sc = readlane_t(hi ? v1[c] : v0[c], pl) * r;
+            const T n0 = v0[c] - m0 * sc;
+            v0[c] = own0 ? sc : n0;
+            if (TWO) {
+                const T n1 = v1[c] - m1 * sc;
+                v1[c] = own1 ? sc : n1;
+            }
+        }
+        v0[t] = own0 ? r : -m0 * r;
+        if (TWO) v1[t] = own1 ? r : -m1 * r;
+        if (own0) used |= 1u;
+        if (own1) used |= 2u;
+        if (lane == 0) {
+            perm[k + t] = p;
+            stepof[p] = k + t;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        if (row0) A[lane + ld * (k + c)] = v0[c];
+        if (row1) A[lane + 64 + ld * (k + c)] = v1[c];
+    }
+    return ok;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kInvThreads) void k_block_inv_mfma(const uint32_t* __restrict__ list,
+                                                                const uint32_t* __restrict__ npv,
+                                                                const uint64_t* __restrict__ blkoff, int D, int ld,
+                                                                const double* __restrict__ M, T* __restrict__ out,
+                                                                int* __restrict__ useblk)
+{
+    typedef typename MfmaVec<T>::type V4;
+    extern __shared__ double inv_lds[];
+    T* A = (T*)inv_lds;                                   // ld * (ld - 1)
+    int* perm = (int*)(A + (size_t)ld * (ld - 1));        // pivot row of each step
+    int* stepof = perm + kInvNB;                          // step at which each row was pivot
+    int* info = stepof + kInvNB;
+    const uint32_t f = list[blockIdx.x];
+    const int bs = D * (int)npv[f];
+    const int nb = (bs + 15) & ~15;
+    const uint64_t off = blkoff[f];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+
+    for (int j = w; j < nb; j += kInvThreads / kWave)
+        for (int i = lane; i < nb; i += kWave)
+            A[i + ld * j] = (i < bs && j < bs) ? (T)M[off + i + (size_t)bs * j] : (i == j ? T(1) : T(0));
+    if (tid < kInvNB) { perm[tid] = tid; stepof[tid] = -1; }
+    if (tid == 0) *info = 0;
+    __syncthreads();
+
+    const int nt = nb / 16;
+    unsigned used = 0;
+    for (int k = 0; k < bs; k += 16) {
+        const int steps = min(16, bs - k);
+        if (w == 0) {
+            if (nb > 64) inv_panel<T, true>(A, ld, nb, bs, k, steps, used, perm, stepof, info);
+            else inv_panel<T, false>(A, ld, nb, bs, k, steps, used, perm, stepof, info);
+        }
+        __syncthreads();
+        if (*info) break;
+        const int kt = k / 16;
+        // B operands: the panel's pivot rows (step order) in every other column tile
+        T bop[kInvNB / 16][4];
+        const int s0 = lane >> 4, jl = lane & 15;
+#pragma unroll
+        for (int jt = 0; jt < kInvNB / 16; ++jt)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int s = 4 * c + s0;
+                bop[jt][c] = (jt < nt && jt != kt && s < steps) ? A[perm[k + s] + ld * (16 * jt + jl)] : T(0);
+            }
+        __syncthreads();
+        int q = 0;
+#pragma unroll
+        for (int jt = 0; jt < kInvNB / 16; ++jt) {
+            if (jt >= nt || jt == kt) continue;
+            for (int it = 0; it < nt; ++it, ++q) {
+                if ((q & 3) != w) continue;
+                const int j = 16 * jt + jl;
+                V4 acc;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = 16 * it + MfmaVec<T>::row(s0, r);
+                    const int st = stepof[i];
+                    acc[r] = (st >= k && st < k + steps) ? T(0) : A[i + ld * j];
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    acc = mfma_16x16x4(A[16 * it + jl + ld * (k + 4 * c + s0)], bop[jt][c], acc);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) A[16 * it + MfmaVec<T>::row(s0, r) + ld * j] = acc[r];
+            }
+        }
+        __syncthreads();
+    }
+    const bool ok = (*info == 0);
+    if (ok) {
+        T* dst = out + off;
+        for (int cidx = w; cidx < bs; cidx += kInvThreads / kWave) {
+            const int t = stepof[cidx];
+            for (int s = lane; s < bs; s += kWave) dst[s + (size_t)bs * cidx] = A[perm[s] + ld * t];
+        }
+    }
+    if (tid == 0) useblk[f] = ok;
+}
+
+static size_t inv_mfma_lds_bytes(int nbmax, size_t elem)
+{
+    const int ld = nbmax + 1;
+    return (size_t)ld * nbmax * elem + (2 * kInvNB + 4) * sizeof(int);
+}
+
 // Fibers grouped by block size for the register kernels: [<=32 | <=64 | <=96 |
-// <=128 | larger], uploaded once per layout.
+// <=128 | larger], uploaded once per layout.  Stored as [<=96 | <=64 | <=32 |
+// <=128 | larger] so that the matrix-core kernel takes the blocks up to 96
+// unknowns in one range, largest class first.
 static void gj_lists(Context& c)
 {
     if (c.gj_nfib == c.nfib && c.gj_largest == c.largest && !c.gj_start.empty()) return;
@@ -505,12 +712,13 @@ static void gj_lists(Context& c)
         lists[b].push_back(f);
     }
     std::vector<uint32_t> all;
-    c.gj_start.assign(6, 0);
-    for (int b = 0; b < 5; ++b) {
+    c.gj_start.assign(10, 0);   // [b] start, [5 + b] end of class b
+    const int order[5] = { 2, 1, 0, 3, 4 };
+    for (int b : order) {
         c.gj_start[b] = (uint32_t)all.size();
         all.insert(all.end(), lists[b].begin(), lists[b].end());
+        c.gj_start[5 + b] = (uint32_t)all.size();
     }
-    c.gj_start[5] = (uint32_t)all.size();
     c.d_gj_list.resize(all.size() ? all.size() : 1);
     if (!all.empty()) {
         CYMG_HIP(hipMemcpyAsync(c.d_gj_list.get(), all.data(), all.size() * 4, hipMemcpyHostToDevice, c.stream));
@@ -527,6 +735,24 @@ static void launch_gj(Context& c, uint32_t begin, uint32_t end, T* out)
     hipLaunchKernelGGL((k_block_gj<T, BSMAX, NW>), dim3(end - begin), dim3(NW * kWave), 0, c.stream,
                        c.d_gj_list.get() + begin, c.d_np.get(), c.d_blkoff.get(), c.dim, c.d_binv.get(), out,
                        c.d_useblk.get());
+}
+
+// blocks list[begin .. begin + count) (at most 96 unknowns, largest first)
+template <typename T>
+static void launch_inv_mfma(Context& c, uint32_t begin, uint32_t count, T* out)
+{
+    if (!count) return;
+    int nbmax = 0;
+    for (uint32_t f = 0; f < c.nfib; ++f) {
+        const int bs = c.dim * (int)c.h_np[f];
+        if (bs <= kInvNB) nbmax = std::max(nbmax, (bs + 15) & ~15);
+    }
+    const size_t lds = inv_mfma_lds_bytes(nbmax, sizeof(T));
+    CYMG_HIP(hipFuncSetAttribute((const void*)k_block_inv_mfma<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds));
+    hipLaunchKernelGGL(k_block_inv_mfma<T>, dim3(count), dim3(kInvThreads), lds, c.stream,
+                       c.d_gj_list.get() + begin, c.d_np.get(), c.d_blkoff.get(), c.dim, nbmax + 1,
+                       c.d_binv.get(), out, c.d_useblk.get());
 }
 
 static void launch_lds_build(Context& c, const PcArgs& a0, int precision, void* dst, const uint32_t* list,
@@ -585,30 +811,36 @@ void launch_precond_build(Context& c, int precision, double* block_out)
     c.d_binv.resize(c.blk_total);
     if (precision == CYMG_PRECOND_FP32) c.d_binv32.resize(c.blk_total);
     void* dst = precision == CYMG_PRECOND_FP32 ? (void*)c.d_binv32.get() : (void*)c.d_binv.get();
-    if (c.pc_method != 0) {
+    if (c.pc_method == 1 || c.pc_method == 2) {
         // LDS kernels for every block: 1 getrf + getri, 2 Gauss-Jordan
         launch_lds_build(c, a, precision, dst, nullptr, c.nfib);
     } else {
-        // assemble every block into d_binv, invert in registers by size class;
-        // blocks beyond the register classes are rebuilt and inverted in LDS
+        // assemble every block into d_binv, invert by size class (method 0:
+        // in registers, method 3: on the matrix cores); blocks beyond those
+        // classes are rebuilt and inverted in LDS
         gj_lists(c);
         PcArgs e = a;
         e.factor = 0;
         launch_lds_build(c, e, CYMG_PRECOND_FP64, c.d_binv.get(), nullptr, c.nfib);
-        const std::vector<uint32_t>& st = c.gj_start;
-        if (precision == CYMG_PRECOND_FP32) {
+        const std::vector<uint32_t>& st = c.gj_start;   // [b] start, [5 + b] end of class b
+        if (c.pc_method == 3) {
+            const uint32_t n96 = st[5 + 0] - st[2];
+            if (precision == CYMG_PRECOND_FP32) launch_inv_mfma<float>(c, st[2], n96, c.d_binv32.get());
+            else launch_inv_mfma<double>(c, st[2], n96, c.d_binv.get());
+            launch_lds_build(c, a, precision, dst, c.d_gj_list.get() + st[3], st[5 + 4] - st[3]);
+        } else if (precision == CYMG_PRECOND_FP32) {
             float* o = c.d_binv32.get();
-            launch_gj<float, 32, 1>(c, st[0], st[1], o);
-            launch_gj<float, 64, 1>(c, st[1], st[2], o);
-            launch_gj<float, 96, 2>(c, st[2], st[3], o);
-            launch_gj<float, 128, 2>(c, st[3], st[4], o);
-            launch_lds_build(c, a, precision, dst, c.d_gj_list.get() + st[4], st[5] - st[4]);
+            launch_gj<float, 32, 1>(c, st[0], st[5 + 0], o);
+            launch_gj<float, 64, 1>(c, st[1], st[5 + 1], o);
+            launch_gj<float, 96, 2>(c, st[2], st[5 + 2], o);
+            launch_gj<float, 128, 2>(c, st[3], st[5 + 3], o);
+            launch_lds_build(c, a, precision, dst, c.d_gj_list.get() + st[4], st[5 + 4] - st[4]);
         } else {
             double* o = c.d_binv.get();
-            launch_gj<double, 32, 1>(c, st[0], st[1], o);
-            launch_gj<double, 64, 1>(c, st[1], st[2], o);
-            launch_gj<double, 96, 2>(c, st[2], st[3], o);
-            launch_lds_build(c, a, precision, dst, c.d_gj_list.get() + st[3], st[5] - st[3]);
+            launch_gj<double, 32, 1>(c, st[0], st[5 + 0], o);
+            launch_gj<double, 64, 1>(c, st[1], st[5 + 1], o);
+            launch_gj<double, 96, 2>(c, st[2], st[5 + 2], o);
+            launch_lds_build(c, a, precision, dst, c.d_gj_list.get() + st[3], st[5 + 4] - st[3]);
         }
     }
     CYMG_HIP(hipGetLastError());

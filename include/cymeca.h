@@ -207,8 +207,10 @@ int         cymg_get_matrix(cymg_context* ctx, int which, uint32_t* nnz, uint32_
    CYMG_OPT_KRYLOV_LAUNCHES: 2 (default, recurrences for the preconditioned
      vectors) or 4 launches per BiCGStab iteration;
    CYMG_OPT_BLOCK_INVERSE: 0 Gauss-Jordan with the block in registers
-     (default; blocks beyond 96 unknowns in FP64, 128 in FP32, use 2),
-     1 getrf + getri order in LDS, 2 Gauss-Jordan in LDS;
+     (blocks beyond 96 unknowns in FP64, 128 in FP32, use 2),
+     1 getrf + getri order in LDS, 2 Gauss-Jordan in LDS, 3 (default)
+     blocked Gauss-Jordan in LDS with the panel updates on the matrix cores
+     (blocks beyond 96 unknowns use 2);
    CYMG_OPT_TILE_POINTS: target points per tile of whole fibers in the
      two-launch iteration, 1..CYMG_MAX_FIBER_POINTS (1: one fiber per tile),
      0 (default): chosen from the layout (or by cymg_autotune);
@@ -218,7 +220,7 @@ enum { CYMG_OPT_KRYLOV_LAUNCHES = 1, CYMG_OPT_BLOCK_INVERSE = 2, CYMG_OPT_TILE_P
 int         cymg_set_option(cymg_context* ctx, int option, int value);
 /* autotuner (no counterpart in the reference): on the current system (after
    setup), times the block-inverse methods (Gauss-Jordan in registers / in
-   LDS) and the tile shapes of the two-launch iteration (target points per
+   LDS / blocked on the matrix cores) and the tile shapes of the two-launch iteration (target points per
    tile 16..128 and the layout default, `reps` launches each) and keeps the
    fastest; reports the choices and the chosen launch time (ms) */
 int         cymg_autotune(cymg_context* ctx, int reps, int* tile_points, int* block_inverse, double* launch_ms);
@@ -280,6 +282,10 @@ int         cymg_guess_check(cymg_context* ctx, const float* X, const float* T, 
 /* features [nfib][288] (as FP32 values of the BF16 inputs) and scales [nfib] of
    the last iterate() that used or recorded the learned guess */
 int         cymg_guess_features(cymg_context* ctx, float* X, float* scale);
+/* the learned initial guess x0 = M^-1 b + s*c of the current system (after
+   cymg_setup) without solving, and optionally its base M^-1 b; builds the
+   preconditioner of the given precision first (0: M^-1 = 0) */
+int         cymg_guess_predict(cymg_context* ctx, int precondition, double* x0, double* base);
 
 #ifdef __cplusplus
 }

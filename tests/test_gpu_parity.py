@@ -164,6 +164,41 @@ def test_blocks_and_precondition(pkg, scenes, oracle, dim):
     assert rel_err(gpu.precondition(x), orc.precondition(x)) < 1e-10
 
 
+@pytest.mark.parametrize("dim,lmax", [(3, 16.0), (2, 24.2)])
+@pytest.mark.parametrize("inverse", [0, 2, 3])
+@pytest.mark.parametrize("precision", [1, 2])
+def test_block_inverse_methods(pkg, scenes, oracle, dim, lmax, inverse, precision):
+    """Every block-inverse method against the oracle's LU inverse, over block
+    sizes from 8 to above 96 unknowns (the larger ones take the LDS kernel):
+    FP64 within 1e-10 relative, FP32 (factorised in FP32) within 1e-4 (the
+    blocks have condition numbers up to ~4e3; a float32 LAPACK inverse of
+    them is off by 1.2e-5)."""
+    if dim == 3:
+        sc = scenes.config2(seed=28, nfib=40, nlinks=150, radius=4.0, lmin=1.5, lmax=lmax)
+    else:
+        sc = scenes.mini(dim=2, seed=28, nfib=40, nlinks=150, radius=4.0, lmin=1.5, lmax=lmax)
+    print(f"block inverse dim={dim} method={inverse} precision={precision} "
+          f"bs={dim * sc.nb_points.min()}..{dim * sc.nb_points.max()}", flush=True)
+    gpu = gpu_from_scene(pkg, sc)
+    gpu.set_option(pkg.OPT_BLOCK_INVERSE, inverse)
+    orc = oracle_from_scene(oracle, sc)
+    g = sc.gauss(0)
+    gpu.setup(g)
+    orc.setup(g)
+    orc.compute_preconditioner()
+    gpu.compute_preconditioner(precision)
+    bar = 1e-10 if precision == pkg.PRECOND_FP64 else 1e-4
+    sizes = set()
+    for f in range(sc.nfib):
+        Bi = gpu.get_block(f, inverse=True)
+        sizes.add(Bi.shape[0])
+        assert rel_err(Bi, orc.get_inverse_block(f)) < bar, f
+    assert min(sizes) <= 32 and max(sizes) > 96
+    x = np.random.default_rng(5).standard_normal(sc.size)
+    assert rel_err(gpu.precondition(x), orc.precondition(x)) < bar
+    gpu.close()
+
+
 def _step_pair(pkg, oracle, sc, steps, tol, tol_mode, precondition=1, launches=2, inverse=0, tile_points=0):
     gpu = gpu_from_scene(pkg, sc)
     gpu.set_option(pkg.OPT_KRYLOV_LAUNCHES, launches)
@@ -198,7 +233,7 @@ def test_config1_one_iteration_and_positions(pkg, scenes, oracle):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-@pytest.mark.parametrize("launches,inverse", [(2, 0), (4, 0), (2, 1), (2, 2)])
+@pytest.mark.parametrize("launches,inverse", [(2, 0), (4, 0), (2, 1), (2, 2), (2, 3)])
 def test_crosslinked_step_fp64(pkg, scenes, oracle, dim, launches, inverse):
     sc = scenes.mini(dim=dim, seed=15, nfib=12, nlinks=40, radius=2.0)
     gpu, orc, reps = _step_pair(pkg, oracle, sc, 2, 1e-11, 1, launches=launches, inverse=inverse)
@@ -295,7 +330,7 @@ def test_autotune_then_step_matches_oracle(pkg, scenes, oracle):
     print("autotune", flush=True)
     t = gpu.autotune(reps=6)
     assert t["tile_points"] in (0, 16, 24, 32, 48, 64, 96, 128)
-    assert t["block_inverse"] in (0, 2) and t["launch_ms"] > 0
+    assert t["block_inverse"] in (0, 2, 3) and t["launch_ms"] > 0
     r1 = gpu.solve(1e-11, sc.gauss(0), tol_mode=pkg.TOL_RELATIVE)
     orc = oracle_from_scene(oracle, sc)
     orc.solve(1e-11, sc.gauss(0), tol_mode=1)
