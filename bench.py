@@ -243,11 +243,25 @@ def main():
                 its.append(rep["iterations"])
         torch.cuda.synchronize()
         bdt = time.perf_counter() - t0
+        # full-work iteration launch over all 64 systems (every tile iterating)
+        mb.setup(gauss_device_ptr=gb[0].data_ptr())
+        mb.compute_preconditioner(pkg.PRECOND_FP64)
+        b_ms = mb.time_kernel(6, reps=10)
+        rB, cB, _ = mb.matrix_entries(0)
+        rC, cC, _ = mb.matrix_entries(1)
+        b_kb = 2 * int((rB != cB).sum()) + int((rB == cB).sum())
+        b_kc = 2 * int((rC != cC).sum()) + int((rC == cC).sum())
+        del rB, cB, rC, cC
+        b_blk = int(((big.dim * big.nb_points.astype(np.int64)) ** 2).sum())
+        b_bytes = algorithmic_bytes_iteration_launch(big, b_kb, b_kc, 8 * b_blk)
+        b_gbs = b_bytes / (b_ms * 1e-3) / 1e9
         batched = {"workload": f"batch-64-systems-{big.nfib}-fibers-{big.npts}-points-{big.nlinks}-links",
                    "systems": batch.nsys, "fibers": big.nfib, "unknowns": big.size,
                    "steps_per_s": round(args.batched_steps / bdt, 3),
                    "system_steps_per_s": round(batch.nsys * args.batched_steps / bdt, 2),
-                   "max_bicgstab_iters": float(np.mean(its))}
+                   "max_bicgstab_iters": float(np.mean(its)),
+                   "iteration_launch": {"avg_ms": round(b_ms, 4), "bytes_per_launch": int(b_bytes),
+                                        "achieved_gbs": round(b_gbs, 1), "frac_of_hbm_peak": round(b_gbs / 8000.0, 4)}}
         print(f"bench: batched {batched}", file=sys.stderr, flush=True)
         mb.close()
 

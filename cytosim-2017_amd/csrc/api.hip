@@ -152,6 +152,7 @@ void build_tiles(Context& c)
 
 void upload_layout(Context& c)
 {
+    ++c.layout_gen;
     const uint32_t F = c.nfib;
     c.d_np.resize(F); c.d_idx.resize(F); c.d_seg.resize(F); c.d_cut.resize(F); c.d_rig.resize(F);
     c.d_mob.resize(F); c.d_rfrig.resize(F); c.d_blkoff.resize(F + 1); c.d_fib_sys.resize(F);
@@ -219,10 +220,15 @@ struct Timer {
     }
 };
 
+// 0 for a phase not (yet) recorded; the failed query's error is cleared so
+// that it does not surface in a later error check
 float elapsed(Context& c, int slot)
 {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, c.ev[2 * slot], c.ev[2 * slot + 1]) != hipSuccess) return 0;
+    if (hipEventElapsedTime(&ms, c.ev[2 * slot], c.ev[2 * slot + 1]) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
     return ms;
 }
 
@@ -766,6 +772,7 @@ int cymg_set_couples(cymg_context* ctx, uint32_t n, const uint32_t* fa, const do
                 throw Error(CYMG_ERR_INVALID, "couple links two independent systems");
         }
         c.ncouples = n;
+        ++c.couples_gen;
         const size_t m = n ? n : 1;
         c.cpl_fa.resize(m); c.cpl_fb.resize(m); c.cpl_aa.resize(m); c.cpl_ab.resize(m); c.cpl_w.resize(m);
         if (n) {
@@ -800,6 +807,7 @@ int cymg_set_aster_links(cymg_context* ctx, uint32_t n, const int* kind, const u
             if (!(w[k] >= 0)) throw Error(CYMG_ERR_INVALID, "aster stiffness must be >= 0");
         }
         c.nasters = n;
+        ++c.asters_gen;
         const size_t m = n ? n : 1;
         c.ast_kind.resize(m); c.ast_s1.resize(m); c.ast_s2.resize(m); c.ast_f.resize(m);
         c.ast_ab.resize(m); c.ast_w.resize(m);

@@ -11,6 +11,12 @@
 // (v - x == v + (-x) in IEEE arithmetic).  The unique lower entries are then
 // mirrored into a full symmetric CSR (rows sorted by column) for the
 // gather-only matrix-vector products.
+//
+// The index structure (sort permutation, runs, CSR) depends only on the keys.
+// When the host knows that a stream's keys are those of the previous build
+// (same layout, same host-emitted keys, same couple and aster-link lists, no
+// emitter whose keys follow the positions), the structure is reused and only
+// the in-order sums and the CSR values are recomputed.
 #include <hipcub/hipcub.hpp>
 
 #include "context.h"
@@ -106,48 +112,68 @@ __global__ void k_csr_fill(uint32_t n2, const uint64_t* __restrict__ key2, const
     val[i] = low_val[u];
 }
 
+// CSR values of a reused structure: slot i takes the lower entry it mirrors
+__global__ void k_csr_vals(uint32_t n2, const uint32_t* __restrict__ rowptr, uint32_t size,
+                           const uint32_t* __restrict__ map, const double* __restrict__ low_val,
+                           double* __restrict__ val)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n2 || i >= rowptr[size]) return;
+    val[i] = low_val[map[i]];
+}
+
 static inline dim3 grid_for(size_t n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
 
-// sort + run-length + in-order sums of one stream; returns the unique entries in m.low_*
-static void reduce_stream(Context& c, int which, DevSym& m, int end_bit)
+// sort + run-length + in-order sums of one stream; returns the unique entries
+// in m.low_*.  reuse: the keys are those of the last build of m, whose
+// permutation and runs are kept in m.perm / m.ukey / m.runoff / m.runlen.
+static void reduce_stream(Context& c, int which, DevSym& m, int end_bit, bool reuse)
 {
     const size_t total = c.s_total[which];
     hipStream_t st = c.stream;
-    c.s_pos.resize(total);
-    c.s_pos_sorted.resize(total);
-    c.s_key_sorted.resize(total);
-    c.s_ukey.resize(total);
-    c.s_runlen.resize(total);
-    c.s_runoff.resize(total);
     m.nruns.resize(1);
     m.low_key.resize(total);
     m.low_val.resize(total);
-    hipLaunchKernelGGL(k_iota, grid_for(total), dim3(256), 0, st, (uint32_t)total, c.s_pos.get());
-    size_t tb1 = 0, tb2 = 0, tb3 = 0;
-    CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, c.s_key[which].get(), c.s_key_sorted.get(),
-                                                c.s_pos.get(), c.s_pos_sorted.get(), (int)total, 0, end_bit, st));
-    CYMG_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb2, c.s_key_sorted.get(), c.s_ukey.get(),
-                                                   c.s_runlen.get(), m.nruns.get(), (int)total, st));
-    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, c.s_runlen.get(), c.s_runoff.get(), (int)total, st));
-    size_t tb = std::max(tb1, std::max(tb2, tb3));
-    c.s_temp.resize(tb);
-    CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(c.s_temp.get(), tb1, c.s_key[which].get(), c.s_key_sorted.get(),
-                                                c.s_pos.get(), c.s_pos_sorted.get(), (int)total, 0, end_bit, st));
-    CYMG_HIP(hipcub::DeviceRunLengthEncode::Encode(c.s_temp.get(), tb2, c.s_key_sorted.get(), c.s_ukey.get(),
-                                                   c.s_runlen.get(), m.nruns.get(), (int)total, st));
-    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.s_temp.get(), tb3, c.s_runlen.get(), c.s_runoff.get(), (int)total,
-                                              st));
-    hipLaunchKernelGGL(k_run_sum, grid_for(total), dim3(256), 0, st, (uint32_t)total, m.nruns.get(), c.s_ukey.get(),
-                       c.s_runoff.get(), c.s_runlen.get(), c.s_pos_sorted.get(), c.s_val[which].get(),
-                       m.low_key.get(), m.low_val.get());
+    if (!reuse) {
+        c.s_pos.resize(total);
+        c.s_key_sorted.resize(total);
+        m.perm.resize(total);
+        m.ukey.resize(total);
+        m.runlen.resize(total);
+        m.runoff.resize(total);
+        hipLaunchKernelGGL(k_iota, grid_for(total), dim3(256), 0, st, (uint32_t)total, c.s_pos.get());
+        size_t tb1 = 0, tb2 = 0, tb3 = 0;
+        CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, c.s_key[which].get(), c.s_key_sorted.get(),
+                                                    c.s_pos.get(), m.perm.get(), (int)total, 0, end_bit, st));
+        CYMG_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb2, c.s_key_sorted.get(), m.ukey.get(),
+                                                       m.runlen.get(), m.nruns.get(), (int)total, st));
+        CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, m.runlen.get(), m.runoff.get(), (int)total, st));
+        size_t tb = std::max(tb1, std::max(tb2, tb3));
+        c.s_temp.resize(tb);
+        CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(c.s_temp.get(), tb1, c.s_key[which].get(), c.s_key_sorted.get(),
+                                                    c.s_pos.get(), m.perm.get(), (int)total, 0, end_bit, st));
+        CYMG_HIP(hipcub::DeviceRunLengthEncode::Encode(c.s_temp.get(), tb2, c.s_key_sorted.get(), m.ukey.get(),
+                                                       m.runlen.get(), m.nruns.get(), (int)total, st));
+        CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.s_temp.get(), tb3, m.runlen.get(), m.runoff.get(), (int)total,
+                                                  st));
+    }
+    hipLaunchKernelGGL(k_run_sum, grid_for(total), dim3(256), 0, st, (uint32_t)total, m.nruns.get(), m.ukey.get(),
+                       m.runoff.get(), m.runlen.get(), m.perm.get(), c.s_val[which].get(), m.low_key.get(),
+                       m.low_val.get());
     CYMG_HIP(hipGetLastError());
 }
 
-static void build_csr(Context& c, DevSym& m)
+static void build_csr(Context& c, DevSym& m, bool reuse)
 {
     const size_t total = m.low_key.n;
     const size_t n2 = 2 * total;
     hipStream_t st = c.stream;
+    if (reuse) {
+        hipLaunchKernelGGL(k_csr_vals, grid_for(n2), dim3(256), 0, st, (uint32_t)n2, m.rowptr.get(), m.size,
+                           m.map.get(), m.low_val.get(), m.val.get());
+        CYMG_HIP(hipGetLastError());
+        return;
+    }
     c.c_key.resize(n2);
     c.c_key_sorted.resize(n2);
     c.c_u.resize(n2);
@@ -224,6 +250,27 @@ void build_streams(Context& c)
     }
     for (int w = 0; w < 3; ++w) copy_host(w, hs[w]->key.size());
 
+    // what each stream's keys are built from; a confinement emitter's keys
+    // follow the positions, so mC and vBAS are rebuilt whenever it contributes
+    DevSym* all[3] = { &c.mB, &c.mC, &c.mBas };
+    bool reuse[3];
+    for (int w = 0; w < 3; ++w) {
+        std::vector<uint64_t> sig = { total[w], w == 0 ? c.npts : c.n(), c.layout_gen, c.nsys, hs[w]->key.size() };
+        bool cacheable = true;
+        for (const EmitterMark& mk : c.marks) {
+            sig.push_back((uint64_t)mk.kind);
+            sig.push_back(w == 0 ? mk.posB : (w == 1 ? mk.posC : mk.posBas));
+            if (mk.kind == 1 && w != 0) cacheable = false;
+            if (mk.kind == 2) { sig.push_back(c.couples_gen); sig.push_back(c.ncouples); }
+            if (mk.kind == 3) { sig.push_back(c.asters_gen); sig.push_back(c.nasters); }
+        }
+        sig.insert(sig.end(), hs[w]->key.begin(), hs[w]->key.end());
+        DevSym& m = *all[w];
+        reuse[w] = cacheable && !m.sig.empty() && m.sig == sig;
+        if (cacheable) m.sig.swap(sig);
+        else m.sig.clear();
+    }
+
     // mB over points, mC over degrees of freedom
     c.mB.size = c.npts;
     c.mC.size = c.n();
@@ -232,15 +279,15 @@ void build_streams(Context& c)
         DevSym& m = *mats[w];
         m.used = total[w] > 0;
         if (!m.used) continue;
-        reduce_stream(c, w, m, 32 + bits_for(m.size));
-        build_csr(c, m);
+        reduce_stream(c, w, m, 32 + bits_for(m.size), reuse[w]);
+        build_csr(c, m, reuse[w]);
     }
     // vBAS
     CYMG_HIP(hipMemsetAsync(c.d_bas.get(), 0, (size_t)c.n() * sizeof(double), c.stream));
     if (total[2]) {
         DevSym& m = c.mBas;
         m.size = c.n();
-        reduce_stream(c, 2, m, bits_for(m.size));
+        reduce_stream(c, 2, m, bits_for(m.size), reuse[2]);
         hipLaunchKernelGGL(k_bas_scatter, grid_for(total[2]), dim3(256), 0, c.stream, (uint32_t)total[2],
                            m.nruns.get(), m.low_key.get(), m.low_val.get(), c.d_bas.get());
         CYMG_HIP(hipGetLastError());

@@ -112,6 +112,11 @@ struct DevSym {
     DevBuf<double> val;
     DevBuf<uint32_t> map;       // CSR slot -> lower entry
     DevBuf<uint32_t> nruns;     // device count of runs (unique keys incl. invalid)
+    // index structure of the last build, reused while the stream's keys are
+    // known to be unchanged: stable sort permutation and runs of equal keys
+    DevBuf<uint32_t> perm, runoff, runlen;
+    DevBuf<uint64_t> ukey;
+    std::vector<uint64_t> sig;  // what the keys were built from (empty: no reuse)
 };
 
 struct Context;

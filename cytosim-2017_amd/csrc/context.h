@@ -59,8 +59,8 @@ struct Context {
     DevBuf<uint64_t> s_key[3];         // stream inputs (B, C, vBAS) in call order
     DevBuf<double> s_val[3];
     size_t s_total[3] = {0, 0, 0};
-    DevBuf<uint64_t> s_key_sorted, s_ukey;
-    DevBuf<uint32_t> s_pos, s_pos_sorted, s_runlen, s_runoff;
+    DevBuf<uint64_t> s_key_sorted;
+    DevBuf<uint32_t> s_pos;
     DevBuf<uint64_t> c_key, c_key_sorted;
     DevBuf<uint32_t> c_u, c_u_sorted;
     DevBuf<unsigned char> s_temp;
@@ -68,6 +68,8 @@ struct Context {
 
     // couples (device emitter)
     uint32_t ncouples = 0;
+    // generations of the inputs that the device emitters' keys depend on
+    uint64_t layout_gen = 0, couples_gen = 0, asters_gen = 0;
     DevBuf<uint32_t> cpl_fa, cpl_fb;
     DevBuf<double> cpl_aa, cpl_ab, cpl_w;
 
