@@ -282,15 +282,21 @@ def main():
                 pkg.assemble(ma, ast, upload_couples=False)
             rep = ma.solve(args.tol, tol_mode=pkg.TOL_RELATIVE, precondition=pkg.PRECOND_FP64,
                            gauss_device_ptr=ga[s % 2].data_ptr())
+            # the motors walk under their loads between solves (Couple::stepAA)
+            ma.step_couples(ast.motor_max_speed, ast.motor_stall_force)
             if s >= 2:
                 its.append(rep["iterations"])
         torch.cuda.synchronize()
         adt = time.perf_counter() - t0
+        _, _, cst = ma.get_couples()
         asters = {"workload": f"asters-10x200-fibers-{ast.nfib}-solids-{ast.nsol}-points-{ast.npts}-motors-{ast.nlinks}",
                   "unknowns": ast.size, "steps_per_s": round(args.aster_steps / adt, 3),
                   "ms_per_step": round(1e3 * adt / args.aster_steps, 3),
                   "mean_bicgstab_iters": float(np.mean(its)),
-                  "iteration": "four launches per iteration (Solid rows)"}
+                  "iteration": "four launches per iteration (Solid rows)",
+                  "motors": f"stepped on the device between solves (speed {ast.motor_max_speed} um/s, "
+                            f"stall {ast.motor_stall_force} pN); {int(((cst & 3) == 3).sum())} of "
+                            f"{cst.size} couples still bridging"}
         print(f"bench: asters {asters}", file=sys.stderr, flush=True)
         ma.close()
 

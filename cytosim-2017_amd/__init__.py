@@ -30,7 +30,8 @@ EXPORTED = [
     "cymg_prepare", "cymg_add_pure_force",
     "cymg_inter_link_ee", "cymg_inter_link_ie", "cymg_inter_link_ii", "cymg_inter_long_link_ee",
     "cymg_inter_clamp_e", "cymg_inter_clamp_i", "cymg_inter_long_clamp", "cymg_inter_plane_e",
-    "cymg_confine_sphere", "cymg_set_couples", "cymg_couples_interactions", "cymg_set_aster_links",
+    "cymg_confine_sphere", "cymg_set_couples", "cymg_couples_interactions", "cymg_set_couple_state",
+    "cymg_get_couples", "cymg_step_couples", "cymg_set_aster_links",
     "cymg_aster_interactions", "cymg_solve",
     "cymg_setup", "cymg_iterate", "cymg_apply", "cymg_multiply", "cymg_compute_preconditioner",
     "cymg_precondition", "cymg_compute_forces", "cymg_multiply_device", "cymg_precondition_device",
@@ -71,6 +72,11 @@ class Timings(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MotorProps(C.Structure):
+    _fields_ = [("max_speed", C.c_double), ("stall_force", C.c_double), ("limit_speed", C.c_int),
+                ("hold_growing_end", C.c_int)]
 
 
 class CymgError(RuntimeError):
@@ -133,6 +139,9 @@ def lib():
         "cymg_confine_sphere": (C.c_int, [vp, C.c_double, C.c_double]),
         "cymg_set_couples": (C.c_int, [vp, u32, vp, vp, vp, vp, vp]),
         "cymg_couples_interactions": (C.c_int, [vp]),
+        "cymg_set_couple_state": (C.c_int, [vp, vp]),
+        "cymg_get_couples": (C.c_int, [vp, vp, vp, vp]),
+        "cymg_step_couples": (C.c_int, [vp, C.POINTER(MotorProps)]),
         "cymg_set_aster_links": (C.c_int, [vp, u32, vp, vp, vp, vp, vp, vp]),
         "cymg_aster_interactions": (C.c_int, [vp]),
         "cymg_solve": (C.c_int, [vp, C.POINTER(SolveParams), vp, C.POINTER(SolveReport)]),
@@ -196,6 +205,7 @@ class Meca:
             raise CymgError(rc, "cymg_create failed (dim must be 2 or 3, and a GPU must be visible)")
         self.h = h
         self.np = []
+        self._ncouples = 0
         if params is not None:
             self.set_params(**params)
 
@@ -323,9 +333,29 @@ class Meca:
         ab = np.ascontiguousarray(abs_b, dtype=np.float64)
         w = np.ascontiguousarray(stiffness, dtype=np.float64)
         self._chk(self.L.cymg_set_couples(self.h, fa.size, _p(fa), _p(aa), _p(fb), _p(ab), _p(w)))
+        self._ncouples = int(fa.size)
 
     def couples_interactions(self):
         self._chk(self.L.cymg_couples_interactions(self.h))
+
+    def set_couple_state(self, state):
+        """Per uploaded couple: 1 hand 1 attached, 2 hand 2 attached, 4 hand 1
+        is a motor, 8 hand 2 is a motor."""
+        st = np.ascontiguousarray(state, dtype=np.uint8)
+        self._chk(self.L.cymg_set_couple_state(self.h, _p(st)))
+
+    def get_couples(self):
+        """(abscissa of hand 1, abscissa of hand 2, state) of the uploaded couples."""
+        n = self._ncouples
+        aa, ab, st = np.empty(n), np.empty(n), np.empty(n, np.uint8)
+        if n:
+            self._chk(self.L.cymg_get_couples(self.h, _p(aa), _p(ab), _p(st)))
+        return aa, ab, st
+
+    def step_couples(self, max_speed, stall_force, limit_speed=True, hold_growing_end=False):
+        """Couple::stepAA/AF/FA with the motor hands, from the current points."""
+        mp = MotorProps(float(max_speed), float(stall_force), int(bool(limit_speed)), int(bool(hold_growing_end)))
+        self._chk(self.L.cymg_step_couples(self.h, C.byref(mp)))
 
     def set_aster_links(self, kind, s1, s2, fiber, abscissa, stiffness):
         kd = np.ascontiguousarray(kind, dtype=np.int32)
@@ -543,11 +573,16 @@ def assemble(meca, scene, couples_on_device=True, upload_couples=True):
             if upload_couples:
                 meca.set_couples(scene.link_fiber_a, scene.link_abs_a, scene.link_fiber_b,
                                  scene.link_abs_b, scene.link_stiffness)
+                if getattr(scene, "link_state", None) is not None:
+                    meca.set_couple_state(scene.link_state)
             meca.couples_interactions()
         else:
             a1, a2, ca = scene.interpolate(scene.link_fiber_a, scene.link_abs_a)
             b1, b2, cb = scene.interpolate(scene.link_fiber_b, scene.link_abs_b)
+            state = getattr(scene, "link_state", None)
             for k in range(scene.nlinks):
+                if state is not None and (int(state[k]) & 3) != 3:
+                    continue      # only bridging couples interact (couple.cc:87-92)
                 meca.inter_link_ii(int(a1[k]), int(a2[k]), float(ca[k]), int(b1[k]), int(b2[k]),
                                    float(cb[k]), float(scene.link_stiffness[k]))
     # Aster::setInteractions (aster.cc:53-89), after the couples (simul_solve.cc:238-239)

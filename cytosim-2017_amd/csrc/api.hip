@@ -775,6 +775,9 @@ int cymg_set_couples(cymg_context* ctx, uint32_t n, const uint32_t* fa, const do
         ++c.couples_gen;
         const size_t m = n ? n : 1;
         c.cpl_fa.resize(m); c.cpl_fb.resize(m); c.cpl_aa.resize(m); c.cpl_ab.resize(m); c.cpl_w.resize(m);
+        c.cpl_state.resize(m);
+        // both hands attached, passive (a crosslinker)
+        CYMG_HIP(hipMemsetAsync(c.cpl_state.get(), 3, m, c.stream));
         if (n) {
             CYMG_HIP(hipMemcpyAsync(c.cpl_fa.get(), fa, n * 4, hipMemcpyHostToDevice, c.stream));
             CYMG_HIP(hipMemcpyAsync(c.cpl_fb.get(), fb, n * 4, hipMemcpyHostToDevice, c.stream));
@@ -783,6 +786,57 @@ int cymg_set_couples(cymg_context* ctx, uint32_t n, const uint32_t* fa, const do
             CYMG_HIP(hipMemcpyAsync(c.cpl_w.get(), w, n * 8, hipMemcpyHostToDevice, c.stream));
             CYMG_HIP(hipStreamSynchronize(c.stream));
         }
+    });
+}
+
+static void d2h(Context& c, void* dst, const void* src, size_t bytes);
+
+int cymg_set_couple_state(cymg_context* ctx, const uint8_t* state)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!state) throw Error(CYMG_ERR_INVALID, "null array");
+        for (uint32_t k = 0; k < c.ncouples; ++k)
+            if (state[k] > 15) throw Error(CYMG_ERR_INVALID, "couple state has bits other than 1, 2, 4, 8");
+        if (c.ncouples) {
+            CYMG_HIP(hipMemcpyAsync(c.cpl_state.get(), state, c.ncouples, hipMemcpyHostToDevice, c.stream));
+            CYMG_HIP(hipStreamSynchronize(c.stream));
+        }
+        ++c.couples_gen;
+    });
+}
+
+int cymg_get_couples(cymg_context* ctx, double* aa, double* ab, uint8_t* state)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.ncouples) return;
+        if (aa) d2h(c, aa, c.cpl_aa.get(), (size_t)c.ncouples * 8);
+        if (ab) d2h(c, ab, c.cpl_ab.get(), (size_t)c.ncouples * 8);
+        if (state) d2h(c, state, c.cpl_state.get(), c.ncouples);
+    });
+}
+
+int cymg_step_couples(cymg_context* ctx, const cymg_motor_props* mp)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!mp) throw Error(CYMG_ERR_INVALID, "null motor properties");
+        if (!(mp->stall_force > 0)) throw Error(CYMG_ERR_INVALID, "motor:stall_force must be > 0");
+        if (!c.prepared) throw Error(CYMG_ERR_INVALID, "no points: cymg_prepare first");
+        // MotorProp::complete (motor_prop.cc:44-66)
+        MotorStep ms;
+        ms.max_speed_dt = c.prm.time_step * mp->max_speed;
+        const double abs_speed_dt = fabs(ms.max_speed_dt);
+        ms.var_speed_dt = abs_speed_dt / mp->stall_force;
+        if (mp->max_speed > 0) {
+            ms.min_dabs = 0;
+            ms.max_dabs = 2 * ms.max_speed_dt;
+        } else {
+            ms.min_dabs = 2 * ms.max_speed_dt;
+            ms.max_dabs = 0;
+        }
+        ms.limit_speed = mp->limit_speed != 0;
+        ms.hold_end = mp->hold_growing_end != 0;
+        launch_step_couples(c, ms);
+        ++c.couples_gen;   // the hands' segments, hence the couple keys, may change
     });
 }
 

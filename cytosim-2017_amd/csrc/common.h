@@ -121,6 +121,12 @@ struct DevSym {
 
 struct Context;
 
+// motor hands of the couples (MotorProp::complete, motor_prop.cc:44-66)
+struct MotorStep {
+    double max_speed_dt, var_speed_dt, min_dabs, max_dabs;
+    int limit_speed, hold_end;
+};
+
 // launchers (assembly.hip)
 void build_streams(Context& c);
 // launchers (fiber.hip)
@@ -129,6 +135,7 @@ void launch_fiber_prepare(Context& c);
 void launch_emit_confine(Context& c, uint64_t* keyC, double* valC, uint64_t* keyBas,
                          double* valBas, double radius, double stiffness);
 void launch_emit_couples(Context& c, uint64_t* keyB, double* valB);
+void launch_step_couples(Context& c, const MotorStep& ms);
 void launch_emit_asters(Context& c, uint64_t* keyB, double* valB);
 void launch_setup(Context& c, const double* gauss);
 void launch_matvec(Context& c, const double* X, double* Y);

@@ -72,6 +72,7 @@ struct Context {
     uint64_t layout_gen = 0, couples_gen = 0, asters_gen = 0;
     DevBuf<uint32_t> cpl_fa, cpl_fb;
     DevBuf<double> cpl_aa, cpl_ab, cpl_w;
+    DevBuf<uint8_t> cpl_state;          // attached hands and motor hands of each couple
 
     // aster links (device emitter)
     uint32_t nasters = 0;

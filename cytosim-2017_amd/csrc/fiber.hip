@@ -221,9 +221,10 @@ __device__ __forceinline__ void interp(uint32_t np, uint32_t idx, double cut, do
 
 __global__ void k_emit_couples(uint32_t n, const uint32_t* __restrict__ fa, const double* __restrict__ aa,
                                const uint32_t* __restrict__ fb, const double* __restrict__ ab,
-                               const double* __restrict__ wv, const uint32_t* __restrict__ npv,
-                               const uint32_t* __restrict__ idxv, const double* __restrict__ cutv,
-                               uint64_t* __restrict__ keyB, double* __restrict__ valB)
+                               const double* __restrict__ wv, const uint8_t* __restrict__ state,
+                               const uint32_t* __restrict__ npv, const uint32_t* __restrict__ idxv,
+                               const double* __restrict__ cutv, uint64_t* __restrict__ keyB,
+                               double* __restrict__ valB)
 {
 #pragma clang fp contract(off)
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -236,7 +237,8 @@ __global__ void k_emit_couples(uint32_t n, const uint32_t* __restrict__ fa, cons
     const uint32_t a2 = a1 + 1, b2 = b1 + 1;
     uint64_t* kk = keyB + (size_t)10 * k;
     double* vv = valB + (size_t)10 * k;
-    if (f1 == f2 && (a1 == b1 || a1 == b2 || a2 == b1 || a2 == b2)) {
+    // only bridging couples (both hands attached) interact (couple.cc:87-92)
+    if ((state[k] & 3u) != 3u || (f1 == f2 && (a1 == b1 || a1 == b2 || a2 == b1 || a2 == b2))) {
         for (int i = 0; i < 10; ++i) { kk[i] = kInvalidKey; vv[i] = 0.0; }
         return;
     }
@@ -320,7 +322,120 @@ void launch_emit_couples(Context& c, uint64_t* keyB, double* valB)
     if (!c.ncouples) return;
     const dim3 b(256), g((c.ncouples + 255) / 256);
     hipLaunchKernelGGL(k_emit_couples, g, b, 0, c.stream, c.ncouples, c.cpl_fa.get(), c.cpl_aa.get(), c.cpl_fb.get(),
-                       c.cpl_ab.get(), c.cpl_w.get(), c.d_np.get(), c.d_idx.get(), c.d_cut.get(), keyB, valB);
+                       c.cpl_ab.get(), c.cpl_w.get(), c.cpl_state.get(), c.d_np.get(), c.d_idx.get(),
+                       c.d_cut.get(), keyB, valB);
+    CYMG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Couple::stepAA / stepAF / stepFA (couple.cc:125-150) for couples with motor
+// hands, from the points the fibers hold after the solve (the reshaped points,
+// FiberNaked::getPoints).  Couple::force1 (couple.cc:274-282)
+// is f = k (pos2 - pos1); hand 1 steps under f and hand 2 under -f.
+// Motor::stepLoaded (motor.cc:32-57): load = f . dirFiber, with dirFiber =
+// (P[p+1] - P[p]) / fnCut at the hand's first interpolation point
+// (fiber_naked.h:205), dabs = max_speed_dt + load * var_speed_dt, clamped to
+// [min_dabs, max_dabs] when limit_speed (motor_prop.cc:51-65); an attached
+// motor whose partner hand is free steps unloaded by max_speed_dt
+// (motor.cc:19-28).  FiberBinder::moveBy / checkFiberRange
+// (fiber_binder.cc:59-65, 111-118) and Hand::handleOutOfRange (hand.cc:102-122)
+// for fibers that neither grow nor shrink: past an end the hand holds the end
+// or detaches.  The unbinding tests (hand.cc:145-205) need a random stream and
+// an unbinding rate; they are not modelled (unbinding off).
+// state bits: 1 hand 1 attached, 2 hand 2 attached, 4 hand 1 is a motor,
+// 8 hand 2 is a motor.
+template <int D>
+__device__ __forceinline__ void hand_geometry(const double* __restrict__ pts, uint32_t np, uint32_t idx, double cut,
+                                              double ab, double* pos, double* dir)
+{
+#pragma clang fp contract(off)
+    uint32_t m1;
+    double co;
+    interp(np, idx, cut, ab, m1, co);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const double p1 = pts[D * m1 + d], p2 = pts[D * (m1 + 1) + d];
+        pos[d] = p1 + co * (p2 - p1);
+        dir[d] = (p2 - p1) / cut;
+    }
+}
+
+__device__ __forceinline__ void move_hand(double& ab, uint32_t& st, uint32_t bit, double dabs, double len, int hold)
+{
+#pragma clang fp contract(off)
+    ab = ab + dabs;
+    if (ab < 0.0) {
+        if (hold) ab = 0.0;
+        else st &= ~bit;
+    } else if (ab > len) {
+        if (hold) ab = len;
+        else st &= ~bit;
+    }
+}
+
+template <int D>
+__global__ void k_step_couples(uint32_t n, const uint32_t* __restrict__ fa, double* __restrict__ aa,
+                               const uint32_t* __restrict__ fb, double* __restrict__ ab,
+                               const double* __restrict__ wv, uint8_t* __restrict__ state,
+                               const uint32_t* __restrict__ npv, const uint32_t* __restrict__ idxv,
+                               const double* __restrict__ cutv, const double* __restrict__ pts, MotorStep ms)
+{
+#pragma clang fp contract(off)
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t st = state[k];
+    if (!(st & 12u)) return;
+    const uint32_t f1 = fa[k], f2 = fb[k];
+    double a1 = aa[k], a2 = ab[k];
+    const double len1 = cutv[f1] * (double)(npv[f1] - 1), len2 = cutv[f2] * (double)(npv[f2] - 1);
+    if ((st & 3u) == 3u) {
+        double p1[3], d1[3], p2[3], d2[3], f[3], g[3];
+        hand_geometry<D>(pts, npv[f1], idxv[f1], cutv[f1], a1, p1, d1);
+        hand_geometry<D>(pts, npv[f2], idxv[f2], cutv[f2], a2, p2, d2);
+        const double w = wv[k];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            f[d] = w * (p2[d] - p1[d]);
+            g[d] = -f[d];
+        }
+        if (st & 4u) {
+            double load = f[0] * d1[0];
+#pragma unroll
+            for (int d = 1; d < D; ++d) load = load + f[d] * d1[d];
+            double dabs = ms.max_speed_dt + load * ms.var_speed_dt;
+            if (ms.limit_speed) dabs = dabs < ms.min_dabs ? ms.min_dabs : (dabs > ms.max_dabs ? ms.max_dabs : dabs);
+            move_hand(a1, st, 1u, dabs, len1, ms.hold_end);
+        }
+        if (st & 8u) {
+            double load = g[0] * d2[0];
+#pragma unroll
+            for (int d = 1; d < D; ++d) load = load + g[d] * d2[d];
+            double dabs = ms.max_speed_dt + load * ms.var_speed_dt;
+            if (ms.limit_speed) dabs = dabs < ms.min_dabs ? ms.min_dabs : (dabs > ms.max_dabs ? ms.max_dabs : dabs);
+            move_hand(a2, st, 2u, dabs, len2, ms.hold_end);
+        }
+    } else if (st & 1u) {
+        if (st & 4u) move_hand(a1, st, 1u, ms.max_speed_dt, len1, ms.hold_end);
+    } else if (st & 2u) {
+        if (st & 8u) move_hand(a2, st, 2u, ms.max_speed_dt, len2, ms.hold_end);
+    }
+    aa[k] = a1;
+    ab[k] = a2;
+    state[k] = (uint8_t)st;
+}
+
+void launch_step_couples(Context& c, const MotorStep& ms)
+{
+    if (!c.ncouples) return;
+    const dim3 b(256), g((c.ncouples + 255) / 256);
+    if (c.dim == 3)
+        hipLaunchKernelGGL(k_step_couples<3>, g, b, 0, c.stream, c.ncouples, c.cpl_fa.get(), c.cpl_aa.get(),
+                           c.cpl_fb.get(), c.cpl_ab.get(), c.cpl_w.get(), c.cpl_state.get(), c.d_np.get(),
+                           c.d_idx.get(), c.d_cut.get(), c.d_pos.get(), ms);
+    else
+        hipLaunchKernelGGL(k_step_couples<2>, g, b, 0, c.stream, c.ncouples, c.cpl_fa.get(), c.cpl_aa.get(),
+                           c.cpl_fb.get(), c.cpl_ab.get(), c.cpl_w.get(), c.cpl_state.get(), c.d_np.get(),
+                           c.d_idx.get(), c.d_cut.get(), c.d_pos.get(), ms);
     CYMG_HIP(hipGetLastError());
 }
 

@@ -27,6 +27,8 @@ Everything here is host-side NumPy; nothing in this module touches the GPU.
 """
 from dataclasses import dataclass, field
 
+from typing import Optional
+
 import numpy as np
 
 
@@ -58,6 +60,12 @@ class Scene:
     link_fiber_b: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
     link_abs_b: np.ndarray = field(default_factory=lambda: np.zeros(0))
     link_stiffness: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    # per couple: 1 hand 1 attached, 2 hand 2 attached, 4 hand 1 is a motor, 8
+    # hand 2 is a motor (None: attached passive crosslinkers); motor properties
+    # (MotorProp max_speed, stall_force) of the motor hands
+    link_state: Optional[np.ndarray] = None
+    motor_max_speed: float = 0.0
+    motor_stall_force: float = 0.0
     # Solids, registered after the fibers (simul_solve.cc:204-207): radius of
     # each point (0: no drag) and the initial reshape timer (solid.cc:28)
     sol_radius: list = field(default_factory=list)
@@ -456,6 +464,10 @@ def asters(dim=3, seed=3, nasters=10, fibers_per_aster=200, lmin=5.0, lmax=10.0,
         fa, aa, fb, ab = _bind_crosslinkers_fast(rng, dim, nbp, cut, fiber_pts, nmotors, binding_range, group)
         sc.link_fiber_a, sc.link_abs_a, sc.link_fiber_b, sc.link_abs_b = fa, aa, fb, ab
         sc.link_stiffness = np.full(fa.size, motor_stiffness)
+        # both hands of every couple are motors (speed 0.5 um/s, stall force 5 pN)
+        sc.link_state = np.full(fa.size, 15, np.uint8)
+        sc.motor_max_speed = 0.5
+        sc.motor_stall_force = 5.0
     # Aster::setInteractions: per fiber, centre -- minus end, then the fiber at
     # abscissa `core` -- its Solid point
     fidx = sc.mat_index

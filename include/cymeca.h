@@ -154,6 +154,26 @@ int         cymg_set_couples(cymg_context* ctx, uint32_t n, const uint32_t* fibe
                              const double* abscissa_a, const uint32_t* fiber_b,
                              const double* abscissa_b, const double* stiffness);
 int         cymg_couples_interactions(cymg_context* ctx);
+/* motor couples (Couple::stepAA / stepAF / stepFA, couple.cc:125-150, with
+   Motor::stepLoaded / stepUnloaded, motor.cc:19-57): per uploaded couple a
+   state byte, bit 1 hand 1 attached, 2 hand 2 attached, 4 hand 1 is a motor,
+   8 hand 2 is a motor (cymg_set_couples sets 3: attached crosslinkers).  Only
+   couples with both hands attached interact (couple.cc:87-92).  Stepping
+   uses the current points (after cymg_apply): f = k (pos2 - pos1), a loaded
+   motor moves by max_speed*dt + (f . dir)*|max_speed|*dt/stall_force, clamped to
+   [0, 2 max_speed dt] when limit_speed; a motor whose partner is free moves by
+   max_speed*dt; past a fiber end it holds the end (hold_growing_end) or
+   detaches (hand.cc:102-122, fibers that neither grow nor shrink).  The
+   random unbinding tests are not modelled (unbinding off). */
+typedef struct {
+    double max_speed;        /* MotorProp::max_speed, um/s */
+    double stall_force;      /* MotorProp::stall_force, pN, > 0 */
+    int    limit_speed;      /* MotorProp::limit_speed */
+    int    hold_growing_end; /* HandProp::hold_growing_end */
+} cymg_motor_props;
+int         cymg_set_couple_state(cymg_context* ctx, const uint8_t* state);
+int         cymg_get_couples(cymg_context* ctx, double* aa, double* ab, uint8_t* state);
+int         cymg_step_couples(cymg_context* ctx, const cymg_motor_props* props);
 /* Aster::setInteractions (aster.cc:53-89) for a list of aster links, uploaded
    by set_aster_links and emitted on the device by aster_interactions, each as
    the host interLink call it stands for:

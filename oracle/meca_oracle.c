@@ -1834,3 +1834,89 @@ void orc_get_matrix_dense(const orc_meca* m, int which, double* out)
             out[j + n * i] = s->col[j][k].val;
         }
 }
+
+/* ------------------------------------------------------------------------ */
+/* motor couples between solves                                             */
+
+/* position and dirFiber of a hand at abscissa ab: PointInterpolated::pos
+   (point_set.h:191-200) and FiberNaked::dirPoint (fiber_naked.h:205) at the
+   first interpolation point */
+static void hand_geometry(const orc_meca* m, int fi, double ab, double* pos, double* dir)
+{
+    uint32_t m1, m2;
+    double co;
+    orc_interpolate(m, fi, ab, &m1, &m2, &co);
+    const double cut = m->fib[fi].cut;
+    for (int d = 0; d < m->dim; ++d) {
+        const double p1 = m->pos[m->dim * m1 + d], p2 = m->pos[m->dim * m2 + d];
+        pos[d] = p1 + co * (p2 - p1);
+        dir[d] = (p2 - p1) / cut;
+    }
+}
+
+/* FiberBinder::moveBy + checkFiberRange (fiber_binder.cc:59-65, 111-118),
+   Hand::handleOutOfRange for an end that neither grows nor shrinks
+   (hand.cc:102-122): hold the end (moveToEnd) or detach */
+static void move_hand(double* ab, uint8_t* st, uint8_t bit, double dabs, double len, int hold)
+{
+    *ab += dabs;
+    if (*ab < 0.0) {
+        if (hold) *ab = 0.0;
+        else *st &= (uint8_t)~bit;
+    } else if (*ab > len) {
+        if (hold) *ab = len;
+        else *st &= (uint8_t)~bit;
+    }
+}
+
+/* Motor::stepLoaded (motor.cc:32-57) without the Kramers detachment test */
+static double loaded_dabs(int dim, const double* f, const double* dir, const orc_motor* mp, double dt)
+{
+    /* MotorProp::complete (motor_prop.cc:44-66) */
+    const double max_speed_dt = dt * mp->max_speed;
+    const double var_speed_dt = fabs(max_speed_dt) / mp->stall_force;
+    double min_dabs, max_dabs;
+    if (mp->max_speed > 0) { min_dabs = 0; max_dabs = 2 * max_speed_dt; }
+    else { min_dabs = 2 * max_speed_dt; max_dabs = 0; }
+    double load = f[0] * dir[0];
+    for (int d = 1; d < dim; ++d) load += f[d] * dir[d];
+    double dabs = max_speed_dt + load * var_speed_dt;
+    if (mp->limit_speed) {
+        if (dabs < min_dabs) dabs = min_dabs;
+        else if (dabs > max_dabs) dabs = max_dabs;
+    }
+    return dabs;
+}
+
+/* Couple::stepAA / stepAF / stepFA (couple.cc:125-150) with Couple::force1
+   (couple.cc:274-282) on the points held by the Mecables */
+void orc_step_couples(const orc_meca* m, uint32_t n, const uint32_t* fa, double* aa, const uint32_t* fb, double* ab,
+                      const double* w, uint8_t* state, const orc_motor* mp)
+{
+    const double dt = m->prm.time_step;
+    const int D = m->dim;
+    for (uint32_t k = 0; k < n; ++k) {
+        uint8_t st = state[k];
+        if (!(st & 12)) continue;
+        const orc_fiber* f1 = &m->fib[fa[k]];
+        const orc_fiber* f2 = &m->fib[fb[k]];
+        const double len1 = f1->cut * (double)(f1->np - 1), len2 = f2->cut * (double)(f2->np - 1);
+        if ((st & 3) == 3) {
+            double p1[3], d1[3], p2[3], d2[3], f[3], g[3];
+            hand_geometry(m, (int)fa[k], aa[k], p1, d1);
+            hand_geometry(m, (int)fb[k], ab[k], p2, d2);
+            for (int d = 0; d < D; ++d) {
+                f[d] = w[k] * (p2[d] - p1[d]);
+                g[d] = -f[d];
+            }
+            if (st & 4) move_hand(&aa[k], &st, 1, loaded_dabs(D, f, d1, mp, dt), len1, mp->hold_growing_end);
+            if (st & 8) move_hand(&ab[k], &st, 2, loaded_dabs(D, g, d2, mp, dt), len2, mp->hold_growing_end);
+        } else if (st & 1) {
+            /* Motor::stepUnloaded (motor.cc:19-28) without the detachment test */
+            if (st & 4) move_hand(&aa[k], &st, 1, dt * mp->max_speed, len1, mp->hold_growing_end);
+        } else if (st & 2) {
+            if (st & 8) move_hand(&ab[k], &st, 2, dt * mp->max_speed, len2, mp->hold_growing_end);
+        }
+        state[k] = st;
+    }
+}

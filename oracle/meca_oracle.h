@@ -120,6 +120,19 @@ uint32_t  orc_get_matrix_nr(orc_meca*, int which, uint32_t* ija, double* sa);
 /* dense (size x size) symmetric matrix of mB or mC, column-major */
 void      orc_get_matrix_dense(const orc_meca*, int which, double* out);
 
+/* motor couples between solves: Couple::stepAA / stepAF / stepFA
+   (couple.cc:125-150) with Motor::stepLoaded / stepUnloaded (motor.cc:19-57)
+   and the end handling of Hand::handleOutOfRange (hand.cc:102-122) for
+   fibers that neither grow nor shrink; the random unbinding tests are not
+   modelled.  state[k]: 1 hand 1 attached, 2 hand 2 attached, 4 hand 1 is a
+   motor, 8 hand 2 is a motor.  Uses the points held by the Mecables. */
+typedef struct {
+    double max_speed, stall_force;
+    int    limit_speed, hold_growing_end;
+} orc_motor;
+void      orc_step_couples(const orc_meca*, uint32_t n, const uint32_t* fa, double* aa, const uint32_t* fb,
+                           double* ab, const double* w, uint8_t* state, const orc_motor* mp);
+
 /* LAPACK restatements (documented algorithms), exported for pinning tests */
 int       orc_dgetrf(int n, double* a, int lda, int* ipiv);
 int       orc_dgetri(int n, double* a, int lda, const int* ipiv);

@@ -22,6 +22,11 @@ class Params(C.Structure):
                 ("hydro_radius0", C.c_double), ("hydro_radius1", C.c_double)]
 
 
+class Motor(C.Structure):
+    _fields_ = [("max_speed", C.c_double), ("stall_force", C.c_double), ("limit_speed", C.c_int),
+                ("hold_growing_end", C.c_int)]
+
+
 class SolveParams(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("tol_mode", C.c_int),
                 ("precondition", C.c_int), ("max_iter", C.c_int)]
@@ -80,6 +85,7 @@ def lib():
         "orc_inter_long_clamp": (None, [vp, C.c_uint32, _f64p, C.c_double, C.c_double]),
         "orc_inter_plane_e": (None, [vp, C.c_uint32, _f64p, _f64p, C.c_double]),
         "orc_confine_sphere": (None, [vp, C.c_double, C.c_double]),
+        "orc_step_couples": (None, [vp, C.c_uint32, vp, _f64p, vp, _f64p, _f64p, vp, C.POINTER(Motor)]),
         "orc_setup": (C.c_int, [vp, vp]),
         "orc_iterate": (C.c_int, [vp, C.POINTER(SolveParams), vp, C.POINTER(Report)]),
         "orc_apply": (C.c_int, [vp]),
@@ -240,6 +246,19 @@ class OracleMeca:
 
     def confine_sphere(self, radius, stiffness):
         self.L.orc_confine_sphere(self.h, radius, stiffness)
+
+    def step_couples(self, fiber_a, abs_a, fiber_b, abs_b, stiffness, state, max_speed, stall_force,
+                     limit_speed=True, hold_growing_end=False):
+        """Couple::stepAA/AF/FA with motor hands (couple.cc:125-150); abs_a,
+        abs_b (float64) and state (uint8) are updated in place."""
+        fa = np.ascontiguousarray(fiber_a, dtype=np.uint32)
+        fb = np.ascontiguousarray(fiber_b, dtype=np.uint32)
+        w = np.ascontiguousarray(stiffness, dtype=np.float64)
+        for arr, t in ((abs_a, np.float64), (abs_b, np.float64), (state, np.uint8)):
+            assert arr.dtype == t and arr.flags.c_contiguous
+        mp = Motor(float(max_speed), float(stall_force), int(bool(limit_speed)), int(bool(hold_growing_end)))
+        self.L.orc_step_couples(self.h, fa.size, fa.ctypes.data, abs_a, fb.ctypes.data, abs_b, w,
+                                state.ctypes.data, C.byref(mp))
 
     def setup(self, gauss=None):
         g = None if gauss is None else np.ascontiguousarray(gauss, dtype=np.float64)
@@ -425,7 +444,10 @@ def assemble_scene(m, sc):
     if sc.nlinks:
         a1, a2, ca = sc.interpolate(sc.link_fiber_a, sc.link_abs_a)
         b1, b2, cb = sc.interpolate(sc.link_fiber_b, sc.link_abs_b)
+        state = getattr(sc, "link_state", None)
         for k in range(sc.nlinks):
+            if state is not None and (int(state[k]) & 3) != 3:
+                continue      # only bridging couples interact (couple.cc:87-92)
             m.inter_link_ii(int(a1[k]), int(a2[k]), float(ca[k]), int(b1[k]), int(b2[k]),
                             float(cb[k]), float(sc.link_stiffness[k]))
     for k in range(getattr(sc, "naster_links", 0)):
