@@ -124,6 +124,131 @@ __global__ void k_csr_vals(uint32_t n2, const uint32_t* __restrict__ rowptr, uin
 
 static inline dim3 grid_for(size_t n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
 
+// A stream made only of the slots of one sphere-confinement emitter
+// (k_emit_confine: D(D+1)/2 mC slots and D vBAS slots per fiber point, in
+// point order, the lower entries of the point's own block in (column, row)
+// order, invalid keys for points inside) is already in key order and its keys
+// are unique: the sort, run-length and CSR mirror reduce to a compaction of
+// the confined points.  Each sum is 0.0 + its single contribution, as the
+// in-order run sum computes it.
+constexpr uint32_t kRankThreads = 1024;
+
+// exclusive count of confined points before each point (one workgroup)
+__global__ __launch_bounds__(kRankThreads) void k_confine_rank(uint32_t npts, uint32_t stride,
+                                                               const uint64_t* __restrict__ key,
+                                                               uint32_t* __restrict__ rank)
+{
+    __shared__ uint32_t part[kRankThreads];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (npts + kRankThreads - 1) / kRankThreads;
+    const uint32_t b = min(npts, t * per), e = min(npts, b + per);
+    uint32_t cnt = 0;
+    for (uint32_t p = b; p < e; ++p) cnt += key[(size_t)stride * p] != kInvalidKey;
+    part[t] = cnt;
+    __syncthreads();
+    for (uint32_t o = 1; o < kRankThreads; o <<= 1) {
+        const uint32_t v = t >= o ? part[t - o] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - cnt;
+    for (uint32_t p = b; p < e; ++p) {
+        rank[p] = run;
+        run += key[(size_t)stride * p] != kInvalidKey;
+    }
+    if (t == kRankThreads - 1) rank[npts] = part[t];
+}
+
+// unique lower entries and the full symmetric CSR of a confinement-only mC
+template <int D>
+__global__ void k_confine_csr(uint32_t npts, uint32_t size, const uint64_t* __restrict__ key,
+                              const double* __restrict__ val, const uint32_t* __restrict__ rank,
+                              uint64_t* __restrict__ low_key, double* __restrict__ low_val,
+                              uint32_t* __restrict__ nruns, uint32_t* __restrict__ rowptr,
+                              uint32_t* __restrict__ col, uint32_t* __restrict__ map, double* __restrict__ cval)
+{
+    constexpr uint32_t NC = D * (D + 1) / 2;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nconf = rank[npts];
+    if (p == 0) nruns[0] = NC * nconf;
+    // rows past the fiber points (Solid points) and the end pointer
+    const uint32_t tail = size + 1 - D * npts;
+    if (p < tail) rowptr[D * npts + p] = D * D * nconf;
+    if (p >= npts) return;
+    const uint32_t r0 = rank[p];
+    const bool on = key[(size_t)NC * p] != kInvalidKey;
+#pragma unroll
+    for (int i = 0; i < D; ++i) rowptr[D * p + i] = D * D * r0 + (on ? D * i : 0u);
+    if (!on) return;
+    const uint32_t base = NC * r0;
+#pragma unroll
+    for (uint32_t q = 0; q < NC; ++q) {
+        low_key[base + q] = key[(size_t)NC * p + q];
+        low_val[base + q] = 0.0 + val[(size_t)NC * p + q];
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            const int a = i < j ? i : j, b = i < j ? j : i;
+            const uint32_t lo = base + (uint32_t)(a * D - a * (a - 1) / 2 + (b - a));
+            const uint32_t slot = D * D * r0 + D * i + j;
+            col[slot] = D * p + j;
+            map[slot] = lo;
+            cval[slot] = 0.0 + val[(size_t)NC * p + (lo - base)];
+        }
+}
+
+// vBAS of a confinement-only stream
+template <int D>
+__global__ void k_confine_bas(uint32_t npts, const uint64_t* __restrict__ key, const double* __restrict__ val,
+                              double* __restrict__ bas)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts || key[(size_t)D * p] == kInvalidKey) return;
+#pragma unroll
+    for (int d = 0; d < D; ++d) bas[D * p + d] = 0.0 + val[(size_t)D * p + d];
+}
+
+static void confine_only_mC(Context& c, DevSym& m)
+{
+    const int D = c.dim;
+    const uint32_t npts = c.fib_pts, NC = (uint32_t)(D * (D + 1) / 2);
+    hipStream_t st = c.stream;
+    c.conf_rank.resize(npts + 1);
+    m.nruns.resize(1);
+    m.low_key.resize((size_t)NC * npts);
+    m.low_val.resize((size_t)NC * npts);
+    m.rowptr.resize(m.size + 1);
+    m.col.resize((size_t)D * D * npts);
+    m.map.resize((size_t)D * D * npts);
+    m.val.resize((size_t)D * D * npts);
+    hipLaunchKernelGGL(k_confine_rank, dim3(1), dim3(kRankThreads), 0, st, npts, NC, c.s_key[1].get(),
+                       c.conf_rank.get());
+    const uint32_t threads = std::max(npts, m.size + 1 - D * npts);
+    if (D == 3)
+        hipLaunchKernelGGL(k_confine_csr<3>, grid_for(threads), dim3(256), 0, st, npts, m.size, c.s_key[1].get(),
+                           c.s_val[1].get(), c.conf_rank.get(), m.low_key.get(), m.low_val.get(), m.nruns.get(),
+                           m.rowptr.get(), m.col.get(), m.map.get(), m.val.get());
+    else
+        hipLaunchKernelGGL(k_confine_csr<2>, grid_for(threads), dim3(256), 0, st, npts, m.size, c.s_key[1].get(),
+                           c.s_val[1].get(), c.conf_rank.get(), m.low_key.get(), m.low_val.get(), m.nruns.get(),
+                           m.rowptr.get(), m.col.get(), m.map.get(), m.val.get());
+    CYMG_HIP(hipGetLastError());
+}
+
+static void confine_only_bas(Context& c)
+{
+    if (c.dim == 3)
+        hipLaunchKernelGGL(k_confine_bas<3>, grid_for(c.fib_pts), dim3(256), 0, c.stream, c.fib_pts,
+                           c.s_key[2].get(), c.s_val[2].get(), c.d_bas.get());
+    else
+        hipLaunchKernelGGL(k_confine_bas<2>, grid_for(c.fib_pts), dim3(256), 0, c.stream, c.fib_pts,
+                           c.s_key[2].get(), c.s_val[2].get(), c.d_bas.get());
+    CYMG_HIP(hipGetLastError());
+}
+
 // sort + run-length + in-order sums of one stream; returns the unique entries
 // in m.low_*.  reuse: the keys are those of the last build of m, whose
 // permutation and runs are kept in m.perm / m.ukey / m.runoff / m.runlen.
@@ -250,6 +375,19 @@ void build_streams(Context& c)
     }
     for (int w = 0; w < 3; ++w) copy_host(w, hs[w]->key.size());
 
+    // streams fed by one sphere-confinement emitter and nothing else
+    bool conf_only[3] = { false, false, false };
+    {
+        int nconf = 0, nother = 0;
+        for (const EmitterMark& mk : c.marks) {
+            if (mk.kind == 1) ++nconf;
+            else if (mk.kind != 2 && mk.kind != 3) ++nother;
+        }
+        const bool one = nconf == 1 && nother == 0 && c.fib_pts > 0;
+        conf_only[1] = one && hs[1]->key.empty() && total[1] == NC * c.fib_pts;
+        conf_only[2] = one && hs[2]->key.empty() && total[2] == (size_t)D * c.fib_pts;
+    }
+
     // what each stream's keys are built from; a confinement emitter's keys
     // follow the positions, so mC and vBAS are rebuilt whenever it contributes
     DevSym* all[3] = { &c.mB, &c.mC, &c.mBas };
@@ -279,12 +417,18 @@ void build_streams(Context& c)
         DevSym& m = *mats[w];
         m.used = total[w] > 0;
         if (!m.used) continue;
+        if (conf_only[w]) {
+            confine_only_mC(c, m);
+            continue;
+        }
         reduce_stream(c, w, m, 32 + bits_for(m.size), reuse[w]);
         build_csr(c, m, reuse[w]);
     }
     // vBAS
     CYMG_HIP(hipMemsetAsync(c.d_bas.get(), 0, (size_t)c.n() * sizeof(double), c.stream));
-    if (total[2]) {
+    if (total[2] && conf_only[2]) {
+        confine_only_bas(c);
+    } else if (total[2]) {
         DevSym& m = c.mBas;
         m.size = c.n();
         reduce_stream(c, 2, m, bits_for(m.size), reuse[2]);
