@@ -1132,7 +1132,7 @@ int cymg_set_option(cymg_context* ctx, int option, int value)
         if (option == CYMG_OPT_KRYLOV_LAUNCHES && (value == 2 || value == 4)) c.krylov_fused = value == 2;
         else if (option == CYMG_OPT_BLOCK_INVERSE && value >= 0 && value <= 3) c.pc_method = value;
         else if (option == CYMG_OPT_GRAPHS && (value == 0 || value == 1)) c.use_graphs = value;
-        else if (option == CYMG_OPT_TILE_POINTS && value >= 0 && value <= kMaxFiberPoints) {
+        else if (option == CYMG_OPT_TILE_POINTS && value >= 0 && value <= kTileMaxPoints) {
             c.tile_points = (uint32_t)value;
             if (!c.layout_dirty && c.nfib) build_tiles(c);
         }
@@ -1170,9 +1170,9 @@ int cymg_autotune(cymg_context* ctx, int reps, int* tile_points, int* block_inve
         int best_tile = (int)c.tile_points;
         double best = 1e300;
         if (!c.nsol) {
-            const int cands[] = { 0, 16, 24, 32, 48, 64, 96, 128 };
+            const int cands[] = { 0, 16, 24, 32, 48, 64, 96, 128, 170 };
             for (int t : cands) {
-                if (t > kMaxFiberPoints) continue;
+                if (t > kTileMaxPoints) continue;
                 c.tile_points = (uint32_t)t;
                 build_tiles(c);
                 const double ms = krylov_time_launch(c, precision, reps);

@@ -15,7 +15,7 @@ namespace cymg {
 
 constexpr int kTileThreads = 256;
 constexpr int kTileWaves = kTileThreads / kWave;
-constexpr int kTileMaxPoints = kMaxFiberPoints;
+constexpr int kTileMaxPoints = 170;   // 510 unknowns in 3D: one per thread of the 512-thread launches
 constexpr int kTileMaxFibers = 64;
 
 // The tiling (TileDesc, one per tile, read with one scalar load).

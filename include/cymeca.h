@@ -232,7 +232,8 @@ int         cymg_get_matrix(cymg_context* ctx, int which, uint32_t* nnz, uint32_
      blocked Gauss-Jordan in LDS with the panel updates on the matrix cores
      (blocks beyond 96 unknowns use 2);
    CYMG_OPT_TILE_POINTS: target points per tile of whole fibers in the
-     two-launch iteration, 1..CYMG_MAX_FIBER_POINTS (1: one fiber per tile),
+     two-launch iteration, 1..170 (1: one fiber per tile; above 256/DIM points
+     each unknown gets one thread instead of two, which suits many tiles),
      0 (default): chosen from the layout (or by cymg_autotune);
    CYMG_OPT_GRAPHS: 1 (default) replays chunks of 2 and 8 iterations of the
      two-launch iteration as captured HIP graphs, 0 launches every kernel */
@@ -241,7 +242,7 @@ int         cymg_set_option(cymg_context* ctx, int option, int value);
 /* autotuner (no counterpart in the reference): on the current system (after
    setup), times the block-inverse methods (Gauss-Jordan in registers / in
    LDS / blocked on the matrix cores) and the tile shapes of the two-launch iteration (target points per
-   tile 16..128 and the layout default, `reps` launches each) and keeps the
+   tile 16..170 and the layout default, `reps` launches each) and keeps the
    fastest; reports the choices and the chosen launch time (ms) */
 int         cymg_autotune(cymg_context* ctx, int reps, int* tile_points, int* block_inverse, double* launch_ms);
 

@@ -243,11 +243,12 @@ def test_crosslinked_step_fp64(pkg, scenes, oracle, dim, launches, inverse):
     assert rel_err(gpu.vector(pkg.VEC_FOR), orc.vector(oracle.FOR)) < 1e-7
 
 
-@pytest.mark.parametrize("tile_points", [1, 32, 128])
+@pytest.mark.parametrize("tile_points", [1, 32, 128, 170])
 def test_tiles_of_fibers(pkg, scenes, oracle, tile_points):
     """Two-launch iteration with one fiber per tile, tiles of <= 32 points
-    (several fibers, one wavefront solves their multipliers) and of <= 128
-    points (more than 64 segments: multipliers solved through LDS)."""
+    (several fibers, one wavefront solves their multipliers), of <= 128
+    points (more than 64 segments: multipliers solved through LDS) and of
+    <= 170 points (up to 510 unknowns, one thread each)."""
     sc = scenes.config2(seed=24, nfib=80, nlinks=400, radius=4.0, lmin=2.0, lmax=20.0)
     gpu, orc, reps = _step_pair(pkg, oracle, sc, 2, 1e-11, 1, tile_points=tile_points)
     for r1, r2 in reps:
@@ -329,7 +330,7 @@ def test_autotune_then_step_matches_oracle(pkg, scenes, oracle):
     gpu.compute_preconditioner(pkg.PRECOND_FP64)
     print("autotune", flush=True)
     t = gpu.autotune(reps=6)
-    assert t["tile_points"] in (0, 16, 24, 32, 48, 64, 96, 128)
+    assert t["tile_points"] in (0, 16, 24, 32, 48, 64, 96, 128, 170)
     assert t["block_inverse"] in (0, 2, 3) and t["launch_ms"] > 0
     r1 = gpu.solve(1e-11, sc.gauss(0), tol_mode=pkg.TOL_RELATIVE)
     orc = oracle_from_scene(oracle, sc)
