@@ -298,7 +298,7 @@ def main():
                   "unknowns": ast.size, "steps_per_s": round(args.aster_steps / adt, 3),
                   "ms_per_step": round(1e3 * adt / args.aster_steps, 3),
                   "mean_bicgstab_iters": float(np.mean(its)),
-                  "iteration": "four launches per iteration (Solid rows)",
+                  "iteration": "two launches per iteration (a tile per Solid)",
                   "motors": f"stepped on the device between solves (speed {ast.motor_max_speed} um/s, "
                             f"stall {ast.motor_stall_force} pN); {int(((cst & 3) == 3).sum())} of "
                             f"{cst.size} couples still bridging"}

@@ -124,12 +124,30 @@ void build_tiles(Context& c)
             ++nf;
         }
     }
-    c.ntiles = (uint32_t)c.h_tile_fib.size();
+    const uint32_t nft = (uint32_t)c.h_tile_fib.size();
+    // one tile per Solid after the fiber tiles (a layout with Solids is one
+    // system), when every Solid fits a workgroup of the iteration launches
+    c.solid_tiles = c.nsol > 0;
+    for (uint32_t k = 0; k < c.nsol; ++k)
+        if (c.dim * c.h_sol_np[k] > 1024u) c.solid_tiles = false;
+    c.ntiles = nft + (c.solid_tiles ? c.nsol : 0);
     c.h_sys_tile[c.nsys] = c.ntiles;
     c.h_tile_fib.push_back(c.nfib);
     std::vector<TileDesc> desc(c.ntiles);
     for (uint32_t t = 0; t < c.ntiles; ++t) {
         TileDesc& d = desc[t];
+        if (t >= nft) {
+            const uint32_t k = t - nft;
+            d.fb = c.nfib + k;       // the Solid's Mecable
+            d.nft = 0;               // no fiber: a Solid tile
+            d.P0 = c.h_sol_idx[k];
+            d.tp = c.h_sol_np[k];
+            d.sys = 0;
+            d.tb = c.h_sys_tile[0];
+            d.te = c.h_sys_tile[1];
+            d.S0 = 0;
+            continue;
+        }
         const uint32_t fb = c.h_tile_fib[t], fe = c.h_tile_fib[t + 1];
         d.fb = fb;
         d.nft = fe - fb;

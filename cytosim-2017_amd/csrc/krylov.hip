@@ -735,6 +735,67 @@ __device__ __forceinline__ bool first_tile_of_system(const Tile& T)
     return T.t == T.tb;
 }
 
+// ---------------------------------------------------------------------------
+// Solid tiles of the two-launch iteration: a tile without fibers (nft == 0)
+// holds one Solid (Mecable fb), after the fiber tiles of its system.  Its
+// rows follow the recurrences of the fiber rows; the operator's Solid rows
+// are the operand plus the rigid motion that the moments of the coupling
+// forces produce (solid_operator) and the block inverse is I - G W
+// (solid_precond).  Thread t owns the unknowns t, t + blockDim.x, ...; the
+// operand at coupled unknowns outside the Solid comes from their records, as
+// in the fiber tiles.  The per-launch partial sums go to the tile's slots.
+constexpr uint32_t kSolidTileRows = 1024;
+
+struct SolidTileLds {
+    double a[kSolidTileRows];       // operand of the Solid's rows
+    double y[kSolidTileRows];       // operator result
+    double e[kSolidTileRows];       // a row value needed with the block apply's output
+    double red[16 * 6];
+    double bc[8];
+    double wr[5 * 16];
+    uint32_t lrow[kSolidTileRows + 1];
+};
+
+// k_tinit for a Solid tile
+template <int D, int PC>
+__device__ void solid_tile_init(const KArgs& k, const Tile& T, const double* __restrict__ x0, SolidTileLds& S)
+{
+    const uint32_t ks = T.fb - k.nfiber, U0 = T.U0, nu = T.nu;
+    const auto X0 = [&](uint32_t i) -> double { return x0 ? x0[i] : 0.0; };
+    if (x0) {
+        const auto O = [&](uint32_t i, double v) { S.y[i - U0] = v; };
+        solid_operator<D>(k.fa, k.sa, ks, X0, O, S.red, S.lrow, kSolidTileRows + 1);
+        __syncthreads();
+    }
+    double rr = 0.0;
+    for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x) {
+        const uint32_t g = U0 + u;
+        double ri = k.b[g];
+        if (x0) ri = ri + (-1.0) * S.y[u];
+        k.r[g] = ri;
+        k.rt[g] = ri;
+        k.x[g] = X0(g);
+        S.e[u] = ri;
+        rr += ri * ri;
+    }
+    rr = wave_sum(rr);
+    if ((threadIdx.x & (kWave - 1)) == 0) S.wr[threadIdx.x / kWave] = rr;
+    __syncthreads();
+    double* y0 = reinterpret_cast<double*>(k.g2);     // (y, z) records: y = M^-1 r
+    if (PC == CYMG_PRECOND_NONE) {
+        for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x) y0[2 * (size_t)(U0 + u)] = S.e[u];
+    } else {
+        const auto In = [&](uint32_t i) -> double { return S.e[i - U0]; };
+        const auto Out = [&](uint32_t i, double v) { y0[2 * (size_t)i] = v; };
+        solid_precond<D>(k.sa, ks, In, Out, S.red);
+    }
+    if (threadIdx.x == 0) {
+        double t = S.wr[0];
+        for (uint32_t q = 1; q < blockDim.x / kWave; ++q) t += S.wr[q];
+        k.part[P_RR * k.ta.ntiles + T.t] = t;
+    }
+}
+
 // r = b - M x0 (or b), rt = r, x = x0 (or 0), y = M^-1 r; <rt, r> partials;
 // the first tile of each system sets the threshold from the setup partials
 template <int D, int PC>
@@ -744,8 +805,13 @@ __global__ __launch_bounds__(kTileThreads) void k_tinit(KArgs k, const double* _
     constexpr int R = TileRows<D>::R;
     __shared__ TileLds<D> L;
     __shared__ double wr[kTileWaves];
+    __shared__ SolidTileLds SL;
     const FiberArgs& a = k.fa;
     const Tile T = tile_begin<D>(k.ta, blockIdx.x);
+    if (T.nft == 0) {
+        solid_tile_init<D, PC>(k, T, x0, SL);
+        return;
+    }
     tile_tables<PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
     for (int u = threadIdx.x; u < (int)T.nu; u += kTileThreads) L.x[u] = x0 ? x0[T.U0 + u] : 0.0;
     __syncthreads();
@@ -1180,14 +1246,286 @@ __device__ __forceinline__ double tp_block_apply(LDS& L, const Tile& T, const Tp
     return R.split ? part + L.hb[R.u] : part;
 }
 
+// Partial sums of the previous launch re-reduced by wavefront 1 of every tile
+// of the system (same order everywhere, so every tile derives the same
+// scalars), and the scalar updates of the P and Q launches.
+template <bool FIRST>
+__device__ __forceinline__ void tp_partials(const KArgs& k, const Tile& T, double* bc)
+{
+    const uint32_t nt = k.ta.ntiles;
+    if (threadIdx.x >= kWave && threadIdx.x < 2 * kWave) {
+        if (FIRST) {
+            const double rr = wsum(k.part + P_RR * nt, T.tb, T.te);
+            if (threadIdx.x == kWave) bc[0] = rr;
+        } else {
+            double S4[4];
+            wsum4(k.part4, T.tb, T.te, S4);
+            const double m = wmax(k.part + P_SMAX * nt, T.tb, T.te);
+            if (threadIdx.x == kWave) { bc[0] = S4[0]; bc[1] = S4[1]; bc[2] = S4[2]; bc[3] = S4[3]; bc[4] = m; }
+        }
+    }
+}
+
+__device__ __forceinline__ void tp_scalars(SysScalars& sc, const double* bc)
+{
+    sc.pending = 0;
+    if (sc.stage == ST_INIT) {
+        sc.rho2 = sc.rho1;
+        sc.rho1 = bc[0];
+        sc.beta = (sc.rho1 / sc.rho2) * (sc.alpha / sc.omega);
+    } else {
+        sc.resid_s = (bc[4] != bc[4]) ? __builtin_inf() : bc[4];
+        const double tt = bc[0], ts = bc[1], rts = bc[2], rtt = bc[3];
+        if (tt == 0.0) {
+            sc.done = 1; sc.flag = 0; sc.resid = sc.resid_s;
+        } else {
+            const double om = ts / tt;
+            if (om == 0.0) {
+                sc.done = 1; sc.flag = 3; sc.resid = sc.resid_s;
+            } else {
+                sc.omega = om;
+                sc.pending = 1;
+                sc.rho2 = sc.rho1;
+                sc.rho1 = rts - om * rtt;
+                sc.beta = (sc.rho1 / sc.rho2) * (sc.alpha / sc.omega);
+                sc.iter += 1;
+            }
+        }
+    }
+    sc.stage = ST_A;
+}
+
+__device__ __forceinline__ void tq_partials(const KArgs& k, const Tile& T, double* bc)
+{
+    const uint32_t nt = k.ta.ntiles;
+    if (threadIdx.x >= kWave && threadIdx.x < 2 * kWave) {
+        const double res = wmax(k.part + P_RMAX * nt, T.tb, T.te);
+        const double dsum = wsum(k.part + P_DELTA * nt, T.tb, T.te);
+        if (threadIdx.x == kWave) { bc[0] = res; bc[1] = dsum; }
+    }
+}
+
+template <int PC>
+__device__ __forceinline__ void tq_scalars(SysScalars& sc, const double* bc)
+{
+    decide(sc, bc[0]);
+    if (!sc.done) {
+        const double delta = bc[1];
+        if (PC != CYMG_PRECOND_NONE && delta == 0.0) { sc.done = 1; sc.flag = 4; }
+        else sc.alpha = sc.rho1 / delta;
+    }
+    sc.stage = ST_C;
+}
+
+// sums of the wavefronts' partials in wavefront order (NaN-propagating max)
+__device__ __forceinline__ double waves_max(const double* w, int nw)
+{
+    double m = w[0];
+    for (int q = 1; q < nw; ++q) m = nan_max(m, w[q]);
+    return m;
+}
+__device__ __forceinline__ double waves_sum(const double* w, int nw)
+{
+    double t = w[0];
+    for (int q = 1; q < nw; ++q) t += w[q];
+    return t;
+}
+
+constexpr int kSolidRowsPerThread = kSolidTileRows / kTpThreads;
+
+// k_tp for a Solid tile
+template <int D, int PC, bool FIRST>
+__device__ void solid_tile_p(const KArgs& k, const Tile& T, SolidTileLds& S)
+{
+    constexpr int RS = kSolidRowsPerThread;
+    const uint32_t ks = T.fb - k.nfiber, U0 = T.U0, nu = T.nu, nt = k.ta.ntiles;
+    const int nw = blockDim.x / kWave;
+    double vr[RS], vt[RS], vx[RS], vrt[RS], vy[RS];
+    double4 q1[RS];
+#pragma unroll
+    for (int q = 0; q < RS; ++q) {
+        const uint32_t u = threadIdx.x + q * kTpThreads;
+        vr[q] = vt[q] = vx[q] = vrt[q] = vy[q] = 0.0;
+        q1[q] = make_double4(0, 0, 0, 0);
+        if (u < nu) {
+            const uint32_t g = U0 + u;
+            vr[q] = k.r[g]; vt[q] = k.t[g]; vx[q] = k.x[g]; vrt[q] = k.rt[g];
+            q1[q] = k.g1a[g];
+            if (FIRST) vy[q] = k.g2[g].x;
+        }
+    }
+    SysScalars sc = state_load(k.st_in + T.sys);
+    tp_partials<FIRST>(k, T, S.bc);
+    __syncthreads();
+    if (sc.done) return;
+    tp_scalars(sc, S.bc);
+    if (sc.done) return;
+    double rmax = 0.0, dl = 0.0;
+#pragma unroll
+    for (int q = 0; q < RS; ++q) {
+        const uint32_t u = threadIdx.x + q * kTpThreads;
+        if (u >= nu) continue;
+        const uint32_t g = U0 + u;
+        double ri = vr[q], yi = vy[q];
+        if (sc.pending) {
+            k.x[g] = vx[q] + sc.omega * q1[q].x;
+            ri = ri + (-sc.omega) * vt[q];
+            k.r[g] = ri;
+            yi = q1[q].x + (-sc.omega) * q1[q].y;
+        }
+        rmax = nan_max(rmax, fabs(ri));
+        S.a[u] = sc.beta == 0.0 ? yi : yi + sc.beta * (q1[q].z + (-sc.omega) * q1[q].w);
+        S.e[u] = yi;
+    }
+    __syncthreads();
+    const auto O = [&](uint32_t i, double v) { S.y[i - U0] = v; };
+    if constexpr (FIRST) {
+        const PhatFirstComb C{ sc.beta };
+        const auto X = [&](uint32_t i) -> double { return i - U0 < nu ? S.a[i - U0] : C(k.g2[i]); };
+        solid_operator<D>(k.fa, k.sa, ks, X, O, S.red, S.lrow, kSolidTileRows + 1);
+    } else {
+        const PhatComb C{ sc.beta, sc.omega };
+        const auto X = [&](uint32_t i) -> double { return i - U0 < nu ? S.a[i - U0] : C(k.g1a[i]); };
+        solid_operator<D>(k.fa, k.sa, ks, X, O, S.red, S.lrow, kSolidTileRows + 1);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RS; ++q) {
+        const uint32_t u = threadIdx.x + q * kTpThreads;
+        if (u >= nu) continue;
+        const double v = S.y[u];
+        k.v[U0 + u] = v;
+        dl += vrt[q] * v;
+    }
+    rmax = wave_max(rmax);
+    dl = wave_sum(dl);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        S.wr[threadIdx.x / kWave] = rmax;
+        S.wr[16 + threadIdx.x / kWave] = dl;
+    }
+    // z = M^-1 v; records (y, z) and (phat, z)
+    const auto Rec = [&](uint32_t i, double z) {
+        const uint32_t u = i - U0;
+        k.g2[i] = make_double2(S.e[u], z);
+        reinterpret_cast<double2*>(k.g1b + i)[1] = make_double2(S.a[u], z);
+    };
+    if (PC == CYMG_PRECOND_NONE) {
+        for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x) Rec(U0 + u, S.y[u]);
+        __syncthreads();
+    } else {
+        const auto In = [&](uint32_t i) -> double { return S.y[i - U0]; };
+        solid_precond<D>(k.sa, ks, In, Rec, S.red);
+    }
+    if (threadIdx.x == 0) {
+        k.part[P_RMAX * nt + T.t] = waves_max(S.wr, nw);
+        k.part[P_DELTA * nt + T.t] = waves_sum(S.wr + 16, nw);
+    }
+}
+
+// k_tq for a Solid tile
+template <int D, int PC>
+__device__ void solid_tile_q(const KArgs& k, const Tile& T, SolidTileLds& S)
+{
+    constexpr int RS = kSolidRowsPerThread;
+    const uint32_t ks = T.fb - k.nfiber, U0 = T.U0, nu = T.nu, nt = k.ta.ntiles;
+    const int nw = blockDim.x / kWave;
+    double vr[RS], vv[RS], vx[RS], vph[RS], vrt[RS], si[RS];
+    double2 q2[RS];
+#pragma unroll
+    for (int q = 0; q < RS; ++q) {
+        const uint32_t u = threadIdx.x + q * kTpThreads;
+        vr[q] = vv[q] = vx[q] = vph[q] = vrt[q] = si[q] = 0.0;
+        q2[q] = make_double2(0, 0);
+        if (u < nu) {
+            const uint32_t g = U0 + u;
+            vr[q] = k.r[g]; vv[q] = k.v[g]; vx[q] = k.x[g]; vrt[q] = k.rt[g];
+            q2[q] = k.g2[g];
+            vph[q] = k.g1b[g].z;
+        }
+    }
+    SysScalars sc = state_load(k.st_in + T.sys);
+    tq_partials(k, T, S.bc);
+    __syncthreads();
+    if (sc.done) return;
+    tq_scalars<PC>(sc, S.bc);
+    if (sc.done) return;
+    double smax = 0.0;
+#pragma unroll
+    for (int q = 0; q < RS; ++q) {
+        const uint32_t u = threadIdx.x + q * kTpThreads;
+        if (u >= nu) continue;
+        const uint32_t g = U0 + u;
+        si[q] = vr[q] + (-sc.alpha) * vv[q];
+        k.r[g] = si[q];
+        k.x[g] = vx[q] + sc.alpha * vph[q];
+        smax = nan_max(smax, fabs(si[q]));
+        S.a[u] = q2[q].x + (-sc.alpha) * q2[q].y;
+    }
+    __syncthreads();
+    {
+        const ShatComb C{ sc.alpha };
+        const auto X = [&](uint32_t i) -> double { return i - U0 < nu ? S.a[i - U0] : C(k.g2[i]); };
+        const auto O = [&](uint32_t i, double v) { S.y[i - U0] = v; };
+        solid_operator<D>(k.fa, k.sa, ks, X, O, S.red, S.lrow, kSolidTileRows + 1);
+    }
+    __syncthreads();
+    double tt = 0, ts = 0, rts = 0, rtt = 0;
+#pragma unroll
+    for (int q = 0; q < RS; ++q) {
+        const uint32_t u = threadIdx.x + q * kTpThreads;
+        if (u >= nu) continue;
+        const double ti = S.y[u];
+        k.t[U0 + u] = ti;
+        tt += ti * ti;
+        ts += ti * si[q];
+        rts += vrt[q] * si[q];
+        rtt += vrt[q] * ti;
+    }
+    smax = wave_max(smax);
+    tt = wave_sum(tt);
+    ts = wave_sum(ts);
+    rts = wave_sum(rts);
+    rtt = wave_sum(rtt);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        const int w = threadIdx.x / kWave;
+        S.wr[w] = smax;
+        S.wr[16 + w] = tt;
+        S.wr[32 + w] = ts;
+        S.wr[48 + w] = rts;
+        S.wr[64 + w] = rtt;
+    }
+    // w = M^-1 t; records (shat, w)
+    const auto Rec = [&](uint32_t i, double w) {
+        reinterpret_cast<double2*>(k.g1b + i)[0] = make_double2(S.a[i - U0], w);
+    };
+    if (PC == CYMG_PRECOND_NONE) {
+        for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x) Rec(U0 + u, S.y[u]);
+        __syncthreads();
+    } else {
+        const auto In = [&](uint32_t i) -> double { return S.y[i - U0]; };
+        solid_precond<D>(k.sa, ks, In, Rec, S.red);
+    }
+    if (threadIdx.x == 0) {
+        k.part[P_SMAX * nt + T.t] = waves_max(S.wr, nw);
+        *reinterpret_cast<double4*>(k.part4 + 4 * (size_t)T.t) =
+            make_double4(waves_sum(S.wr + 16, nw), waves_sum(S.wr + 32, nw), waves_sum(S.wr + 48, nw),
+                         waves_sum(S.wr + 64, nw));
+    }
+}
+
 template <int D, int PC, bool FIRST>
 __global__ __launch_bounds__(kTpThreads) void k_tp(KArgs k)
 {
     using TB = typename PcBlocks<PC>::T;
     using Rec = typename std::conditional<FIRST, double2, double4>::type;
     __shared__ TpLds<D> L;
+    __shared__ SolidTileLds SL;
     const FiberArgs& a = k.fa;
     const Tile T = tile_begin<D>(k.ta, blockIdx.x);
+    if (T.nft == 0) {
+        solid_tile_p<D, PC, FIRST>(k, T, SL);
+        return;
+    }
     TpRow<D, TB> R;
     tp_roles<D>(T, R.u, R.act, R.own, R.split, R.h);
     // prologue: everything independent of the iteration scalars
@@ -1221,48 +1559,14 @@ __global__ __launch_bounds__(kTpThreads) void k_tp(KArgs k)
     SysScalars sc = state_load(k.st_in + T.sys);
     tile_tables<PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
     const uint32_t nt = k.ta.ntiles;
-    if (threadIdx.x >= kWave && threadIdx.x < 2 * kWave) {
-        if (FIRST) {
-            const double rr = wsum(k.part + P_RR * nt, T.tb, T.te);
-            if (threadIdx.x == kWave) L.bc[0] = rr;
-        } else {
-            double S4[4];
-            wsum4(k.part4, T.tb, T.te, S4);
-            const double m = wmax(k.part + P_SMAX * nt, T.tb, T.te);
-            if (threadIdx.x == kWave) { L.bc[0] = S4[0]; L.bc[1] = S4[1]; L.bc[2] = S4[2]; L.bc[3] = S4[3]; L.bc[4] = m; }
-        }
-    }
+    tp_partials<FIRST>(k, T, L.bc);
     __syncthreads();
     const bool first = T.t == T.tb;
     if (sc.done) {
         if (first && threadIdx.x == 0) state_store(k.st_out + T.sys, sc);
         return;
     }
-    sc.pending = 0;
-    if (sc.stage == ST_INIT) {
-        sc.rho2 = sc.rho1;
-        sc.rho1 = L.bc[0];
-        sc.beta = (sc.rho1 / sc.rho2) * (sc.alpha / sc.omega);
-    } else {
-        sc.resid_s = (L.bc[4] != L.bc[4]) ? __builtin_inf() : L.bc[4];
-        const double tt = L.bc[0], ts = L.bc[1], rts = L.bc[2], rtt = L.bc[3];
-        if (tt == 0.0) {
-            sc.done = 1; sc.flag = 0; sc.resid = sc.resid_s;
-        } else {
-            const double om = ts / tt;
-            if (om == 0.0) {
-                sc.done = 1; sc.flag = 3; sc.resid = sc.resid_s;
-            } else {
-                sc.omega = om;
-                sc.pending = 1;
-                sc.rho2 = sc.rho1;
-                sc.rho1 = rts - om * rtt;
-                sc.beta = (sc.rho1 / sc.rho2) * (sc.alpha / sc.omega);
-                sc.iter += 1;
-            }
-        }
-    }
-    sc.stage = ST_A;
+    tp_scalars(sc, L.bc);
     if (first && threadIdx.x == 0) state_store(k.st_out + T.sys, sc);
     if (sc.done) return;
     tp_after_tables<D, TB>(a, L, T, PcBlocks<PC>::get(k), R);
@@ -1329,8 +1633,13 @@ __global__ __launch_bounds__(kTpThreads) void k_tq(KArgs k)
 {
     using TB = typename PcBlocks<PC>::T;
     __shared__ TpLds<D> L;
+    __shared__ SolidTileLds SL;
     const FiberArgs& a = k.fa;
     const Tile T = tile_begin<D>(k.ta, blockIdx.x);
+    if (T.nft == 0) {
+        solid_tile_q<D, PC>(k, T, SL);
+        return;
+    }
     TpRow<D, TB> R;
     tp_roles<D>(T, R.u, R.act, R.own, R.split, R.h);
     const uint32_t g = T.U0 + R.u;
@@ -1360,22 +1669,10 @@ __global__ __launch_bounds__(kTpThreads) void k_tq(KArgs k)
     SysScalars sc = state_load(k.st_in + T.sys);
     tile_tables<PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
     const uint32_t nt = k.ta.ntiles;
-    if (threadIdx.x >= kWave && threadIdx.x < 2 * kWave) {
-        const double res = wmax(k.part + P_RMAX * nt, T.tb, T.te);
-        const double dsum = wsum(k.part + P_DELTA * nt, T.tb, T.te);
-        if (threadIdx.x == kWave) { L.bc[0] = res; L.bc[1] = dsum; }
-    }
+    tq_partials(k, T, L.bc);
     __syncthreads();
     const bool first = T.t == T.tb;
-    if (!sc.done) {
-        decide(sc, L.bc[0]);
-        if (!sc.done) {
-            const double delta = L.bc[1];
-            if (PC != CYMG_PRECOND_NONE && delta == 0.0) { sc.done = 1; sc.flag = 4; }
-            else sc.alpha = sc.rho1 / delta;
-        }
-        sc.stage = ST_C;
-    }
+    if (!sc.done) tq_scalars<PC>(sc, L.bc);
     if (first && threadIdx.x == 0) state_store(k.st_out + T.sys, sc);
     if (sc.done) return;
     tp_after_tables<D, TB>(a, L, T, PcBlocks<PC>::get(k), R);
@@ -1754,7 +2051,8 @@ static void tiled_buffers(Context& c)
 
 double krylov_time_launch(Context& c, int precond, int launches)
 {
-    if (c.nsol) throw Error(CYMG_ERR_UNSUPPORTED, "the two-launch iteration holds fibers only");
+    if (c.nsol && !c.solid_tiles)
+        throw Error(CYMG_ERR_UNSUPPORTED, "a Solid is too large for the two-launch iteration");
     tiled_buffers(c);
     const int pairs = std::max(1, launches / 2);
     if (c.dim == 3) {
@@ -1770,8 +2068,9 @@ double krylov_time_launch(Context& c, int precond, int launches)
 void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter, double tol, int tol_mode,
                   cymg_solve_report* rep)
 {
-    // tiles hold whole fibers only: a system with Solids runs the four-launch iteration
-    if (c.krylov_fused && !c.nsol) {
+    // tiles hold whole fibers or one Solid of at most kSolidTileRows unknowns;
+    // a larger Solid runs the four-launch iteration
+    if (c.krylov_fused && (!c.nsol || c.solid_tiles)) {
         tiled_buffers(c);
         if (c.dim == 3) {
             if (precond == CYMG_PRECOND_FP64) run_tiled<3, CYMG_PRECOND_FP64>(c, guess_dev, max_iter, tol, tol_mode, rep);

@@ -68,8 +68,9 @@ def test_aster_operator_blocks_precondition(pkg, scenes, oracle, dim):
     gpu.close()
 
 
-def _aster_steps(pkg, oracle, sc, steps, tol, precondition, guess=0, init_guess=False):
+def _aster_steps(pkg, oracle, sc, steps, tol, precondition, guess=0, init_guess=False, launches=2):
     gpu = gpu_from_scene(pkg, sc)
+    gpu.set_option(pkg.OPT_KRYLOV_LAUNCHES, launches)
     if init_guess:
         gpu.guess_init(4)
     orc = oracle_from_scene(oracle, sc)
@@ -89,11 +90,13 @@ def _aster_steps(pkg, oracle, sc, steps, tol, precondition, guess=0, init_guess=
     return gpu, orc, reps
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_aster_steps_fp64(pkg, scenes, oracle, dim):
-    # 9 steps: every Solid gets at least one reshape (initial timers in [0, 7))
+@pytest.mark.parametrize("dim,launches", [(2, 2), (3, 2), (3, 4)])
+def test_aster_steps_fp64(pkg, scenes, oracle, dim, launches):
+    """9 steps (every Solid gets at least one reshape: initial timers in
+    [0, 7)) through the two-launch iteration (the Solids in tiles of their
+    own) and the four-launch one."""
     sc = scenes.mini_asters(dim=dim, seed=13)
-    gpu, orc, reps = _aster_steps(pkg, oracle, sc, 9, 1e-12, pkg.PRECOND_FP64)
+    gpu, orc, reps = _aster_steps(pkg, oracle, sc, 9, 1e-12, pkg.PRECOND_FP64, launches=launches)
     for r1, r2 in reps:
         assert r1["converged"] and r2["converged"]
     assert rel_err(gpu.vector(pkg.VEC_FOR), orc.vector(oracle.FOR)) < 1e-7
@@ -135,3 +138,25 @@ def test_solid_registration_rules(pkg):
     with pytest.raises(pkg.CymgError):
         m.set_systems([0, 1])               # no batches with Solids
     m.close()
+
+
+@pytest.mark.parametrize("precondition", [1, 2, 0])
+def test_aster_graph_replay_matches_eager(pkg, scenes, precondition):
+    """Solid tiles in the graph-replayed iteration chunks give the same bits as
+    the same launches issued one by one."""
+    sc = scenes.mini_asters(dim=3, seed=17)
+    out = []
+    for graphs in (1, 0):
+        gpu = gpu_from_scene(pkg, sc)
+        gpu.set_option(pkg.OPT_GRAPHS, graphs)
+        its = []
+        for s in range(3):
+            print(f"aster graph replay precond={precondition} graphs={graphs} step={s}", flush=True)
+            if s:
+                gpu.prepare(None)
+                pkg.assemble(gpu, sc, upload_couples=False)
+            its.append(gpu.solve(1e-8, sc.gauss(s), tol_mode=pkg.TOL_RELATIVE, precondition=precondition)["iterations"])
+        out.append((gpu.fiber_points(), its))
+        gpu.close()
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
