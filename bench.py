@@ -278,8 +278,11 @@ def main():
         ma = pkg.Meca(3, 0)
         pkg.load_scene(ma, ast)
         its = []
+        atune = None
         for s in range(2 + args.aster_steps):
             if s == 2:
+                if args.autotune:
+                    atune = ma.autotune(reps=6)      # untimed
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
             if s > 0:
@@ -299,6 +302,7 @@ def main():
                   "ms_per_step": round(1e3 * adt / args.aster_steps, 3),
                   "mean_bicgstab_iters": float(np.mean(its)),
                   "iteration": "two launches per iteration (a tile per Solid)",
+                  "untimed_setup": {"autotune": atune},
                   "motors": f"stepped on the device between solves (speed {ast.motor_max_speed} um/s, "
                             f"stall {ast.motor_stall_force} pN); {int(((cst & 3) == 3).sum())} of "
                             f"{cst.size} couples still bridging"}

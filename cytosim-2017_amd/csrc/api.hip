@@ -1187,7 +1187,7 @@ int cymg_autotune(cymg_context* ctx, int reps, int* tile_points, int* block_inve
         // with the shortest launch on the current system
         int best_tile = (int)c.tile_points;
         double best = 1e300;
-        if (!c.nsol) {
+        if (!c.nsol || c.solid_tiles) {
             const int cands[] = { 0, 16, 24, 32, 48, 64, 96, 128, 170 };
             for (int t : cands) {
                 if (t > kTileMaxPoints) continue;
