@@ -35,7 +35,11 @@ __global__ void k_iota(uint32_t n, uint32_t* __restrict__ out)
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = i;
 }
 
-// sequential in-order sum of each run of equal keys
+// sequential in-order sum of each run of equal keys; the contributions of a
+// long run (a point with many links) are loaded kRunBatch at a time before
+// they are added in order
+constexpr uint32_t kRunBatch = 16;
+
 __global__ void k_run_sum(uint32_t total, const uint32_t* __restrict__ nruns, const uint64_t* __restrict__ ukey,
                           const uint32_t* __restrict__ off, const uint32_t* __restrict__ len,
                           const uint32_t* __restrict__ perm, const double* __restrict__ val,
@@ -47,7 +51,18 @@ __global__ void k_run_sum(uint32_t total, const uint32_t* __restrict__ nruns, co
     double s = 0.0;
     if (k != kInvalidKey) {
         const uint32_t b = off[u], e = b + len[u];
-        for (uint32_t i = b; i < e; ++i) s = s + val[perm[i]];
+        uint32_t i = b;
+        for (; i + kRunBatch <= e; i += kRunBatch) {
+            uint32_t pq[kRunBatch];
+            double vq[kRunBatch];
+#pragma unroll
+            for (uint32_t j = 0; j < kRunBatch; ++j) pq[j] = perm[i + j];
+#pragma unroll
+            for (uint32_t j = 0; j < kRunBatch; ++j) vq[j] = val[pq[j]];
+#pragma unroll
+            for (uint32_t j = 0; j < kRunBatch; ++j) s = s + vq[j];
+        }
+        for (; i < e; ++i) s = s + val[perm[i]];
     }
     low_key[u] = k;
     low_val[u] = s;
@@ -131,33 +146,13 @@ static inline dim3 grid_for(size_t n, int b = 256) { return dim3((unsigned)((n +
 // are unique: the sort, run-length and CSR mirror reduce to a compaction of
 // the confined points.  Each sum is 0.0 + its single contribution, as the
 // in-order run sum computes it.
-constexpr uint32_t kRankThreads = 1024;
-
-// exclusive count of confined points before each point (one workgroup)
-__global__ __launch_bounds__(kRankThreads) void k_confine_rank(uint32_t npts, uint32_t stride,
-                                                               const uint64_t* __restrict__ key,
-                                                               uint32_t* __restrict__ rank)
+// 1 for a confined point (its first slot holds a key), 0 otherwise and past the end
+__global__ void k_confine_flags(uint32_t npts, uint32_t stride, const uint64_t* __restrict__ key,
+                                uint32_t* __restrict__ flag)
 {
-    __shared__ uint32_t part[kRankThreads];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (npts + kRankThreads - 1) / kRankThreads;
-    const uint32_t b = min(npts, t * per), e = min(npts, b + per);
-    uint32_t cnt = 0;
-    for (uint32_t p = b; p < e; ++p) cnt += key[(size_t)stride * p] != kInvalidKey;
-    part[t] = cnt;
-    __syncthreads();
-    for (uint32_t o = 1; o < kRankThreads; o <<= 1) {
-        const uint32_t v = t >= o ? part[t - o] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[t] - cnt;
-    for (uint32_t p = b; p < e; ++p) {
-        rank[p] = run;
-        run += key[(size_t)stride * p] != kInvalidKey;
-    }
-    if (t == kRankThreads - 1) rank[npts] = part[t];
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > npts) return;
+    flag[p] = p < npts && key[(size_t)stride * p] != kInvalidKey;
 }
 
 // unique lower entries and the full symmetric CSR of a confinement-only mC
@@ -224,8 +219,16 @@ static void confine_only_mC(Context& c, DevSym& m)
     m.col.resize((size_t)D * D * npts);
     m.map.resize((size_t)D * D * npts);
     m.val.resize((size_t)D * D * npts);
-    hipLaunchKernelGGL(k_confine_rank, dim3(1), dim3(kRankThreads), 0, st, npts, NC, c.s_key[1].get(),
-                       c.conf_rank.get());
+    // rank[p] = confined points before p, rank[npts] = all of them
+    c.conf_flag.resize(npts + 1);
+    hipLaunchKernelGGL(k_confine_flags, grid_for(npts + 1), dim3(256), 0, st, npts, NC, c.s_key[1].get(),
+                       c.conf_flag.get());
+    size_t tb = 0;
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, c.conf_flag.get(), c.conf_rank.get(), (int)(npts + 1),
+                                              st));
+    c.s_temp.resize(std::max(tb, c.s_temp.n));
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.s_temp.get(), tb, c.conf_flag.get(), c.conf_rank.get(),
+                                              (int)(npts + 1), st));
     const uint32_t threads = std::max(npts, m.size + 1 - D * npts);
     if (D == 3)
         hipLaunchKernelGGL(k_confine_csr<3>, grid_for(threads), dim3(256), 0, st, npts, m.size, c.s_key[1].get(),

@@ -60,7 +60,7 @@ struct Context {
     DevBuf<double> s_val[3];
     size_t s_total[3] = {0, 0, 0};
     DevBuf<uint64_t> s_key_sorted;
-    DevBuf<uint32_t> conf_rank;         // confined points before each point (confinement-only streams)
+    DevBuf<uint32_t> conf_rank, conf_flag;   // confined points before each point (confinement-only streams)
     DevBuf<uint32_t> s_pos;
     DevBuf<uint64_t> c_key, c_key_sorted;
     DevBuf<uint32_t> c_u, c_u_sorted;
