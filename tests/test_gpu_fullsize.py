@@ -1,0 +1,73 @@
+"""Parity at the benchmark workloads' full sizes: the 1000-fiber crosslinked
+scene (assembly bit-exact, one implicit step), the aster scene (10 Solids of
+201 points, 2000 fibers, 50000 motors; one step, then a motor step) and three
+systems of the 64-system batch, each against the oracle on the same inputs
+at a tight tolerance."""
+import numpy as np
+import pytest
+
+from conftest import oracle_from_scene, rel_err
+from test_gpu_parity import _assert_assembly_equal, gpu_from_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def test_crosslinked_1000_fibers_full_size(pkg, scenes, oracle):
+    sc = scenes.config2(seed=2)
+    print(f"full size: {sc.name} unknowns={sc.size}", flush=True)
+    gpu = gpu_from_scene(pkg, sc)
+    orc = oracle_from_scene(oracle, sc)
+    g = sc.gauss(0)
+    gpu.setup(g)
+    orc.setup(g)
+    _assert_assembly_equal(pkg, oracle, gpu, orc)
+    r1 = gpu.iterate(1e-10, tol_mode=pkg.TOL_RELATIVE)
+    r2 = orc.iterate(1e-10, tol_mode=1)
+    assert r1["converged"] and r2["converged"]
+    gpu.apply()
+    orc.apply()
+    assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-9
+    gpu.close()
+
+
+def test_asters_full_size(pkg, scenes, oracle):
+    sc = scenes.config3(seed=3)
+    print(f"full size: {sc.name} unknowns={sc.size}", flush=True)
+    gpu = gpu_from_scene(pkg, sc)
+    orc = oracle_from_scene(oracle, sc)
+    g = sc.gauss(0)
+    r1 = gpu.solve(1e-10, g, tol_mode=pkg.TOL_RELATIVE)
+    r2 = orc.solve(1e-10, g, tol_mode=1)
+    assert r1["converged"] and r2["converged"]
+    assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-9
+    gpu.step_couples(sc.motor_max_speed, sc.motor_stall_force)
+    orc.step_couples(sc.link_fiber_a, sc.link_abs_a, sc.link_fiber_b, sc.link_abs_b, sc.link_stiffness,
+                     sc.link_state, sc.motor_max_speed, sc.motor_stall_force)
+    aa, ab, st = gpu.get_couples()
+    assert np.array_equal(st, sc.link_state)
+    assert np.abs(aa - sc.link_abs_a).max() < 1e-9 and np.abs(ab - sc.link_abs_b).max() < 1e-9
+    gpu.close()
+
+
+def test_batch_of_64_systems_full_size(pkg, scenes, oracle):
+    print("full size: generating 64 systems", flush=True)
+    batch = scenes.config5(seed=5)
+    big = batch.scene
+    print(f"full size: 64 systems, unknowns={big.size}", flush=True)
+    gpu = pkg.Meca(3, 0)
+    pkg.load_batch(gpu, batch)
+    rep = gpu.solve(1e-12, big.gauss(0), tol_mode=pkg.TOL_RELATIVE, max_iter=6000)
+    print(f"batch solved: converged={rep['converged']} iterations={rep['iterations']}", flush=True)
+    assert rep["converged"]
+    reps = gpu.system_reports()
+    pts = gpu.fiber_points()
+    offs = np.concatenate([[0], np.cumsum([sc.size for sc in batch.systems])])
+    g = big.gauss(0)
+    for k in (0, 31, 63):
+        sc = batch.systems[k]
+        print(f"system {k}: fibers={sc.nfib}", flush=True)
+        orc = oracle_from_scene(oracle, sc)
+        orep = orc.solve(1e-12, g[offs[k]:offs[k + 1]], tol_mode=1)
+        assert reps[k]["converged"] and orep["converged"]
+        assert rel_err(pts[offs[k]:offs[k + 1]], orc.fiber_points()) < 1e-9
+    gpu.close()
