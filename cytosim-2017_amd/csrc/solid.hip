@@ -199,8 +199,24 @@ __global__ __launch_bounds__(kSolidThreads) void k_solid_pc_build(FiberArgs a, S
         double h[6] = { 0, 0, 0, 0, 0, 0 };
         double g[6];
         if (a.Buse) {
+            // the columns of a long row (an aster centre) are loaded 8 at a time
             const uint32_t e1 = a.Brow[r + 1];
-            for (uint32_t q = a.Brow[r]; q < e1; ++q) {
+            uint32_t q = a.Brow[r];
+            for (; q + 8 <= e1; q += 8) {
+                uint32_t cq[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) cq[j] = a.Bcol[q + j];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t cp = cq[j];
+                    if (cp < idx || cp >= idx + np) continue;
+                    rigid_row<D>(s.pos + D * cp, d, g);
+                    const double v = a.Bval[q + j];
+#pragma unroll
+                    for (int c = 0; c < NG; ++c) h[c] += v * g[c];
+                }
+            }
+            for (; q < e1; ++q) {
                 const uint32_t cp = a.Bcol[q];
                 if (cp < idx || cp >= idx + np) continue;
                 rigid_row<D>(s.pos + D * cp, d, g);
@@ -302,28 +318,40 @@ __device__ void sym4_top_eigenvector(double A[16], double v[4])
 {
     double V[16];
     for (int i = 0; i < 16; ++i) V[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    // cyclic sweeps until the off-diagonal part is below the rounding of the
+    // diagonal (Jacobi converges quadratically: a few sweeps); the loops over
+    // the pairs are unrolled so that A and V stay in registers
     for (int sweep = 0; sweep < 64; ++sweep) {
-        double off = 0.0;
+        double off = 0.0, dia = 0.0;
+#pragma unroll
         for (int q = 1; q < 4; ++q)
+#pragma unroll
             for (int p = 0; p < q; ++p) off += A[p + 4 * q] * A[p + 4 * q];
-        if (off == 0.0) break;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) dia += A[p + 4 * p] * A[p + 4 * p];
+        if (!(off > 1e-34 * dia)) break;
+#pragma unroll
         for (int p = 0; p < 3; ++p)
+#pragma unroll
             for (int q = p + 1; q < 4; ++q) {
                 const double apq = A[p + 4 * q];
                 if (apq == 0.0) continue;
                 const double th = (A[q + 4 * q] - A[p + 4 * p]) / (2 * apq);
                 const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
                 const double c = 1 / sqrt(t * t + 1), sn = t * c;
+#pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const double akp = A[k + 4 * p], akq = A[k + 4 * q];
                     A[k + 4 * p] = c * akp - sn * akq;
                     A[k + 4 * q] = sn * akp + c * akq;
                 }
+#pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const double apk = A[p + 4 * k], aqk = A[q + 4 * k];
                     A[p + 4 * k] = c * apk - sn * aqk;
                     A[q + 4 * k] = sn * apk + c * aqk;
                 }
+#pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const double vkp = V[k + 4 * p], vkq = V[k + 4 * q];
                     V[k + 4 * p] = c * vkp - sn * vkq;

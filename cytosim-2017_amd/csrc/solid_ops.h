@@ -164,8 +164,25 @@ __device__ __forceinline__ void coupling_row(const FiberArgs& a, uint32_t r, con
 #pragma unroll
     for (int d = 0; d < D; ++d) fp[d] = 0.0;
     if (a.Buse) {
+        // a point with many links (an aster centre) has a long row: its entries
+        // and operands are loaded 8 at a time, then added in order
         const uint32_t e1 = a.Brow[r + 1];
-        for (uint32_t q = a.Brow[r]; q < e1; ++q) {
+        uint32_t q = a.Brow[r];
+        for (; q + 8 <= e1; q += 8) {
+            double v[8], x[8][D];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                v[j] = a.Bval[q + j];
+                const uint32_t c = a.Bcol[q + j];
+#pragma unroll
+                for (int d = 0; d < D; ++d) x[j][d] = X(D * c + d);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+#pragma unroll
+                for (int d = 0; d < D; ++d) fp[d] += v[j] * x[j][d];
+        }
+        for (; q < e1; ++q) {
             const double v = a.Bval[q];
             const uint32_t c = a.Bcol[q];
 #pragma unroll
