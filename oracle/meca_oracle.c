@@ -1835,6 +1835,62 @@ void orc_get_matrix_dense(const orc_meca* m, int which, double* out)
         }
 }
 
+/* the storage on its own: thin wrappers of the sp_* routines above */
+struct orc_sparse { spsym m; };
+
+orc_sparse* orc_sparse_create(uint32_t n)
+{
+    orc_sparse* s = (orc_sparse*)calloc(1, sizeof(orc_sparse));
+    sp_allocate(&s->m, n);
+    return s;
+}
+
+void orc_sparse_destroy(orc_sparse* s)
+{
+    if (!s) return;
+    sp_free(&s->m);
+    free(s);
+}
+
+void orc_sparse_zero(orc_sparse* s) { sp_zero(&s->m); }
+
+void orc_sparse_add(orc_sparse* s, uint32_t i, uint32_t j, double v) { *sp_ref(&s->m, i, j) += v; }
+
+uint32_t orc_sparse_nnz(const orc_sparse* s)
+{
+    uint32_t n = 0;
+    for (uint32_t j = 0; j < s->m.size; ++j) n += s->m.colSize[j];
+    return n;
+}
+
+int orc_sparse_nonzero(const orc_sparse* s) { return sp_nonzero(&s->m); }
+
+void orc_sparse_prepare(orc_sparse* s) { sp_prepare(&s->m); }
+
+void orc_sparse_mul_add(const orc_sparse* s, int dim, const double* X, double* Y)
+{
+    if (dim > 1)
+        sp_mul_add_iso(&s->m, dim, X, Y);
+    else
+        sp_mul_add(&s->m, X, Y);
+}
+
+void orc_sparse_block(const orc_sparse* s, int full, uint32_t x, uint32_t sx, double* M)
+{
+    if (full)
+        sp_add_diagonal_block(&s->m, M, x, sx);
+    else
+        sp_add_triangular_block(&s->m, M, x, sx);
+}
+
+uint32_t orc_sparse_nr(const orc_sparse* s, uint32_t* ija, double* sa)
+{
+    const uint32_t nbe = s->m.ija ? s->m.ija[s->m.size] : 0;
+    if (ija && nbe) memcpy(ija, s->m.ija, sizeof(uint32_t) * nbe);
+    if (sa && nbe) memcpy(sa, s->m.sa, sizeof(double) * nbe);
+    return nbe;
+}
+
 /* ------------------------------------------------------------------------ */
 /* motor couples between solves                                             */
 

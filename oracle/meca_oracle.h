@@ -120,6 +120,23 @@ uint32_t  orc_get_matrix_nr(orc_meca*, int which, uint32_t* ija, double* sa);
 /* dense (size x size) symmetric matrix of mB or mC, column-major */
 void      orc_get_matrix_dense(const orc_meca*, int which, double* out);
 
+/* the storage of mB/mC on its own (MatrixSparseSymmetric1, matsparsesym1.cc),
+   for checks against that class: add = operator()(i,j) += v (137-224),
+   nnz = nbNonZeroElements (355-362), prepare = prepareForMultiply (472-523),
+   mul_add = vecMulAdd (dim 1) / vecMulAddIso2D / vecMulAddIso3D (526-631),
+   block = addDiagonalBlock (full) / addTriangularBlock (262-310) */
+typedef struct orc_sparse orc_sparse;
+orc_sparse* orc_sparse_create(uint32_t n);
+void      orc_sparse_destroy(orc_sparse*);
+void      orc_sparse_zero(orc_sparse*);
+void      orc_sparse_add(orc_sparse*, uint32_t i, uint32_t j, double v);
+uint32_t  orc_sparse_nnz(const orc_sparse*);
+int       orc_sparse_nonzero(const orc_sparse*);
+void      orc_sparse_prepare(orc_sparse*);
+void      orc_sparse_mul_add(const orc_sparse*, int dim, const double* X, double* Y);
+void      orc_sparse_block(const orc_sparse*, int full, uint32_t x, uint32_t sx, double* M);
+uint32_t  orc_sparse_nr(const orc_sparse*, uint32_t* ija, double* sa);
+
 /* motor couples between solves: Couple::stepAA / stepAF / stepFA
    (couple.cc:125-150) with Motor::stepLoaded / stepUnloaded (motor.cc:19-57)
    and the end handling of Hand::handleOutOfRange (hand.cc:102-122) for
