@@ -31,7 +31,7 @@ EXPORTED = [
     "cymg_inter_link_ee", "cymg_inter_link_ie", "cymg_inter_link_ii", "cymg_inter_long_link_ee",
     "cymg_inter_clamp_e", "cymg_inter_clamp_i", "cymg_inter_long_clamp", "cymg_inter_plane_e",
     "cymg_confine_sphere", "cymg_set_couples", "cymg_couples_interactions", "cymg_set_couple_state",
-    "cymg_get_couples", "cymg_step_couples", "cymg_set_aster_links",
+    "cymg_get_couples", "cymg_step_couples", "cymg_nearby_segments", "cymg_set_aster_links",
     "cymg_aster_interactions", "cymg_solve",
     "cymg_setup", "cymg_iterate", "cymg_apply", "cymg_multiply", "cymg_compute_preconditioner",
     "cymg_precondition", "cymg_compute_forces", "cymg_multiply_device", "cymg_precondition_device",
@@ -142,6 +142,8 @@ def lib():
         "cymg_set_couple_state": (C.c_int, [vp, vp]),
         "cymg_get_couples": (C.c_int, [vp, vp, vp, vp]),
         "cymg_step_couples": (C.c_int, [vp, C.POINTER(MotorProps)]),
+        "cymg_nearby_segments": (C.c_int, [vp, u32, vp, vp, C.c_double, C.c_int, vp, u32, vp, vp, vp, vp,
+                                           C.POINTER(u32)]),
         "cymg_set_aster_links": (C.c_int, [vp, u32, vp, vp, vp, vp, vp, vp]),
         "cymg_aster_interactions": (C.c_int, [vp]),
         "cymg_solve": (C.c_int, [vp, C.POINTER(SolveParams), vp, C.POINTER(SolveReport)]),
@@ -356,6 +358,29 @@ class Meca:
         """Couple::stepAA/AF/FA with the motor hands, from the current points."""
         mp = MotorProps(float(max_speed), float(stall_force), int(bool(limit_speed)), int(bool(hold_growing_end)))
         self._chk(self.L.cymg_step_couples(self.h, C.byref(mp)))
+
+    def nearby_segments(self, places, range_, inclusive=False, exclude=None):
+        """FiberGrid::nearbySegments for many places (fiber_grid.cc:302-337):
+        counts per place and the concatenated (fiber, point, abscissa,
+        dist_sqr) of the kept segments, place after place, each in fiber
+        order."""
+        w = np.ascontiguousarray(places, dtype=np.float64).reshape(-1, self.dim)
+        nq = w.shape[0]
+        ex = None if exclude is None else np.ascontiguousarray(exclude, dtype=np.uint32)
+        counts = np.zeros(max(nq, 1), np.uint32)
+        total = C.c_uint32(0)
+        cap = 0
+        out = None
+        for _ in range(2):
+            f, p = np.zeros(max(cap, 1), np.uint32), np.zeros(max(cap, 1), np.uint32)
+            a, d = np.zeros(max(cap, 1)), np.zeros(max(cap, 1))
+            self._chk(self.L.cymg_nearby_segments(self.h, nq, _p(w), _p(ex), float(range_), int(bool(inclusive)),
+                                                  _p(counts), cap, _p(f), _p(p), _p(a), _p(d), C.byref(total)))
+            if total.value <= cap:
+                out = (f[:total.value], p[:total.value], a[:total.value], d[:total.value])
+                break
+            cap = total.value
+        return counts[:nq], out
 
     def set_aster_links(self, kind, s1, s2, fiber, abscissa, stiffness):
         kd = np.ascontiguousarray(kind, dtype=np.int32)

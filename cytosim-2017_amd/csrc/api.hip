@@ -823,6 +823,20 @@ int cymg_set_couple_state(cymg_context* ctx, const uint8_t* state)
     });
 }
 
+int cymg_nearby_segments(cymg_context* ctx, uint32_t nq, const double* places, const uint32_t* exclude,
+                         double range, int inclusive, uint32_t* counts, uint32_t capacity, uint32_t* fiber,
+                         uint32_t* point, double* abscissa, double* dist_sqr, uint32_t* total)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.prepared) throw Error(CYMG_ERR_INVALID, "no points: cymg_prepare first");
+        if (!(range >= 0)) throw Error(CYMG_ERR_INVALID, "range must be >= 0");
+        if (!total || (nq && (!places || !counts))) throw Error(CYMG_ERR_INVALID, "null argument");
+        if (capacity && (!fiber || !point || !abscissa || !dist_sqr)) throw Error(CYMG_ERR_INVALID, "null output");
+        nearby_segments(c, nq, places, exclude, range, inclusive, counts, capacity, fiber, point, abscissa,
+                        dist_sqr, total);
+    });
+}
+
 int cymg_get_couples(cymg_context* ctx, double* aa, double* ab, uint8_t* state)
 {
     return guarded(ctx, [&](Context& c) {

@@ -7,6 +7,15 @@
 
 namespace cymg {
 
+// buffers of the FiberGrid query (grid.hip)
+struct GridBufs {
+    DevBuf<double> places, oabs, odis;
+    DevBuf<uint32_t> excl, count, offset, key, key2, val, val2, start, end, ofib, opt;
+    DevBuf<unsigned long long> half;
+    DevBuf<int> flag;
+    DevBuf<unsigned char> temp;
+};
+
 struct Context {
     int dim = 3;
     int device = 0;
@@ -115,6 +124,9 @@ struct Context {
 
     // learned initial guess
     MlpState mlp;
+
+    // segments near places (the FiberGrid query)
+    GridBufs grid;
 
     // timing
     bool timing = false;

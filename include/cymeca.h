@@ -174,6 +174,24 @@ typedef struct {
 int         cymg_set_couple_state(cymg_context* ctx, const uint8_t* state);
 int         cymg_get_couples(cymg_context* ctx, double* aa, double* ab, uint8_t* state);
 int         cymg_step_couples(cymg_context* ctx, const cymg_motor_props* props);
+
+/* Segments near places: FiberGrid::nearbySegments (fiber_grid.cc:302-337) for
+   nq places at once, with FiberLocus::projectPoint (fiber_locus.cc:62-97) on
+   the current points; inclusive != 0 applies tryToAttach's range test instead
+   (fiber_grid.cc:273-274: dis <= range^2 is kept, nearbySegments keeps
+   dis < range^2).  places: host [nq][dim]; exclude: host [nq] fiber to skip per
+   place (nearbySegments' `exclude`), or NULL.  Writes counts[nq] and *total;
+   when *total <= capacity, also the kept segments place after place, each
+   place's in fiber order then point order (the reference's list order):
+   fiber, the segment's first point in the fiber, the abscissa from the minus
+   end (segmentation * point + projection, the FiberBinder site of
+   fiber_grid.cc:278) and the squared distance.  Fibers only (Solids have no
+   segments); no periodic space.  At most 64 segments per place
+   (CYMG_ERR_UNSUPPORTED beyond). */
+int         cymg_nearby_segments(cymg_context* ctx, uint32_t nq, const double* places, const uint32_t* exclude,
+                                 double range, int inclusive, uint32_t* counts, uint32_t capacity,
+                                 uint32_t* fiber, uint32_t* point, double* abscissa, double* dist_sqr,
+                                 uint32_t* total);
 /* Aster::setInteractions (aster.cc:53-89) for a list of aster links, uploaded
    by set_aster_links and emitted on the device by aster_interactions, each as
    the host interLink call it stands for:

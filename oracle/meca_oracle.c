@@ -1892,6 +1892,72 @@ uint32_t orc_sparse_nr(const orc_sparse* s, uint32_t* ija, double* sa)
 }
 
 /* ------------------------------------------------------------------------ */
+/* segments near a place (the FiberGrid query)                               */
+
+/* FiberGrid::nearbySegments (fiber_grid.cc:302-337) by brute force: every
+   segment of every fiber (Solids have none), in list order, is tested with
+   FiberLocus::projectPoint (fiber_locus.cc:62-97): abs = (w - P1).(P2 - P1) /
+   segmentation; inside the segment dis = |w - P1|^2 - abs^2; before the first
+   segment of a fiber or after its last one, the squared distance to that end;
+   otherwise dis stays infinite (the neighbouring segment covers the place).
+   A segment is kept when dis < range^2 (inclusive: dis <= range^2, the test
+   of tryToAttach, fiber_grid.cc:273-274), skipping fiber `exclude` (-1:
+   none).  The kept segments are written up to `cap`: the fiber, the local
+   point of the segment's first end, the abscissa from the minus end
+   (segmentation * point + abs, FiberBinder site of fiber_grid.cc:278) and
+   dis.  Returns the number kept. */
+uint32_t orc_nearby_segments(const orc_meca* m, const double* w, double range, int inclusive, int exclude,
+                             uint32_t cap, uint32_t* fiber, uint32_t* point, double* abscissa, double* dis)
+{
+    const int D = m->dim;
+    const double rr = range * range;
+    uint32_t n = 0;
+    for (int f = 0; f < m->nfib; ++f) {
+        const orc_fiber* F = &m->fib[f];
+        if (F->kind != 0 || f == exclude) continue;
+        const double ls = F->cut;
+        for (uint32_t p = 0; p + 1 < F->np; ++p) {
+            const double* P1 = m->pos + (size_t)D * (F->idx + p);
+            const double* P2 = P1 + D;
+            double dx[3] = { 0, 0, 0 }, aw[3] = { 0, 0, 0 };
+            for (int d = 0; d < D; ++d) {
+                dx[d] = P2[d] - P1[d];
+                aw[d] = w[d] - P1[d];
+            }
+            double dot = aw[0] * dx[0];
+            for (int d = 1; d < D; ++d) dot = dot + aw[d] * dx[d];
+            const double ab = dot / ls;
+            double ds = INFINITY;
+            if (ab < 0) {
+                if (p == 0) {
+                    ds = (P1[0] - w[0]) * (P1[0] - w[0]);
+                    for (int d = 1; d < D; ++d) ds = ds + (P1[d] - w[d]) * (P1[d] - w[d]);
+                }
+            } else if (ab > ls) {
+                if (p + 2 == F->np) {
+                    ds = (P2[0] - w[0]) * (P2[0] - w[0]);
+                    for (int d = 1; d < D; ++d) ds = ds + (P2[d] - w[d]) * (P2[d] - w[d]);
+                }
+            } else {
+                double nn = aw[0] * aw[0];
+                for (int d = 1; d < D; ++d) nn = nn + aw[d] * aw[d];
+                ds = nn - ab * ab;
+            }
+            const int keep = inclusive ? !(ds > rr) : (ds < rr);
+            if (!keep) continue;
+            if (n < cap) {
+                fiber[n] = (uint32_t)f;
+                point[n] = p;
+                abscissa[n] = ls * (double)p + ab;
+                dis[n] = ds;
+            }
+            ++n;
+        }
+    }
+    return n;
+}
+
+/* ------------------------------------------------------------------------ */
 /* motor couples between solves                                             */
 
 /* position and dirFiber of a hand at abscissa ab: PointInterpolated::pos

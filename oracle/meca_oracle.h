@@ -120,6 +120,11 @@ uint32_t  orc_get_matrix_nr(orc_meca*, int which, uint32_t* ija, double* sa);
 /* dense (size x size) symmetric matrix of mB or mC, column-major */
 void      orc_get_matrix_dense(const orc_meca*, int which, double* out);
 
+/* FiberGrid::nearbySegments / tryToAttach's distance test by brute force
+   (fiber_grid.cc:248-337, fiber_locus.cc:62-97); returns the number kept */
+uint32_t  orc_nearby_segments(const orc_meca*, const double* place, double range, int inclusive, int exclude,
+                              uint32_t cap, uint32_t* fiber, uint32_t* point, double* abscissa, double* dis);
+
 /* the storage of mB/mC on its own (MatrixSparseSymmetric1, matsparsesym1.cc),
    for checks against that class: add = operator()(i,j) += v (137-224),
    nnz = nbNonZeroElements (355-362), prepare = prepareForMultiply (472-523),

@@ -86,6 +86,8 @@ def lib():
         "orc_inter_plane_e": (None, [vp, C.c_uint32, _f64p, _f64p, C.c_double]),
         "orc_confine_sphere": (None, [vp, C.c_double, C.c_double]),
         "orc_step_couples": (None, [vp, C.c_uint32, vp, _f64p, vp, _f64p, _f64p, vp, C.POINTER(Motor)]),
+        "orc_nearby_segments": (C.c_uint32, [vp, _f64p, C.c_double, C.c_int, C.c_int, C.c_uint32,
+                                             vp, vp, vp, vp]),
         "orc_setup": (C.c_int, [vp, vp]),
         "orc_iterate": (C.c_int, [vp, C.POINTER(SolveParams), vp, C.POINTER(Report)]),
         "orc_apply": (C.c_int, [vp]),
@@ -259,6 +261,21 @@ class OracleMeca:
         mp = Motor(float(max_speed), float(stall_force), int(bool(limit_speed)), int(bool(hold_growing_end)))
         self.L.orc_step_couples(self.h, fa.size, fa.ctypes.data, abs_a, fb.ctypes.data, abs_b, w,
                                 state.ctypes.data, C.byref(mp))
+
+    def nearby_segments(self, place, range_, inclusive=False, exclude=-1):
+        """FiberGrid::nearbySegments by brute force (fiber_grid.cc:302-337):
+        (fiber, point, abscissa, dist_sqr) of the segments near `place`, in
+        fiber order."""
+        w = np.ascontiguousarray(place, dtype=np.float64)
+        n = self.L.orc_nearby_segments(self.h, w, float(range_), int(bool(inclusive)), int(exclude), 0,
+                                       None, None, None, None)
+        f = np.zeros(n, np.uint32)
+        p = np.zeros(n, np.uint32)
+        a = np.zeros(n, np.float64)
+        d = np.zeros(n, np.float64)
+        self.L.orc_nearby_segments(self.h, w, float(range_), int(bool(inclusive)), int(exclude), n,
+                                   f.ctypes.data, p.ctypes.data, a.ctypes.data, d.ctypes.data)
+        return f, p, a, d
 
     def setup(self, gauss=None):
         g = None if gauss is None else np.ascontiguousarray(gauss, dtype=np.float64)
