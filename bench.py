@@ -86,6 +86,8 @@ def parse():
                     help="also run a plain device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--batched-steps", type=int, default=3,
                     help="steps of the 64-system batched workload (0: skip)")
+    ap.add_argument("--grid-places", type=int, default=50000,
+                    help="places of the timed FiberGrid query (0: skip)")
     ap.add_argument("--aster-steps", type=int, default=5,
                     help="timed steps of the aster workload (10 Solids + 2000 fibers, 50000 motors; 0: skip)")
     return ap.parse_args()
@@ -309,6 +311,29 @@ def main():
         print(f"bench: asters {asters}", file=sys.stderr, flush=True)
         ma.close()
 
+    grid = None
+    if args.grid_places > 0:
+        # FiberGrid query for many places at once (cymg_nearby_segments) on the
+        # 1000-fiber system: places near the fibers, tryToAttach's range test;
+        # host arrays in and out are included in the time
+        mg = pkg.Meca(3, 0)
+        pkg.load_scene(mg, sc)
+        rng = np.random.default_rng(11)
+        P = sc.points.reshape(-1, 3)
+        W = P[rng.integers(0, P.shape[0], args.grid_places)] + rng.uniform(-0.1, 0.1, (args.grid_places, 3))
+        mg.nearby_segments(W, 0.05, inclusive=True)          # untimed: buffers
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            counts, out = mg.nearby_segments(W, 0.05, inclusive=True)
+        gdt = (time.perf_counter() - t0) / reps
+        grid = {"workload": f"{args.grid_places} places near the fibers of the 1000-fiber system, range 0.05",
+                "ms_per_query": round(1e3 * gdt, 3), "places_per_s": round(args.grid_places / gdt, 1),
+                "segments_found": int(counts.sum()),
+                "note": "one cymg_nearby_segments call, host arrays in and out included"}
+        print(f"bench: grid {grid}", file=sys.stderr, flush=True)
+        mg.close()
+
     cpu = None
     if args.cpu_steps > 0:
         from oracle import pyoracle
@@ -372,6 +397,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "asters": asters,
+        "nearby_segments": grid,
     }
     print(json.dumps(line), flush=True)
 
