@@ -369,7 +369,7 @@ class Meca:
         ex = None if exclude is None else np.ascontiguousarray(exclude, dtype=np.uint32)
         counts = np.zeros(max(nq, 1), np.uint32)
         total = C.c_uint32(0)
-        cap = 0
+        cap = 8 * nq + 64          # usually enough: one call; otherwise a second with the exact size
         out = None
         for _ in range(2):
             f, p = np.zeros(max(cap, 1), np.uint32), np.zeros(max(cap, 1), np.uint32)

@@ -178,10 +178,14 @@ __device__ __forceinline__ bool project_keep(const GridQuery& q, const double* w
 
 // one thread per place; FILL = false counts, true writes the kept segments
 // sorted by first point, i.e. by segment (insertion into a register list of
-// kGridMaxKeep)
+// kGridMaxKeep).  The bucket ranges of all neighbour cells are loaded first,
+// then the candidates four at a time, so that the loads of a place are in
+// flight together rather than one chain per cell and candidate.
 template <int D, bool FILL>
 __global__ __launch_bounds__(kGridThreads) void k_query(uint32_t nq, GridQuery q)
 {
+    constexpr int NC = D == 3 ? 27 : 9;
+    constexpr int NB = 4;
     const uint32_t i = blockIdx.x * kGridThreads + threadIdx.x;
     if (i >= nq) return;
     double w[3] = { 0, 0, 0 };
@@ -190,39 +194,57 @@ __global__ __launch_bounds__(kGridThreads) void k_query(uint32_t nq, GridQuery q
     const uint32_t skip = q.exclude ? q.exclude[i] : 0xffffffffu;
     long long c[3];
     cell_of<D>(w, q.inv_h, c);
+    uint32_t bs[NC], be[NC];
+#pragma unroll
+    for (int t = 0; t < NC; ++t) {
+        const uint32_t h = cell_hash(c[0] + (t % 3 - 1), c[1] + ((t / 3) % 3 - 1), c[2] + (D == 3 ? t / 9 - 1 : 0));
+        be[t] = q.end[h];
+        bs[t] = q.start[h];
+    }
     uint32_t n = 0;
     uint32_t keep[kGridMaxKeep];
-    const int zr = D == 3 ? 1 : 0;
-    for (int oz = -zr; oz <= zr; ++oz)
-        for (int oy = -1; oy <= 1; ++oy)
-            for (int ox = -1; ox <= 1; ++ox) {
-                const long long cx = c[0] + ox, cy = c[1] + oy, cz = c[2] + oz;
-                const uint32_t h = cell_hash(cx, cy, cz);
-                const uint32_t e = q.end[h];
-                for (uint32_t k = q.start[h]; k < e; ++k) {
-                    const uint32_t P = q.sorted[k];
-                    const uint32_t f = q.pfib[P];
-                    if (f == skip) continue;
-                    double m[3] = { 0, 0, 0 };
+    for (int t = 0; t < NC; ++t) {
+        const long long cx = c[0] + (t % 3 - 1), cy = c[1] + ((t / 3) % 3 - 1), cz = c[2] + (D == 3 ? t / 9 - 1 : 0);
+        const uint32_t e = be[t];
+        for (uint32_t k = e > 0 ? bs[t] : 0; k < e; k += NB) {
+            uint32_t Pk[NB];
 #pragma unroll
-                    for (int d = 0; d < D; ++d) m[d] = 0.5 * (q.pos[D * P + d] + q.pos[D * (P + 1) + d]);
-                    long long sc[3];
-                    cell_of<D>(m, q.inv_h, sc);
-                    if (sc[0] != cx || sc[1] != cy || (D == 3 && sc[2] != cz)) continue;
-                    double ab, ds;
-                    if (!project_keep<D>(q, w, f, P - q.idx[f], ab, ds)) continue;
-                    if (FILL) {
-                        if (n >= (uint32_t)kGridMaxKeep) { *q.overflow = 1; continue; }
-                        uint32_t j = n;
-                        while (j > 0 && keep[j - 1] > P) {
-                            keep[j] = keep[j - 1];
-                            --j;
-                        }
-                        keep[j] = P;
-                    }
-                    ++n;
+            for (int j = 0; j < NB; ++j) Pk[j] = k + j < e ? q.sorted[k + j] : 0xffffffffu;
+            uint32_t fk[NB];
+            double mk[NB][3];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                fk[j] = 0xffffffffu;
+                mk[j][0] = mk[j][1] = mk[j][2] = 0.0;
+                if (Pk[j] != 0xffffffffu) {
+                    const uint32_t P = Pk[j];
+                    fk[j] = q.pfib[P];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) mk[j][d] = 0.5 * (q.pos[D * P + d] + q.pos[D * (P + 1) + d]);
                 }
             }
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const uint32_t P = Pk[j], f = fk[j];
+                if (P == 0xffffffffu || f == skip) continue;
+                long long sc[3];
+                cell_of<D>(mk[j], q.inv_h, sc);
+                if (sc[0] != cx || sc[1] != cy || (D == 3 && sc[2] != cz)) continue;
+                double ab, ds;
+                if (!project_keep<D>(q, w, f, P - q.idx[f], ab, ds)) continue;
+                if (FILL) {
+                    if (n >= (uint32_t)kGridMaxKeep) { *q.overflow = 1; continue; }
+                    uint32_t m = n;
+                    while (m > 0 && keep[m - 1] > P) {
+                        keep[m] = keep[m - 1];
+                        --m;
+                    }
+                    keep[m] = P;
+                }
+                ++n;
+            }
+        }
+    }
     if (!FILL) {
         q.count[i] = n;
         return;
