@@ -67,6 +67,17 @@ def test_nearby_segments_full_size(pkg, scenes, oracle):
     gpu.close()
 
 
+def test_nearby_segments_with_solids(pkg, scenes, oracle):
+    """Solids (asters) have no segments: only the fibers are listed."""
+    print("grid: asters", flush=True)
+    sc = scenes.mini_asters(dim=3, seed=12)
+    gpu = gpu_from_scene(pkg, sc)
+    orc = oracle_from_scene(oracle, sc)
+    W = _places(sc, np.random.default_rng(5), 300)
+    assert _compare(gpu, orc, W, 0.3, False) > 20
+    gpu.close()
+
+
 def test_nearby_segments_validation(pkg, scenes):
     sc = scenes.mini(dim=3, seed=3, nfib=4, nlinks=5)
     gpu = gpu_from_scene(pkg, sc)
