@@ -12,7 +12,8 @@
 // sorted by cell hash (one radix sort), each bucket is a run of the sorted
 // list, and a place visits its neighbour cells, keeping a segment only when
 // its own cell is the visited one (which removes hash collisions and
-// duplicates) and projectPoint accepts it.  The kept set is the reference's
+// duplicates) and projectPoint accepts it; a wavefront serves one place, a
+// lane per neighbour cell.  The kept set is the reference's
 // (the grid only prunes); the output is sorted by segment, which is the
 // reference's order (segments are painted fiber by fiber, point by point).
 #include "common.h"
@@ -31,7 +32,7 @@ namespace cymg {
 constexpr int kGridHashBits = 20;
 constexpr uint32_t kGridBuckets = 1u << kGridHashBits;
 constexpr int kGridThreads = 256;
-constexpr int kGridMaxKeep = 64;     // kept segments sorted in registers per place (more: error)
+constexpr int kGridMaxKeep = 64;     // kept segments per place (more: error)
 
 __device__ __forceinline__ uint32_t cell_hash(long long x, long long y, long long z)
 {
@@ -176,90 +177,76 @@ __device__ __forceinline__ bool project_keep(const GridQuery& q, const double* w
     return q.inclusive ? !(ds > q.range2) : (ds < q.range2);
 }
 
-// one thread per place; FILL = false counts, true writes the kept segments
-// sorted by first point, i.e. by segment (insertion into a register list of
-// kGridMaxKeep).  The bucket ranges of all neighbour cells are loaded first,
-// then the candidates four at a time, so that the loads of a place are in
-// flight together rather than one chain per cell and candidate.
+// A wavefront per place: lane t < 27 (9 in 2D) walks neighbour cell t, so the
+// cells are searched side by side; kept segments are appended to a per-wave
+// LDS list, and each of them then finds its rank (the number of kept segments
+// with a smaller first point) to write the output in segment order.
+constexpr int kGridWaves = kGridThreads / 64;
+
 template <int D, bool FILL>
-__global__ __launch_bounds__(kGridThreads) void k_query(uint32_t nq, GridQuery q)
+__global__ __launch_bounds__(kGridThreads) void k_query_wave(uint32_t nq, GridQuery q)
 {
     constexpr int NC = D == 3 ? 27 : 9;
-    constexpr int NB = 4;
-    const uint32_t i = blockIdx.x * kGridThreads + threadIdx.x;
-    if (i >= nq) return;
+    __shared__ uint32_t list[kGridWaves][kGridMaxKeep];
+    __shared__ uint32_t fill[kGridWaves];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t i = blockIdx.x * kGridWaves + wv;
+    if (lane == 0) fill[wv] = 0;
+    __syncthreads();
+    if (i >= nq) return;    // the whole wavefront leaves together
     double w[3] = { 0, 0, 0 };
 #pragma unroll
     for (int d = 0; d < D; ++d) w[d] = q.places[(size_t)D * i + d];
     const uint32_t skip = q.exclude ? q.exclude[i] : 0xffffffffu;
     long long c[3];
     cell_of<D>(w, q.inv_h, c);
-    uint32_t bs[NC], be[NC];
-#pragma unroll
-    for (int t = 0; t < NC; ++t) {
-        const uint32_t h = cell_hash(c[0] + (t % 3 - 1), c[1] + ((t / 3) % 3 - 1), c[2] + (D == 3 ? t / 9 - 1 : 0));
-        be[t] = q.end[h];
-        bs[t] = q.start[h];
-    }
     uint32_t n = 0;
-    uint32_t keep[kGridMaxKeep];
-    for (int t = 0; t < NC; ++t) {
-        const long long cx = c[0] + (t % 3 - 1), cy = c[1] + ((t / 3) % 3 - 1), cz = c[2] + (D == 3 ? t / 9 - 1 : 0);
-        const uint32_t e = be[t];
-        for (uint32_t k = e > 0 ? bs[t] : 0; k < e; k += NB) {
-            uint32_t Pk[NB];
+    if (lane < NC) {
+        const long long cx = c[0] + (lane % 3 - 1), cy = c[1] + ((lane / 3) % 3 - 1),
+                        cz = c[2] + (D == 3 ? lane / 9 - 1 : 0);
+        const uint32_t h = cell_hash(cx, cy, cz);
+        const uint32_t e = q.end[h];
+        for (uint32_t k = e > 0 ? q.start[h] : 0; k < e; ++k) {
+            const uint32_t P = q.sorted[k];
+            const uint32_t f = q.pfib[P];
+            if (f == skip) continue;
+            double m[3] = { 0, 0, 0 };
 #pragma unroll
-            for (int j = 0; j < NB; ++j) Pk[j] = k + j < e ? q.sorted[k + j] : 0xffffffffu;
-            uint32_t fk[NB];
-            double mk[NB][3];
-#pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                fk[j] = 0xffffffffu;
-                mk[j][0] = mk[j][1] = mk[j][2] = 0.0;
-                if (Pk[j] != 0xffffffffu) {
-                    const uint32_t P = Pk[j];
-                    fk[j] = q.pfib[P];
-#pragma unroll
-                    for (int d = 0; d < D; ++d) mk[j][d] = 0.5 * (q.pos[D * P + d] + q.pos[D * (P + 1) + d]);
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                const uint32_t P = Pk[j], f = fk[j];
-                if (P == 0xffffffffu || f == skip) continue;
-                long long sc[3];
-                cell_of<D>(mk[j], q.inv_h, sc);
-                if (sc[0] != cx || sc[1] != cy || (D == 3 && sc[2] != cz)) continue;
-                double ab, ds;
-                if (!project_keep<D>(q, w, f, P - q.idx[f], ab, ds)) continue;
-                if (FILL) {
-                    if (n >= (uint32_t)kGridMaxKeep) { *q.overflow = 1; continue; }
-                    uint32_t m = n;
-                    while (m > 0 && keep[m - 1] > P) {
-                        keep[m] = keep[m - 1];
-                        --m;
-                    }
-                    keep[m] = P;
-                }
-                ++n;
+            for (int d = 0; d < D; ++d) m[d] = 0.5 * (q.pos[D * P + d] + q.pos[D * (P + 1) + d]);
+            long long sc[3];
+            cell_of<D>(m, q.inv_h, sc);
+            if (sc[0] != cx || sc[1] != cy || (D == 3 && sc[2] != cz)) continue;
+            double ab, ds;
+            if (!project_keep<D>(q, w, f, P - q.idx[f], ab, ds)) continue;
+            ++n;
+            if (FILL) {
+                const uint32_t slot = atomicAdd(&fill[wv], 1u);
+                if (slot < (uint32_t)kGridMaxKeep) list[wv][slot] = P;
+                else *q.overflow = 1;
             }
         }
     }
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o, 64);
     if (!FILL) {
-        q.count[i] = n;
+        if (lane == 0) q.count[i] = n;
         return;
     }
-    const uint32_t o = q.offset[i];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const uint32_t m = n < (uint32_t)kGridMaxKeep ? n : (uint32_t)kGridMaxKeep;
-    for (uint32_t j = 0; j < m; ++j) {
-        const uint32_t P = keep[j];
+    if ((uint32_t)lane < m) {
+        const uint32_t P = list[wv][lane];
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < m; ++j) rank += list[wv][j] < P ? 1u : 0u;
         const uint32_t f = q.pfib[P], p = P - q.idx[f];
         double ab, ds;
         project_keep<D>(q, w, f, p, ab, ds);
-        q.out_fib[o + j] = f;
-        q.out_pt[o + j] = p;
-        q.out_abs[o + j] = ab;
-        q.out_dis[o + j] = ds;
+        const uint32_t o = q.offset[i] + rank;
+        q.out_fib[o] = f;
+        q.out_pt[o] = p;
+        q.out_abs[o] = ab;
+        q.out_dis[o] = ds;
     }
 }
 
@@ -349,9 +336,9 @@ void nearby_segments(Context& c, uint32_t nq, const double* places, const uint32
     q.count = G.count.get();
     q.offset = G.offset.get();
     q.overflow = G.flag.get();
-    const dim3 gq((nq + kGridThreads - 1) / kGridThreads);
-    if (D == 3) hipLaunchKernelGGL((k_query<3, false>), gq, b, 0, st, nq, q);
-    else hipLaunchKernelGGL((k_query<2, false>), gq, b, 0, st, nq, q);
+    const dim3 gq((nq + kGridWaves - 1) / kGridWaves);
+    if (D == 3) hipLaunchKernelGGL((k_query_wave<3, false>), gq, b, 0, st, nq, q);
+    else hipLaunchKernelGGL((k_query_wave<2, false>), gq, b, 0, st, nq, q);
     CYMG_HIP(hipGetLastError());
     CYMG_HIP(hipMemsetAsync(G.count.get() + nq, 0, 4, st));
     size_t tb = 0;
@@ -368,8 +355,8 @@ void nearby_segments(Context& c, uint32_t nq, const double* places, const uint32
     q.out_pt = G.opt.get();
     q.out_abs = G.oabs.get();
     q.out_dis = G.odis.get();
-    if (D == 3) hipLaunchKernelGGL((k_query<3, true>), gq, b, 0, st, nq, q);
-    else hipLaunchKernelGGL((k_query<2, true>), gq, b, 0, st, nq, q);
+    if (D == 3) hipLaunchKernelGGL((k_query_wave<3, true>), gq, b, 0, st, nq, q);
+    else hipLaunchKernelGGL((k_query_wave<2, true>), gq, b, 0, st, nq, q);
     CYMG_HIP(hipGetLastError());
     int overflow = 0;
     CYMG_HIP(hipMemcpyAsync(&overflow, G.flag.get(), sizeof(int), hipMemcpyDeviceToHost, st));
