@@ -457,3 +457,147 @@ def test_reshape_sure_fallback_keeps_centroid(oracle):
     X = out.reshape(np_, dim)
     assert np.abs(np.linalg.norm(np.diff(X, axis=0), axis=1) - cut).max() < 1e-12
     assert np.abs(X.mean(0) - pts.reshape(np_, dim).mean(0)).max() < 1e-12
+
+
+# ----------------------------------------- multipliers, P', reshape form ---
+# These pin the three oracle functions that the operator construction above
+# only checks for consistency with the oracle's own pieces: the Lagrange
+# multipliers (rfLag, rigid_fiber_project.cc:137-161), the P' coefficients
+# built from them (makeProjectionDiff, rigid_fiber_project.cc:168-219) and the
+# Newton-on-multipliers form of FiberNaked::reshape_it (fiber_naked.cc:221-354).
+
+def _straight_fiber(dim, np_, cut, seed=0):
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal(dim)
+    u /= np.linalg.norm(u)
+    x0 = rng.standard_normal(dim)
+    return (x0[None, :] + cut * np.arange(np_)[:, None] * u[None, :]).ravel(), u
+
+
+def _forced_fiber(oracle, dim, pts, cut, forces):
+    """One fiber, no Brownian term (kT = 0), external forces {point: vector}
+    placed in vBAS; rigidity 0 so that vFOR holds exactly these forces."""
+    m = oracle.OracleMeca(dim, dict(time_step=0.005, kT=0.0, viscosity=0.02))
+    m.add_fiber(len(pts) // dim, cut, 0.0)
+    m.prepare(pts)
+    for p, f in forces.items():
+        m.add_pure_force(p, np.asarray(f, dtype=np.float64))
+    m.setup(None)
+    return m
+
+
+def _chain_laplacian(np_, coef, dim):
+    """Matrix of Y_s += c_s (X_{s+1} - X_s), Y_{s+1} -= c_s (X_{s+1} - X_s)."""
+    L = np.zeros((np_, np_))
+    for s, c in enumerate(coef):
+        L[s, s] -= c
+        L[s, s + 1] += c
+        L[s + 1, s] += c
+        L[s + 1, s + 1] -= c
+    return np.kron(L, np.eye(dim))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("np_", [2, 3, 9, 31])
+def test_multipliers_and_projection_diff_on_a_pulled_straight_fiber(oracle, dim, np_):
+    """A straight fiber pulled apart by +-F at its two ends carries the tension F
+    in every segment, so every multiplier is F and every P' coefficient F/cut
+    (P' = -(F/cut) * chain Laplacian). Pushed together, the multipliers are -F
+    and the compressive multipliers are dropped: P' = 0."""
+    cut, F = 0.5, 3.25
+    pts, u = _straight_fiber(dim, np_, cut, seed=np_ + dim)
+    n = dim * np_
+    for sign in (+1.0, -1.0):
+        m = _forced_fiber(oracle, dim, pts, cut, {0: -sign * F * u, np_ - 1: sign * F * u})
+        lag = m.tensions()
+        assert np.abs(lag - sign * F).max() < 1e-12 * F
+        D = np.stack([m.add_projection_diff(0, e) for e in np.eye(n)], axis=1)
+        c = F / cut if sign > 0 else 0.0
+        assert np.abs(D - _chain_laplacian(np_, [c] * (np_ - 1), dim)).max() < 1e-12 * F / cut
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_projection_diff_keeps_tension_and_drops_compression(oracle, dim):
+    """Held at its middle point m, with the minus end pushed inward and the
+    plus end pulled outward (forces F u, -2F u, F u at points 0, m, last): the
+    segments left of m are compressed (multiplier -F), those right of m are
+    stretched (+F). P' keeps only the stretched half, at F/cut."""
+    np_, cut, F = 11, 0.5, 1.5
+    mid = 5
+    pts, u = _straight_fiber(dim, np_, cut, seed=7)
+    m = _forced_fiber(oracle, dim, pts, cut, {0: F * u, mid: -2 * F * u, np_ - 1: F * u})
+    lag = m.tensions()
+    expect = np.where(np.arange(np_ - 1) < mid, -F, F)
+    assert np.abs(lag - expect).max() < 1e-12 * F
+    D = np.stack([m.add_projection_diff(0, e) for e in np.eye(dim * np_)], axis=1)
+    coef = np.where(np.arange(np_ - 1) < mid, 0.0, F / cut)
+    assert np.abs(D - _chain_laplacian(np_, coef, dim)).max() < 1e-12 * F / cut
+
+
+def _constraint_jacobian(X, cut):
+    """J with rows d_s = (x_{s+1} - x_s)/cut acting on (x_{s+1} - x_s)."""
+    np_, dim = X.shape
+    J = np.zeros((np_ - 1, dim * np_))
+    for s in range(np_ - 1):
+        ds = (X[s + 1] - X[s]) / cut
+        J[s, dim * s:dim * s + dim] = -ds
+        J[s, dim * (s + 1):dim * (s + 1) + dim] = ds
+    return J
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("np_", [3, 8, 23])
+def test_tensions_equal_dense_multipliers(oracle, dim, np_):
+    """rfLag = (J J^T)^-1 J F, with J from the fiber's own points and the
+    normal equations solved densely, for random forces on a bent fiber; both
+    from the right-hand side (setSpeedsFromForces, rhs = true) and from
+    Meca::computeForces (computeTensions)."""
+    cut = 0.5
+    rng = np.random.default_rng(100 * dim + np_)
+    pts = _bent_fiber(dim, np_, cut, seed=np_) * 1.07      # segments 7% longer than cut
+    forces = {p: rng.standard_normal(dim) for p in range(np_)}
+    m = _forced_fiber(oracle, dim, pts, cut, forces)
+    Fv = np.concatenate([forces[p] for p in range(np_)])
+    J = _constraint_jacobian(pts.reshape(np_, dim), cut)
+    lam = np.linalg.solve(J @ J.T, J @ Fv)
+    assert np.abs(m.tensions() - lam).max() < 1e-12 * np.abs(lam).max()
+    m.compute_forces()
+    assert np.abs(m.tensions() - lam).max() < 1e-12 * np.abs(lam).max()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("np_", [3, 8, 31])
+def test_reshape_it_moves_along_the_constraint_gradients(oracle, dim, np_):
+    """reshape_it solves for multipliers sigma: the new points are
+    src + J(src)^T sigma with J from the segments at src, chosen so that every
+    segment has length cut. So (1) the displacement lies in the range of
+    J(src)^T (which also keeps the centroid exactly), and (2) it is the root of
+    the length equations found here independently by scipy from sigma = 0."""
+    from scipy.optimize import fsolve
+    cut = 0.5
+    rng = np.random.default_rng(50 + np_)
+    src = _bent_fiber(dim, np_, cut, seed=np_ + 3) + 0.02 * rng.standard_normal(dim * np_)
+    out, failed = oracle.reshape(dim, src, cut)
+    assert not failed
+    X = src.reshape(np_, dim)
+    Jt = (_constraint_jacobian(X, cut) * cut).T          # columns x_{s+1} - x_s placed as in J
+    delta = out - src
+    sig, *_ = np.linalg.lstsq(Jt, delta, rcond=None)
+    assert np.abs(Jt @ sig - delta).max() < 1e-12 * np.abs(delta).max()
+
+    def lengths(s):
+        Y = (src + Jt @ s).reshape(np_, dim)
+        return (np.diff(Y, axis=0) ** 2).sum(1) - cut * cut
+
+    s_ref = fsolve(lengths, np.zeros(np_ - 1))
+    assert np.abs(lengths(s_ref)).max() < 1e-10
+    assert np.abs((src + Jt @ s_ref) - out).max() < 1e-6
+    # a reshape that is not of this form fails check (1): reshape_sure, the
+    # fallback taken when Newton fails (fiber_naked.cc:367-401)
+    far = np.cumsum(rng.uniform(0.05, 0.9, (31, 2)), axis=0).ravel()
+    out2, failed2 = oracle.reshape(2, far, cut)
+    assert failed2
+    Jt2 = (_constraint_jacobian(far.reshape(31, 2), cut) * cut).T
+    d2 = out2 - far
+    sig2, *_ = np.linalg.lstsq(Jt2, d2, rcond=None)
+    assert np.abs(Jt2 @ sig2 - d2).max() > 1e-3 * np.abs(d2).max()
