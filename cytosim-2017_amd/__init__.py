@@ -22,11 +22,11 @@ MAX_FIBER_POINTS = 128
 TOL_REFERENCE, TOL_RELATIVE = 0, 1
 PRECOND_NONE, PRECOND_FP64, PRECOND_FP32 = 0, 1, 2
 GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED = 0, 1, 2
-VEC_PTS, VEC_SOL, VEC_BAS, VEC_RHS, VEC_FOR = range(5)
+VEC_PTS, VEC_SOL, VEC_BAS, VEC_RHS, VEC_FOR, VEC_PREV = range(6)
 
 EXPORTED = [
     "cymg_create", "cymg_destroy", "cymg_last_error", "cymg_set_params", "cymg_clear",
-    "cymg_add_fiber", "cymg_add_solid", "cymg_get_solid_timers", "cymg_nb_points", "cymg_set_systems", "cymg_get_system_reports",
+    "cymg_add_fiber", "cymg_add_fiber_ex", "cymg_set_identities", "cymg_add_solid", "cymg_get_solid_timers", "cymg_nb_points", "cymg_set_systems", "cymg_get_system_reports",
     "cymg_prepare", "cymg_add_pure_force",
     "cymg_inter_link_ee", "cymg_inter_link_ie", "cymg_inter_link_ii", "cymg_inter_long_link_ee",
     "cymg_inter_clamp_e", "cymg_inter_clamp_i", "cymg_inter_long_clamp", "cymg_inter_plane_e",
@@ -47,6 +47,12 @@ GUESS_INPUTS, GUESS_OUTPUTS, GUESS_POINTS, GUESS_WEIGHTS = 288, 96, 31, 230240
 class SimParams(C.Structure):
     _fields_ = [("time_step", C.c_double), ("kT", C.c_double), ("viscosity", C.c_double),
                 ("hydro_radius0", C.c_double), ("hydro_radius1", C.c_double)]
+
+
+class FiberDesc(C.Structure):
+    _fields_ = [("nb_points", C.c_uint32), ("segmentation", C.c_double), ("rigidity", C.c_double),
+                ("hydro_radius0", C.c_double), ("hydro_radius1", C.c_double), ("surface_effect", C.c_int),
+                ("cylinder_height", C.c_double)]
 
 
 class SolveParams(C.Structure):
@@ -121,6 +127,8 @@ def lib():
         "cymg_set_params": (C.c_int, [vp, C.POINTER(SimParams)]),
         "cymg_clear": (C.c_int, [vp]),
         "cymg_add_fiber": (C.c_int, [vp, u32, C.c_double, C.c_double, C.POINTER(u32)]),
+        "cymg_add_fiber_ex": (C.c_int, [vp, C.POINTER(FiberDesc), C.POINTER(u32)]),
+        "cymg_set_identities": (C.c_int, [vp, u32, vp]),
         "cymg_add_solid": (C.c_int, [vp, u32, vp, vp, C.c_int, C.POINTER(u32)]),
         "cymg_get_solid_timers": (C.c_int, [vp, u32, vp]),
         "cymg_nb_points": (C.c_int, [vp, C.POINTER(u32)]),
@@ -240,12 +248,27 @@ class Meca:
         self._chk(self.L.cymg_clear(self.h))
         self.np = []
 
-    def add_fiber(self, nb_points, segmentation, rigidity):
+    def add_fiber(self, nb_points, segmentation, rigidity, hydro_radius0=None, hydro_radius1=None,
+                  surface_height=None):
+        """Meca::add of a fiber; optional per-fiber FiberProp drag settings
+        (hydrodynamic radii, surface_height > 0 for the surface drag model)."""
         mi = C.c_uint32()
-        self._chk(self.L.cymg_add_fiber(self.h, int(nb_points), float(segmentation), float(rigidity),
-                                        C.byref(mi)))
+        if hydro_radius0 is None and hydro_radius1 is None and surface_height is None:
+            self._chk(self.L.cymg_add_fiber(self.h, int(nb_points), float(segmentation), float(rigidity),
+                                            C.byref(mi)))
+        else:
+            d = FiberDesc(int(nb_points), float(segmentation), float(rigidity), float(hydro_radius0 or 0.0),
+                          float(hydro_radius1 or 0.0), int(bool(surface_height)), float(surface_height or 0.0))
+            self._chk(self.L.cymg_add_fiber_ex(self.h, C.byref(d), C.byref(mi)))
         self.np.append(int(nb_points))
         return mi.value
+
+    def set_identities(self, ids):
+        """Inventoried::number() of every Mecable (fibers, then Solids): the
+        device then carries each object's points and previous solution over a
+        new registration order."""
+        a = np.ascontiguousarray(ids, dtype=np.uint32)
+        self._chk(self.L.cymg_set_identities(self.h, a.size, _p(a)))
 
     def add_solid(self, radius, points, reshape_timer=0):
         """A Solid (after every fiber): per-point radii and its reference shape."""
@@ -613,7 +636,7 @@ def assemble(meca, scene, couples_on_device=True, upload_couples=True):
     # Aster::setInteractions (aster.cc:53-89), after the couples (simul_solve.cc:238-239)
     if getattr(scene, "naster_links", 0) and couples_on_device:
         if upload_couples:
-            meca.set_aster_links(scene.aster_kind, scene.aster_s1, scene.aster_s1, scene.aster_fiber,
+            meca.set_aster_links(scene.aster_kind, scene.aster_s1, scene.aster_s2, scene.aster_fiber,
                                  scene.aster_abs, scene.aster_w)
         meca.aster_interactions()
         return

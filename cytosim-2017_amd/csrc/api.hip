@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 
 #include "context.h"
 #include "tile_ops.h"
@@ -168,10 +169,55 @@ void build_tiles(Context& c)
     c.last_iters[0] = c.last_iters[1] = c.last_iters[2] = 0;
 }
 
-void upload_layout(Context& c)
+// Point map from the previous registration to the current one: each Mecable
+// takes the points of the Mecable with the same identity and point count, or,
+// when neither registration gave identities, of the one at the same position
+// if every point count is unchanged.  Returns whether any point is mapped;
+// `all` tells whether every point is.
+bool remap_plan(const Context& c, std::vector<uint32_t>& map, bool& all)
+{
+    all = false;
+    if (!c.prev_layout) return false;
+    const uint32_t M = c.nmec();
+    std::vector<uint32_t> np(M), idx(M);
+    for (uint32_t f = 0; f < c.nfib; ++f) { np[f] = c.h_np[f]; idx[f] = c.h_idx[f]; }
+    for (uint32_t k = 0; k < c.nsol; ++k) { np[c.nfib + k] = c.h_sol_np[k]; idx[c.nfib + k] = c.h_sol_idx[k]; }
+    const std::vector<uint32_t>& onp = c.h_prev_np;
+    std::vector<uint32_t> oidx(onp.size() + 1, 0);
+    for (size_t m = 0; m < onp.size(); ++m) oidx[m + 1] = oidx[m] + onp[m];
+    std::vector<uint32_t> src(M, kInvalid32);   // previous Mecable of each Mecable
+    if (!c.h_ids.empty() && !c.h_prev_ids.empty()) {
+        std::unordered_map<uint32_t, uint32_t> old;
+        old.reserve(c.h_prev_ids.size() * 2);
+        for (uint32_t m = 0; m < (uint32_t)c.h_prev_ids.size(); ++m) old.emplace(c.h_prev_ids[m], m);
+        for (uint32_t m = 0; m < M; ++m) {
+            auto it = old.find(c.h_ids[m]);
+            if (it != old.end() && onp[it->second] == np[m]) src[m] = it->second;
+        }
+    } else if (c.h_ids.empty() && c.h_prev_ids.empty() && onp == np) {
+        for (uint32_t m = 0; m < M; ++m) src[m] = m;
+    }
+    map.assign(c.npts, kInvalid32);
+    bool any = false;
+    all = true;
+    for (uint32_t m = 0; m < M; ++m) {
+        if (src[m] == kInvalid32) {
+            all = false;
+            continue;
+        }
+        any = true;
+        for (uint32_t p = 0; p < np[m]; ++p) map[idx[m] + p] = oidx[src[m]] + p;
+    }
+    return any;
+}
+
+void upload_layout(Context& c, bool need_points)
 {
     ++c.layout_gen;
     const uint32_t F = c.nfib;
+    c.d_drag.resize(3 * (size_t)(F ? F : 1));
+    if (F) CYMG_HIP(hipMemcpyAsync(c.d_drag.get(), c.h_drag.data(), 3 * (size_t)F * 8, hipMemcpyHostToDevice,
+                                   c.stream));
     c.d_np.resize(F); c.d_idx.resize(F); c.d_seg.resize(F); c.d_cut.resize(F); c.d_rig.resize(F);
     c.d_mob.resize(F); c.d_rfrig.resize(F); c.d_blkoff.resize(F + 1); c.d_fib_sys.resize(F);
     std::vector<uint32_t> sb = c.h_sys_begin;
@@ -212,13 +258,34 @@ void upload_layout(Context& c)
     c.d_pfib.resize(c.npts ? c.npts : 1);
     if (c.npts) CYMG_HIP(hipMemcpyAsync(c.d_pfib.get(), pf.data(), c.npts * 4, hipMemcpyHostToDevice, st));
     const size_t n = c.n();
+    // the previous registration's points and solution, carried over per Mecable
+    std::vector<uint32_t> map;
+    bool all = false;
+    const bool any = remap_plan(c, map, all);
+    if (any) {
+        c.d_remap.resize(c.npts);
+        CYMG_HIP(hipMemcpyAsync(c.d_remap.get(), map.data(), (size_t)c.npts * 4, hipMemcpyHostToDevice, st));
+        for (DevBuf<double>* v : { &c.d_pos, &c.d_prev_sol }) {
+            if (v == &c.d_pos && !(need_points && c.have_solution)) continue;
+            c.d_old.swap(*v);
+            v->resize(n ? n : 1);
+            launch_gather_points(c, c.d_remap.get(), c.d_old.get(), v->get(), c.npts);
+        }
+    } else {
+        c.d_prev_sol.resize(n ? n : 1);
+        CYMG_HIP(hipMemsetAsync(c.d_prev_sol.get(), 0, (n ? n : 1) * 8, st));
+    }
+    c.pos_complete = any && all && c.have_solution;
+    c.h_prev_ids = c.h_ids;
+    c.h_prev_np.assign(c.h_np.begin(), c.h_np.end());
+    c.h_prev_np.insert(c.h_prev_np.end(), c.h_sol_np.begin(), c.h_sol_np.end());
+    c.prev_layout = true;
     for (DevBuf<double>* v : { &c.d_pts, &c.d_pos, &c.d_sol, &c.d_bas, &c.d_rhs, &c.d_for, &c.d_tmp, &c.d_prev_sol })
         v->resize(n ? n : 1);
     const size_t S = c.nseg ? c.nseg : 1;
     for (DevBuf<double>* v : { &c.d_diff }) v->resize(S * c.dim);
     for (DevBuf<double>* v : { &c.d_jd, &c.d_je, &c.d_jd0, &c.d_je0, &c.d_lag, &c.d_pdiff }) v->resize(S);
     CYMG_HIP(hipMemsetAsync(c.d_lag.get(), 0, S * 8, st));
-    CYMG_HIP(hipMemsetAsync(c.d_prev_sol.get(), 0, (n ? n : 1) * 8, st));
     c.d_part.resize(P_SLOTS * (size_t)(c.nmec() ? c.nmec() : 1));
     c.d_scal.resize(2 * c.nsys);
     CYMG_HIP(hipMemsetAsync(c.d_scal.get(), 0, 2 * c.nsys * sizeof(SysScalars), st));
@@ -273,6 +340,7 @@ void do_setup(Context& c, const double* gauss, int on_device)
     }
     c.setup_done = true;
     c.pc_ready = false;
+    c.solved = false;
 }
 
 // d_status: [0] fibers whose projection failed, [1] Solids that could not be
@@ -346,6 +414,7 @@ void do_iterate(Context& c, const cymg_solve_params* p, cymg_solve_report* rep)
     r.guess_used = guess_used;
     if (rep) *rep = r;
     if (r.fallback == 3) throw Error(CYMG_ERR_CONVERGENCE, "convergence failure in solver");
+    c.solved = true;
 }
 
 void do_apply(Context& c)
@@ -437,7 +506,8 @@ int cymg_set_params(cymg_context* ctx, const cymg_sim_params* p)
 int cymg_clear(cymg_context* ctx)
 {
     return guarded(ctx, [&](Context& c) {
-        c.h_np.clear(); c.h_idx.clear(); c.h_seg.clear(); c.h_cut.clear(); c.h_rig.clear();
+        c.h_np.clear(); c.h_idx.clear(); c.h_seg.clear(); c.h_cut.clear(); c.h_rig.clear(); c.h_drag.clear();
+        c.h_ids.clear();
         c.h_blkoff.assign(1, 0);
         c.nfib = c.npts = c.nseg = c.largest = 0;
         c.nsol = c.fib_pts = 0;
@@ -446,13 +516,13 @@ int cymg_clear(cymg_context* ctx)
         c.blk_total = 0;
         c.h_sys_begin.clear();
         c.layout_dirty = true;
-        c.prepared = c.setup_done = c.pc_ready = false;
+        c.prepared = c.setup_done = c.pc_ready = c.solved = false;
     });
 }
 
-int cymg_add_fiber(cymg_context* ctx, uint32_t np, double segmentation, double rigidity, uint32_t* mat_index)
+static void add_fiber(Context& c, uint32_t np, double segmentation, double rigidity, const double drag[3],
+                      uint32_t* mat_index)
 {
-    return guarded(ctx, [&](Context& c) {
         if (np < 2) throw Error(CYMG_ERR_INVALID, "a fiber needs at least 2 points");
         if (np > (uint32_t)kMaxFiberPoints)
             throw Error(CYMG_ERR_UNSUPPORTED, "fiber has more than CYMG_MAX_FIBER_POINTS points");
@@ -466,6 +536,8 @@ int cymg_add_fiber(cymg_context* ctx, uint32_t np, double segmentation, double r
         c.h_seg.push_back(c.nseg);
         c.h_cut.push_back(segmentation);
         c.h_rig.push_back(rigidity);
+        c.h_drag.insert(c.h_drag.end(), drag, drag + 3);
+        c.h_ids.clear();
         const uint64_t bs = (uint64_t)c.dim * np;
         c.blk_total += bs * bs;
         c.h_blkoff.push_back(c.blk_total);
@@ -475,7 +547,42 @@ int cymg_add_fiber(cymg_context* ctx, uint32_t np, double segmentation, double r
         c.largest = std::max(c.largest, np);
         c.nfib += 1;
         c.layout_dirty = true;
-        c.prepared = c.setup_done = c.pc_ready = false;
+        c.prepared = c.setup_done = c.pc_ready = c.solved = false;
+}
+
+int cymg_add_fiber(cymg_context* ctx, uint32_t np, double segmentation, double rigidity, uint32_t* mat_index)
+{
+    return guarded(ctx, [&](Context& c) {
+        const double drag[3] = { 0.0, 0.0, 0.0 };
+        add_fiber(c, np, segmentation, rigidity, drag, mat_index);
+    });
+}
+
+int cymg_add_fiber_ex(cymg_context* ctx, const cymg_fiber_desc* d, uint32_t* mat_index)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!d) throw Error(CYMG_ERR_INVALID, "null fiber description");
+        if (!(d->hydro_radius0 >= 0) || !(d->hydro_radius1 >= 0))
+            throw Error(CYMG_ERR_INVALID, "fiber:hydrodynamic_radius must be >= 0 (0: the context's value)");
+        // Fiber::dragCoefficientSurface (fiber.cc:405-406)
+        if (d->surface_effect && !(d->cylinder_height > 0))
+            throw Error(CYMG_ERR_INVALID, "fiber:surface_effect[1] (height above surface) must set and > 0!");
+        const double drag[3] = { d->hydro_radius0, d->hydro_radius1, d->surface_effect ? d->cylinder_height : 0.0 };
+        add_fiber(c, d->nb_points, d->segmentation, d->rigidity, drag, mat_index);
+    });
+}
+
+int cymg_set_identities(cymg_context* ctx, uint32_t n, const uint32_t* ids)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (n != c.nmec() || (n && !ids)) throw Error(CYMG_ERR_INVALID, "one identity per registered Mecable");
+        std::vector<uint32_t> v(ids, ids + n);
+        std::vector<uint32_t> sorted = v;
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            throw Error(CYMG_ERR_INVALID, "identities must be unique");
+        c.h_ids = v;
+        if (!c.layout_dirty) c.h_prev_ids = v;   // the uploaded registration is the one named
     });
 }
 
@@ -515,8 +622,9 @@ int cymg_add_solid(cymg_context* ctx, uint32_t np, const double* radius, const d
         c.h_sol_timer.push_back(reshape_timer);
         c.npts += np;
         c.nsol += 1;
+        c.h_ids.clear();
         c.layout_dirty = true;
-        c.prepared = c.setup_done = c.pc_ready = false;
+        c.prepared = c.setup_done = c.pc_ready = c.solved = false;
     });
 }
 
@@ -546,7 +654,7 @@ int cymg_set_systems(cymg_context* ctx, uint32_t nsys, const uint32_t* first_fib
             if (sb[s + 1] <= sb[s]) throw Error(CYMG_ERR_INVALID, "systems must be non-empty and in order");
         c.h_sys_begin = sb;
         c.layout_dirty = true;
-        c.prepared = c.setup_done = c.pc_ready = false;
+        c.prepared = c.setup_done = c.pc_ready = c.solved = false;
     });
 }
 
@@ -577,12 +685,16 @@ int cymg_prepare(cymg_context* ctx, const double* points)
             if (st) throw Error(CYMG_ERR_INVALID, "Cannot rescale the Solid, because it has collapsed!");
         }
         const bool fresh = c.layout_dirty;
-        if (c.layout_dirty) upload_layout(c);
+        if (!points && !(fresh ? c.prev_layout : c.have_solution))
+            throw Error(CYMG_ERR_INVALID, "no points on the device: pass the Mecable points");
+        if (c.layout_dirty) upload_layout(c, points == nullptr);
         if (points) {
             CYMG_HIP(hipMemcpyAsync(c.d_pos.get(), points, (size_t)c.n() * 8, hipMemcpyHostToDevice, c.stream));
             c.have_solution = true;
-        } else if (fresh && !c.have_solution) {
-            throw Error(CYMG_ERR_INVALID, "no points on the device: pass the Mecable points");
+        } else if (fresh && !c.pos_complete) {
+            c.have_solution = false;
+            throw Error(CYMG_ERR_INVALID, "the device does not hold the points of every Mecable registered: "
+                                          "pass the Mecable points (or give identities, cymg_set_identities)");
         }
         c.hB.clear(); c.hC.clear(); c.hBas.clear();
         c.marks.clear();
@@ -590,6 +702,7 @@ int cymg_prepare(cymg_context* ctx, const double* points)
         c.prepared = true;
         c.setup_done = false;
         c.pc_ready = false;
+        c.solved = false;
     });
 }
 
@@ -936,7 +1049,8 @@ int cymg_iterate(cymg_context* ctx, const cymg_solve_params* p, cymg_solve_repor
 int cymg_apply(cymg_context* ctx)
 {
     return guarded(ctx, [&](Context& c) {
-        if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "nothing solved");
+        if (!c.setup_done || !c.solved)
+            throw Error(CYMG_ERR_INVALID, "nothing solved: cymg_iterate must succeed after the last setup");
         do_apply(c);
     });
 }
@@ -1036,6 +1150,7 @@ int cymg_get_vector(cymg_context* ctx, int which, double* out)
             case CYMG_VEC_BAS: src = c.d_bas.get(); break;
             case CYMG_VEC_RHS: src = c.d_rhs.get(); break;
             case CYMG_VEC_FOR: src = c.d_for.get(); break;
+            case CYMG_VEC_PREV: src = c.d_prev_sol.get(); break;
             default: throw Error(CYMG_ERR_INVALID, "unknown vector");
         }
         if (!src) throw Error(CYMG_ERR_INVALID, "vector not allocated");
@@ -1178,6 +1293,7 @@ int cymg_autotune(cymg_context* ctx, int reps, int* tile_points, int* block_inve
     return guarded(ctx, [&](Context& c) {
         if (reps <= 0) throw Error(CYMG_ERR_INVALID, "reps must be > 0");
         if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_autotune");
+        c.solved = false;   // the timed launches run on the solver's vectors
         const int precision = c.pc_ready ? c.pc_precision : CYMG_PRECOND_FP64;
         // block inverse: the fastest of Gauss-Jordan in registers, in LDS and
         // blocked on the matrix cores (all produce the inverse the iteration reads)
@@ -1230,6 +1346,7 @@ int cymg_time_kernel(cymg_context* ctx, int which, int reps, double* avg_ms)
     return guarded(ctx, [&](Context& c) {
         if (!avg_ms || reps <= 0 || which < 0 || which > 7) throw Error(CYMG_ERR_INVALID, "invalid arguments");
         if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_time_kernel");
+        c.solved = false;   // the timed launches run on the solver's vectors
         if (which == 6) {
             if (!c.pc_ready) throw Error(CYMG_ERR_INVALID, "no preconditioner computed");
             *avg_ms = krylov_time_launch(c, c.pc_precision, reps);

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/cymeca.h"
@@ -54,6 +55,11 @@ struct DevBuf {
     }
     T* get() const { return ptr; }
     size_t bytes() const { return n * sizeof(T); }
+    void swap(DevBuf& o) {
+        std::swap(ptr, o.ptr);
+        std::swap(cap, o.cap);
+        std::swap(n, o.n);
+    }
 };
 
 // Per-Krylov-system scalar state, one record per independent system; two
@@ -141,6 +147,7 @@ void launch_setup(Context& c, const double* gauss);
 void launch_matvec(Context& c, const double* X, double* Y);
 void launch_forces(Context& c, const double* X, double* Y, bool with_rigidity);
 void launch_apply(Context& c);
+void launch_gather_points(Context& c, const uint32_t* map, const double* src, double* dst, uint32_t npts);
 void launch_tensions(Context& c);
 // launchers (precond.hip)
 void launch_precond_build(Context& c, int precision, double* block_out = nullptr);

@@ -27,6 +27,13 @@ struct Context {
     std::vector<uint32_t> h_np, h_idx, h_seg;
     std::vector<uint64_t> h_blkoff;
     std::vector<double> h_cut, h_rig;
+    std::vector<double> h_drag;        // per fiber: hydrodynamic radii r0, r1 (0: the context's) and the
+                                       // surface model's cylinder height (0: drag in a volume)
+    std::vector<uint32_t> h_ids;       // identity of every Mecable (fibers, then Solids), empty if not given
+    // the registration the device points and previous solution belong to
+    std::vector<uint32_t> h_prev_ids, h_prev_np;
+    bool prev_layout = false;          // a previous registration has been uploaded
+    bool pos_complete = false;         // every point of the current layout has its device position
     uint32_t nfib = 0, npts = 0, nseg = 0, largest = 0;
     uint64_t blk_total = 0;
     bool layout_dirty = true;
@@ -54,7 +61,9 @@ struct Context {
     DevBuf<uint32_t> d_pfib;           // fiber of each point
     DevBuf<TileDesc> d_tile_desc;
     DevBuf<uint64_t> d_blkoff;
-    DevBuf<double> d_cut, d_rig, d_mob, d_rfrig;
+    DevBuf<double> d_cut, d_rig, d_mob, d_rfrig, d_drag;
+    DevBuf<uint32_t> d_remap;          // new point -> point of the previous registration (or kInvalid32)
+    DevBuf<double> d_old;              // the previous registration's vector while it is remapped
 
     // vectors of DIM*nbPts (meca.h:101-106) and per-segment arrays
     DevBuf<double> d_pts, d_pos, d_sol, d_bas, d_rhs, d_for, d_tmp, d_prev_sol, d_gauss;
@@ -121,6 +130,7 @@ struct Context {
     std::vector<unsigned char> iter_graph_key[3][2];
 
     bool prepared = false, setup_done = false, pc_ready = false, have_solution = false;
+    bool solved = false;               // d_sol holds the solution of the current setup (cymg_apply needs it)
 
     // learned initial guess
     MlpState mlp;

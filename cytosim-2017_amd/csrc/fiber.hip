@@ -12,6 +12,8 @@ constexpr double kRealEpsilon = 128.0 * 2.220446049250313e-16;  // src/math/real
 // (fiber.cc:450-460): setDragCoefficient (fiber.cc:315-363, 428-447),
 // storeDifferences (rigid_fiber.cc:111-117), makeProjection
 // (rigid_fiber_project.cc:50-87), rfRigidity = rigidity / fnCut^3.
+// dragp: per fiber the hydrodynamic radii (0: the context's r0, r1) and the
+// cylinder height of the surface drag model (0: drag in a volume).
 template <int D>
 __global__ __launch_bounds__(kWave) void k_prepare(const uint32_t* __restrict__ npv, const uint32_t* __restrict__ idxv,
                                                    const uint32_t* __restrict__ segv, const double* __restrict__ cutv,
@@ -20,7 +22,8 @@ __global__ __launch_bounds__(kWave) void k_prepare(const uint32_t* __restrict__ 
                                                    double* __restrict__ rfrig, double* __restrict__ diff,
                                                    double* __restrict__ jd, double* __restrict__ je,
                                                    double* __restrict__ jd0, double* __restrict__ je0,
-                                                   int* __restrict__ status, double visc, double r0, double r1)
+                                                   int* __restrict__ status, const double* __restrict__ dragp,
+                                                   double visc, double r0d, double r1d)
 {
     __shared__ double sdif[kMaxFiberPoints * D];
     __shared__ double sd[kMaxFiberPoints], se[kMaxFiberPoints];
@@ -63,11 +66,21 @@ __global__ __launch_bounds__(kWave) void k_prepare(const uint32_t* __restrict__ 
         if (!info && S > 0 && sd[S - 1] <= 0.0) info = S;
         if (info) atomicAdd(status, 1);
         // drag and mobility
+        const double r0 = dragp[3 * f] > 0 ? dragp[3 * f] : r0d;
+        const double r1 = dragp[3 * f + 1] > 0 ? dragp[3 * f + 1] : r1d;
+        const double height = dragp[3 * f + 2];
         const double len = (double)S * cut;
-        double lenc = len > r1 ? r1 : len;
-        const double drag_sphere = 6 * r0;
-        const double drag_cyl = 3.0 * len / (log(0.5 * lenc / r0) + 0.312);
-        const double drag = M_PI * visc * (drag_cyl > drag_sphere ? drag_cyl : drag_sphere);
+        double drag;
+        if (height > 0) {
+            // Fiber::dragCoefficientSurface (fiber.cc:401-414)
+            drag = 2 * M_PI * visc * len / acosh(1 + height / r0);
+        } else {
+            // Fiber::dragCoefficientVolume (fiber.cc:315-363)
+            double lenc = len > r1 ? r1 : len;
+            const double drag_sphere = 6 * r0;
+            const double drag_cyl = 3.0 * len / (log(0.5 * lenc / r0) + 0.312);
+            drag = M_PI * visc * (drag_cyl > drag_sphere ? drag_cyl : drag_sphere);
+        }
         mob[f] = (double)np / drag;
         rfrig[f] = rigv[f] / (cut * cut * cut);
     }
@@ -93,13 +106,32 @@ void launch_fiber_prepare(Context& c)
     if (c.dim == 3)
         hipLaunchKernelGGL(k_prepare<3>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_seg.get(), c.d_cut.get(),
                            c.d_rig.get(), c.d_pos.get(), c.d_pts.get(), c.d_mob.get(), c.d_rfrig.get(),
-                           c.d_diff.get(), c.d_jd.get(), c.d_je.get(), c.d_jd0.get(), c.d_je0.get(), c.d_status.get(), c.prm.viscosity,
-                           c.prm.hydro_radius0, c.prm.hydro_radius1);
+                           c.d_diff.get(), c.d_jd.get(), c.d_je.get(), c.d_jd0.get(), c.d_je0.get(), c.d_status.get(), c.d_drag.get(),
+                           c.prm.viscosity, c.prm.hydro_radius0, c.prm.hydro_radius1);
     else
         hipLaunchKernelGGL(k_prepare<2>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_seg.get(), c.d_cut.get(),
                            c.d_rig.get(), c.d_pos.get(), c.d_pts.get(), c.d_mob.get(), c.d_rfrig.get(),
-                           c.d_diff.get(), c.d_jd.get(), c.d_je.get(), c.d_jd0.get(), c.d_je0.get(), c.d_status.get(), c.prm.viscosity,
-                           c.prm.hydro_radius0, c.prm.hydro_radius1);
+                           c.d_diff.get(), c.d_jd.get(), c.d_je.get(), c.d_jd0.get(), c.d_je0.get(), c.d_status.get(), c.d_drag.get(),
+                           c.prm.viscosity, c.prm.hydro_radius0, c.prm.hydro_radius1);
+    CYMG_HIP(hipGetLastError());
+}
+
+// dst[point] = src[map[point]] for every coordinate, 0 where map is invalid:
+// the values a new registration inherits from the previous one
+__global__ void k_gather_points(uint32_t n, int D, const uint32_t* __restrict__ map, const double* __restrict__ src,
+                                double* __restrict__ dst)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = i / (uint32_t)D, m = map[p];
+    dst[i] = m == kInvalid32 ? 0.0 : src[(size_t)D * m + (i - (uint32_t)D * p)];
+}
+
+void launch_gather_points(Context& c, const uint32_t* map, const double* src, double* dst, uint32_t npts)
+{
+    const uint32_t n = (uint32_t)c.dim * npts;
+    if (!n) return;
+    hipLaunchKernelGGL(k_gather_points, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, c.dim, map, src, dst);
     CYMG_HIP(hipGetLastError());
 }
 

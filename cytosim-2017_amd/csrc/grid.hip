@@ -345,10 +345,17 @@ void nearby_segments(Context& c, uint32_t nq, const double* places, const uint32
     CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, G.count.get(), G.offset.get(), (int)(nq + 1), st));
     G.temp.resize(std::max(tb, G.temp.n));
     CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(G.temp.get(), tb, G.count.get(), G.offset.get(), (int)(nq + 1), st));
-    CYMG_HIP(hipMemcpyAsync(total, G.offset.get() + nq, 4, hipMemcpyDeviceToHost, st));
-    CYMG_HIP(hipMemcpyAsync(counts, G.count.get(), (size_t)nq * 4, hipMemcpyDeviceToHost, st));
+    uint32_t n = 0;
+    std::vector<uint32_t> cnt(nq);
+    CYMG_HIP(hipMemcpyAsync(&n, G.offset.get() + nq, 4, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemcpyAsync(cnt.data(), G.count.get(), (size_t)nq * 4, hipMemcpyDeviceToHost, st));
     CYMG_HIP(hipStreamSynchronize(st));
-    const uint32_t n = *total;
+    // the fill pass ranks a place's segments in a list of kGridMaxKeep: refuse
+    // before any output is written
+    for (uint32_t k = 0; k < nq; ++k)
+        if (cnt[k] > (uint32_t)kGridMaxKeep) throw Error(CYMG_ERR_UNSUPPORTED, "more than 64 segments near one place");
+    std::copy(cnt.begin(), cnt.end(), counts);
+    *total = n;
     if (n > capacity || !n) return;
     G.ofib.resize(n); G.opt.resize(n); G.oabs.resize(n); G.odis.resize(n);
     q.out_fib = G.ofib.get();

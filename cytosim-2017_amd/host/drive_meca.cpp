@@ -7,8 +7,9 @@
 //            tolerance tol_mode radius stiffness / per fiber: np cut rigidity
 //            then DIM*np points / per Solid: np timer, np radii, DIM*np points /
 //            per link: fiber_a abscissa_a fiber_b abscissa_b stiffness /
-//            per aster link: kind s1 fiber abscissa stiffness (s1: matIndex of
-//            the Solid point) / per step: DIM*nbPts gaussian draws
+//            per aster link: kind s1 s2 fiber abscissa stiffness (s1, s2:
+//            matIndex of Solid points; kind 2: abscissa is the interpolation
+//            coefficient between s1 and s2) / per step: DIM*nbPts gaussian draws
 // out.txt:   per step "iterations residual", then the final points.
 #include <cstdio>
 #include <fstream>
@@ -50,9 +51,9 @@ int main(int argc, char** argv)
     std::vector<double> aa(nlinks), ab(nlinks), w(nlinks);
     for (int k = 0; k < nlinks; ++k) in >> fa[k] >> aa[k] >> fb[k] >> ab[k] >> w[k];
     std::vector<int> akind(nast);
-    std::vector<uint32_t> as1(nast), af(nast);
+    std::vector<uint32_t> as1(nast), as2(nast), af(nast);
     std::vector<double> aab(nast), aw(nast);
-    for (int k = 0; k < nast; ++k) in >> akind[k] >> as1[k] >> af[k] >> aab[k] >> aw[k];
+    for (int k = 0; k < nast; ++k) in >> akind[k] >> as1[k] >> as2[k] >> af[k] >> aab[k] >> aw[k];
     if (!in) {
         std::fprintf(stderr, "bad scene file\n");
         return 2;
@@ -79,7 +80,11 @@ int main(int argc, char** argv)
             for (int k = 0; k < nast; ++k) {
                 const cymeca::Fiber& f = fibers[af[k]];
                 if (akind[k] == 0) meca.interLink(cymeca::PointExact{ as1[k] }, cymeca::PointExact{ f.matIndex }, aw[k]);
-                else meca.interLink(cymeca::interpolate(f, dim, aab[k]), cymeca::PointExact{ as1[k] }, aw[k]);
+                else if (akind[k] == 1)
+                    meca.interLink(cymeca::interpolate(f, dim, aab[k]), cymeca::PointExact{ as1[k] }, aw[k]);
+                else   // a fiber shorter than the clamp distance: its plus end (aster.cc:77-84)
+                    meca.interLink(cymeca::PointInterpolated{ as1[k], as2[k], aab[k] },
+                                   cymeca::PointExact{ f.matIndex + f.nbPoints(dim) - 1 }, aw[k]);
             }
             cymg_solve_report rep = meca.solve(&prop, prop.precondition != 0, gauss.data(), tol_mode);
             out << rep.iterations << " " << rep.residual << "\n";

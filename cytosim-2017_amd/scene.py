@@ -81,6 +81,7 @@ class Scene:
     aster_w: np.ndarray = field(default_factory=lambda: np.zeros(0))
     # the same links as (Solid point, fiber, abscissa) for the device emitter
     aster_s1: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    aster_s2: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
     aster_fiber: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
     aster_abs: np.ndarray = field(default_factory=lambda: np.zeros(0))
     seed: int = 0
@@ -468,26 +469,42 @@ def asters(dim=3, seed=3, nasters=10, fibers_per_aster=200, lmin=5.0, lmax=10.0,
         sc.link_state = np.full(fa.size, 15, np.uint8)
         sc.motor_max_speed = 0.5
         sc.motor_stall_force = 5.0
-    # Aster::setInteractions: per fiber, centre -- minus end, then the fiber at
-    # abscissa `core` -- its Solid point
+    # Aster::setInteractions (aster.cc:53-89): per fiber, centre -- minus end,
+    # then, for a fiber longer than the clamp distance `core`, the fiber at
+    # abscissa `core` -- its Solid point (kind 1), otherwise the plus end -- the
+    # Solid's centre/point interpolated at length/core (kind 2)
     fidx = sc.mat_index
     sidx = sc.sol_index
-    kind, a1, a2, ca, b, w, s1, fl, ab = [], [], [], [], [], [], [], [], []
+    kind, a1, a2, ca, b, w, s1, s2, fl, ab = [], [], [], [], [], [], [], [], [], []
     fib = np.arange(nfib)
     m1, m2, co = sc.interpolate(fib, np.full(nfib, core))
     for a in range(nasters):
         for i in range(fibers_per_aster):
             f = a * fibers_per_aster + i
-            kind += [0, 1]
-            a1 += [sidx[a], m1[f]]
-            a2 += [sidx[a], m2[f]]
-            ca += [0.0, co[f]]
-            b += [fidx[f], sidx[a] + 1 + i]
+            length = cut[f] * (int(nbp[f]) - 1)      # FiberNaked::length()
+            clamp2 = sidx[a] + 1 + i
+            if length > core:
+                kind += [0, 1]
+                a1 += [sidx[a], m1[f]]
+                a2 += [sidx[a], m2[f]]
+                ca += [0.0, co[f]]
+                b += [fidx[f], clamp2]
+                s2 += [sidx[a], clamp2]
+                ab += [0.0, core]
+            else:
+                c = length / core
+                kind += [0, 2]
+                a1 += [sidx[a], sidx[a]]
+                a2 += [sidx[a], clamp2]
+                ca += [0.0, c]
+                b += [fidx[f], fidx[f] + int(nbp[f]) - 1]
+                s2 += [sidx[a], clamp2]
+                ab += [0.0, c]
             w += [stiffness[0], stiffness[1]]
-            s1 += [sidx[a], sidx[a] + 1 + i]
+            s1 += [sidx[a], sidx[a] if length <= core else clamp2]
             fl += [f, f]
-            ab += [0.0, core]
     sc.aster_s1 = np.array(s1, np.uint32)
+    sc.aster_s2 = np.array(s2, np.uint32)
     sc.aster_fiber = np.array(fl, np.uint32)
     sc.aster_abs = np.array(ab)
     sc.aster_kind = np.array(kind, np.int64)
@@ -506,10 +523,11 @@ def config3(seed=3):
     return asters(dim=3, seed=seed, spread=5.0, min_separation=3.0)
 
 
-def mini_asters(dim=3, seed=11, nasters=2, fibers_per_aster=6, nmotors=12):
-    """Small aster scene for operator-level parity checks."""
-    return asters(dim=dim, seed=seed, nasters=nasters, fibers_per_aster=fibers_per_aster, lmin=1.5, lmax=3.0,
-                  spread=1.0, radius=2.5, nmotors=nmotors, binding_range=0.3,
+def mini_asters(dim=3, seed=11, nasters=2, fibers_per_aster=6, nmotors=12, lmin=1.5, lmax=3.0, core=0.5):
+    """Small aster scene for operator-level parity checks; with lmin below
+    `core` some fibers are shorter than the clamp distance (aster.cc:77-84)."""
+    return asters(dim=dim, seed=seed, nasters=nasters, fibers_per_aster=fibers_per_aster, lmin=lmin, lmax=lmax,
+                  core=core, spread=1.0, radius=2.5, nmotors=nmotors, binding_range=0.3,
                   name=f"mini-asters-{dim}d-{nasters}x{fibers_per_aster}")
 
 

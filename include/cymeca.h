@@ -97,6 +97,32 @@ int         cymg_clear(cymg_context* ctx);
 int         cymg_add_fiber(cymg_context* ctx, uint32_t nb_points, double segmentation,
                            double rigidity, uint32_t* mat_index);
 int         cymg_nb_points(const cymg_context* ctx, uint32_t* nb_points);
+/* Meca::add of a Fiber with its own FiberProp drag settings (fiber.cc:315-447):
+   hydrodynamic_radius[2] (0: the context's cymg_sim_params values) and
+   surface_effect with cylinder_height (> 0: Fiber::dragCoefficientSurface,
+   drag = 2 pi viscosity length / acosh(1 + height / radius0); 0: the drag in
+   a volume, Fiber::dragCoefficientVolume) */
+typedef struct {
+    uint32_t nb_points;
+    double   segmentation;     /* fnCut */
+    double   rigidity;         /* FiberProp::rigidity */
+    double   hydro_radius0;    /* FiberProp::hydrodynamic_radius[0], 0: context value */
+    double   hydro_radius1;    /* FiberProp::hydrodynamic_radius[1], 0: context value */
+    int      surface_effect;   /* FiberProp::surface_effect */
+    double   cylinder_height;  /* FiberProp::cylinder_height (> 0 when surface_effect) */
+} cymg_fiber_desc;
+int         cymg_add_fiber_ex(cymg_context* ctx, const cymg_fiber_desc* desc, uint32_t* mat_index);
+/* Inventoried::number() (inventoried.h:43) of every Mecable registered since cymg_clear (fibers in
+   registration order, then Solids), unique.  When the previous registration
+   gave identities too, the device keeps per Mecable what it knew of the same
+   object (same identity and number of points) across the new order: its points
+   (so cymg_prepare(NULL) can follow a reshuffle) and its share of the previous
+   solution (CYMG_GUESS_PREVIOUS and the learned guess's inputs).  This is the
+   per-object keying that a warm start needs when Simul::step reshuffles the
+   Mecables (node_list.cc:244-280; meca.cc:695-703 explains why the reference
+   seeds with zero instead).  Without identities these survive only an
+   unchanged registration (same point counts in the same order). */
+int         cymg_set_identities(cymg_context* ctx, uint32_t n, const uint32_t* ids);
 /* Meca::add of a Solid (solid.h), after every fiber (simul_solve.cc:204-207):
    per-point radii (Solid::addSphere, solid.cc:352-362; 0 = no drag), the
    reference shape fixed from `points` (DIM per point, Solid::fixShape,
@@ -217,6 +243,9 @@ int         cymg_solve(cymg_context* ctx, const cymg_solve_params* params, const
 int         cymg_setup(cymg_context* ctx, const double* gauss, int gauss_on_device);
 int         cymg_iterate(cymg_context* ctx, const cymg_solve_params* params,
                          cymg_solve_report* report);
+/* needs a successful cymg_iterate since the last prepare/setup; the timing
+   calls (cymg_autotune, cymg_time_kernel) reuse the solver's vectors, so they
+   also require a new iterate before apply */
 int         cymg_apply(cymg_context* ctx);
 
 /* operators, valid after setup (host arrays of DIM*nbPts) */
@@ -229,7 +258,9 @@ int         cymg_multiply_device(cymg_context* ctx, const double* dX, double* dY
 int         cymg_precondition_device(cymg_context* ctx, const double* dX, double* dY);
 
 /* export */
-enum { CYMG_VEC_PTS = 0, CYMG_VEC_SOL, CYMG_VEC_BAS, CYMG_VEC_RHS, CYMG_VEC_FOR };
+/* CYMG_VEC_PREV (no counterpart in the reference): the previous step's solution
+   as the current registration sees it, the seed of CYMG_GUESS_PREVIOUS */
+enum { CYMG_VEC_PTS = 0, CYMG_VEC_SOL, CYMG_VEC_BAS, CYMG_VEC_RHS, CYMG_VEC_FOR, CYMG_VEC_PREV };
 int         cymg_get_vector(cymg_context* ctx, int which, double* out);  /* vPTS..vFOR (meca.h:101-106) */
 int         cymg_get_points(cymg_context* ctx, double* out);   /* Mecable points after getPoints (fiber_naked.cc:414-425) */
 int         cymg_get_tensions(cymg_context* ctx, double* out); /* RigidFiber::tension, one per segment (rigid_fiber.h:90) */

@@ -421,6 +421,8 @@ typedef struct {
     double   rigidity;      /* FiberProp::rigidity */
     double   rf_rigidity;   /* rigidity / cut^3 (fiber.cc:459) */
     double   mobility;      /* rfMobility (fiber.cc:446) */
+    double   hr0, hr1;      /* FiberProp::hydrodynamic_radius (0: the Meca's parameters) */
+    double   height;        /* FiberProp::cylinder_height with surface_effect (0: volume drag) */
     double*  blk;           /* preconditioner block (mecable.cc:15-34) */
     uint32_t blk_alloc;
     int      use_blk;
@@ -515,9 +517,18 @@ static orc_fiber* new_mecable(orc_meca* m)
 
 int orc_add_fiber(orc_meca* m, uint32_t np, double segmentation, double rigidity)
 {
+    return orc_add_fiber_ex(m, np, segmentation, rigidity, 0.0, 0.0, 0.0);
+}
+
+int orc_add_fiber_ex(orc_meca* m, uint32_t np, double segmentation, double rigidity, double hr0, double hr1,
+                     double height)
+{
     if (np < 2) return -1;
     orc_fiber* f = new_mecable(m);
     f->kind = 0;
+    f->hr0 = hr0;
+    f->hr1 = hr1;
+    f->height = height;
     f->np = np;
     f->idx = m->npts;
     f->seg = m->nseg;
@@ -581,15 +592,23 @@ uint32_t orc_best_nb_points(double ratio)
     return n + 1;
 }
 
-/* Fiber::dragCoefficientVolume (fiber.cc:315-363) and setDragCoefficient (428-447) */
+/* Fiber::dragCoefficientVolume (fiber.cc:315-363), dragCoefficientSurface
+   (fiber.cc:401-414) and setDragCoefficient (428-447) */
 static double fiber_mobility(const orc_meca* m, const orc_fiber* f)
 {
+    const double r0 = f->hr0 > 0 ? f->hr0 : m->prm.hydro_radius0;
+    const double r1 = f->hr1 > 0 ? f->hr1 : m->prm.hydro_radius1;
     double len = f->cut * (double)(f->np - 1);      /* length() = nbSegments()*fnCut */
-    double lenc = len;
-    if (lenc > m->prm.hydro_radius1) lenc = m->prm.hydro_radius1;
-    double drag_sphere = 6.0 * m->prm.hydro_radius0;
-    double drag_cyl = 3.0 * len / (log(0.5 * lenc / m->prm.hydro_radius0) + 0.312);
-    double drag = M_PI * m->prm.viscosity * (drag_cyl > drag_sphere ? drag_cyl : drag_sphere);
+    double drag;
+    if (f->height > 0) {
+        drag = 2 * M_PI * m->prm.viscosity * len / acosh(1 + f->height / r0);
+    } else {
+        double lenc = len;
+        if (lenc > r1) lenc = r1;
+        double drag_sphere = 6.0 * r0;
+        double drag_cyl = 3.0 * len / (log(0.5 * lenc / r0) + 0.312);
+        drag = M_PI * m->prm.viscosity * (drag_cyl > drag_sphere ? drag_cyl : drag_sphere);
+    }
     return (double)f->np / drag;
 }
 

@@ -54,6 +54,10 @@ void      orc_set_params(orc_meca*, const orc_params*);
 /* Meca::clear / Meca::add (meca.cc:55-79) for a RigidFiber Mecable */
 void      orc_clear(orc_meca*);
 int       orc_add_fiber(orc_meca*, uint32_t nb_points, double segmentation, double rigidity);
+/* the same with the fiber's own FiberProp::hydrodynamic_radius (0: the
+   parameters') and, height > 0, the surface drag model (fiber.cc:401-447) */
+int       orc_add_fiber_ex(orc_meca*, uint32_t nb_points, double segmentation, double rigidity,
+                           double hydro_radius0, double hydro_radius1, double height);
 int       orc_add_solid(orc_meca*, uint32_t nb_points, const double* radius, const double* points,
                         int reshape_timer);   /* Solid after fixShape (solid.cc:352-400) */
 uint32_t  orc_nb_points(const orc_meca*);

@@ -65,6 +65,8 @@ def lib():
         "orc_set_params": (None, [vp, C.POINTER(Params)]),
         "orc_clear": (None, [vp]),
         "orc_add_fiber": (C.c_int, [vp, C.c_uint32, C.c_double, C.c_double]),
+        "orc_add_fiber_ex": (C.c_int, [vp, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_double]),
         "orc_add_solid": (C.c_int, [vp, C.c_uint32, _f64p, _f64p, C.c_int]),
         "orc_get_solid": (None, [vp, C.c_int, _f64p]),
         "orc_solid_timer": (C.c_int, [vp, C.c_int]),
@@ -163,8 +165,12 @@ class OracleMeca:
         self.np = []
         self.nseg = 0
 
-    def add_fiber(self, nb_points, segmentation, rigidity):
-        r = self.L.orc_add_fiber(self.h, int(nb_points), float(segmentation), float(rigidity))
+    def add_fiber(self, nb_points, segmentation, rigidity, hydro_radius0=0.0, hydro_radius1=0.0,
+                  surface_height=0.0):
+        """A fiber; hydro radii 0 take the Meca's parameters, surface_height > 0
+        selects the surface drag model (fiber.cc:401-447)."""
+        r = self.L.orc_add_fiber_ex(self.h, int(nb_points), float(segmentation), float(rigidity),
+                                    float(hydro_radius0), float(hydro_radius1), float(surface_height))
         if r < 0:
             raise ValueError("fiber needs at least 2 points")
         self.nfib += 1

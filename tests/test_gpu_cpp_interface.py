@@ -33,7 +33,7 @@ def write_scene(path, sc, nsteps, tolerance, tol_mode):
             f.write(f"{int(sc.link_fiber_a[k])} {float(sc.link_abs_a[k])!r} {int(sc.link_fiber_b[k])} "
                     f"{float(sc.link_abs_b[k])!r} {float(sc.link_stiffness[k])!r}\n")
         for k in range(sc.naster_links):
-            f.write(f"{int(sc.aster_kind[k])} {int(sc.aster_s1[k])} {int(sc.aster_fiber[k])} "
+            f.write(f"{int(sc.aster_kind[k])} {int(sc.aster_s1[k])} {int(sc.aster_s2[k])} {int(sc.aster_fiber[k])} "
                     f"{float(sc.aster_abs[k])!r} {float(sc.aster_w[k])!r}\n")
         for s in range(nsteps):
             f.write(" ".join(repr(float(x)) for x in sc.gauss(s)) + "\n")
@@ -59,12 +59,18 @@ def test_cpp_driver_matches_oracle(scenes, oracle, tmp_path):
     assert rel_err(pts, orc.fiber_points()) < 1e-9
 
 
-def test_cpp_driver_asters_match_oracle(scenes, oracle, tmp_path):
+@pytest.mark.parametrize("short", [False, True])
+def test_cpp_driver_asters_match_oracle(scenes, oracle, tmp_path, short):
     """Solids through the C++ interface, re-registered every step as
     Simul::solve does (their reshape timers kept by the host objects): 9 steps
-    so that every Solid is reshaped at least once."""
+    so that every Solid is reshaped at least once.  With `short`, some fibers
+    are shorter than the clamp distance and are held by their plus end
+    (aster.cc:77-84)."""
     assert os.path.exists(DRIVER), "build the package first"
-    sc = scenes.mini_asters(dim=3, seed=42)
+    sc = scenes.mini_asters(dim=3, seed=42, lmin=0.3, lmax=1.4, core=0.8) if short else \
+        scenes.mini_asters(dim=3, seed=42)
+    if short:
+        assert (sc.aster_kind == 2).any() and (sc.aster_kind == 1).any()
     nsteps, tol = 9, 1e-11
     write_scene(tmp_path / "scene.txt", sc, nsteps, tol, 1)
     out = tmp_path / "out.txt"

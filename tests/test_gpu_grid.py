@@ -86,3 +86,22 @@ def test_nearby_segments_validation(pkg, scenes):
     counts, out = gpu.nearby_segments(np.zeros((0, 3)), 0.1)
     assert counts.size == 0 and out[0].size == 0
     gpu.close()
+
+
+def test_more_than_64_segments_near_a_place_is_refused(pkg, scenes, oracle):
+    """A range that brings more than 64 segments near one place: the call
+    fails with CYMG_ERR_UNSUPPORTED before writing any output."""
+    sc = scenes.config2(seed=47, nfib=80, nlinks=50, radius=2.5)
+    gpu = gpu_from_scene(pkg, sc)
+    orc = oracle_from_scene(oracle, sc)
+    W = np.zeros((2, 3))
+    assert orc.nearby_segments(W[0], 2.0)[0].size > 64
+    import ctypes as C
+    counts = np.full(2, 7, np.uint32)
+    total = C.c_uint32(12345)
+    print("grid: overflow refused", flush=True)
+    rc = gpu.L.cymg_nearby_segments(gpu.h, 2, W.ctypes.data_as(C.c_void_p), None, 2.0, 0,
+                                    counts.ctypes.data_as(C.c_void_p), 0, None, None, None, None, C.byref(total))
+    assert rc == pkg.ERR_UNSUPPORTED
+    assert np.all(counts == 7) and total.value == 12345
+    gpu.close()

@@ -15,9 +15,15 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("device_emitters", [True, False])
-def test_aster_assembly_bit_exact(pkg, scenes, oracle, dim, device_emitters):
-    sc = scenes.mini_asters(dim=dim)
-    print(f"aster assembly dim={dim} device emitters={device_emitters}", flush=True)
+@pytest.mark.parametrize("short", [False, True])
+def test_aster_assembly_bit_exact(pkg, scenes, oracle, dim, device_emitters, short):
+    """With `short`, fibers shorter than the clamp distance are held by their
+    plus end to the Solid's centre and clamp point interpolated at
+    length/clampA (aster.cc:77-84, the device emitter's kind 2)."""
+    sc = scenes.mini_asters(dim=dim, lmin=0.3, lmax=1.4, core=0.8) if short else scenes.mini_asters(dim=dim)
+    if short:
+        assert (sc.aster_kind == 2).any()
+    print(f"aster assembly dim={dim} device emitters={device_emitters} short={short}", flush=True)
     gpu = gpu_from_scene(pkg, sc, device_emitters)
     orc = oracle_from_scene(oracle, sc)
     g = sc.gauss(0)

@@ -601,3 +601,29 @@ def test_reshape_it_moves_along_the_constraint_gradients(oracle, dim, np_):
     d2 = out2 - far
     sig2, *_ = np.linalg.lstsq(Jt2, d2, rcond=None)
     assert np.abs(Jt2 @ sig2 - d2).max() > 1e-3 * np.abs(d2).max()
+
+
+@pytest.mark.parametrize("L", [0.3, 2.0, 8.5, 15.0])
+@pytest.mark.parametrize("h", [0.005, 0.05, 1.0])
+def test_mobility_surface_model(oracle, L, h):
+    """Near a surface (fiber:surface_effect), drag = 2 pi eta L / acosh(1 + h/r)
+    (Hunt et al. 1994, the formula of Fiber::dragCoefficientSurface): for
+    x = 1 + h/r >> 1, acosh(x) = ln(2x) to O(x^-2); a per-fiber radius
+    overrides the Meca's parameters."""
+    eta, r = 0.02, 0.01
+    npts = 6
+    cut = L / (npts - 1)
+    m = oracle.OracleMeca(3, dict(time_step=0.005, kT=0.0042, viscosity=eta))
+    m.add_fiber(npts, cut, 20.0, hydro_radius0=r, surface_height=h)
+    m.add_fiber(npts, cut, 20.0, hydro_radius0=r)
+    pts = np.concatenate([_bent_fiber(3, npts, cut), _bent_fiber(3, npts, cut, seed=1) + 3])
+    m.prepare(pts)
+    x = 1 + h / r
+    drag = 2 * np.pi * eta * L / np.log(x + np.sqrt(x * x - 1))
+    assert m.mobility(0) == pytest.approx(npts / drag, rel=1e-13)
+    if x > 50:
+        assert m.mobility(0) == pytest.approx(npts * np.log(2 * x) / (2 * np.pi * eta * L), rel=1e-4)
+    # volume model with the fiber's own radius (r = 0.01 instead of 0.0125)
+    p = min(L, 10.0) / (2 * r)
+    vol = np.pi * eta * max(3 * L / (np.log(p) + 0.312), 6 * r)
+    assert m.mobility(1) == pytest.approx(npts / vol, rel=1e-13)
