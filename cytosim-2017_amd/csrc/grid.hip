@@ -262,8 +262,10 @@ void nearby_segments(Context& c, uint32_t nq, const double* places, const uint32
     hipStream_t st = c.stream;
     const int D = c.dim;
     const uint32_t fib_pts = c.fib_pts, nseg = c.nseg;
-    *total = 0;
-    if (!nq) return;
+    if (!nq) {
+        *total = 0;
+        return;
+    }
     GridBufs& G = c.grid;
     G.places.resize((size_t)D * nq);
     CYMG_HIP(hipMemcpyAsync(G.places.get(), places, (size_t)D * nq * 8, hipMemcpyHostToDevice, st));
