@@ -31,7 +31,7 @@ EXPORTED = [
     "cymg_inter_link_ee", "cymg_inter_link_ie", "cymg_inter_link_ii", "cymg_inter_long_link_ee",
     "cymg_inter_clamp_e", "cymg_inter_clamp_i", "cymg_inter_long_clamp", "cymg_inter_plane_e",
     "cymg_confine_sphere", "cymg_set_couples", "cymg_couples_interactions", "cymg_set_couple_state",
-    "cymg_get_couples", "cymg_step_couples", "cymg_nearby_segments", "cymg_set_aster_links",
+    "cymg_get_couples", "cymg_step_couples", "cymg_nearby_segments", "cymg_nearby_segments_noalloc", "cymg_set_aster_links",
     "cymg_aster_interactions", "cymg_solve",
     "cymg_setup", "cymg_iterate", "cymg_apply", "cymg_multiply", "cymg_compute_preconditioner",
     "cymg_precondition", "cymg_compute_forces", "cymg_multiply_device", "cymg_precondition_device",
@@ -152,6 +152,8 @@ def lib():
         "cymg_step_couples": (C.c_int, [vp, C.POINTER(MotorProps)]),
         "cymg_nearby_segments": (C.c_int, [vp, u32, vp, vp, C.c_double, C.c_int, vp, u32, vp, vp, vp, vp,
                                            C.POINTER(u32)]),
+        "cymg_nearby_segments_noalloc": (C.c_int, [vp, u32, vp, vp, C.c_double, C.c_int, vp, u32, vp, vp, vp, vp,
+                                                   C.POINTER(u32)]),
         "cymg_set_aster_links": (C.c_int, [vp, u32, vp, vp, vp, vp, vp, vp]),
         "cymg_aster_interactions": (C.c_int, [vp]),
         "cymg_solve": (C.c_int, [vp, C.POINTER(SolveParams), vp, C.POINTER(SolveReport)]),

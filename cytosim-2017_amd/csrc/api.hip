@@ -458,6 +458,7 @@ int cymg_create(int dim, int device, cymg_context** out)
         CYMG_HIP(hipHostMalloc((void**)&c.h_status, 4 * sizeof(int), hipHostMallocDefault));
         for (int i = 0; i < 10; ++i) CYMG_HIP(hipEventCreate(&c.ev[i]));
         for (int i = 0; i < 2; ++i) CYMG_HIP(hipEventCreateWithFlags(&c.chunk_ev[i], hipEventDisableTiming));
+        grid_scratch_init(c.device);   // the no-allocation query's array on this device
     });
     if (rc) {
         delete ctx;
@@ -947,6 +948,20 @@ int cymg_nearby_segments(cymg_context* ctx, uint32_t nq, const double* places, c
         if (capacity && (!fiber || !point || !abscissa || !dist_sqr)) throw Error(CYMG_ERR_INVALID, "null output");
         nearby_segments(c, nq, places, exclude, range, inclusive, counts, capacity, fiber, point, abscissa,
                         dist_sqr, total);
+    });
+}
+
+int cymg_nearby_segments_noalloc(cymg_context* ctx, uint32_t nq, const double* places, const uint32_t* exclude,
+                                 double range, int inclusive, uint32_t* counts, uint32_t capacity, uint32_t* fiber,
+                                 uint32_t* point, double* abscissa, double* dist_sqr, uint32_t* total)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.prepared) throw Error(CYMG_ERR_INVALID, "no points: cymg_prepare first");
+        if (!(range >= 0)) throw Error(CYMG_ERR_INVALID, "range must be >= 0");
+        if (!total || (nq && (!places || !counts))) throw Error(CYMG_ERR_INVALID, "null argument");
+        if (capacity && (!fiber || !point || !abscissa || !dist_sqr)) throw Error(CYMG_ERR_INVALID, "null output");
+        nearby_segments_noalloc(c, nq, places, exclude, range, inclusive, counts, capacity, fiber, point, abscissa,
+                                dist_sqr, total);
     });
 }
 

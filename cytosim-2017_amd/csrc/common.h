@@ -165,6 +165,10 @@ double krylov_time_launch(Context& c, int precond, int launches);
 void nearby_segments(Context& c, uint32_t nq, const double* places, const uint32_t* exclude, double range,
                      int inclusive, uint32_t* counts, uint32_t capacity, uint32_t* fiber, uint32_t* point,
                      double* abscissa, double* dist_sqr, uint32_t* total);
+void nearby_segments_noalloc(Context& c, uint32_t nq, const double* places, const uint32_t* exclude, double range,
+                             int inclusive, uint32_t* counts, uint32_t capacity, uint32_t* fiber, uint32_t* point,
+                             double* abscissa, double* dist_sqr, uint32_t* total);
+void grid_scratch_init(int device);
 void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter, double tol,
                   int tol_mode, cymg_solve_report* rep);
 

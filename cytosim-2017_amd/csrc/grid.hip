@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #pragma clang fp contract(off)
 
@@ -250,14 +251,31 @@ __global__ __launch_bounds__(kGridThreads) void k_query_wave(uint32_t nq, GridQu
     }
 }
 
+// Device memory of one query: staging of the inputs, the bucket structure,
+// the count/offset pass, the sort/scan work space and the output staging.
+struct GridSpace {
+    double* places;
+    uint32_t *excl, *count, *offset, *key, *key2, *val, *val2, *start, *end;
+    int* flag;
+    unsigned long long* half;
+    void* temp;
+    size_t temp_bytes;
+    uint32_t *ofib, *opt;
+    double *oabs, *odis;
+};
+
 // Host side of cymg_nearby_segments.  Cell side h: a kept segment has
 // |w - P1|^2 - abs^2 <= range^2 with 0 <= abs <= segmentation (or is within
 // range of a fiber end), so its midpoint lies within sqrt(range^2 +
 // segmentation^2) + half its length of the place; h is that bound for the
 // longest segmentation and segment, enlarged by 1e-6 against rounding.
-void nearby_segments(Context& c, uint32_t nq, const double* places, const uint32_t* exclude, double range,
-                     int inclusive, uint32_t* counts, uint32_t capacity, uint32_t* fiber, uint32_t* point,
-                     double* abscissa, double* dist_sqr, uint32_t* total)
+// `sp` provides the device memory: sp.inputs(G) before anything runs, then
+// sp.temp(G, bytes) for the sort and scan, sp.outputs(G, n) for n kept
+// segments (after the count pass, before any output is written).
+template <class Space>
+static void grid_query(Context& c, Space& sp, uint32_t nq, const double* places, const uint32_t* exclude,
+                       double range, int inclusive, uint32_t* counts, uint32_t capacity, uint32_t* fiber,
+                       uint32_t* point, double* abscissa, double* dist_sqr, uint32_t* total)
 {
     hipStream_t st = c.stream;
     const int D = c.dim;
@@ -266,31 +284,25 @@ void nearby_segments(Context& c, uint32_t nq, const double* places, const uint32
         *total = 0;
         return;
     }
-    GridBufs& G = c.grid;
-    G.places.resize((size_t)D * nq);
-    CYMG_HIP(hipMemcpyAsync(G.places.get(), places, (size_t)D * nq * 8, hipMemcpyHostToDevice, st));
-    if (exclude) {
-        G.excl.resize(nq);
-        CYMG_HIP(hipMemcpyAsync(G.excl.get(), exclude, (size_t)nq * 4, hipMemcpyHostToDevice, st));
-    }
-    G.count.resize(nq + 1);
-    G.offset.resize(nq + 1);
-    G.flag.resize(1);
-    G.half.resize(1);
-    CYMG_HIP(hipMemsetAsync(G.half.get(), 0, 8, st));
-    CYMG_HIP(hipMemsetAsync(G.flag.get(), 0, sizeof(int), st));
+    GridSpace G;
+    std::memset(&G, 0, sizeof(G));
+    sp.inputs(G, nq, nseg, exclude != nullptr);
+    CYMG_HIP(hipMemcpyAsync(G.places, places, (size_t)D * nq * 8, hipMemcpyHostToDevice, st));
+    if (exclude) CYMG_HIP(hipMemcpyAsync(G.excl, exclude, (size_t)nq * 4, hipMemcpyHostToDevice, st));
+    CYMG_HIP(hipMemsetAsync(G.half, 0, 8, st));
+    CYMG_HIP(hipMemsetAsync(G.flag, 0, sizeof(int), st));
     const dim3 b(kGridThreads), gp((fib_pts + kGridThreads - 1) / kGridThreads), gs((nseg + kGridThreads - 1) / kGridThreads);
     if (fib_pts) {
         if (D == 3)
             hipLaunchKernelGGL(k_seg_half<3>, gp, b, 0, st, fib_pts, c.d_pfib.get(), c.d_idx.get(), c.d_np.get(),
-                               c.d_pos.get(), G.half.get());
+                               c.d_pos.get(), G.half);
         else
             hipLaunchKernelGGL(k_seg_half<2>, gp, b, 0, st, fib_pts, c.d_pfib.get(), c.d_idx.get(), c.d_np.get(),
-                               c.d_pos.get(), G.half.get());
+                               c.d_pos.get(), G.half);
         CYMG_HIP(hipGetLastError());
     }
     unsigned long long hb = 0;
-    CYMG_HIP(hipMemcpyAsync(&hb, G.half.get(), 8, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemcpyAsync(&hb, G.half, 8, hipMemcpyDeviceToHost, st));
     CYMG_HIP(hipStreamSynchronize(st));
     double half = 0.0;
     std::memcpy(&half, &hb, 8);
@@ -299,82 +311,230 @@ void nearby_segments(Context& c, uint32_t nq, const double* places, const uint32
     double h = (std::sqrt(range * range + cut_max * cut_max) + half) * (1.0 + 1e-6);
     if (!(h > 0)) h = 1.0;
     // buckets: segments sorted by the hash of their midpoint's cell
-    G.key.resize(nseg ? nseg : 1); G.key2.resize(nseg ? nseg : 1);
-    G.val.resize(nseg ? nseg : 1); G.val2.resize(nseg ? nseg : 1);
-    G.start.resize(kGridBuckets); G.end.resize(kGridBuckets);
-    CYMG_HIP(hipMemsetAsync(G.end.get(), 0, (size_t)kGridBuckets * 4, st));
+    CYMG_HIP(hipMemsetAsync(G.end, 0, (size_t)kGridBuckets * 4, st));
     if (nseg) {
         if (D == 3)
             hipLaunchKernelGGL(k_seg_keys<3>, gp, b, 0, st, fib_pts, c.d_pfib.get(), c.d_idx.get(), c.d_np.get(),
-                               c.d_pos.get(), 1.0 / h, G.key.get(), G.val.get());
+                               c.d_pos.get(), 1.0 / h, G.key, G.val);
         else
             hipLaunchKernelGGL(k_seg_keys<2>, gp, b, 0, st, fib_pts, c.d_pfib.get(), c.d_idx.get(), c.d_np.get(),
-                               c.d_pos.get(), 1.0 / h, G.key.get(), G.val.get());
+                               c.d_pos.get(), 1.0 / h, G.key, G.val);
         CYMG_HIP(hipGetLastError());
         size_t tb = 0;
-        CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, G.key.get(), G.key2.get(), G.val.get(), G.val2.get(),
-                                                    (int)nseg, 0, kGridHashBits, st));
-        G.temp.resize(std::max(tb, G.temp.n));
-        CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(G.temp.get(), tb, G.key.get(), G.key2.get(), G.val.get(),
-                                                    G.val2.get(), (int)nseg, 0, kGridHashBits, st));
-        hipLaunchKernelGGL(k_buckets, gs, b, 0, st, nseg, G.key2.get(), G.start.get(), G.end.get());
+        CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, G.key, G.key2, G.val, G.val2, (int)nseg, 0,
+                                                    kGridHashBits, st));
+        sp.temp(G, tb);
+        CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(G.temp, tb, G.key, G.key2, G.val, G.val2, (int)nseg, 0,
+                                                    kGridHashBits, st));
+        hipLaunchKernelGGL(k_buckets, gs, b, 0, st, nseg, G.key2, G.start, G.end);
         CYMG_HIP(hipGetLastError());
     }
     GridQuery q;
     std::memset(&q, 0, sizeof(q));
-    q.places = G.places.get();
-    q.exclude = exclude ? G.excl.get() : nullptr;
+    q.places = G.places;
+    q.exclude = exclude ? G.excl : nullptr;
     q.range2 = range * range;
     q.inclusive = inclusive;
     q.inv_h = 1.0 / h;
-    q.sorted = G.val2.get();
-    q.start = G.start.get();
-    q.end = G.end.get();
+    q.sorted = G.val2;
+    q.start = G.start;
+    q.end = G.end;
     q.pfib = c.d_pfib.get();
     q.idx = c.d_idx.get();
     q.np = c.d_np.get();
     q.cut = c.d_cut.get();
     q.pos = c.d_pos.get();
-    q.count = G.count.get();
-    q.offset = G.offset.get();
-    q.overflow = G.flag.get();
+    q.count = G.count;
+    q.offset = G.offset;
+    q.overflow = G.flag;
     const dim3 gq((nq + kGridWaves - 1) / kGridWaves);
     if (D == 3) hipLaunchKernelGGL((k_query_wave<3, false>), gq, b, 0, st, nq, q);
     else hipLaunchKernelGGL((k_query_wave<2, false>), gq, b, 0, st, nq, q);
     CYMG_HIP(hipGetLastError());
-    CYMG_HIP(hipMemsetAsync(G.count.get() + nq, 0, 4, st));
+    CYMG_HIP(hipMemsetAsync(G.count + nq, 0, 4, st));
     size_t tb = 0;
-    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, G.count.get(), G.offset.get(), (int)(nq + 1), st));
-    G.temp.resize(std::max(tb, G.temp.n));
-    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(G.temp.get(), tb, G.count.get(), G.offset.get(), (int)(nq + 1), st));
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, G.count, G.offset, (int)(nq + 1), st));
+    sp.temp(G, tb);
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(G.temp, tb, G.count, G.offset, (int)(nq + 1), st));
     uint32_t n = 0;
     std::vector<uint32_t> cnt(nq);
-    CYMG_HIP(hipMemcpyAsync(&n, G.offset.get() + nq, 4, hipMemcpyDeviceToHost, st));
-    CYMG_HIP(hipMemcpyAsync(cnt.data(), G.count.get(), (size_t)nq * 4, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemcpyAsync(&n, G.offset + nq, 4, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemcpyAsync(cnt.data(), G.count, (size_t)nq * 4, hipMemcpyDeviceToHost, st));
     CYMG_HIP(hipStreamSynchronize(st));
     // the fill pass ranks a place's segments in a list of kGridMaxKeep: refuse
     // before any output is written
     for (uint32_t k = 0; k < nq; ++k)
         if (cnt[k] > (uint32_t)kGridMaxKeep) throw Error(CYMG_ERR_UNSUPPORTED, "more than 64 segments near one place");
+    if (n <= capacity && n) sp.outputs(G, n);
     std::copy(cnt.begin(), cnt.end(), counts);
     *total = n;
     if (n > capacity || !n) return;
-    G.ofib.resize(n); G.opt.resize(n); G.oabs.resize(n); G.odis.resize(n);
-    q.out_fib = G.ofib.get();
-    q.out_pt = G.opt.get();
-    q.out_abs = G.oabs.get();
-    q.out_dis = G.odis.get();
+    q.out_fib = G.ofib;
+    q.out_pt = G.opt;
+    q.out_abs = G.oabs;
+    q.out_dis = G.odis;
     if (D == 3) hipLaunchKernelGGL((k_query_wave<3, true>), gq, b, 0, st, nq, q);
     else hipLaunchKernelGGL((k_query_wave<2, true>), gq, b, 0, st, nq, q);
     CYMG_HIP(hipGetLastError());
     int overflow = 0;
-    CYMG_HIP(hipMemcpyAsync(&overflow, G.flag.get(), sizeof(int), hipMemcpyDeviceToHost, st));
-    CYMG_HIP(hipMemcpyAsync(fiber, G.ofib.get(), (size_t)n * 4, hipMemcpyDeviceToHost, st));
-    CYMG_HIP(hipMemcpyAsync(point, G.opt.get(), (size_t)n * 4, hipMemcpyDeviceToHost, st));
-    CYMG_HIP(hipMemcpyAsync(abscissa, G.oabs.get(), (size_t)n * 8, hipMemcpyDeviceToHost, st));
-    CYMG_HIP(hipMemcpyAsync(dist_sqr, G.odis.get(), (size_t)n * 8, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemcpyAsync(&overflow, G.flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemcpyAsync(fiber, G.ofib, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemcpyAsync(point, G.opt, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemcpyAsync(abscissa, G.oabs, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemcpyAsync(dist_sqr, G.odis, (size_t)n * 8, hipMemcpyDeviceToHost, st));
     CYMG_HIP(hipStreamSynchronize(st));
     if (overflow) throw Error(CYMG_ERR_UNSUPPORTED, "more than 64 segments near one place");
+}
+
+// the context's grow-only buffers (cymg_nearby_segments)
+struct ContextSpace {
+    GridBufs& B;
+    int D;
+    void inputs(GridSpace& G, uint32_t nq, uint32_t nseg, bool excl)
+    {
+        B.places.resize((size_t)D * nq);
+        if (excl) B.excl.resize(nq);
+        B.count.resize(nq + 1);
+        B.offset.resize(nq + 1);
+        B.flag.resize(1);
+        B.half.resize(1);
+        const size_t ns = nseg ? nseg : 1;
+        B.key.resize(ns); B.key2.resize(ns); B.val.resize(ns); B.val2.resize(ns);
+        B.start.resize(kGridBuckets); B.end.resize(kGridBuckets);
+        G.places = B.places.get();
+        G.excl = B.excl.get();
+        G.count = B.count.get();
+        G.offset = B.offset.get();
+        G.flag = B.flag.get();
+        G.half = B.half.get();
+        G.key = B.key.get(); G.key2 = B.key2.get(); G.val = B.val.get(); G.val2 = B.val2.get();
+        G.start = B.start.get(); G.end = B.end.get();
+    }
+    void temp(GridSpace& G, size_t bytes)
+    {
+        B.temp.resize(std::max(bytes, B.temp.n));
+        G.temp = B.temp.get();
+        G.temp_bytes = B.temp.n;
+    }
+    void outputs(GridSpace& G, uint32_t n)
+    {
+        B.ofib.resize(n); B.opt.resize(n); B.oabs.resize(n); B.odis.resize(n);
+        G.ofib = B.ofib.get(); G.opt = B.opt.get(); G.oabs = B.oabs.get(); G.odis = B.odis.get();
+    }
+};
+
+void nearby_segments(Context& c, uint32_t nq, const double* places, const uint32_t* exclude, double range,
+                     int inclusive, uint32_t* counts, uint32_t capacity, uint32_t* fiber, uint32_t* point,
+                     double* abscissa, double* dist_sqr, uint32_t* total)
+{
+    ContextSpace sp{ c.grid, c.dim };
+    grid_query(c, sp, nq, places, exclude, range, inclusive, counts, capacity, fiber, point, abscissa, dist_sqr,
+               total);
+}
+
+// ---------------------------------------------------------------------------
+// The same query without any allocation (cymg_nearby_segments_noalloc): all
+// of its device memory is carved out of one file-scope array per GPU,
+// `scratch`, whose device address each GPU's first context looks up once
+// (grid_scratch_init, from cymg_create).  Host code never uses the array's
+// name as an address; it uses the looked-up pointer.  Every call on a GPU
+// shares that GPU's array: calls that overlap on one GPU (two streams, two
+// threads) race.  What must start at zero (the bucket ends, the half-length
+// maximum, the overflow flag, the sentinel count) is cleared by every call.
+constexpr size_t kScratchAlign = 256;
+constexpr size_t align_up(size_t x) { return (x + kScratchAlign - 1) / kScratchAlign * kScratchAlign; }
+constexpr size_t kNaPlaces = CYMG_NOALLOC_MAX_PLACES, kNaSegs = CYMG_NOALLOC_MAX_SEGMENTS,
+                 kNaKept = CYMG_NOALLOC_MAX_KEPT;
+constexpr size_t kOffPlaces = 0;
+constexpr size_t kOffExcl = kOffPlaces + align_up(3 * 8 * kNaPlaces);
+constexpr size_t kOffCount = kOffExcl + align_up(4 * kNaPlaces);
+constexpr size_t kOffOffset = kOffCount + align_up(4 * (kNaPlaces + 1));
+constexpr size_t kOffFlag = kOffOffset + align_up(4 * (kNaPlaces + 1));
+constexpr size_t kOffHalf = kOffFlag + kScratchAlign;
+constexpr size_t kOffKey = kOffHalf + kScratchAlign;
+constexpr size_t kOffKey2 = kOffKey + align_up(4 * kNaSegs);
+constexpr size_t kOffVal = kOffKey2 + align_up(4 * kNaSegs);
+constexpr size_t kOffVal2 = kOffVal + align_up(4 * kNaSegs);
+constexpr size_t kOffStart = kOffVal2 + align_up(4 * kNaSegs);
+constexpr size_t kOffEnd = kOffStart + align_up(4 * (size_t)kGridBuckets);
+constexpr size_t kOffTemp = kOffEnd + align_up(4 * (size_t)kGridBuckets);
+constexpr size_t kTempBytes = 4u << 20;
+constexpr size_t kOffFib = kOffTemp + kTempBytes;
+constexpr size_t kOffPt = kOffFib + align_up(4 * kNaKept);
+constexpr size_t kOffAbs = kOffPt + align_up(4 * kNaKept);
+constexpr size_t kOffDis = kOffAbs + align_up(8 * kNaKept);
+constexpr size_t kScratchBytes = kOffDis + align_up(8 * kNaKept);
+static_assert(kScratchBytes <= CYMG_NOALLOC_SCRATCH_BYTES, "the no-allocation query's regions exceed its array");
+
+static __device__ __align__(256) unsigned char scratch[CYMG_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// once per device, with that device current: the device address of its copy
+// of `scratch`
+void grid_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) throw Error(CYMG_ERR_UNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        throw Error(CYMG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<unsigned char*>(ptr);
+}
+
+struct ScratchSpace {
+    unsigned char* base;
+    void inputs(GridSpace& G, uint32_t, uint32_t, bool)
+    {
+        G.places = reinterpret_cast<double*>(base + kOffPlaces);
+        G.excl = reinterpret_cast<uint32_t*>(base + kOffExcl);
+        G.count = reinterpret_cast<uint32_t*>(base + kOffCount);
+        G.offset = reinterpret_cast<uint32_t*>(base + kOffOffset);
+        G.flag = reinterpret_cast<int*>(base + kOffFlag);
+        G.half = reinterpret_cast<unsigned long long*>(base + kOffHalf);
+        G.key = reinterpret_cast<uint32_t*>(base + kOffKey);
+        G.key2 = reinterpret_cast<uint32_t*>(base + kOffKey2);
+        G.val = reinterpret_cast<uint32_t*>(base + kOffVal);
+        G.val2 = reinterpret_cast<uint32_t*>(base + kOffVal2);
+        G.start = reinterpret_cast<uint32_t*>(base + kOffStart);
+        G.end = reinterpret_cast<uint32_t*>(base + kOffEnd);
+    }
+    void temp(GridSpace& G, size_t bytes)
+    {
+        if (bytes > kTempBytes) throw Error(CYMG_ERR_UNSUPPORTED, "sort work space beyond the no-allocation array");
+        G.temp = base + kOffTemp;
+        G.temp_bytes = kTempBytes;
+    }
+    void outputs(GridSpace& G, uint32_t n)
+    {
+        if (n > kNaKept)
+            throw Error(CYMG_ERR_UNSUPPORTED, "more than CYMG_NOALLOC_MAX_KEPT segments kept in one call");
+        G.ofib = reinterpret_cast<uint32_t*>(base + kOffFib);
+        G.opt = reinterpret_cast<uint32_t*>(base + kOffPt);
+        G.oabs = reinterpret_cast<double*>(base + kOffAbs);
+        G.odis = reinterpret_cast<double*>(base + kOffDis);
+    }
+};
+
+void nearby_segments_noalloc(Context& c, uint32_t nq, const double* places, const uint32_t* exclude, double range,
+                             int inclusive, uint32_t* counts, uint32_t capacity, uint32_t* fiber, uint32_t* point,
+                             double* abscissa, double* dist_sqr, uint32_t* total)
+{
+    if (nq > kNaPlaces) throw Error(CYMG_ERR_UNSUPPORTED, "more than CYMG_NOALLOC_MAX_PLACES places");
+    if (c.nseg > kNaSegs) throw Error(CYMG_ERR_UNSUPPORTED, "more than CYMG_NOALLOC_MAX_SEGMENTS fiber segments");
+    unsigned char* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        base = (c.device >= 0 && c.device < kMaxDevices) ? g_scratch_ptr[c.device] : nullptr;
+    }
+    if (!base) throw Error(CYMG_ERR_INVALID, "no-allocation array not set up on this device");
+    ScratchSpace sp{ base };
+    grid_query(c, sp, nq, places, exclude, range, inclusive, counts, capacity, fiber, point, abscissa, dist_sqr,
+               total);
 }
 
 }  // namespace cymg

@@ -218,6 +218,25 @@ int         cymg_nearby_segments(cymg_context* ctx, uint32_t nq, const double* p
                                  double range, int inclusive, uint32_t* counts, uint32_t capacity,
                                  uint32_t* fiber, uint32_t* point, double* abscissa, double* dist_sqr,
                                  uint32_t* total);
+/* cymg_nearby_segments without allocating: no counterpart in the reference;
+   for callers that cannot allocate device memory (code under HIP graph
+   capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+   results and errors as cymg_nearby_segments; its device memory (input and
+   output staging, cell buckets, counts, sort work space) is one fixed array of
+   CYMG_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, so it
+   refuses (CYMG_ERR_UNSUPPORTED) more than CYMG_NOALLOC_MAX_PLACES places,
+   a layout of more than CYMG_NOALLOC_MAX_SEGMENTS fiber segments, or more than
+   CYMG_NOALLOC_MAX_KEPT kept segments when they would be written out.  Every
+   call on a GPU shares that GPU's array: calls that overlap on one GPU (from
+   two contexts or threads) race. */
+#define CYMG_NOALLOC_MAX_PLACES    65536
+#define CYMG_NOALLOC_MAX_SEGMENTS  131072
+#define CYMG_NOALLOC_MAX_KEPT      262144
+#define CYMG_NOALLOC_SCRATCH_BYTES 25165824
+int         cymg_nearby_segments_noalloc(cymg_context* ctx, uint32_t nq, const double* places,
+                                         const uint32_t* exclude, double range, int inclusive, uint32_t* counts,
+                                         uint32_t capacity, uint32_t* fiber, uint32_t* point, double* abscissa,
+                                         double* dist_sqr, uint32_t* total);
 /* Aster::setInteractions (aster.cc:53-89) for a list of aster links, uploaded
    by set_aster_links and emitted on the device by aster_interactions, each as
    the host interLink call it stands for:
