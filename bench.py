@@ -5,14 +5,24 @@ Workload: 3D, 1000 fibers with lengths uniform in [2, 15] um (segmentation 0.5,
 20000 crosslinkers (stiffness 100) bound at crossings within 0.05 um,
 viscosity 0.02, time_step 0.005, kT 0.0042.  A step is one implicit step:
 Meca::prepare + device assembly (confinement, couples) + Meca::solve at
-relative residual |r|_inf/|b|_inf < 1e-6 + apply (update, forces, reshape).
-Fiber points and the per-step N(0,1) Brownian draws are resident in HBM before
-the timed region.
+relative residual |r|_inf/|b|_inf < 1e-6 + apply (update, forces, reshape) +
+the export of the points and forces to the host (Meca::solve's getPoints /
+getForces, meca.cc:826-835).  Fiber points and the per-step N(0,1) Brownian
+draws are resident in HBM before the timed region.
 
-Modes: fp64 (zero seed, FP64 preconditioner blocks: `value`), fp32 (FP32
-blocks) and learned-fp32 (FP32 blocks, seeded by the per-fiber network trained
-on the device from an untimed rollout of FP64 steps; rollout and training time
-are reported beside it, not inside the timed steps).
+Modes (each on its own context, from the scene's initial state, same
+Brownian draws): fp64 (zero seed, FP64 preconditioner blocks: `value`),
+fp64-previous (seeded by the previous step's solution, the initial guess the
+learned one is compared with), fp32 (FP32 blocks), learned-fp64 and
+learned-fp32 (seeded by the per-fiber network, trained on the device from
+(input, solution) pairs of an untimed rollout of zero-seeded FP64 steps with
+draws of their own; rollout and training times are reported beside the steps,
+not inside them).  Every mode's first step is compared with the oracle's FP64
+step solved to 1e-12 on the same input.
+
+Beside it: the 64-system batched workload (FP64, FP32 and learned + FP32), the
+aster workload (FP64 and FP32, first step against the FP64 oracle), the
+FiberGrid query, and the CPU oracle on the same steps as the FP64 mode.
 
 Prints one JSON line.  --gpus N with N > 1 exits: every box has one GPU.
 """
@@ -30,6 +40,17 @@ sys.path.insert(0, ROOT)
 import __graft_entry__  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+MODES = {   # name: (preconditioner, initial guess)
+    "fp64": ("PRECOND_FP64", "GUESS_ZERO"),
+    "fp64-previous": ("PRECOND_FP64", "GUESS_PREVIOUS"),
+    "fp32": ("PRECOND_FP32", "GUESS_ZERO"),
+    "learned-fp64": ("PRECOND_FP64", "GUESS_LEARNED"),
+    "learned-fp32": ("PRECOND_FP32", "GUESS_LEARNED"),
+}
+
+
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def algorithmic_bytes_matvec(sc, kb_full, kc_full):
@@ -63,19 +84,19 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nfib", type=int, default=1000)
     ap.add_argument("--nlinks", type=int, default=20000)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps timed for cpu_baseline (0: skip)")
-    ap.add_argument("--modes", default="fp64,fp32,learned-fp32",
-                    help="fp64 / fp32: zero-seeded BCGSP with FP64 / FP32 preconditioner blocks; "
-                         "learned-fp32: seeded by the trained per-fiber network, FP32 blocks")
+    ap.add_argument("--cpu-steps", type=int, default=24, help="oracle steps timed for cpu_baseline (0: skip)")
+    ap.add_argument("--modes", default=",".join(MODES), help="comma-separated subset of " + ", ".join(MODES))
+    ap.add_argument("--oracle-check", type=int, default=1,
+                    help="compare each mode's first step with the oracle's FP64 step solved to 1e-12")
     ap.add_argument("--autotune", type=int, default=1,
                     help="run the library's autotuner (tile shape, block-inverse method) before the warmup")
-    ap.add_argument("--guess-rollout", type=int, default=32,
-                    help="untimed FP64 steps recorded as training samples for the learned guess")
-    ap.add_argument("--guess-train-steps", type=int, default=1000)
+    ap.add_argument("--guess-rollout", type=int, default=200,
+                    help="untimed FP64 steps recorded as training pairs (one per fiber and step)")
+    ap.add_argument("--guess-train-steps", type=int, default=4000)
     ap.add_argument("--guess-batch", type=int, default=512)
     ap.add_argument("--guess-lr", type=float, default=1e-3)
     ap.add_argument("--timed-launches", type=int, default=40,
@@ -85,75 +106,388 @@ def parse():
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run a plain device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--batched-steps", type=int, default=3,
-                    help="steps of the 64-system batched workload (0: skip)")
+                    help="timed steps of the 64-system batched workload per mode (0: skip)")
+    ap.add_argument("--batched-modes", default="fp64,fp32,learned-fp32")
+    ap.add_argument("--batched-rollout", type=int, default=4,
+                    help="untimed FP64 steps of the batched workload recorded for its learned guess")
     ap.add_argument("--grid-places", type=int, default=50000,
                     help="places of the timed FiberGrid query (0: skip)")
     ap.add_argument("--aster-steps", type=int, default=5,
                     help="timed steps of the aster workload (10 Solids + 2000 fibers, 50000 motors; 0: skip)")
+    ap.add_argument("--only", default="all", choices=["all", "c2", "batched", "asters", "grid"],
+                    help="run one workload alone (for profiler runs); the JSON line is printed with 'all' or 'c2'")
     return ap.parse_args()
 
 
-def train_guess(pkg, m, sc, args, tol, torch):
+def cpu_description():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return model, os.cpu_count(), usable
+
+
+def train_guess(pkg, m, sc, rollout, train_steps, batch, lr, tol, torch, assemble):
     """Record a rollout of zero-seeded FP64 steps (Brownian draws of their own,
-    distinct from the timed steps'), then train the network on the device."""
+    distinct from the timed steps'), then train the network on the device;
+    returns the training record and the weights."""
     t0 = time.perf_counter()
-    m.guess_record(args.guess_rollout * sc.nfib)
-    for s in range(args.guess_rollout):
+    m.guess_record(rollout * sc.nfib)
+    for s in range(rollout):
         if s > 0:
             m.prepare(None)
-            pkg.assemble(m, sc, upload_couples=False)
+            assemble(m)
         m.solve(tol, sc.gauss(1000 + s), tol_mode=pkg.TOL_RELATIVE, precondition=pkg.PRECOND_FP64)
     m.guess_record(0)
+    samples = m.guess_samples()
     t1 = time.perf_counter()
     m.guess_init(1)
-    first = m.guess_train(1, batch=args.guess_batch, lr=args.guess_lr, seed=1)
-    last = m.guess_train(args.guess_train_steps, batch=args.guess_batch, lr=args.guess_lr, seed=2)
+    first = m.guess_train(1, batch=batch, lr=lr, seed=1)
+    last = m.guess_train(train_steps, batch=batch, lr=lr, seed=2)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    return {"rollout_steps": args.guess_rollout, "samples": args.guess_rollout * sc.nfib,
-            "rollout_s": round(t1 - t0, 3), "train_steps": args.guess_train_steps, "batch": args.guess_batch,
+    info = {"rollout_steps": rollout, "pairs": samples, "rollout_s": round(t1 - t0, 3),
+            "train_steps": train_steps, "batch": batch, "epochs": round(train_steps * batch / max(samples, 1), 2),
             "train_s": round(t2 - t1, 3), "loss_first": first, "loss_last": last}
+    return info, m.guess_weights()
 
 
-def run_mode(pkg, sc, precond, steps, warmup, tol, gauss_dev, torch, guess=0, train=None, autotune=True):
+def errors(pts, sol, ref_pts, ref_sol):
+    return {"position_rel": float(np.abs(pts - ref_pts).max() / np.abs(ref_pts).max()),
+            "displacement_rel": float(np.abs(sol - ref_sol).max() / np.abs(ref_sol).max())}
+
+
+def run_c2_mode(pkg, sc, mode, args, gauss_dev, torch, weights, ref):
+    precond = getattr(pkg, MODES[mode][0])
+    guess = getattr(pkg, MODES[mode][1])
     m = pkg.Meca(sc.dim, 0)
     pkg.load_scene(m, sc)
-    iters = []
     tuned = None
-    if autotune:
+    if args.autotune:
         # untimed: the autotuner picks the tile shape and block-inverse method on this system
         m.setup(gauss_device_ptr=gauss_dev[0].data_ptr())
-        m.compute_preconditioner(precond if precond else pkg.PRECOND_FP64)
+        m.compute_preconditioner(precond)
         tuned = m.autotune(reps=20)
-    info = train(m) if train else None
-    if train:
-        # the timed steps start from the scene's initial state, like the other modes
         m.prepare(sc.points)
         pkg.assemble(m, sc, upload_couples=False)
+    if guess == pkg.GUESS_LEARNED:
+        m.set_guess_weights(weights)
+    pts_host = np.empty(sc.size)
+    for_host = np.empty(sc.size)
+    iters = []
+    err = err_typ = None
 
     def one(step):
         if step > 0:
             m.prepare(None)
             pkg.assemble(m, sc, upload_couples=False)
-        rep = m.solve(tol, tol_mode=pkg.TOL_RELATIVE, precondition=precond, guess=guess,
+        rep = m.solve(args.tol, tol_mode=pkg.TOL_RELATIVE, precondition=precond, guess=guess,
                       gauss_device_ptr=gauss_dev[step % len(gauss_dev)].data_ptr())
-        if guess and rep["guess_used"] != guess:
-            raise RuntimeError("learned guess not used")
+        if step > 0 and rep["guess_used"] != guess:
+            raise RuntimeError(f"{mode}: initial guess {guess} not used")
+        # Meca::solve's export: getPoints and getForces of every Mecable
+        m.export_points(pts_host)
+        m.export_vector(pkg.VEC_FOR, for_host)
         iters.append(rep["iterations"])
-        return rep
 
-    for s in range(warmup):
+    for s in range(args.warmup):
+        if s == args.warmup - 1 and args.oracle_check and s > 0:
+            before = pts_host.copy()          # the points this step starts from
         one(s)
+        if s == 0 and ref is not None:
+            err = errors(pts_host, m.vector(pkg.VEC_SOL), ref[0], ref[1])
+        if s == args.warmup - 1 and args.oracle_check and s > 0:
+            # a step of the running simulation (after the relaxation of the
+            # generated configuration): the oracle solves it from the same points
+            from oracle import pyoracle
+            orc = pyoracle.OracleMeca(sc.dim, sc.params())
+            pyoracle.register_scene(orc, sc)
+            orc.prepare(before)
+            pyoracle.assemble_scene(orc, sc)
+            orc.solve(1e-12, sc.gauss(s % len(gauss_dev)), tol_mode=1)
+            err_typ = dict(errors(pts_host, m.vector(pkg.VEC_SOL), orc.fiber_points(), orc.vector(pyoracle.SOL)),
+                           step=s)
     torch.cuda.synchronize()
-    iters.clear()
     t0 = time.perf_counter()
-    for s in range(steps):
-        one(warmup + s)
+    for s in range(args.steps):
+        one(args.warmup + s)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    if tuned:
-        info = dict(info or {}, autotune=tuned)
-    return m, dt, iters, info
+    timed = iters[args.warmup:]
+    return m, {"seconds": dt, "steps_per_s": args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
+               "mean_iters": float(np.mean(timed)), "iters": iters, "error": err, "error_step": err_typ,
+               "autotune": tuned}
+
+
+def c2(args, pkg, scenes, torch):
+    log(f"generating scene nfib={args.nfib} nlinks={args.nlinks}")
+    sc = scenes.config2(seed=2, nfib=args.nfib, nlinks=args.nlinks)
+    gauss_dev = [torch.from_numpy(sc.gauss(s)).to("cuda:0") for s in range(8)]
+    torch.cuda.synchronize()
+    modes = [x for x in args.modes.split(",") if x]
+    for x in modes:
+        if x not in MODES:
+            raise SystemExit(f"unknown mode {x}")
+    if "fp64" not in modes:
+        modes.insert(0, "fp64")
+    from oracle import pyoracle
+
+    ref = None
+    if args.oracle_check:
+        t0 = time.perf_counter()
+        orc = pyoracle.from_scene(sc)
+        orc.solve(1e-12, sc.gauss(0), tol_mode=1)
+        ref = (orc.fiber_points(), orc.vector(pyoracle.SOL))
+        log(f"oracle FP64 step at 1e-12: {time.perf_counter() - t0:.1f} s")
+        del orc
+
+    training = weights = None
+    if any(MODES[x][1] == "GUESS_LEARNED" for x in modes):
+        mt = pkg.Meca(sc.dim, 0)
+        pkg.load_scene(mt, sc)
+        training, weights = train_guess(pkg, mt, sc, args.guess_rollout, args.guess_train_steps, args.guess_batch,
+                                        args.guess_lr, args.tol, torch,
+                                        lambda mm: pkg.assemble(mm, sc, upload_couples=False))
+        mt.close()
+        log(f"learned guess: {training}")
+
+    results = {}
+    keep = None
+    for mode in modes:
+        log(f"mode {mode} warmup={args.warmup} steps={args.steps}")
+        m, r = run_c2_mode(pkg, sc, mode, args, gauss_dev, torch, weights, ref)
+        results[mode] = r
+        log(f"{mode}: {r['steps_per_s']:.2f} steps/s, mean iterations {r['mean_iters']:.1f}, error {r['error']}")
+        if mode == "fp64":
+            keep = m
+        else:
+            m.close()
+
+    # dominant-kernel timing on the final state of the FP64 run
+    m = keep
+    m.setup(gauss_device_ptr=gauss_dev[0].data_ptr())
+    mv_ms = m.time_kernel(0, reps=200)
+    m.compute_preconditioner(pkg.PRECOND_FP64)
+    pc_ms = m.time_kernel(1, reps=200)
+    build_ms = m.time_kernel(2, reps=5)
+    build_only_ms = m.time_kernel(4, reps=5)
+    build_lu_ms = m.time_kernel(5, reps=5)
+    rB, cB, _ = m.matrix_entries(0)
+    rC, cC, _ = m.matrix_entries(1)
+    kb_full = 2 * int((rB != cB).sum()) + int((rB == cB).sum())
+    kc_full = 2 * int((rC != cC).sum()) + int((rC == cC).sum())
+    mv_bytes = algorithmic_bytes_matvec(sc, kb_full, kc_full)
+    nblk = int(((sc.dim * sc.nb_points.astype(np.int64)) ** 2).sum())
+    m.compute_preconditioner(pkg.PRECOND_FP64)
+    it_ms = m.time_kernel(6, reps=args.timed_launches)
+    it_bytes = algorithmic_bytes_iteration_launch(sc, kb_full, kc_full, 8 * nblk)
+    m.compute_preconditioner(pkg.PRECOND_FP32)
+    it32_ms = m.time_kernel(6, reps=args.timed_launches)
+    it32_bytes = algorithmic_bytes_iteration_launch(sc, kb_full, kc_full, 4 * nblk)
+    m.close()
+    if args.copy_calibration_mb > 0:
+        nb = args.copy_calibration_mb << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    kern = {"iteration_launch_fp64_ms": it_ms, "iteration_launch_fp64_bytes": int(it_bytes),
+            "iteration_launch_fp32_ms": it32_ms, "iteration_launch_fp32_bytes": int(it32_bytes),
+            "iteration_launch_fp32_gbs": round(it32_bytes / (it32_ms * 1e-3) / 1e9, 1),
+            "matvec_ms": mv_ms, "matvec_gbs": round(mv_bytes / (mv_ms * 1e-3) / 1e9, 1), "precond_apply_ms": pc_ms,
+            "precond_build_fp64_ms": build_ms, "precond_assemble_only_ms": build_only_ms,
+            "precond_build_getrf_getri_ms": build_lu_ms}
+    return sc, results, training, kern, (it_ms, it_bytes)
+
+
+def batched(args, pkg, scenes, torch):
+    log("generating 64 independent systems")
+    batch = scenes.config5(seed=5)
+    big = batch.scene
+    gb = [torch.from_numpy(big.gauss(s)).to("cuda:0") for s in range(2)]
+    torch.cuda.synchronize()
+    out = {"workload": f"batch-64-systems-{big.nfib}-fibers-{big.npts}-points-{big.nlinks}-links",
+           "systems": batch.nsys, "fibers": big.nfib, "unknowns": big.size, "modes": {}}
+    weights = None
+    for mode in [x for x in args.batched_modes.split(",") if x]:
+        precond = getattr(pkg, MODES[mode][0])
+        guess = getattr(pkg, MODES[mode][1])
+        mb = pkg.Meca(3, 0)
+        pkg.load_batch(mb, batch)
+        rec = {}
+        if guess == pkg.GUESS_LEARNED:
+            if weights is None:
+                info, weights = train_guess(pkg, mb, big, args.batched_rollout, args.guess_train_steps,
+                                            args.guess_batch, args.guess_lr, args.tol, torch,
+                                            lambda mm: pkg.assemble(mm, big, True, False))
+                rec["untimed_training"] = info
+                mb.prepare(big.points)
+                pkg.assemble(mb, big, True, False)
+            mb.set_guess_weights(weights)
+        if args.autotune:
+            # untimed: tile shape and block-inverse method for this workload
+            mb.setup(gauss_device_ptr=gb[0].data_ptr())
+            mb.compute_preconditioner(precond)
+            rec["untimed_autotune"] = mb.autotune(reps=6)
+            mb.prepare(big.points)
+            pkg.assemble(mb, big, True, False)
+        its = []
+        t0 = None
+        for s in range(1 + args.batched_steps):
+            if s == 1:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+            if s > 0:
+                mb.prepare(None)
+                pkg.assemble(mb, big, True, False)
+            rep = mb.solve(args.tol, tol_mode=pkg.TOL_RELATIVE, precondition=precond, guess=guess,
+                           gauss_device_ptr=gb[s % 2].data_ptr())
+            if s > 0:
+                its.append(rep["iterations"])
+        torch.cuda.synchronize()
+        bdt = time.perf_counter() - t0
+        rec.update({"steps_per_s": round(args.batched_steps / bdt, 3),
+                    "system_steps_per_s": round(batch.nsys * args.batched_steps / bdt, 2),
+                    "max_bicgstab_iters": float(np.mean(its))})
+        if guess == pkg.GUESS_ZERO:
+            # full-work iteration launch over all 64 systems (every tile iterating)
+            mb.setup(gauss_device_ptr=gb[0].data_ptr())
+            mb.compute_preconditioner(precond)
+            b_ms = mb.time_kernel(6, reps=10)
+            rB, cB, _ = mb.matrix_entries(0)
+            rC, cC, _ = mb.matrix_entries(1)
+            b_kb = 2 * int((rB != cB).sum()) + int((rB == cB).sum())
+            b_kc = 2 * int((rC != cC).sum()) + int((rC == cC).sum())
+            del rB, cB, rC, cC
+            b_blk = int(((big.dim * big.nb_points.astype(np.int64)) ** 2).sum())
+            b_bytes = algorithmic_bytes_iteration_launch(big, b_kb, b_kc, (8 if precond == pkg.PRECOND_FP64 else 4)
+                                                         * b_blk)
+            b_gbs = b_bytes / (b_ms * 1e-3) / 1e9
+            rec["iteration_launch"] = {"avg_ms": round(b_ms, 4), "bytes_per_launch": int(b_bytes),
+                                       "achieved_gbs": round(b_gbs, 1), "frac_of_hbm_peak": round(b_gbs / 8000.0, 4)}
+        out["modes"][mode] = rec
+        log(f"batched {mode}: {rec}")
+        mb.close()
+    return out
+
+
+def asters_workload(args, pkg, scenes, torch):
+    log("generating the aster scene")
+    ast = scenes.config3(seed=3)
+    ga = [torch.from_numpy(ast.gauss(s)).to("cuda:0") for s in range(2)]
+    torch.cuda.synchronize()
+    ref = None
+    if args.oracle_check:
+        from oracle import pyoracle
+        t0 = time.perf_counter()
+        orc = pyoracle.from_scene(ast)
+        orc.solve(1e-12, ast.gauss(0), tol_mode=1)
+        ref = (orc.fiber_points(), orc.vector(pyoracle.SOL))
+        log(f"oracle FP64 aster step at 1e-12: {time.perf_counter() - t0:.1f} s")
+        del orc
+    out = {"workload": f"asters-10x200-fibers-{ast.nfib}-solids-{ast.nsol}-points-{ast.npts}-motors-{ast.nlinks}",
+           "unknowns": ast.size, "iteration": "two launches per iteration (a tile per Solid)",
+           "motors": f"stepped on the device between solves (speed {ast.motor_max_speed} um/s, "
+                     f"stall {ast.motor_stall_force} pN)", "modes": {}}
+    for mode in ("fp64", "fp32"):
+        precond = getattr(pkg, MODES[mode][0])
+        ma = pkg.Meca(3, 0)
+        pkg.load_scene(ma, ast)
+        its = []
+        rec = {}
+        t0 = None
+        for s in range(2 + args.aster_steps):
+            if s == 2:
+                if args.autotune:
+                    rec["untimed_autotune"] = ma.autotune(reps=6)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+            if s > 0:
+                ma.prepare(None)
+                pkg.assemble(ma, ast, upload_couples=False)
+            rep = ma.solve(args.tol, tol_mode=pkg.TOL_RELATIVE, precondition=precond,
+                           gauss_device_ptr=ga[s % 2].data_ptr())
+            if s == 0 and ref is not None:
+                rec["first_step_error_vs_fp64_oracle"] = errors(ma.fiber_points(), ma.vector(pkg.VEC_SOL), *ref)
+            # the motors walk under their loads between solves (Couple::stepAA)
+            ma.step_couples(ast.motor_max_speed, ast.motor_stall_force)
+            if s >= 2:
+                its.append(rep["iterations"])
+        torch.cuda.synchronize()
+        adt = time.perf_counter() - t0
+        _, _, cst = ma.get_couples()
+        rec.update({"steps_per_s": round(args.aster_steps / adt, 3), "ms_per_step": round(1e3 * adt / args.aster_steps, 3),
+                    "mean_bicgstab_iters": float(np.mean(its)),
+                    "couples_bridging": f"{int(((cst & 3) == 3).sum())} of {cst.size}"})
+        out["modes"][mode] = rec
+        log(f"asters {mode}: {rec}")
+        ma.close()
+    return out
+
+
+def grid_query(args, pkg, scenes, sc=None):
+    # FiberGrid query for many places at once on the 1000-fiber system:
+    # places near the fibers, tryToAttach's range test; host arrays in and out
+    # are included in the time
+    if sc is None:
+        sc = scenes.config2(seed=2, nfib=args.nfib, nlinks=args.nlinks)
+    mg = pkg.Meca(3, 0)
+    pkg.load_scene(mg, sc)
+    rng = np.random.default_rng(11)
+    P = sc.points.reshape(-1, 3)
+    W = P[rng.integers(0, P.shape[0], args.grid_places)] + rng.uniform(-0.1, 0.1, (args.grid_places, 3))
+    mg.nearby_segments(W, 0.05, inclusive=True)          # untimed: buffers
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        counts, out = mg.nearby_segments(W, 0.05, inclusive=True)
+    gdt = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        counts2, _ = mg.nearby_segments(W, 0.05, inclusive=True, noalloc=True)
+    ndt = (time.perf_counter() - t0) / reps
+    mg.close()
+    res = {"workload": f"{args.grid_places} places near the fibers of the 1000-fiber system, range 0.05",
+           "ms_per_query": round(1e3 * gdt, 3), "places_per_s": round(args.grid_places / gdt, 1),
+           "ms_per_query_noalloc": round(1e3 * ndt, 3), "segments_found": int(counts.sum()),
+           "noalloc_same_counts": bool(np.array_equal(counts, counts2)),
+           "note": "one cymg_nearby_segments call, host arrays in and out included"}
+    log(f"grid {res}")
+    return res
+
+
+def cpu_baseline(args, sc, gpu_iters):
+    from oracle import pyoracle
+    log(f"cpu baseline, {args.cpu_steps} oracle steps")
+    orc = pyoracle.from_scene(sc)
+    its = []
+    t0 = time.perf_counter()
+    for s in range(args.cpu_steps):
+        if s > 0:
+            orc.prepare(orc.fiber_points())
+            pyoracle.assemble_scene(orc, sc)
+        rep = orc.solve(args.tol, sc.gauss(s % 8), tol_mode=1)
+        its.append(rep["iterations"])
+    cdt = time.perf_counter() - t0
+    model, nproc, usable = cpu_description()
+    k = min(len(its), len(gpu_iters))
+    return ({"value": args.cpu_steps / cdt, "unit": "implicit steps/s", "cores": 1, "kind": "port",
+             "cpu_model": model, "nproc": nproc, "usable_cpus": usable,
+             "sample": f"steps 0-{args.cpu_steps - 1} of the same 1000-fiber workload (same draws as the FP64 GPU "
+                       f"mode), FP64 block-LU BCGSP, |r|inf/|b|inf<{args.tol:g}, mean {np.mean(its):.1f} iterations, "
+                       f"single thread; assembly of the 20000 couples in one C call per step"},
+            {"steps": k, "gpu_fp64": [int(x) for x in gpu_iters[:k]], "oracle_fp64": [int(x) for x in its[:k]]})
 
 
 def main():
@@ -166,190 +500,25 @@ def main():
     pkg = __graft_entry__.load_package()
     import importlib
     scenes = importlib.import_module(__graft_entry__.PKG_NAME + ".scene")
-    print(f"bench: generating scene nfib={args.nfib} nlinks={args.nlinks}", file=sys.stderr, flush=True)
-    sc = scenes.config2(seed=2, nfib=args.nfib, nlinks=args.nlinks)
-    nsteps_total = args.warmup + args.steps
-    gauss_dev = [torch.from_numpy(sc.gauss(s)).to("cuda:0") for s in range(min(nsteps_total, 8))]
-    torch.cuda.synchronize()
+    if args.only == "batched":
+        batched(args, pkg, scenes, torch)
+        return
+    if args.only == "asters":
+        asters_workload(args, pkg, scenes, torch)
+        return
+    if args.only == "grid":
+        grid_query(args, pkg, scenes)
+        return
 
-    results = {}
-    for mode in args.modes.split(","):
-        precond = {"fp64": pkg.PRECOND_FP64, "fp32": pkg.PRECOND_FP32, "learned-fp32": pkg.PRECOND_FP32}[mode]
-        learned = mode.startswith("learned")
-        print(f"bench: mode {mode} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
-        m, dt, iters, info = run_mode(pkg, sc, precond, args.steps, args.warmup, args.tol, gauss_dev, torch,
-                                      guess=pkg.GUESS_LEARNED if learned else pkg.GUESS_ZERO,
-                                      train=(lambda mm: train_guess(pkg, mm, sc, args, args.tol, torch))
-                                      if learned else None, autotune=bool(args.autotune))
-        results[mode] = dict(seconds=dt, steps_per_s=args.steps / dt, ms_per_step=1e3 * dt / args.steps,
-                             mean_iters=float(np.mean(iters)), meca=m, training=info)
-        print(f"bench: {mode} {args.steps / dt:.2f} steps/s, mean iterations {np.mean(iters):.1f}"
-              + (f", training {info}" if info else ""), file=sys.stderr, flush=True)
-        if mode != "fp64":
-            m.close()
-
-    # dominant-kernel timing on the final state of the FP64 run
-    m = results["fp64"]["meca"]
-    m.setup(gauss_device_ptr=gauss_dev[0].data_ptr())
-    mv_ms = m.time_kernel(0, reps=200)
-    m.compute_preconditioner(pkg.PRECOND_FP64)
-    pc_ms = m.time_kernel(1, reps=200)
-    build_ms = m.time_kernel(2, reps=5)
-    build_only_ms = m.time_kernel(4, reps=5)
-    build_lu_ms = m.time_kernel(5, reps=5)
-    rB, cB, vB = m.matrix_entries(0)
-    rC, cC, vC = m.matrix_entries(1)
-    kb_full = 2 * int((rB != cB).sum()) + int((rB == cB).sum())
-    kc_full = 2 * int((rC != cC).sum()) + int((rC == cC).sum())
-    mv_bytes = algorithmic_bytes_matvec(sc, kb_full, kc_full)
-    mv_achieved = mv_bytes / (mv_ms * 1e-3) / 1e9
-    nblk = int(((sc.dim * sc.nb_points.astype(np.int64)) ** 2).sum())
-    m.compute_preconditioner(pkg.PRECOND_FP64)
-    it_ms = m.time_kernel(6, reps=args.timed_launches)
-    it_bytes = algorithmic_bytes_iteration_launch(sc, kb_full, kc_full, 8 * nblk)
+    sc, results, training, kern, (it_ms, it_bytes) = c2(args, pkg, scenes, torch)
     achieved = it_bytes / (it_ms * 1e-3) / 1e9
-    it32_ms = it32_achieved = None
-    if "fp32" in results:
-        m.compute_preconditioner(pkg.PRECOND_FP32)
-        it32_ms = m.time_kernel(6, reps=args.timed_launches)
-        it32_achieved = algorithmic_bytes_iteration_launch(sc, kb_full, kc_full, 4 * nblk) / (it32_ms * 1e-3) / 1e9
-
-    if args.copy_calibration_mb > 0:
-        nb = args.copy_calibration_mb << 20
-        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda:0")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-
-    batched = None
-    if args.batched_steps > 0:
-        print("bench: generating 64 independent systems", file=sys.stderr, flush=True)
-        batch = scenes.config5(seed=5)
-        big = batch.scene
-        gb = [torch.from_numpy(big.gauss(s)).to("cuda:0") for s in range(2)]
-        mb = pkg.Meca(3, 0)
-        pkg.load_batch(mb, batch)
-        its = []
-        btune = None
-        for s in range(1 + args.batched_steps):
-            if s == 1:
-                if args.autotune:
-                    # tile shape and block-inverse method for this workload (untimed)
-                    btune = mb.autotune(reps=6)
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-            if s > 0:
-                mb.prepare(None)
-                pkg.assemble(mb, big, upload_couples=False)
-            rep = mb.solve(args.tol, tol_mode=pkg.TOL_RELATIVE, precondition=pkg.PRECOND_FP64,
-                           gauss_device_ptr=gb[s % 2].data_ptr())
-            if s > 0:
-                its.append(rep["iterations"])
-        torch.cuda.synchronize()
-        bdt = time.perf_counter() - t0
-        # full-work iteration launch over all 64 systems (every tile iterating)
-        mb.setup(gauss_device_ptr=gb[0].data_ptr())
-        mb.compute_preconditioner(pkg.PRECOND_FP64)
-        b_ms = mb.time_kernel(6, reps=10)
-        rB, cB, _ = mb.matrix_entries(0)
-        rC, cC, _ = mb.matrix_entries(1)
-        b_kb = 2 * int((rB != cB).sum()) + int((rB == cB).sum())
-        b_kc = 2 * int((rC != cC).sum()) + int((rC == cC).sum())
-        del rB, cB, rC, cC
-        b_blk = int(((big.dim * big.nb_points.astype(np.int64)) ** 2).sum())
-        b_bytes = algorithmic_bytes_iteration_launch(big, b_kb, b_kc, 8 * b_blk)
-        b_gbs = b_bytes / (b_ms * 1e-3) / 1e9
-        batched = {"workload": f"batch-64-systems-{big.nfib}-fibers-{big.npts}-points-{big.nlinks}-links",
-                   "systems": batch.nsys, "fibers": big.nfib, "unknowns": big.size,
-                   "steps_per_s": round(args.batched_steps / bdt, 3),
-                   "system_steps_per_s": round(batch.nsys * args.batched_steps / bdt, 2),
-                   "max_bicgstab_iters": float(np.mean(its)),
-                   "untimed_setup": {"autotune": btune},
-                   "iteration_launch": {"avg_ms": round(b_ms, 4), "bytes_per_launch": int(b_bytes),
-                                        "achieved_gbs": round(b_gbs, 1), "frac_of_hbm_peak": round(b_gbs / 8000.0, 4)}}
-        print(f"bench: batched {batched}", file=sys.stderr, flush=True)
-        mb.close()
-
-    asters = None
-    if args.aster_steps > 0:
-        print("bench: generating the aster scene", file=sys.stderr, flush=True)
-        ast = scenes.config3(seed=3)
-        ga = [torch.from_numpy(ast.gauss(s)).to("cuda:0") for s in range(2)]
-        ma = pkg.Meca(3, 0)
-        pkg.load_scene(ma, ast)
-        its = []
-        atune = None
-        for s in range(2 + args.aster_steps):
-            if s == 2:
-                if args.autotune:
-                    atune = ma.autotune(reps=6)      # untimed
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-            if s > 0:
-                ma.prepare(None)
-                pkg.assemble(ma, ast, upload_couples=False)
-            rep = ma.solve(args.tol, tol_mode=pkg.TOL_RELATIVE, precondition=pkg.PRECOND_FP64,
-                           gauss_device_ptr=ga[s % 2].data_ptr())
-            # the motors walk under their loads between solves (Couple::stepAA)
-            ma.step_couples(ast.motor_max_speed, ast.motor_stall_force)
-            if s >= 2:
-                its.append(rep["iterations"])
-        torch.cuda.synchronize()
-        adt = time.perf_counter() - t0
-        _, _, cst = ma.get_couples()
-        asters = {"workload": f"asters-10x200-fibers-{ast.nfib}-solids-{ast.nsol}-points-{ast.npts}-motors-{ast.nlinks}",
-                  "unknowns": ast.size, "steps_per_s": round(args.aster_steps / adt, 3),
-                  "ms_per_step": round(1e3 * adt / args.aster_steps, 3),
-                  "mean_bicgstab_iters": float(np.mean(its)),
-                  "iteration": "two launches per iteration (a tile per Solid)",
-                  "untimed_setup": {"autotune": atune},
-                  "motors": f"stepped on the device between solves (speed {ast.motor_max_speed} um/s, "
-                            f"stall {ast.motor_stall_force} pN); {int(((cst & 3) == 3).sum())} of "
-                            f"{cst.size} couples still bridging"}
-        print(f"bench: asters {asters}", file=sys.stderr, flush=True)
-        ma.close()
-
-    grid = None
-    if args.grid_places > 0:
-        # FiberGrid query for many places at once (cymg_nearby_segments) on the
-        # 1000-fiber system: places near the fibers, tryToAttach's range test;
-        # host arrays in and out are included in the time
-        mg = pkg.Meca(3, 0)
-        pkg.load_scene(mg, sc)
-        rng = np.random.default_rng(11)
-        P = sc.points.reshape(-1, 3)
-        W = P[rng.integers(0, P.shape[0], args.grid_places)] + rng.uniform(-0.1, 0.1, (args.grid_places, 3))
-        mg.nearby_segments(W, 0.05, inclusive=True)          # untimed: buffers
-        reps = 5
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            counts, out = mg.nearby_segments(W, 0.05, inclusive=True)
-        gdt = (time.perf_counter() - t0) / reps
-        grid = {"workload": f"{args.grid_places} places near the fibers of the 1000-fiber system, range 0.05",
-                "ms_per_query": round(1e3 * gdt, 3), "places_per_s": round(args.grid_places / gdt, 1),
-                "segments_found": int(counts.sum()),
-                "note": "one cymg_nearby_segments call, host arrays in and out included"}
-        print(f"bench: grid {grid}", file=sys.stderr, flush=True)
-        mg.close()
-
-    cpu = None
+    full = args.only == "all"
+    bat = batched(args, pkg, scenes, torch) if full and args.batched_steps > 0 else None
+    ast = asters_workload(args, pkg, scenes, torch) if full and args.aster_steps > 0 else None
+    grid = grid_query(args, pkg, scenes, sc) if full and args.grid_places > 0 else None
+    cpu = same = None
     if args.cpu_steps > 0:
-        from oracle import pyoracle
-        print(f"bench: cpu baseline, {args.cpu_steps} oracle steps", file=sys.stderr, flush=True)
-        orc = pyoracle.from_scene(sc)
-        t0 = time.perf_counter()
-        its = []
-        for s in range(args.cpu_steps):
-            rep = orc.solve(args.tol, sc.gauss(s), tol_mode=1)
-            its.append(rep["iterations"])
-            orc.prepare(orc.fiber_points())
-            pyoracle.assemble_scene(orc, sc)
-        cdt = time.perf_counter() - t0
-        cpu = {"value": args.cpu_steps / cdt, "unit": "implicit steps/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the same 1000-fiber workload, FP64 block-LU BCGSP, "
-                         f"mean {np.mean(its):.1f} iterations, single thread"}
+        cpu, same = cpu_baseline(args, sc, results["fp64"]["iters"])
 
     traffic = args.traffic
     if traffic is None:
@@ -382,21 +551,26 @@ def main():
                    "tolerance": f"|r|inf/|b|inf<{args.tol:g}"},
         "modes": {k: {"steps_per_s": round(v["steps_per_s"], 3), "ms_per_step": round(v["ms_per_step"], 3),
                       "mean_bicgstab_iters": round(v["mean_iters"], 2),
-                      **({"untimed_setup": v["training"]} if v["training"] else {})} for k, v in results.items()},
+                      "first_step_error_vs_fp64_oracle": v["error"],
+                      "running_step_error_vs_fp64_oracle": v["error_step"],
+                      **({"untimed_autotune": v["autotune"]} if v["autotune"] else {})}
+                  for k, v in results.items()},
+        "error_reference": "oracle FP64 step solved to |r|inf/|b|inf<1e-12 on the same input: the first step "
+                           "(a relaxation of the generated configuration, points outside the sphere pulled in) "
+                           "and the last warmup step (from the mode's own points); position_rel = "
+                           "max|dpos|/max|pos|, displacement_rel = max|dx|/max|x| of the step's solution",
+        "learned_guess_training": training,
         "learned_mixed_vs_plain_fp64": (round(results["learned-fp32"]["steps_per_s"] / best["steps_per_s"], 3)
                                         if "learned-fp32" in results else None),
         "roofline": {"bound": "hbm", "kernel": "k_tp/k_tq (one launch of the FP64 BiCGStab iteration)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": int(it_bytes), "avg_launch_ms": round(it_ms, 5)},
-        "kernels_ms": {"iteration_launch_fp64": it_ms, "iteration_launch_fp32": it32_ms,
-                       "iteration_launch_fp32_gbs": it32_achieved and round(it32_achieved, 1),
-                       "matvec": mv_ms, "matvec_gbs": round(mv_achieved, 1), "precond_apply": pc_ms,
-                       "precond_build_fp64": build_ms, "precond_assemble_only": build_only_ms,
-                       "precond_build_getrf_getri": build_lu_ms},
+        "kernels": kern,
         "cpu_baseline": cpu,
-        "batched": batched,
-        "asters": asters,
+        "iterations_same_steps": same,
+        "batched": bat,
+        "asters": ast,
         "nearby_segments": grid,
     }
     print(json.dumps(line), flush=True)

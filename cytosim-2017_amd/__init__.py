@@ -36,12 +36,12 @@ EXPORTED = [
     "cymg_setup", "cymg_iterate", "cymg_apply", "cymg_multiply", "cymg_compute_preconditioner",
     "cymg_precondition", "cymg_compute_forces", "cymg_multiply_device", "cymg_precondition_device",
     "cymg_get_vector", "cymg_get_points", "cymg_get_tensions", "cymg_get_block", "cymg_get_matrix",
-    "cymg_get_timings", "cymg_enable_timing", "cymg_time_kernel", "cymg_set_option", "cymg_autotune",
+    "cymg_get_timings", "cymg_enable_timing", "cymg_time_kernel", "cymg_set_option", "cymg_get_option", "cymg_autotune",
     "cymg_guess_init", "cymg_guess_weights", "cymg_guess_record", "cymg_guess_samples",
     "cymg_guess_train", "cymg_guess_check", "cymg_guess_features", "cymg_guess_predict",
 ]
-OPT_KRYLOV_LAUNCHES, OPT_BLOCK_INVERSE, OPT_TILE_POINTS, OPT_GRAPHS = 1, 2, 3, 4
-GUESS_INPUTS, GUESS_OUTPUTS, GUESS_POINTS, GUESS_WEIGHTS = 288, 96, 31, 230240
+OPT_KRYLOV_LAUNCHES, OPT_BLOCK_INVERSE, OPT_TILE_POINTS, OPT_GRAPHS, OPT_TILE_THREADS = 1, 2, 3, 4, 5
+GUESS_INPUTS, GUESS_OUTPUTS, GUESS_POINTS, GUESS_WEIGHTS = 384, 96, 31, 254816
 
 
 class SimParams(C.Structure):
@@ -175,6 +175,7 @@ def lib():
         "cymg_enable_timing": (C.c_int, [vp, C.c_int]),
         "cymg_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "cymg_set_option": (C.c_int, [vp, C.c_int, C.c_int]),
+        "cymg_get_option": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
         "cymg_autotune": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]),
         "cymg_guess_init": (C.c_int, [vp, C.c_uint64]),
         "cymg_guess_weights": (C.c_int, [vp, C.c_int, vp]),
@@ -384,7 +385,7 @@ class Meca:
         mp = MotorProps(float(max_speed), float(stall_force), int(bool(limit_speed)), int(bool(hold_growing_end)))
         self._chk(self.L.cymg_step_couples(self.h, C.byref(mp)))
 
-    def nearby_segments(self, places, range_, inclusive=False, exclude=None):
+    def nearby_segments(self, places, range_, inclusive=False, exclude=None, noalloc=False):
         """FiberGrid::nearbySegments for many places (fiber_grid.cc:302-337):
         counts per place and the concatenated (fiber, point, abscissa,
         dist_sqr) of the kept segments, place after place, each in fiber
@@ -399,8 +400,9 @@ class Meca:
         for _ in range(2):
             f, p = np.zeros(max(cap, 1), np.uint32), np.zeros(max(cap, 1), np.uint32)
             a, d = np.zeros(max(cap, 1)), np.zeros(max(cap, 1))
-            self._chk(self.L.cymg_nearby_segments(self.h, nq, _p(w), _p(ex), float(range_), int(bool(inclusive)),
-                                                  _p(counts), cap, _p(f), _p(p), _p(a), _p(d), C.byref(total)))
+            fn = self.L.cymg_nearby_segments_noalloc if noalloc else self.L.cymg_nearby_segments
+            self._chk(fn(self.h, nq, _p(w), _p(ex), float(range_), int(bool(inclusive)), _p(counts), cap, _p(f),
+                         _p(p), _p(a), _p(d), C.byref(total)))
             if total.value <= cap:
                 out = (f[:total.value], p[:total.value], a[:total.value], d[:total.value])
                 break
@@ -490,6 +492,15 @@ class Meca:
         self._chk(self.L.cymg_get_points(self.h, _p(out)))
         return out
 
+    def export_points(self, out):
+        """getPoints of every Mecable into a preallocated float64 array."""
+        assert out.dtype == np.float64 and out.size == self.size and out.flags.c_contiguous
+        self._chk(self.L.cymg_get_points(self.h, _p(out)))
+
+    def export_vector(self, which, out):
+        assert out.dtype == np.float64 and out.size == self.size and out.flags.c_contiguous
+        self._chk(self.L.cymg_get_vector(self.h, which, _p(out)))
+
     def tensions(self):
         nfib = len(self.np) - getattr(self, "nsol", 0)
         out = np.empty(sum(self.np[:nfib]) - nfib)
@@ -522,7 +533,13 @@ class Meca:
         current system (after setup); returns the choices."""
         t, b, ms = C.c_int(), C.c_int(), C.c_double()
         self._chk(self.L.cymg_autotune(self.h, int(reps), C.byref(t), C.byref(b), C.byref(ms)))
-        return {"tile_points": t.value, "block_inverse": b.value, "launch_ms": ms.value}
+        return {"tile_points": t.value, "tile_threads": self.get_option(OPT_TILE_THREADS),
+                "block_inverse": b.value, "launch_ms": ms.value}
+
+    def get_option(self, option):
+        v = C.c_int()
+        self._chk(self.L.cymg_get_option(self.h, option, C.byref(v)))
+        return v.value
 
     def time_kernel(self, which, reps=50):
         ms = C.c_double()

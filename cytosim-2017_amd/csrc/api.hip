@@ -106,8 +106,15 @@ void same_system(const Context& c, std::initializer_list<uint32_t> pts)
 void build_tiles(Context& c)
 {
     const std::vector<uint32_t>& sb = c.h_sys_begin_used;
+    // the tile limits of the iteration's workgroup: tiles of at most
+    // threads/2 unknowns give every unknown two threads (the default), larger
+    // ones one thread; a fiber longer than the target gets a tile of its own
+    const int threads = (c.tile_threads == 256 && c.nsol == 0 &&
+                         c.largest <= (uint32_t)tile_max_points(c, 256)) ? 256 : 512;
+    const uint32_t maxp = (uint32_t)tile_max_points(c, threads);
     uint32_t target = c.tile_points;
-    if (!target) target = std::min<uint32_t>(256 / (uint32_t)c.dim, kMaxFiberPoints);
+    if (!target) target = std::min<uint32_t>((uint32_t)threads / 2 / (uint32_t)c.dim, kMaxFiberPoints);
+    target = std::min(target, maxp);
     c.h_tile_fib.clear();
     c.h_tile_sys.clear();
     c.h_sys_tile.assign(c.nsys + 1, 0);
@@ -1298,8 +1305,25 @@ int cymg_set_option(cymg_context* ctx, int option, int value)
             c.tile_points = (uint32_t)value;
             if (!c.layout_dirty && c.nfib) build_tiles(c);
         }
+        else if (option == CYMG_OPT_TILE_THREADS && (value == 256 || value == 512)) {
+            c.tile_threads = (uint32_t)value;
+            if (!c.layout_dirty && c.nfib) build_tiles(c);
+        }
         else throw Error(CYMG_ERR_INVALID, "unknown option or value");
         c.last_iters[0] = c.last_iters[1] = c.last_iters[2] = 0;
+    });
+}
+
+int cymg_get_option(cymg_context* ctx, int option, int* value)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!value) throw Error(CYMG_ERR_INVALID, "null value");
+        if (option == CYMG_OPT_KRYLOV_LAUNCHES) *value = c.krylov_fused ? 2 : 4;
+        else if (option == CYMG_OPT_BLOCK_INVERSE) *value = c.pc_method;
+        else if (option == CYMG_OPT_GRAPHS) *value = c.use_graphs;
+        else if (option == CYMG_OPT_TILE_POINTS) *value = (int)c.tile_points;
+        else if (option == CYMG_OPT_TILE_THREADS) *value = tile_threads_of(c);
+        else throw Error(CYMG_ERR_INVALID, "unknown option");
     });
 }
 
@@ -1330,18 +1354,23 @@ int cymg_autotune(cymg_context* ctx, int reps, int* tile_points, int* block_inve
         launch_precond_build(c, precision);
         // tile shape of the two-launch iteration: the target points per tile
         // with the shortest launch on the current system
-        int best_tile = (int)c.tile_points;
+        int best_tile = (int)c.tile_points, best_threads = (int)c.tile_threads;
         double best = 1e300;
         if (!c.nsol || c.solid_tiles) {
-            const int cands[] = { 0, 16, 24, 32, 48, 64, 96, 128, 170 };
-            for (int t : cands) {
-                if (t > kTileMaxPoints) continue;
-                c.tile_points = (uint32_t)t;
-                build_tiles(c);
-                const double ms = krylov_time_launch(c, precision, reps);
-                if (ms < best) { best = ms; best_tile = t; }
+            const int cands[] = { 0, 16, 24, 32, 48, 64, 85, 96, 128, 170 };
+            for (int nt : { 512, 256 }) {
+                if (nt == 256 && (c.nsol || c.largest > (uint32_t)tile_max_points(c, 256))) continue;
+                for (int t : cands) {
+                    if (t > tile_max_points(c, nt)) continue;
+                    c.tile_threads = (uint32_t)nt;
+                    c.tile_points = (uint32_t)t;
+                    build_tiles(c);
+                    const double ms = krylov_time_launch(c, precision, reps);
+                    if (ms < best) { best = ms; best_tile = t; best_threads = nt; }
+                }
             }
             c.tile_points = (uint32_t)best_tile;
+            c.tile_threads = (uint32_t)best_threads;
             build_tiles(c);
         }
         c.last_iters[0] = c.last_iters[1] = c.last_iters[2] = 0;

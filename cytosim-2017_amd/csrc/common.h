@@ -162,6 +162,8 @@ void launch_solid_pc_apply(Context& c, const double* X, double* Y);
 void launch_solid_points(Context& c);
 // launchers (krylov.hip)
 double krylov_time_launch(Context& c, int precond, int launches);
+int tile_threads_of(const Context& c);
+int tile_max_points(const Context& c, int threads);
 void nearby_segments(Context& c, uint32_t nq, const double* places, const uint32_t* exclude, double range,
                      int inclusive, uint32_t* counts, uint32_t capacity, uint32_t* fiber, uint32_t* point,
                      double* abscissa, double* dist_sqr, uint32_t* total);

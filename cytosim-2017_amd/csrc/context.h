@@ -45,6 +45,7 @@ struct Context {
     std::vector<uint32_t> h_tile_fib, h_tile_sys, h_sys_tile;
     uint32_t ntiles = 0;
     uint32_t tile_points = 0;          // target points per tile, 0: chosen from the layout
+    uint32_t tile_threads = 512;       // workgroup of the two-launch iteration (256: fiber-only layouts)
 
     // Solids, registered after every fiber (simul_solve.cc:204-207)
     uint32_t nsol = 0;

@@ -867,7 +867,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tinit(KArgs k, const double* _
 }
 
 // ---------------------------------------------------------------------------
-// P and Q run 512 threads per tile.  A tile of at most kTpSplit unknowns
+// P and Q run NT threads per tile: 512, or 256 on layouts without Solid
+// tiles, whose smaller workgroup and LDS let two tiles share a compute unit so
+// that one tile's loads overlap the other's dependent phases (the long,
+// HBM-bound launches of many tiles).  A tile of at most NT/2 unknowns
 // gives each row two threads (h = 0 owns the row, h = 1 helps): each takes
 // half of the row's mB and mC entries and half of the columns of its row of
 // the block inverse.  Everything that does not depend on the iteration
@@ -876,23 +879,42 @@ __global__ __launch_bounds__(kTileThreads) void k_tinit(KArgs k, const double* _
 // (the operand of another tile is a combination of vectors the launch does
 // not write), P' and rfDiff constants; after the first barrier the block
 // inverse columns and the segment constants.  A longer tile (one fiber of
-// more than kTpSplit / DIM points) gives each row one thread.
-constexpr int kTpThreads = 512;
-constexpr int kTpWaves = kTpThreads / kWave;
-constexpr int kTpSplit = 256;
+// more than NT/2 / DIM points) gives each row one thread.
+constexpr int kTpThreads = 512;        // the workgroup of layouts with Solid tiles
 constexpr int kTpPreB = 6;      // mB entries per thread in registers
 constexpr int kTpPreC = 3;      // mC entries per thread in registers
-constexpr int kTpBlk = 48;      // block-inverse columns per thread in registers
+// 256-thread tiles prefetch less per thread (8 block columns, 2 mB and 1 mC
+// entries): at most 128 VGPRs, so four tiles share a compute unit and one
+// tile's loads overlap the others' dependent phases.  On the 64-system
+// workload this takes a full-work launch from 647 to 522 us (FP64) and from
+// 595 to 394 us (FP32); 16/3/2 and 16/2/1 (3 tiles per compute unit), or
+// batches of 16 or 8 columns, measured slower.
+constexpr int kTp256Blk = 8, kTp256PreB = 2, kTp256PreC = 1;
 
-template <int D>
+template <int NT>
+struct TpCfg {
+    static constexpr int waves = NT / kWave;
+    static constexpr int split = NT / 2;                  // unknowns a split tile may hold
+    static constexpr int blk = NT == 512 ? 48 : kTp256Blk;   // block-inverse columns per thread in registers
+    static constexpr int preb = NT == 512 ? kTpPreB : kTp256PreB;
+    static constexpr int prec = NT == 512 ? kTpPreC : kTp256PreC;
+    static constexpr int chunk = kApplyChunk;                 // block columns per load batch beyond blk
+    static __host__ __device__ constexpr int max_points(int D)
+    {
+        return NT / D < kTileMaxPoints ? NT / D : kTileMaxPoints;
+    }
+};
+
+template <int D, int NT>
 struct TpLds {
-    double x[kTileMaxPoints * D];      // operand slice
-    double f[kTileMaxPoints * D];      // forces
-    double y[kTileMaxPoints * D];      // operator result
-    double hb[kTileMaxPoints * D];     // helper's mB partial sum, then its block partial sum
-    double hc[kTileMaxPoints * D];     // helper's mC partial sum
-    double lam[kTileMaxPoints];
-    double pa[kTileMaxPoints], pb[kTileMaxPoints], pc[kTileMaxPoints];
+    static constexpr int MP = TpCfg<NT>::max_points(D);
+    double x[MP * D];      // operand slice
+    double f[MP * D];      // forces
+    double y[MP * D];      // operator result
+    double hb[MP * D];     // helper's mB partial sum, then its block partial sum
+    double hc[MP * D];     // helper's mC partial sum
+    double lam[MP];
+    double pa[MP], pb[MP], pc[MP];
     double fk[kTileMaxFibers];
     double fsca[kTileMaxFibers];
     uint64_t fblk[kTileMaxFibers];
@@ -901,7 +923,7 @@ struct TpLds {
     int fuse[kTileMaxFibers];
     int maxS;
     double bc[5];
-    double wr[5 * kTpWaves];
+    double wr[5 * TpCfg<NT>::waves];
 };
 
 // the thread's share of a row: entries/columns [b, e) of a range of n,
@@ -1008,7 +1030,7 @@ struct ShatComb {
 };
 
 // Row state of one thread for the operator and the block apply.
-template <int D, typename TB>
+template <int D, typename TB, int NB>
 struct TpRow {
     int u;             // row in the tile
     bool act, own, split;
@@ -1020,7 +1042,7 @@ struct TpRow {
     uint32_t pt, f;
     // block inverse share
     uint32_t j0, j1;
-    TB blk[kTpBlk];
+    TB blk[NB];
     // segment constants (thread s < ts), masked at the fiber ends after the tables
     double sdf[D], sjd, sja, sjc;
     int sk, sp;
@@ -1028,8 +1050,8 @@ struct TpRow {
 
 // the thread's segment (s = threadIdx.x < ts): rfDiff and the JJ' entries of
 // global segment S0 + s and its predecessor, loaded before the tables are known
-template <int D, typename TB>
-__device__ __forceinline__ void tp_seg_constants(const FiberArgs& a, const Tile& T, TpRow<D, TB>& R)
+template <int D, typename TB, int NB>
+__device__ __forceinline__ void tp_seg_constants(const FiberArgs& a, const Tile& T, TpRow<D, TB, NB>& R)
 {
 #pragma unroll
     for (int d = 0; d < D; ++d) R.sdf[d] = 0.0;
@@ -1046,21 +1068,22 @@ __device__ __forceinline__ void tp_seg_constants(const FiberArgs& a, const Tile&
     }
 }
 
-template <int D>
+template <int D, int NT>
 __device__ __forceinline__ void tp_roles(const Tile& T, int& u, bool& act, bool& own, bool& split, int& h)
 {
-    split = T.nu <= (uint32_t)kTpSplit;
+    constexpr int SP = TpCfg<NT>::split;
+    split = T.nu <= (uint32_t)SP;
     const int tid = threadIdx.x;
-    u = split ? (tid & (kTpSplit - 1)) : tid;
-    h = split ? (tid >> 8) : 0;
+    u = split ? (tid & (SP - 1)) : tid;
+    h = split ? (tid / SP) : 0;
     act = u < (int)T.nu;
     own = act && h == 0;
 }
 
 // P' and rfDiff of the segments after and before the owner's point, by
 // global index (masked later with the fiber's extent)
-template <int D, typename TB>
-__device__ __forceinline__ void tp_row_constants(const FiberArgs& a, const Tile& T, TpRow<D, TB>& R)
+template <int D, typename TB, int NB>
+__device__ __forceinline__ void tp_row_constants(const FiberArgs& a, const Tile& T, TpRow<D, TB, NB>& R)
 {
     R.pd0 = R.pd1 = R.df0 = R.df1 = 0.0;
     R.pt = 0;
@@ -1083,14 +1106,14 @@ __device__ __forceinline__ void tp_row_constants(const FiberArgs& a, const Tile&
 
 // after the tables: the row's fiber, the block-inverse share, and the
 // segment's fiber (masking the JJ' couplings at the fiber ends)
-template <int D, typename TB, typename LDS>
+template <int D, typename TB, int NB, typename LDS>
 __device__ __forceinline__ void tp_after_tables(const FiberArgs& a, const LDS& L, const Tile& T, const TB* B,
-                                                TpRow<D, TB>& R)
+                                                TpRow<D, TB, NB>& R)
 {
     R.j0 = R.j1 = 0;
     R.k = R.fo = R.np = R.p = 0;
 #pragma unroll
-    for (int v = 0; v < kTpBlk; ++v) R.blk[v] = TB(0);
+    for (int v = 0; v < NB; ++v) R.blk[v] = TB(0);
     if (R.act) {
         const uint32_t pt = (uint32_t)R.u / D;
         R.k = tile_fiber_of(L.fo, (int)T.nft, pt);
@@ -1101,9 +1124,25 @@ __device__ __forceinline__ void tp_after_tables(const FiberArgs& a, const LDS& L
             const int bs = D * R.np;
             share(0, (uint32_t)bs, R.split, R.h, R.j0, R.j1);
             const TB* col = B + L.fblk[R.k] + (R.u - D * R.fo);
+            if constexpr (std::is_same<TB, float>::value) {
+                // FP32: addresses clamped to the share's last column, entries
+                // beyond it zeroed by a select, so the loads are issued together
+                if (R.j1 > R.j0) {
+                    const uint32_t jl = R.j1 - 1;
 #pragma unroll
-            for (int v = 0; v < kTpBlk; ++v)
-                if (R.j0 + v < R.j1) R.blk[v] = __builtin_nontemporal_load(col + (size_t)bs * (R.j0 + v));
+                    for (int v = 0; v < NB; ++v) {
+                        const uint32_t j = R.j0 + v < jl ? R.j0 + v : jl;
+                        const TB e = __builtin_nontemporal_load(col + (size_t)bs * j);
+                        R.blk[v] = R.j0 + v < R.j1 ? e : TB(0);
+                    }
+                }
+            } else {
+                // FP64: predicated (the branch-free form holds twice the
+                // registers live and measures slower)
+#pragma unroll
+                for (int v = 0; v < NB; ++v)
+                    if (R.j0 + v < R.j1) R.blk[v] = __builtin_nontemporal_load(col + (size_t)bs * (R.j0 + v));
+            }
         }
     }
     R.sk = R.sp = 0;
@@ -1122,8 +1161,9 @@ __device__ __forceinline__ void tp_after_tables(const FiberArgs& a, const LDS& L
 // The operator over the tile once L.x holds the operand and the helpers'
 // coupling sums are in L.hb/L.hc: forces, multipliers, L.y.  bsum/csum: the
 // owner's own shares.  Ends synchronised.
-template <int D, typename TB, typename LDS>
-__device__ __forceinline__ void tp_operator(LDS& L, const Tile& T, const TpRow<D, TB>& R, double bsum, double csum)
+template <int D, typename TB, int NB, typename LDS>
+__device__ __forceinline__ void tp_operator(LDS& L, const Tile& T, const TpRow<D, TB, NB>& R, double bsum,
+                                            double csum)
 {
     if (R.own) {
         const int k = R.k;
@@ -1198,41 +1238,47 @@ __device__ __forceinline__ void tp_operator(LDS& L, const Tile& T, const TpRow<D
 
 // out[u] = Binv xin over the tile: each thread its column share, the helper's
 // partial through LDS; copies where there is no usable block.  Ends synchronised.
-template <int D, typename TB, typename LDS>
-__device__ __forceinline__ double tp_block_apply(LDS& L, const Tile& T, const TpRow<D, TB>& R, const TB* B,
+template <int D, typename TB, int NB, int CH = kApplyChunk, typename LDS>
+__device__ __forceinline__ double tp_block_apply(LDS& L, const Tile& T, const TpRow<D, TB, NB>& R, const TB* B,
                                                  const double* xin)
 {
     double acc[4] = { 0, 0, 0, 0 };
     const int k = R.k, fo = R.fo, bs = D * R.np;
     const bool use = R.act && B != nullptr && L.fuse[k];
     if (use && R.j1 > R.j0) {
-        // FP64 entries: branch-free, every operand read stays inside the
+        // Branch-free: every operand read stays inside the
         // fiber's rows (the index is clamped to the last column of this
         // thread's share) and is multiplied by a zero entry beyond it, so the
         // LDS reads of the loop are in flight together instead of one per
         // branch (16.1 instead of 17.5 us per launch on the 1000-fiber system).
         const double* xs = xin + D * fo;
         const uint32_t jl = R.j1 - 1;
-        if constexpr (std::is_same<TB, double>::value) {
 #pragma unroll
-            for (int v = 0; v < kTpBlk; ++v) {
-                const uint32_t j = R.j0 + v < jl ? R.j0 + v : jl;
-                acc[v & 3] += R.blk[v] * xs[j];
-            }
-        } else {
-            // FP32 entries: the predicated form measures faster here
-#pragma unroll
-            for (int v = 0; v < kTpBlk; ++v)
-                if (R.j0 + v < R.j1) acc[v & 3] += (double)R.blk[v] * xs[R.j0 + v];
+        for (int v = 0; v < NB; ++v) {
+            const uint32_t j = R.j0 + v < jl ? R.j0 + v : jl;
+            acc[v & 3] += (double)R.blk[v] * xs[j];
         }
         const TB* col = B + L.fblk[k] + (R.u - D * fo);
-        for (uint32_t j = R.j0 + kTpBlk; j < R.j1; j += kApplyChunk) {
-            TB c[kApplyChunk];
+        for (uint32_t j = R.j0 + NB; j < R.j1; j += CH) {
+            TB c[CH];
+            if constexpr (std::is_same<TB, float>::value) {
+                // FP32: clamped addresses and a select (no branch per load), the
+                // chunk's loads in flight together (a predicated load per column
+                // cost FP32 blocks 40-60% of the launch on tiles of one thread
+                // per row)
 #pragma unroll
-            for (int v = 0; v < kApplyChunk; ++v)
-                c[v] = (j + v < R.j1) ? __builtin_nontemporal_load(col + (size_t)bs * (j + v)) : TB(0);
+                for (int v = 0; v < CH; ++v) {
+                    const uint32_t jj = j + v < jl ? j + v : jl;
+                    const TB e = __builtin_nontemporal_load(col + (size_t)bs * jj);
+                    c[v] = j + v < R.j1 ? e : TB(0);
+                }
+            } else {
 #pragma unroll
-            for (int v = 0; v < kApplyChunk; ++v) {
+                for (int v = 0; v < CH; ++v)
+                    c[v] = (j + v < R.j1) ? __builtin_nontemporal_load(col + (size_t)bs * (j + v)) : TB(0);
+            }
+#pragma unroll
+            for (int v = 0; v < CH; ++v) {
                 const uint32_t jj = j + v < jl ? j + v : jl;
                 acc[v & 3] += (double)c[v] * xs[jj];
             }
@@ -1513,21 +1559,24 @@ __device__ void solid_tile_q(const KArgs& k, const Tile& T, SolidTileLds& S)
     }
 }
 
-template <int D, int PC, bool FIRST>
-__global__ __launch_bounds__(kTpThreads) void k_tp(KArgs k)
+template <int D, int PC, bool FIRST, int NT>
+__global__ __launch_bounds__(NT) void k_tp(KArgs k)
 {
     using TB = typename PcBlocks<PC>::T;
     using Rec = typename std::conditional<FIRST, double2, double4>::type;
-    __shared__ TpLds<D> L;
-    __shared__ SolidTileLds SL;
+    constexpr int NW = TpCfg<NT>::waves;
+    __shared__ TpLds<D, NT> L;
     const FiberArgs& a = k.fa;
     const Tile T = tile_begin<D>(k.ta, blockIdx.x);
-    if (T.nft == 0) {
-        solid_tile_p<D, PC, FIRST>(k, T, SL);
-        return;
+    if constexpr (NT == kTpThreads) {
+        if (T.nft == 0) {
+            __shared__ SolidTileLds SL;
+            solid_tile_p<D, PC, FIRST>(k, T, SL);
+            return;
+        }
     }
-    TpRow<D, TB> R;
-    tp_roles<D>(T, R.u, R.act, R.own, R.split, R.h);
+    TpRow<D, TB, TpCfg<NT>::blk> R;
+    tp_roles<D, NT>(T, R.u, R.act, R.own, R.split, R.h);
     // prologue: everything independent of the iteration scalars
     const uint32_t g = T.U0 + R.u;
     double vr = 0, vt = 0, vx = 0, vrt = 0;
@@ -1541,8 +1590,8 @@ __global__ __launch_bounds__(kTpThreads) void k_tp(KArgs k)
     const Rec* rec;
     if constexpr (FIRST) rec = k.g2;
     else rec = k.g1a;
-    Gath<kTpPreB, Rec> GB;
-    Gath<kTpPreC, Rec> GC;
+    Gath<TpCfg<NT>::preb, Rec> GB;
+    Gath<TpCfg<NT>::prec, Rec> GC;
     GB.e = GB.e1 = GC.e = GC.e1 = 0;
     const int dd = R.u - D * (R.u / D);
     if (R.act && a.Buse) {
@@ -1609,39 +1658,42 @@ __global__ __launch_bounds__(kTpThreads) void k_tp(KArgs k)
     dl = wave_sum(dl);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         L.wr[threadIdx.x / kWave] = rmax;
-        L.wr[kTpWaves + threadIdx.x / kWave] = dl;
+        L.wr[NW + threadIdx.x / kWave] = dl;
     }
-    const double zi = tp_block_apply<D, TB>(L, T, R, PcBlocks<PC>::get(k), L.y);
+    const double zi = tp_block_apply<D, TB, TpCfg<NT>::blk, TpCfg<NT>::chunk>(L, T, R, PcBlocks<PC>::get(k), L.y);
     if (R.own) {
         k.g2[g] = make_double2(yi, zi);
         reinterpret_cast<double2*>(k.g1b + g)[1] = make_double2(phi, zi);
     }
     if (threadIdx.x == 0) {
-        double m = L.wr[0], d = L.wr[kTpWaves];
+        double m = L.wr[0], d = L.wr[NW];
 #pragma unroll
-        for (int q = 1; q < kTpWaves; ++q) {
+        for (int q = 1; q < NW; ++q) {
             m = nan_max(m, L.wr[q]);
-            d += L.wr[kTpWaves + q];
+            d += L.wr[NW + q];
         }
         k.part[P_RMAX * nt + T.t] = m;
         k.part[P_DELTA * nt + T.t] = d;
     }
 }
 
-template <int D, int PC>
-__global__ __launch_bounds__(kTpThreads) void k_tq(KArgs k)
+template <int D, int PC, int NT>
+__global__ __launch_bounds__(NT) void k_tq(KArgs k)
 {
     using TB = typename PcBlocks<PC>::T;
-    __shared__ TpLds<D> L;
-    __shared__ SolidTileLds SL;
+    constexpr int NW = TpCfg<NT>::waves;
+    __shared__ TpLds<D, NT> L;
     const FiberArgs& a = k.fa;
     const Tile T = tile_begin<D>(k.ta, blockIdx.x);
-    if (T.nft == 0) {
-        solid_tile_q<D, PC>(k, T, SL);
-        return;
+    if constexpr (NT == kTpThreads) {
+        if (T.nft == 0) {
+            __shared__ SolidTileLds SL;
+            solid_tile_q<D, PC>(k, T, SL);
+            return;
+        }
     }
-    TpRow<D, TB> R;
-    tp_roles<D>(T, R.u, R.act, R.own, R.split, R.h);
+    TpRow<D, TB, TpCfg<NT>::blk> R;
+    tp_roles<D, NT>(T, R.u, R.act, R.own, R.split, R.h);
     const uint32_t g = T.U0 + R.u;
     double vr = 0, vv = 0, vx = 0, vph = 0, vrt = 0;
     double2 q2 = make_double2(0, 0);
@@ -1651,8 +1703,8 @@ __global__ __launch_bounds__(kTpThreads) void k_tq(KArgs k)
         vph = k.g1b[g].z;
     }
     const double2* rec = k.g2;
-    Gath<kTpPreB, double2> GB;
-    Gath<kTpPreC, double2> GC;
+    Gath<TpCfg<NT>::preb, double2> GB;
+    Gath<TpCfg<NT>::prec, double2> GC;
     GB.e = GB.e1 = GC.e = GC.e1 = 0;
     const int dd = R.u - D * (R.u / D);
     if (R.act && a.Buse) {
@@ -1711,23 +1763,22 @@ __global__ __launch_bounds__(kTpThreads) void k_tq(KArgs k)
     if ((threadIdx.x & (kWave - 1)) == 0) {
         const int wv = threadIdx.x / kWave;
         L.wr[wv] = smax;
-        L.wr[kTpWaves + wv] = tt;
-        L.wr[2 * kTpWaves + wv] = ts;
-        L.wr[3 * kTpWaves + wv] = rts;
-        L.wr[4 * kTpWaves + wv] = rtt;
+        L.wr[NW + wv] = tt;
+        L.wr[2 * NW + wv] = ts;
+        L.wr[3 * NW + wv] = rts;
+        L.wr[4 * NW + wv] = rtt;
     }
-    const double wi = tp_block_apply<D, TB>(L, T, R, PcBlocks<PC>::get(k), L.y);
+    const double wi = tp_block_apply<D, TB, TpCfg<NT>::blk, TpCfg<NT>::chunk>(L, T, R, PcBlocks<PC>::get(k), L.y);
     if (R.own) reinterpret_cast<double2*>(k.g1b + g)[0] = make_double2(shi, wi);
     if (threadIdx.x == 0) {
-        double m = L.wr[0], s0 = L.wr[kTpWaves], s1 = L.wr[2 * kTpWaves], s2 = L.wr[3 * kTpWaves],
-               s3 = L.wr[4 * kTpWaves];
+        double m = L.wr[0], s0 = L.wr[NW], s1 = L.wr[2 * NW], s2 = L.wr[3 * NW], s3 = L.wr[4 * NW];
 #pragma unroll
-        for (int q = 1; q < kTpWaves; ++q) {
+        for (int q = 1; q < NW; ++q) {
             m = nan_max(m, L.wr[q]);
-            s0 += L.wr[kTpWaves + q];
-            s1 += L.wr[2 * kTpWaves + q];
-            s2 += L.wr[3 * kTpWaves + q];
-            s3 += L.wr[4 * kTpWaves + q];
+            s0 += L.wr[NW + q];
+            s1 += L.wr[2 * NW + q];
+            s2 += L.wr[3 * NW + q];
+            s3 += L.wr[4 * NW + q];
         }
         k.part[P_SMAX * nt + T.t] = m;
         *reinterpret_cast<double4*>(k.part4 + 4 * (size_t)T.t) = make_double4(s0, s1, s2, s3);
@@ -1840,17 +1891,18 @@ static void run(Context& c, const double* guess, int max_iter, double tol, int t
 
 // Captures (or reuses) the graphs of 2 and 8 iterations of the two-launch
 // iteration for these launch arguments.  Only kernel launches are captured.
-template <int D, int PC>
+template <int D, int PC, int NT>
 static void iteration_graphs(Context& c, const KArgs& P0, const KArgs& Q0, const KArgs& P1, const KArgs& Q1,
                              dim3 g, dim3 bt)
 {
-    std::vector<unsigned char> key(4 * sizeof(KArgs) + sizeof(dim3));
+    std::vector<unsigned char> key(4 * sizeof(KArgs) + 2 * sizeof(dim3));
     unsigned char* w = key.data();
     for (const KArgs* a : { &P0, &Q0, &P1, &Q1 }) {
         std::memcpy(w, a, sizeof(KArgs));
         w += sizeof(KArgs);
     }
     std::memcpy(w, &g, sizeof(dim3));
+    std::memcpy(w + sizeof(dim3), &bt, sizeof(dim3));
     const int iters[2] = { 2, 8 };
     for (int v = 0; v < 2; ++v) {
         hipGraphExec_t& exec = c.iter_graph[PC][v];
@@ -1862,10 +1914,10 @@ static void iteration_graphs(Context& c, const KArgs& P0, const KArgs& Q0, const
         hipGraph_t graph = nullptr;
         CYMG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
         for (int it = 0; it < iters[v]; it += 2) {
-            hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, P0);
-            hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, Q0);
-            hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, P1);
-            hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, Q1);
+            hipLaunchKernelGGL((k_tp<D, PC, false, NT>), g, bt, 0, c.stream, P0);
+            hipLaunchKernelGGL((k_tq<D, PC, NT>), g, bt, 0, c.stream, Q0);
+            hipLaunchKernelGGL((k_tp<D, PC, false, NT>), g, bt, 0, c.stream, P1);
+            hipLaunchKernelGGL((k_tq<D, PC, NT>), g, bt, 0, c.stream, Q1);
         }
         const hipError_t launch_err = hipGetLastError();
         const hipError_t end_err = hipStreamEndCapture(c.stream, &graph);
@@ -1878,7 +1930,7 @@ static void iteration_graphs(Context& c, const KArgs& P0, const KArgs& Q0, const
     }
 }
 
-template <int D, int PC>
+template <int D, int PC, int NT>
 static void run_tiled(Context& c, const double* guess, int max_iter, double tol, int tol_mode,
                       cymg_solve_report* rep)
 {
@@ -1901,7 +1953,7 @@ static void run_tiled(Context& c, const double* guess, int max_iter, double tol,
     const KArgs kI = with(S0, S1, 0);
     const KArgs kP0 = with(S1, S0, 0), kQ0 = with(S0, S1, 0);
     const KArgs kP1 = with(S1, S0, 1), kQ1 = with(S0, S1, 1);
-    const dim3 g(c.ntiles), b(kTileThreads), bt(kTpThreads);
+    const dim3 g(c.ntiles), b(kTileThreads), bt(NT);
     hipLaunchKernelGGL((k_tinit<D, PC>), g, b, 0, c.stream, kI, guess, tol, tol_mode, max_iter, c.prm.time_step);
     bool first_p = true;
     CYMG_HIP(hipGetLastError());
@@ -1923,15 +1975,15 @@ static void run_tiled(Context& c, const double* guess, int max_iter, double tol,
     hipGraphExec_t* graphs = nullptr;
     if (c.use_graphs) {
         graphs = c.iter_graph[PC];
-        iteration_graphs<D, PC>(c, kP0, kQ0, kP1, kQ1, g, bt);
+        iteration_graphs<D, PC, NT>(c, kP0, kQ0, kP1, kQ1, g, bt);
     }
     auto enqueue = [&](int n, int slot) {
         int it = 0;
         if (first_p) {
-            hipLaunchKernelGGL((k_tp<D, PC, true>), g, bt, 0, c.stream, kP0);
-            hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ0);
-            hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, kP1);
-            hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ1);
+            hipLaunchKernelGGL((k_tp<D, PC, true, NT>), g, bt, 0, c.stream, kP0);
+            hipLaunchKernelGGL((k_tq<D, PC, NT>), g, bt, 0, c.stream, kQ0);
+            hipLaunchKernelGGL((k_tp<D, PC, false, NT>), g, bt, 0, c.stream, kP1);
+            hipLaunchKernelGGL((k_tq<D, PC, NT>), g, bt, 0, c.stream, kQ1);
             first_p = false;
             it = 2;
         }
@@ -1940,10 +1992,10 @@ static void run_tiled(Context& c, const double* guess, int max_iter, double tol,
             for (; it + 2 <= n; it += 2) CYMG_HIP(hipGraphLaunch(graphs[0], c.stream));
         }
         for (; it < n; it += 2) {
-            hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, kP0);
-            hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ0);
-            hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, kP1);
-            hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ1);
+            hipLaunchKernelGGL((k_tp<D, PC, false, NT>), g, bt, 0, c.stream, kP0);
+            hipLaunchKernelGGL((k_tq<D, PC, NT>), g, bt, 0, c.stream, kQ0);
+            hipLaunchKernelGGL((k_tp<D, PC, false, NT>), g, bt, 0, c.stream, kP1);
+            hipLaunchKernelGGL((k_tq<D, PC, NT>), g, bt, 0, c.stream, kQ1);
         }
         CYMG_HIP(hipGetLastError());
         CYMG_HIP(hipMemcpyAsync(snap[slot], S1, sbytes, hipMemcpyDeviceToHost, c.stream));
@@ -1987,7 +2039,7 @@ static void run_tiled(Context& c, const double* guess, int max_iter, double tol,
 // alternately), from events around `pairs` iterations on the context's
 // stream.  The iteration runs with a zero threshold so that every launch does
 // its full work; an iteration that stops early (breakdown) is an error.
-template <int D, int PC>
+template <int D, int PC, int NT>
 static double time_tiled(Context& c, int pairs)
 {
     KArgs k = make_kargs(c);
@@ -2008,17 +2060,17 @@ static double time_tiled(Context& c, int pairs)
     const KArgs kI = with(S0, S1, 0);
     const KArgs kP0 = with(S1, S0, 0), kQ0 = with(S0, S1, 0);
     const KArgs kP1 = with(S1, S0, 1), kQ1 = with(S0, S1, 1);
-    const dim3 g(c.ntiles), b(kTileThreads), bt(kTpThreads);
+    const dim3 g(c.ntiles), b(kTileThreads), bt(NT);
     hipLaunchKernelGGL((k_tinit<D, PC>), g, b, 0, c.stream, kI, nullptr, 0.0, CYMG_TOL_RELATIVE, 1 << 30,
                        c.prm.time_step);
     pairs += pairs & 1;
     CYMG_HIP(hipEventRecord(c.ev[8], c.stream));
     for (int it = 0; it < pairs; it += 2) {
-        if (it == 0) hipLaunchKernelGGL((k_tp<D, PC, true>), g, bt, 0, c.stream, kP0);
-        else hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, kP0);
-        hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ0);
-        hipLaunchKernelGGL((k_tp<D, PC, false>), g, bt, 0, c.stream, kP1);
-        hipLaunchKernelGGL((k_tq<D, PC>), g, bt, 0, c.stream, kQ1);
+        if (it == 0) hipLaunchKernelGGL((k_tp<D, PC, true, NT>), g, bt, 0, c.stream, kP0);
+        else hipLaunchKernelGGL((k_tp<D, PC, false, NT>), g, bt, 0, c.stream, kP0);
+        hipLaunchKernelGGL((k_tq<D, PC, NT>), g, bt, 0, c.stream, kQ0);
+        hipLaunchKernelGGL((k_tp<D, PC, false, NT>), g, bt, 0, c.stream, kP1);
+        hipLaunchKernelGGL((k_tq<D, PC, NT>), g, bt, 0, c.stream, kQ1);
     }
     CYMG_HIP(hipGetLastError());
     CYMG_HIP(hipEventRecord(c.ev[9], c.stream));
@@ -2049,20 +2101,41 @@ static void tiled_buffers(Context& c)
     }
 }
 
+// the workgroup of the two-launch iteration: 256 threads when asked for and
+// every tile fits (no Solid tile, every fiber within 256/DIM points), else 512
+int tile_threads_of(const Context& c)
+{
+    if (c.tile_threads == 256 && !c.solid_tiles && c.nsol == 0 &&
+        c.largest <= (uint32_t)TpCfg<256>::max_points(c.dim))
+        return 256;
+    return 512;
+}
+
+int tile_max_points(const Context& c, int threads)
+{
+    return threads == 256 ? TpCfg<256>::max_points(c.dim) : TpCfg<512>::max_points(c.dim);
+}
+
 double krylov_time_launch(Context& c, int precond, int launches)
 {
     if (c.nsol && !c.solid_tiles)
         throw Error(CYMG_ERR_UNSUPPORTED, "a Solid is too large for the two-launch iteration");
     tiled_buffers(c);
     const int pairs = std::max(1, launches / 2);
-    if (c.dim == 3) {
-        if (precond == CYMG_PRECOND_FP64) return time_tiled<3, CYMG_PRECOND_FP64>(c, pairs);
-        if (precond == CYMG_PRECOND_FP32) return time_tiled<3, CYMG_PRECOND_FP32>(c, pairs);
-        return time_tiled<3, CYMG_PRECOND_NONE>(c, pairs);
-    }
-    if (precond == CYMG_PRECOND_FP64) return time_tiled<2, CYMG_PRECOND_FP64>(c, pairs);
-    if (precond == CYMG_PRECOND_FP32) return time_tiled<2, CYMG_PRECOND_FP32>(c, pairs);
-    return time_tiled<2, CYMG_PRECOND_NONE>(c, pairs);
+    const bool small = tile_threads_of(c) == 256;
+    auto go = [&](auto dim) -> double {
+        constexpr int D = decltype(dim)::value;
+        if (small) {
+            if (precond == CYMG_PRECOND_FP64) return time_tiled<D, CYMG_PRECOND_FP64, 256>(c, pairs);
+            if (precond == CYMG_PRECOND_FP32) return time_tiled<D, CYMG_PRECOND_FP32, 256>(c, pairs);
+            return time_tiled<D, CYMG_PRECOND_NONE, 256>(c, pairs);
+        }
+        if (precond == CYMG_PRECOND_FP64) return time_tiled<D, CYMG_PRECOND_FP64, 512>(c, pairs);
+        if (precond == CYMG_PRECOND_FP32) return time_tiled<D, CYMG_PRECOND_FP32, 512>(c, pairs);
+        return time_tiled<D, CYMG_PRECOND_NONE, 512>(c, pairs);
+    };
+    if (c.dim == 3) return go(std::integral_constant<int, 3>());
+    return go(std::integral_constant<int, 2>());
 }
 
 void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter, double tol, int tol_mode,
@@ -2072,14 +2145,19 @@ void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter
     // a larger Solid runs the four-launch iteration
     if (c.krylov_fused && (!c.nsol || c.solid_tiles)) {
         tiled_buffers(c);
+        const bool small = tile_threads_of(c) == 256;
+        auto go = [&](auto dim, auto nt) {
+            constexpr int D = decltype(dim)::value, NT = decltype(nt)::value;
+            if (precond == CYMG_PRECOND_FP64) run_tiled<D, CYMG_PRECOND_FP64, NT>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            else if (precond == CYMG_PRECOND_FP32) run_tiled<D, CYMG_PRECOND_FP32, NT>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            else run_tiled<D, CYMG_PRECOND_NONE, NT>(c, guess_dev, max_iter, tol, tol_mode, rep);
+        };
         if (c.dim == 3) {
-            if (precond == CYMG_PRECOND_FP64) run_tiled<3, CYMG_PRECOND_FP64>(c, guess_dev, max_iter, tol, tol_mode, rep);
-            else if (precond == CYMG_PRECOND_FP32) run_tiled<3, CYMG_PRECOND_FP32>(c, guess_dev, max_iter, tol, tol_mode, rep);
-            else run_tiled<3, CYMG_PRECOND_NONE>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            if (small) go(std::integral_constant<int, 3>(), std::integral_constant<int, 256>());
+            else go(std::integral_constant<int, 3>(), std::integral_constant<int, 512>());
         } else {
-            if (precond == CYMG_PRECOND_FP64) run_tiled<2, CYMG_PRECOND_FP64>(c, guess_dev, max_iter, tol, tol_mode, rep);
-            else if (precond == CYMG_PRECOND_FP32) run_tiled<2, CYMG_PRECOND_FP32>(c, guess_dev, max_iter, tol, tol_mode, rep);
-            else run_tiled<2, CYMG_PRECOND_NONE>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            if (small) go(std::integral_constant<int, 2>(), std::integral_constant<int, 256>());
+            else go(std::integral_constant<int, 2>(), std::integral_constant<int, 512>());
         }
         return;
     }

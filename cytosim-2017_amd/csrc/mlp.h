@@ -7,7 +7,7 @@
 namespace cymg {
 
 constexpr int kMlpPoints = 31;                 // fibers of at most this many points get a learned guess
-constexpr int kMlpIn = 288;                    // 3 groups of 3*kMlpPoints features, padded to 16
+constexpr int kMlpIn = 384;                    // 4 groups of 3*kMlpPoints features, padded to 16
 constexpr int kMlpHidden = 256;
 constexpr int kMlpOut = 96;                    // 3*kMlpPoints outputs, padded; targets keep the count in the last
 constexpr int kMlpLayers = 4;
@@ -29,6 +29,7 @@ struct MlpState {
     DevBuf<uint16_t> fx;                       // features of the step
     DevBuf<float> fs;                          // per-fiber scale |rhs|_inf
     DevBuf<double> y, x0;                      // M^-1 b and the guess
+    DevBuf<double> z1;                         // M^-1 (b - A M^-1 b): what the coupled neighbours add
     // recorded (features, targets) ring
     bool recording = false;
     uint32_t ds_cap = 0, ds_count = 0, ds_next = 0;

@@ -1,6 +1,7 @@
 // Learned initial guess of the implicit step: a per-fiber multilayer
-// perceptron (input 288 = 31 points x 3 coordinates x {position relative to
-// the centroid, rhs, previous solution}, zero-padded; three hidden layers of
+// perceptron (input 384 = 31 points x 3 coordinates x {position relative to
+// the centroid, rhs, previous solution, block-Jacobi correction of M^-1 b},
+// zero-padded; three hidden layers of
 // 256 with GELU; output 96) that predicts the correction of the block-Jacobi
 // solution: x0 = M^-1 b + s * MLP(features), s = |rhs|_inf of the fiber.
 // It has no counterpart in the reference (meca.cc:704 starts from zero); the
@@ -157,11 +158,16 @@ __global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y,
 }
 
 // Features of each fiber (one wavefront per fiber; fibers of more than
-// kMlpPoints points get zero features and scale 0).
+// kMlpPoints points get zero features and scale 0): per point, the position
+// about the centroid / 5, then rhs, previous solution and the block-Jacobi
+// correction z1 = M^-1 (b - A M^-1 b), each / s with s = |rhs|_inf of the
+// fiber.  z1 is the coupling term: the fiber's own block is solved exactly by
+// M^-1 b, so its residual holds what the linked fibers add through mB/mC.
 template <int D>
 __global__ __launch_bounds__(kWave) void k_features(FiberArgs a, const double* __restrict__ pts,
                                                     const double* __restrict__ rhs, const double* __restrict__ prev,
-                                                    uint16_t* __restrict__ X, float* __restrict__ scale)
+                                                    const double* __restrict__ corr, uint16_t* __restrict__ X,
+                                                    float* __restrict__ scale)
 {
     const uint32_t f = blockIdx.x;
     const int lane = threadIdx.x;
@@ -169,15 +175,16 @@ __global__ __launch_bounds__(kWave) void k_features(FiberArgs a, const double* _
     const uint32_t o = D * a.idx[f];
     uint16_t* row = X + (long)f * kMlpIn;
     const bool ok = np <= kMlpPoints;
-    double c[D], r[D], q[D], cs[D];
+    double c[D], r[D], q[D], z[D], cs[D];
     double rm = 0.0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        c[d] = r[d] = q[d] = 0.0;
+        c[d] = r[d] = q[d] = z[d] = 0.0;
         if (ok && lane < np) {
             c[d] = pts[o + D * lane + d];
             r[d] = rhs[o + D * lane + d];
             q[d] = prev[o + D * lane + d];
+            z[d] = corr[o + D * lane + d];
             rm = fmax(rm, fabs(r[d]));
         }
         cs[d] = wave_sum(c[d]) / (double)np;
@@ -192,6 +199,7 @@ __global__ __launch_bounds__(kWave) void k_features(FiberArgs a, const double* _
             row[3 * lane + d] = to_bf16((float)((c[d] - cs[d]) * 0.2));
             row[3 * kMlpPoints + 3 * lane + d] = to_bf16((float)(r[d] * inv));
             row[6 * kMlpPoints + 3 * lane + d] = to_bf16((float)(q[d] * inv));
+            row[9 * kMlpPoints + 3 * lane + d] = to_bf16((float)(z[d] * inv));
         }
     }
     if (lane == 0) scale[f] = (float)s;
@@ -413,27 +421,44 @@ double mlp_train(Context& c, int steps, int B, double lr, uint64_t seed)
     return last;
 }
 
-// features of the step and y = M^-1 b (0 without preconditioner)
+// r = b - r
+__global__ void k_residual(uint32_t n, const double* __restrict__ b, double* __restrict__ r)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = b[i] - r[i];
+}
+
+// y = M^-1 b (0 without preconditioner), z1 = M^-1 (b - A y) and the
+// features of the step
 void mlp_prepare_step(Context& c, int precond)
 {
     MlpState& s = c.mlp;
     const uint32_t F = c.nfib;
+    const uint32_t n = c.n();
     const int M = (int)((F + 15) / 16 * 16);
     ensure_work(c, M);
     s.fx.resize((size_t)M * kMlpIn);
     s.fs.resize(M);
-    s.y.resize(c.n());
-    s.x0.resize(c.n());
+    s.y.resize(n);
+    s.x0.resize(n);
+    s.z1.resize(n);
     CYMG_HIP(hipMemsetAsync(s.fx.get(), 0, (size_t)M * kMlpIn * 2, c.stream));
+    if (precond) {
+        launch_precond_apply(c, c.d_rhs.get(), s.y.get());
+        launch_matvec(c, s.y.get(), s.x0.get());          // x0 holds A y until the guess is formed
+        hipLaunchKernelGGL(k_residual, grid1(n), dim3(256), 0, c.stream, n, c.d_rhs.get(), s.x0.get());
+        launch_precond_apply(c, s.x0.get(), s.z1.get());
+    } else {
+        CYMG_HIP(hipMemsetAsync(s.y.get(), 0, (size_t)n * 8, c.stream));
+        CYMG_HIP(hipMemsetAsync(s.z1.get(), 0, (size_t)n * 8, c.stream));
+    }
     const FiberArgs a = make_fiber_args(c);
     if (c.dim == 3)
         hipLaunchKernelGGL(k_features<3>, dim3(F), dim3(kWave), 0, c.stream, a, c.d_pts.get(), c.d_rhs.get(),
-                           c.d_prev_sol.get(), s.fx.get(), s.fs.get());
+                           c.d_prev_sol.get(), s.z1.get(), s.fx.get(), s.fs.get());
     else
         hipLaunchKernelGGL(k_features<2>, dim3(F), dim3(kWave), 0, c.stream, a, c.d_pts.get(), c.d_rhs.get(),
-                           c.d_prev_sol.get(), s.fx.get(), s.fs.get());
-    if (precond) launch_precond_apply(c, c.d_rhs.get(), s.y.get());
-    else CYMG_HIP(hipMemsetAsync(s.y.get(), 0, (size_t)c.n() * 8, c.stream));
+                           c.d_prev_sol.get(), s.z1.get(), s.fx.get(), s.fs.get());
     CYMG_HIP(hipGetLastError());
     s.step_ready = true;
 }

@@ -304,13 +304,19 @@ int         cymg_get_matrix(cymg_context* ctx, int which, uint32_t* nnz, uint32_
      each unknown gets one thread instead of two, which suits many tiles),
      0 (default): chosen from the layout (or by cymg_autotune);
    CYMG_OPT_GRAPHS: 1 (default) replays chunks of 2 and 8 iterations of the
-     two-launch iteration as captured HIP graphs, 0 launches every kernel */
-enum { CYMG_OPT_KRYLOV_LAUNCHES = 1, CYMG_OPT_BLOCK_INVERSE = 2, CYMG_OPT_TILE_POINTS = 3, CYMG_OPT_GRAPHS = 4 };
+     two-launch iteration as captured HIP graphs, 0 launches every kernel;
+   CYMG_OPT_TILE_THREADS: 512 (default) or 256 threads per tile of the
+     two-launch iteration; 256 (tiles of at most 256/DIM points, two or more
+     tiles resident per compute unit) applies to layouts without Solids whose
+     fibers all fit such a tile, others keep 512 */
+enum { CYMG_OPT_KRYLOV_LAUNCHES = 1, CYMG_OPT_BLOCK_INVERSE = 2, CYMG_OPT_TILE_POINTS = 3, CYMG_OPT_GRAPHS = 4,
+       CYMG_OPT_TILE_THREADS = 5 };
 int         cymg_set_option(cymg_context* ctx, int option, int value);
+int         cymg_get_option(cymg_context* ctx, int option, int* value);   /* the value in use */
 /* autotuner (no counterpart in the reference): on the current system (after
    setup), times the block-inverse methods (Gauss-Jordan in registers / in
    LDS / blocked on the matrix cores) and the tile shapes of the two-launch iteration (target points per
-   tile 16..170 and the layout default, `reps` launches each) and keeps the
+   tile 16..170 and the layout default, for 512- and 256-thread tiles, `reps` launches each) and keeps the
    fastest; reports the choices and the chosen launch time (ms) */
 int         cymg_autotune(cymg_context* ctx, int reps, int* tile_points, int* block_inverse, double* launch_ms);
 
@@ -339,19 +345,21 @@ int         cymg_time_kernel(cymg_context* ctx, int which, int reps, double* avg
 
 /* learned initial guess (no counterpart in the reference, which seeds BCGS with
    vSOL = 0, meca.cc:704).  A per-fiber network (inputs: the fiber's points
-   relative to their centroid / 5, rhs / s and previous solution / s, s = the
-   fiber's |rhs|_inf, 31 points x 3 each, zero-padded to 288; three hidden
+   relative to their centroid / 5, rhs / s, previous solution / s and the
+   block-Jacobi correction M^-1 (b - A M^-1 b) / s, which carries what the
+   linked fibers add (the fiber's own block is solved exactly by M^-1 b),
+   s = the fiber's |rhs|_inf, 31 points x 3 each, zero-padded to 384; three hidden
    layers of 256 with GELU; 96 outputs) predicts the correction c of the
    block-Jacobi solution: x0 = M^-1 b + s*c (M^-1 = 0 without preconditioner).
    Fibers of more than 31 points are seeded with M^-1 b.  GEMMs run on BF16
    MFMA with FP32 accumulation, weights are FP32 with BF16 copies.  The solver
    iterates from x0 to the same stop test as from any other guess.
-   Weights are flat FP32, layer by layer: W1 [288][256], b1 [256], W2 [256][256],
+   Weights are flat FP32, layer by layer: W1 [384][256], b1 [256], W2 [256][256],
    b2, W3 [256][256], b3, W4 [256][96], b4 [96]; W[in][out] row-major. */
-#define CYMG_GUESS_INPUTS   288
+#define CYMG_GUESS_INPUTS   384
 #define CYMG_GUESS_OUTPUTS  96
 #define CYMG_GUESS_POINTS   31
-#define CYMG_GUESS_WEIGHTS  230240
+#define CYMG_GUESS_WEIGHTS  254816
 int         cymg_guess_init(cymg_context* ctx, uint64_t seed);             /* uniform(+-1/sqrt(in)) weights, zero biases */
 int         cymg_guess_weights(cymg_context* ctx, int set, float* w);      /* set != 0: load w; else copy the weights out */
 /* record (features, targets) of every fiber of each converged solve into a
@@ -363,12 +371,12 @@ int         cymg_guess_samples(cymg_context* ctx, uint32_t* count);
    `batch` recorded samples (batch a multiple of 16); loss of the last step:
    sum of squared errors over the valid outputs / batch */
 int         cymg_guess_train(cymg_context* ctx, int steps, int batch, double lr, uint64_t seed, double* loss);
-/* forward + backward on host inputs X [batch][288] (rounded to BF16) and
+/* forward + backward on host inputs X [batch][384] (rounded to BF16) and
    targets T [batch][96] (T[.][95] = number of valid outputs); loss and the
    gradient of every weight (CYMG_GUESS_WEIGHTS floats) */
 int         cymg_guess_check(cymg_context* ctx, const float* X, const float* T, int batch, double* loss,
                              float* grads);
-/* features [nfib][288] (as FP32 values of the BF16 inputs) and scales [nfib] of
+/* features [nfib][384] (as FP32 values of the BF16 inputs) and scales [nfib] of
    the last iterate() that used or recorded the learned guess */
 int         cymg_guess_features(cymg_context* ctx, float* X, float* scale);
 /* the learned initial guess x0 = M^-1 b + s*c of the current system (after

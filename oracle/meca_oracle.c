@@ -854,6 +854,24 @@ void orc_inter_link_ii(orc_meca* m, uint32_t a1, uint32_t a2, double ca,
             *sp_ref(&m->mB, inx[i], inx[j]) -= c[j] * cw[i];
 }
 
+/* Couple::setInteractions for a list of couples (couple.cc:87-92): each
+   doubly-attached couple (state & 3 == 3; state NULL: all) links its two
+   hands, interpolated on their fibers (fiber_naked.cc:1050-1065), with
+   interLink(PointInterpolated, PointInterpolated); one call per step so that
+   the CPU baseline is not timed on its caller's loop */
+void orc_inter_couples(orc_meca* m, uint32_t n, const uint32_t* fa, const double* aa, const uint32_t* fb,
+                       const double* ab, const double* w, const uint8_t* state)
+{
+    for (uint32_t k = 0; k < n; ++k) {
+        if (state && (state[k] & 3) != 3) continue;
+        uint32_t a1, a2, b1, b2;
+        double ca, cb;
+        orc_interpolate(m, (int)fa[k], aa[k], &a1, &a2, &ca);
+        orc_interpolate(m, (int)fb[k], ab[k], &b1, &b2, &cb);
+        orc_inter_link_ii(m, a1, a2, ca, b1, b2, cb, w[k]);
+    }
+}
+
 /* interLongLink(PointExact, PointExact) (meca_inter.cc:334-415), no periodic space */
 void orc_inter_long_link_ee(orc_meca* m, uint32_t a, uint32_t b, double len, double w)
 {

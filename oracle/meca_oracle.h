@@ -79,6 +79,10 @@ void      orc_inter_link_ee(orc_meca*, uint32_t a, uint32_t b, double w);
 void      orc_inter_link_ie(orc_meca*, uint32_t a1, uint32_t a2, double ca, uint32_t b, double w);
 void      orc_inter_link_ii(orc_meca*, uint32_t a1, uint32_t a2, double ca,
                             uint32_t b1, uint32_t b2, double cb, double w);
+/* Couple::setInteractions (couple.cc:87-92) for n couples (hands: fiber,
+   abscissa from the minus end); state: bit 1|2 both attached, NULL: all */
+void      orc_inter_couples(orc_meca*, uint32_t n, const uint32_t* fa, const double* aa, const uint32_t* fb,
+                            const double* ab, const double* w, const uint8_t* state);
 void      orc_inter_long_link_ee(orc_meca*, uint32_t a, uint32_t b, double len, double w);
 void      orc_inter_clamp_e(orc_meca*, uint32_t a, const double* g, double w);
 void      orc_inter_clamp_i(orc_meca*, uint32_t a1, uint32_t a2, double ca, const double* g, double w);

@@ -13,7 +13,9 @@ include/cymeca.h):
     [0, 93)    (x_p - centroid) / 5, three slots per point (z = 0 in 2D)
     [93, 186)  rhs_p / s
     [186, 279) previous solution_p / s
-    [279, 288) zero
+    [279, 372) z1_p / s, z1 = M^-1 (b - A M^-1 b) (the block-Jacobi correction:
+               what the linked fibers add to the fiber's own solution M^-1 b)
+    [372, 384) zero
   every feature rounded to BF16 (round to nearest even);
   layers q = 0..3: Z_q = H_q W_q + b_q with BF16 H_q and W_q and exact (here
   float64; device FP32) accumulation, H_{q+1} = bf16(gelu(Z_q)), output Z_3;
@@ -26,7 +28,7 @@ GELU and its derivative, BF16 rounding on values with known results.
 import numpy as np
 from scipy.special import erf
 
-POINTS, INPUTS, HIDDEN, OUTPUTS = 31, 288, 256, 96
+POINTS, INPUTS, HIDDEN, OUTPUTS = 31, 384, 256, 96
 LAYER_IN = (INPUTS, HIDDEN, HIDDEN, HIDDEN)
 LAYER_OUT = (HIDDEN, HIDDEN, HIDDEN, OUTPUTS)
 WEIGHTS = sum(i * o + o for i, o in zip(LAYER_IN, LAYER_OUT))
@@ -80,7 +82,7 @@ def _r(x, rounding):
 
 
 def forward(w, X, rounding=True):
-    """Pre-activations Z_q and layer inputs H_q of rows X [B][288]."""
+    """Pre-activations Z_q and layer inputs H_q of rows X [B][384]."""
     layers = unpack(w)
     H = [_r(X, rounding)]
     Z = []
@@ -117,8 +119,11 @@ def loss_and_grad(w, X, T, rounding=True):
     return loss, flat
 
 
-def features(dim, nb_points, pts, rhs, prev):
-    """Features [nfib][288] (BF16 values as float32) and scales [nfib] (float32)."""
+def features(dim, nb_points, pts, rhs, prev, corr=None):
+    """Features [nfib][384] (BF16 values as float32) and scales [nfib] (float32);
+    corr: the block-Jacobi correction z1 (zero if None)."""
+    if corr is None:
+        corr = np.zeros_like(np.asarray(rhs, dtype=np.float64))
     nb_points = np.asarray(nb_points)
     F = nb_points.size
     X = np.zeros((F, INPUTS), np.float32)
@@ -133,10 +138,11 @@ def features(dim, nb_points, pts, rhs, prev):
         p = np.asarray(pts[sl], dtype=np.float64).reshape(n, dim)
         r = np.asarray(rhs[sl], dtype=np.float64).reshape(n, dim)
         q = np.asarray(prev[sl], dtype=np.float64).reshape(n, dim)
+        z = np.asarray(corr[sl], dtype=np.float64).reshape(n, dim)
         s = np.abs(r).max()
         inv = 1.0 / s if s > 0 else 0.0
         c = p.mean(axis=0)
-        for k, block in enumerate(((p - c) * 0.2, r * inv, q * inv)):
+        for k, block in enumerate(((p - c) * 0.2, r * inv, q * inv, z * inv)):
             g = np.zeros((n, 3))
             g[:, :dim] = block
             X[f, 3 * POINTS * k:3 * POINTS * k + 3 * n] = g.ravel().astype(np.float32)

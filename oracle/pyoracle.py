@@ -65,6 +65,7 @@ def lib():
         "orc_set_params": (None, [vp, C.POINTER(Params)]),
         "orc_clear": (None, [vp]),
         "orc_add_fiber": (C.c_int, [vp, C.c_uint32, C.c_double, C.c_double]),
+        "orc_inter_couples": (None, [vp, C.c_uint32, vp, vp, vp, vp, vp, vp]),
         "orc_add_fiber_ex": (C.c_int, [vp, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_double]),
         "orc_add_solid": (C.c_int, [vp, C.c_uint32, _f64p, _f64p, C.c_int]),
@@ -226,6 +227,16 @@ class OracleMeca:
 
     def inter_link_ii(self, a1, a2, ca, b1, b2, cb, w):
         self.L.orc_inter_link_ii(self.h, a1, a2, ca, b1, b2, cb, w)
+
+    def inter_couples(self, fiber_a, abs_a, fiber_b, abs_b, stiffness, state=None):
+        fa = np.ascontiguousarray(fiber_a, dtype=np.uint32)
+        fb = np.ascontiguousarray(fiber_b, dtype=np.uint32)
+        aa = np.ascontiguousarray(abs_a, dtype=np.float64)
+        ab = np.ascontiguousarray(abs_b, dtype=np.float64)
+        w = np.ascontiguousarray(stiffness, dtype=np.float64)
+        st = None if state is None else np.ascontiguousarray(state, dtype=np.uint8)
+        self.L.orc_inter_couples(self.h, fa.size, fa.ctypes.data, aa.ctypes.data, fb.ctypes.data, ab.ctypes.data,
+                                 w.ctypes.data, None if st is None else st.ctypes.data)
 
     def inter_link_ie(self, a1, a2, ca, b, w):
         self.L.orc_inter_link_ie(self.h, a1, a2, ca, b, w)
@@ -465,14 +476,9 @@ def assemble_scene(m, sc):
     if sc.confine_radius > 0:
         m.confine_sphere(sc.confine_radius, sc.confine_stiffness)
     if sc.nlinks:
-        a1, a2, ca = sc.interpolate(sc.link_fiber_a, sc.link_abs_a)
-        b1, b2, cb = sc.interpolate(sc.link_fiber_b, sc.link_abs_b)
-        state = getattr(sc, "link_state", None)
-        for k in range(sc.nlinks):
-            if state is not None and (int(state[k]) & 3) != 3:
-                continue      # only bridging couples interact (couple.cc:87-92)
-            m.inter_link_ii(int(a1[k]), int(a2[k]), float(ca[k]), int(b1[k]), int(b2[k]),
-                            float(cb[k]), float(sc.link_stiffness[k]))
+        # only bridging couples interact (couple.cc:87-92); one C call
+        m.inter_couples(sc.link_fiber_a, sc.link_abs_a, sc.link_fiber_b, sc.link_abs_b, sc.link_stiffness,
+                        getattr(sc, "link_state", None))
     for k in range(getattr(sc, "naster_links", 0)):
         if sc.aster_kind[k] == 0:
             m.inter_link_ee(int(sc.aster_a1[k]), int(sc.aster_b[k]), float(sc.aster_w[k]))

@@ -88,7 +88,11 @@ def test_features_match_oracle(pkg, scenes, dim):
         print(f"features dim={dim} step={step}", flush=True)
         _step(m, pkg, sc, step, apply=False)
         X, S = m.guess_features()
-        Xo, So = mo.features(dim, sc.nb_points, m.vector(pkg.VEC_PTS), m.vector(pkg.VEC_RHS), prev)
+        # the coupling channel z1 = M^-1 (b - A M^-1 b), from the context's own
+        # Meca::precondition / Meca::multiply (each checked against the oracle)
+        b = m.vector(pkg.VEC_RHS)
+        z1 = m.precondition(b - m.multiply(m.precondition(b)))
+        Xo, So = mo.features(dim, sc.nb_points, m.vector(pkg.VEC_PTS), b, prev, z1)
         assert np.array_equal(S, So)
         pos = np.zeros(mo.INPUTS, bool)
         pos[:93] = True

@@ -199,11 +199,13 @@ def test_block_inverse_methods(pkg, scenes, oracle, dim, lmax, inverse, precisio
     gpu.close()
 
 
-def _step_pair(pkg, oracle, sc, steps, tol, tol_mode, precondition=1, launches=2, inverse=0, tile_points=0):
+def _step_pair(pkg, oracle, sc, steps, tol, tol_mode, precondition=1, launches=2, inverse=0, tile_points=0,
+               threads=512):
     gpu = gpu_from_scene(pkg, sc)
     gpu.set_option(pkg.OPT_KRYLOV_LAUNCHES, launches)
     gpu.set_option(pkg.OPT_BLOCK_INVERSE, inverse)
     gpu.set_option(pkg.OPT_TILE_POINTS, tile_points)
+    gpu.set_option(pkg.OPT_TILE_THREADS, threads)
     orc = oracle_from_scene(oracle, sc)
     reps = []
     for s in range(steps):
@@ -254,6 +256,40 @@ def test_tiles_of_fibers(pkg, scenes, oracle, tile_points):
     for r1, r2 in reps:
         assert r1["converged"] and r2["converged"]
     assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-9
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("precondition", [0, 1, 2])
+@pytest.mark.parametrize("tile_points", [0, 16, 85])
+def test_tiles_of_256_threads(pkg, scenes, oracle, dim, precondition, tile_points):
+    """The 256-thread tiles of the two-launch iteration (fiber-only layouts):
+    split tiles of at most 128 unknowns, small tiles, and tiles of up to 255
+    unknowns with one thread each, in 2D and 3D, with each preconditioner."""
+    sc = scenes.config2(seed=26 + dim, nfib=80, nlinks=400, radius=4.0) if dim == 3 else \
+        scenes.mini(dim=2, seed=26, nfib=30, nlinks=90, radius=3.0)
+    print(f"256-thread tiles: dim={dim} precondition={precondition} tile_points={tile_points}", flush=True)
+    # without preconditioner the error at 1e-11 sits at the 1e-9 bar (the Krylov
+    # path's rounding, not the kernels'): solve tighter
+    tol = {0: 1e-12, 1: 1e-11, 2: 1e-9}[precondition]
+    gpu, orc, reps = _step_pair(pkg, oracle, sc, 2, tol, 1, precondition=precondition, tile_points=tile_points,
+                                threads=256)
+    assert gpu.get_option(pkg.OPT_TILE_THREADS) == 256
+    for r1, r2 in reps:
+        assert r1["converged"] and r2["converged"]
+    assert rel_err(gpu.fiber_points(), orc.fiber_points()) < (1e-9 if precondition != 2 else 1e-6)
+    # the time of one full-work launch is measurable on this path too
+    assert gpu.time_kernel(7, reps=4) > 0
+    gpu.close()
+
+
+def test_256_threads_fall_back_for_long_fibers(pkg, scenes):
+    """A fiber longer than a 256-thread tile keeps the 512-thread launches."""
+    sc = scenes.config2(seed=25, nfib=30, nlinks=150, radius=4.0, lmin=20.0, lmax=50.0)
+    assert sc.nb_points.max() > 85
+    gpu = gpu_from_scene(pkg, sc)
+    gpu.set_option(pkg.OPT_TILE_THREADS, 256)
+    assert gpu.get_option(pkg.OPT_TILE_THREADS) == 512
+    gpu.close()
 
 
 def test_long_fibers_unpreconditioned(pkg, scenes, oracle):

@@ -11,7 +11,7 @@ from oracle import mlp_oracle as mo
 
 
 def test_weight_count_matches_header(pkg):
-    assert mo.WEIGHTS == pkg.GUESS_WEIGHTS == 230240
+    assert mo.WEIGHTS == pkg.GUESS_WEIGHTS == 254816
     assert (mo.INPUTS, mo.OUTPUTS, mo.POINTS) == (pkg.GUESS_INPUTS, pkg.GUESS_OUTPUTS, pkg.GUESS_POINTS)
 
 
@@ -129,8 +129,10 @@ def test_feature_layout(dim):
     pts = rng.standard_normal(dim * n) * 4
     rhs = rng.standard_normal(dim * n)
     prev = rng.standard_normal(dim * n)
+    corr = rng.standard_normal(dim * n)
     rhs[dim * (5 + 31 + 32):] = 0.0              # a fiber without load
-    X, S = mo.features(dim, nb, pts, rhs, prev)
+    X, S = mo.features(dim, nb, pts, rhs, prev, corr)
+    z0 = corr[:dim * 5].reshape(5, dim)
     p0 = pts[:dim * 5].reshape(5, dim)
     r0 = rhs[:dim * 5].reshape(5, dim)
     q0 = prev[:dim * 5].reshape(5, dim)
@@ -142,9 +144,11 @@ def test_feature_layout(dim):
             assert X[0, 3 * p + d] == mo.bf16(np.float32((p0[p, d] - c[d]) / 5))
             assert X[0, 93 + 3 * p + d] == mo.bf16(np.float32(r0[p, d] / s0))
             assert X[0, 186 + 3 * p + d] == mo.bf16(np.float32(q0[p, d] / s0))
+            assert X[0, 279 + 3 * p + d] == mo.bf16(np.float32(z0[p, d] / s0))
         if dim == 2:
             assert X[0, 3 * p + 2] == 0 and X[0, 93 + 3 * p + 2] == 0 and X[0, 186 + 3 * p + 2] == 0
-    assert not np.any(X[0, 15:93]) and not np.any(X[0, 279:])
+            assert X[0, 279 + 3 * p + 2] == 0
+    assert not np.any(X[0, 15:93]) and not np.any(X[0, 294:372]) and not np.any(X[0, 372:])
     # the largest rhs component of a fiber is +-1 after scaling
     assert np.abs(X[1, 93:186]).max() == 1.0
     assert not np.any(X[2]) and S[2] == 0           # 32 points: no learned guess

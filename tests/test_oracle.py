@@ -627,3 +627,26 @@ def test_mobility_surface_model(oracle, L, h):
     p = min(L, 10.0) / (2 * r)
     vol = np.pi * eta * max(3 * L / (np.log(p) + 0.312), 6 * r)
     assert m.mobility(1) == pytest.approx(npts / vol, rel=1e-13)
+
+
+def test_couple_list_assembly_equals_one_link_per_call(oracle, scenes):
+    """orc_inter_couples (one C call for the couple list, used by the CPU
+    baseline) records exactly the interLink calls of the per-couple loop, in
+    the same order: bit-identical mB; couples with a free hand are skipped."""
+    sc = scenes.mini(dim=3, seed=61, nfib=10, nlinks=40)
+    state = np.full(sc.nlinks, 3, np.uint8)
+    state[::5] = 1
+    m1 = oracle.OracleMeca(3, sc.params())
+    m2 = oracle.OracleMeca(3, sc.params())
+    for m in (m1, m2):
+        oracle.register_scene(m, sc)
+        m.prepare(sc.points)
+    m1.inter_couples(sc.link_fiber_a, sc.link_abs_a, sc.link_fiber_b, sc.link_abs_b, sc.link_stiffness, state)
+    a1, a2, ca = sc.interpolate(sc.link_fiber_a, sc.link_abs_a)
+    b1, b2, cb = sc.interpolate(sc.link_fiber_b, sc.link_abs_b)
+    for k in range(sc.nlinks):
+        if state[k] == 3:
+            m2.inter_link_ii(int(a1[k]), int(a2[k]), float(ca[k]), int(b1[k]), int(b2[k]), float(cb[k]),
+                             float(sc.link_stiffness[k]))
+    assert np.array_equal(m1.matrix_dense(0), m2.matrix_dense(0))
+    assert np.any(m1.matrix_dense(0))
