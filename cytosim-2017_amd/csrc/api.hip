@@ -393,6 +393,7 @@ void do_iterate(Context& c, const cymg_solve_params* p, cymg_solve_report* rep)
         Timer t(c, 3);
         krylov_solve(c, pc, guess, maxit, p->tolerance, p->tol_mode, &r);
         check_status(c);   // the solve has synchronised past the status copy of prepare
+        if (learned && mlp_guess_dropped(c)) guess_used = CYMG_GUESS_ZERO;
         if (r.converged && c.mlp.recording) mlp_record(c);
         int total = r.iterations;
         if (!r.converged && p->fallback) {
@@ -486,6 +487,7 @@ int cymg_destroy(cymg_context* ctx)
     if (c.h_scal) (void)hipHostFree(c.h_scal);
     if (c.h_snap) (void)hipHostFree(c.h_snap);
     if (c.h_status) (void)hipHostFree(c.h_status);
+    if (c.mlp.h_gflag) (void)hipHostFree(c.mlp.h_gflag);
     for (int i = 0; i < 2; ++i)
         if (c.chunk_ev[i]) (void)hipEventDestroy(c.chunk_ev[i]);
     for (auto& row : c.iter_graph)
@@ -1305,6 +1307,7 @@ int cymg_set_option(cymg_context* ctx, int option, int value)
             c.tile_points = (uint32_t)value;
             if (!c.layout_dirty && c.nfib) build_tiles(c);
         }
+        else if (option == CYMG_OPT_GUESS_GUARD && (value == 0 || value == 1)) c.mlp.guard = value;
         else if (option == CYMG_OPT_TILE_THREADS && (value == 256 || value == 512)) {
             c.tile_threads = (uint32_t)value;
             if (!c.layout_dirty && c.nfib) build_tiles(c);
@@ -1323,6 +1326,7 @@ int cymg_get_option(cymg_context* ctx, int option, int* value)
         else if (option == CYMG_OPT_GRAPHS) *value = c.use_graphs;
         else if (option == CYMG_OPT_TILE_POINTS) *value = (int)c.tile_points;
         else if (option == CYMG_OPT_TILE_THREADS) *value = tile_threads_of(c);
+        else if (option == CYMG_OPT_GUESS_GUARD) *value = c.mlp.guard;
         else throw Error(CYMG_ERR_INVALID, "unknown option");
     });
 }

@@ -11,6 +11,7 @@ constexpr int kMlpIn = 384;                    // 4 groups of 3*kMlpPoints featu
 constexpr int kMlpHidden = 256;
 constexpr int kMlpOut = 96;                    // 3*kMlpPoints outputs, padded; targets keep the count in the last
 constexpr int kMlpLayers = 4;
+constexpr float kMlpTargetClip = 32.f;        // samples with a larger |target| are not trained on
 constexpr long kMlpWeights = (long)kMlpIn * kMlpHidden + kMlpHidden + 2L * (kMlpHidden * kMlpHidden + kMlpHidden) +
                              (long)kMlpHidden * kMlpOut + kMlpOut;
 
@@ -30,6 +31,10 @@ struct MlpState {
     DevBuf<float> fs;                          // per-fiber scale |rhs|_inf
     DevBuf<double> y, x0;                      // M^-1 b and the guess
     DevBuf<double> z1;                         // M^-1 (b - A M^-1 b): what the coupled neighbours add
+    DevBuf<double> gpart;                      // guard: per-block max |b - A x0| and max |b|
+    DevBuf<int> gflag;                         // guard: 1 when the guess was dropped
+    int* h_gflag = nullptr;                    // pinned copy of gflag
+    int guard = 1;                             // CYMG_OPT_GUESS_GUARD
     // recorded (features, targets) ring
     bool recording = false;
     uint32_t ds_cap = 0, ds_count = 0, ds_next = 0;
@@ -45,6 +50,7 @@ double mlp_check(Context& c, const float* X, const float* T, int B, float* grads
 double mlp_train(Context& c, int steps, int B, double lr, uint64_t seed);
 void mlp_prepare_step(Context& c, int precond);
 const double* mlp_guess(Context& c);
+bool mlp_guess_dropped(Context& c);
 void mlp_record(Context& c);
 void mlp_features(Context& c, float* X, float* scale);
 void mlp_set_record(Context& c, uint32_t capacity);

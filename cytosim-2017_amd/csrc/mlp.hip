@@ -235,12 +235,20 @@ __global__ __launch_bounds__(kWave) void k_targets(FiberArgs a, const double* __
     const float s = scale[f];
     float* row = T + (long)f * kMlpOut;
     for (int t = threadIdx.x; t < kMlpOut; t += kWave) row[t] = 0.f;
-    const bool ok = s > 0.f && np <= kMlpPoints;
+    bool ok = s > 0.f && np <= kMlpPoints;
+    float big = 0.f;
     if (ok)
         for (int t = threadIdx.x; t < D * np; t += kWave) {
             const int p = t / D, d = t - D * (t / D);
-            row[3 * p + d] = (float)((x[o + t] - y[o + t]) / (double)s);
+            const float v = (float)((x[o + t] - y[o + t]) / (double)s);
+            row[3 * p + d] = v;
+            big = fmaxf(big, fabsf(v));
         }
+    // a fiber whose correction is far beyond its own load scale (a lightly
+    // loaded fiber moved by its links) is not a sample: it would dominate the
+    // squared loss
+    for (int w = 32; w > 0; w >>= 1) big = fmaxf(big, __shfl_xor(big, w, kWave));
+    if (!(big <= kMlpTargetClip)) ok = false;
     // valid outputs: 3 per point (the third coordinate stays zero in 2D)
     if (threadIdx.x == 0) row[kMlpOut - 1] = ok ? (float)(3 * np) : 0.f;
 }
@@ -421,6 +429,60 @@ double mlp_train(Context& c, int steps, int B, double lr, uint64_t seed)
     return last;
 }
 
+// guard of the learned guess: per block, max |b - Ax| and max |b| (NaN kept)
+__global__ __launch_bounds__(256) void k_guard_part(uint32_t n, const double* __restrict__ b,
+                                                    const double* __restrict__ ax, double* __restrict__ part)
+{
+    double rm = 0.0, bm = 0.0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double r = fabs(b[i] - ax[i]);
+        rm = (r > rm || r != r) ? r : rm;
+        bm = fmax(bm, fabs(b[i]));
+    }
+    for (int w = 32; w > 0; w >>= 1) {
+        const double o = __shfl_xor(rm, w, kWave);
+        rm = (o > rm || o != o) ? o : rm;
+        bm = fmax(bm, __shfl_xor(bm, w, kWave));
+    }
+    __shared__ double sr[4], sb[4];
+    if ((threadIdx.x & 63) == 0) {
+        sr[threadIdx.x / 64] = rm;
+        sb[threadIdx.x / 64] = bm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 4; ++q) {
+            rm = (sr[q] > rm || sr[q] != sr[q]) ? sr[q] : rm;
+            bm = fmax(bm, sb[q]);
+        }
+        part[2 * blockIdx.x] = rm;
+        part[2 * blockIdx.x + 1] = bm;
+    }
+}
+
+// one wavefront: the guess is dropped (flag 1) when max |b - A x0| is not below max |b|
+__global__ void k_guard_decide(uint32_t nb, const double* __restrict__ part, int* __restrict__ flag)
+{
+    double rm = 0.0, bm = 0.0;
+    for (uint32_t i = threadIdx.x; i < nb; i += kWave) {
+        const double r = part[2 * i];
+        rm = (r > rm || r != r) ? r : rm;
+        bm = fmax(bm, part[2 * i + 1]);
+    }
+    for (int w = 32; w > 0; w >>= 1) {
+        const double o = __shfl_xor(rm, w, kWave);
+        rm = (o > rm || o != o) ? o : rm;
+        bm = fmax(bm, __shfl_xor(bm, w, kWave));
+    }
+    if (threadIdx.x == 0) *flag = (rm < bm) ? 0 : 1;
+}
+
+__global__ void k_guard_apply(uint32_t n, const int* __restrict__ flag, double* __restrict__ x0)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && *flag) x0[i] = 0.0;
+}
+
 // r = b - r
 __global__ void k_residual(uint32_t n, const double* __restrict__ b, double* __restrict__ r)
 {
@@ -484,8 +546,32 @@ const double* mlp_guess(Context& c)
     else
         hipLaunchKernelGGL(k_guess<2>, dim3(F), dim3(kWave), 0, c.stream, a, s.y.get(),
                            s.z[kMlpLayers - 1].get(), s.fs.get(), s.x0.get());
+    // the plain method stays the judge: a guess whose residual is not below
+    // that of the zero guess (|b - A x0|_inf >= |b|_inf) is replaced by zero,
+    // on the device, without a host round trip
+    const uint32_t n = c.n();
+    const uint32_t nb = std::min<uint32_t>(1024u, (n + 255) / 256);
+    s.gpart.resize(2 * (size_t)nb);
+    s.gflag.resize(1);
+    if (!s.h_gflag) CYMG_HIP(hipHostMalloc((void**)&s.h_gflag, sizeof(int), hipHostMallocDefault));
+    if (!s.guard) {
+        *s.h_gflag = 0;
+        return s.x0.get();
+    }
+    launch_matvec(c, s.x0.get(), s.z1.get());       // z1 is free once the features are built
+    hipLaunchKernelGGL(k_guard_part, dim3(nb), dim3(256), 0, c.stream, n, c.d_rhs.get(), s.z1.get(),
+                       s.gpart.get());
+    hipLaunchKernelGGL(k_guard_decide, dim3(1), dim3(kWave), 0, c.stream, nb, s.gpart.get(), s.gflag.get());
+    hipLaunchKernelGGL(k_guard_apply, grid1(n), dim3(256), 0, c.stream, n, s.gflag.get(), s.x0.get());
+    CYMG_HIP(hipMemcpyAsync(s.h_gflag, s.gflag.get(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CYMG_HIP(hipGetLastError());
     return s.x0.get();
+}
+
+// whether the last guess was dropped (valid once the stream has passed mlp_guess)
+bool mlp_guess_dropped(Context& c)
+{
+    return c.mlp.h_gflag && *c.mlp.h_gflag != 0;
 }
 
 // append the prepared step's features and the targets of the converged solution

@@ -308,9 +308,12 @@ int         cymg_get_matrix(cymg_context* ctx, int which, uint32_t* nnz, uint32_
    CYMG_OPT_TILE_THREADS: 512 (default) or 256 threads per tile of the
      two-launch iteration; 256 (tiles of at most 256/DIM points, two or more
      tiles resident per compute unit) applies to layouts without Solids whose
-     fibers all fit such a tile, others keep 512 */
+     fibers all fit such a tile, others keep 512;
+   CYMG_OPT_GUESS_GUARD: 1 (default) replaces a learned guess whose residual
+     |b - A x0|_inf is not below |b|_inf (that of the zero guess) by zero, on
+     the device, and reports guess_used = CYMG_GUESS_ZERO; 0 keeps every guess */
 enum { CYMG_OPT_KRYLOV_LAUNCHES = 1, CYMG_OPT_BLOCK_INVERSE = 2, CYMG_OPT_TILE_POINTS = 3, CYMG_OPT_GRAPHS = 4,
-       CYMG_OPT_TILE_THREADS = 5 };
+       CYMG_OPT_TILE_THREADS = 5, CYMG_OPT_GUESS_GUARD = 6 };
 int         cymg_set_option(cymg_context* ctx, int option, int value);
 int         cymg_get_option(cymg_context* ctx, int option, int* value);   /* the value in use */
 /* autotuner (no counterpart in the reference): on the current system (after

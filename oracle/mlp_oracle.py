@@ -150,8 +150,12 @@ def features(dim, nb_points, pts, rhs, prev, corr=None):
     return bf16(X), S
 
 
+TARGET_CLIP = 32.0
+
+
 def targets(dim, nb_points, x, y, scale):
-    """Training targets (x - y)/s [nfib][96]; T[.][95] = valid outputs (3 np)."""
+    """Training targets (x - y)/s [nfib][96]; T[.][95] = valid outputs (3 np),
+    0 for a fiber with a target beyond TARGET_CLIP in magnitude (not trained on)."""
     F = len(nb_points)
     T = np.zeros((F, OUTPUTS), np.float32)
     off = 0
@@ -164,8 +168,8 @@ def targets(dim, nb_points, x, y, scale):
             continue
         g = np.zeros((n, 3))
         g[:, :dim] = ((np.asarray(x[sl]) - np.asarray(y[sl])) / s).reshape(n, dim)
-        T[f, :3 * n] = g.ravel()
-        T[f, OUTPUTS - 1] = 3 * n
+        T[f, :3 * n] = g.ravel().astype(np.float32)
+        T[f, OUTPUTS - 1] = 3 * n if np.abs(T[f, :3 * n]).max() <= TARGET_CLIP else 0
     return T
 
 

@@ -3,6 +3,8 @@ scene (assembly bit-exact, one implicit step), the aster scene (10 Solids of
 201 points, 2000 fibers, 50000 motors; one step, then a motor step) and three
 systems of the 64-system batch, each against the oracle on the same inputs
 at a tight tolerance."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -71,3 +73,52 @@ def test_batch_of_64_systems_full_size(pkg, scenes, oracle):
         assert reps[k]["converged"] and orep["converged"]
         assert rel_err(pts[offs[k]:offs[k + 1]], orc.fiber_points()) < 1e-9
     gpu.close()
+
+
+def test_learned_fp32_on_the_1000_fiber_system(pkg, scenes, oracle):
+    """The learned guess + FP32 preconditioner on the 1000-fiber crosslinked
+    system, the network trained on the device from a rollout of plain FP64
+    steps: a step of the running simulation solved at relative residual 1e-6
+    lands within 1e-6 of the oracle's FP64 step (2-norm of the positions,
+    relative), as the plain FP64 solve at the same residual does, whether the
+    guard kept the network's guess or replaced it by zero (a guess whose
+    max-norm residual is not below |b|_inf is dropped)."""
+    sc = scenes.config2(seed=2)
+    gpu = pkg.Meca(3, 0)
+    pkg.load_scene(gpu, sc)
+    print("learned fp32, 1000 fibers: rollout", flush=True)
+    gpu.guess_record(40 * sc.nfib)
+    for s in range(40):
+        if s:
+            gpu.prepare(None)
+            pkg.assemble(gpu, sc, upload_couples=False)
+        gpu.solve(1e-6, sc.gauss(500 + s), tol_mode=pkg.TOL_RELATIVE)
+    gpu.guess_record(0)
+    gpu.guess_init(3)
+    gpu.guess_train(1500, batch=512, lr=1e-3, seed=4)
+    start = gpu.fiber_points()
+    print("learned fp32, 1000 fibers: step", flush=True)
+    gpu.prepare(None)
+    pkg.assemble(gpu, sc, upload_couples=False)
+    g = sc.gauss(7)
+    rep = gpu.solve(1e-6, g, tol_mode=pkg.TOL_RELATIVE, precondition=pkg.PRECOND_FP32, guess=pkg.GUESS_LEARNED)
+    assert rep["converged"] and rep["guess_used"] in (pkg.GUESS_LEARNED, pkg.GUESS_ZERO)
+    learned = gpu.fiber_points()
+    # plain FP64 from the same points and draws
+    plain = pkg.Meca(3, 0)
+    pkg.load_scene(plain, dataclasses.replace(sc, points=start))
+    rep64 = plain.solve(1e-6, g, tol_mode=pkg.TOL_RELATIVE)
+    assert rep64["converged"]
+    orc = oracle.OracleMeca(3, sc.params())
+    oracle.register_scene(orc, sc)
+    orc.prepare(start)
+    oracle.assemble_scene(orc, sc)
+    orc.solve(1e-12, g, tol_mode=1)
+    ref = orc.fiber_points()
+    e_learned = np.linalg.norm(learned - ref) / np.linalg.norm(ref)
+    e_plain = np.linalg.norm(plain.fiber_points() - ref) / np.linalg.norm(ref)
+    print(f"learned fp32 (guess used: {rep['guess_used']}) iterations {rep['iterations']} error {e_learned:.2e}; "
+          f"plain fp64 iterations {rep64['iterations']} error {e_plain:.2e}", flush=True)
+    assert e_learned < 1e-6 and e_plain < 1e-6
+    gpu.close()
+    plain.close()

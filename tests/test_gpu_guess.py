@@ -126,6 +126,7 @@ def test_learned_guess_step_matches_oracle(pkg, scenes, oracle, precond):
     m = pkg.Meca(3, 0)
     pkg.load_scene(m, sc)
     m.guess_init(9)
+    m.set_option(pkg.OPT_GUESS_GUARD, 0)      # seed with the network's guess whatever it is
     print(f"learned step precond={precond}", flush=True)
     rep = m.solve(1e-12, sc.gauss(0), tol_mode=pkg.TOL_RELATIVE, precondition=precond, guess=pkg.GUESS_LEARNED)
     assert rep["converged"] and rep["guess_used"] == pkg.GUESS_LEARNED
@@ -158,6 +159,7 @@ def test_training_lowers_loss_and_guess_error(pkg, scenes):
     assert last < 0.5 * first
     m.guess_record(0)
     assert m.guess_samples() == nrec * sc.nfib      # stopping keeps the samples
+    m.set_option(pkg.OPT_GUESS_GUARD, 0)            # iterate from the network's guess itself
     zero, learned, err_learned, err_base = [], [], [], []
     for step in range(nrec, nrec + 4):
         r0 = _step(m, pkg, sc, step, apply=False)
@@ -175,4 +177,33 @@ def test_training_lowers_loss_and_guess_error(pkg, scenes):
     print(f"guess error: learned {np.round(err_learned, 4)} M^-1 b {np.round(err_base, 4)}", flush=True)
     print(f"iterations zero {zero} learned {learned}", flush=True)
     assert np.mean(err_learned) < np.mean(err_base) < 1.0
+    m.close()
+
+
+def test_guard_drops_a_guess_worse_than_zero(pkg, scenes, oracle):
+    """A network whose output is far off (a large output bias) gives a guess
+    with |b - A x0|_inf above |b|_inf: the guard replaces it by zero on the
+    device, the report says so, and the step is the oracle's; with the guard
+    off the same guess seeds the solve and the step still converges to it."""
+    sc = scenes.mini(dim=3, seed=7, nfib=8, nlinks=20)
+    m = pkg.Meca(3, 0)
+    pkg.load_scene(m, sc)
+    m.guess_init(2)
+    w = m.guess_weights()
+    w[-pkg.GUESS_OUTPUTS:] = 50.0                  # b4: every output 50 (times the fiber's |rhs|)
+    m.set_guess_weights(w)
+    m.setup(sc.gauss(0))
+    x0, base = m.guess_predict(pkg.PRECOND_FP64)
+    assert not np.any(x0)                          # dropped: the zero guess
+    rep = m.solve(1e-12, sc.gauss(0), tol_mode=pkg.TOL_RELATIVE, guess=pkg.GUESS_LEARNED)
+    assert rep["converged"] and rep["guess_used"] == pkg.GUESS_ZERO
+    orc = oracle_from_scene(oracle, sc)
+    orc.solve(1e-12, sc.gauss(0), tol_mode=1)
+    assert rel_err(m.fiber_points(), orc.fiber_points()) <= 1e-9
+    m.set_option(pkg.OPT_GUESS_GUARD, 0)
+    m.prepare(sc.points)
+    pkg.assemble(m, sc, upload_couples=False)
+    rep = m.solve(1e-12, sc.gauss(0), tol_mode=pkg.TOL_RELATIVE, guess=pkg.GUESS_LEARNED)
+    assert rep["converged"] and rep["guess_used"] == pkg.GUESS_LEARNED
+    assert rel_err(m.fiber_points(), orc.fiber_points()) <= 1e-8
     m.close()

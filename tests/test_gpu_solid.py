@@ -79,6 +79,7 @@ def _aster_steps(pkg, oracle, sc, steps, tol, precondition, guess=0, init_guess=
     gpu.set_option(pkg.OPT_KRYLOV_LAUNCHES, launches)
     if init_guess:
         gpu.guess_init(4)
+        gpu.set_option(pkg.OPT_GUESS_GUARD, 0)
     orc = oracle_from_scene(oracle, sc)
     reps = []
     for s in range(steps):

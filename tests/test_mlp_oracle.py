@@ -169,3 +169,16 @@ def test_guess_inverts_targets(dim):
     m = dim * (7 + 31)
     assert np.allclose(x0[:m], x[:m], rtol=0, atol=1e-6)
     assert np.array_equal(x0[m:], y[m:])             # fibers beyond 31 points keep y
+
+
+def test_targets_beyond_the_clip_are_not_samples():
+    """A fiber whose correction is more than 32 times its load scale gives no
+    training output (valid count 0), so it cannot dominate the squared loss."""
+    nb = np.array([4, 4])
+    x = np.zeros(3 * 8)
+    y = np.zeros(3 * 8)
+    x[2] = 10.0          # fiber 0: 10 / 0.5 = 20 <= 32
+    x[12 + 5] = 40.0     # fiber 1: 40 / 1 = 40 > 32
+    T = mo.targets(3, nb, x, y, np.array([0.5, 1.0], np.float32))
+    assert T[0, -1] == 12 and T[0, 2] == 20.0
+    assert T[1, -1] == 0
