@@ -114,6 +114,10 @@ def parse():
                     help="places of the timed FiberGrid query (0: skip)")
     ap.add_argument("--aster-steps", type=int, default=5,
                     help="timed steps of the aster workload (10 Solids + 2000 fibers, 50000 motors; 0: skip)")
+    ap.add_argument("--tile-threads", type=int, default=0,
+                    help="with --autotune 0: threads per tile of the iteration launches (256 or 512; 0: default)")
+    ap.add_argument("--tile-points", type=int, default=-1,
+                    help="with --autotune 0: target points per tile (-1: default)")
     ap.add_argument("--only", default="all", choices=["all", "c2", "batched", "asters", "grid"],
                     help="run one workload alone (for profiler runs); the JSON line is printed with 'all' or 'c2'")
     return ap.parse_args()
@@ -167,11 +171,34 @@ def errors(pts, sol, ref_pts, ref_sol):
             "displacement_rel": float(np.abs(sol - ref_sol).max() / np.abs(ref_sol).max())}
 
 
+def fixed_tiles(pkg, m, args):
+    """The tile shape given on the command line (profiler runs without the autotuner)."""
+    if args.tile_threads:
+        m.set_option(pkg.OPT_TILE_THREADS, args.tile_threads)
+    if args.tile_points >= 0:
+        m.set_option(pkg.OPT_TILE_POINTS, args.tile_points)
+
+
+def copy_calibration(args, torch):
+    """A plain device-to-device copy of known size, for scaling the HBM byte
+    counters (FETCH_SIZE, WRITE_SIZE) of a profiler run."""
+    if args.copy_calibration_mb <= 0:
+        return
+    nb = args.copy_calibration_mb << 20
+    src = torch.ones(nb // 8, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def run_c2_mode(pkg, sc, mode, args, gauss_dev, torch, weights, ref):
     precond = getattr(pkg, MODES[mode][0])
     guess = getattr(pkg, MODES[mode][1])
     m = pkg.Meca(sc.dim, 0)
     pkg.load_scene(m, sc)
+    fixed_tiles(pkg, m, args)
     tuned = None
     if args.autotune:
         # untimed: the autotuner picks the tile shape and block-inverse method on this system
@@ -298,14 +325,6 @@ def c2(args, pkg, scenes, torch):
     it32_ms = m.time_kernel(6, reps=args.timed_launches)
     it32_bytes = algorithmic_bytes_iteration_launch(sc, kb_full, kc_full, 4 * nblk)
     m.close()
-    if args.copy_calibration_mb > 0:
-        nb = args.copy_calibration_mb << 20
-        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda:0")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
     kern = {"iteration_launch_fp64_ms": it_ms, "iteration_launch_fp64_bytes": int(it_bytes),
             "iteration_launch_fp32_ms": it32_ms, "iteration_launch_fp32_bytes": int(it32_bytes),
             "iteration_launch_fp32_gbs": round(it32_bytes / (it32_ms * 1e-3) / 1e9, 1),
@@ -329,6 +348,7 @@ def batched(args, pkg, scenes, torch):
         guess = getattr(pkg, MODES[mode][1])
         mb = pkg.Meca(3, 0)
         pkg.load_batch(mb, batch)
+        fixed_tiles(pkg, mb, args)
         rec = {}
         if guess == pkg.GUESS_LEARNED:
             if weights is None:
@@ -360,10 +380,11 @@ def batched(args, pkg, scenes, torch):
             if s > 0:
                 its.append(rep["iterations"])
         torch.cuda.synchronize()
-        bdt = time.perf_counter() - t0
-        rec.update({"steps_per_s": round(args.batched_steps / bdt, 3),
-                    "system_steps_per_s": round(batch.nsys * args.batched_steps / bdt, 2),
-                    "max_bicgstab_iters": float(np.mean(its))})
+        if t0 is not None:
+            bdt = time.perf_counter() - t0
+            rec.update({"steps_per_s": round(args.batched_steps / bdt, 3),
+                        "system_steps_per_s": round(batch.nsys * args.batched_steps / bdt, 2),
+                        "max_bicgstab_iters": float(np.mean(its))})
         if guess == pkg.GUESS_ZERO:
             # full-work iteration launch over all 64 systems (every tile iterating)
             mb.setup(gauss_device_ptr=gb[0].data_ptr())
@@ -504,6 +525,7 @@ def main():
     pkg = __graft_entry__.load_package()
     import importlib
     scenes = importlib.import_module(__graft_entry__.PKG_NAME + ".scene")
+    copy_calibration(args, torch)
     if args.only == "batched":
         batched(args, pkg, scenes, torch)
         return
