@@ -30,6 +30,10 @@ EXPORTED = [
     "cymg_prepare", "cymg_add_pure_force",
     "cymg_inter_link_ee", "cymg_inter_link_ie", "cymg_inter_link_ii", "cymg_inter_long_link_ee",
     "cymg_inter_clamp_e", "cymg_inter_clamp_i", "cymg_inter_long_clamp", "cymg_inter_plane_e",
+    "cymg_add_pure_force_i", "cymg_inter_plane_i", "cymg_inter_long_link_ie", "cymg_inter_long_link_ii",
+    "cymg_inter_side_link_ie", "cymg_inter_side_link_ii", "cymg_inter_side_side_link", "cymg_inter_sliding_link_ie",
+    "cymg_inter_sliding_link_ii", "cymg_inter_side_sliding_link_ie", "cymg_inter_side_sliding_link_ii",
+    "cymg_inter_torque", "cymg_inter_side_clamp", "cymg_inter_sliding_clamp",
     "cymg_confine_sphere", "cymg_set_couples", "cymg_couples_interactions", "cymg_set_couple_state",
     "cymg_get_couples", "cymg_step_couples", "cymg_nearby_segments", "cymg_nearby_segments_noalloc", "cymg_set_aster_links",
     "cymg_aster_interactions", "cymg_solve",
@@ -144,6 +148,20 @@ def lib():
         "cymg_inter_clamp_i": (C.c_int, [vp, u32, u32, C.c_double, dp, C.c_double]),
         "cymg_inter_long_clamp": (C.c_int, [vp, u32, dp, C.c_double, C.c_double]),
         "cymg_inter_plane_e": (C.c_int, [vp, u32, dp, dp, C.c_double]),
+        "cymg_add_pure_force_i": (C.c_int, [vp, u32, u32, C.c_double, dp]),
+        "cymg_inter_plane_i": (C.c_int, [vp, u32, u32, C.c_double, dp, dp, C.c_double]),
+        "cymg_inter_long_link_ie": (C.c_int, [vp, u32, u32, C.c_double, u32, C.c_double, C.c_double]),
+        "cymg_inter_long_link_ii": (C.c_int, [vp, u32, u32, C.c_double, u32, u32, C.c_double, C.c_double, C.c_double]),
+        "cymg_inter_side_link_ie": (C.c_int, [vp, u32, u32, C.c_double, u32, C.c_double, C.c_double]),
+        "cymg_inter_side_link_ii": (C.c_int, [vp, u32, u32, C.c_double, u32, u32, C.c_double, C.c_double, C.c_double]),
+        "cymg_inter_side_side_link": (C.c_int, [vp, u32, u32, C.c_double, u32, u32, C.c_double, C.c_double, C.c_double]),
+        "cymg_inter_sliding_link_ie": (C.c_int, [vp, u32, u32, C.c_double, u32, C.c_double]),
+        "cymg_inter_sliding_link_ii": (C.c_int, [vp, u32, u32, C.c_double, u32, u32, C.c_double, C.c_double]),
+        "cymg_inter_side_sliding_link_ie": (C.c_int, [vp, u32, u32, C.c_double, u32, C.c_double, C.c_double]),
+        "cymg_inter_side_sliding_link_ii": (C.c_int, [vp, u32, u32, C.c_double, u32, u32, C.c_double, C.c_double, C.c_double]),
+        "cymg_inter_torque": (C.c_int, [vp, u32, u32, C.c_double, u32, u32, C.c_double, C.c_double, C.c_double, C.c_double]),
+        "cymg_inter_side_clamp": (C.c_int, [vp, u32, u32, C.c_double, dp, C.c_double, C.c_double]),
+        "cymg_inter_sliding_clamp": (C.c_int, [vp, u32, u32, C.c_double, dp, C.c_double]),
         "cymg_confine_sphere": (C.c_int, [vp, C.c_double, C.c_double]),
         "cymg_set_couples": (C.c_int, [vp, u32, vp, vp, vp, vp, vp]),
         "cymg_couples_interactions": (C.c_int, [vp]),
@@ -350,6 +368,70 @@ class Meca:
     def add_pure_force(self, a, force):
         fp, keep = _vec3(force, self.dim)
         self._chk(self.L.cymg_add_pure_force(self.h, a, fp))
+
+    def add_pure_force_i(self, a1, a2, ca, force):
+        """Meca::addPureForce(PointInterpolated) (meca_inter.cc:140-156)"""
+        fp, keep = _vec3(force, self.dim)
+        self._chk(self.L.cymg_add_pure_force_i(self.h, a1, a2, ca, fp))
+
+    def inter_plane_i(self, a1, a2, ca, direction, g, w):
+        """Meca::interPlane(PointInterpolated) (meca_inter.cc:2340-2378)"""
+        dp, k1 = _vec3(direction, self.dim)
+        gp, k2 = _vec3(g, self.dim)
+        self._chk(self.L.cymg_inter_plane_i(self.h, a1, a2, ca, dp, gp, w))
+
+    # Interactions that read the positions: evaluated on the device from the
+    # points of prepare() when the matrices are built.
+
+    def inter_long_link_ie(self, a1, a2, ca, b, length, w):
+        """Meca::interLongLink(PointInterpolated, PointExact) (meca_inter.cc:425-505)"""
+        self._chk(self.L.cymg_inter_long_link_ie(self.h, a1, a2, ca, b, length, w))
+
+    def inter_long_link_ii(self, a1, a2, ca, b1, b2, cb, length, w):
+        """Meca::interLongLink(PointInterpolated, PointInterpolated) (meca_inter.cc:516-596)"""
+        self._chk(self.L.cymg_inter_long_link_ii(self.h, a1, a2, ca, b1, b2, cb, length, w))
+
+    def inter_side_link_ie(self, a1, a2, ca, b, length, w):
+        """Meca::interSideLink(PointInterpolated, PointExact) (meca_inter.cc:617-759)"""
+        self._chk(self.L.cymg_inter_side_link_ie(self.h, a1, a2, ca, b, length, w))
+
+    def inter_side_link_ii(self, a1, a2, ca, b1, b2, cb, length, w):
+        """Meca::interSideLink(PointInterpolated, PointInterpolated) (meca_inter.cc:764-935)"""
+        self._chk(self.L.cymg_inter_side_link_ii(self.h, a1, a2, ca, b1, b2, cb, length, w))
+
+    def inter_side_side_link(self, a1, a2, ca, b1, b2, cb, length, w):
+        """Meca::interSideSideLink (meca_inter.cc:943-1059), 2D only"""
+        self._chk(self.L.cymg_inter_side_side_link(self.h, a1, a2, ca, b1, b2, cb, length, w))
+
+    def inter_sliding_link_ie(self, a1, a2, ca, b, w):
+        """Meca::interSlidingLink(PointInterpolated, PointExact) (meca_inter.cc:1075-1140)"""
+        self._chk(self.L.cymg_inter_sliding_link_ie(self.h, a1, a2, ca, b, w))
+
+    def inter_sliding_link_ii(self, a1, a2, ca, b1, b2, cb, w):
+        """Meca::interSlidingLink(PointInterpolated, PointInterpolated) (meca_inter.cc:1153-1258)"""
+        self._chk(self.L.cymg_inter_sliding_link_ii(self.h, a1, a2, ca, b1, b2, cb, w))
+
+    def inter_side_sliding_link_ie(self, a1, a2, ca, b, length, w):
+        """Meca::interSideSlidingLink(PointInterpolated, PointExact) (meca_inter.cc:1266-1499)"""
+        self._chk(self.L.cymg_inter_side_sliding_link_ie(self.h, a1, a2, ca, b, length, w))
+
+    def inter_side_sliding_link_ii(self, a1, a2, ca, b1, b2, cb, length, w):
+        """Meca::interSideSlidingLink(PointInterpolated, PointInterpolated) (meca_inter.cc:1508-1751)"""
+        self._chk(self.L.cymg_inter_side_sliding_link_ii(self.h, a1, a2, ca, b1, b2, cb, length, w))
+
+    def inter_torque(self, a1, a2, ca, b1, b2, cb, cosinus, sinus, w):
+        """Meca::interTorque2D (meca_inter.cc:1772-1869), 2D only"""
+        self._chk(self.L.cymg_inter_torque(self.h, a1, a2, ca, b1, b2, cb, cosinus, sinus, w))
+
+    def inter_side_clamp(self, a1, a2, ca, g, length, w):
+        """Meca::interSideClamp (meca_inter.cc:2032-2226)"""
+        gp, keep = _vec3(g, self.dim)
+        self._chk(self.L.cymg_inter_side_clamp(self.h, a1, a2, ca, gp, length, w))
+
+    def inter_sliding_clamp(self, a1, a2, ca, g, w):
+        """Meca::interSlidingClamp (meca_inter.cc:2241-2283)"""
+        gp, keep = _vec3(g, self.dim)
+        self._chk(self.L.cymg_inter_sliding_clamp(self.h, a1, a2, ca, gp, w))
 
     def confine_sphere(self, radius, stiffness):
         self._chk(self.L.cymg_confine_sphere(self.h, radius, stiffness))

@@ -708,6 +708,7 @@ int cymg_prepare(cymg_context* ctx, const double* points)
         }
         c.hB.clear(); c.hC.clear(); c.hBas.clear();
         c.marks.clear();
+        c.h_def.clear();
         launch_prepare(c);
         c.prepared = true;
         c.setup_done = false;
@@ -773,48 +774,78 @@ int cymg_inter_link_ii(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, u
     });
 }
 
-static void fetch_points(Context& c, uint32_t a, double* x)
+// A position-dependent emitter is evaluated on the device when the streams are
+// built (emit.hip); consecutive calls share one mark, placed after the host
+// entries recorded before the first of them.
+static void defer(Context& c, DeferredCall q)
 {
-    CYMG_HIP(hipMemcpyAsync(x, c.d_pos.get() + (size_t)c.dim * a, c.dim * 8, hipMemcpyDeviceToHost, c.stream));
-    CYMG_HIP(hipStreamSynchronize(c.stream));
+    uint32_t s[3];
+    deferred_slots(q.kind, c.dim, s);
+    const size_t pos[3] = { c.hB.key.size(), c.hC.key.size(), c.hBas.key.size() };
+    if (c.marks.empty() || c.marks.back().kind != 4 || c.marks.back().posB != pos[0] ||
+        c.marks.back().posC != pos[1] || c.marks.back().posBas != pos[2]) {
+        EmitterMark mk{ 4, 0.0, 0.0, pos[0], pos[1], pos[2] };
+        mk.first = (uint32_t)c.h_def.size();
+        c.marks.push_back(mk);
+    }
+    EmitterMark& mk = c.marks.back();
+    for (int w = 0; w < 3; ++w) {
+        if (mk.slots[w] + s[w] > 0xFFFFFFFFull) throw Error(CYMG_ERR_UNSUPPORTED, "too many deferred interactions");
+        q.off[w] = (uint32_t)mk.slots[w];
+        mk.slots[w] += s[w];
+    }
+    c.h_def.push_back(q);
+    ++mk.count;
+}
+
+static DeferredCall call_of(uint32_t kind, std::initializer_list<uint32_t> pts, double ca, double cb, double len,
+                            double w, const double* g = nullptr, int ng = 0)
+{
+    DeferredCall q{};
+    q.kind = kind;
+    int k = 0;
+    for (uint32_t p : pts) q.p[k++] = p;
+    q.ca = ca;
+    q.cb = cb;
+    q.len = len;
+    q.w = w;
+    for (int d = 0; d < ng; ++d) q.g[d] = g[d];
+    return q;
+}
+
+// PointInterpolated: two points of one Mecable (point_interpolated.h:33-60)
+static void check_interpolated(const Context& c, uint32_t p1, uint32_t p2)
+{
+    check_point(c, p1);
+    check_point(c, p2);
+    if (p1 == p2 || mecable_of(c, p1) != mecable_of(c, p2))
+        throw Error(CYMG_ERR_INVALID, "an interpolated point needs two distinct points of one Mecable");
+}
+
+static void check_weight(double w)
+{
+    if (!(w >= 0)) throw Error(CYMG_ERR_INVALID, "weight must be >= 0");
+}
+
+// PointInterpolated::overlapping (point_interpolated.cc:21-33)
+static bool overlapping(const Context& c, uint32_t a1, uint32_t a2, uint32_t b)
+{
+    return mecable_of(c, a1) == mecable_of(c, b) && (a1 == b || a2 == b);
+}
+
+static bool overlapping(const Context& c, uint32_t a1, uint32_t a2, uint32_t b1, uint32_t b2)
+{
+    return mecable_of(c, a1) == mecable_of(c, b1) && (a1 == b1 || a1 == b2 || a2 == b1 || a2 == b2);
 }
 
 int cymg_inter_long_link_ee(cymg_context* ctx, uint32_t a, uint32_t b, double len, double w)
 {
     return guarded(ctx, [&](Context& c) {
         need_prepared(c); check_point(c, a); check_point(c, b);
-        if (!(w >= 0)) throw Error(CYMG_ERR_INVALID, "weight must be >= 0");
+        check_weight(w);
         same_system(c, { a, b });
         if (mecable_of(c, a) == mecable_of(c, b) && (a == b || a == b + 1 || a + 1 == b)) return;
-        const int D = c.dim;
-        double pa[3], pb[3], ab[3] = { 0, 0, 0 };
-        fetch_points(c, a, pa);
-        fetch_points(c, b, pb);
-        double abn = 0;
-        for (int d = 0; d < D; ++d) { ab[d] = pb[d] - pa[d]; abn += ab[d] * ab[d]; }
-        abn = std::sqrt(abn);
-        if (abn < 128.0 * 2.220446049250313e-16) return;
-        for (int d = 0; d < D; ++d) ab[d] /= abn;
-        const uint32_t iA = D * a, iB = D * b;
-        for (int d = 0; d < D; ++d) {
-            addBas(c, iA + d, -(w * ab[d] * len));
-            addBas(c, iB + d, w * ab[d] * len);
-        }
-        len /= abn;
-        const bool cooked = len > 1.0;
-        for (int i = 0; i < D; ++i) {
-            double v = cooked ? w * ab[i] * ab[i] : w * (1.0 + len * (ab[i] * ab[i] - 1.0));
-            addC(c, iA + i, iA + i, -v);
-            addC(c, iA + i, iB + i, v);
-            addC(c, iB + i, iB + i, -v);
-            for (int j = i + 1; j < D; ++j) {
-                v = cooked ? w * ab[i] * ab[j] : w * len * ab[i] * ab[j];
-                addC(c, iA + i, iA + j, -v);
-                addC(c, iA + i, iB + j, v);
-                addC(c, iA + j, iB + i, v);
-                addC(c, iB + i, iB + j, -v);
-            }
-        }
+        defer(c, call_of(kDefLongLinkEE, { a, b }, 0.0, 0.0, len, w));
     });
 }
 
@@ -847,33 +878,8 @@ int cymg_inter_long_clamp(cymg_context* ctx, uint32_t a, const double* center, d
 {
     return guarded(ctx, [&](Context& c) {
         need_prepared(c); check_point(c, a);
-        const int D = c.dim;
-        const uint32_t inx = D * a;
-        double x[3], axis[3] = { 0, 0, 0 };
-        fetch_points(c, a, x);
-        double an = 0;
-        for (int d = 0; d < D; ++d) { axis[d] = x[d] - center[d]; an += axis[d] * axis[d]; }
-        an = std::sqrt(an);
-        if (an < 128.0 * 2.220446049250313e-16) return;
-        for (int d = 0; d < D; ++d) axis[d] /= an;
-        double ac = 0;
-        for (int d = 0; d < D; ++d) ac += axis[d] * center[d];
-        if (len < an) {
-            len /= an;
-            const double facX = w * len * (an + ac);
-            const double facC = w * (1.0 - len);
-            for (int i = 0; i < D; ++i) {
-                addC(c, inx + i, inx + i, w * (len * (1.0 - axis[i] * axis[i]) - 1.0));
-                for (int j = i + 1; j < D; ++j) addC(c, inx + i, inx + j, -(w * len * axis[i] * axis[j]));
-                addBas(c, inx + i, facX * axis[i] + facC * center[i]);
-            }
-        } else {
-            const double facX = w * (len + ac);
-            for (int i = 0; i < D; ++i) {
-                for (int j = i; j < D; ++j) addC(c, inx + i, inx + j, -(w * axis[i] * axis[j]));
-                addBas(c, inx + i, facX * axis[i]);
-            }
-        }
+        if (!center) throw Error(CYMG_ERR_INVALID, "null center");
+        defer(c, call_of(kDefLongClamp, { a }, 0.0, 0.0, len, w, center, c.dim));
     });
 }
 
@@ -890,6 +896,157 @@ int cymg_inter_plane_e(cymg_context* ctx, uint32_t a, const double* dir, const d
             for (int j = i; j < D; ++j) addC(c, inx + i, inx + j, -(dir[i] * dir[j] * w));
             addBas(c, inx + i, pr * dir[i]);
         }
+    });
+}
+
+int cymg_add_pure_force_i(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, const double* force)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_interpolated(c, a1, a2);
+        if (!force) throw Error(CYMG_ERR_INVALID, "null force");
+        const double a = ca, b = 1.0 - ca;
+        for (int d = 0; d < c.dim; ++d) {
+            addBas(c, c.dim * a1 + d, b * force[d]);
+            addBas(c, c.dim * a2 + d, a * force[d]);
+        }
+    });
+}
+
+int cymg_inter_plane_i(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, const double* dir, const double* g,
+                       double w)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_interpolated(c, a1, a2);
+        if (!dir || !g) throw Error(CYMG_ERR_INVALID, "null vector");
+        const int D = c.dim;
+        const double aa = ca, bb = 1.0 - ca;
+        const double bbbb = bb * bb, aaaa = aa * aa, aabb = aa * bb;
+        const uint32_t i1 = D * a1, i2 = D * a2;
+        for (int x = 0; x < D; ++x) {
+            for (int y = x; y < D; ++y) {
+                const double P = w * dir[x] * dir[y];
+                addC(c, i1 + x, i1 + y, -(bbbb * P));
+                addC(c, i2 + x, i2 + y, -(aaaa * P));
+                addC(c, i1 + x, i2 + y, -(aabb * P));
+                if (x != y) addC(c, i1 + y, i2 + x, -(aabb * P));
+            }
+        }
+        double gd = g[0] * dir[0];
+        for (int d = 1; d < D; ++d) gd = gd + g[d] * dir[d];
+        const double pr = w * gd;
+        for (int x = 0; x < D; ++x) {
+            addBas(c, i1 + x, pr * bb * dir[x]);
+            addBas(c, i2 + x, pr * aa * dir[x]);
+        }
+    });
+}
+
+// interpolated-to-exact and interpolated-to-interpolated emitters evaluated
+// from the device points
+static int defer_ie(cymg_context* ctx, uint32_t kind, uint32_t a1, uint32_t a2, double ca, uint32_t b, double len,
+                    double w)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_interpolated(c, a1, a2); check_point(c, b);
+        check_weight(w);
+        same_system(c, { a1, a2, b });
+        if (overlapping(c, a1, a2, b)) return;
+        defer(c, call_of(kind, { a1, a2, b }, ca, 0.0, len, w));
+    });
+}
+
+static int defer_ii(cymg_context* ctx, uint32_t kind, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2,
+                    double cb, double len, double w, const double* g = nullptr, bool plane_only = false)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_interpolated(c, a1, a2); check_interpolated(c, b1, b2);
+        check_weight(w);
+        if (plane_only && c.dim != 2) {
+            throw Error(CYMG_ERR_UNSUPPORTED, kind == kDefTorque ? "interTorque2D exists in 2D only"
+                                                                  : "interSideSideLink() was not implemented in 3D");
+        }
+        same_system(c, { a1, a2, b1, b2 });
+        if (overlapping(c, a1, a2, b1, b2)) return;
+        defer(c, call_of(kind, { a1, a2, b1, b2 }, ca, cb, len, w, g, g ? 2 : 0));
+    });
+}
+
+int cymg_inter_long_link_ie(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b, double len, double w)
+{
+    return defer_ie(ctx, kDefLongLinkIE, a1, a2, ca, b, len, w);
+}
+
+int cymg_inter_long_link_ii(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2,
+                            double cb, double len, double w)
+{
+    return defer_ii(ctx, kDefLongLinkII, a1, a2, ca, b1, b2, cb, len, w);
+}
+
+int cymg_inter_side_link_ie(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b, double len, double w)
+{
+    return defer_ie(ctx, kDefSideLinkIE, a1, a2, ca, b, len, w);
+}
+
+int cymg_inter_side_link_ii(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2,
+                            double cb, double len, double w)
+{
+    return defer_ii(ctx, kDefSideLinkII, a1, a2, ca, b1, b2, cb, len, w);
+}
+
+int cymg_inter_side_side_link(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2,
+                              double cb, double len, double w)
+{
+    return defer_ii(ctx, kDefSideSideLink, a1, a2, ca, b1, b2, cb, len, w, nullptr, true);
+}
+
+int cymg_inter_sliding_link_ie(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b, double w)
+{
+    return defer_ie(ctx, kDefSlidingLinkIE, a1, a2, ca, b, 0.0, w);
+}
+
+int cymg_inter_sliding_link_ii(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2,
+                               double cb, double w)
+{
+    return defer_ii(ctx, kDefSlidingLinkII, a1, a2, ca, b1, b2, cb, 0.0, w);
+}
+
+int cymg_inter_side_sliding_link_ie(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b, double len,
+                                    double w)
+{
+    return defer_ie(ctx, kDefSideSlidingIE, a1, a2, ca, b, len, w);
+}
+
+int cymg_inter_side_sliding_link_ii(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b1,
+                                    uint32_t b2, double cb, double len, double w)
+{
+    return defer_ii(ctx, kDefSideSlidingII, a1, a2, ca, b1, b2, cb, len, w);
+}
+
+int cymg_inter_torque(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2, double cb,
+                      double cosinus, double sinus, double w)
+{
+    const double cs[2] = { cosinus, sinus };
+    return defer_ii(ctx, kDefTorque, a1, a2, ca, b1, b2, cb, 0.0, w, cs, true);
+}
+
+int cymg_inter_side_clamp(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, const double* g, double len,
+                          double w)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_interpolated(c, a1, a2);
+        check_weight(w);
+        if (!g) throw Error(CYMG_ERR_INVALID, "null position");
+        defer(c, call_of(kDefSideClamp, { a1, a2 }, ca, 0.0, len, w, g, c.dim));
+    });
+}
+
+int cymg_inter_sliding_clamp(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, const double* g, double w)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c); check_interpolated(c, a1, a2);
+        check_weight(w);
+        if (!g) throw Error(CYMG_ERR_INVALID, "null position");
+        defer(c, call_of(kDefSlidingClamp, { a1, a2 }, ca, 0.0, 0.0, w, g, c.dim));
     });
 }
 

@@ -340,6 +340,13 @@ void build_streams(Context& c)
         if (mk.kind == 1) { total[1] += NC * c.fib_pts; total[2] += (size_t)D * c.fib_pts; }
         if (mk.kind == 2) { total[0] += (size_t)10 * c.ncouples; }
         if (mk.kind == 3) { total[0] += (size_t)6 * c.nasters; }
+        if (mk.kind == 4)
+            for (int w = 0; w < 3; ++w) total[w] += mk.slots[w];
+    }
+    if (!c.h_def.empty()) {
+        c.d_def.resize(c.h_def.size());
+        CYMG_HIP(hipMemcpyAsync(c.d_def.get(), c.h_def.data(), c.h_def.size() * sizeof(DeferredCall),
+                                hipMemcpyHostToDevice, c.stream));
     }
     for (int w = 0; w < 3; ++w) {
         c.s_total[w] = total[w];
@@ -374,6 +381,11 @@ void build_streams(Context& c)
         } else if (mk.kind == 3) {
             launch_emit_asters(c, c.s_key[0].get() + dpos[0], c.s_val[0].get() + dpos[0]);
             dpos[0] += (size_t)6 * c.nasters;
+        } else if (mk.kind == 4) {
+            launch_emit_deferred(c, mk, c.s_key[0].get() + dpos[0], c.s_val[0].get() + dpos[0],
+                                 c.s_key[1].get() + dpos[1], c.s_val[1].get() + dpos[1], c.s_key[2].get() + dpos[2],
+                                 c.s_val[2].get() + dpos[2]);
+            for (int w = 0; w < 3; ++w) dpos[w] += mk.slots[w];
         }
     }
     for (int w = 0; w < 3; ++w) copy_host(w, hs[w]->key.size());
@@ -391,8 +403,9 @@ void build_streams(Context& c)
         conf_only[2] = one && hs[2]->key.empty() && total[2] == (size_t)D * c.fib_pts;
     }
 
-    // what each stream's keys are built from; a confinement emitter's keys
-    // follow the positions, so mC and vBAS are rebuilt whenever it contributes
+    // what each stream's keys are built from; the keys of a confinement
+    // emitter and of deferred calls follow the positions, so a stream they
+    // contribute to is rebuilt
     DevSym* all[3] = { &c.mB, &c.mC, &c.mBas };
     bool reuse[3];
     for (int w = 0; w < 3; ++w) {
@@ -402,6 +415,7 @@ void build_streams(Context& c)
             sig.push_back((uint64_t)mk.kind);
             sig.push_back(w == 0 ? mk.posB : (w == 1 ? mk.posC : mk.posBas));
             if (mk.kind == 1 && w != 0) cacheable = false;
+            if (mk.kind == 4 && mk.slots[w]) cacheable = false;
             if (mk.kind == 2) { sig.push_back(c.couples_gen); sig.push_back(c.ncouples); }
             if (mk.kind == 3) { sig.push_back(c.asters_gen); sig.push_back(c.nasters); }
         }

@@ -100,10 +100,78 @@ struct HostStream {
 // device emitter placed in call order: the host-recorded entries before it are
 // those below its recorded positions in each host stream
 struct EmitterMark {
-    int kind;                     // 1 sphere confinement, 2 couples, 3 aster links
+    int kind;                     // 1 sphere confinement, 2 couples, 3 aster links, 4 deferred calls
     double a, b;                  // emitter parameters (radius, stiffness)
     size_t posB, posC, posBas;    // host stream sizes when the emitter was called
+    uint32_t first = 0, count = 0;        // deferred calls [first, first+count) of Context::h_def
+    size_t slots[3] = { 0, 0, 0 };        // their stream entries (B, C, vBAS)
 };
+
+// An interaction whose entries depend on the positions at assembly time
+// (meca_inter.cc), recorded by the host in call order and evaluated on the
+// device from the points when the streams are built: every call owns a fixed
+// number of entries in each stream (deferred_slots), written in the order of
+// the reference's contributions; entries it does not make keep an invalid key.
+enum DeferredKind : uint32_t {
+    kDefLongLinkEE = 0,   // interLongLink(PointExact, PointExact)
+    kDefLongClamp,        // interLongClamp
+    kDefLongLinkIE,       // interLongLink(PointInterpolated, PointExact)
+    kDefLongLinkII,       // interLongLink(PointInterpolated, PointInterpolated)
+    kDefSideLinkIE,       // interSideLink(PointInterpolated, PointExact)
+    kDefSideLinkII,       // interSideLink(PointInterpolated, PointInterpolated)
+    kDefSideSideLink,     // interSideSideLink (2D)
+    kDefSlidingLinkIE,    // interSlidingLink(PointInterpolated, PointExact)
+    kDefSlidingLinkII,    // interSlidingLink(PointInterpolated, PointInterpolated)
+    kDefSideSlidingIE,    // interSideSlidingLink(PointInterpolated, PointExact)
+    kDefSideSlidingII,    // interSideSlidingLink(PointInterpolated, PointInterpolated)
+    kDefTorque,           // interTorque2D
+    kDefSideClamp,        // interSideClamp
+    kDefSlidingClamp,     // interSlidingClamp
+    kDefKinds
+};
+
+struct DeferredCall {
+    uint32_t kind;
+    uint32_t p[4];        // model points: a1 a2 b1 b2 (interpolated), a b (exact)
+    uint32_t off[3];      // first entry in the B, C and vBAS ranges of its mark
+    double ca, cb;        // interpolation coefficients (PointInterpolated::mCoef)
+    double len, w;        // resting length / arm, weight
+    double g[3];          // fixed position (clamps, long clamp center); cosinus, sinus (torque)
+};
+
+// entries of one deferred call in the B, C and vBAS streams
+__host__ __device__ inline void deferred_slots(uint32_t kind, int D, uint32_t s[3])
+{
+    const uint32_t sym = D * (D + 1) / 2, off = D * (D - 1) / 2;
+    s[0] = s[1] = s[2] = 0;
+    switch (kind) {
+    case kDefLongLinkEE: s[1] = 3 * D + 4 * off; s[2] = 2 * D; break;
+    case kDefLongClamp: s[1] = sym; s[2] = D; break;
+    case kDefLongLinkIE: s[1] = 6 * D + 9 * off; s[2] = 3 * D; break;
+    case kDefLongLinkII: s[1] = 10 * D + 16 * off; s[2] = 4 * D; break;
+    case kDefSideLinkIE:
+        if (D == 2) { s[0] = 6; s[1] = 6; } else { s[1] = 45; s[2] = 9; }
+        break;
+    case kDefSideLinkII:
+        if (D == 2) { s[0] = 10; s[1] = 10; } else { s[1] = 78; s[2] = 12; }
+        break;
+    case kDefSideSideLink: s[0] = 10; s[1] = 12; break;
+    case kDefSlidingLinkIE: s[1] = 6 * sym + 3 * off; break;
+    case kDefSlidingLinkII: s[1] = D == 2 ? 36 : 78; break;
+    case kDefSideSlidingIE:
+        if (D == 2) { s[1] = 21; } else { s[1] = 45; s[2] = 9; }
+        break;
+    case kDefSideSlidingII:
+        if (D == 2) { s[1] = 36; } else { s[1] = 78; s[2] = 12; }
+        break;
+    case kDefTorque: s[1] = 36; s[2] = 72; break;
+    case kDefSideClamp:
+        if (D == 2) { s[0] = 3; s[1] = 2; s[2] = 4; } else { s[1] = 21; s[2] = 6; }
+        break;
+    case kDefSlidingClamp: s[1] = 3 * sym + off; s[2] = 2 * D; break;
+    default: break;
+    }
+}
 
 // symmetric sparse matrix on the device, built from a stream
 struct DevSym {
@@ -143,6 +211,9 @@ void launch_emit_confine(Context& c, uint64_t* keyC, double* valC, uint64_t* key
 void launch_emit_couples(Context& c, uint64_t* keyB, double* valB);
 void launch_step_couples(Context& c, const MotorStep& ms);
 void launch_emit_asters(Context& c, uint64_t* keyB, double* valB);
+// launchers (emit.hip)
+void launch_emit_deferred(Context& c, const EmitterMark& mk, uint64_t* keyB, double* valB, uint64_t* keyC,
+                          double* valC, uint64_t* keyBas, double* valBas);
 void launch_setup(Context& c, const double* gauss);
 void launch_matvec(Context& c, const double* X, double* Y);
 void launch_forces(Context& c, const double* X, double* Y, bool with_rigidity);

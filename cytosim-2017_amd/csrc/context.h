@@ -75,6 +75,8 @@ struct Context {
     // assembly streams (mB, mC, vBAS) in call order
     HostStream hB, hC, hBas;
     std::vector<EmitterMark> marks;
+    std::vector<DeferredCall> h_def;   // position-dependent emitters, in call order
+    DevBuf<DeferredCall> d_def;
     DevBuf<uint64_t> s_key[3];         // stream inputs (B, C, vBAS) in call order
     DevBuf<double> s_val[3];
     size_t s_total[3] = {0, 0, 0};

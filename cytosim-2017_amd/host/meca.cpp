@@ -103,6 +103,77 @@ void Meca::addPureForce(const PointExact& a, const double force[])
     check(cymg_add_pure_force(ctx_, a.mat, force));
 }
 
+void Meca::addPureForce(const PointInterpolated& a, const double force[])
+{
+    check(cymg_add_pure_force_i(ctx_, a.mat1, a.mat2, a.coef, force));
+}
+
+void Meca::interPlane(const PointInterpolated& a, const double dir[], const double g[], double w)
+{
+    check(cymg_inter_plane_i(ctx_, a.mat1, a.mat2, a.coef, dir, g, w));
+}
+
+void Meca::interLongLink(const PointInterpolated& a, const PointExact& b, double len, double w)
+{
+    check(cymg_inter_long_link_ie(ctx_, a.mat1, a.mat2, a.coef, b.mat, len, w));
+}
+
+void Meca::interLongLink(const PointInterpolated& a, const PointInterpolated& b, double len, double w)
+{
+    check(cymg_inter_long_link_ii(ctx_, a.mat1, a.mat2, a.coef, b.mat1, b.mat2, b.coef, len, w));
+}
+
+void Meca::interSideLink(const PointInterpolated& a, const PointExact& b, double len, double w)
+{
+    check(cymg_inter_side_link_ie(ctx_, a.mat1, a.mat2, a.coef, b.mat, len, w));
+}
+
+void Meca::interSideLink(const PointInterpolated& a, const PointInterpolated& b, double len, double w)
+{
+    check(cymg_inter_side_link_ii(ctx_, a.mat1, a.mat2, a.coef, b.mat1, b.mat2, b.coef, len, w));
+}
+
+void Meca::interSideSideLink(const PointInterpolated& a, const PointInterpolated& b, double len, double w)
+{
+    check(cymg_inter_side_side_link(ctx_, a.mat1, a.mat2, a.coef, b.mat1, b.mat2, b.coef, len, w));
+}
+
+void Meca::interSlidingLink(const PointInterpolated& a, const PointExact& b, double w)
+{
+    check(cymg_inter_sliding_link_ie(ctx_, a.mat1, a.mat2, a.coef, b.mat, w));
+}
+
+void Meca::interSlidingLink(const PointInterpolated& a, const PointInterpolated& b, double w)
+{
+    check(cymg_inter_sliding_link_ii(ctx_, a.mat1, a.mat2, a.coef, b.mat1, b.mat2, b.coef, w));
+}
+
+void Meca::interSideSlidingLink(const PointInterpolated& a, const PointExact& b, double len, double w)
+{
+    check(cymg_inter_side_sliding_link_ie(ctx_, a.mat1, a.mat2, a.coef, b.mat, len, w));
+}
+
+void Meca::interSideSlidingLink(const PointInterpolated& a, const PointInterpolated& b, double len, double w)
+{
+    check(cymg_inter_side_sliding_link_ii(ctx_, a.mat1, a.mat2, a.coef, b.mat1, b.mat2, b.coef, len, w));
+}
+
+void Meca::interTorque2D(const PointInterpolated& a, const PointInterpolated& b, double cosinus, double sinus,
+                         double w)
+{
+    check(cymg_inter_torque(ctx_, a.mat1, a.mat2, a.coef, b.mat1, b.mat2, b.coef, cosinus, sinus, w));
+}
+
+void Meca::interSideClamp(const PointInterpolated& a, const double g[], double len, double w)
+{
+    check(cymg_inter_side_clamp(ctx_, a.mat1, a.mat2, a.coef, g, len, w));
+}
+
+void Meca::interSlidingClamp(const PointInterpolated& a, const double g[], double w)
+{
+    check(cymg_inter_sliding_clamp(ctx_, a.mat1, a.mat2, a.coef, g, w));
+}
+
 void Meca::confineSphere(double radius, double stiffness)
 {
     check(cymg_confine_sphere(ctx_, radius, stiffness));

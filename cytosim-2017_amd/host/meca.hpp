@@ -101,6 +101,22 @@ public:
     void interLongClamp(const PointExact& a, const double center[], double len, double weight);
     void interPlane(const PointExact& a, const double dir[], const double g[], double weight);
     void addPureForce(const PointExact& a, const double force[]);
+    void addPureForce(const PointInterpolated& a, const double force[]);
+    void interPlane(const PointInterpolated& a, const double dir[], const double g[], double weight);
+    // read the positions: evaluated on the device when the matrices are built
+    void interLongLink(const PointInterpolated& a, const PointExact& b, double len, double weight);
+    void interLongLink(const PointInterpolated& a, const PointInterpolated& b, double len, double weight);
+    void interSideLink(const PointInterpolated& a, const PointExact& b, double len, double weight);
+    void interSideLink(const PointInterpolated& a, const PointInterpolated& b, double len, double weight);
+    void interSideSideLink(const PointInterpolated& a, const PointInterpolated& b, double len, double weight);
+    void interSlidingLink(const PointInterpolated& a, const PointExact& b, double weight);
+    void interSlidingLink(const PointInterpolated& a, const PointInterpolated& b, double weight);
+    void interSideSlidingLink(const PointInterpolated& a, const PointExact& b, double len, double weight);
+    void interSideSlidingLink(const PointInterpolated& a, const PointInterpolated& b, double len, double weight);
+    void interTorque2D(const PointInterpolated& a, const PointInterpolated& b, double cosinus, double sinus,
+                       double weight);
+    void interSideClamp(const PointInterpolated& a, const double g[], double len, double weight);
+    void interSlidingClamp(const PointInterpolated& a, const double g[], double weight);
     // device-side emitters for the common cases
     void confineSphere(double radius, double stiffness);          // Fiber::setInteractions + SpaceSphere
     void setCouples(const std::vector<uint32_t>& fa, const std::vector<double>& aa,

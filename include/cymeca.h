@@ -157,15 +157,56 @@ int         cymg_inter_link_ie(cymg_context* ctx, uint32_t a1, uint32_t a2, doub
                                uint32_t b, double weight);                                     /* meca_inter.cc:240-275 */
 int         cymg_inter_link_ii(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca,
                                uint32_t b1, uint32_t b2, double cb, double weight);           /* meca_inter.cc:285-321 */
+/* evaluated on the device from the points, as the interactions below */
 int         cymg_inter_long_link_ee(cymg_context* ctx, uint32_t a, uint32_t b, double len,
                                     double weight);                                            /* meca_inter.cc:334-415 */
 int         cymg_inter_clamp_e(cymg_context* ctx, uint32_t a, const double* g, double weight); /* meca_inter.cc:1883-1907 */
 int         cymg_inter_clamp_i(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca,
                                const double* g, double weight);                                /* meca_inter.cc:1918-1963 */
+/* evaluated on the device from the points, as the interactions below */
 int         cymg_inter_long_clamp(cymg_context* ctx, uint32_t a, const double* center,
                                   double len, double weight);                                 /* meca_inter.cc:1977-2022 */
 int         cymg_inter_plane_e(cymg_context* ctx, uint32_t a, const double* dir,
                                const double* g, double weight);                               /* meca_inter.cc:2301-2326 */
+int         cymg_add_pure_force_i(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca,
+                                  const double* force);                                       /* meca_inter.cc:140-156 */
+int         cymg_inter_plane_i(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, const double* dir,
+                               const double* g, double weight);                               /* meca_inter.cc:2340-2378 */
+
+/* Interactions that read the positions (meca_inter.cc).  Each call is recorded
+   in call order and evaluated on the device from the points of cymg_prepare
+   when the matrices are built (no host copy of the points); the entries it
+   makes are those of the reference, summed in call order.  An interpolated
+   point needs two distinct points of one Mecable.  Links between points that
+   share a model point of one Mecable record nothing (point_interpolated.cc:21-33).
+   Where the reference draws a random side for a target exactly on the fiber
+   axis (Random::sign_exc, random.cc:73-82) the side is +1.  No periodic space. */
+int         cymg_inter_long_link_ie(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b,
+                                    double len, double weight);                               /* meca_inter.cc:425-505 */
+int         cymg_inter_long_link_ii(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b1,
+                                    uint32_t b2, double cb, double len, double weight);       /* meca_inter.cc:516-596 */
+int         cymg_inter_side_link_ie(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b,
+                                    double len, double weight);                               /* meca_inter.cc:617-759 */
+int         cymg_inter_side_link_ii(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b1,
+                                    uint32_t b2, double cb, double len, double weight);       /* meca_inter.cc:764-935 */
+/* 2D only: CYMG_ERR_UNSUPPORTED in 3D, where the reference throws */
+int         cymg_inter_side_side_link(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b1,
+                                      uint32_t b2, double cb, double len, double weight);     /* meca_inter.cc:943-1059 */
+int         cymg_inter_sliding_link_ie(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b,
+                                       double weight);                                        /* meca_inter.cc:1075-1140 */
+int         cymg_inter_sliding_link_ii(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b1,
+                                       uint32_t b2, double cb, double weight);                /* meca_inter.cc:1153-1258 */
+int         cymg_inter_side_sliding_link_ie(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b,
+                                            double len, double weight);                       /* meca_inter.cc:1266-1499 */
+int         cymg_inter_side_sliding_link_ii(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b1,
+                                            uint32_t b2, double cb, double len, double weight); /* meca_inter.cc:1508-1751 */
+/* 2D only (interTorque2D); the interpolation coefficients play no part */
+int         cymg_inter_torque(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2,
+                              double cb, double cosinus, double sinus, double weight);        /* meca_inter.cc:1772-1869 */
+int         cymg_inter_side_clamp(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, const double* g,
+                                  double len, double weight);                                 /* meca_inter.cc:2032-2226 */
+int         cymg_inter_sliding_clamp(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, const double* g,
+                                     double weight);                                          /* meca_inter.cc:2241-2283 */
 
 /* device-side emitters, placed in the call order like the calls above.
    Fiber::setInteractions with CONFINE_INSIDE in a SpaceSphere centred at the

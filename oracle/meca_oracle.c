@@ -977,6 +977,708 @@ void orc_inter_plane_e(orc_meca* m, uint32_t a, const double* dir, const double*
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* position-dependent emitters on interpolated points (meca_inter.cc)       */
+/*                                                                          */
+/* A PointInterpolated (point_interpolated.h:97-124) is (p1, p2, c) on one  */
+/* Mecable: pos = P1 + c (P2 - P1), diff = P2 - P1, coef1 = c, coef2 = 1-c. */
+/* Vector arithmetic as vector2.h / vector3.h: norm = sqrt of the squares  */
+/* summed in order, normalized = v * (1 / norm), dot products in order.     */
+/* No periodic space (modulo == 0).  Random::sign_exc (random.cc:73-82)    */
+/* draws a random sign at exactly 0; this restatement returns +1 there.    */
+/* BLAS calls follow the reference BLAS (netlib dgemm, dspr, dsyrk) order. */
+
+static const double* pt_of(const orc_meca* m, uint32_t p) { return m->vPTS + (size_t)m->dim * p; }
+
+static void pi_pos(const orc_meca* m, uint32_t p1, uint32_t p2, double c, double* out)
+{
+    const double *x = pt_of(m, p1), *y = pt_of(m, p2);
+    for (int d = 0; d < m->dim; ++d) out[d] = x[d] + c * (y[d] - x[d]);
+}
+
+static void pi_diff(const orc_meca* m, uint32_t p1, uint32_t p2, double* out)
+{
+    const double *x = pt_of(m, p1), *y = pt_of(m, p2);
+    for (int d = 0; d < m->dim; ++d) out[d] = y[d] - x[d];
+}
+
+static double v_dot(int D, const double* a, const double* b)
+{
+    double s = a[0] * b[0];
+    for (int d = 1; d < D; ++d) s = s + a[d] * b[d];
+    return s;
+}
+
+static double v_norm(int D, const double* a) { return sqrt(v_dot(D, a, a)); }
+
+static void pi_dir(const orc_meca* m, uint32_t p1, uint32_t p2, double* out)
+{
+    pi_diff(m, p1, p2, out);
+    const double sn = 1.0 / v_norm(m->dim, out);
+    for (int d = 0; d < m->dim; ++d) out[d] = out[d] * sn;
+}
+
+static int sign_exc(double a) { return a < 0 ? -1 : 1; }
+
+/* vecProd in 2D (vector2.h:372-375) */
+static double cross2(const double* a, const double* b) { return a[0] * b[1] - a[1] * b[0]; }
+
+/* PointInterpolated::overlapping (point_interpolated.cc:21-33) */
+static int overlap_ie(const orc_meca* m, uint32_t a1, uint32_t a2, uint32_t b)
+{
+    return mecable_of(m, a1) == mecable_of(m, b) && (a1 == b || a2 == b);
+}
+
+static int overlap_ii(const orc_meca* m, uint32_t a1, uint32_t a2, uint32_t b1, uint32_t b2)
+{
+    return mecable_of(m, a1) == mecable_of(m, b1) && (a1 == b1 || a1 == b2 || a2 == b1 || a2 == b2);
+}
+
+#define MC(i, j) (*sp_ref(&m->mC, (i), (j)))
+#define MB(i, j) (*sp_ref(&m->mB, (i), (j)))
+
+/* addPureForce(PointInterpolated) (meca_inter.cc:140-156) */
+void orc_add_pure_force_i(orc_meca* m, uint32_t a1, uint32_t a2, double ca, const double* force)
+{
+    const int D = m->dim;
+    const double a = ca, b = 1.0 - ca;
+    for (int d = 0; d < D; ++d) {
+        m->vBAS[D * a1 + d] += b * force[d];
+        m->vBAS[D * a2 + d] += a * force[d];
+    }
+}
+
+/* interLongLink(PointInterpolated, PointExact) (meca_inter.cc:425-505) and
+   (PointInterpolated, PointInterpolated) (516-596): one body over the n = 3
+   or 4 points with coefficients c and the target at the end */
+static void long_link_n(orc_meca* m, int n, const uint32_t* pts, const double* c, const double* ab_in,
+                        double len, double w)
+{
+    const int D = m->dim;
+    double cw[4], ab[3] = { 0, 0, 0 };
+    uint32_t inx[4];
+    for (int k = 0; k < n; ++k) { cw[k] = w * c[k]; inx[k] = D * pts[k]; }
+    for (int d = 0; d < D; ++d) ab[d] = ab_in[d];
+    const double abn = v_norm(D, ab);
+    if (abn < REAL_EPSILON) return;
+    for (int d = 0; d < D; ++d) ab[d] /= abn;
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < D; ++d) m->vBAS[inx[i] + d] -= cw[i] * ab[d] * len;
+    len /= abn;
+    const int cooked = (len > 1.0);
+    for (int i = 0; i < D; ++i) {
+        double v = cooked ? ab[i] * ab[i] : 1.0 + len * (ab[i] * ab[i] - 1.0);
+        for (int k = 0; k < n; ++k)
+            for (int l = k; l < n; ++l) MC(inx[k] + i, inx[l] + i) -= c[k] * cw[l] * v;
+        for (int j = i + 1; j < D; ++j) {
+            v = cooked ? ab[i] * ab[j] : len * ab[i] * ab[j];
+            for (int k = 0; k < n; ++k)
+                for (int l = 0; l < n; ++l) MC(inx[k] + i, inx[l] + j) -= c[k] * cw[l] * v;
+        }
+    }
+}
+
+void orc_inter_long_link_ie(orc_meca* m, uint32_t a1, uint32_t a2, double ca, uint32_t b, double len, double w)
+{
+    if (overlap_ie(m, a1, a2, b)) return;
+    const uint32_t pts[3] = { a1, a2, b };
+    const double c[3] = { 1.0 - ca, ca, -1.0 };
+    double pa[3], ab[3];
+    pi_pos(m, a1, a2, ca, pa);
+    for (int d = 0; d < m->dim; ++d) ab[d] = pt_of(m, b)[d] - pa[d];
+    long_link_n(m, 3, pts, c, ab, len, w);
+}
+
+void orc_inter_long_link_ii(orc_meca* m, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2, double cb,
+                            double len, double w)
+{
+    if (overlap_ii(m, a1, a2, b1, b2)) return;
+    const uint32_t pts[4] = { a1, a2, b1, b2 };
+    const double c[4] = { 1.0 - ca, ca, -(1.0 - cb), -cb };
+    double pa[3], pb[3], ab[3];
+    pi_pos(m, a1, a2, ca, pa);
+    pi_pos(m, b1, b2, cb, pb);
+    for (int d = 0; d < m->dim; ++d) ab[d] = pb[d] - pa[d];
+    long_link_n(m, 4, pts, c, ab, len, w);
+}
+
+/* the 3D arm of the side links (meca_inter.cc:748-756, 921-929, 1489-1497):
+   the part of (target - pta.pos()) orthogonal to the segment, scaled to len;
+   returns 0 when that part is shorter than REAL_EPSILON */
+static int side_arm3(const double* a, const double* as, double len, double* arm)
+{
+    const double s = v_dot(3, as, a) / v_dot(3, a, a);
+    for (int d = 0; d < 3; ++d) arm[d] = as[d] - s * a[d];
+    const double n = v_norm(3, arm);
+    if (!(n > REAL_EPSILON)) return 0;
+    const double f = len / n;
+    for (int d = 0; d < 3; ++d) arm[d] = arm[d] * f;
+    return 1;
+}
+
+/* interSideLinkS (PE: meca_inter.cc:671-726, PI: 834-886) and
+   interSideSlidingLinkS (PE: 1374-1453, PI: 1626-1706) share one body:
+   T = a a' + b b' (side link) or b b' (side sliding link), RB = arm or len b */
+static void side_s(orc_meca* m, int n, const uint32_t* pts, const double* c, const double* a, const double* arm,
+                   double len, double w, int sliding)
+{
+    double b[3];
+    for (int d = 0; d < 3; ++d) b[d] = arm[d] / len;
+    double xx, xy, xz, yy, yz, zz;
+    if (sliding) {
+        xx = b[0] * b[0]; xy = b[0] * b[1]; xz = b[0] * b[2];
+        yy = b[1] * b[1]; yz = b[1] * b[2]; zz = b[2] * b[2];
+    } else {
+        xx = a[0] * a[0] + b[0] * b[0]; xy = a[0] * a[1] + b[0] * b[1]; xz = a[0] * a[2] + b[0] * b[2];
+        yy = a[1] * a[1] + b[1] * b[1]; yz = a[1] * a[2] + b[1] * b[2]; zz = a[2] * a[2] + b[2] * b[2];
+    }
+    const double T[9] = { xx, xy, xz, xy, yy, yz, xz, yz, zz };
+    double RB[3];
+    for (int d = 0; d < 3; ++d) RB[d] = sliding ? len * b[d] : arm[d];
+    double cw[4];
+    uint32_t inx[4];
+    for (int k = 0; k < n; ++k) { cw[k] = -w * c[k]; inx[k] = 3 * pts[k]; }
+    for (int i = 0; i < n; ++i) {
+        for (int x = 0; x < 3; ++x) m->vBAS[inx[i] + x] += cw[i] * RB[x];
+        const double g = c[i] * cw[i];
+        for (int x = 0; x < 3; ++x)
+            for (int y = x; y < 3; ++y) MC(inx[i] + x, inx[i] + y) += g * T[x + 3 * y];
+        for (int j = i + 1; j < n; ++j) {
+            const double h = c[i] * cw[j];
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y) MC(inx[i] + x, inx[j] + y) += h * T[x + 3 * y];
+        }
+    }
+}
+
+/* interSideLink2D(PointInterpolated, PointExact) (meca_inter.cc:617-667) */
+static void side_link2d_ie(orc_meca* m, uint32_t a1, uint32_t a2, double ca, uint32_t b, double arm, double w)
+{
+    double df[2];
+    pi_diff(m, a1, a2, df);
+    const double aa = ca, bb = 1.0 - ca;
+    const double ee = arm / v_norm(2, df);
+    uint32_t i1 = a1, i2 = a2, iE = b;
+    const double aaw = aa * w, bbw = bb * w;
+    const double eew = w * ee, eeeew = ee * eew;
+    MB(i1, i1) -= bbw * bb + eeeew;
+    MB(i1, i2) -= aaw * bb - eeeew;
+    MB(i2, i2) -= aaw * aa + eeeew;
+    MB(iE, iE) -= w;
+    MB(i1, iE) += bbw;
+    MB(i2, iE) += aaw;
+    i1 *= 2; i2 *= 2; iE *= 2;
+    MC(i1, i2 + 1) += eew;
+    MC(i1 + 1, i2) -= eew;
+    MC(i1, iE + 1) -= eew;
+    MC(i1 + 1, iE) += eew;
+    MC(i2, iE + 1) += eew;
+    MC(i2 + 1, iE) -= eew;
+}
+
+/* interSideLink(PointInterpolated, PointExact) (meca_inter.cc:730-759) */
+void orc_inter_side_link_ie(orc_meca* m, uint32_t a1, uint32_t a2, double ca, uint32_t b, double len, double w)
+{
+    if (overlap_ie(m, a1, a2, b)) return;
+    double pa[3], df[3], as[3];
+    pi_pos(m, a1, a2, ca, pa);
+    pi_diff(m, a1, a2, df);
+    for (int d = 0; d < m->dim; ++d) as[d] = pt_of(m, b)[d] - pa[d];
+    if (m->dim == 2) {
+        side_link2d_ie(m, a1, a2, ca, b, len * sign_exc(cross2(df, as)), w);
+    } else if (m->dim == 3) {
+        double arm[3], dir[3];
+        if (!side_arm3(df, as, len, arm)) return;
+        pi_dir(m, a1, a2, dir);
+        const uint32_t pts[3] = { a1, a2, b };
+        const double c[3] = { 1.0 - ca, ca, -1.0 };
+        side_s(m, 3, pts, c, dir, arm, len, w, 0);
+    }
+}
+
+/* interSideLink2D(PointInterpolated, PointInterpolated) (meca_inter.cc:764-830) */
+static void side_link2d_ii(orc_meca* m, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2, double cb,
+                           double arm, double w)
+{
+    double df[2];
+    pi_diff(m, a1, a2, df);
+    const double aa1 = ca, bb1 = 1.0 - ca;
+    const double aa2 = cb, bb2 = 1.0 - cb;
+    const double ee1 = arm / v_norm(2, df);
+    uint32_t i11 = a1, i12 = a2, i21 = b1, i22 = b2;
+    const double aa1w = aa1 * w, aa2w = aa2 * w;
+    const double bb1w = bb1 * w, bb2w = bb2 * w;
+    const double ee1w = ee1 * w, ee1ee1w = ee1 * ee1w;
+    MB(i11, i11) -= bb1w * bb1 + ee1ee1w;
+    MB(i11, i12) += -aa1w * bb1 + ee1ee1w;
+    MB(i12, i12) -= aa1w * aa1 + ee1ee1w;
+    MB(i21, i21) -= bb2w * bb2;
+    MB(i21, i22) += -aa2w * bb2;
+    MB(i22, i22) -= aa2w * aa2;
+    MB(i11, i21) += bb1w * bb2;
+    MB(i11, i22) += bb1w * aa2;
+    MB(i12, i22) += aa1w * aa2;
+    MB(i12, i21) += aa1w * bb2;
+    i11 *= 2; i12 *= 2; i21 *= 2; i22 *= 2;
+    MC(i11, i12 + 1) += ee1w;
+    MC(i11 + 1, i12) -= ee1w;
+    const double ee1bb2w = ee1w * bb2;
+    const double ee1aa2w = ee1w * aa2;
+    MC(i11, i21 + 1) -= ee1bb2w;
+    MC(i11, i22 + 1) -= ee1aa2w;
+    MC(i11 + 1, i21) += ee1bb2w;
+    MC(i11 + 1, i22) += ee1aa2w;
+    MC(i12, i21 + 1) += ee1bb2w;
+    MC(i12, i22 + 1) += ee1aa2w;
+    MC(i12 + 1, i21) -= ee1bb2w;
+    MC(i12 + 1, i22) -= ee1aa2w;
+}
+
+/* interSideLink(PointInterpolated, PointInterpolated) (meca_inter.cc:903-935) */
+void orc_inter_side_link_ii(orc_meca* m, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2, double cb,
+                            double len, double w)
+{
+    if (overlap_ii(m, a1, a2, b1, b2)) return;
+    double pa[3], pb[3], df[3], as[3];
+    pi_pos(m, a1, a2, ca, pa);
+    pi_pos(m, b1, b2, cb, pb);
+    pi_diff(m, a1, a2, df);
+    for (int d = 0; d < m->dim; ++d) as[d] = pb[d] - pa[d];
+    if (m->dim == 2) {
+        side_link2d_ii(m, a1, a2, ca, b1, b2, cb, len * sign_exc(cross2(df, as)), w);
+    } else if (m->dim == 3) {
+        double arm[3], dir[3];
+        if (!side_arm3(df, as, len, arm)) return;
+        pi_dir(m, a1, a2, dir);
+        const uint32_t pts[4] = { a1, a2, b1, b2 };
+        const double c[4] = { 1.0 - ca, ca, -(1.0 - cb), -cb };
+        side_s(m, 4, pts, c, dir, arm, len, w, 0);
+    }
+}
+
+/* interSideSideLink (meca_inter.cc:1038-1059) and interSideSideLink2D
+   (943-1021); the reference has no 3D version (it throws): -1 */
+int orc_inter_side_side_link(orc_meca* m, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2, double cb,
+                             double len, double w)
+{
+    if (m->dim != 2) return -1;
+    double pa[2], pb[2], da[2], db[2], dir[2];
+    pi_pos(m, a1, a2, ca, pa);
+    pi_pos(m, b1, b2, cb, pb);
+    pi_diff(m, a1, a2, da);
+    pi_diff(m, b1, b2, db);
+    for (int d = 0; d < 2; ++d) dir[d] = pb[d] - pa[d];
+    const int side1 = sign_exc(cross2(da, dir));
+    const int side2 = sign_exc(cross2(dir, db));
+    if (overlap_ii(m, a1, a2, b1, b2)) return 0;
+    const double aa1 = ca, bb1 = 1.0 - ca;
+    const double aa2 = cb, bb2 = 1.0 - cb;
+    const double ee1 = side1 * len / (2 * v_norm(2, da));
+    const double ee2 = side2 * len / (2 * v_norm(2, db));
+    uint32_t i11 = a1, i12 = a2, i21 = b1, i22 = b2;
+    const double aa1w = aa1 * w, aa2w = aa2 * w;
+    const double bb1w = bb1 * w, bb2w = bb2 * w;
+    const double ee1w = ee1 * w, ee1ee1w = ee1 * ee1w;
+    const double ee2w = ee2 * w, ee2ee2w = ee2 * ee2w;
+    const double ee1ee2w = ee1 * ee2w;
+    MB(i11, i11) -= bb1w * bb1 + ee1ee1w;
+    MB(i11, i12) -= aa1w * bb1 - ee1ee1w;
+    MB(i12, i12) -= aa1w * aa1 + ee1ee1w;
+    MB(i21, i21) -= bb2w * bb2 + ee2ee2w;
+    MB(i21, i22) -= aa2w * bb2 - ee2ee2w;
+    MB(i22, i22) -= aa2w * aa2 + ee2ee2w;
+    MB(i11, i21) += bb1w * bb2 + ee1ee2w;
+    MB(i11, i22) += bb1w * aa2 - ee1ee2w;
+    MB(i12, i22) += aa1w * aa2 + ee1ee2w;
+    MB(i12, i21) += aa1w * bb2 - ee1ee2w;
+    i11 *= 2; i12 *= 2; i21 *= 2; i22 *= 2;
+    MC(i11, i12 + 1) += ee1w;
+    MC(i11 + 1, i12) -= ee1w;
+    MC(i21, i22 + 1) += ee2w;
+    MC(i21 + 1, i22) -= ee2w;
+    const double ee1aa2w = ee1w * aa2;
+    const double ee1bb2w = ee1w * bb2;
+    const double ee2aa1w = ee2w * aa1;
+    const double ee2bb1w = ee2w * bb1;
+    MC(i11, i21 + 1) += ee2bb1w - ee1bb2w;
+    MC(i11, i22 + 1) -= ee2bb1w + ee1aa2w;
+    MC(i11 + 1, i21) += ee1bb2w - ee2bb1w;
+    MC(i11 + 1, i22) += ee1aa2w + ee2bb1w;
+    MC(i12, i21 + 1) += ee2aa1w + ee1bb2w;
+    MC(i12, i22 + 1) += ee1aa2w - ee2aa1w;
+    MC(i12 + 1, i21) -= ee1bb2w + ee2aa1w;
+    MC(i12 + 1, i22) -= ee1aa2w - ee2aa1w;
+    return 0;
+}
+
+/* interSlidingLink(PointInterpolated, PointExact) (meca_inter.cc:1075-1140) */
+void orc_inter_sliding_link_ie(orc_meca* m, uint32_t a1, uint32_t a2, double ca, uint32_t b, double w)
+{
+    if (overlap_ie(m, a1, a2, b)) return;
+    const int D = m->dim;
+    const double aa = ca, bb = 1.0 - ca;
+    const double bbbb = bb * bb, aaaa = aa * aa, aabb = aa * bb;
+    double dir[3];
+    pi_dir(m, a1, a2, dir);
+    const uint32_t i1 = D * a1, i2 = D * a2, iE = D * b;
+    for (int x = 0; x < D; ++x) {
+        for (int y = x; y < D; ++y) {
+            const double P = w * ((x == y) - dir[x] * dir[y]);
+            MC(i1 + x, i1 + y) -= bbbb * P;
+            MC(i1 + x, i2 + y) -= aabb * P;
+            MC(i2 + x, i2 + y) -= aaaa * P;
+            MC(i1 + x, iE + y) += bb * P;
+            MC(i2 + x, iE + y) += aa * P;
+            MC(iE + x, iE + y) -= P;
+            if (x != y) {
+                MC(i1 + y, i2 + x) -= aabb * P;
+                MC(i1 + y, iE + x) += bb * P;
+                MC(i2 + y, iE + x) += aa * P;
+            }
+        }
+    }
+}
+
+/* interSlidingLink(PointInterpolated, PointInterpolated) (meca_inter.cc:1153-1258) */
+void orc_inter_sliding_link_ii(orc_meca* m, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2, double cb,
+                               double w)
+{
+    if (overlap_ii(m, a1, a2, b1, b2)) return;
+    const int D = m->dim;
+    const double c[4] = { 1.0 - ca, ca, -(1.0 - cb), -cb };
+    double cw[4];
+    for (int k = 0; k < 4; ++k) cw[k] = -w * c[k];
+    const uint32_t inx[4] = { D * a1, D * a2, D * b1, D * b2 };
+    double dir[3];
+    pi_dir(m, a1, a2, dir);
+    if (D == 2) {
+        const double dd[3] = { dir[0] * dir[0], dir[0] * dir[1], dir[1] * dir[1] };
+        for (int j = 0; j < 4; ++j) {
+            const double g = c[j] * cw[j];
+            MC(inx[j], inx[j]) += (1.0 - dd[0]) * g;
+            MC(inx[j] + 1, inx[j] + 1) += (1.0 - dd[2]) * g;
+            MC(inx[j], inx[j] + 1) += (-dd[1]) * g;
+            for (int i = j + 1; i < 4; ++i) {
+                const double h = c[j] * cw[i];
+                MC(inx[i], inx[j]) += (1.0 - dd[0]) * h;
+                MC(inx[i] + 1, inx[j] + 1) += (1.0 - dd[2]) * h;
+                MC(inx[i], inx[j] + 1) += (-dd[1]) * h;
+                MC(inx[i] + 1, inx[j]) += (-dd[1]) * h;
+            }
+        }
+    } else if (D == 3) {
+        const double dd[6] = { dir[0] * dir[0], dir[0] * dir[1], dir[0] * dir[2],
+                               dir[1] * dir[1], dir[1] * dir[2], dir[2] * dir[2] };
+        for (int j = 0; j < 4; ++j) {
+            const double g = c[j] * cw[j];
+            MC(inx[j], inx[j]) += (1.0 - dd[0]) * g;
+            MC(inx[j] + 1, inx[j] + 1) += (1.0 - dd[3]) * g;
+            MC(inx[j] + 2, inx[j] + 2) += (1.0 - dd[5]) * g;
+            MC(inx[j], inx[j] + 1) += (-dd[1]) * g;
+            MC(inx[j], inx[j] + 2) += (-dd[2]) * g;
+            MC(inx[j] + 1, inx[j] + 2) += (-dd[4]) * g;
+            for (int i = j + 1; i < 4; ++i) {
+                const double h = c[j] * cw[i];
+                MC(inx[i], inx[j]) += (1.0 - dd[0]) * h;
+                MC(inx[i] + 1, inx[j] + 1) += (1.0 - dd[3]) * h;
+                MC(inx[i] + 2, inx[j] + 2) += (1.0 - dd[5]) * h;
+                MC(inx[i], inx[j] + 1) += (-dd[1]) * h;
+                MC(inx[i], inx[j] + 2) += (-dd[2]) * h;
+                MC(inx[i] + 1, inx[j] + 2) += (-dd[4]) * h;
+                MC(inx[i] + 1, inx[j]) += (-dd[1]) * h;
+                MC(inx[i] + 2, inx[j]) += (-dd[2]) * h;
+                MC(inx[i] + 2, inx[j] + 1) += (-dd[4]) * h;
+            }
+        }
+    }
+}
+
+/* netlib dgemm('N','N', 2, n, 2, alpha, P, 2, T, 2, 0, PT, 2) then
+   dgemm('T','N', n, n, 2, 1, T, 2, PT, 2, 0, TPT, n): the reference BLAS
+   order (C(i,j) accumulated over l from 0, the transposed product as one
+   dot of length 2) */
+static void tpt2(int n, double alpha, const double* P, const double* T, double* TPT)
+{
+    double PT[16];
+    for (int j = 0; j < n; ++j) {
+        for (int i = 0; i < 2; ++i) PT[i + 2 * j] = 0.0;
+        for (int l = 0; l < 2; ++l) {
+            const double t = alpha * T[l + 2 * j];
+            for (int i = 0; i < 2; ++i) PT[i + 2 * j] = PT[i + 2 * j] + t * P[i + 2 * l];
+        }
+    }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            double t = 0.0;
+            for (int l = 0; l < 2; ++l) t = t + T[l + 2 * i] * PT[l + 2 * j];
+            TPT[i + n * j] = 1.0 * t;
+        }
+}
+
+/* interSideSlidingLink2D (PE: meca_inter.cc:1266-1366, PI: 1508-1618), the
+   active '#if (1)' branch: TPT = T' (-w P) T over the n = 6 or 8 coordinates */
+static void side_sliding2d(orc_meca* m, int n, const uint32_t* pts, double ca, double cb, double arm, double w)
+{
+    const uint32_t a1 = pts[0], a2 = pts[1];
+    double dir[2], df[2];
+    pi_dir(m, a1, a2, dir);
+    pi_diff(m, a1, a2, df);
+    const double aa = ca, bb = 1.0 - ca;
+    const double ee = arm / v_norm(2, df);
+    double T[16] = { bb, -ee, ee, bb, aa, ee, -ee, aa, -1, 0, 0, -1, 0, 0, 0, 0 };
+    if (n == 8) {
+        const double cc = cb, dd = 1.0 - cb;
+        const double t2[8] = { -dd, 0, 0, -dd, -cc, 0, 0, -cc };
+        for (int k = 0; k < 8; ++k) T[8 + k] = t2[k];
+    }
+    const double P[4] = { 1 - dir[0] * dir[0], -dir[0] * dir[1], -dir[0] * dir[1], 1 - dir[1] * dir[1] };
+    double TPT[64];
+    tpt2(n, -w, P, T, TPT);
+    uint32_t inx[8];
+    for (int k = 0; k < n / 2; ++k) { inx[2 * k] = 2 * pts[k]; inx[2 * k + 1] = 2 * pts[k] + 1; }
+    for (int i = 0; i < n; ++i)
+        for (int j = i; j < n; ++j) MC(inx[i], inx[j]) += TPT[i + n * j];
+}
+
+/* interSideSlidingLink(PointInterpolated, PointExact) (meca_inter.cc:1469-1499) */
+void orc_inter_side_sliding_link_ie(orc_meca* m, uint32_t a1, uint32_t a2, double ca, uint32_t b, double len,
+                                    double w)
+{
+    double pa[3], df[3], as[3];
+    pi_pos(m, a1, a2, ca, pa);
+    pi_diff(m, a1, a2, df);
+    for (int d = 0; d < m->dim; ++d) as[d] = pt_of(m, b)[d] - pa[d];
+    if (m->dim == 2) {
+        const double arm = len * sign_exc(cross2(df, as));
+        if (overlap_ie(m, a1, a2, b)) return;
+        const uint32_t pts[3] = { a1, a2, b };
+        side_sliding2d(m, 6, pts, ca, 0.0, arm, w);
+    } else if (m->dim == 3) {
+        double arm[3], dir[3];
+        if (!side_arm3(df, as, len, arm)) return;
+        if (overlap_ie(m, a1, a2, b)) return;
+        pi_dir(m, a1, a2, dir);
+        const uint32_t pts[3] = { a1, a2, b };
+        const double c[3] = { 1.0 - ca, ca, -1.0 };
+        side_s(m, 3, pts, c, dir, arm, len, w, 1);
+    }
+}
+
+/* interSideSlidingLink(PointInterpolated, PointInterpolated) (meca_inter.cc:1724-1751) */
+void orc_inter_side_sliding_link_ii(orc_meca* m, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2,
+                                    double cb, double len, double w)
+{
+    double pa[3], pb[3], df[3], as[3];
+    pi_pos(m, a1, a2, ca, pa);
+    pi_pos(m, b1, b2, cb, pb);
+    pi_diff(m, a1, a2, df);
+    for (int d = 0; d < m->dim; ++d) as[d] = pb[d] - pa[d];
+    if (m->dim == 2) {
+        const double arm = len * sign_exc(cross2(df, as));
+        if (overlap_ii(m, a1, a2, b1, b2)) return;
+        const uint32_t pts[4] = { a1, a2, b1, b2 };
+        side_sliding2d(m, 8, pts, ca, cb, arm, w);
+    } else if (m->dim == 3) {
+        double arm[3], dir[3];
+        if (!side_arm3(df, as, len, arm)) return;
+        if (overlap_ii(m, a1, a2, b1, b2)) return;
+        pi_dir(m, a1, a2, dir);
+        const uint32_t pts[4] = { a1, a2, b1, b2 };
+        const double c[4] = { 1.0 - ca, ca, -(1.0 - cb), -cb };
+        side_s(m, 4, pts, c, dir, arm, len, w, 1);
+    }
+}
+
+/* interTorque2D (meca_inter.cc:1772-1869); the reference has it in 2D only: -1 */
+int orc_inter_torque(orc_meca* m, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2, double cb,
+                     double cosinus, double sinus, double w)
+{
+    (void)ca; (void)cb;
+    if (m->dim != 2) return -1;
+    if (overlap_ii(m, a1, a2, b1, b2)) return 0;
+    const uint32_t index[8] = { 2 * a1, 2 * a1 + 1, 2 * a2, 2 * a2 + 1, 2 * b1, 2 * b1 + 1, 2 * b2, 2 * b2 + 1 };
+    double ab[2], cd[2];
+    pi_diff(m, a1, a2, ab);
+    pi_diff(m, b1, b2, cd);
+    const double *a = pt_of(m, a1), *b = pt_of(m, a2), *c = pt_of(m, b1), *d = pt_of(m, b2);
+    const double coord[8] = { a[0], a[1], b[0], b[1], c[0], c[1], d[0], d[1] };
+    double ce[2];
+    ce[0] = cd[0] * cosinus + cd[1] * sinus;
+    ce[1] = -cd[0] * sinus + cd[1] * cosinus;
+    const double abn = v_norm(2, ab), abnS = v_dot(2, ab, ab);
+    const double cdn = v_norm(2, cd), cdnS = v_dot(2, cd, cd);
+    if (abn < REAL_EPSILON || cdn < REAL_EPSILON) return 0;
+    for (int k = 0; k < 2; ++k) { ab[k] /= abn; cd[k] /= cdn; ce[k] /= cdn; }
+    const double h[8] = { ab[1] / abn, -ab[0] / abn, -ab[1] / abn, ab[0] / abn,
+                          -cd[1] / cdn, cd[0] / cdn, cd[1] / cdn, -cd[0] / cdn };
+    const double dangle = atan2(ab[0] * ce[1] - ab[1] * ce[0], v_dot(2, ab, ce));
+    const double w1 = w;
+    const double w2 = w * dangle;
+    double mm[36];
+    for (int k = 0; k < 36; ++k) mm[k] = 0.0;
+    /* dspr('L', 8, -w1, h, 1, mm): lower packed, column by column */
+    for (int j = 0, kk = 0; j < 8; kk += 8 - j, ++j) {
+        if (h[j] != 0.0) {
+            const double t = -w1 * h[j];
+            for (int i = j, k = kk; i < 8; ++i, ++k) mm[k] = mm[k] + h[i] * t;
+        }
+    }
+    const double Da = w2 * (-2 * ab[0] * ab[1]) / abnS;
+    const double da = w2 * (ab[0] * ab[0] - ab[1] * ab[1]) / abnS;
+    const double Dc = w2 * (-2 * cd[0] * cd[1]) / cdnS;
+    const double dc = w2 * (cd[0] * cd[0] - cd[1] * cd[1]) / cdnS;
+    const double ea[4] = { -Da, -da, Da, da };
+    const double ec[4] = { Dc, dc, -Dc, -dc };
+    int sa = 0, sc = 26;
+    for (int j = 0; j < 4; ++j) {
+        const double sg = (j & 1) ? -1.0 : 1.0;
+        for (int i = j; i < 4; ++i) {
+            mm[i + sa] += sg * ea[i - j];
+            mm[i + sc] += sg * ec[i - j];
+        }
+        sa += 7 - j;
+        sc += 3 - j;
+    }
+    for (int i = 0; i < 8; ++i) {
+        m->vBAS[index[i]] += w2 * h[i];
+        for (int j = 0; j < 8; ++j) {
+            int k;
+            if (j < i) {
+                k = (int)(j * (7.5 - 0.5 * j) + i);
+            } else {
+                k = (int)(i * (7.5 - 0.5 * i) + j);
+                MC(index[i], index[j]) += mm[k];
+            }
+            m->vBAS[index[i]] -= mm[k] * coord[j];
+        }
+    }
+    return 0;
+}
+
+/* interSideClamp (meca_inter.cc:2200-2226) with interSideClamp2D (2032-2074)
+   and interSideClamp3D (2089-2186) */
+void orc_inter_side_clamp(orc_meca* m, uint32_t a1, uint32_t a2, double ca, const double* g, double len, double w)
+{
+    const int D = m->dim;
+    double pa[3], df[3];
+    pi_pos(m, a1, a2, ca, pa);
+    pi_diff(m, a1, a2, df);
+    if (D == 2) {
+        double gs[2] = { g[0] - pa[0], g[1] - pa[1] };
+        const double arm = len * sign_exc(cross2(df, gs));
+        const double aa = ca, aaw = w * aa;
+        const double bb = 1.0 - ca, bbw = w * bb;
+        const double ee = arm / v_norm(2, df);
+        const double eew = w * ee;
+        const double ee2w = w * ee * ee;
+        uint32_t i1 = a1, i2 = a2;
+        MB(i1, i1) -= bb * bbw + ee2w;
+        MB(i1, i2) -= aa * bbw - ee2w;
+        MB(i2, i2) -= aa * aaw + ee2w;
+        i1 *= 2; i2 *= 2;
+        MC(i1, i2 + 1) += eew;
+        MC(i1 + 1, i2) -= eew;
+        m->vBAS[i1] += bbw * g[0] - eew * g[1];
+        m->vBAS[i1 + 1] += bbw * g[1] + eew * g[0];
+        m->vBAS[i2] += aaw * g[0] + eew * g[1];
+        m->vBAS[i2 + 1] += aaw * g[1] - eew * g[0];
+    } else if (D == 3) {
+        const double ag[3] = { pa[0] - g[0], pa[1] - g[1], pa[2] - g[2] };
+        double arm[3] = { ag[1] * df[2] - ag[2] * df[1], ag[2] * df[0] - ag[0] * df[2], ag[0] * df[1] - ag[1] * df[0] };
+        const double n = v_norm(3, arm);
+        if (!(n > REAL_EPSILON)) return;
+        const double f = len / n;
+        for (int d = 0; d < 3; ++d) arm[d] = arm[d] * f;
+        const double aa = 1.0 - ca, bb = ca;
+        const double s = 1.0 / v_norm(3, df);
+        const double ex = s * arm[0], ey = s * arm[1], ez = s * arm[2];
+        const uint32_t i1 = 3 * a1, i2 = 3 * a2;
+        const uint32_t inx[6] = { i1, i1 + 1, i1 + 2, i2, i2 + 1, i2 + 2 };
+        const double T[18] = { aa, ez, -ey, bb, -ez, ey,
+                               -ez, aa, ex, ez, bb, -ex,
+                               ey, -ex, aa, -ey, ex, bb };
+        const double a2 = aa * aa, b2 = bb * bb, ab = aa * bb;
+        const double exx = ex * ex, exy = ex * ey, exz = ex * ez;
+        const double eyy = ey * ey, eyz = ey * ez, ezz = ez * ez;
+        double TT[36] = { eyy + ezz + a2, 0, 0, 0, 0, 0,
+                          -exy, exx + ezz + a2, 0, 0, 0, 0,
+                          -exz, -eyz, exx + eyy + a2, 0, 0, 0,
+                          -ezz - eyy + ab, ez + exy, exz - ey, eyy + ezz + b2, 0, 0,
+                          -ez + exy, -ezz - exx + ab, eyz + ex, -exy, exx + ezz + b2, 0,
+                          exz + ey, eyz - ex, -eyy - exx + ab, -exz, -eyz, exx + eyy + b2 };
+        const double sca = 1.0 / v_norm(3, arm);
+        const double aan = aa * sca, bbn = bb * sca;
+        const double TP[6] = { aan * ex, aan * ey, aan * ez, bbn * ex, bbn * ey, bbn * ez };
+        /* dsyrk('U','N', 6, 1, w, TP, 6, -w, TT, 6) */
+        for (int j = 0; j < 6; ++j) {
+            for (int i = 0; i <= j; ++i) TT[i + 6 * j] = -w * TT[i + 6 * j];
+            if (TP[j] != 0.0) {
+                const double t = w * TP[j];
+                for (int i = 0; i <= j; ++i) TT[i + 6 * j] = TT[i + 6 * j] + t * TP[i];
+            }
+        }
+        for (int i = 0; i < 6; ++i)
+            for (int j = i; j < 6; ++j) MC(inx[i], inx[j]) += TT[i + 6 * j];
+        const double ws = v_dot(3, arm, g) * sca * sca;
+        const double gx = w * (g[0] - ws * arm[0]);
+        const double gy = w * (g[1] - ws * arm[1]);
+        const double gz = w * (g[2] - ws * arm[2]);
+        for (int i = 0; i < 6; ++i) m->vBAS[inx[i]] += T[i] * gx + T[i + 6] * gy + T[i + 12] * gz;
+    }
+}
+
+/* interSlidingClamp (meca_inter.cc:2241-2283) */
+void orc_inter_sliding_clamp(orc_meca* m, uint32_t a1, uint32_t a2, double ca, const double* g, double w)
+{
+    const int D = m->dim;
+    const double aa = ca, bb = 1.0 - ca;
+    const double bbbb = bb * bb, aaaa = aa * aa, aabb = aa * bb;
+    double dir[3];
+    pi_dir(m, a1, a2, dir);
+    const uint32_t i1 = D * a1, i2 = D * a2;
+    for (int x = 0; x < D; ++x) {
+        for (int y = x; y < D; ++y) {
+            const double P = w * ((x == y) - dir[x] * dir[y]);
+            MC(i1 + x, i1 + y) -= bbbb * P;
+            MC(i2 + x, i2 + y) -= aaaa * P;
+            MC(i1 + x, i2 + y) -= aabb * P;
+            if (x != y) MC(i1 + y, i2 + x) -= aabb * P;
+        }
+    }
+    const double pr = v_dot(D, g, dir);
+    for (int x = 0; x < D; ++x) {
+        m->vBAS[i1 + x] += w * bb * (g[x] - pr * dir[x]);
+        m->vBAS[i2 + x] += w * aa * (g[x] - pr * dir[x]);
+    }
+}
+
+/* interPlane(PointInterpolated) (meca_inter.cc:2340-2378) */
+void orc_inter_plane_i(orc_meca* m, uint32_t a1, uint32_t a2, double ca, const double* dir, const double* g,
+                       double w)
+{
+    const int D = m->dim;
+    const double aa = ca, bb = 1.0 - ca;
+    const double bbbb = bb * bb, aaaa = aa * aa, aabb = aa * bb;
+    const uint32_t i1 = D * a1, i2 = D * a2;
+    for (int x = 0; x < D; ++x) {
+        for (int y = x; y < D; ++y) {
+            const double P = w * dir[x] * dir[y];
+            MC(i1 + x, i1 + y) -= bbbb * P;
+            MC(i2 + x, i2 + y) -= aaaa * P;
+            MC(i1 + x, i2 + y) -= aabb * P;
+            if (x != y) MC(i1 + y, i2 + x) -= aabb * P;
+        }
+    }
+    const double pr = w * v_dot(D, g, dir);
+    for (int x = 0; x < D; ++x) {
+        m->vBAS[i1 + x] += pr * bb * dir[x];
+        m->vBAS[i2 + x] += pr * aa * dir[x];
+    }
+}
+
+#undef MC
+#undef MB
+
 /* Fiber::setInteractions, CONFINE_INSIDE (fiber.cc:471-481), with
    SpaceSphere::inside (space_sphere.cc:113-116) and setInteraction (148-151) */
 void orc_confine_sphere(orc_meca* m, double radius, double stiffness)

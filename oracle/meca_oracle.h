@@ -88,6 +88,30 @@ void      orc_inter_clamp_e(orc_meca*, uint32_t a, const double* g, double w);
 void      orc_inter_clamp_i(orc_meca*, uint32_t a1, uint32_t a2, double ca, const double* g, double w);
 void      orc_inter_long_clamp(orc_meca*, uint32_t a, const double* center, double len, double w);
 void      orc_inter_plane_e(orc_meca*, uint32_t a, const double* dir, const double* g, double w);
+/* position-dependent emitters on interpolated points (meca_inter.cc) */
+void      orc_add_pure_force_i(orc_meca*, uint32_t a1, uint32_t a2, double ca, const double* force);
+void      orc_inter_long_link_ie(orc_meca*, uint32_t a1, uint32_t a2, double ca, uint32_t b, double len, double w);
+void      orc_inter_long_link_ii(orc_meca*, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2,
+                                 double cb, double len, double w);
+void      orc_inter_side_link_ie(orc_meca*, uint32_t a1, uint32_t a2, double ca, uint32_t b, double len, double w);
+void      orc_inter_side_link_ii(orc_meca*, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2,
+                                 double cb, double len, double w);
+int       orc_inter_side_side_link(orc_meca*, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2,
+                                   double cb, double len, double w);
+void      orc_inter_sliding_link_ie(orc_meca*, uint32_t a1, uint32_t a2, double ca, uint32_t b, double w);
+void      orc_inter_sliding_link_ii(orc_meca*, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2,
+                                    double cb, double w);
+void      orc_inter_side_sliding_link_ie(orc_meca*, uint32_t a1, uint32_t a2, double ca, uint32_t b, double len,
+                                         double w);
+void      orc_inter_side_sliding_link_ii(orc_meca*, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2,
+                                         double cb, double len, double w);
+int       orc_inter_torque(orc_meca*, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2, double cb,
+                           double cosinus, double sinus, double w);
+void      orc_inter_side_clamp(orc_meca*, uint32_t a1, uint32_t a2, double ca, const double* g, double len,
+                               double w);
+void      orc_inter_sliding_clamp(orc_meca*, uint32_t a1, uint32_t a2, double ca, const double* g, double w);
+void      orc_inter_plane_i(orc_meca*, uint32_t a1, uint32_t a2, double ca, const double* dir, const double* g,
+                            double w);
 /* Fiber::setInteractions CONFINE_INSIDE with SpaceSphere (fiber.cc:471-481; space_sphere.cc:148-151) */
 void      orc_confine_sphere(orc_meca*, double radius, double stiffness);
 

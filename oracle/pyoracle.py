@@ -88,6 +88,20 @@ def lib():
         "orc_inter_long_clamp": (None, [vp, C.c_uint32, _f64p, C.c_double, C.c_double]),
         "orc_inter_plane_e": (None, [vp, C.c_uint32, _f64p, _f64p, C.c_double]),
         "orc_confine_sphere": (None, [vp, C.c_double, C.c_double]),
+        "orc_add_pure_force_i": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, _f64p]),
+        "orc_inter_long_link_ie": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_double, C.c_double]),
+        "orc_inter_long_link_ii": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double]),
+        "orc_inter_side_link_ie": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_double, C.c_double]),
+        "orc_inter_side_link_ii": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double]),
+        "orc_inter_side_side_link": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double]),
+        "orc_inter_sliding_link_ie": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_double]),
+        "orc_inter_sliding_link_ii": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_uint32, C.c_double, C.c_double]),
+        "orc_inter_side_sliding_link_ie": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_double, C.c_double]),
+        "orc_inter_side_sliding_link_ii": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double]),
+        "orc_inter_torque": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double]),
+        "orc_inter_side_clamp": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, _f64p, C.c_double, C.c_double]),
+        "orc_inter_sliding_clamp": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, _f64p, C.c_double]),
+        "orc_inter_plane_i": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, _f64p, _f64p, C.c_double]),
         "orc_step_couples": (None, [vp, C.c_uint32, vp, _f64p, vp, _f64p, _f64p, vp, C.POINTER(Motor)]),
         "orc_nearby_segments": (C.c_uint32, [vp, _f64p, C.c_double, C.c_int, C.c_int, C.c_uint32,
                                              vp, vp, vp, vp]),
@@ -262,6 +276,51 @@ class OracleMeca:
 
     def add_pure_force(self, a, force):
         self.L.orc_add_pure_force(self.h, a, np.ascontiguousarray(force, dtype=np.float64))
+
+    def add_pure_force_i(self, a1, a2, ca, force):
+        self.L.orc_add_pure_force_i(self.h, a1, a2, ca, np.ascontiguousarray(force, dtype=np.float64))
+
+    def inter_long_link_ie(self, a1, a2, ca, b, length, w):
+        self.L.orc_inter_long_link_ie(self.h, a1, a2, ca, b, length, w)
+
+    def inter_long_link_ii(self, a1, a2, ca, b1, b2, cb, length, w):
+        self.L.orc_inter_long_link_ii(self.h, a1, a2, ca, b1, b2, cb, length, w)
+
+    def inter_side_link_ie(self, a1, a2, ca, b, length, w):
+        self.L.orc_inter_side_link_ie(self.h, a1, a2, ca, b, length, w)
+
+    def inter_side_link_ii(self, a1, a2, ca, b1, b2, cb, length, w):
+        self.L.orc_inter_side_link_ii(self.h, a1, a2, ca, b1, b2, cb, length, w)
+
+    def inter_side_side_link(self, a1, a2, ca, b1, b2, cb, length, w):
+        if self.L.orc_inter_side_side_link(self.h, a1, a2, ca, b1, b2, cb, length, w):
+            raise ValueError("interSideSideLink() was not implemented in 3D")
+
+    def inter_sliding_link_ie(self, a1, a2, ca, b, w):
+        self.L.orc_inter_sliding_link_ie(self.h, a1, a2, ca, b, w)
+
+    def inter_sliding_link_ii(self, a1, a2, ca, b1, b2, cb, w):
+        self.L.orc_inter_sliding_link_ii(self.h, a1, a2, ca, b1, b2, cb, w)
+
+    def inter_side_sliding_link_ie(self, a1, a2, ca, b, length, w):
+        self.L.orc_inter_side_sliding_link_ie(self.h, a1, a2, ca, b, length, w)
+
+    def inter_side_sliding_link_ii(self, a1, a2, ca, b1, b2, cb, length, w):
+        self.L.orc_inter_side_sliding_link_ii(self.h, a1, a2, ca, b1, b2, cb, length, w)
+
+    def inter_torque(self, a1, a2, ca, b1, b2, cb, cosinus, sinus, w):
+        if self.L.orc_inter_torque(self.h, a1, a2, ca, b1, b2, cb, cosinus, sinus, w):
+            raise ValueError("interTorque2D exists in 2D only")
+
+    def inter_side_clamp(self, a1, a2, ca, g, length, w):
+        self.L.orc_inter_side_clamp(self.h, a1, a2, ca, np.ascontiguousarray(g, dtype=np.float64), length, w)
+
+    def inter_sliding_clamp(self, a1, a2, ca, g, w):
+        self.L.orc_inter_sliding_clamp(self.h, a1, a2, ca, np.ascontiguousarray(g, dtype=np.float64), w)
+
+    def inter_plane_i(self, a1, a2, ca, direction, g, w):
+        self.L.orc_inter_plane_i(self.h, a1, a2, ca, np.ascontiguousarray(direction, dtype=np.float64),
+                                 np.ascontiguousarray(g, dtype=np.float64), w)
 
     def confine_sphere(self, radius, stiffness):
         self.L.orc_confine_sphere(self.h, radius, stiffness)
