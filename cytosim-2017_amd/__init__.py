@@ -39,7 +39,7 @@ EXPORTED = [
     "cymg_aster_interactions", "cymg_solve",
     "cymg_setup", "cymg_iterate", "cymg_apply", "cymg_multiply", "cymg_compute_preconditioner",
     "cymg_precondition", "cymg_compute_forces", "cymg_multiply_device", "cymg_precondition_device",
-    "cymg_get_vector", "cymg_get_points", "cymg_get_tensions", "cymg_get_block", "cymg_get_matrix",
+    "cymg_get_vector", "cymg_dump", "cymg_get_points", "cymg_get_tensions", "cymg_get_block", "cymg_get_matrix",
     "cymg_get_timings", "cymg_enable_timing", "cymg_time_kernel", "cymg_set_option", "cymg_get_option", "cymg_autotune",
     "cymg_guess_init", "cymg_guess_weights", "cymg_guess_record", "cymg_guess_samples",
     "cymg_guess_train", "cymg_guess_check", "cymg_guess_features", "cymg_guess_predict",
@@ -150,6 +150,7 @@ def lib():
         "cymg_inter_plane_e": (C.c_int, [vp, u32, dp, dp, C.c_double]),
         "cymg_add_pure_force_i": (C.c_int, [vp, u32, u32, C.c_double, dp]),
         "cymg_inter_plane_i": (C.c_int, [vp, u32, u32, C.c_double, dp, dp, C.c_double]),
+        "cymg_dump": (C.c_int, [vp, C.c_char_p]),
         "cymg_inter_long_link_ie": (C.c_int, [vp, u32, u32, C.c_double, u32, C.c_double, C.c_double]),
         "cymg_inter_long_link_ii": (C.c_int, [vp, u32, u32, C.c_double, u32, u32, C.c_double, C.c_double, C.c_double]),
         "cymg_inter_side_link_ie": (C.c_int, [vp, u32, u32, C.c_double, u32, C.c_double, C.c_double]),
@@ -594,6 +595,12 @@ class Meca:
         out = np.empty(bs * bs)
         self._chk(self.L.cymg_get_block(self.h, f, int(bool(inverse)), _p(out)))
         return out.reshape(bs, bs).T.copy()
+
+    def dump(self, directory=None):
+        """Meca::dump (meca.cc:957-990): ddim.txt, diagonal.txt, drhs.bin, dsol.bin,
+        dpts.bin and dmat.bin in `directory` (None: the working directory)"""
+        d = None if directory is None else str(directory).encode()
+        self._chk(self.L.cymg_dump(self.h, d))
 
     def matrix_entries(self, which):
         n = C.c_uint32()

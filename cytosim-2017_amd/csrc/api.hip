@@ -7,6 +7,7 @@
 #include <unordered_map>
 
 #include "context.h"
+#include "solid_ops.h"
 #include "tile_ops.h"
 
 using namespace cymg;
@@ -1336,6 +1337,67 @@ int cymg_get_vector(cymg_context* ctx, int which, double* out)
         }
         if (!src) throw Error(CYMG_ERR_INVALID, "vector not allocated");
         d2h(c, out, src, (size_t)c.n() * 8);
+    });
+}
+
+// Meca::dump (meca.cc:957-990) with dumpDiagonal (934-947) and dumpMatrix
+// (883-903): the same files, formats and order, written into `directory`
+// (NULL or "": the working directory, as the reference).  The matrix columns
+// are the operator applied to the unit vectors on the device.
+int cymg_dump(cymg_context* ctx, const char* directory)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!c.setup_done) throw Error(CYMG_ERR_INVALID, "cymg_setup must precede cymg_dump");
+        const std::string dir = (directory && *directory) ? std::string(directory) + "/" : std::string();
+        auto open = [&](const char* name, const char* mode) {
+            FILE* f = std::fopen((dir + name).c_str(), mode);
+            if (!f) throw Error(CYMG_ERR_INVALID, "cannot open " + dir + name);
+            return f;
+        };
+        const size_t n = c.n();
+        const int D = c.dim;
+        FILE* f = open("ddim.txt", "w");
+        std::fprintf(f, "%i\n", (int)n);
+        std::fclose(f);
+        // drag per point of every Mecable at its current points
+        std::vector<double> pos(n), mob(c.nfib), geo((size_t)kSolidGeo * c.nsol);
+        d2h(c, pos.data(), c.d_pos.get(), n * 8);
+        if (c.nfib) d2h(c, mob.data(), c.d_mob.get(), c.nfib * 8);
+        if (c.nsol) d2h(c, geo.data(), c.d_sol_geo.get(), geo.size() * 8);
+        f = open("diagonal.txt", "w");
+        for (uint32_t i = 0; i < c.nfib + c.nsol; ++i) {
+            const bool fib = i < c.nfib;
+            const uint32_t np = fib ? c.h_np[i] : c.h_sol_np[i - c.nfib];
+            const uint32_t first = fib ? c.h_idx[i] : c.h_sol_idx[i - c.nfib];
+            // RigidFiber::dragCoefficient = nbPoints / rfMobility; Solid: soDrag
+            const double drag = (fib ? np / mob[i] : geo[(size_t)kSolidGeo * (i - c.nfib) + SG_DRAG]) / np;
+            for (uint32_t p = 0; p < np; ++p)
+                for (int d = 0; d < D; ++d)
+                    std::fprintf(f, "%5u %16.8e %16.8e\n", i, pos[(size_t)D * (first + p) + d], drag);
+        }
+        std::fclose(f);
+        std::vector<double> v(n);
+        const std::pair<const char*, const double*> vecs[3] = {
+            { "drhs.bin", c.d_rhs.get() }, { "dsol.bin", c.d_sol.get() }, { "dpts.bin", c.d_pts.get() } };
+        for (const auto& kv : vecs) {
+            if (kv.second) d2h(c, v.data(), kv.second, n * 8);
+            else std::fill(v.begin(), v.end(), 0.0);
+            f = open(kv.first, "wb");
+            std::fwrite(v.data(), sizeof(double), n, f);
+            std::fclose(f);
+        }
+        c.k_v.resize(n);
+        c.k_t.resize(n);
+        f = open("dmat.bin", "wb");
+        const double one = 1.0;
+        for (size_t i = 0; i < n; ++i) {
+            CYMG_HIP(hipMemsetAsync(c.k_v.get(), 0, n * 8, c.stream));
+            CYMG_HIP(hipMemcpyAsync(c.k_v.get() + i, &one, 8, hipMemcpyHostToDevice, c.stream));
+            launch_matvec(c, c.k_v.get(), c.k_t.get());
+            d2h(c, v.data(), c.k_t.get(), n * 8);
+            std::fwrite(v.data(), sizeof(double), n, f);
+        }
+        std::fclose(f);
     });
 }
 

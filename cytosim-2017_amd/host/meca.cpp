@@ -264,4 +264,9 @@ void Meca::getPoints()
     }
 }
 
+void Meca::dump(const char* directory) const
+{
+    check(cymg_dump(ctx_, directory));
+}
+
 }  // namespace cymeca

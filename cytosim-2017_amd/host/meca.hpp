@@ -144,6 +144,7 @@ public:
     // export after solve: vPTS/vFOR and the Mecable points (getPoints)
     void getVector(int which, double* out) const;
     void getPoints();                                             // fills the Mecables' points
+    void dump(const char* directory = nullptr) const;             // meca.cc:957-990
 
     cymg_context* context() const { return ctx_; }
 

@@ -322,6 +322,10 @@ int         cymg_precondition_device(cymg_context* ctx, const double* dX, double
    as the current registration sees it, the seed of CYMG_GUESS_PREVIOUS */
 enum { CYMG_VEC_PTS = 0, CYMG_VEC_SOL, CYMG_VEC_BAS, CYMG_VEC_RHS, CYMG_VEC_FOR, CYMG_VEC_PREV };
 int         cymg_get_vector(cymg_context* ctx, int which, double* out);  /* vPTS..vFOR (meca.h:101-106) */
+/* Meca::dump (meca.cc:957-990): ddim.txt, diagonal.txt, drhs.bin, dsol.bin, dpts.bin and
+   dmat.bin (the full operator, column by column) in `directory` (NULL: the working directory);
+   after cymg_setup.  For debugging small systems: dmat.bin holds size^2 doubles. */
+int         cymg_dump(cymg_context* ctx, const char* directory);
 int         cymg_get_points(cymg_context* ctx, double* out);   /* Mecable points after getPoints (fiber_naked.cc:414-425) */
 int         cymg_get_tensions(cymg_context* ctx, double* out); /* RigidFiber::tension, one per segment (rigid_fiber.h:90) */
 int         cymg_get_block(cymg_context* ctx, uint32_t mecable, int inverse, double* out); /* column-major bs*bs */

@@ -74,3 +74,32 @@ def test_surface_drag_needs_a_height(pkg):
     assert gpu.L.cymg_add_fiber_ex(gpu.h, C.byref(d), C.byref(mi)) == pkg.ERR_INVALID
     assert b"must set and > 0" in gpu.L.cymg_last_error(gpu.h)
     gpu.close()
+
+
+def test_dump_writes_the_reference_files(pkg, scenes, oracle, tmp_path):
+    """Meca::dump (meca.cc:957-990): the system size, the drag per point of
+    every Mecable, vRHS, vSOL, vPTS and the operator column by column."""
+    sc = scenes.mini(dim=2, seed=5, nfib=4, nlinks=6)
+    gpu = pkg.Meca(2, 0)
+    pkg.load_scene(gpu, sc)
+    g = sc.gauss(0)
+    gpu.setup(g)
+    gpu.iterate(1e-10, tol_mode=pkg.TOL_RELATIVE)
+    print("dump: 2D mini scene", flush=True)
+    gpu.dump(tmp_path)
+    n = sc.size
+    assert int((tmp_path / "ddim.txt").read_text()) == n
+    for name, vec in (("drhs.bin", pkg.VEC_RHS), ("dsol.bin", pkg.VEC_SOL), ("dpts.bin", pkg.VEC_PTS)):
+        assert np.array_equal(np.fromfile(tmp_path / name, dtype=np.float64), gpu.vector(vec))
+    mat = np.fromfile(tmp_path / "dmat.bin", dtype=np.float64).reshape(n, n)   # row k = M e_k
+    orc = oracle.from_scene(sc)
+    orc.setup(g)
+    dense = np.stack([orc.multiply(e) for e in np.eye(n)])
+    assert rel_err(mat, dense) < 1e-12
+    rows = np.loadtxt(tmp_path / "diagonal.txt")
+    assert rows.shape == (n, 3)
+    want_drag = np.concatenate([np.full(2 * int(sc.nb_points[f]), 1.0 / orc.mobility(f)) for f in range(sc.nfib)])
+    assert np.array_equal(rows[:, 0].astype(int), np.repeat(np.arange(sc.nfib), 2 * sc.nb_points))
+    assert rel_err(rows[:, 2], want_drag) < 1e-7   # printed with 9 significant digits
+    assert rel_err(rows[:, 1], gpu.fiber_points().ravel()) < 1e-7
+    gpu.close()
