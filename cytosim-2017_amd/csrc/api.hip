@@ -1579,7 +1579,7 @@ int cymg_autotune(cymg_context* ctx, int reps, int* tile_points, int* block_inve
         // with the shortest launch on the current system
         int best_tile = (int)c.tile_points, best_threads = (int)c.tile_threads;
         double best = 1e300;
-        if (!c.nsol || c.solid_tiles) {
+        if ((!c.nsol || c.solid_tiles) && tiles_fit(c)) {
             const int cands[] = { 0, 16, 24, 32, 48, 64, 85, 96, 128, 170 };
             for (int nt : { 512, 256 }) {
                 if (nt == 256 && (c.nsol || c.largest > (uint32_t)tile_max_points(c, 256))) continue;

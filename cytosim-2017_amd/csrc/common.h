@@ -17,6 +17,7 @@ namespace cymg {
 
 constexpr int kWave = 64;               // CDNA wavefront
 constexpr int kMaxFiberPoints = CYMG_MAX_FIBER_POINTS;
+constexpr int kShortFiberPoints = 128;  // per-fiber launches of layouts without longer fibers
 constexpr uint32_t kInvalid32 = 0xFFFFFFFFu;
 constexpr uint64_t kInvalidKey = 0xFFFFFFFFFFFFFFFFull;
 
@@ -220,6 +221,7 @@ void launch_forces(Context& c, const double* X, double* Y, bool with_rigidity);
 void launch_apply(Context& c);
 void launch_gather_points(Context& c, const uint32_t* map, const double* src, double* dst, uint32_t npts);
 void launch_tensions(Context& c);
+void launch_block_columns(Context& c, const uint32_t* list, uint32_t count, double* M);
 // launchers (precond.hip)
 void launch_precond_build(Context& c, int precision, double* block_out = nullptr);
 void launch_precond_apply(Context& c, const double* X, double* Y);
@@ -235,6 +237,7 @@ void launch_solid_points(Context& c);
 double krylov_time_launch(Context& c, int precond, int launches);
 int tile_threads_of(const Context& c);
 int tile_max_points(const Context& c, int threads);
+bool tiles_fit(const Context& c);
 void nearby_segments(Context& c, uint32_t nq, const double* places, const uint32_t* exclude, double range,
                      int inclusive, uint32_t* counts, uint32_t capacity, uint32_t* fiber, uint32_t* point,
                      double* abscissa, double* dist_sqr, uint32_t* total);

@@ -108,7 +108,8 @@ struct Context {
                                        // 3 blocked Gauss-Jordan with matrix-core panel updates
     DevBuf<uint32_t> d_gj_list;        // fibers by block-size class
     std::vector<uint32_t> gj_start;    // class boundaries in d_gj_list
-    uint32_t gj_nfib = 0, gj_largest = 0;
+    uint64_t gj_gen = ~0ull;           // layout generation the lists were made for
+    int gj_bs_lds = 0;                 // largest block of the workgroup-local kernels
     DevBuf<double> d_binv;
     DevBuf<float> d_binv32;
     DevBuf<int> d_useblk;

@@ -1,9 +1,21 @@
 // Per-fiber kernels of the implicit step: Meca::prepare, the device-side
 // emitters, the force/RHS setup, the standalone operator, and the post-solve
 // update (vPTS += vSOL, forces, getPoints reshape).  One wavefront per fiber.
+#include <algorithm>
+
 #include "args.h"
 
 namespace cymg {
+
+// launch kern<D, P> with the LDS capacity P the layout needs: the short one
+// unless a fiber is longer (fewer resident workgroups otherwise)
+#define CYMG_FIBER_LAUNCH(kern, D, ...)                                                    \
+    do {                                                                                   \
+        if (c.largest <= (uint32_t)kShortFiberPoints)                                      \
+            hipLaunchKernelGGL((kern<D, kShortFiberPoints>), __VA_ARGS__);                 \
+        else                                                                               \
+            hipLaunchKernelGGL((kern<D, kMaxFiberPoints>), __VA_ARGS__);                   \
+    } while (0)
 
 constexpr double kRealEpsilon = 128.0 * 2.220446049250313e-16;  // src/math/real.h:32
 
@@ -14,7 +26,7 @@ constexpr double kRealEpsilon = 128.0 * 2.220446049250313e-16;  // src/math/real
 // (rigid_fiber_project.cc:50-87), rfRigidity = rigidity / fnCut^3.
 // dragp: per fiber the hydrodynamic radii (0: the context's r0, r1) and the
 // cylinder height of the surface drag model (0: drag in a volume).
-template <int D>
+template <int D, int P>
 __global__ __launch_bounds__(kWave) void k_prepare(const uint32_t* __restrict__ npv, const uint32_t* __restrict__ idxv,
                                                    const uint32_t* __restrict__ segv, const double* __restrict__ cutv,
                                                    const double* __restrict__ rigv, const double* __restrict__ pos,
@@ -25,8 +37,8 @@ __global__ __launch_bounds__(kWave) void k_prepare(const uint32_t* __restrict__ 
                                                    int* __restrict__ status, const double* __restrict__ dragp,
                                                    double visc, double r0d, double r1d)
 {
-    __shared__ double sdif[kMaxFiberPoints * D];
-    __shared__ double sd[kMaxFiberPoints], se[kMaxFiberPoints];
+    __shared__ double sdif[P * D];
+    __shared__ double sd[P], se[P];
     const uint32_t f = blockIdx.x;
     const uint32_t np = npv[f], idx = idxv[f], seg = segv[f];
     const int S = (int)np - 1;
@@ -104,12 +116,12 @@ void launch_fiber_prepare(Context& c)
 {
     const dim3 g(c.nfib), b(kWave);
     if (c.dim == 3)
-        hipLaunchKernelGGL(k_prepare<3>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_seg.get(), c.d_cut.get(),
+        CYMG_FIBER_LAUNCH(k_prepare, 3, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_seg.get(), c.d_cut.get(),
                            c.d_rig.get(), c.d_pos.get(), c.d_pts.get(), c.d_mob.get(), c.d_rfrig.get(),
                            c.d_diff.get(), c.d_jd.get(), c.d_je.get(), c.d_jd0.get(), c.d_je0.get(), c.d_status.get(), c.d_drag.get(),
                            c.prm.viscosity, c.prm.hydro_radius0, c.prm.hydro_radius1);
     else
-        hipLaunchKernelGGL(k_prepare<2>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_seg.get(), c.d_cut.get(),
+        CYMG_FIBER_LAUNCH(k_prepare, 2, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_seg.get(), c.d_cut.get(),
                            c.d_rig.get(), c.d_pos.get(), c.d_pts.get(), c.d_mob.get(), c.d_rfrig.get(),
                            c.d_diff.get(), c.d_jd.get(), c.d_je.get(), c.d_jd0.get(), c.d_je0.get(), c.d_status.get(), c.d_drag.get(),
                            c.prm.viscosity, c.prm.hydro_radius0, c.prm.hydro_radius1);
@@ -478,14 +490,14 @@ void launch_step_couples(Context& c, const MotorStep& ms)
 // (rigid_fiber.cc:124-132), P' coefficients (rigid_fiber_project.cc:168-219).
 // Per-fiber partials of noise = min(mobility*b) and bnorm = |vRHS|_inf; the
 // solver's first kernel reduces them per system.
-template <int D>
+template <int D, int P>
 __global__ __launch_bounds__(kWave) void k_setup(FiberArgs a, const double* __restrict__ pts,
                                                  const double* __restrict__ bas, const double* __restrict__ gauss,
                                                  double kT_dt, double* __restrict__ vfor, double* __restrict__ rhs,
                                                  double* __restrict__ lag, double* __restrict__ pdiff,
                                                  double* __restrict__ part, uint32_t nfib)
 {
-    __shared__ FiberLds<D> s;
+    __shared__ FiberLds<D, P> s;
     const uint32_t f = blockIdx.x;
     const uint32_t np = a.np[f], idx = a.idx[f], seg = a.seg[f];
     const int S = (int)np - 1;
@@ -532,11 +544,11 @@ void launch_setup(Context& c, const double* gauss)
     const dim3 g(c.nfib), b(kWave);
     const double kT_dt = c.prm.kT / c.prm.time_step;
     if (c.dim == 3)
-        hipLaunchKernelGGL(k_setup<3>, g, b, 0, c.stream, a, c.d_pts.get(), c.d_bas.get(), gauss, kT_dt,
+        CYMG_FIBER_LAUNCH(k_setup, 3, g, b, 0, c.stream, a, c.d_pts.get(), c.d_bas.get(), gauss, kT_dt,
                            c.d_for.get(), c.d_rhs.get(), c.d_lag.get(), c.d_pdiff.get(), c.d_part.get(),
                            c.nmec());
     else
-        hipLaunchKernelGGL(k_setup<2>, g, b, 0, c.stream, a, c.d_pts.get(), c.d_bas.get(), gauss, kT_dt,
+        CYMG_FIBER_LAUNCH(k_setup, 2, g, b, 0, c.stream, a, c.d_pts.get(), c.d_bas.get(), gauss, kT_dt,
                            c.d_for.get(), c.d_rhs.get(), c.d_lag.get(), c.d_pdiff.get(), c.d_part.get(),
                            c.nmec());
     CYMG_HIP(hipGetLastError());
@@ -544,10 +556,10 @@ void launch_setup(Context& c, const double* gauss)
 
 // ---------------------------------------------------------------------------
 // Meca::multiply (meca.cc:157-181) as a standalone launch
-template <int D>
+template <int D, int P>
 __global__ __launch_bounds__(kWave) void k_matvec(FiberArgs a, const double* __restrict__ X, double* __restrict__ Y)
 {
-    __shared__ FiberLds<D> s;
+    __shared__ FiberLds<D, P> s;
     const uint32_t f = blockIdx.x;
     const uint32_t np = a.np[f], o = D * a.idx[f];
     for (int i = threadIdx.x; i < (int)(D * np); i += kWave) s.x[i] = X[o + i];
@@ -564,18 +576,18 @@ void launch_matvec(Context& c, const double* X, double* Y)
     FiberArgs a = make_fiber_args(c);
     const dim3 g(c.nfib), b(kWave);
     if (c.dim == 3)
-        hipLaunchKernelGGL(k_matvec<3>, g, b, 0, c.stream, a, X, Y);
+        CYMG_FIBER_LAUNCH(k_matvec, 3, g, b, 0, c.stream, a, X, Y);
     else
-        hipLaunchKernelGGL(k_matvec<2>, g, b, 0, c.stream, a, X, Y);
+        CYMG_FIBER_LAUNCH(k_matvec, 2, g, b, 0, c.stream, a, X, Y);
     CYMG_HIP(hipGetLastError());
 }
 
 // Meca::computeForces(X, Y, with_rigidity) (meca.cc:143-147): Y = vBAS + (R) + (B + C) X
-template <int D>
+template <int D, int P>
 __global__ __launch_bounds__(kWave) void k_forces(FiberArgs a, const double* __restrict__ X,
                                                   const double* __restrict__ bas, double* __restrict__ Y, int rig)
 {
-    __shared__ FiberLds<D> s;
+    __shared__ FiberLds<D, P> s;
     const uint32_t f = blockIdx.x;
     const uint32_t np = a.np[f], idx = a.idx[f], o = D * idx;
     for (int i = threadIdx.x; i < (int)(D * np); i += kWave) s.x[i] = X[o + i];
@@ -592,9 +604,9 @@ void launch_forces(Context& c, const double* X, double* Y, bool with_rigidity)
     FiberArgs a = make_fiber_args(c);
     const dim3 g(c.nfib), b(kWave);
     if (c.dim == 3)
-        hipLaunchKernelGGL(k_forces<3>, g, b, 0, c.stream, a, X, c.d_bas.get(), Y, with_rigidity ? 1 : 0);
+        CYMG_FIBER_LAUNCH(k_forces, 3, g, b, 0, c.stream, a, X, c.d_bas.get(), Y, with_rigidity ? 1 : 0);
     else
-        hipLaunchKernelGGL(k_forces<2>, g, b, 0, c.stream, a, X, c.d_bas.get(), Y, with_rigidity ? 1 : 0);
+        CYMG_FIBER_LAUNCH(k_forces, 2, g, b, 0, c.stream, a, X, c.d_bas.get(), Y, with_rigidity ? 1 : 0);
     CYMG_HIP(hipGetLastError());
 }
 
@@ -608,14 +620,14 @@ __global__ void k_axpy(uint32_t n, const double* __restrict__ x, double* __restr
 // FiberNaked::getPoints (fiber_naked.cc:414-425): reshape_two (172-193),
 // reshape_it (221-354: Newton on the segment lengths, dgtsv solve, stop at
 // sum|dsca| <= 0.001 or after 32 iterations), reshape_sure (367-401) on failure.
-template <int D>
+template <int D, int P>
 __global__ __launch_bounds__(kWave) void k_reshape(const uint32_t* __restrict__ npv, const uint32_t* __restrict__ idxv,
                                                    const double* __restrict__ cutv, const double* __restrict__ src,
                                                    double* __restrict__ dst)
 {
-    __shared__ double dif[3 * kMaxFiberPoints], vec[3 * kMaxFiberPoints];
-    __shared__ double sca[kMaxFiberPoints], val[kMaxFiberPoints], dia[kMaxFiberPoints];
-    __shared__ double low[kMaxFiberPoints], upe[kMaxFiberPoints];
+    __shared__ double dif[3 * P], vec[3 * P];
+    __shared__ double sca[P], val[P], dia[P];
+    __shared__ double low[P], upe[P];
     __shared__ int fail;
     const uint32_t f = blockIdx.x;
     const uint32_t np = npv[f], o = D * idxv[f];
@@ -860,19 +872,62 @@ void launch_apply(Context& c)
     if (!c.nfib) return;
     const dim3 g(c.nfib), b(kWave);
     if (c.dim == 3)
-        hipLaunchKernelGGL(k_reshape<3>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_cut.get(), c.d_pts.get(),
+        CYMG_FIBER_LAUNCH(k_reshape, 3, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_cut.get(), c.d_pts.get(),
                            c.d_pos.get());
     else
-        hipLaunchKernelGGL(k_reshape<2>, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_cut.get(), c.d_pts.get(),
+        CYMG_FIBER_LAUNCH(k_reshape, 2, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_cut.get(), c.d_pts.get(),
                            c.d_pos.get());
     CYMG_HIP(hipGetLastError());
 }
 
-// RigidFiber::computeTensions (rigid_fiber_project.cc:153-161) on vFOR
+// The diagonal block of fiber f (Meca::getBlock, meca.cc:231-283) for blocks
+// too large for the workgroup-local builder: column j is the fiber's operator
+// (R + B + C + P', projection, identity) applied to the unit vector e_j, with
+// the couplings to other Mecables reading zero.  One wavefront per (fiber,
+// column); columns j = blockIdx.y, blockIdx.y + gridDim.y, ...; M column-major.
+struct UnitOperand {
+    uint32_t at;   // the one non-zero entry of the global operand
+    __device__ __forceinline__ double operator()(uint32_t j) const { return j == at ? 1.0 : 0.0; }
+};
+
 template <int D>
-__global__ __launch_bounds__(kWave) void k_tensions(FiberArgs a, const double* __restrict__ F, double* __restrict__ lag)
+__global__ __launch_bounds__(kWave) void k_block_columns(FiberArgs a, const uint32_t* __restrict__ list,
+                                                         const uint64_t* __restrict__ blkoff, double* __restrict__ M)
 {
     __shared__ FiberLds<D> s;
+    const uint32_t f = list[blockIdx.x];
+    const uint32_t np = a.np[f], o = D * a.idx[f];
+    const int bs = D * (int)np;
+    double* B = M + blkoff[f];
+    load_factors<D>(a, s, a.seg[f], (int)np - 1);
+    for (int j = blockIdx.y; j < bs; j += gridDim.y) {
+        for (int i = threadIdx.x; i < bs; i += kWave) s.x[i] = (i == j) ? 1.0 : 0.0;
+        __syncthreads();
+        fiber_operator_op<D>(a, s, UnitOperand{ o + (uint32_t)j }, f);
+        for (int i = threadIdx.x; i < bs; i += kWave) B[(size_t)bs * j + i] = s.y[i];
+        __syncthreads();
+    }
+}
+
+void launch_block_columns(Context& c, const uint32_t* list, uint32_t count, double* M)
+{
+    if (!count) return;
+    FiberArgs a = make_fiber_args(c);
+    // enough (fiber, column) wavefronts to fill the device several times
+    const uint32_t cols = std::max<uint32_t>(1, std::min<uint32_t>(c.dim * c.largest, 4096u / count + 1));
+    const dim3 g(count, cols), b(kWave);
+    if (c.dim == 3)
+        hipLaunchKernelGGL(k_block_columns<3>, g, b, 0, c.stream, a, list, c.d_blkoff.get(), M);
+    else
+        hipLaunchKernelGGL(k_block_columns<2>, g, b, 0, c.stream, a, list, c.d_blkoff.get(), M);
+    CYMG_HIP(hipGetLastError());
+}
+
+// RigidFiber::computeTensions (rigid_fiber_project.cc:153-161) on vFOR
+template <int D, int P>
+__global__ __launch_bounds__(kWave) void k_tensions(FiberArgs a, const double* __restrict__ F, double* __restrict__ lag)
+{
+    __shared__ FiberLds<D, P> s;
     const uint32_t f = blockIdx.x;
     const uint32_t np = a.np[f], o = D * a.idx[f], seg = a.seg[f];
     const int S = (int)np - 1;
@@ -897,9 +952,9 @@ void launch_tensions(Context& c)
     FiberArgs a = make_fiber_args(c);
     const dim3 g(c.nfib), b(kWave);
     if (c.dim == 3)
-        hipLaunchKernelGGL(k_tensions<3>, g, b, 0, c.stream, a, c.d_for.get(), c.d_lag.get());
+        CYMG_FIBER_LAUNCH(k_tensions, 3, g, b, 0, c.stream, a, c.d_for.get(), c.d_lag.get());
     else
-        hipLaunchKernelGGL(k_tensions<2>, g, b, 0, c.stream, a, c.d_for.get(), c.d_lag.get());
+        CYMG_FIBER_LAUNCH(k_tensions, 2, g, b, 0, c.stream, a, c.d_for.get(), c.d_lag.get());
     CYMG_HIP(hipGetLastError());
 }
 

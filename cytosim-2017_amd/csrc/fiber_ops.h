@@ -32,16 +32,18 @@ struct FiberArgs {
     double dt;
 };
 
-template <int D>
+// P: the longest fiber the workgroup takes (kShortFiberPoints for the
+// launches of layouts without longer fibers: less LDS, more workgroups per CU)
+template <int D, int P = kMaxFiberPoints>
 struct FiberLds {
-    double x[kMaxFiberPoints * D];     // fiber coordinates of the operand
-    double f[kMaxFiberPoints * D];     // forces
-    double y[kMaxFiberPoints * D];     // result
-    double t[kMaxFiberPoints];         // J*f, then multipliers
-    double d[kMaxFiberPoints];         // pttrf D
-    double e[kMaxFiberPoints];         // pttrf L subdiagonal
+    double x[P * D];     // fiber coordinates of the operand
+    double f[P * D];     // forces
+    double y[P * D];     // result
+    double t[P];         // J*f, then multipliers
+    double d[P];         // pttrf D
+    double e[P];         // pttrf L subdiagonal
     double red[kWave];
-    uint32_t hub[kMaxFiberPoints];     // points whose mB row is long (many links on one point)
+    uint32_t hub[P];     // points whose mB row is long (many links on one point)
     uint32_t nhub;
 };
 
@@ -122,8 +124,8 @@ __device__ __forceinline__ double pcr_solve(int S, double a, double b, double c,
 
 // Load the fiber's pttrf factors into LDS.
 // (only the serial path for fibers longer than a wavefront needs them, unless forced)
-template <int D>
-__device__ __forceinline__ void load_factors(const FiberArgs& a, FiberLds<D>& s, uint32_t seg, int S,
+template <int D, int P>
+__device__ __forceinline__ void load_factors(const FiberArgs& a, FiberLds<D, P>& s, uint32_t seg, int S,
                                              bool force = false)
 {
     if (!force && S <= kWave) return;
@@ -141,8 +143,8 @@ struct Plain {
     __device__ __forceinline__ double operator()(uint32_t j) const { return X[j]; }
 };
 
-template <int D, typename Op>
-__device__ __forceinline__ void fiber_forces_op(const FiberArgs& a, FiberLds<D>& s, const Op& X,
+template <int D, int P, typename Op>
+__device__ __forceinline__ void fiber_forces_op(const FiberArgs& a, FiberLds<D, P>& s, const Op& X,
                                                 uint32_t np, uint32_t idx, uint32_t seg, double k,
                                                 bool rigidity, bool with_pdiff)
 {
@@ -270,8 +272,8 @@ __device__ __forceinline__ void fiber_forces_op(const FiberArgs& a, FiberLds<D>&
     }
 }
 
-template <int D>
-__device__ __forceinline__ void fiber_forces(const FiberArgs& a, FiberLds<D>& s, const double* __restrict__ X,
+template <int D, int P>
+__device__ __forceinline__ void fiber_forces(const FiberArgs& a, FiberLds<D, P>& s, const double* __restrict__ X,
                                              uint32_t np, uint32_t idx, uint32_t seg, double k,
                                              bool rigidity, bool with_pdiff)
 {
@@ -280,8 +282,8 @@ __device__ __forceinline__ void fiber_forces(const FiberArgs& a, FiberLds<D>& s,
 
 // s.y = sca * P s.f   with P = I - J'(JJ')^-1 J; the multipliers (JJ')^-1 J f
 // are left in s.t (rigid_fiber_project.cc:95-149).  s.d/s.e hold the factors.
-template <int D>
-__device__ __forceinline__ void fiber_project(const FiberArgs& a, FiberLds<D>& s, uint32_t np, uint32_t seg,
+template <int D, int P>
+__device__ __forceinline__ void fiber_project(const FiberArgs& a, FiberLds<D, P>& s, uint32_t np, uint32_t seg,
                                               double sca)
 {
     const int S = (int)np - 1;
@@ -321,8 +323,8 @@ __device__ __forceinline__ void fiber_project(const FiberArgs& a, FiberLds<D>& s
 
 // s.y = X - dt*mob*P (R + B + C + P') X  (Meca::multiply, meca.cc:157-181),
 // for one fiber; s.x must hold the fiber's slice of X, X(j) gives any entry.
-template <int D, typename Op>
-__device__ __forceinline__ void fiber_operator_op(const FiberArgs& a, FiberLds<D>& s, const Op& X, uint32_t f)
+template <int D, int P, typename Op>
+__device__ __forceinline__ void fiber_operator_op(const FiberArgs& a, FiberLds<D, P>& s, const Op& X, uint32_t f)
 {
     const uint32_t np = a.np[f], idx = a.idx[f], seg = a.seg[f];
     fiber_forces_op<D>(a, s, X, np, idx, seg, a.rfrig[f], true, true);
@@ -333,8 +335,8 @@ __device__ __forceinline__ void fiber_operator_op(const FiberArgs& a, FiberLds<D
     __syncthreads();
 }
 
-template <int D>
-__device__ __forceinline__ void fiber_operator(const FiberArgs& a, FiberLds<D>& s, const double* __restrict__ X,
+template <int D, int P>
+__device__ __forceinline__ void fiber_operator(const FiberArgs& a, FiberLds<D, P>& s, const double* __restrict__ X,
                                                uint32_t f)
 {
     fiber_operator_op<D>(a, s, Plain{ X }, f);

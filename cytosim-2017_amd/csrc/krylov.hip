@@ -2111,6 +2111,14 @@ int tile_threads_of(const Context& c)
     return 512;
 }
 
+// the two-launch iteration takes fibers of at most TpCfg<512>::max_points
+// points (a tile of one fiber, a thread per unknown); a system with a longer
+// fiber runs the four-launch iteration
+bool tiles_fit(const Context& c)
+{
+    return c.largest <= (uint32_t)TpCfg<512>::max_points(c.dim);
+}
+
 int tile_max_points(const Context& c, int threads)
 {
     return threads == 256 ? TpCfg<256>::max_points(c.dim) : TpCfg<512>::max_points(c.dim);
@@ -2120,6 +2128,7 @@ double krylov_time_launch(Context& c, int precond, int launches)
 {
     if (c.nsol && !c.solid_tiles)
         throw Error(CYMG_ERR_UNSUPPORTED, "a Solid is too large for the two-launch iteration");
+    if (!tiles_fit(c)) throw Error(CYMG_ERR_UNSUPPORTED, "a fiber is too long for the two-launch iteration");
     tiled_buffers(c);
     const int pairs = std::max(1, launches / 2);
     const bool small = tile_threads_of(c) == 256;
@@ -2142,8 +2151,8 @@ void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter
                   cymg_solve_report* rep)
 {
     // tiles hold whole fibers or one Solid of at most kSolidTileRows unknowns;
-    // a larger Solid runs the four-launch iteration
-    if (c.krylov_fused && (!c.nsol || c.solid_tiles)) {
+    // a larger Solid or a longer fiber runs the four-launch iteration
+    if (c.krylov_fused && (!c.nsol || c.solid_tiles) && tiles_fit(c)) {
         tiled_buffers(c);
         const bool small = tile_threads_of(c) == 256;
         auto go = [&](auto dim, auto nt) {

@@ -696,36 +696,167 @@ static size_t inv_mfma_lds_bytes(int nbmax, size_t elem)
     return (size_t)ld * nbmax * elem + (2 * kInvNB + 4) * sizeof(int);
 }
 
-// Fibers grouped by block size for the register kernels: [<=32 | <=64 | <=96 |
-// <=128 | larger], uploaded once per layout.  Stored as [<=96 | <=64 | <=32 |
-// <=128 | larger] so that the matrix-core kernel takes the blocks up to 96
-// unknowns in one range, largest class first.
+// LDS of one workgroup on gfx950
+constexpr size_t kLdsBudget = 160 * 1024;
+
+// a block the workgroup-local kernels take (assembly and inversion in LDS, in
+// either precision); larger blocks are assembled column by column and
+// inverted in device memory
+static bool lds_block(int D, int bs)
+{
+    return precond_lds_bytes(D, bs, CYMG_PRECOND_FP32) <= kLdsBudget;
+}
+
+// Fibers grouped by block size: [<=32 | <=64 | <=96 | <=128 | larger in LDS |
+// beyond LDS], made once per layout.  Stored as [<=96 | <=64 | <=32 | <=128 |
+// larger | beyond] so that the matrix-core kernel takes the blocks up to 96
+// unknowns in one range, largest class first, and the classes the LDS
+// kernels take are one range from the start.
 static void gj_lists(Context& c)
 {
-    if (c.gj_nfib == c.nfib && c.gj_largest == c.largest && !c.gj_start.empty()) return;
+    if (c.gj_gen == c.layout_gen && !c.gj_start.empty()) return;
     const int edges[4] = { 32, 64, 96, 128 };
-    std::vector<uint32_t> lists[5];
+    std::vector<uint32_t> lists[6];
+    int bs_lds = 0;
     for (uint32_t f = 0; f < c.nfib; ++f) {
         const int bs = c.dim * (int)c.h_np[f];
         int b = 0;
-        while (b < 4 && bs > edges[b]) ++b;
+        if (!lds_block(c.dim, bs)) {
+            b = 5;
+        } else {
+            while (b < 4 && bs > edges[b]) ++b;
+            bs_lds = std::max(bs_lds, bs);
+        }
         lists[b].push_back(f);
     }
     std::vector<uint32_t> all;
-    c.gj_start.assign(10, 0);   // [b] start, [5 + b] end of class b
-    const int order[5] = { 2, 1, 0, 3, 4 };
+    c.gj_start.assign(12, 0);   // [b] start, [6 + b] end of class b
+    const int order[6] = { 2, 1, 0, 3, 4, 5 };
     for (int b : order) {
         c.gj_start[b] = (uint32_t)all.size();
         all.insert(all.end(), lists[b].begin(), lists[b].end());
-        c.gj_start[5 + b] = (uint32_t)all.size();
+        c.gj_start[6 + b] = (uint32_t)all.size();
     }
     c.d_gj_list.resize(all.size() ? all.size() : 1);
     if (!all.empty()) {
         CYMG_HIP(hipMemcpyAsync(c.d_gj_list.get(), all.data(), all.size() * 4, hipMemcpyHostToDevice, c.stream));
         CYMG_HIP(hipStreamSynchronize(c.stream));   // pageable source
     }
-    c.gj_nfib = c.nfib;
-    c.gj_largest = c.largest;
+    c.gj_gen = c.layout_gen;
+    c.gj_bs_lds = bs_lds;
+}
+
+// Gauss-Jordan inverse with partial pivoting in device memory, for the blocks
+// beyond LDS: one workgroup per block, in place on the FP64 block M (column
+// major), then stored in T.  Step k takes the largest magnitude of column k
+// on or below the diagonal (the first on ties, dgetrf's rule), swaps it into
+// row k, and eliminates column k from every other row; the row interchanges
+// are undone on the columns at the end.  A zero or NaN pivot leaves the block
+// unused (identity), as the workgroup-local kernels do.
+constexpr int kBigThreads = 512;
+constexpr int kBigMaxBs = CYMG_MAX_FIBER_POINTS * 3;
+
+template <typename T>
+__global__ __launch_bounds__(kBigThreads) void k_block_gj_big(const uint32_t* __restrict__ list,
+                                                              const uint32_t* __restrict__ npv,
+                                                              const uint64_t* __restrict__ blkoff, int D,
+                                                              double* __restrict__ M, T* __restrict__ out,
+                                                              int* __restrict__ useblk)
+{
+    __shared__ double col[kBigMaxBs];
+    __shared__ int perm[kBigMaxBs];
+    __shared__ double rv[kBigThreads / kWave];
+    __shared__ int ri[kBigThreads / kWave];
+    __shared__ int info;
+    const uint32_t f = list[blockIdx.x];
+    const int bs = D * (int)npv[f];
+    double* A = M + blkoff[f];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+    constexpr int NW = kBigThreads / kWave;
+    if (tid == 0) info = 0;
+    __syncthreads();
+    for (int k = 0; k < bs; ++k) {
+        // pivot search over rows k..bs-1 of column k
+        double bv = -1.0;
+        int bi = bs;
+        for (int i = k + tid; i < bs; i += kBigThreads) {
+            const double v = fabs(A[i + (size_t)bs * k]);
+            if (v > bv || (v != v && bv == bv)) { bv = v; bi = i; }
+        }
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const double ov = __shfl_xor(bv, o, kWave);
+            const int oi = __shfl_xor(bi, o, kWave);
+            if (ov > bv || (ov == bv && oi < bi) || (ov != ov && bv == bv)) { bv = ov; bi = oi; }
+        }
+        if (lane == 0) { rv[w] = bv; ri[w] = bi; }
+        __syncthreads();
+        bv = rv[0];
+        bi = ri[0];
+        for (int q = 1; q < NW; ++q)
+            if (rv[q] > bv || (rv[q] == bv && ri[q] < bi) || (rv[q] != rv[q] && bv == bv)) { bv = rv[q]; bi = ri[q]; }
+        if (!(bv > 0.0)) {
+            if (tid == 0) info = k + 1;
+            break;
+        }
+        const int p = bi;
+        if (tid == 0) perm[k] = p;
+        if (p != k)
+            for (int j = tid; j < bs; j += kBigThreads) {
+                const double t = A[k + (size_t)bs * j];
+                A[k + (size_t)bs * j] = A[p + (size_t)bs * j];
+                A[p + (size_t)bs * j] = t;
+            }
+        __syncthreads();
+        for (int i = tid; i < bs; i += kBigThreads) col[i] = A[i + (size_t)bs * k];
+        __syncthreads();
+        const double r = 1.0 / col[k];
+        // one wavefront per column, lanes over rows
+        for (int j = w; j < bs; j += NW) {
+            double* cj = A + (size_t)bs * j;
+            if (j == k) {
+                for (int i = lane; i < bs; i += kWave) cj[i] = (i == k) ? r : -col[i] * r;
+            } else {
+                const double akj = cj[k] * r;
+                for (int i = lane; i < bs; i += kWave) cj[i] = (i == k) ? akj : cj[i] - col[i] * akj;
+            }
+        }
+        __syncthreads();
+    }
+    const bool ok = info == 0;
+    if (ok) {
+        for (int k = bs - 1; k >= 0; --k) {
+            const int p = perm[k];
+            if (p != k)
+                for (int i = tid; i < bs; i += kBigThreads) {
+                    const double t = A[i + (size_t)bs * k];
+                    A[i + (size_t)bs * k] = A[i + (size_t)bs * p];
+                    A[i + (size_t)bs * p] = t;
+                }
+            __syncthreads();
+        }
+        if constexpr (!std::is_same<T, double>::value) {
+            T* dst = out + blkoff[f];
+            for (size_t e = tid; e < (size_t)bs * bs; e += kBigThreads) dst[e] = (T)A[e];
+        }
+    }
+    if (tid == 0) useblk[f] = ok;
+}
+
+// blocks list[begin .. begin + count): assembled in d_binv column by column,
+// then (invert) inverted there and stored in out
+static void launch_big(Context& c, uint32_t begin, uint32_t count, int precision, bool invert, double* M)
+{
+    if (!count) return;
+    const uint32_t* list = c.d_gj_list.get() + begin;
+    launch_block_columns(c, list, count, M);
+    if (!invert) return;
+    if (precision == CYMG_PRECOND_FP32)
+        hipLaunchKernelGGL(k_block_gj_big<float>, dim3(count), dim3(kBigThreads), 0, c.stream, list, c.d_np.get(),
+                           c.d_blkoff.get(), c.dim, M, c.d_binv32.get(), c.d_useblk.get());
+    else
+        hipLaunchKernelGGL(k_block_gj_big<double>, dim3(count), dim3(kBigThreads), 0, c.stream, list, c.d_np.get(),
+                           c.d_blkoff.get(), c.dim, M, c.d_binv.get(), c.d_useblk.get());
+    CYMG_HIP(hipGetLastError());
 }
 
 template <typename T, int BSMAX, int NW>
@@ -783,12 +914,10 @@ void launch_precond_build(Context& c, int precision, double* block_out)
         c.pc_ready = true;
         return;
     }
-    const int D = c.dim;
-    const int bs_max = D * (int)c.largest;
-    const size_t lds = precond_lds_bytes(D, bs_max, precision);
-    if (lds > 160 * 1024)
-        throw Error(CYMG_ERR_UNSUPPORTED, "preconditioner block of " + std::to_string(bs_max) +
-                                              " unknowns exceeds the LDS budget of one workgroup");
+    gj_lists(c);
+    const std::vector<uint32_t>& st = c.gj_start;   // [b] start, [6 + b] end of class b
+    const uint32_t nlds = st[6 + 4];                 // classes 0-4: one range from the start
+    const uint32_t big0 = st[5], nbig = st[6 + 5] - st[5];
     PcArgs a;
     a.np = c.d_np.get(); a.idx = c.d_idx.get(); a.seg = c.d_seg.get(); a.blkoff = c.d_blkoff.get();
     a.mob = c.d_mob.get(); a.rfrig = c.d_rfrig.get(); a.diff = c.d_diff.get(); a.jd = c.d_jd.get();
@@ -796,15 +925,17 @@ void launch_precond_build(Context& c, int precision, double* block_out)
     a.Brow = c.mB.rowptr.get(); a.Bcol = c.mB.col.get(); a.Bval = c.mB.val.get(); a.Buse = c.mB.used;
     a.Crow = c.mC.rowptr.get(); a.Ccol = c.mC.col.get(); a.Cval = c.mC.val.get(); a.Cuse = c.mC.used;
     a.dt = c.prm.time_step;
-    a.bs_max = bs_max;
+    a.bs_max = c.gj_bs_lds;
     a.factor = 1;
     a.method = c.pc_method;
     a.list = nullptr;
     c.d_useblk.resize(c.nfib);
+    const uint32_t* lds_list = c.d_gj_list.get();
     if (block_out) {
         // the assembled blocks themselves
         a.factor = 0;
-        launch_lds_build(c, a, CYMG_PRECOND_FP64, block_out, nullptr, c.nfib);
+        launch_lds_build(c, a, CYMG_PRECOND_FP64, block_out, lds_list, nlds);
+        launch_big(c, big0, nbig, CYMG_PRECOND_FP64, false, block_out);
         CYMG_HIP(hipGetLastError());
         return;
     }
@@ -812,37 +943,36 @@ void launch_precond_build(Context& c, int precision, double* block_out)
     if (precision == CYMG_PRECOND_FP32) c.d_binv32.resize(c.blk_total);
     void* dst = precision == CYMG_PRECOND_FP32 ? (void*)c.d_binv32.get() : (void*)c.d_binv.get();
     if (c.pc_method == 1 || c.pc_method == 2) {
-        // LDS kernels for every block: 1 getrf + getri, 2 Gauss-Jordan
-        launch_lds_build(c, a, precision, dst, nullptr, c.nfib);
+        // LDS kernels for every block they take: 1 getrf + getri, 2 Gauss-Jordan
+        launch_lds_build(c, a, precision, dst, lds_list, nlds);
     } else {
-        // assemble every block into d_binv, invert by size class (method 0:
-        // in registers, method 3: on the matrix cores); blocks beyond those
-        // classes are rebuilt and inverted in LDS
-        gj_lists(c);
+        // assemble the blocks into d_binv, invert by size class (method 0:
+        // in registers, method 3: on the matrix cores); the classes beyond
+        // are rebuilt and inverted in LDS
         PcArgs e = a;
         e.factor = 0;
-        launch_lds_build(c, e, CYMG_PRECOND_FP64, c.d_binv.get(), nullptr, c.nfib);
-        const std::vector<uint32_t>& st = c.gj_start;   // [b] start, [5 + b] end of class b
+        launch_lds_build(c, e, CYMG_PRECOND_FP64, c.d_binv.get(), lds_list, nlds);
         if (c.pc_method == 3) {
-            const uint32_t n96 = st[5 + 0] - st[2];
+            const uint32_t n96 = st[6 + 0] - st[2];
             if (precision == CYMG_PRECOND_FP32) launch_inv_mfma<float>(c, st[2], n96, c.d_binv32.get());
             else launch_inv_mfma<double>(c, st[2], n96, c.d_binv.get());
-            launch_lds_build(c, a, precision, dst, c.d_gj_list.get() + st[3], st[5 + 4] - st[3]);
+            launch_lds_build(c, a, precision, dst, c.d_gj_list.get() + st[3], st[6 + 4] - st[3]);
         } else if (precision == CYMG_PRECOND_FP32) {
             float* o = c.d_binv32.get();
-            launch_gj<float, 32, 1>(c, st[0], st[5 + 0], o);
-            launch_gj<float, 64, 1>(c, st[1], st[5 + 1], o);
-            launch_gj<float, 96, 2>(c, st[2], st[5 + 2], o);
-            launch_gj<float, 128, 2>(c, st[3], st[5 + 3], o);
-            launch_lds_build(c, a, precision, dst, c.d_gj_list.get() + st[4], st[5 + 4] - st[4]);
+            launch_gj<float, 32, 1>(c, st[0], st[6 + 0], o);
+            launch_gj<float, 64, 1>(c, st[1], st[6 + 1], o);
+            launch_gj<float, 96, 2>(c, st[2], st[6 + 2], o);
+            launch_gj<float, 128, 2>(c, st[3], st[6 + 3], o);
+            launch_lds_build(c, a, precision, dst, c.d_gj_list.get() + st[4], st[6 + 4] - st[4]);
         } else {
             double* o = c.d_binv.get();
-            launch_gj<double, 32, 1>(c, st[0], st[5 + 0], o);
-            launch_gj<double, 64, 1>(c, st[1], st[5 + 1], o);
-            launch_gj<double, 96, 2>(c, st[2], st[5 + 2], o);
-            launch_lds_build(c, a, precision, dst, c.d_gj_list.get() + st[3], st[5 + 4] - st[3]);
+            launch_gj<double, 32, 1>(c, st[0], st[6 + 0], o);
+            launch_gj<double, 64, 1>(c, st[1], st[6 + 1], o);
+            launch_gj<double, 96, 2>(c, st[2], st[6 + 2], o);
+            launch_lds_build(c, a, precision, dst, c.d_gj_list.get() + st[3], st[6 + 4] - st[3]);
         }
     }
+    launch_big(c, big0, nbig, precision, true, c.d_binv.get());
     CYMG_HIP(hipGetLastError());
     c.pc_precision = precision;
     c.pc_ready = true;

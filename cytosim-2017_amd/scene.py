@@ -344,10 +344,15 @@ def config2(seed=2, nfib=1000, nlinks=20000, radius=10.0, lmin=2.0, lmax=15.0):
     return sc
 
 
-def mini(dim=3, seed=3, nfib=8, nlinks=20, radius=1.5, lmin=2.0, lmax=6.0, binding_range=0.5):
-    """Small crosslinked scene for operator-level parity checks."""
+def mini(dim=3, seed=3, nfib=8, nlinks=20, radius=1.5, lmin=2.0, lmax=6.0, binding_range=0.5, lengths=None):
+    """Small crosslinked scene for operator-level parity checks (fiber lengths
+    uniform in [lmin, lmax], or the given lengths)."""
     rng = np.random.default_rng(seed)
-    lengths = rng.uniform(lmin, lmax, nfib)
+    if lengths is None:
+        lengths = rng.uniform(lmin, lmax, nfib)
+    else:
+        lengths = np.asarray(lengths, dtype=np.float64)
+        nfib = lengths.size
     nbp, cut, pts = _straight_fibers(rng, dim, lengths, 0.5, radius)
     sc = Scene(dim=dim, time_step=0.005, kT=0.0042, viscosity=0.02, hydro_radius0=0.0125,
                hydro_radius1=10.0, nb_points=nbp, segmentation=cut,

@@ -28,8 +28,12 @@ extern "C" {
 #define CYMG_ERR_CONVERGENCE 4   /* "convergence failure in solver" (meca.cc:795) */
 #define CYMG_ERR_UNSUPPORTED 5   /* size beyond the kernels' stated limits */
 
-/* largest fiber handled by the per-fiber kernels (points) */
-#define CYMG_MAX_FIBER_POINTS 128
+/* largest fiber handled by the per-fiber kernels (points).  Fibers of up to
+   170 points run the two-launch iteration; a system with a longer fiber runs
+   the four-launch one.  Preconditioner blocks beyond the workgroup-local
+   kernels (more than 111 unknowns in 3D, 128 in 2D) are assembled column by
+   column and inverted in device memory. */
+#define CYMG_MAX_FIBER_POINTS 512
 #define CYMG_MAX_SOLID_POINTS 65536
 
 typedef struct cymg_context cymg_context;
