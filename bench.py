@@ -99,6 +99,8 @@ def parse():
     ap.add_argument("--guess-train-steps", type=int, default=4000)
     ap.add_argument("--guess-batch", type=int, default=512)
     ap.add_argument("--guess-lr", type=float, default=1e-3)
+    ap.add_argument("--batched-guess-lr", type=float, default=2e-4,
+                    help="Adam step of the batched workload's training (its loss rose at 1e-3)")
     ap.add_argument("--timed-launches", type=int, default=40,
                     help="iteration launches timed with HIP events for the roofline")
     ap.add_argument("--traffic", type=float, default=None,
@@ -353,7 +355,7 @@ def batched(args, pkg, scenes, torch):
         if guess == pkg.GUESS_LEARNED:
             if weights is None:
                 info, weights = train_guess(pkg, mb, big, args.batched_rollout, args.guess_train_steps,
-                                            args.guess_batch, args.guess_lr, args.tol, torch,
+                                            args.guess_batch, args.batched_guess_lr, args.tol, torch,
                                             lambda mm: pkg.assemble(mm, big, True, False))
                 rec["untimed_training"] = info
                 mb.prepare(big.points)

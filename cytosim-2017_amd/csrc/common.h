@@ -221,7 +221,8 @@ void launch_forces(Context& c, const double* X, double* Y, bool with_rigidity);
 void launch_apply(Context& c);
 void launch_gather_points(Context& c, const uint32_t* map, const double* src, double* dst, uint32_t npts);
 void launch_tensions(Context& c);
-void launch_block_columns(Context& c, const uint32_t* list, uint32_t count, double* M);
+void launch_block_columns(Context& c, const uint32_t* list, uint32_t count, const uint64_t* off, int by_position,
+                          double* M);
 // launchers (precond.hip)
 void launch_precond_build(Context& c, int precision, double* block_out = nullptr);
 void launch_precond_apply(Context& c, const double* X, double* Y);

@@ -110,6 +110,13 @@ struct Context {
     std::vector<uint32_t> gj_start;    // class boundaries in d_gj_list
     uint64_t gj_gen = ~0ull;           // layout generation the lists were made for
     int gj_bs_lds = 0;                 // largest block of the workgroup-local kernels
+    // blocks beyond LDS: work space of the blocked Gauss-Jordan, per block by
+    // position in their class (matrix, pivot rows and steps, pivot-row copies)
+    uint32_t big_bsmax = 0;
+    DevBuf<uint64_t> d_big_sofs;
+    DevBuf<uint32_t> d_big_iofs;
+    DevBuf<double> d_big_A, d_big_AP;
+    DevBuf<int> d_big_piv, d_big_step, d_big_fail;
     DevBuf<double> d_binv;
     DevBuf<float> d_binv32;
     DevBuf<int> d_useblk;

@@ -892,13 +892,14 @@ struct UnitOperand {
 
 template <int D>
 __global__ __launch_bounds__(kWave) void k_block_columns(FiberArgs a, const uint32_t* __restrict__ list,
-                                                         const uint64_t* __restrict__ blkoff, double* __restrict__ M)
+                                                         const uint64_t* __restrict__ off, int by_position,
+                                                         double* __restrict__ M)
 {
     __shared__ FiberLds<D> s;
     const uint32_t f = list[blockIdx.x];
     const uint32_t np = a.np[f], o = D * a.idx[f];
     const int bs = D * (int)np;
-    double* B = M + blkoff[f];
+    double* B = M + (by_position ? off[blockIdx.x] : off[f]);
     load_factors<D>(a, s, a.seg[f], (int)np - 1);
     for (int j = blockIdx.y; j < bs; j += gridDim.y) {
         for (int i = threadIdx.x; i < bs; i += kWave) s.x[i] = (i == j) ? 1.0 : 0.0;
@@ -909,7 +910,9 @@ __global__ __launch_bounds__(kWave) void k_block_columns(FiberArgs a, const uint
     }
 }
 
-void launch_block_columns(Context& c, const uint32_t* list, uint32_t count, double* M)
+// off: block offsets by fiber (by_position 0) or by position in list
+void launch_block_columns(Context& c, const uint32_t* list, uint32_t count, const uint64_t* off, int by_position,
+                          double* M)
 {
     if (!count) return;
     FiberArgs a = make_fiber_args(c);
@@ -917,9 +920,9 @@ void launch_block_columns(Context& c, const uint32_t* list, uint32_t count, doub
     const uint32_t cols = std::max<uint32_t>(1, std::min<uint32_t>(c.dim * c.largest, 4096u / count + 1));
     const dim3 g(count, cols), b(kWave);
     if (c.dim == 3)
-        hipLaunchKernelGGL(k_block_columns<3>, g, b, 0, c.stream, a, list, c.d_blkoff.get(), M);
+        hipLaunchKernelGGL(k_block_columns<3>, g, b, 0, c.stream, a, list, off, by_position, M);
     else
-        hipLaunchKernelGGL(k_block_columns<2>, g, b, 0, c.stream, a, list, c.d_blkoff.get(), M);
+        hipLaunchKernelGGL(k_block_columns<2>, g, b, 0, c.stream, a, list, off, by_position, M);
     CYMG_HIP(hipGetLastError());
 }
 

@@ -698,6 +698,8 @@ static size_t inv_mfma_lds_bytes(int nbmax, size_t elem)
 
 // LDS of one workgroup on gfx950
 constexpr size_t kLdsBudget = 160 * 1024;
+// columns per panel of the blocked Gauss-Jordan of the blocks beyond LDS
+constexpr int kBigPanel = 16;
 
 // a block the workgroup-local kernels take (assembly and inversion in LDS, in
 // either precision); larger blocks are assembled column by column and
@@ -744,42 +746,92 @@ static void gj_lists(Context& c)
     }
     c.gj_gen = c.layout_gen;
     c.gj_bs_lds = bs_lds;
+    // work space of the blocks beyond LDS, by position in their class
+    std::vector<uint64_t> sofs;
+    std::vector<uint32_t> iofs;
+    uint64_t sa = 0;
+    uint32_t si = 0, bmax = 0;
+    for (uint32_t f : lists[5]) {
+        const uint32_t bs = c.dim * c.h_np[f];
+        sofs.push_back(sa);
+        iofs.push_back(si);
+        sa += (uint64_t)bs * bs;
+        si += bs;
+        bmax = std::max(bmax, bs);
+    }
+    c.big_bsmax = bmax;
+    if (!sofs.empty()) {
+        c.d_big_sofs.resize(sofs.size());
+        c.d_big_iofs.resize(iofs.size());
+        c.d_big_A.resize(sa);
+        c.d_big_AP.resize((size_t)kBigPanel * si);
+        c.d_big_piv.resize(si);
+        c.d_big_step.resize(si);
+        c.d_big_fail.resize(sofs.size());
+        CYMG_HIP(hipMemcpyAsync(c.d_big_sofs.get(), sofs.data(), sofs.size() * 8, hipMemcpyHostToDevice, c.stream));
+        CYMG_HIP(hipMemcpyAsync(c.d_big_iofs.get(), iofs.data(), iofs.size() * 4, hipMemcpyHostToDevice, c.stream));
+        CYMG_HIP(hipStreamSynchronize(c.stream));   // pageable sources
+    }
 }
 
-// Gauss-Jordan inverse with partial pivoting in device memory, for the blocks
-// beyond LDS: one workgroup per block, in place on the FP64 block M (column
-// major), then stored in T.  Step k takes the largest magnitude of column k
-// on or below the diagonal (the first on ties, dgetrf's rule), swaps it into
-// row k, and eliminates column k from every other row; the row interchanges
-// are undone on the columns at the end.  A zero or NaN pivot leaves the block
-// unused (identity), as the workgroup-local kernels do.
+// Gauss-Jordan inverse of the blocks beyond LDS, blocked by panels of
+// kBigPanel columns, in device memory.  Pivot rows are chosen among the rows
+// not yet used (largest magnitude in the column, the first on ties), without
+// row interchanges: after all steps the array holds the inverse with its rows
+// and columns permuted, inverse(s, c) = A(piv[s], step[c]).  A panel is
+// eliminated by one workgroup (k_big_panel); its transformation reaches every
+// other column as one product over the panel, A(:, j) <- [i not pivot of the
+// panel] A(:, j) + W A(P, j) with W the eliminated panel and P its pivot rows,
+// spread over many workgroups (k_big_update).  A zero or NaN pivot leaves the
+// block unused (identity), as the workgroup-local kernels do.
 constexpr int kBigThreads = 512;
 constexpr int kBigMaxBs = CYMG_MAX_FIBER_POINTS * 3;
+constexpr int kBigRows = 256, kBigCols = 64;
 
-template <typename T>
-__global__ __launch_bounds__(kBigThreads) void k_block_gj_big(const uint32_t* __restrict__ list,
-                                                              const uint32_t* __restrict__ npv,
-                                                              const uint64_t* __restrict__ blkoff, int D,
-                                                              double* __restrict__ M, T* __restrict__ out,
-                                                              int* __restrict__ useblk)
+struct BigArgs {
+    const uint32_t* list;
+    const uint32_t* np;
+    const uint64_t* sofs;
+    const uint32_t* iofs;
+    double* A;
+    double* AP;
+    int* piv;
+    int* step;
+    int* fail;
+    int D;
+};
+
+__global__ __launch_bounds__(256) void k_big_init(BigArgs g)
+{
+    const uint32_t b = blockIdx.x;
+    const int bs = g.D * (int)g.np[g.list[b]];
+    int* step = g.step + g.iofs[b];
+    for (int i = threadIdx.x; i < bs; i += blockDim.x) step[i] = bs;   // bs: not yet a pivot row
+    if (threadIdx.x == 0) g.fail[b] = 0;
+}
+
+__global__ __launch_bounds__(kBigThreads) void k_big_panel(BigArgs g, int k0)
 {
     __shared__ double col[kBigMaxBs];
-    __shared__ int perm[kBigMaxBs];
     __shared__ double rv[kBigThreads / kWave];
     __shared__ int ri[kBigThreads / kWave];
-    __shared__ int info;
-    const uint32_t f = list[blockIdx.x];
-    const int bs = D * (int)npv[f];
-    double* A = M + blkoff[f];
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+    __shared__ int sp[kBigPanel];
+    __shared__ double apj[kBigPanel];
     constexpr int NW = kBigThreads / kWave;
-    if (tid == 0) info = 0;
-    __syncthreads();
-    for (int k = 0; k < bs; ++k) {
-        // pivot search over rows k..bs-1 of column k
+    const uint32_t b = blockIdx.x;
+    const int bs = g.D * (int)g.np[g.list[b]];
+    if (k0 >= bs || g.fail[b]) return;
+    double* A = g.A + g.sofs[b];
+    int* step = g.step + g.iofs[b];
+    int* piv = g.piv + g.iofs[b];
+    const int m = bs - k0 < kBigPanel ? bs - k0 : kBigPanel;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+    for (int q = 0; q < m; ++q) {
+        const int k = k0 + q;
         double bv = -1.0;
         int bi = bs;
-        for (int i = k + tid; i < bs; i += kBigThreads) {
+        for (int i = tid; i < bs; i += kBigThreads) {
+            if (step[i] != bs) continue;
             const double v = fabs(A[i + (size_t)bs * k]);
             if (v > bv || (v != v && bv == bv)) { bv = v; bi = i; }
         }
@@ -792,70 +844,118 @@ __global__ __launch_bounds__(kBigThreads) void k_block_gj_big(const uint32_t* __
         __syncthreads();
         bv = rv[0];
         bi = ri[0];
-        for (int q = 1; q < NW; ++q)
-            if (rv[q] > bv || (rv[q] == bv && ri[q] < bi) || (rv[q] != rv[q] && bv == bv)) { bv = rv[q]; bi = ri[q]; }
+        for (int t = 1; t < NW; ++t)
+            if (rv[t] > bv || (rv[t] == bv && ri[t] < bi) || (rv[t] != rv[t] && bv == bv)) { bv = rv[t]; bi = ri[t]; }
         if (!(bv > 0.0)) {
-            if (tid == 0) info = k + 1;
-            break;
+            if (tid == 0) g.fail[b] = k + 1;
+            return;
         }
         const int p = bi;
-        if (tid == 0) perm[k] = p;
-        if (p != k)
-            for (int j = tid; j < bs; j += kBigThreads) {
-                const double t = A[k + (size_t)bs * j];
-                A[k + (size_t)bs * j] = A[p + (size_t)bs * j];
-                A[p + (size_t)bs * j] = t;
-            }
-        __syncthreads();
         for (int i = tid; i < bs; i += kBigThreads) col[i] = A[i + (size_t)bs * k];
+        if (tid < m) apj[tid] = A[p + (size_t)bs * (k0 + tid)];
         __syncthreads();
-        const double r = 1.0 / col[k];
-        // one wavefront per column, lanes over rows
-        for (int j = w; j < bs; j += NW) {
-            double* cj = A + (size_t)bs * j;
-            if (j == k) {
-                for (int i = lane; i < bs; i += kWave) cj[i] = (i == k) ? r : -col[i] * r;
-            } else {
-                const double akj = cj[k] * r;
-                for (int i = lane; i < bs; i += kWave) cj[i] = (i == k) ? akj : cj[i] - col[i] * akj;
+        const double r = 1.0 / col[p];
+        for (int i = tid; i < bs; i += kBigThreads) {
+            const double ci = col[i];
+            for (int t = 0; t < m; ++t) {
+                double* cj = A + (size_t)bs * (k0 + t);
+                if (t == q) cj[i] = (i == p) ? r : -ci * r;
+                else {
+                    const double a = apj[t] * r;
+                    cj[i] = (i == p) ? a : cj[i] - ci * a;
+                }
             }
         }
+        if (tid == 0) { step[p] = k; piv[k] = p; sp[q] = p; }
         __syncthreads();
     }
-    const bool ok = info == 0;
-    if (ok) {
-        for (int k = bs - 1; k >= 0; --k) {
-            const int p = perm[k];
-            if (p != k)
-                for (int i = tid; i < bs; i += kBigThreads) {
-                    const double t = A[i + (size_t)bs * k];
-                    A[i + (size_t)bs * k] = A[i + (size_t)bs * p];
-                    A[i + (size_t)bs * p] = t;
-                }
-            __syncthreads();
-        }
-        if constexpr (!std::is_same<T, double>::value) {
-            T* dst = out + blkoff[f];
-            for (size_t e = tid; e < (size_t)bs * bs; e += kBigThreads) dst[e] = (T)A[e];
-        }
-    }
-    if (tid == 0) useblk[f] = ok;
+    // the panel's pivot rows of every column, before the update overwrites them
+    double* AP = g.AP + (size_t)kBigPanel * g.iofs[b];
+    for (int t = 0; t < m; ++t)
+        for (int j = tid; j < bs; j += kBigThreads) AP[(size_t)bs * t + j] = A[sp[t] + (size_t)bs * j];
 }
 
-// blocks list[begin .. begin + count): assembled in d_binv column by column,
-// then (invert) inverted there and stored in out
+__global__ __launch_bounds__(kBigRows) void k_big_update(BigArgs g, int k0)
+{
+    __shared__ double aps[kBigPanel][kBigCols];
+    const uint32_t b = blockIdx.x;
+    const int bs = g.D * (int)g.np[g.list[b]];
+    const int j0 = blockIdx.y * kBigCols, i = blockIdx.z * kBigRows + threadIdx.x;
+    if (k0 >= bs || j0 >= bs || blockIdx.z * kBigRows >= (unsigned)bs || g.fail[b]) return;
+    double* A = g.A + g.sofs[b];
+    const int* step = g.step + g.iofs[b];
+    const double* AP = g.AP + (size_t)kBigPanel * g.iofs[b];
+    const int m = bs - k0 < kBigPanel ? bs - k0 : kBigPanel;
+    for (int e = threadIdx.x; e < kBigPanel * kBigCols; e += kBigRows) {
+        const int t = e / kBigCols, jj = e % kBigCols;
+        aps[t][jj] = (t < m && j0 + jj < bs) ? AP[(size_t)bs * t + j0 + jj] : 0.0;
+    }
+    __syncthreads();
+    if (i >= bs) return;
+    double W[kBigPanel];
+#pragma unroll
+    for (int t = 0; t < kBigPanel; ++t) W[t] = t < m ? A[i + (size_t)bs * (k0 + t)] : 0.0;
+    const bool pivot_row = step[i] >= k0 && step[i] < k0 + m;
+    const int je = bs - j0 < kBigCols ? bs - j0 : kBigCols;
+    for (int jj = 0; jj < je; ++jj) {
+        const int j = j0 + jj;
+        if (j >= k0 && j < k0 + m) continue;
+        double v = pivot_row ? 0.0 : A[i + (size_t)bs * j];
+#pragma unroll
+        for (int t = 0; t < kBigPanel; ++t) v += W[t] * aps[t][jj];
+        A[i + (size_t)bs * j] = v;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_big_store(BigArgs g, const uint64_t* __restrict__ blkoff,
+                                                   T* __restrict__ out, int* __restrict__ useblk)
+{
+    const uint32_t b = blockIdx.x, f = g.list[b];
+    const int bs = g.D * (int)g.np[f];
+    const bool ok = g.fail[b] == 0;
+    if (blockIdx.y == 0 && threadIdx.x == 0) useblk[f] = ok;
+    if (!ok) return;
+    const double* A = g.A + g.sofs[b];
+    const int* step = g.step + g.iofs[b];
+    const int* piv = g.piv + g.iofs[b];
+    T* dst = out + blkoff[f];
+    for (int cc = blockIdx.y; cc < bs; cc += gridDim.y) {
+        const double* src = A + (size_t)bs * step[cc];
+        for (int s2 = threadIdx.x; s2 < bs; s2 += blockDim.x) dst[s2 + (size_t)bs * cc] = (T)src[piv[s2]];
+    }
+}
+
+// blocks list[begin .. begin + count) of the class beyond LDS: assembled
+// column by column (into M by fiber offsets when only the blocks are asked
+// for), or assembled into the work space and inverted into d_binv / d_binv32
 static void launch_big(Context& c, uint32_t begin, uint32_t count, int precision, bool invert, double* M)
 {
     if (!count) return;
     const uint32_t* list = c.d_gj_list.get() + begin;
-    launch_block_columns(c, list, count, M);
-    if (!invert) return;
+    if (!invert) {
+        launch_block_columns(c, list, count, c.d_blkoff.get(), 0, M);
+        return;
+    }
+    launch_block_columns(c, list, count, c.d_big_sofs.get(), 1, c.d_big_A.get());
+    BigArgs g;
+    g.list = list; g.np = c.d_np.get(); g.sofs = c.d_big_sofs.get(); g.iofs = c.d_big_iofs.get();
+    g.A = c.d_big_A.get(); g.AP = c.d_big_AP.get(); g.piv = c.d_big_piv.get(); g.step = c.d_big_step.get();
+    g.fail = c.d_big_fail.get(); g.D = c.dim;
+    const int bsmax = (int)c.big_bsmax;
+    hipLaunchKernelGGL(k_big_init, dim3(count), dim3(256), 0, c.stream, g);
+    const dim3 ug(count, (bsmax + kBigCols - 1) / kBigCols, (bsmax + kBigRows - 1) / kBigRows);
+    for (int k0 = 0; k0 < bsmax; k0 += kBigPanel) {
+        hipLaunchKernelGGL(k_big_panel, dim3(count), dim3(kBigThreads), 0, c.stream, g, k0);
+        hipLaunchKernelGGL(k_big_update, ug, dim3(kBigRows), 0, c.stream, g, k0);
+    }
+    const dim3 sg(count, std::min(bsmax, 64));
     if (precision == CYMG_PRECOND_FP32)
-        hipLaunchKernelGGL(k_block_gj_big<float>, dim3(count), dim3(kBigThreads), 0, c.stream, list, c.d_np.get(),
-                           c.d_blkoff.get(), c.dim, M, c.d_binv32.get(), c.d_useblk.get());
+        hipLaunchKernelGGL(k_big_store<float>, sg, dim3(256), 0, c.stream, g, c.d_blkoff.get(), c.d_binv32.get(),
+                           c.d_useblk.get());
     else
-        hipLaunchKernelGGL(k_block_gj_big<double>, dim3(count), dim3(kBigThreads), 0, c.stream, list, c.d_np.get(),
-                           c.d_blkoff.get(), c.dim, M, c.d_binv.get(), c.d_useblk.get());
+        hipLaunchKernelGGL(k_big_store<double>, sg, dim3(256), 0, c.stream, g, c.d_blkoff.get(), c.d_binv.get(),
+                           c.d_useblk.get());
     CYMG_HIP(hipGetLastError());
 }
 
