@@ -1679,6 +1679,482 @@ void orc_inter_plane_i(orc_meca* m, uint32_t a1, uint32_t a2, double ca, const d
 #undef MC
 #undef MB
 
+/* ------------------------------------------------------------------------ */
+/* steric interactions: Simul::setStericInteractions (simul_solve.cc:12-188) */
+/* with PointGrid (point_grid.cc, point_grid.h) over Grid<DIM> with side     */
+/* regions (grid.h:354-370, 600-861).                                        */
+/*                                                                           */
+/* Items: the segments of the steric fibers ("loci", in fiber order) and the */
+/* points of the steric Solids with a radius above REAL_EPSILON ("points"),  */
+/* each kept in the list of the cell that holds its centre, in insertion     */
+/* order.  Where the reference compares two loci by the address of their     */
+/* records (`&bb < &aa`, point_grid.cc:205,295), which depends on the memory */
+/* the per-cell arrays were given, this restatement compares their ranks in  */
+/* the cell-major order (cell index, then insertion order).                  */
+
+typedef struct {
+    uint32_t pt;      /* matIndex of the first point of the segment */
+    uint32_t mec;     /* Mecable */
+    int first, last;  /* FiberLocus::isFirst / isLast */
+    double cut;       /* FiberLocus::len() = segmentation */
+    double rad, rng;  /* FatLocus::radius, range */
+} st_locus;
+
+typedef struct {
+    uint32_t pt;      /* matIndex of the point */
+    double rad;
+} st_point;
+
+typedef struct {
+    int D;
+    int gdim[3];
+    double inf[3], delta[3];
+    uint32_t ncells;
+    st_locus* loc; uint32_t nloc;
+    st_point* pnt; uint32_t npnt;
+    uint32_t* loc_start; uint32_t* loc_list;  /* per cell: [start[c], start[c+1]) in loc_list */
+    uint32_t* pnt_start; uint32_t* pnt_list;
+    uint32_t* loc_rank;                        /* cell-major rank of each locus */
+    double push, pull;
+    orc_meca* m;
+    orc_steric_call* log; uint32_t nlog, caplog;
+} st_grid;
+
+static const double* st_pos(const st_grid* g, uint32_t p) { return g->m->vPTS + (size_t)g->D * p; }
+
+static void st_record(st_grid* g, int kind, uint32_t a1, uint32_t a2, double ca, uint32_t b1, uint32_t b2, double cb,
+                      double len, double w)
+{
+    if (g->log && g->nlog < g->caplog) {
+        orc_steric_call* c = &g->log[g->nlog];
+        c->kind = kind; c->p[0] = a1; c->p[1] = a2; c->p[2] = b1; c->p[3] = b2;
+        c->ca = ca; c->cb = cb; c->len = len; c->w = w;
+    }
+    ++g->nlog;
+}
+
+/* the three Meca calls the steric checks make */
+static void st_long_link(st_grid* g, uint32_t a, uint32_t b, double len, double w)
+{
+    st_record(g, 0, a, b, 0.0, 0, 0, 0.0, len, w);
+    orc_inter_long_link_ee(g->m, a, b, len, w);
+}
+
+static void st_side_sliding_ie(st_grid* g, uint32_t a1, double ca, uint32_t b, double len, double w)
+{
+    st_record(g, 1, a1, a1 + 1, ca, b, 0, 0.0, len, w);
+    orc_inter_side_sliding_link_ie(g->m, a1, a1 + 1, ca, b, len, w);
+}
+
+static void st_side_sliding_ii(st_grid* g, uint32_t a1, double ca, uint32_t b1, double cb, double len, double w)
+{
+    st_record(g, 2, a1, a1 + 1, ca, b1, b1 + 1, cb, len, w);
+    orc_inter_side_sliding_link_ii(g->m, a1, a1 + 1, ca, b1, b1 + 1, cb, len, w);
+}
+
+/* Grid::index (grid.h:445-455) with the bounded image (grid.h:147-150) */
+static uint32_t st_cell(const st_grid* g, const double* w)
+{
+    uint32_t inx = 0;
+    for (int d = g->D - 1; d >= 0; --d) {
+        const double c = 0.0 + (w[d] - g->inf[d]) * g->delta[d];
+        const int s = g->gdim[d];
+        const int i = c <= 0 ? 0 : (c >= s ? s - 1 : (int)c);
+        inx = (uint32_t)g->gdim[d] * inx + (uint32_t)i;
+    }
+    return inx;
+}
+
+/* FiberLocus::projectPoint0 (fiber_locus.cc:25-49): dis is set only when the
+   projection falls inside [0, len] */
+static void st_project0(const st_grid* g, const st_locus* L, const double* w, double* abs_, double* dis)
+{
+    const int D = g->D;
+    const double *p1 = st_pos(g, L->pt), *p2 = st_pos(g, L->pt + 1);
+    double dx[3] = { 0, 0, 0 }, aw[3] = { 0, 0, 0 };
+    for (int d = 0; d < D; ++d) { dx[d] = p2[d] - p1[d]; aw[d] = w[d] - p1[d]; }
+    const double ls = L->cut;
+    *abs_ = v_dot(D, aw, dx) / ls;
+    if (0 <= *abs_ && *abs_ <= ls) *dis = v_dot(D, aw, aw) - *abs_ * *abs_;
+}
+
+/* checkPP (point_grid.cc:103-118): two spheres (Solid points or fiber ends) */
+static void st_check_pp(st_grid* g, uint32_t pa, const double* wa, double ra, uint32_t pb, const double* wb, double rb)
+{
+    const int D = g->D;
+    const double len = ra + rb;
+    double vab[3] = { 0, 0, 0 };
+    for (int d = 0; d < D; ++d) vab[d] = wb[d] - wa[d];
+    if (v_dot(D, vab, vab) < len * len) st_long_link(g, pa, pb, len, g->push);
+}
+
+/* checkPL (point_grid.cc:125-172): a sphere against a segment */
+static void st_check_pl(st_grid* g, const st_point* aa, const st_locus* bb)
+{
+    const int D = g->D;
+    const double* apos = st_pos(g, aa->pt);
+    const double len = aa->rad + bb->rad;
+    double a, d = 0.0;
+    st_project0(g, bb, apos, &a, &d);
+    if (0 <= a) {
+        if (a <= bb->cut) {
+            if (d < len * len) st_side_sliding_ie(g, bb->pt, a / bb->cut, aa->pt, len, g->push);
+        } else if (bb->last) {
+            st_check_pp(g, aa->pt, apos, aa->rad, bb->pt + 1, st_pos(g, bb->pt + 1), bb->rad);
+        }
+    } else {
+        if (bb->first) {
+            st_check_pp(g, aa->pt, apos, aa->rad, bb->pt, st_pos(g, bb->pt), bb->rad);
+        } else {
+            /* projection on the previous segment: interact with the node if it falls on its right */
+            const double* q1 = st_pos(g, bb->pt);
+            const double* q0 = st_pos(g, bb->pt - 1);
+            double vab[3] = { 0, 0, 0 }, dp[3] = { 0, 0, 0 };
+            for (int k = 0; k < D; ++k) { vab[k] = apos[k] - q1[k]; dp[k] = q1[k] - q0[k]; }
+            if (v_dot(D, vab, dp) >= 0) {
+                if (v_dot(D, vab, vab) < len * len) st_long_link(g, aa->pt, bb->pt, len, g->push);
+            }
+        }
+    }
+}
+
+/* checkLL1 (point_grid.cc:178-242): the first point of bb against segment aa */
+static void st_check_ll1(st_grid* g, const st_locus* aa, uint32_t ra, const st_locus* bb, uint32_t rb)
+{
+    const int D = g->D;
+    const double ran = aa->rng + bb->rad;
+    double a, d = INFINITY;
+    const double* b1 = st_pos(g, bb->pt);
+    st_project0(g, aa, b1, &a, &d);
+    if (d < ran * ran) {
+        const double len = aa->rad + bb->rad;
+        st_side_sliding_ie(g, aa->pt, a / aa->cut, bb->pt, len, d > len * len ? g->pull : g->push);
+    } else if (a < 0) {
+        const double* a1 = st_pos(g, aa->pt);
+        if (aa->first) {
+            if (rb < ra && bb->first) {
+                double vab[3] = { 0, 0, 0 }, db[3] = { 0, 0, 0 };
+                const double* b2 = st_pos(g, bb->pt + 1);
+                for (int k = 0; k < D; ++k) { vab[k] = b1[k] - a1[k]; db[k] = b2[k] - b1[k]; }
+                const double len = aa->rad + bb->rad;
+                if (v_dot(D, vab, vab) < len * len && v_dot(D, vab, db) >= 0)
+                    st_long_link(g, aa->pt, bb->pt, len, g->push);
+            }
+        } else {
+            double vab[3] = { 0, 0, 0 }, dp[3] = { 0, 0, 0 };
+            const double* a0 = st_pos(g, aa->pt - 1);
+            for (int k = 0; k < D; ++k) { vab[k] = b1[k] - a1[k]; dp[k] = a1[k] - a0[k]; }
+            if (v_dot(D, vab, dp) >= 0) {
+                const double dd = v_dot(D, vab, vab);
+                if (dd < ran * ran) {
+                    const double len = aa->rad + bb->rad;
+                    st_long_link(g, aa->pt, bb->pt, len, dd > len * len ? g->pull : g->push);
+                }
+            }
+        }
+    }
+}
+
+/* checkLL2 (point_grid.cc:248-315): the last point of bb against segment aa */
+static void st_check_ll2(st_grid* g, const st_locus* aa, uint32_t ra, const st_locus* bb, uint32_t rb)
+{
+    const int D = g->D;
+    const double ran = aa->rng + bb->rad;
+    double a, d = INFINITY;
+    const double* b2 = st_pos(g, bb->pt + 1);
+    st_project0(g, aa, b2, &a, &d);
+    if (d < ran * ran) {
+        const double len = aa->rad + bb->rad;
+        st_side_sliding_ie(g, aa->pt, a / aa->cut, bb->pt + 1, len, d > len * len ? g->pull : g->push);
+    } else if (a < 0) {
+        const double* a1 = st_pos(g, aa->pt);
+        double vab[3] = { 0, 0, 0 };
+        for (int k = 0; k < D; ++k) vab[k] = b2[k] - a1[k];
+        if (aa->first) {
+            double db[3] = { 0, 0, 0 };
+            const double* b1 = st_pos(g, bb->pt);
+            for (int k = 0; k < D; ++k) db[k] = b2[k] - b1[k];
+            const double len = aa->rad + bb->rad;
+            if (v_dot(D, vab, vab) < len * len && v_dot(D, vab, db) <= 0)
+                st_long_link(g, aa->pt, bb->pt + 1, len, g->push);
+        } else {
+            double dp[3] = { 0, 0, 0 };
+            const double* a0 = st_pos(g, aa->pt - 1);
+            for (int k = 0; k < D; ++k) dp[k] = a1[k] - a0[k];
+            if (v_dot(D, vab, dp) >= 0) {
+                const double dd = v_dot(D, vab, vab);
+                if (dd < ran * ran) {
+                    const double len = aa->rad + bb->rad;
+                    st_long_link(g, aa->pt, bb->pt + 1, len, dd > len * len ? g->pull : g->push);
+                }
+            }
+        }
+    } else if (rb < ra && aa->last && a > aa->cut) {
+        const double* a2 = st_pos(g, aa->pt + 1);
+        const double* b1 = st_pos(g, bb->pt);
+        double vab[3] = { 0, 0, 0 }, db[3] = { 0, 0, 0 };
+        for (int k = 0; k < D; ++k) { vab[k] = b2[k] - a2[k]; db[k] = b2[k] - b1[k]; }
+        const double len = aa->rad + bb->rad;
+        if (v_dot(D, vab, vab) < len * len && v_dot(D, vab, db) <= 0)
+            st_long_link(g, aa->pt + 1, bb->pt + 1, len, g->push);
+    }
+}
+
+/* FiberLocus::projectSegment (fiber_locus.cc:206-270) */
+static int st_project_segment(const st_grid* g, const st_locus* A, const st_locus* B, double* abs1, double* abs2,
+                              double* dist)
+{
+    const int D = g->D;
+    const double *a1 = st_pos(g, A->pt), *a2 = st_pos(g, A->pt + 1);
+    const double *b1 = st_pos(g, B->pt), *b2 = st_pos(g, B->pt + 1);
+    double d1[3] = { 0, 0, 0 }, d2[3] = { 0, 0, 0 }, d12[3] = { 0, 0, 0 };
+    for (int k = 0; k < D; ++k) { d1[k] = a2[k] - a1[k]; d2[k] = b2[k] - b1[k]; d12[k] = b1[k] - a1[k]; }
+    const double len1 = A->cut, len2 = B->cut;
+    const double beta = v_dot(D, d1, d2) / (len1 * len2);
+    const double scal = 1.0 - beta * beta;
+    if (scal > REAL_EPSILON) {
+        const double d1d12 = v_dot(D, d1, d12) / (scal * len1);
+        const double d2d12 = v_dot(D, d2, d12) / (scal * len2);
+        *abs1 = d1d12 - beta * d2d12;
+        if (*abs1 < 0 || len1 < *abs1) return 0;
+        *abs2 = beta * d1d12 - d2d12;
+        if (*abs2 < 0 || len2 < *abs2) return 0;
+        double v[3] = { 0, 0, 0 };
+        const double s = *abs2 / len2;
+        for (int k = 0; k < D; ++k) v[k] = d12[k] + d2[k] * s;
+        *dist = v_dot(D, v, v) - *abs1 * *abs1;
+        return 1;
+    }
+    double p1 = v_dot(D, d12, d1) / len1;
+    double p2 = p1 + beta * len2;
+    if (p1 < 0 && p2 < 0) return 0;
+    if (p1 > len1 && p2 > len1) return 0;
+    const double p = p1;
+    *dist = v_dot(D, d12, d12) - p1 * p1;
+    if (p1 < 0) p1 = 0; else if (p1 > len1) p1 = len1;
+    if (p2 < 0) p2 = 0; else if (p2 > len1) p2 = len1;
+    *abs1 = 0.5 * (p1 + p2);
+    *abs2 = (*abs1 - p) * beta;
+    return 1;
+}
+
+/* checkLL (point_grid.cc:321-354) */
+static void st_check_ll(st_grid* g, uint32_t ia, uint32_t ib)
+{
+    const st_locus *aa = &g->loc[ia], *bb = &g->loc[ib];
+    const uint32_t ra = g->loc_rank[ia], rb = g->loc_rank[ib];
+    st_check_ll1(g, aa, ra, bb, rb);
+    if (aa->last) st_check_ll2(g, bb, rb, aa, ra);
+    st_check_ll1(g, bb, rb, aa, ra);
+    if (bb->last) st_check_ll2(g, aa, ra, bb, rb);
+    if (g->D == 3) {
+        const double ran = fmax(aa->rng + bb->rad, aa->rad + bb->rng);
+        double a, b, d;
+        if (st_project_segment(g, aa, bb, &a, &b, &d) && d < ran * ran) {
+            const double len = aa->rad + bb->rad;
+            st_side_sliding_ii(g, aa->pt, a / aa->cut, bb->pt, b / bb->cut, len, d > len * len ? g->pull : g->push);
+        }
+    }
+}
+
+static void st_check_pp_items(st_grid* g, uint32_t i, uint32_t j)
+{
+    const st_point *a = &g->pnt[i], *b = &g->pnt[j];
+    st_check_pp(g, a->pt, st_pos(g, a->pt), a->rad, b->pt, st_pos(g, b->pt), b->rad);
+}
+
+/* Grid::createSideRegions(1) for one cell (grid.h:650-861): the shifts of
+   makeRectangularGrid in its order, kept when inside and at an index offset >= 0 */
+static int st_region(const st_grid* g, uint32_t indx, int* off)
+{
+    const int D = g->D;
+    int ori[3] = { 0, 0, 0 };
+    uint32_t r = indx;
+    for (int d = 0; d < D; ++d) { ori[d] = (int)(r % (uint32_t)g->gdim[d]); r /= (uint32_t)g->gdim[d]; }
+    int shifts[27][3];
+    int nb = 1;
+    for (int k = 0; k < 3; ++k) shifts[0][k] = 0;
+    for (int d = 0; d < D; ++d) {
+        int h = nb;
+        for (int s = -1; s <= 1; s += 2)
+            for (int n = 0; n < nb; ++n, ++h) {
+                for (int e = 0; e < 3; ++e) shifts[h][e] = shifts[n][e];
+                shifts[h][d] = s;
+            }
+        nb = h;
+    }
+    int cnt = 0;
+    for (int q = 0; q < nb; ++q) {
+        int cc[3], in = 1;
+        for (int d = 0; d < D; ++d) {
+            cc[d] = ori[d] + shifts[q][d];
+            if (cc[d] < 0 || cc[d] >= g->gdim[d]) in = 0;
+        }
+        if (!in) continue;
+        int idx = 0;
+        for (int d = D - 1; d >= 0; --d) idx = g->gdim[d] * idx + cc[d];
+        const int o = idx - (int)indx;
+        if (o >= 0) off[cnt++] = o;
+    }
+    return cnt;
+}
+
+/* PointGrid::setInteractions (point_grid.cc:361-413) */
+static void st_traverse(st_grid* g)
+{
+    for (uint32_t indx = 0; indx < g->ncells; ++indx) {
+        int reg[27];
+        const int nr = st_region(g, indx, reg);
+        const uint32_t p0 = g->pnt_start[indx], p1 = g->pnt_start[indx + 1];
+        const uint32_t l0 = g->loc_start[indx], l1 = g->loc_start[indx + 1];
+        for (uint32_t ii = p0; ii < p1; ++ii) {
+            for (uint32_t jj = ii + 1; jj < p1; ++jj) st_check_pp_items(g, g->pnt_list[ii], g->pnt_list[jj]);
+            for (uint32_t jj = l0; jj < l1; ++jj) st_check_pl(g, &g->pnt[g->pnt_list[ii]], &g->loc[g->loc_list[jj]]);
+        }
+        for (uint32_t ii = l0; ii < l1; ++ii)
+            for (uint32_t jj = ii + 1; jj < l1; ++jj) st_check_ll(g, g->loc_list[ii], g->loc_list[jj]);
+        for (int r = 1; r < nr; ++r) {
+            const uint32_t side = indx + (uint32_t)reg[r];
+            const uint32_t q0 = g->pnt_start[side], q1 = g->pnt_start[side + 1];
+            const uint32_t s0 = g->loc_start[side], s1 = g->loc_start[side + 1];
+            for (uint32_t ii = p0; ii < p1; ++ii) {
+                for (uint32_t jj = q0; jj < q1; ++jj) st_check_pp_items(g, g->pnt_list[ii], g->pnt_list[jj]);
+                for (uint32_t jj = s0; jj < s1; ++jj) st_check_pl(g, &g->pnt[g->pnt_list[ii]], &g->loc[g->loc_list[jj]]);
+            }
+            for (uint32_t ii = l0; ii < l1; ++ii) {
+                for (uint32_t jj = q0; jj < q1; ++jj) st_check_pl(g, &g->pnt[g->pnt_list[jj]], &g->loc[g->loc_list[ii]]);
+                for (uint32_t jj = s0; jj < s1; ++jj) st_check_ll(g, g->loc_list[ii], g->loc_list[jj]);
+            }
+        }
+    }
+}
+
+/* counting sort of items into cells, insertion order kept */
+static void st_bucket(uint32_t ncells, uint32_t n, const uint32_t* cell, uint32_t** start, uint32_t** list)
+{
+    *start = (uint32_t*)calloc((size_t)ncells + 1, sizeof(uint32_t));
+    *list = (uint32_t*)malloc((n ? n : 1) * sizeof(uint32_t));
+    for (uint32_t i = 0; i < n; ++i) (*start)[cell[i] + 1]++;
+    for (uint32_t c = 0; c < ncells; ++c) (*start)[c + 1] += (*start)[c];
+    uint32_t* fill = (uint32_t*)malloc(((size_t)ncells + 1) * sizeof(uint32_t));
+    memcpy(fill, *start, ((size_t)ncells + 1) * sizeof(uint32_t));
+    for (uint32_t i = 0; i < n; ++i) (*list)[fill[cell[i]]++] = i;
+    free(fill);
+}
+
+int orc_steric(orc_meca* m, const orc_steric_params* sp, const int* steric, const double* radius, const double* range,
+               int brute_force, orc_steric_call* log, uint32_t caplog, uint32_t* ncalls)
+{
+    const int D = m->dim;
+    st_grid g;
+    memset(&g, 0, sizeof(g));
+    g.D = D; g.m = m; g.push = sp->stiff_push; g.pull = sp->stiff_pull;
+    g.log = log; g.caplog = caplog;
+    /* Simul::estimateStericRange (simul_solve.cc:12-82) when no range is given,
+       with the fibers' actual segmentation for FiberProp::segmentation */
+    double step = sp->max_range;
+    if (step <= 0) {
+        double ran = 0, len = 0;
+        for (int f = 0; f < m->nfib; ++f) {
+            const orc_fiber* F = &m->fib[f];
+            if (F->kind != 0 || !steric[f]) continue;
+            if (len < 4.1 / 3 * F->cut) len = 4.1 / 3 * F->cut;
+            if (ran < radius[f] + range[f]) ran = radius[f] + range[f];
+        }
+        ran = len + 2 * ran;
+        for (int f = 0; f < m->nfib; ++f) {
+            const orc_fiber* F = &m->fib[f];
+            if (F->kind != 1 || !steric[f]) continue;
+            for (uint32_t p = 0; p < F->np; ++p)
+                if (ran < 2 * F->radius[p]) ran = 2 * F->radius[p];
+        }
+        step = ran;
+    }
+    if (!(step > REAL_EPSILON)) return -1;
+    /* PointGrid::setGrid (point_grid.cc:25-64) with Grid::setDimensions (grid.h:281-307) */
+    uint32_t ncells = 1;
+    double width[3] = { 0, 0, 0 };
+    for (int d = 0; d < D; ++d) {
+        double ext = sp->extension[d];
+        if (ext < 0) return -2;
+        const double n = 2 * ext / step;
+        g.gdim[d] = (int)ceil(n) + 2;
+        ext = g.gdim[d] * 0.5 * step;
+        g.inf[d] = -ext;
+        const double sup = ext;
+        width[d] = (sup - g.inf[d]) / (double)g.gdim[d];
+        g.delta[d] = (double)g.gdim[d] / (sup - g.inf[d]);
+        ncells *= (uint32_t)g.gdim[d];
+    }
+    g.ncells = ncells;
+    /* Grid::minimumWidth(1) (grid.h:354-364) */
+    double maxd = 0;
+    for (int d = 0; d < D; ++d) if (g.gdim[d] > 1) maxd = width[d];
+    for (int d = 0; d < D; ++d) if (g.gdim[d] > 1 && width[d] < maxd) maxd = width[d];
+    /* items (simul_solve.cc:153-184), PointGrid::add (point_grid.cc:71-96) */
+    uint32_t nl = 0, npn = 0;
+    for (int f = 0; f < m->nfib; ++f) {
+        const orc_fiber* F = &m->fib[f];
+        if (!steric[f]) continue;
+        if (F->kind == 0) nl += F->np - 1;
+        else for (uint32_t p = 0; p < F->np; ++p) npn += F->radius[p] > REAL_EPSILON;
+    }
+    g.loc = (st_locus*)malloc((nl ? nl : 1) * sizeof(st_locus));
+    g.pnt = (st_point*)malloc((npn ? npn : 1) * sizeof(st_point));
+    uint32_t* lcell = (uint32_t*)malloc((nl ? nl : 1) * sizeof(uint32_t));
+    uint32_t* pcell = (uint32_t*)malloc((npn ? npn : 1) * sizeof(uint32_t));
+    int err = 0;
+    for (int f = 0; f < m->nfib; ++f) {
+        const orc_fiber* F = &m->fib[f];
+        if (!steric[f]) continue;
+        if (F->kind == 0) {
+            for (uint32_t r = 0; r + 1 < F->np; ++r) {
+                st_locus* L = &g.loc[g.nloc];
+                L->pt = F->idx + r; L->mec = (uint32_t)f;
+                L->first = r == 0; L->last = r + 2 == F->np;
+                L->cut = F->cut; L->rad = radius[f]; L->rng = radius[f] + range[f];
+                const double *p1 = st_pos(&g, L->pt), *p2 = st_pos(&g, L->pt + 1);
+                double c[3] = { 0, 0, 0 };
+                for (int d = 0; d < D; ++d) c[d] = 0.5 * (p1[d] + p2[d]);
+                lcell[g.nloc] = st_cell(&g, c);
+                if (maxd < sqrt(L->cut * L->cut + L->rng * L->rng)) err = -3;
+                ++g.nloc;
+            }
+        } else {
+            for (uint32_t p = 0; p < F->np; ++p) {
+                if (!(F->radius[p] > REAL_EPSILON)) continue;
+                st_point* P = &g.pnt[g.npnt];
+                P->pt = F->idx + p; P->rad = F->radius[p];
+                pcell[g.npnt] = st_cell(&g, st_pos(&g, P->pt));
+                if (maxd < 2 * P->rad) err = -3;
+                ++g.npnt;
+            }
+        }
+    }
+    if (!err) {
+        st_bucket(ncells, g.nloc, lcell, &g.loc_start, &g.loc_list);
+        st_bucket(ncells, g.npnt, pcell, &g.pnt_start, &g.pnt_list);
+        g.loc_rank = (uint32_t*)malloc((g.nloc ? g.nloc : 1) * sizeof(uint32_t));
+        for (uint32_t k = 0; k < g.nloc; ++k) g.loc_rank[g.loc_list[k]] = k;
+        if (!brute_force) {
+            st_traverse(&g);
+        } else {
+            /* every pair once, in the cell-major order of the items: the set of
+               calls the grid must find when its cells are wide enough */
+            for (uint32_t i = 0; i < g.npnt; ++i)
+                for (uint32_t j = i + 1; j < g.npnt; ++j) st_check_pp_items(&g, g.pnt_list[i], g.pnt_list[j]);
+            for (uint32_t i = 0; i < g.npnt; ++i)
+                for (uint32_t j = 0; j < g.nloc; ++j) st_check_pl(&g, &g.pnt[g.pnt_list[i]], &g.loc[g.loc_list[j]]);
+            for (uint32_t i = 0; i < g.nloc; ++i)
+                for (uint32_t j = i + 1; j < g.nloc; ++j) st_check_ll(&g, g.loc_list[i], g.loc_list[j]);
+        }
+        free(g.loc_start); free(g.loc_list); free(g.pnt_start); free(g.pnt_list); free(g.loc_rank);
+    }
+    free(lcell); free(pcell); free(g.loc); free(g.pnt);
+    if (ncalls) *ncalls = g.nlog;
+    return err;
+}
+
 /* Fiber::setInteractions, CONFINE_INSIDE (fiber.cc:471-481), with
    SpaceSphere::inside (space_sphere.cc:113-116) and setInteraction (148-151) */
 void orc_confine_sphere(orc_meca* m, double radius, double stiffness)

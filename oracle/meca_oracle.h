@@ -112,6 +112,23 @@ void      orc_inter_side_clamp(orc_meca*, uint32_t a1, uint32_t a2, double ca, c
 void      orc_inter_sliding_clamp(orc_meca*, uint32_t a1, uint32_t a2, double ca, const double* g, double w);
 void      orc_inter_plane_i(orc_meca*, uint32_t a1, uint32_t a2, double ca, const double* dir, const double* g,
                             double w);
+/* steric interactions (simul_solve.cc:12-188, point_grid.cc); returns 0, -1 (no range),
+   -2 (negative extension) or -3 (steric_max_range too short for a segment or a point);
+   brute_force: every pair once instead of the grid's neighbour cells; the calls
+   made are logged (kind 0 interLongLink PE-PE, 1 interSideSlidingLink PI-PE, 2 PI-PI) */
+typedef struct {
+    double extension[3];
+    double max_range;
+    double stiff_push, stiff_pull;
+} orc_steric_params;
+typedef struct {
+    int      kind;
+    uint32_t p[4];
+    double   ca, cb, len, w;
+} orc_steric_call;
+int       orc_steric(orc_meca*, const orc_steric_params*, const int* steric, const double* radius,
+                     const double* range, int brute_force, orc_steric_call* log, uint32_t caplog,
+                     uint32_t* ncalls);
 /* Fiber::setInteractions CONFINE_INSIDE with SpaceSphere (fiber.cc:471-481; space_sphere.cc:148-151) */
 void      orc_confine_sphere(orc_meca*, double radius, double stiffness);
 

@@ -45,6 +45,16 @@ class Report(C.Structure):
 _lib = None
 
 
+class StericParams(C.Structure):
+    _fields_ = [("extension", C.c_double * 3), ("max_range", C.c_double), ("stiff_push", C.c_double),
+                ("stiff_pull", C.c_double)]
+
+
+class StericCall(C.Structure):
+    _fields_ = [("kind", C.c_int), ("p", C.c_uint32 * 4), ("ca", C.c_double), ("cb", C.c_double),
+                ("len", C.c_double), ("w", C.c_double)]
+
+
 def build():
     """Compile the oracle with its Makefile (gcc only)."""
     import subprocess
@@ -88,6 +98,8 @@ def lib():
         "orc_inter_long_clamp": (None, [vp, C.c_uint32, _f64p, C.c_double, C.c_double]),
         "orc_inter_plane_e": (None, [vp, C.c_uint32, _f64p, _f64p, C.c_double]),
         "orc_confine_sphere": (None, [vp, C.c_double, C.c_double]),
+        "orc_steric": (C.c_int, [vp, C.POINTER(StericParams), vp, _f64p, _f64p, C.c_int, vp, C.c_uint32,
+                                 C.POINTER(C.c_uint32)]),
         "orc_add_pure_force_i": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, _f64p]),
         "orc_inter_long_link_ie": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_double, C.c_double]),
         "orc_inter_long_link_ii": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double]),
@@ -321,6 +333,27 @@ class OracleMeca:
     def inter_plane_i(self, a1, a2, ca, direction, g, w):
         self.L.orc_inter_plane_i(self.h, a1, a2, ca, np.ascontiguousarray(direction, dtype=np.float64),
                                  np.ascontiguousarray(g, dtype=np.float64), w)
+
+    def steric(self, extension, max_range, push, pull, steric, radius, range_, brute_force=False, cap=1 << 20):
+        """Simul::setStericInteractions (simul_solve.cc:139-188) with PointGrid; returns
+        the list of Meca calls made: (kind, points, ca, cb, len, weight) with kind 0
+        interLongLink PE-PE, 1 interSideSlidingLink PI-PE, 2 PI-PI"""
+        sp = StericParams()
+        for d in range(3):
+            sp.extension[d] = extension[d] if d < len(extension) else 0.0
+        sp.max_range, sp.stiff_push, sp.stiff_pull = max_range, push, pull
+        st = np.ascontiguousarray(steric, dtype=np.int32)
+        rd = np.ascontiguousarray(radius, dtype=np.float64)
+        rg = np.ascontiguousarray(range_, dtype=np.float64)
+        log = (StericCall * cap)()
+        n = C.c_uint32()
+        rc = self.L.orc_steric(self.h, C.byref(sp), st.ctypes.data, rd, rg, int(brute_force), log, cap, C.byref(n))
+        if rc == -3:
+            raise ValueError("simul:steric_max_range is too short")
+        if rc:
+            raise ValueError(f"steric grid error {rc}")
+        assert n.value <= cap
+        return [(c.kind, tuple(c.p), c.ca, c.cb, c.len, c.w) for c in log[:n.value]]
 
     def confine_sphere(self, radius, stiffness):
         self.L.orc_confine_sphere(self.h, radius, stiffness)
