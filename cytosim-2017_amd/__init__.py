@@ -34,6 +34,7 @@ EXPORTED = [
     "cymg_inter_side_link_ie", "cymg_inter_side_link_ii", "cymg_inter_side_side_link", "cymg_inter_sliding_link_ie",
     "cymg_inter_sliding_link_ii", "cymg_inter_side_sliding_link_ie", "cymg_inter_side_sliding_link_ii",
     "cymg_inter_torque", "cymg_inter_side_clamp", "cymg_inter_sliding_clamp",
+    "cymg_set_steric", "cymg_steric_interactions",
     "cymg_confine_sphere", "cymg_set_couples", "cymg_couples_interactions", "cymg_set_couple_state",
     "cymg_get_couples", "cymg_step_couples", "cymg_nearby_segments", "cymg_nearby_segments_noalloc", "cymg_set_aster_links",
     "cymg_aster_interactions", "cymg_solve",
@@ -151,6 +152,8 @@ def lib():
         "cymg_add_pure_force_i": (C.c_int, [vp, u32, u32, C.c_double, dp]),
         "cymg_inter_plane_i": (C.c_int, [vp, u32, u32, C.c_double, dp, dp, C.c_double]),
         "cymg_dump": (C.c_int, [vp, C.c_char_p]),
+        "cymg_set_steric": (C.c_int, [vp, C.POINTER(StericParams), u32, vp, vp, vp]),
+        "cymg_steric_interactions": (C.c_int, [vp]),
         "cymg_inter_long_link_ie": (C.c_int, [vp, u32, u32, C.c_double, u32, C.c_double, C.c_double]),
         "cymg_inter_long_link_ii": (C.c_int, [vp, u32, u32, C.c_double, u32, u32, C.c_double, C.c_double, C.c_double]),
         "cymg_inter_side_link_ie": (C.c_int, [vp, u32, u32, C.c_double, u32, C.c_double, C.c_double]),
@@ -215,6 +218,11 @@ def lib():
 
 def _p(a):
     return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+class StericParams(C.Structure):
+    _fields_ = [("extension", C.c_double * 3), ("max_range", C.c_double), ("stiff_push", C.c_double),
+                ("stiff_pull", C.c_double)]
 
 
 def _vec3(v, dim):
@@ -433,6 +441,23 @@ class Meca:
         """Meca::interSlidingClamp (meca_inter.cc:2241-2283)"""
         gp, keep = _vec3(g, self.dim)
         self._chk(self.L.cymg_inter_sliding_clamp(self.h, a1, a2, ca, gp, w))
+
+    def set_steric(self, extension, max_range, push, pull, steric, radius, range_):
+        """steric parameters (SimulProp::steric_max_range, steric_stiffness_push/pull,
+        Space::extension) and, per Mecable in registration order, the steric flag
+        and the fibers' steric_radius / steric_range"""
+        sp = StericParams()
+        for d in range(3):
+            sp.extension[d] = extension[d] if d < len(extension) else 0.0
+        sp.max_range, sp.stiff_push, sp.stiff_pull = max_range, push, pull
+        st = np.ascontiguousarray(steric, dtype=np.int32)
+        rd = np.ascontiguousarray(radius, dtype=np.float64)
+        rg = np.ascontiguousarray(range_, dtype=np.float64)
+        self._chk(self.L.cymg_set_steric(self.h, C.byref(sp), st.size, _p(st), _p(rd), _p(rg)))
+
+    def steric_interactions(self):
+        """Simul::setStericInteractions (simul_solve.cc:139-188) on the device"""
+        self._chk(self.L.cymg_steric_interactions(self.h))
 
     def confine_sphere(self, radius, stiffness):
         self._chk(self.L.cymg_confine_sphere(self.h, radius, stiffness))

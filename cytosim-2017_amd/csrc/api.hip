@@ -488,6 +488,7 @@ int cymg_destroy(cymg_context* ctx)
     if (c.h_scal) (void)hipHostFree(c.h_scal);
     if (c.h_snap) (void)hipHostFree(c.h_snap);
     if (c.h_status) (void)hipHostFree(c.h_status);
+    if (c.h_st_tot) (void)hipHostFree(c.h_st_tot);
     if (c.mlp.h_gflag) (void)hipHostFree(c.mlp.h_gflag);
     for (int i = 0; i < 2; ++i)
         if (c.chunk_ev[i]) (void)hipEventDestroy(c.chunk_ev[i]);
@@ -1048,6 +1049,32 @@ int cymg_inter_sliding_clamp(cymg_context* ctx, uint32_t a1, uint32_t a2, double
         check_weight(w);
         if (!g) throw Error(CYMG_ERR_INVALID, "null position");
         defer(c, call_of(kDefSlidingClamp, { a1, a2 }, ca, 0.0, 0.0, w, g, c.dim));
+    });
+}
+
+int cymg_set_steric(cymg_context* ctx, const cymg_steric_params* params, uint32_t nmec, const int* steric,
+                    const double* radius, const double* range)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!params || (nmec && (!steric || !radius || !range))) throw Error(CYMG_ERR_INVALID, "null argument");
+        if (!(params->stiff_push >= 0) || !(params->stiff_pull >= 0))
+            throw Error(CYMG_ERR_INVALID, "steric stiffness must be >= 0");
+        c.st_prm = *params;
+        c.h_st_flag.assign(steric, steric + nmec);
+        c.h_st_rad.assign(radius, radius + nmec);
+        c.h_st_rng.assign(range, range + nmec);
+        c.st_set = true;
+        c.st_gen = ~0ull;
+    });
+}
+
+int cymg_steric_interactions(cymg_context* ctx)
+{
+    return guarded(ctx, [&](Context& c) {
+        need_prepared(c);
+        if (!c.st_set) throw Error(CYMG_ERR_INVALID, "call cymg_set_steric first");
+        if (c.nsys > 1) throw Error(CYMG_ERR_UNSUPPORTED, "steric interactions in a batch of independent systems");
+        steric_interactions(c);
     });
 }
 

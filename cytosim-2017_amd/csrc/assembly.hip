@@ -340,7 +340,7 @@ void build_streams(Context& c)
         if (mk.kind == 1) { total[1] += NC * c.fib_pts; total[2] += (size_t)D * c.fib_pts; }
         if (mk.kind == 2) { total[0] += (size_t)10 * c.ncouples; }
         if (mk.kind == 3) { total[0] += (size_t)6 * c.nasters; }
-        if (mk.kind == 4)
+        if (mk.kind == 4 || mk.kind == 5)
             for (int w = 0; w < 3; ++w) total[w] += mk.slots[w];
     }
     if (!c.h_def.empty()) {
@@ -381,7 +381,7 @@ void build_streams(Context& c)
         } else if (mk.kind == 3) {
             launch_emit_asters(c, c.s_key[0].get() + dpos[0], c.s_val[0].get() + dpos[0]);
             dpos[0] += (size_t)6 * c.nasters;
-        } else if (mk.kind == 4) {
+        } else if (mk.kind == 4 || mk.kind == 5) {
             launch_emit_deferred(c, mk, c.s_key[0].get() + dpos[0], c.s_val[0].get() + dpos[0],
                                  c.s_key[1].get() + dpos[1], c.s_val[1].get() + dpos[1], c.s_key[2].get() + dpos[2],
                                  c.s_val[2].get() + dpos[2]);
@@ -415,7 +415,7 @@ void build_streams(Context& c)
             sig.push_back((uint64_t)mk.kind);
             sig.push_back(w == 0 ? mk.posB : (w == 1 ? mk.posC : mk.posBas));
             if (mk.kind == 1 && w != 0) cacheable = false;
-            if (mk.kind == 4 && mk.slots[w]) cacheable = false;
+            if ((mk.kind == 4 || mk.kind == 5) && mk.slots[w]) cacheable = false;
             if (mk.kind == 2) { sig.push_back(c.couples_gen); sig.push_back(c.ncouples); }
             if (mk.kind == 3) { sig.push_back(c.asters_gen); sig.push_back(c.nasters); }
         }

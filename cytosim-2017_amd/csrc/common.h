@@ -101,7 +101,8 @@ struct HostStream {
 // device emitter placed in call order: the host-recorded entries before it are
 // those below its recorded positions in each host stream
 struct EmitterMark {
-    int kind;                     // 1 sphere confinement, 2 couples, 3 aster links, 4 deferred calls
+    int kind;                     // 1 sphere confinement, 2 couples, 3 aster links, 4 deferred calls,
+                                  // 5 steric grid calls (made on the device)
     double a, b;                  // emitter parameters (radius, stiffness)
     size_t posB, posC, posBas;    // host stream sizes when the emitter was called
     uint32_t first = 0, count = 0;        // deferred calls [first, first+count) of Context::h_def
@@ -212,6 +213,8 @@ void launch_emit_confine(Context& c, uint64_t* keyC, double* valC, uint64_t* key
 void launch_emit_couples(Context& c, uint64_t* keyB, double* valB);
 void launch_step_couples(Context& c, const MotorStep& ms);
 void launch_emit_asters(Context& c, uint64_t* keyB, double* valB);
+// steric.hip: the steric grid's calls as a device emitter mark
+void steric_interactions(Context& c);
 // launchers (emit.hip)
 void launch_emit_deferred(Context& c, const EmitterMark& mk, uint64_t* keyB, double* valB, uint64_t* keyC,
                           double* valC, uint64_t* keyBas, double* valBas);

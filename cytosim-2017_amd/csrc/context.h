@@ -76,6 +76,23 @@ struct Context {
     HostStream hB, hC, hBas;
     std::vector<EmitterMark> marks;
     std::vector<DeferredCall> h_def;   // position-dependent emitters, in call order
+    // steric interactions (simul_solve.cc:139-188): parameters, the items of the
+    // steric Mecables for the current layout, and the grid's work space
+    bool st_set = false;
+    cymg_steric_params st_prm{};
+    std::vector<int> h_st_flag;
+    std::vector<double> h_st_rad, h_st_rng;
+    uint64_t st_gen = ~0ull;           // layout generation the items were made for
+    int st_gdim[3] = { 1, 1, 1 };
+    double st_inf[3] = { 0, 0, 0 }, st_delta[3] = { 0, 0, 0 };
+    uint32_t st_ncells = 0, st_nloc = 0, st_npnt = 0;
+    DevBuf<uint32_t> st_lpt, st_lfib, st_lflag, st_ppt, st_pmec;
+    DevBuf<double> st_lcut, st_lrad, st_lrng, st_prad;
+    DevBuf<uint32_t> st_key, st_key2, st_val, st_val2, st_lstart, st_pstart, st_lrank;
+    DevBuf<uint32_t> st_cnt, st_cntC, st_cntS, st_ocnt, st_ocntC, st_ocntS;
+    DevBuf<unsigned char> st_temp;
+    DevBuf<DeferredCall> d_st_calls;
+    uint32_t* h_st_tot = nullptr;      // pinned: calls, C entries, vBAS entries
     DevBuf<DeferredCall> d_def;
     DevBuf<uint64_t> s_key[3];         // stream inputs (B, C, vBAS) in call order
     DevBuf<double> s_val[3];

@@ -776,7 +776,8 @@ void launch_emit_deferred(Context& c, const EmitterMark& mk, uint64_t* keyB, dou
                           double* valC, uint64_t* keyBas, double* valBas)
 {
     if (!mk.count) return;
-    const DeferredCall* calls = c.d_def.get() + mk.first;
+    // kind 4: calls recorded by the host; kind 5: made on the device by the steric grid
+    const DeferredCall* calls = (mk.kind == 5 ? c.d_st_calls.get() : c.d_def.get()) + mk.first;
     const dim3 grid((mk.count + 127) / 128);
     if (c.dim == 2)
         hipLaunchKernelGGL(k_emit_deferred<2>, grid, dim3(128), 0, c.stream, mk.count, calls, c.d_pos.get(), keyB,

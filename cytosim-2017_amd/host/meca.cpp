@@ -174,6 +174,19 @@ void Meca::interSlidingClamp(const PointInterpolated& a, const double g[], doubl
     check(cymg_inter_sliding_clamp(ctx_, a.mat1, a.mat2, a.coef, g, w));
 }
 
+void Meca::setSteric(const cymg_steric_params& p, const std::vector<int>& steric, const std::vector<double>& radius,
+                     const std::vector<double>& range)
+{
+    if (radius.size() != steric.size() || range.size() != steric.size())
+        throw Exception(CYMG_ERR_INVALID, "one steric flag, radius and range per Mecable");
+    check(cymg_set_steric(ctx_, &p, (uint32_t)steric.size(), steric.data(), radius.data(), range.data()));
+}
+
+void Meca::stericInteractions()
+{
+    check(cymg_steric_interactions(ctx_));
+}
+
 void Meca::confineSphere(double radius, double stiffness)
 {
     check(cymg_confine_sphere(ctx_, radius, stiffness));

@@ -119,6 +119,11 @@ public:
     void interSlidingClamp(const PointInterpolated& a, const double g[], double weight);
     // device-side emitters for the common cases
     void confineSphere(double radius, double stiffness);          // Fiber::setInteractions + SpaceSphere
+    // Simul::setStericInteractions (simul_solve.cc:139-188): per Mecable in registration
+    // order the steric flag, steric_radius and steric_range
+    void setSteric(const cymg_steric_params& p, const std::vector<int>& steric, const std::vector<double>& radius,
+                   const std::vector<double>& range);
+    void stericInteractions();
     void setCouples(const std::vector<uint32_t>& fa, const std::vector<double>& aa,
                     const std::vector<uint32_t>& fb, const std::vector<double>& ab,
                     const std::vector<double>& stiffness);

@@ -212,6 +212,29 @@ int         cymg_inter_side_clamp(cymg_context* ctx, uint32_t a1, uint32_t a2, d
 int         cymg_inter_sliding_clamp(cymg_context* ctx, uint32_t a1, uint32_t a2, double ca, const double* g,
                                      double weight);                                          /* meca_inter.cc:2241-2283 */
 
+/* steric interactions: Simul::setStericInteractions (simul_solve.cc:139-188) with
+   PointGrid::setInteractions (point_grid.cc:361-413), on the device, placed in the
+   call order like the calls above.  Items: the segments of the steric fibers
+   (FiberProp::steric_radius, steric_range) and the points of the steric Solids whose
+   radius exceeds REAL_EPSILON, in registration order; the grid covers
+   Space::extension() with cells of steric_max_range (0: estimated as
+   Simul::estimateStericRange does, with the fibers' segmentation).  The checks make
+   the reference's interLongLink / interSideSlidingLink calls, evaluated from the
+   device points.  Where the reference orders two segments by the address of their
+   records (point_grid.cc:205,295) the cell-major order of the items is used.  One
+   device-to-host copy per call (the number of interactions). */
+typedef struct {
+    double extension[3];   /* Space::extension() (the third ignored in 2D) */
+    double max_range;      /* SimulProp::steric_max_range; 0: estimated */
+    double stiff_push;     /* SimulProp::steric_stiffness_push[0] */
+    double stiff_pull;     /* SimulProp::steric_stiffness_pull[0] */
+} cymg_steric_params;
+/* per Mecable in registration order: steric flag (FiberProp/SolidProp::steric), and
+   for fibers steric_radius and steric_range (ignored for Solids) */
+int         cymg_set_steric(cymg_context* ctx, const cymg_steric_params* params, uint32_t nmec,
+                            const int* steric, const double* radius, const double* range);
+int         cymg_steric_interactions(cymg_context* ctx);
+
 /* device-side emitters, placed in the call order like the calls above.
    Fiber::setInteractions with CONFINE_INSIDE in a SpaceSphere centred at the
    origin, for every registered fiber (fiber.cc:464-481, space_sphere.cc:113-116,148-151) */

@@ -1,0 +1,124 @@
+"""Steric interactions on the device (Simul::setStericInteractions with
+PointGrid, simul_solve.cc:139-188, point_grid.cc) against the oracle's grid:
+mB, mC and vBAS bit-exact on random scenes of fibers and Solids in 2D and 3D,
+the steric calls placed between host-recorded interactions, over two steps;
+then positions after a solve to 1e-10 relative within 1e-9."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from test_gpu_emitters import _assert_same
+
+pytestmark = pytest.mark.gpu
+
+PARAMS = dict(time_step=0.005, kT=0.0042, viscosity=0.02)
+
+
+def _walk(rng, dim, n, cut, box):
+    start = rng.uniform(-box, box, dim)
+    d = rng.standard_normal(dim)
+    d /= np.linalg.norm(d)
+    pts = [start]
+    for _ in range(n - 1):
+        d = d + 0.3 * rng.standard_normal(dim)
+        d /= np.linalg.norm(d)
+        pts.append(pts[-1] + cut * d)
+    return np.array(pts)
+
+
+def _scene(dim, seed, nfib=60, nsol=6, box=1.6, cut=0.3):
+    rng = np.random.default_rng(seed)
+    fibers = []
+    for f in range(nfib):
+        n = int(rng.integers(3, 9))
+        fibers.append(_walk(rng, dim, n, cut, box))
+    solids = []
+    for k in range(nsol):
+        n = int(rng.integers(3, 7))
+        rad = rng.uniform(0.03, 0.15, n)
+        rad[rng.uniform(size=n) < 0.2] = 0.0
+        rad[0] = max(rad[0], 0.05)
+        p = rng.uniform(-box, box, dim) + 0.2 * rng.standard_normal((n, dim))
+        solids.append((rad, p))
+    return fibers, solids, cut
+
+
+def _load(m, fibers, solids, cut):
+    for f in fibers:
+        m.add_fiber(len(f), cut, 20.0)
+    for rad, p in solids:
+        m.add_solid(rad, p.ravel())
+    return np.concatenate([f.ravel() for f in fibers] + [p.ravel() for _, p in solids])
+
+
+def _steric_args(nfib, nsol, dim):
+    n = nfib + nsol
+    st = np.ones(n, np.int32)
+    st[1::7] = 0                        # some fibers are not steric
+    rad = np.full(n, 0.05)
+    rng = np.full(n, 0.05)
+    return [2.5] * dim, 0.0, 300.0, 100.0, st, rad, rng
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_steric_bit_exact(pkg, oracle, dim, seed):
+    fibers, solids, cut = _scene(dim, seed)
+    gpu = pkg.Meca(dim, 0)
+    orc = oracle.OracleMeca(dim, PARAMS)
+    x = _load(gpu, fibers, solids, cut)
+    _load(orc, fibers, solids, cut)
+    gpu.set_params(**PARAMS)
+    args = _steric_args(len(fibers), len(solids), dim)
+    gpu.set_steric(*args)
+    print(f"steric dim={dim} seed={seed} fibers={len(fibers)} solids={len(solids)}", flush=True)
+    for step in range(2):
+        gpu.prepare(x)
+        orc.prepare(x)
+        gpu.inter_link_ee(0, len(fibers[0]) + 1, 10.0)
+        orc.inter_link_ee(0, len(fibers[0]) + 1, 10.0)
+        gpu.steric_interactions()
+        calls = orc.steric(*args)
+        gpu.inter_clamp_e(2, np.ones(dim), 5.0)
+        orc.inter_clamp_e(2, np.ones(dim), 5.0)
+        assert len(calls) > 20
+        g0 = np.random.default_rng(step).standard_normal(x.size)
+        gpu.setup(g0)
+        orc.setup(g0)
+        _assert_same(pkg, oracle, gpu, orc)
+        x = x + 0.02 * np.random.default_rng(10 + step).standard_normal(x.size)
+    gpu.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_steric_step_positions(pkg, oracle, dim):
+    fibers, solids, cut = _scene(dim, 5, nfib=40, nsol=4)
+    gpu = pkg.Meca(dim, 0)
+    orc = oracle.OracleMeca(dim, PARAMS)
+    x = _load(gpu, fibers, solids, cut)
+    _load(orc, fibers, solids, cut)
+    gpu.set_params(**PARAMS)
+    args = _steric_args(len(fibers), len(solids), dim)
+    gpu.set_steric(*args)
+    gpu.prepare(x)
+    orc.prepare(x)
+    gpu.steric_interactions()
+    orc.steric(*args)
+    g0 = np.random.default_rng(3).standard_normal(x.size)
+    gpu.solve(1e-10, g0, tol_mode=pkg.TOL_RELATIVE)
+    orc.solve(1e-10, g0, tol_mode=1)
+    assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-9
+    gpu.close()
+
+
+def test_steric_range_too_short_refused(pkg):
+    fibers, solids, cut = _scene(3, 4, nfib=10, nsol=2)
+    gpu = pkg.Meca(3, 0)
+    x = _load(gpu, fibers, solids, cut)
+    gpu.set_params(**PARAMS)
+    ext, _, push, pull, st, rad, rng = _steric_args(len(fibers), len(solids), 3)
+    gpu.set_steric(ext, 0.2, push, pull, st, rad, rng)
+    gpu.prepare(x)
+    with pytest.raises(pkg.CymgError):
+        gpu.steric_interactions()
+    gpu.close()
