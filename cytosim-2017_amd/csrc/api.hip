@@ -397,6 +397,26 @@ void do_iterate(Context& c, const cymg_solve_params* p, cymg_solve_report* rep)
         if (learned && mlp_guess_dropped(c)) guess_used = CYMG_GUESS_ZERO;
         if (r.converged && c.mlp.recording) mlp_record(c);
         int total = r.iterations;
+        if (!r.converged && two_launch(c)) {
+            // The two-launch iteration carries M^-1 r, M^-1 v and M^-1 t by
+            // recurrences, whose rounding can hold the residual just above a
+            // tolerance near the rounding floor; the same BCGSP with M^-1 applied
+            // directly (four launches) continues from the iterate.
+            CYMG_HIP(hipMemcpyAsync(c.d_tmp.get(), c.d_sol.get(), (size_t)c.n() * 8, hipMemcpyDeviceToDevice,
+                                    c.stream));
+            c.krylov_fused = false;
+            cymg_solve_report r1{};
+            try {
+                krylov_solve(c, pc, c.d_tmp.get(), maxit, p->tolerance, p->tol_mode, &r1);
+            } catch (...) {
+                c.krylov_fused = true;
+                throw;
+            }
+            c.krylov_fused = true;
+            total += r1.iterations;
+            r1.iterations += r.iterations;
+            r = r1;
+        }
         if (!r.converged && p->fallback) {
             // meca.cc:749-799: retry with vRHS as the seed, then switch method
             cymg_solve_report r2{};

@@ -242,6 +242,7 @@ double krylov_time_launch(Context& c, int precond, int launches);
 int tile_threads_of(const Context& c);
 int tile_max_points(const Context& c, int threads);
 bool tiles_fit(const Context& c);
+bool two_launch(const Context& c);
 void nearby_segments(Context& c, uint32_t nq, const double* places, const uint32_t* exclude, double range,
                      int inclusive, uint32_t* counts, uint32_t capacity, uint32_t* fiber, uint32_t* point,
                      double* abscissa, double* dist_sqr, uint32_t* total);

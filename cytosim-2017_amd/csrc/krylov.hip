@@ -2119,6 +2119,12 @@ bool tiles_fit(const Context& c)
     return c.largest <= (uint32_t)TpCfg<512>::max_points(c.dim);
 }
 
+// the iteration krylov_solve runs: two launches per iteration over tiles, or four
+bool two_launch(const Context& c)
+{
+    return c.krylov_fused && (!c.nsol || c.solid_tiles) && tiles_fit(c);
+}
+
 int tile_max_points(const Context& c, int threads)
 {
     return threads == 256 ? TpCfg<256>::max_points(c.dim) : TpCfg<512>::max_points(c.dim);
@@ -2152,7 +2158,7 @@ void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter
 {
     // tiles hold whole fibers or one Solid of at most kSolidTileRows unknowns;
     // a larger Solid or a longer fiber runs the four-launch iteration
-    if (c.krylov_fused && (!c.nsol || c.solid_tiles) && tiles_fit(c)) {
+    if (two_launch(c)) {
         tiled_buffers(c);
         const bool small = tile_threads_of(c) == 256;
         auto go = [&](auto dim, auto nt) {

@@ -5,9 +5,9 @@
 // steric Solids whose radius exceeds REAL_EPSILON, each filed in the grid cell
 // that holds its centre (Grid::index with the bounded image, grid.h:147-150,
 // 445-455), in registration order within a cell (a stable radix sort by cell).
-// One thread per cell then walks the reference's traversal
-// (PointGrid::setInteractions, point_grid.cc:361-413): the pairs inside the
-// cell, then the cells of its side region (Grid::createSideRegions(1): the
+// One thread per (cell, side-region slot) then walks the reference's
+// traversal (PointGrid::setInteractions, point_grid.cc:361-413): the pairs
+// inside the cell, or those with one cell of its side region (Grid::createSideRegions(1): the
 // shifts of makeRectangularGrid in order, kept when inside and at an index
 // offset >= 0), running checkPP / checkPL / checkLL with the reference's
 // arithmetic.  A first walk counts the calls each cell makes (and their stream
@@ -359,29 +359,38 @@ __device__ int side_region(const StArgs& a, uint32_t indx, int* off)
     return cnt;
 }
 
+// one thread per (cell, region slot): slot 0 the pairs inside the cell, slot
+// r >= 1 the pairs with the r-th cell of its side region; the slots of a cell
+// follow each other, so the cell-major scan keeps the reference's order
+template <int D>
+constexpr int kSlots = D == 3 ? 14 : 5;   // (3^D + 1) / 2 cells in a side region
+
 template <int D, bool EMIT>
 __global__ __launch_bounds__(128) void k_st_walk(StArgs a)
 {
-    const uint32_t indx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (indx >= a.ncells) return;
-    Sink<D, EMIT> out(a, indx);
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= a.ncells * (uint32_t)kSlots<D>) return;
+    const uint32_t indx = u / kSlots<D>;
+    const int r = (int)(u % kSlots<D>);
+    Sink<D, EMIT> out(a, u);
     const Walk<D> W{ a };
     int reg[27];
     const int nr = side_region<D>(a, indx, reg);
     const uint32_t p0 = a.pstart[indx], p1 = a.pstart[indx + 1];
     const uint32_t l0 = a.lstart[indx], l1 = a.lstart[indx + 1];
-    if (p0 != p1 || l0 != l1) {
-        for (uint32_t ii = p0; ii < p1; ++ii) {
-            const uint32_t i = a.plist[ii];
-            for (uint32_t jj = ii + 1; jj < p1; ++jj) {
-                const uint32_t j = a.plist[jj];
-                W.pp(out, a.ppt[i], a.pmec[i], a.prad[i], a.ppt[j], a.pmec[j], a.prad[j]);
+    if ((p0 != p1 || l0 != l1) && r < nr) {
+        if (r == 0) {
+            for (uint32_t ii = p0; ii < p1; ++ii) {
+                const uint32_t i = a.plist[ii];
+                for (uint32_t jj = ii + 1; jj < p1; ++jj) {
+                    const uint32_t j = a.plist[jj];
+                    W.pp(out, a.ppt[i], a.pmec[i], a.prad[i], a.ppt[j], a.pmec[j], a.prad[j]);
+                }
+                for (uint32_t jj = l0; jj < l1; ++jj) W.pl(out, i, a.llist[jj]);
             }
-            for (uint32_t jj = l0; jj < l1; ++jj) W.pl(out, i, a.llist[jj]);
-        }
-        for (uint32_t ii = l0; ii < l1; ++ii)
-            for (uint32_t jj = ii + 1; jj < l1; ++jj) W.ll(out, a.llist[ii], a.llist[jj]);
-        for (int r = 1; r < nr; ++r) {
+            for (uint32_t ii = l0; ii < l1; ++ii)
+                for (uint32_t jj = ii + 1; jj < l1; ++jj) W.ll(out, a.llist[ii], a.llist[jj]);
+        } else {
             const uint32_t side = indx + (uint32_t)reg[r];
             const uint32_t q0 = a.pstart[side], q1 = a.pstart[side + 1];
             const uint32_t s0 = a.lstart[side], s1 = a.lstart[side + 1];
@@ -400,9 +409,9 @@ __global__ __launch_bounds__(128) void k_st_walk(StArgs a)
         }
     }
     if (!EMIT) {
-        a.cnt[indx] = out.n;
-        a.cntC[indx] = out.nc;
-        a.cntS[indx] = out.ns;
+        a.cnt[u] = out.n;
+        a.cntC[u] = out.nc;
+        a.cntS[u] = out.ns;
     }
 }
 
@@ -505,7 +514,7 @@ static void steric_items(Context& c)
         c.st_delta[d] = (double)c.st_gdim[d] / (ext - c.st_inf[d]);
         ncells *= (uint64_t)c.st_gdim[d];
     }
-    if (ncells > (1ull << 28)) throw Error(CYMG_ERR_UNSUPPORTED, "steric grid of more than 2^28 cells");
+    if (ncells > (1ull << 23)) throw Error(CYMG_ERR_UNSUPPORTED, "steric grid of more than 2^23 cells");
     c.st_ncells = (uint32_t)ncells;
     double maxd = 0;   // Grid::minimumWidth(1), grid.h:354-364
     for (int d = 0; d < D; ++d) if (c.st_gdim[d] > 1) maxd = width[d];
@@ -586,9 +595,10 @@ void steric_interactions(Context& c)
     size_t tb = 0, tb2 = 0, tb3 = 0;
     if (nl) CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, lkey, lkey2, lval, lval2, (int)nl, 0, end_bit, st));
     if (np) CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, pkey, pkey2, pval, pval2, (int)np, 0, end_bit, st));
-    c.st_cnt.resize((size_t)nc + 1); c.st_cntC.resize((size_t)nc + 1); c.st_cntS.resize((size_t)nc + 1);
-    c.st_ocnt.resize((size_t)nc + 1); c.st_ocntC.resize((size_t)nc + 1); c.st_ocntS.resize((size_t)nc + 1);
-    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, c.st_cnt.get(), c.st_ocnt.get(), (int)(nc + 1), st));
+    const uint32_t nw = nc * (uint32_t)(D == 3 ? kSlots<3> : kSlots<2>);   // walk threads
+    c.st_cnt.resize((size_t)nw + 1); c.st_cntC.resize((size_t)nw + 1); c.st_cntS.resize((size_t)nw + 1);
+    c.st_ocnt.resize((size_t)nw + 1); c.st_ocntC.resize((size_t)nw + 1); c.st_ocntS.resize((size_t)nw + 1);
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, c.st_cnt.get(), c.st_ocnt.get(), (int)(nw + 1), st));
     c.st_temp.resize(std::max({ tb, tb2, tb3, (size_t)1 }));
     if (nl) CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(c.st_temp.get(), tb, lkey, lkey2, lval, lval2, (int)nl, 0, end_bit, st));
     if (np) CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(c.st_temp.get(), tb2, pkey, pkey2, pval, pval2, (int)np, 0, end_bit, st));
@@ -602,19 +612,19 @@ void steric_interactions(Context& c)
     a.lrank = c.st_lrank.get();
     a.cnt = c.st_cnt.get(); a.cntC = c.st_cntC.get(); a.cntS = c.st_cntS.get();
     // the last entry stays 0, so the scans end with the totals
-    CYMG_HIP(hipMemsetAsync(a.cnt + nc, 0, 4, st));
-    CYMG_HIP(hipMemsetAsync(a.cntC + nc, 0, 4, st));
-    CYMG_HIP(hipMemsetAsync(a.cntS + nc, 0, 4, st));
-    if (D == 3) hipLaunchKernelGGL((k_st_walk<3, false>), blocks_for(nc, 128), dim3(128), 0, st, a);
-    else hipLaunchKernelGGL((k_st_walk<2, false>), blocks_for(nc, 128), dim3(128), 0, st, a);
+    CYMG_HIP(hipMemsetAsync(a.cnt + nw, 0, 4, st));
+    CYMG_HIP(hipMemsetAsync(a.cntC + nw, 0, 4, st));
+    CYMG_HIP(hipMemsetAsync(a.cntS + nw, 0, 4, st));
+    if (D == 3) hipLaunchKernelGGL((k_st_walk<3, false>), blocks_for(nw, 128), dim3(128), 0, st, a);
+    else hipLaunchKernelGGL((k_st_walk<2, false>), blocks_for(nw, 128), dim3(128), 0, st, a);
     CYMG_HIP(hipGetLastError());
-    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.st_temp.get(), tb3, a.cnt, c.st_ocnt.get(), (int)(nc + 1), st));
-    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.st_temp.get(), tb3, a.cntC, c.st_ocntC.get(), (int)(nc + 1), st));
-    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.st_temp.get(), tb3, a.cntS, c.st_ocntS.get(), (int)(nc + 1), st));
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.st_temp.get(), tb3, a.cnt, c.st_ocnt.get(), (int)(nw + 1), st));
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.st_temp.get(), tb3, a.cntC, c.st_ocntC.get(), (int)(nw + 1), st));
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.st_temp.get(), tb3, a.cntS, c.st_ocntS.get(), (int)(nw + 1), st));
     if (!c.h_st_tot) CYMG_HIP(hipHostMalloc((void**)&c.h_st_tot, 4 * sizeof(uint32_t), hipHostMallocDefault));
-    CYMG_HIP(hipMemcpyAsync(c.h_st_tot, c.st_ocnt.get() + nc, 4, hipMemcpyDeviceToHost, st));
-    CYMG_HIP(hipMemcpyAsync(c.h_st_tot + 1, c.st_ocntC.get() + nc, 4, hipMemcpyDeviceToHost, st));
-    CYMG_HIP(hipMemcpyAsync(c.h_st_tot + 2, c.st_ocntS.get() + nc, 4, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemcpyAsync(c.h_st_tot, c.st_ocnt.get() + nw, 4, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemcpyAsync(c.h_st_tot + 1, c.st_ocntC.get() + nw, 4, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemcpyAsync(c.h_st_tot + 2, c.st_ocntS.get() + nw, 4, hipMemcpyDeviceToHost, st));
     CYMG_HIP(hipStreamSynchronize(st));
     const uint32_t ncalls = c.h_st_tot[0];
     // calls of earlier steric marks of this assembly stay in front
@@ -634,8 +644,8 @@ void steric_interactions(Context& c)
                                     hipMemcpyDeviceToDevice, st));
         a.ocnt = c.st_ocnt.get(); a.ocntC = c.st_ocntC.get(); a.ocntS = c.st_ocntS.get();
         a.calls = c.d_st_calls.get() + first;
-        if (D == 3) hipLaunchKernelGGL((k_st_walk<3, true>), blocks_for(nc, 128), dim3(128), 0, st, a);
-        else hipLaunchKernelGGL((k_st_walk<2, true>), blocks_for(nc, 128), dim3(128), 0, st, a);
+        if (D == 3) hipLaunchKernelGGL((k_st_walk<3, true>), blocks_for(nw, 128), dim3(128), 0, st, a);
+        else hipLaunchKernelGGL((k_st_walk<2, true>), blocks_for(nw, 128), dim3(128), 0, st, a);
         CYMG_HIP(hipGetLastError());
         if (first) CYMG_HIP(hipStreamSynchronize(st));   // `keep` is freed on return
     }
