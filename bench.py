@@ -120,7 +120,7 @@ def parse():
                     help="with --autotune 0: threads per tile of the iteration launches (256 or 512; 0: default)")
     ap.add_argument("--tile-points", type=int, default=-1,
                     help="with --autotune 0: target points per tile (-1: default)")
-    ap.add_argument("--only", default="all", choices=["all", "c2", "batched", "asters", "grid"],
+    ap.add_argument("--only", default="all", choices=["all", "c2", "batched", "asters", "grid", "side"],
                     help="run one workload alone (for profiler runs); the JSON line is printed with 'all' or 'c2'")
     return ap.parse_args()
 
@@ -494,6 +494,44 @@ def grid_query(args, pkg, scenes, sc=None):
     return res
 
 
+def steric_and_frames(args, pkg, scenes, sc=None):
+    # the steric grid (Simul::setStericInteractions) and trajectory frames
+    # (Simul::writeObjects) on the 1000-fiber system; host timings of one call
+    # each, the frame's host bytes included
+    if sc is None:
+        sc = scenes.config2(seed=2, nfib=args.nfib, nlinks=args.nlinks)
+    m = pkg.Meca(3, 0)
+    pkg.load_scene(m, sc)
+    n = sc.nfib
+    m.set_steric([10.0] * 3, 0.0, 100.0, 0.0, np.ones(n, np.int32), np.full(n, 0.025), np.full(n, 0.025))
+    st = []
+    for rep in range(4):
+        m.prepare(None)
+        pkg.assemble(m, sc)
+        t0 = time.perf_counter()
+        m.steric_interactions()
+        st.append(time.perf_counter() - t0)
+        m.setup(sc.gauss(0))
+    m.set_frame_records([(k + 1, 0, 0, 0, 0.5, 0.0) for k in range(n)])
+    fr = {}
+    for name, rb in (("float", pkg.FRAME_FLOAT), ("double", pkg.FRAME_DOUBLE)):
+        data = m.write_frame(0.0, "bench", rb)       # untimed: layout
+        reps = 10
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            data = m.write_frame(0.0, "bench", rb)
+        dt = (time.perf_counter() - t0) / reps
+        fr[name] = {"ms_per_frame": round(1e3 * dt, 3), "bytes": len(data)}
+    m.close()
+    res = {"steric": {"workload": "every fiber of the 1000-fiber system steric, radius 0.025, range 0.025",
+                      "ms_per_call": round(1e3 * float(np.median(st[1:])), 3),
+                      "note": "cymg_steric_interactions: items, grid sort, count walk, scans, emit walk"},
+           "trajectory_frame": dict(fr, workload="the 1000-fiber system, fiber section",
+                                    note="the Python write_frame: a size query, then cymg_write_frame into host memory")}
+    log(f"steric/frames {res}")
+    return res
+
+
 def cpu_baseline(args, sc, gpu_iters):
     from oracle import pyoracle
     log(f"cpu baseline, {args.cpu_steps} oracle steps")
@@ -537,6 +575,9 @@ def main():
     if args.only == "grid":
         grid_query(args, pkg, scenes)
         return
+    if args.only == "side":
+        steric_and_frames(args, pkg, scenes)
+        return
 
     sc, results, training, kern, (it_ms, it_bytes) = c2(args, pkg, scenes, torch)
     achieved = it_bytes / (it_ms * 1e-3) / 1e9
@@ -544,6 +585,7 @@ def main():
     bat = batched(args, pkg, scenes, torch) if full and args.batched_steps > 0 else None
     ast = asters_workload(args, pkg, scenes, torch) if full and args.aster_steps > 0 else None
     grid = grid_query(args, pkg, scenes, sc) if full and args.grid_places > 0 else None
+    side = steric_and_frames(args, pkg, scenes, sc) if full else None
     cpu = same = None
     if args.cpu_steps > 0:
         cpu, same = cpu_baseline(args, sc, results["fp64"]["iters"])
@@ -602,6 +644,8 @@ def main():
         "batched": bat,
         "asters": ast,
         "nearby_segments": grid,
+        "steric": side and side["steric"],
+        "trajectory_frame": side and side["trajectory_frame"],
     }
     print(json.dumps(line), flush=True)
 

@@ -18,7 +18,8 @@ ROOT = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "_build", "libcymeca.so")
 
 OK, ERR_INVALID, ERR_HIP, ERR_PROJECTION, ERR_CONVERGENCE, ERR_UNSUPPORTED = range(6)
-MAX_FIBER_POINTS = 128
+MAX_FIBER_POINTS = 512
+FRAME_FLOAT, FRAME_DOUBLE = 4, 8
 TOL_REFERENCE, TOL_RELATIVE = 0, 1
 PRECOND_NONE, PRECOND_FP64, PRECOND_FP32 = 0, 1, 2
 GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED = 0, 1, 2
@@ -34,7 +35,7 @@ EXPORTED = [
     "cymg_inter_side_link_ie", "cymg_inter_side_link_ii", "cymg_inter_side_side_link", "cymg_inter_sliding_link_ie",
     "cymg_inter_sliding_link_ii", "cymg_inter_side_sliding_link_ie", "cymg_inter_side_sliding_link_ii",
     "cymg_inter_torque", "cymg_inter_side_clamp", "cymg_inter_sliding_clamp",
-    "cymg_set_steric", "cymg_steric_interactions",
+    "cymg_set_steric", "cymg_steric_interactions", "cymg_set_frame_records", "cymg_write_frame", "cymg_read_frame",
     "cymg_confine_sphere", "cymg_set_couples", "cymg_couples_interactions", "cymg_set_couple_state",
     "cymg_get_couples", "cymg_step_couples", "cymg_nearby_segments", "cymg_nearby_segments_noalloc", "cymg_set_aster_links",
     "cymg_aster_interactions", "cymg_solve",
@@ -153,6 +154,10 @@ def lib():
         "cymg_inter_plane_i": (C.c_int, [vp, u32, u32, C.c_double, dp, dp, C.c_double]),
         "cymg_dump": (C.c_int, [vp, C.c_char_p]),
         "cymg_set_steric": (C.c_int, [vp, C.POINTER(StericParams), u32, vp, vp, vp]),
+        "cymg_set_frame_records": (C.c_int, [vp, u32, vp]),
+        "cymg_write_frame": (C.c_int, [vp, C.POINTER(FrameParams), vp, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "cymg_read_frame": (C.c_int, [vp, C.c_size_t, C.c_int, vp, u32, C.POINTER(C.c_uint32), vp, vp, C.c_size_t,
+                                      C.POINTER(C.c_size_t), C.POINTER(C.c_double), C.POINTER(C.c_size_t)]),
         "cymg_steric_interactions": (C.c_int, [vp]),
         "cymg_inter_long_link_ie": (C.c_int, [vp, u32, u32, C.c_double, u32, C.c_double, C.c_double]),
         "cymg_inter_long_link_ii": (C.c_int, [vp, u32, u32, C.c_double, u32, u32, C.c_double, C.c_double, C.c_double]),
@@ -223,6 +228,47 @@ def _p(a):
 class StericParams(C.Structure):
     _fields_ = [("extension", C.c_double * 3), ("max_range", C.c_double), ("stiff_push", C.c_double),
                 ("stiff_pull", C.c_double)]
+
+
+class FrameRecord(C.Structure):
+    _fields_ = [("number", C.c_uint32), ("mark", C.c_int32), ("prop", C.c_uint32), ("signature", C.c_uint32),
+                ("cut_wished", C.c_double), ("abscissa", C.c_double)]
+
+
+class FrameParams(C.Structure):
+    _fields_ = [("time", C.c_double), ("date", C.c_char_p), ("real_bytes", C.c_int), ("before", C.c_void_p),
+                ("nbefore", C.c_size_t), ("after", C.c_void_p), ("nafter", C.c_size_t)]
+
+
+class FrameObject(C.Structure):
+    _fields_ = [("tag", C.c_char), ("prop", C.c_uint32), ("number", C.c_uint32), ("mark", C.c_int32),
+                ("signature", C.c_uint32), ("length", C.c_double), ("cut_wished", C.c_double),
+                ("abscissa", C.c_double), ("points", C.c_uint32), ("first", C.c_uint64)]
+
+
+def read_frame(data, dim, max_objects=1 << 16, max_points=1 << 22):
+    """cymg_read_frame: the first frame of `data` (bytes) -> dict with time,
+    used (bytes up to the frame's end), objects (list of dicts) and the points
+    [npoints, dim] and radii of the fiber and solid sections"""
+    L = lib()
+    buf = C.create_string_buffer(bytes(data), len(data))
+    objs = (FrameObject * max_objects)()
+    pts = np.zeros((max_points, dim))
+    rad = np.zeros(max_points)
+    no, npt, used, t = C.c_uint32(), C.c_size_t(), C.c_size_t(), C.c_double()
+    rc = L.cymg_read_frame(buf, len(data), dim, objs, max_objects, C.byref(no), _p(pts), _p(rad), max_points,
+                           C.byref(npt), C.byref(t), C.byref(used))
+    if rc:
+        raise CymgError(rc, L.cymg_last_error(None).decode())
+    if no.value > max_objects or npt.value > max_points:
+        raise CymgError(ERR_INVALID, "frame larger than the buffers")
+    out = []
+    for o in objs[:no.value]:
+        out.append(dict(tag=o.tag.decode(), prop=o.prop, number=o.number, mark=o.mark, signature=o.signature,
+                        length=o.length, cut_wished=o.cut_wished, abscissa=o.abscissa, points=o.points,
+                        first=o.first))
+    return dict(time=t.value, used=used.value, objects=out, points=pts[:npt.value].copy(),
+                radii=rad[:npt.value].copy())
 
 
 def _vec3(v, dim):
@@ -454,6 +500,27 @@ class Meca:
         rd = np.ascontiguousarray(radius, dtype=np.float64)
         rg = np.ascontiguousarray(range_, dtype=np.float64)
         self._chk(self.L.cymg_set_steric(self.h, C.byref(sp), st.size, _p(st), _p(rd), _p(rg)))
+
+    def set_frame_records(self, records):
+        """per Mecable in registration order: (number, mark, prop, signature,
+        cut_wished, abscissa) of its trajectory record"""
+        arr = (FrameRecord * max(1, len(records)))()
+        for k, r in enumerate(records):
+            arr[k] = FrameRecord(*r)
+        self._chk(self.L.cymg_set_frame_records(self.h, len(records), arr))
+
+    def write_frame(self, time, date, real_bytes=FRAME_FLOAT, before=b"", after=b""):
+        """Simul::writeObjects (simul_file.cc:402-489) in binary mode, the fiber and
+        solid sections packed on the device; returns the frame's bytes"""
+        bb = C.create_string_buffer(bytes(before), max(1, len(before)))
+        ab = C.create_string_buffer(bytes(after), max(1, len(after)))
+        fp = FrameParams(time, date.encode() if isinstance(date, str) else date, real_bytes,
+                         C.cast(bb, C.c_void_p), len(before), C.cast(ab, C.c_void_p), len(after))
+        n = C.c_size_t()
+        self._chk(self.L.cymg_write_frame(self.h, C.byref(fp), None, 0, C.byref(n)))
+        out = C.create_string_buffer(n.value)
+        self._chk(self.L.cymg_write_frame(self.h, C.byref(fp), out, n.value, C.byref(n)))
+        return out.raw[:n.value]
 
     def steric_interactions(self):
         """Simul::setStericInteractions (simul_solve.cc:139-188) on the device"""

@@ -21,7 +21,10 @@ namespace {
 template <typename F>
 int guarded(cymg_context* ctx, F&& fn)
 {
-    if (!ctx) return CYMG_ERR_INVALID;
+    if (!ctx) {
+        set_thread_error("null context");
+        return CYMG_ERR_INVALID;
+    }
     try {
         CYMG_HIP(hipSetDevice(ctx->c.device));
         fn(ctx->c);
@@ -499,7 +502,10 @@ int cymg_create(int dim, int device, cymg_context** out)
 
 int cymg_destroy(cymg_context* ctx)
 {
-    if (!ctx) return CYMG_ERR_INVALID;
+    if (!ctx) {
+        set_thread_error("null context");
+        return CYMG_ERR_INVALID;
+    }
     Context& c = ctx->c;
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
@@ -522,7 +528,7 @@ int cymg_destroy(cymg_context* ctx)
 
 const char* cymg_last_error(const cymg_context* ctx)
 {
-    return ctx ? ctx->c.err.c_str() : "null context";
+    return ctx ? ctx->c.err.c_str() : thread_error();
 }
 
 int cymg_set_params(cymg_context* ctx, const cymg_sim_params* p)
@@ -1095,6 +1101,24 @@ int cymg_steric_interactions(cymg_context* ctx)
         if (!c.st_set) throw Error(CYMG_ERR_INVALID, "call cymg_set_steric first");
         if (c.nsys > 1) throw Error(CYMG_ERR_UNSUPPORTED, "steric interactions in a batch of independent systems");
         steric_interactions(c);
+    });
+}
+
+int cymg_set_frame_records(cymg_context* ctx, uint32_t nmec, const cymg_frame_record* records)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (nmec && !records) throw Error(CYMG_ERR_INVALID, "null argument");
+        c.h_frame_rec.assign(records, records + nmec);
+        ++c.frame_rec_gen;
+    });
+}
+
+int cymg_write_frame(cymg_context* ctx, const cymg_frame_params* params, void* out, size_t capacity, size_t* size)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!params || !size) throw Error(CYMG_ERR_INVALID, "null argument");
+        need_prepared(c);
+        write_frame(c, *params, out, capacity, size);
     });
 }
 

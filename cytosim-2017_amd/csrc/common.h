@@ -141,6 +141,17 @@ struct DeferredCall {
     double g[3];          // fixed position (clamps, long clamp center); cosinus, sinus (torque)
 };
 
+// one Mecable's record of a trajectory frame: its bytes before the points
+// (section line, newline, reference, fiber fields, point count) and where it goes
+struct FrameHead {
+    uint64_t off;         // byte offset of the record in the packed sections
+    uint32_t first;       // its first point (matIndex)
+    uint32_t np;
+    uint32_t hlen;        // bytes in head
+    uint32_t solid;       // 1: a radius follows each point
+    unsigned char head[64];
+};
+
 // entries of one deferred call in the B, C and vBAS streams
 __host__ __device__ inline void deferred_slots(uint32_t kind, int D, uint32_t s[3])
 {
@@ -215,6 +226,12 @@ void launch_step_couples(Context& c, const MotorStep& ms);
 void launch_emit_asters(Context& c, uint64_t* keyB, double* valB);
 // steric.hip: the steric grid's calls as a device emitter mark
 void steric_interactions(Context& c);
+// frame.hip: the fiber and solid sections of a trajectory frame, packed from d_pos
+void write_frame(Context& c, const cymg_frame_params& p, void* out, size_t capacity, size_t* size);
+// the message of the last call of this thread that had no context to hold it
+// (a null context, cymg_read_frame); "null context" before any
+const char* thread_error();
+void set_thread_error(const std::string& msg);
 // launchers (emit.hip)
 void launch_emit_deferred(Context& c, const EmitterMark& mk, uint64_t* keyB, double* valB, uint64_t* keyC,
                           double* valC, uint64_t* keyBas, double* valBas);

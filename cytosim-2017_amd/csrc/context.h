@@ -94,6 +94,14 @@ struct Context {
     DevBuf<DeferredCall> d_st_calls;
     uint32_t* h_st_tot = nullptr;      // pinned: calls, C entries, vBAS entries
     DevBuf<DeferredCall> d_def;
+    // trajectory frames: per-Mecable records, and the packed layout they were made for
+    std::vector<cymg_frame_record> h_frame_rec;
+    uint64_t frame_rec_gen = 0, frame_made_rec = ~0ull, frame_made_layout = ~0ull;
+    int frame_made_rb = 0;
+    size_t frame_bytes = 0;              // length of the packed sections
+    DevBuf<FrameHead> d_frame_head;
+    DevBuf<uint32_t> d_frame_pmap;       // record of each point
+    DevBuf<unsigned char> d_frame;
     DevBuf<uint64_t> s_key[3];         // stream inputs (B, C, vBAS) in call order
     DevBuf<double> s_val[3];
     size_t s_total[3] = {0, 0, 0};

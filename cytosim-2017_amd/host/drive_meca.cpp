@@ -2,7 +2,7 @@
 // interface, the way Simul::solve drives the reference's Meca
 // (simul_solve.cc:199-253): clear, add every fiber then every Solid, prepare,
 // confinement, couples, asters, solve.
-// usage: drive_meca scene.txt out.txt
+// usage: drive_meca scene.txt out.txt [frame.cmo]
 // scene.txt: dim nfib nlinks nsteps nsol naster / time_step kT viscosity
 //            tolerance tol_mode radius stiffness / per fiber: np cut rigidity
 //            then DIM*np points / per Solid: np timer, np radii, DIM*np points /
@@ -11,6 +11,9 @@
 //            matIndex of Solid points; kind 2: abscissa is the interpolation
 //            coefficient between s1 and s2) / per step: DIM*nbPts gaussian draws
 // out.txt:   per step "iterations residual", then the final points.
+// frame.cmo (optional third argument): the final state as a lossless trajectory
+//            frame (Simul::writeObjects, simul_file.cc:402-489), read back and
+//            checked against the points before it is written.
 #include <cstdio>
 #include <fstream>
 #include <iomanip>
@@ -21,7 +24,7 @@
 int main(int argc, char** argv)
 {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s scene.txt out.txt\n", argv[0]);
+        std::fprintf(stderr, "usage: %s scene.txt out.txt [frame.cmo]\n", argv[0]);
         return 2;
     }
     std::ifstream in(argv[1]);
@@ -93,6 +96,28 @@ int main(int argc, char** argv)
             for (double x : f.points) out << x << "\n";
         for (auto& s : solids)
             for (double x : s.points) out << x << "\n";
+        if (argc > 3) {
+            std::vector<cymg_frame_record> recs;
+            uint32_t number = 0;
+            for (auto& f : fibers) recs.push_back({ ++number, 0, 0, 0, f.segmentation, 0.0 });
+            for (size_t k = 0; k < solids.size(); ++k) recs.push_back({ ++number, 0, 1, 0, 0.0, 0.0 });
+            meca.setFrameRecords(recs);
+            const std::string frame = meca.writeFrame(prop.time_step * nsteps, "drive_meca", CYMG_FRAME_DOUBLE);
+            std::vector<cymg_frame_object> objs;
+            std::vector<double> pts, rad;
+            cymeca::Meca::readFrame(frame, dim, objs, pts, rad);
+            bool same = objs.size() == recs.size();
+            size_t i = 0;
+            for (auto& f : fibers)
+                for (double x : f.points) same = same && i < pts.size() && pts[i++] == x;
+            for (auto& s : solids)
+                for (double x : s.points) same = same && i < pts.size() && pts[i++] == x;
+            if (!same || i != pts.size()) {
+                std::fprintf(stderr, "the frame read back differs from the points\n");
+                return 3;
+            }
+            std::ofstream(argv[3], std::ios::binary) << frame;
+        }
     } catch (const cymeca::Exception& e) {
         std::fprintf(stderr, "error %d: %s\n", e.code, e.what());
         return 1;

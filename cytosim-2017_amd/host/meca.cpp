@@ -187,6 +187,39 @@ void Meca::stericInteractions()
     check(cymg_steric_interactions(ctx_));
 }
 
+void Meca::setFrameRecords(const std::vector<cymg_frame_record>& records)
+{
+    check(cymg_set_frame_records(ctx_, (uint32_t)records.size(), records.data()));
+}
+
+std::string Meca::writeFrame(double time, const std::string& date, int real_bytes, const std::string& before,
+                             const std::string& after)
+{
+    cymg_frame_params p = { time, date.c_str(), real_bytes, before.data(), before.size(), after.data(), after.size() };
+    size_t n = 0;
+    check(cymg_write_frame(ctx_, &p, nullptr, 0, &n));
+    std::string out(n, '\0');
+    check(cymg_write_frame(ctx_, &p, &out[0], n, &n));
+    return out;
+}
+
+size_t Meca::readFrame(const std::string& data, int dim, std::vector<cymg_frame_object>& objects,
+                       std::vector<double>& points, std::vector<double>& radii, double* time)
+{
+    uint32_t no = 0;
+    size_t np = 0, used = 0;
+    // a first pass counts, a second fills
+    int rc = cymg_read_frame(data.data(), data.size(), dim, nullptr, 0, &no, nullptr, nullptr, 0, &np, time, &used);
+    if (rc) throw Exception(rc, cymg_last_error(nullptr));
+    objects.resize(no);
+    points.resize((size_t)dim * np);
+    radii.resize(np);
+    rc = cymg_read_frame(data.data(), data.size(), dim, objects.data(), no, &no, points.data(), radii.data(), np, &np,
+                         time, &used);
+    if (rc) throw Exception(rc, cymg_last_error(nullptr));
+    return used;
+}
+
 void Meca::confineSphere(double radius, double stiffness)
 {
     check(cymg_confine_sphere(ctx_, radius, stiffness));

@@ -124,6 +124,17 @@ public:
     void setSteric(const cymg_steric_params& p, const std::vector<int>& steric, const std::vector<double>& radius,
                    const std::vector<double>& range);
     void stericInteractions();
+    // Simul::writeObjects (simul_file.cc:402-489) in binary mode: per Mecable in registration
+    // order its record fields; then the whole frame, the fiber and solid sections packed on the
+    // device between the caller's other sections (`before`: space, field; `after`: bead ...
+    // organizer), reals as floats (CYMG_FRAME_FLOAT) or lossless doubles (CYMG_FRAME_DOUBLE)
+    void setFrameRecords(const std::vector<cymg_frame_record>& records);
+    std::string writeFrame(double time, const std::string& date, int real_bytes = CYMG_FRAME_FLOAT,
+                           const std::string& before = std::string(), const std::string& after = std::string());
+    // the first frame of `data` (cymg_read_frame): records, points (dim per point), radii;
+    // returns the bytes up to the frame's end
+    static size_t readFrame(const std::string& data, int dim, std::vector<cymg_frame_object>& objects,
+                            std::vector<double>& points, std::vector<double>& radii, double* time = nullptr);
     void setCouples(const std::vector<uint32_t>& fa, const std::vector<double>& aa,
                     const std::vector<uint32_t>& fb, const std::vector<double>& ab,
                     const std::vector<double>& stiffness);

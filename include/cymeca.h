@@ -15,6 +15,7 @@
 #ifndef CYMECA_H
 #define CYMECA_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -92,6 +93,7 @@ typedef struct {
 /* context ---------------------------------------------------------------- */
 int         cymg_create(int dim, int device, cymg_context** out);
 int         cymg_destroy(cymg_context* ctx);
+/* NULL: the message of this thread's last call without a context (null context, cymg_read_frame) */
 const char* cymg_last_error(const cymg_context* ctx);
 int         cymg_set_params(cymg_context* ctx, const cymg_sim_params* p);
 
@@ -353,6 +355,60 @@ int         cymg_get_vector(cymg_context* ctx, int which, double* out);  /* vPTS
    dmat.bin (the full operator, column by column) in `directory` (NULL: the working directory);
    after cymg_setup.  For debugging small systems: dmat.bin holds size^2 doubles. */
 int         cymg_dump(cymg_context* ctx, const char* directory);
+/* trajectory frames: Simul::writeObjects (simul_file.cc:402-489) in binary mode, with the
+   fiber and solid sections (ObjectSet::write, object_set.cc:380-389; Object::writeReference,
+   object.cc:76-97; FiberNaked::write, fiber_naked.cc:1195-1202; PointSet::write,
+   point_set.cc:337-342; Solid::write, solid.cc:896-905) packed on the device from the
+   points of the last solve (or prepare).  Per Mecable in registration order, what the
+   records carry beside the points: */
+typedef struct {
+    uint32_t number;       /* Object::number() (> 0) */
+    int32_t  mark;         /* Object::mark() */
+    uint32_t prop;         /* property()->index() (< 65536) */
+    uint32_t signature;    /* FiberNaked::fnSignature (ignored for Solids) */
+    double   cut_wished;   /* FiberNaked::fnCutWished = FiberProp::segmentation (fibers) */
+    double   abscissa;     /* FiberNaked::fnAbscissa, the minus end (fibers) */
+} cymg_frame_record;
+int         cymg_set_frame_records(cymg_context* ctx, uint32_t nmec, const cymg_frame_record* records);
+/* CYMG_FRAME_FLOAT writes every real as the reference does (OutputWrapper::writeFloat,
+   iowrapper.cc:535-547: 4-byte floats), byte for byte; CYMG_FRAME_DOUBLE is the lossless
+   form of the same frame: every real as 8 bytes (OutputWrapper::writeDouble,
+   iowrapper.cc:570-582), the time as %.17g, and ", real 8" after the format on the
+   #time line */
+#define CYMG_FRAME_FLOAT  4
+#define CYMG_FRAME_DOUBLE 8
+typedef struct {
+    double      time;        /* Simul::simTime() */
+    const char* date;        /* TicToc::date (24 characters); NULL: empty */
+    int         real_bytes;  /* CYMG_FRAME_FLOAT or CYMG_FRAME_DOUBLE */
+    const void* before;      /* the sections written before the fibers (space, field) as bytes */
+    size_t      nbefore;
+    const void* after;       /* the sections after the solids (bead, sphere, single, couple, organizer) */
+    size_t      nafter;
+} cymg_frame_params;
+/* the whole frame into out[capacity]; *size = its length (capacity 0: size only) */
+int         cymg_write_frame(cymg_context* ctx, const cymg_frame_params* params, void* out, size_t capacity,
+                             size_t* size);
+/* reads the first frame of data[size] written by cymg_write_frame (either form) or by
+   Simul::writeObjects in binary mode: for each record of its fiber and solid sections the
+   metadata, and its points converted to `dim` coordinates as InputWrapper::readFloatVector
+   does (iowrapper.cc:233-283: missing coordinates are 0), with the Solid radii (0 for fiber
+   points); other sections are skipped to their end.  *used = bytes up to the end of the
+   frame.  *nobjects and *npoints count the whole frame even where they exceed max_objects /
+   max_points (only that many are stored: call with 0 to learn the sizes).  Host code only,
+   no context: errors are reported by cymg_last_error(NULL). */
+typedef struct {
+    char     tag;            /* 'f' fiber, 'd' solid */
+    uint32_t prop, number;
+    int32_t  mark;
+    uint32_t signature;
+    double   length, cut_wished, abscissa;
+    uint32_t points;         /* number of points */
+    uint64_t first;          /* its first point in points[] */
+} cymg_frame_object;
+int         cymg_read_frame(const void* data, size_t size, int dim, cymg_frame_object* objects,
+                            uint32_t max_objects, uint32_t* nobjects, double* points, double* radii,
+                            size_t max_points, size_t* npoints, double* time, size_t* used);
 int         cymg_get_points(cymg_context* ctx, double* out);   /* Mecable points after getPoints (fiber_naked.cc:414-425) */
 int         cymg_get_tensions(cymg_context* ctx, double* out); /* RigidFiber::tension, one per segment (rigid_fiber.h:90) */
 int         cymg_get_block(cymg_context* ctx, uint32_t mecable, int inverse, double* out); /* column-major bs*bs */

@@ -22,6 +22,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -3256,4 +3257,121 @@ void orc_step_couples(const orc_meca* m, uint32_t n, const uint32_t* fa, double*
         }
         state[k] = st;
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* trajectory frame (Simul::writeObjects, binary mode)                       */
+
+typedef struct {
+    unsigned char* out;
+    size_t cap, n;
+} fr_out;
+
+static void fr_put(fr_out* o, const void* p, size_t len)
+{
+    if (o->n + len <= o->cap) memcpy(o->out + o->n, p, len);
+    o->n += len;
+}
+
+static void fr_str(fr_out* o, const char* s) { fr_put(o, s, strlen(s)); }
+
+/* OutputWrapper::writeUInt8/16/32, writeInt32 (iowrapper.cc:452-533): native
+   (little-endian) bytes of the value in binary mode */
+static void fr_u8(fr_out* o, uint32_t v) { const uint8_t x = (uint8_t)v; fr_put(o, &x, 1); }
+static void fr_u16(fr_out* o, uint32_t v) { const uint16_t x = (uint16_t)v; fr_put(o, &x, 2); }
+static void fr_u32(fr_out* o, uint32_t v) { fr_put(o, &v, 4); }
+static void fr_i32(fr_out* o, int32_t v) { fr_put(o, &v, 4); }
+
+/* writeFloat (iowrapper.cc:535-547: the double is converted to float) or
+   writeDouble (iowrapper.cc:570-582) */
+static void fr_real(fr_out* o, double v, int rb)
+{
+    if (rb == 8) {
+        fr_put(o, &v, 8);
+    } else {
+        const float f = (float)v;
+        fr_put(o, &f, 4);
+    }
+}
+
+/* Object::writeReference (object.cc:76-97) */
+static void fr_reference(fr_out* o, char tag, const orc_frame_record* r)
+{
+    if (r->number <= 65535 && r->prop <= 255 && r->mark == 0) {
+        fr_put(o, &tag, 1);
+        fr_u8(o, r->prop);
+        fr_u16(o, r->number);
+    } else {
+        fr_put(o, "$", 1);
+        fr_put(o, &tag, 1);
+        fr_u16(o, r->prop);
+        fr_u32(o, r->number);
+        fr_i32(o, r->mark);
+    }
+}
+
+size_t orc_write_frame(const orc_meca* m, const orc_frame_record* rec, double time, const char* date,
+                       int rb, const void* before, size_t nbefore, const void* after, size_t nafter,
+                       unsigned char* out, size_t cap)
+{
+    const int D = m->dim;
+    fr_out o = { out, out ? cap : 0, 0 };
+    char line[256];
+    if (!date) date = "";
+    /* frame tag, time line, binary signature (simul_file.cc:415-422; FRAME_TAG simul.h:31;
+       OutputWrapper::writeBinarySignature, iowrapper.cc:378-388) */
+    snprintf(line, sizeof line, "\n\n%s %s", "#Cytosim ", date);
+    fr_str(&o, line);
+    if (rb == 8)
+        snprintf(line, sizeof line, "\n#time %.17g, dim %i, format %i, real 8", time, D, 41);
+    else
+        snprintf(line, sizeof line, "\n#time %.6f, dim %i, format %i", time, D, 41);
+    fr_str(&o, line);
+    fr_str(&o, "\n#binary ");
+    fr_u16(&o, 12592U);
+    fr_str(&o, "\n");
+    if (nbefore) fr_put(&o, before, nbefore);
+    int nf = 0, ns = 0;
+    for (int f = 0; f < m->nfib; ++f) {
+        if (m->fib[f].kind == 1) ++ns;
+        else ++nf;
+    }
+    /* fibers (simul_file.cc:443-447): ObjectSet::write (object_set.cc:380-389),
+       FiberNaked::write (fiber_naked.cc:1195-1202), PointSet::write (point_set.cc:337-342) */
+    if (nf) {
+        fr_str(&o, "\n#section fiber");
+        for (int f = 0; f < m->nfib; ++f) {
+            const orc_fiber* F = &m->fib[f];
+            if (F->kind == 1) continue;
+            fr_put(&o, "\n", 1);
+            fr_reference(&o, 'f', &rec[f]);
+            fr_u32(&o, rec[f].signature);
+            fr_real(&o, (double)(F->np - 1) * F->cut, rb);   /* length() = nbSegments() * fnCut */
+            fr_real(&o, rec[f].cut_wished, rb);
+            fr_real(&o, rec[f].abscissa, rb);
+            fr_u16(&o, F->np);
+            for (uint32_t p = 0; p < F->np; ++p)
+                for (int d = 0; d < D; ++d) fr_real(&o, m->pos[D * (F->idx + p) + d], rb);
+        }
+    }
+    /* solids (simul_file.cc:449-453): Solid::write (solid.cc:896-905) */
+    if (ns) {
+        fr_str(&o, "\n#section solid");
+        for (int f = 0; f < m->nfib; ++f) {
+            const orc_fiber* F = &m->fib[f];
+            if (F->kind != 1) continue;
+            fr_put(&o, "\n", 1);
+            fr_reference(&o, 'd', &rec[f]);
+            fr_u16(&o, F->np);
+            for (uint32_t p = 0; p < F->np; ++p) {
+                for (int d = 0; d < D; ++d) fr_real(&o, m->pos[D * (F->idx + p) + d], rb);
+                fr_real(&o, F->radius[p], rb);
+            }
+        }
+    }
+    if (nafter) fr_put(&o, after, nafter);
+    fr_str(&o, "\n#section end");
+    snprintf(line, sizeof line, "\n#end cytosim %s\n\n", date);
+    fr_str(&o, line);
+    return o.n;
 }

@@ -13,6 +13,7 @@
 #define MECA_ORACLE_H
 
 #include <stdint.h>
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -203,6 +204,23 @@ typedef struct {
 } orc_motor;
 void      orc_step_couples(const orc_meca*, uint32_t n, const uint32_t* fa, double* aa, const uint32_t* fb,
                            double* ab, const double* w, uint8_t* state, const orc_motor* mp);
+
+/* trajectory frame: Simul::writeObjects (simul_file.cc:402-489) in binary mode, the
+   fiber and solid sections from the points held by the Mecables.  real_bytes 4 writes
+   reals as OutputWrapper::writeFloat (iowrapper.cc:535-547), 8 as writeDouble
+   (iowrapper.cc:570-582; the lossless form: the time as %.17g and ", real 8" after
+   the format).  Returns the frame's length; writes out[0..cap) only.  Per Mecable
+   in registration order: */
+typedef struct {
+    uint32_t number;
+    int32_t  mark;
+    uint32_t prop;
+    uint32_t signature;
+    double   cut_wished, abscissa;
+} orc_frame_record;
+size_t    orc_write_frame(const orc_meca*, const orc_frame_record* records, double time, const char* date,
+                          int real_bytes, const void* before, size_t nbefore, const void* after, size_t nafter,
+                          unsigned char* out, size_t cap);
 
 /* LAPACK restatements (documented algorithms), exported for pinning tests */
 int       orc_dgetrf(int n, double* a, int lda, int* ipiv);

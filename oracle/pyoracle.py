@@ -55,6 +55,19 @@ class StericCall(C.Structure):
                 ("len", C.c_double), ("w", C.c_double)]
 
 
+class FrameRecord(C.Structure):
+    _fields_ = [("number", C.c_uint32), ("mark", C.c_int32), ("prop", C.c_uint32), ("signature", C.c_uint32),
+                ("cut_wished", C.c_double), ("abscissa", C.c_double)]
+
+
+def frame_records(records):
+    """ctypes array of FrameRecord from (number, mark, prop, signature, cut_wished, abscissa) tuples"""
+    arr = (FrameRecord * max(1, len(records)))()
+    for k, r in enumerate(records):
+        arr[k] = FrameRecord(*r)
+    return arr
+
+
 def build():
     """Compile the oracle with its Makefile (gcc only)."""
     import subprocess
@@ -100,6 +113,8 @@ def lib():
         "orc_confine_sphere": (None, [vp, C.c_double, C.c_double]),
         "orc_steric": (C.c_int, [vp, C.POINTER(StericParams), vp, _f64p, _f64p, C.c_int, vp, C.c_uint32,
                                  C.POINTER(C.c_uint32)]),
+        "orc_write_frame": (C.c_size_t, [vp, vp, C.c_double, C.c_char_p, C.c_int, C.c_char_p, C.c_size_t,
+                                         C.c_char_p, C.c_size_t, vp, C.c_size_t]),
         "orc_add_pure_force_i": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, _f64p]),
         "orc_inter_long_link_ie": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_double, C.c_double]),
         "orc_inter_long_link_ii": (None, [vp, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double]),
@@ -333,6 +348,17 @@ class OracleMeca:
     def inter_plane_i(self, a1, a2, ca, direction, g, w):
         self.L.orc_inter_plane_i(self.h, a1, a2, ca, np.ascontiguousarray(direction, dtype=np.float64),
                                  np.ascontiguousarray(g, dtype=np.float64), w)
+
+    def write_frame(self, records, time, date, real_bytes=4, before=b"", after=b""):
+        """Simul::writeObjects (simul_file.cc:402-489) in binary mode: the frame's bytes;
+        records: per Mecable (number, mark, prop, signature, cut_wished, abscissa)"""
+        arr = frame_records(records)
+        d = date.encode() if isinstance(date, str) else date
+        n = self.L.orc_write_frame(self.h, arr, time, d, real_bytes, before, len(before), after, len(after),
+                                   None, 0)
+        buf = C.create_string_buffer(n)
+        self.L.orc_write_frame(self.h, arr, time, d, real_bytes, before, len(before), after, len(after), buf, n)
+        return buf.raw[:n]
 
     def steric(self, extension, max_range, push, pull, steric, radius, range_, brute_force=False, cap=1 << 20):
         """Simul::setStericInteractions (simul_solve.cc:139-188) with PointGrid; returns

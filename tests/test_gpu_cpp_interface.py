@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from conftest import ROOT, oracle_assemble, oracle_from_scene, rel_err
+from frame_reader import read_frame
 
 pytestmark = pytest.mark.gpu
 
@@ -74,7 +75,8 @@ def test_cpp_driver_asters_match_oracle(scenes, oracle, tmp_path, short):
     nsteps, tol = 9, 1e-11
     write_scene(tmp_path / "scene.txt", sc, nsteps, tol, 1)
     out = tmp_path / "out.txt"
-    r = subprocess.run([DRIVER, str(tmp_path / "scene.txt"), str(out)], capture_output=True, text=True,
+    frame = tmp_path / "frame.cmo"
+    r = subprocess.run([DRIVER, str(tmp_path / "scene.txt"), str(out), str(frame)], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stderr
     vals = open(out).read().split()
@@ -86,3 +88,7 @@ def test_cpp_driver_asters_match_oracle(scenes, oracle, tmp_path, short):
             oracle_assemble(orc, sc)
         orc.solve(tol, sc.gauss(s), tol_mode=1)
     assert rel_err(pts, orc.fiber_points()) < 1e-9
+    # the lossless frame the driver wrote through the C++ interface holds the same points
+    fr = read_frame(open(frame, "rb").read(), 3)
+    assert np.array_equal(np.concatenate([o["points"].ravel() for o in fr.objects]), pts)
+    assert [o["tag"] for o in fr.objects].count("d") == sc.nsol
