@@ -89,7 +89,9 @@ struct Context {
     DevBuf<uint32_t> st_lpt, st_lfib, st_lflag, st_ppt, st_pmec;
     DevBuf<double> st_lcut, st_lrad, st_lrng, st_prad;
     DevBuf<uint32_t> st_key, st_key2, st_val, st_val2, st_lstart, st_pstart, st_lrank;
-    DevBuf<uint32_t> st_cnt, st_cntC, st_cntS, st_ocnt, st_ocntC, st_ocntS;
+    DevBuf<uint32_t> st_cnt, st_cntC, st_cntS, st_ocnt, st_ocntC, st_ocntS;   // per candidate pair
+    DevBuf<uint64_t> st_ntask, st_toff;  // pairs per (cell, region slot) and the first of each
+    DevBuf<uint32_t> st_slot;            // the (cell, region slot) of each pair
     DevBuf<unsigned char> st_temp;
     DevBuf<DeferredCall> d_st_calls;
     uint32_t* h_st_tot = nullptr;      // pinned: calls, C entries, vBAS entries

@@ -5,15 +5,17 @@
 // steric Solids whose radius exceeds REAL_EPSILON, each filed in the grid cell
 // that holds its centre (Grid::index with the bounded image, grid.h:147-150,
 // 445-455), in registration order within a cell (a stable radix sort by cell).
-// One thread per (cell, side-region slot) then walks the reference's
-// traversal (PointGrid::setInteractions, point_grid.cc:361-413): the pairs
-// inside the cell, or those with one cell of its side region (Grid::createSideRegions(1): the
-// shifts of makeRectangularGrid in order, kept when inside and at an index
-// offset >= 0), running checkPP / checkPL / checkLL with the reference's
-// arithmetic.  A first walk counts the calls each cell makes (and their stream
-// entries), a scan places them in cell order, and a second walk writes them as
-// deferred calls (interLongLink PE-PE, interSideSlidingLink PI-PE and PI-PI)
-// that the deferred emitter evaluates from the points in that order.  Calls
+// The reference's traversal (PointGrid::setInteractions, point_grid.cc:361-413)
+// visits, cell by cell, the pairs inside the cell, then those with each cell of
+// its side region (Grid::createSideRegions(1): the shifts of
+// makeRectangularGrid in order, kept when inside and at an index offset >= 0).
+// Here each (cell, region slot) first counts its candidate pairs; a scan gives
+// every pair a rank in the reference's order, and one thread per pair runs
+// checkPP / checkPL / checkLL with the reference's arithmetic.  A first pass
+// counts the calls each pair makes (and their stream entries), scans place
+// them, and a second pass writes them as deferred calls (interLongLink PE-PE,
+// interSideSlidingLink PI-PE and PI-PI) that the deferred emitter evaluates
+// from the points in that order.  Calls
 // that contribute nothing in Meca (points that share a model point) are not
 // made.  Where the reference orders two segments by the address of their
 // records (`&bb < &aa`, point_grid.cc:205,295) their rank in the cell-major
@@ -53,10 +55,10 @@ struct StArgs {
     const uint32_t* plist;
     const uint32_t* lrank;   // cell-major rank of each locus
     double push, pull;
-    uint32_t* cnt;           // count walk: calls, C entries, vBAS entries per cell
+    uint32_t* cnt;           // count pass: calls, C entries, vBAS entries per pair
     uint32_t* cntC;
     uint32_t* cntS;
-    const uint32_t* ocnt;    // emit walk: their exclusive scans
+    const uint32_t* ocnt;    // emit pass: their exclusive scans
     const uint32_t* ocntC;
     const uint32_t* ocntS;
     DeferredCall* calls;
@@ -71,15 +73,15 @@ __device__ __forceinline__ double sdot(const double* a, const double* b)
     return s;
 }
 
-// where the calls of one cell go: counted, or written from the cell's offsets
+// where the calls of one pair go: counted, or written from the pair's offsets
 template <int D, bool EMIT>
 struct Sink {
     const StArgs& a;
     uint32_t n = 0, nc = 0, ns = 0;
     uint32_t base = 0, baseC = 0, baseS = 0;
-    __device__ Sink(const StArgs& args, uint32_t cell) : a(args)
+    __device__ Sink(const StArgs& args, uint32_t pair) : a(args)
     {
-        if (EMIT) { base = a.ocnt[cell]; baseC = a.ocntC[cell]; baseS = a.ocntS[cell]; }
+        if (EMIT) { base = a.ocnt[pair]; baseC = a.ocntC[pair]; baseS = a.ocntS[pair]; }
     }
     __device__ void put(uint32_t kind, uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3, double ca, double cb,
                         double len, double w)
@@ -327,13 +329,15 @@ struct Walk {
     }
 };
 
-// the side region of a cell (Grid::createSideRegions(1), grid.h:650-861)
+// the side region of a cell (Grid::createSideRegions(1), grid.h:650-861):
+// offset of its r-th cell (r = 0: the cell itself), or -1 past its last;
+// r < 0 counts its cells
 template <int D>
-__device__ int side_region(const StArgs& a, uint32_t indx, int* off)
+__device__ int side_region(const StArgs& a, uint32_t indx, int r)
 {
     int ori[3] = { 0, 0, 0 };
-    uint32_t r = indx;
-    for (int d = 0; d < D; ++d) { ori[d] = (int)(r % (uint32_t)a.gdim[d]); r /= (uint32_t)a.gdim[d]; }
+    uint32_t rem = indx;
+    for (int d = 0; d < D; ++d) { ori[d] = (int)(rem % (uint32_t)a.gdim[d]); rem /= (uint32_t)a.gdim[d]; }
     int cnt = 0;
     constexpr int NS = D == 3 ? 27 : 9;
     for (int q = 0; q < NS; ++q) {
@@ -354,64 +358,107 @@ __device__ int side_region(const StArgs& a, uint32_t indx, int* off)
         if (!in) continue;
         for (int d = D - 1; d >= 0; --d) idx = a.gdim[d] * idx + cc[d];
         const int o = idx - (int)indx;
-        if (o >= 0) off[cnt++] = o;
+        if (o < 0) continue;
+        if (cnt == r) return o;
+        ++cnt;
     }
-    return cnt;
+    return r < 0 ? cnt : -1;
 }
 
-// one thread per (cell, region slot): slot 0 the pairs inside the cell, slot
-// r >= 1 the pairs with the r-th cell of its side region; the slots of a cell
-// follow each other, so the cell-major scan keeps the reference's order
+// (cell, region slot) pairs: slot 0 the pairs inside the cell, slot r >= 1
+// the pairs with the r-th cell of its side region; the slots of a cell follow
+// each other, so ranks in cell-major order keep the reference's order
 template <int D>
 constexpr int kSlots = D == 3 ? 14 : 5;   // (3^D + 1) / 2 cells in a side region
 
-template <int D, bool EMIT>
-__global__ __launch_bounds__(128) void k_st_walk(StArgs a)
+// candidate pairs of each slot: inside a cell the n(n-1)/2 pairs of its items
+// (points first, then loci), with a neighbour cell its items times theirs
+template <int D>
+__global__ __launch_bounds__(256) void k_st_tasks(StArgs a, uint64_t* __restrict__ ntask)
 {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= a.ncells * (uint32_t)kSlots<D>) return;
     const uint32_t indx = u / kSlots<D>;
     const int r = (int)(u % kSlots<D>);
-    Sink<D, EMIT> out(a, u);
-    const Walk<D> W{ a };
-    int reg[27];
-    const int nr = side_region<D>(a, indx, reg);
-    const uint32_t p0 = a.pstart[indx], p1 = a.pstart[indx + 1];
-    const uint32_t l0 = a.lstart[indx], l1 = a.lstart[indx + 1];
-    if ((p0 != p1 || l0 != l1) && r < nr) {
+    const uint64_t n = a.pstart[indx + 1] - a.pstart[indx] + a.lstart[indx + 1] - a.lstart[indx];
+    uint64_t m = 0;
+    if (n) {
         if (r == 0) {
-            for (uint32_t ii = p0; ii < p1; ++ii) {
-                const uint32_t i = a.plist[ii];
-                for (uint32_t jj = ii + 1; jj < p1; ++jj) {
-                    const uint32_t j = a.plist[jj];
-                    W.pp(out, a.ppt[i], a.pmec[i], a.prad[i], a.ppt[j], a.pmec[j], a.prad[j]);
-                }
-                for (uint32_t jj = l0; jj < l1; ++jj) W.pl(out, i, a.llist[jj]);
-            }
-            for (uint32_t ii = l0; ii < l1; ++ii)
-                for (uint32_t jj = ii + 1; jj < l1; ++jj) W.ll(out, a.llist[ii], a.llist[jj]);
+            m = n * (n - 1) / 2;
         } else {
-            const uint32_t side = indx + (uint32_t)reg[r];
-            const uint32_t q0 = a.pstart[side], q1 = a.pstart[side + 1];
-            const uint32_t s0 = a.lstart[side], s1 = a.lstart[side + 1];
-            for (uint32_t ii = p0; ii < p1; ++ii) {
-                const uint32_t i = a.plist[ii];
-                for (uint32_t jj = q0; jj < q1; ++jj) {
-                    const uint32_t j = a.plist[jj];
-                    W.pp(out, a.ppt[i], a.pmec[i], a.prad[i], a.ppt[j], a.pmec[j], a.prad[j]);
-                }
-                for (uint32_t jj = s0; jj < s1; ++jj) W.pl(out, i, a.llist[jj]);
-            }
-            for (uint32_t ii = l0; ii < l1; ++ii) {
-                for (uint32_t jj = q0; jj < q1; ++jj) W.pl(out, a.plist[jj], a.llist[ii]);
-                for (uint32_t jj = s0; jj < s1; ++jj) W.ll(out, a.llist[ii], a.llist[jj]);
+            const int o = side_region<D>(a, indx, r);
+            if (o > 0) {
+                const uint32_t side = indx + (uint32_t)o;
+                m = n * (a.pstart[side + 1] - a.pstart[side] + a.lstart[side + 1] - a.lstart[side]);
             }
         }
     }
+    ntask[u] = m;
+}
+
+// the slot of every pair
+__global__ __launch_bounds__(256) void k_st_fill(uint32_t nw, const uint64_t* __restrict__ toff,
+                                                 uint32_t* __restrict__ slot)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nw) return;
+    const uint64_t e = toff[u + 1];
+    for (uint64_t t = toff[u]; t < e; ++t) slot[t] = u;
+}
+
+// one thread per candidate pair, in the order of the reference's loops: inside
+// a cell, for each item (points, then loci) the later items; with a neighbour
+// cell, for each item of the cell the neighbour's points, then its loci
+template <int D, bool EMIT>
+__global__ __launch_bounds__(256) void k_st_pairs(StArgs a, uint32_t npair, const uint64_t* __restrict__ toff,
+                                                  const uint32_t* __restrict__ slot)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npair) return;
+    const uint32_t u = slot[t];
+    uint32_t k = (uint32_t)(t - toff[u]);
+    const uint32_t indx = u / kSlots<D>;
+    const int r = (int)(u % kSlots<D>);
+    Sink<D, EMIT> out(a, t);
+    const Walk<D> W{ a };
+    const uint32_t p0 = a.pstart[indx], P = a.pstart[indx + 1] - p0;
+    const uint32_t l0 = a.lstart[indx];
+    if (r == 0) {
+        const uint32_t n = P + a.lstart[indx + 1] - l0;
+        uint32_t ii = 0;
+        while (k >= n - 1 - ii) { k -= n - 1 - ii; ++ii; }
+        const uint32_t jj = ii + 1 + k;
+        if (jj < P) {
+            const uint32_t i = a.plist[p0 + ii], j = a.plist[p0 + jj];
+            W.pp(out, a.ppt[i], a.pmec[i], a.prad[i], a.ppt[j], a.pmec[j], a.prad[j]);
+        } else if (ii < P) {
+            W.pl(out, a.plist[p0 + ii], a.llist[l0 + jj - P]);
+        } else {
+            W.ll(out, a.llist[l0 + ii - P], a.llist[l0 + jj - P]);
+        }
+    } else {
+        const uint32_t side = indx + (uint32_t)side_region<D>(a, indx, r);
+        const uint32_t q0 = a.pstart[side], Q = a.pstart[side + 1] - q0;
+        const uint32_t s0 = a.lstart[side], n2 = Q + a.lstart[side + 1] - s0;
+        const uint32_t row = k / n2, col = k - row * n2;
+        if (row < P) {
+            const uint32_t i = a.plist[p0 + row];
+            if (col < Q) {
+                const uint32_t j = a.plist[q0 + col];
+                W.pp(out, a.ppt[i], a.pmec[i], a.prad[i], a.ppt[j], a.pmec[j], a.prad[j]);
+            } else {
+                W.pl(out, i, a.llist[s0 + col - Q]);
+            }
+        } else {
+            const uint32_t li = a.llist[l0 + row - P];
+            if (col < Q) W.pl(out, a.plist[q0 + col], li);
+            else W.ll(out, li, a.llist[s0 + col - Q]);
+        }
+    }
     if (!EMIT) {
-        a.cnt[u] = out.n;
-        a.cntC[u] = out.nc;
-        a.cntS[u] = out.ns;
+        a.cnt[t] = out.n;
+        a.cntC[t] = out.nc;
+        a.cntS[t] = out.ns;
     }
 }
 
@@ -595,10 +642,10 @@ void steric_interactions(Context& c)
     size_t tb = 0, tb2 = 0, tb3 = 0;
     if (nl) CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, lkey, lkey2, lval, lval2, (int)nl, 0, end_bit, st));
     if (np) CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, pkey, pkey2, pval, pval2, (int)np, 0, end_bit, st));
-    const uint32_t nw = nc * (uint32_t)(D == 3 ? kSlots<3> : kSlots<2>);   // walk threads
-    c.st_cnt.resize((size_t)nw + 1); c.st_cntC.resize((size_t)nw + 1); c.st_cntS.resize((size_t)nw + 1);
-    c.st_ocnt.resize((size_t)nw + 1); c.st_ocntC.resize((size_t)nw + 1); c.st_ocntS.resize((size_t)nw + 1);
-    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, c.st_cnt.get(), c.st_ocnt.get(), (int)(nw + 1), st));
+    const uint32_t nw = nc * (uint32_t)(D == 3 ? kSlots<3> : kSlots<2>);   // (cell, region slot)
+    c.st_ntask.resize((size_t)nw + 1);
+    c.st_toff.resize((size_t)nw + 1);
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, c.st_ntask.get(), c.st_toff.get(), (int)(nw + 1), st));
     c.st_temp.resize(std::max({ tb, tb2, tb3, (size_t)1 }));
     if (nl) CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(c.st_temp.get(), tb, lkey, lkey2, lval, lval2, (int)nl, 0, end_bit, st));
     if (np) CYMG_HIP(hipcub::DeviceRadixSort::SortPairs(c.st_temp.get(), tb2, pkey, pkey2, pval, pval2, (int)np, 0, end_bit, st));
@@ -610,21 +657,44 @@ void steric_interactions(Context& c)
     a.lstart = c.st_lstart.get(); a.llist = lval2;
     a.pstart = c.st_pstart.get(); a.plist = pval2;
     a.lrank = c.st_lrank.get();
+    // candidate pairs per slot, their ranks, and their number on the host
+    CYMG_HIP(hipMemsetAsync(c.st_ntask.get() + nw, 0, 8, st));
+    if (D == 3) hipLaunchKernelGGL(k_st_tasks<3>, blocks_for(nw, 256), dim3(256), 0, st, a, c.st_ntask.get());
+    else hipLaunchKernelGGL(k_st_tasks<2>, blocks_for(nw, 256), dim3(256), 0, st, a, c.st_ntask.get());
+    CYMG_HIP(hipGetLastError());
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.st_temp.get(), tb3, c.st_ntask.get(), c.st_toff.get(), (int)(nw + 1), st));
+    if (!c.h_st_tot) CYMG_HIP(hipHostMalloc((void**)&c.h_st_tot, 4 * sizeof(uint32_t), hipHostMallocDefault));
+    uint64_t* h_npair = reinterpret_cast<uint64_t*>(c.h_st_tot + 2);
+    CYMG_HIP(hipMemcpyAsync(h_npair, c.st_toff.get() + nw, 8, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipStreamSynchronize(st));
+    const uint64_t npair64 = *h_npair;
+    if (npair64 >= (1ull << 30)) throw Error(CYMG_ERR_UNSUPPORTED, "steric grid with more than 2^30 candidate pairs");
+    const uint32_t npair = (uint32_t)npair64;
+    c.st_slot.resize(npair ? npair : 1);
+    c.st_cnt.resize((size_t)npair + 1); c.st_cntC.resize((size_t)npair + 1); c.st_cntS.resize((size_t)npair + 1);
+    c.st_ocnt.resize((size_t)npair + 1); c.st_ocntC.resize((size_t)npair + 1); c.st_ocntS.resize((size_t)npair + 1);
+    size_t tb4 = 0;
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb4, c.st_cnt.get(), c.st_ocnt.get(), (int)(npair + 1), st));
+    c.st_temp.resize(std::max(c.st_temp.n, tb4));
     a.cnt = c.st_cnt.get(); a.cntC = c.st_cntC.get(); a.cntS = c.st_cntS.get();
     // the last entry stays 0, so the scans end with the totals
-    CYMG_HIP(hipMemsetAsync(a.cnt + nw, 0, 4, st));
-    CYMG_HIP(hipMemsetAsync(a.cntC + nw, 0, 4, st));
-    CYMG_HIP(hipMemsetAsync(a.cntS + nw, 0, 4, st));
-    if (D == 3) hipLaunchKernelGGL((k_st_walk<3, false>), blocks_for(nw, 128), dim3(128), 0, st, a);
-    else hipLaunchKernelGGL((k_st_walk<2, false>), blocks_for(nw, 128), dim3(128), 0, st, a);
-    CYMG_HIP(hipGetLastError());
-    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.st_temp.get(), tb3, a.cnt, c.st_ocnt.get(), (int)(nw + 1), st));
-    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.st_temp.get(), tb3, a.cntC, c.st_ocntC.get(), (int)(nw + 1), st));
-    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.st_temp.get(), tb3, a.cntS, c.st_ocntS.get(), (int)(nw + 1), st));
-    if (!c.h_st_tot) CYMG_HIP(hipHostMalloc((void**)&c.h_st_tot, 4 * sizeof(uint32_t), hipHostMallocDefault));
-    CYMG_HIP(hipMemcpyAsync(c.h_st_tot, c.st_ocnt.get() + nw, 4, hipMemcpyDeviceToHost, st));
-    CYMG_HIP(hipMemcpyAsync(c.h_st_tot + 1, c.st_ocntC.get() + nw, 4, hipMemcpyDeviceToHost, st));
-    CYMG_HIP(hipMemcpyAsync(c.h_st_tot + 2, c.st_ocntS.get() + nw, 4, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemsetAsync(a.cnt + npair, 0, 4, st));
+    CYMG_HIP(hipMemsetAsync(a.cntC + npair, 0, 4, st));
+    CYMG_HIP(hipMemsetAsync(a.cntS + npair, 0, 4, st));
+    if (npair) {
+        hipLaunchKernelGGL(k_st_fill, blocks_for(nw, 256), dim3(256), 0, st, nw, c.st_toff.get(), c.st_slot.get());
+        if (D == 3) hipLaunchKernelGGL((k_st_pairs<3, false>), blocks_for(npair, 256), dim3(256), 0, st, a, npair,
+                                       c.st_toff.get(), c.st_slot.get());
+        else hipLaunchKernelGGL((k_st_pairs<2, false>), blocks_for(npair, 256), dim3(256), 0, st, a, npair,
+                                c.st_toff.get(), c.st_slot.get());
+        CYMG_HIP(hipGetLastError());
+    }
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.st_temp.get(), tb4, a.cnt, c.st_ocnt.get(), (int)(npair + 1), st));
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.st_temp.get(), tb4, a.cntC, c.st_ocntC.get(), (int)(npair + 1), st));
+    CYMG_HIP(hipcub::DeviceScan::ExclusiveSum(c.st_temp.get(), tb4, a.cntS, c.st_ocntS.get(), (int)(npair + 1), st));
+    CYMG_HIP(hipMemcpyAsync(c.h_st_tot, c.st_ocnt.get() + npair, 4, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemcpyAsync(c.h_st_tot + 1, c.st_ocntC.get() + npair, 4, hipMemcpyDeviceToHost, st));
+    CYMG_HIP(hipMemcpyAsync(c.h_st_tot + 2, c.st_ocntS.get() + npair, 4, hipMemcpyDeviceToHost, st));
     CYMG_HIP(hipStreamSynchronize(st));
     const uint32_t ncalls = c.h_st_tot[0];
     // calls of earlier steric marks of this assembly stay in front
@@ -644,8 +714,10 @@ void steric_interactions(Context& c)
                                     hipMemcpyDeviceToDevice, st));
         a.ocnt = c.st_ocnt.get(); a.ocntC = c.st_ocntC.get(); a.ocntS = c.st_ocntS.get();
         a.calls = c.d_st_calls.get() + first;
-        if (D == 3) hipLaunchKernelGGL((k_st_walk<3, true>), blocks_for(nw, 128), dim3(128), 0, st, a);
-        else hipLaunchKernelGGL((k_st_walk<2, true>), blocks_for(nw, 128), dim3(128), 0, st, a);
+        if (D == 3) hipLaunchKernelGGL((k_st_pairs<3, true>), blocks_for(npair, 256), dim3(256), 0, st, a, npair,
+                                       c.st_toff.get(), c.st_slot.get());
+        else hipLaunchKernelGGL((k_st_pairs<2, true>), blocks_for(npair, 256), dim3(256), 0, st, a, npair,
+                                c.st_toff.get(), c.st_slot.get());
         CYMG_HIP(hipGetLastError());
         if (first) CYMG_HIP(hipStreamSynchronize(st));   // `keep` is freed on return
     }
