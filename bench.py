@@ -99,8 +99,6 @@ def parse():
     ap.add_argument("--guess-train-steps", type=int, default=4000)
     ap.add_argument("--guess-batch", type=int, default=512)
     ap.add_argument("--guess-lr", type=float, default=1e-3)
-    ap.add_argument("--batched-guess-lr", type=float, default=2e-4,
-                    help="Adam step of the batched workload's training (its loss rose at 1e-3)")
     ap.add_argument("--timed-launches", type=int, default=40,
                     help="iteration launches timed with HIP events for the roofline")
     ap.add_argument("--traffic", type=float, default=None,
@@ -157,13 +155,16 @@ def train_guess(pkg, m, sc, rollout, train_steps, batch, lr, tol, torch, assembl
     samples = m.guess_samples()
     t1 = time.perf_counter()
     m.guess_init(1)
-    first = m.guess_train(1, batch=batch, lr=lr, seed=1)
-    last = m.guess_train(train_steps, batch=batch, lr=lr, seed=2)
+    first = m.guess_eval(8192, seed=99)
+    last_batch = m.guess_train(train_steps, batch=batch, lr=lr, seed=2)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
+    last = m.guess_eval(8192, seed=99)
     info = {"rollout_steps": rollout, "pairs": samples, "rollout_s": round(t1 - t0, 3),
             "train_steps": train_steps, "batch": batch, "epochs": round(train_steps * batch / max(samples, 1), 2),
-            "train_s": round(t2 - t1, 3), "loss_first": first, "loss_last": last}
+            "train_s": round(t2 - t1, 3), "loss_first": first, "loss_last": last,
+            "loss": "on the same 8192 recorded samples before and after training (cymg_guess_eval)",
+            "last_minibatch_loss": last_batch}
     return info, m.guess_weights()
 
 
@@ -355,7 +356,7 @@ def batched(args, pkg, scenes, torch):
         if guess == pkg.GUESS_LEARNED:
             if weights is None:
                 info, weights = train_guess(pkg, mb, big, args.batched_rollout, args.guess_train_steps,
-                                            args.guess_batch, args.batched_guess_lr, args.tol, torch,
+                                            args.guess_batch, args.guess_lr, args.tol, torch,
                                             lambda mm: pkg.assemble(mm, big, True, False))
                 rec["untimed_training"] = info
                 mb.prepare(big.points)

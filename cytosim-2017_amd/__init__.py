@@ -44,7 +44,7 @@ EXPORTED = [
     "cymg_get_vector", "cymg_dump", "cymg_get_points", "cymg_get_tensions", "cymg_get_block", "cymg_get_matrix",
     "cymg_get_timings", "cymg_enable_timing", "cymg_time_kernel", "cymg_set_option", "cymg_get_option", "cymg_autotune",
     "cymg_guess_init", "cymg_guess_weights", "cymg_guess_record", "cymg_guess_samples",
-    "cymg_guess_train", "cymg_guess_check", "cymg_guess_features", "cymg_guess_predict",
+    "cymg_guess_train", "cymg_guess_eval", "cymg_guess_check", "cymg_guess_features", "cymg_guess_predict",
 ]
 OPT_KRYLOV_LAUNCHES, OPT_BLOCK_INVERSE, OPT_TILE_POINTS, OPT_GRAPHS, OPT_TILE_THREADS, OPT_GUESS_GUARD = range(1, 7)
 GUESS_INPUTS, GUESS_OUTPUTS, GUESS_POINTS, GUESS_WEIGHTS = 384, 96, 31, 254816
@@ -209,6 +209,7 @@ def lib():
         "cymg_guess_record": (C.c_int, [vp, u32]),
         "cymg_guess_samples": (C.c_int, [vp, C.POINTER(u32)]),
         "cymg_guess_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_uint64, C.POINTER(C.c_double)]),
+        "cymg_guess_eval": (C.c_int, [vp, C.c_int, C.c_uint64, C.POINTER(C.c_double)]),
         "cymg_guess_check": (C.c_int, [vp, vp, vp, C.c_int, C.POINTER(C.c_double), vp]),
         "cymg_guess_features": (C.c_int, [vp, vp, vp]),
         "cymg_guess_predict": (C.c_int, [vp, C.c_int, vp, vp]),
@@ -757,6 +758,12 @@ class Meca:
     def guess_train(self, steps, batch=256, lr=1e-3, seed=0):
         loss = C.c_double()
         self._chk(self.L.cymg_guess_train(self.h, int(steps), int(batch), float(lr), int(seed), C.byref(loss)))
+        return loss.value
+
+    def guess_eval(self, batch=8192, seed=99):
+        """loss of the current weights on `batch` recorded samples (no training step)"""
+        loss = C.c_double()
+        self._chk(self.L.cymg_guess_eval(self.h, int(batch), int(seed), C.byref(loss)))
         return loss.value
 
     def guess_check(self, X, T):

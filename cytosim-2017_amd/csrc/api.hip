@@ -1762,6 +1762,15 @@ int cymg_guess_train(cymg_context* ctx, int steps, int batch, double lr, uint64_
     });
 }
 
+int cymg_guess_eval(cymg_context* ctx, int batch, uint64_t seed, double* loss)
+{
+    return guarded(ctx, [&](Context& c) {
+        if (!loss) throw Error(CYMG_ERR_INVALID, "null argument");
+        if (!c.mlp.ready) throw Error(CYMG_ERR_INVALID, "no weights loaded");
+        *loss = mlp_eval(c, batch, seed);
+    });
+}
+
 int cymg_guess_check(cymg_context* ctx, const float* X, const float* T, int batch, double* loss, float* grads)
 {
     return guarded(ctx, [&](Context& c) {

@@ -48,6 +48,8 @@ void mlp_set_weights(Context& c, const float* w);
 void mlp_get_weights(Context& c, float* w);
 double mlp_check(Context& c, const float* X, const float* T, int B, float* grads);
 double mlp_train(Context& c, int steps, int B, double lr, uint64_t seed);
+// loss of the current weights on B recorded samples drawn with `seed` (no update)
+double mlp_eval(Context& c, int B, uint64_t seed);
 void mlp_prepare_step(Context& c, int precond);
 const double* mlp_guess(Context& c);
 bool mlp_guess_dropped(Context& c);

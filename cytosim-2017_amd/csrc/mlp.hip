@@ -429,6 +429,27 @@ double mlp_train(Context& c, int steps, int B, double lr, uint64_t seed)
     return last;
 }
 
+double mlp_eval(Context& c, int B, uint64_t seed)
+{
+    MlpState& s = c.mlp;
+    if (B <= 0 || B % 16) throw Error(CYMG_ERR_INVALID, "batch must be a positive multiple of 16");
+    if (s.ds_count == 0) throw Error(CYMG_ERR_INVALID, "no recorded samples to evaluate on");
+    ensure_work(c, B);
+    hipLaunchKernelGGL(k_gather_batch, dim3(B), dim3(128), 0, c.stream, s.ds_x.get(), s.ds_t.get(), s.ds_count, seed,
+                       0u, B, s.h[0].get(), s.tgt.get());
+    forward(c, B);
+    const int L = kMlpLayers - 1;
+    CYMG_HIP(hipMemsetAsync(s.loss.get(), 0, sizeof(float), c.stream));
+    const long cnt = (long)B * kMlpOut;
+    hipLaunchKernelGGL(k_mse, grid1(cnt), dim3(256), 0, c.stream, s.z[L].get(), s.tgt.get(), B, s.dz[L].get(),
+                       s.dzb[L].get(), s.loss.get());
+    CYMG_HIP(hipGetLastError());
+    float loss = 0.f;
+    CYMG_HIP(hipMemcpyAsync(&loss, s.loss.get(), 4, hipMemcpyDeviceToHost, c.stream));
+    CYMG_HIP(hipStreamSynchronize(c.stream));
+    return loss;
+}
+
 // guard of the learned guess: per block, max |b - Ax| and max |b| (NaN kept)
 __global__ __launch_bounds__(256) void k_guard_part(uint32_t n, const double* __restrict__ b,
                                                     const double* __restrict__ ax, double* __restrict__ part)

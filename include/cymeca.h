@@ -502,6 +502,9 @@ int         cymg_guess_samples(cymg_context* ctx, uint32_t* count);
    `batch` recorded samples (batch a multiple of 16); loss of the last step:
    sum of squared errors over the valid outputs / batch */
 int         cymg_guess_train(cymg_context* ctx, int steps, int batch, double lr, uint64_t seed, double* loss);
+/* the same loss for the current weights on `batch` recorded samples drawn with `seed`, without
+   a training step (a held-still measure of progress; the training loss is one minibatch's) */
+int         cymg_guess_eval(cymg_context* ctx, int batch, uint64_t seed, double* loss);
 /* forward + backward on host inputs X [batch][384] (rounded to BF16) and
    targets T [batch][96] (T[.][95] = number of valid outputs); loss and the
    gradient of every weight (CYMG_GUESS_WEIGHTS floats) */

@@ -153,10 +153,15 @@ def test_training_lowers_loss_and_guess_error(pkg, scenes):
     assert m.guess_samples() == nrec * sc.nfib
     m.guess_init(3)
     print("training", flush=True)
+    held0 = m.guess_eval(1024, seed=5)
     first = m.guess_train(1, batch=256, lr=1e-3, seed=1)
     last = m.guess_train(600, batch=256, lr=1e-3, seed=2)
-    print(f"loss {first:.4g} -> {last:.4g}", flush=True)
+    held1 = m.guess_eval(1024, seed=5)
+    print(f"loss {first:.4g} -> {last:.4g}; on 1024 held samples {held0:.4g} -> {held1:.4g}", flush=True)
     assert last < 0.5 * first
+    assert held1 < held0
+    # evaluating does not train (the loss is summed by atomics: equal to rounding)
+    assert abs(m.guess_eval(1024, seed=5) - held1) <= 1e-5 * held1
     m.guess_record(0)
     assert m.guess_samples() == nrec * sc.nfib      # stopping keeps the samples
     m.set_option(pkg.OPT_GUESS_GUARD, 0)            # iterate from the network's guess itself
