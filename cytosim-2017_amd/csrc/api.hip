@@ -145,6 +145,7 @@ void build_tiles(Context& c)
     c.ntiles = nft + (c.solid_tiles ? c.nsol : 0);
     c.h_sys_tile[c.nsys] = c.ntiles;
     c.h_tile_fib.push_back(c.nfib);
+    c.h_fslot.assign(c.nfib ? c.nfib : 1, 0);
     std::vector<TileDesc> desc(c.ntiles);
     for (uint32_t t = 0; t < c.ntiles; ++t) {
         TileDesc& d = desc[t];
@@ -158,6 +159,7 @@ void build_tiles(Context& c)
             d.tb = c.h_sys_tile[0];
             d.te = c.h_sys_tile[1];
             d.S0 = 0;
+            d.ws = 0;
             continue;
         }
         const uint32_t fb = c.h_tile_fib[t], fe = c.h_tile_fib[t + 1];
@@ -169,10 +171,27 @@ void build_tiles(Context& c)
         d.tb = c.h_sys_tile[d.sys];
         d.te = c.h_sys_tile[d.sys + 1];
         d.S0 = c.h_seg[fb];
+        // segment lanes: the tile's fibers in order, a fiber moved to the next
+        // wavefront when its segments would cross one, so that each wavefront
+        // solves whole tridiagonal systems; not packed when a fiber has more
+        // segments than a wavefront has lanes or the lanes run out
+        uint32_t lane = 0;
+        bool packed = true;
+        for (uint32_t f = fb; f < fe; ++f) {
+            const uint32_t S = c.h_np[f] - 1;
+            if (S > (uint32_t)kWave) packed = false;
+            if (lane % kWave + S > (uint32_t)kWave) lane = (lane / kWave + 1) * kWave;
+            c.h_fslot[f] = lane;
+            lane += S;
+        }
+        d.ws = (packed && lane <= (uint32_t)threads) ? lane : 0;
     }
     c.d_tile_desc.resize(c.ntiles ? c.ntiles : 1);
     c.d_tpart.resize((size_t)P_SLOTS * (c.ntiles ? c.ntiles : 1));
     hipStream_t st = c.stream;
+    c.d_fslot.resize(c.h_fslot.size());
+    CYMG_HIP(hipMemcpyAsync(c.d_fslot.get(), c.h_fslot.data(), c.h_fslot.size() * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, st));
     if (c.ntiles)
         CYMG_HIP(hipMemcpyAsync(c.d_tile_desc.get(), desc.data(), c.ntiles * sizeof(TileDesc), hipMemcpyHostToDevice,
                                 st));
@@ -296,6 +315,7 @@ void upload_layout(Context& c, bool need_points)
     const size_t S = c.nseg ? c.nseg : 1;
     for (DevBuf<double>* v : { &c.d_diff }) v->resize(S * c.dim);
     for (DevBuf<double>* v : { &c.d_jd, &c.d_je, &c.d_jd0, &c.d_je0, &c.d_lag, &c.d_pdiff }) v->resize(S);
+    c.d_pcr.resize(S * (2 * kPcrLevels + 1));
     CYMG_HIP(hipMemsetAsync(c.d_lag.get(), 0, S * 8, st));
     c.d_part.resize(P_SLOTS * (size_t)(c.nmec() ? c.nmec() : 1));
     c.d_scal.resize(2 * c.nsys);

@@ -29,6 +29,7 @@ inline FiberArgs make_fiber_args(const Context& c)
     a.jd0 = c.d_jd0.get();
     a.je0 = c.d_je0.get();
     a.pdiff = c.d_pdiff.get();
+    a.pcr = c.d_pcr.get();
     a.pfib = c.d_pfib.get();
     a.nseg = c.nseg;
     a.Brow = c.mB.rowptr.get();

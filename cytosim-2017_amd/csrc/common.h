@@ -87,6 +87,7 @@ struct TileDesc {
     uint32_t sys;       // system
     uint32_t tb, te;    // the tiles of that system
     uint32_t S0;        // first segment
+    uint32_t ws;        // lanes of the wavefront-packed segment layout (0: not packed)
 };
 
 enum { P_RR = 0, P_RMAX, P_DELTA, P_SMAX, P_TT, P_TS, P_RTS, P_RTT, P_NOISE, P_BNORM, P_SLOTS };

@@ -61,6 +61,8 @@ struct Context {
     DevBuf<uint32_t> d_np, d_idx, d_seg, d_fib_sys, d_sys_begin;
     DevBuf<uint32_t> d_pfib;           // fiber of each point
     DevBuf<TileDesc> d_tile_desc;
+    std::vector<uint32_t> h_fslot;   // each fiber's first segment lane in its tile's packed layout
+    DevBuf<uint32_t> d_fslot;
     DevBuf<uint64_t> d_blkoff;
     DevBuf<double> d_cut, d_rig, d_mob, d_rfrig, d_drag;
     DevBuf<uint32_t> d_remap;          // new point -> point of the previous registration (or kInvalid32)
@@ -69,6 +71,7 @@ struct Context {
     // vectors of DIM*nbPts (meca.h:101-106) and per-segment arrays
     DevBuf<double> d_pts, d_pos, d_sol, d_bas, d_rhs, d_for, d_tmp, d_prev_sol, d_gauss;
     DevBuf<double> d_diff, d_jd, d_je, d_jd0, d_je0, d_lag, d_pdiff;
+    DevBuf<double> d_pcr;   // PCR coefficients of JJ' ((2 kPcrLevels + 1) per segment)
     DevBuf<int> d_status;              // [0]: projection failures
     int* h_status = nullptr;           // pinned copy of d_status, queued at the end of prepare
 

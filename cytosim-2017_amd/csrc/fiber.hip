@@ -35,7 +35,8 @@ __global__ __launch_bounds__(kWave) void k_prepare(const uint32_t* __restrict__ 
                                                    double* __restrict__ jd, double* __restrict__ je,
                                                    double* __restrict__ jd0, double* __restrict__ je0,
                                                    int* __restrict__ status, const double* __restrict__ dragp,
-                                                   double visc, double r0d, double r1d)
+                                                   double visc, double r0d, double r1d, double* __restrict__ pcr,
+                                                   uint32_t nseg)
 {
     __shared__ double sdif[P * D];
     __shared__ double sd[P], se[P];
@@ -64,6 +65,37 @@ __global__ __launch_bounds__(kWave) void k_prepare(const uint32_t* __restrict__ 
             se[q] = -od;
             je0[seg + q] = -od;
         }
+    }
+    __syncthreads();
+    if (S <= kWave) {
+        // The reduction coefficients of parallel cyclic reduction on JJ' (the
+        // recurrence of pcr_solve_span on the rows a, b, c; they do not depend
+        // on the right-hand side), so that each projection during the solve
+        // only combines right-hand sides.  Rows past the fiber are identity rows.
+        const int q = threadIdx.x;
+        double ra = 0.0, rb = 1.0, rc = 0.0;
+        if (q < S) {
+            rb = sd[q];
+            if (q > 0) ra = se[q - 1];
+            if (q + 1 < S) rc = se[q];
+        }
+        for (int l = 0; l < kPcrLevels; ++l) {
+            const int k = 1 << l;
+            double al = __shfl_up(ra, k, kWave), bl = __shfl_up(rb, k, kWave), cl = __shfl_up(rc, k, kWave);
+            double au = __shfl_down(ra, k, kWave), bu = __shfl_down(rb, k, kWave), cu = __shfl_down(rc, k, kWave);
+            if (q < k) { al = 0.0; bl = 1.0; cl = 0.0; }
+            if (q + k >= kWave) { au = 0.0; bu = 1.0; cu = 0.0; }
+            const double alpha = -ra / bl;
+            const double gamma = -rc / bu;
+            rb = rb + alpha * cl + gamma * au;
+            ra = alpha * al;
+            rc = gamma * cu;
+            if (q < S) {
+                pcr[(size_t)(2 * l) * nseg + seg + q] = alpha;
+                pcr[(size_t)(2 * l + 1) * nseg + seg + q] = gamma;
+            }
+        }
+        if (q < S) pcr[(size_t)(2 * kPcrLevels) * nseg + seg + q] = rb;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -119,12 +151,12 @@ void launch_fiber_prepare(Context& c)
         CYMG_FIBER_LAUNCH(k_prepare, 3, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_seg.get(), c.d_cut.get(),
                            c.d_rig.get(), c.d_pos.get(), c.d_pts.get(), c.d_mob.get(), c.d_rfrig.get(),
                            c.d_diff.get(), c.d_jd.get(), c.d_je.get(), c.d_jd0.get(), c.d_je0.get(), c.d_status.get(), c.d_drag.get(),
-                           c.prm.viscosity, c.prm.hydro_radius0, c.prm.hydro_radius1);
+                           c.prm.viscosity, c.prm.hydro_radius0, c.prm.hydro_radius1, c.d_pcr.get(), c.nseg);
     else
         CYMG_FIBER_LAUNCH(k_prepare, 2, g, b, 0, c.stream, c.d_np.get(), c.d_idx.get(), c.d_seg.get(), c.d_cut.get(),
                            c.d_rig.get(), c.d_pos.get(), c.d_pts.get(), c.d_mob.get(), c.d_rfrig.get(),
                            c.d_diff.get(), c.d_jd.get(), c.d_je.get(), c.d_jd0.get(), c.d_je0.get(), c.d_status.get(), c.d_drag.get(),
-                           c.prm.viscosity, c.prm.hydro_radius0, c.prm.hydro_radius1);
+                           c.prm.viscosity, c.prm.hydro_radius0, c.prm.hydro_radius1, c.d_pcr.get(), c.nseg);
     CYMG_HIP(hipGetLastError());
 }
 

@@ -9,6 +9,10 @@
 namespace cymg {
 
 // Everything a per-fiber kernel needs, passed by value.
+// Levels of parallel cyclic reduction that cover a tridiagonal system of up
+// to kWave rows (steps 1, 2, 4, ..., 32).
+constexpr int kPcrLevels = 6;
+
 struct FiberArgs {
     const uint32_t* np;       // points per fiber
     const uint32_t* idx;      // matIndex
@@ -24,6 +28,9 @@ struct FiberArgs {
     const double* jd0;        // JJ' diagonal 2|d_s|^2 (unfactored)
     const double* je0;        // JJ' off-diagonal -d_s.d_{s+1} (unfactored)
     const double* pdiff;      // mtJJtiJforce (P' coefficients)
+    const double* pcr;        // PCR coefficients of JJ' (fibers of at most kWave segments):
+                              // [2l][seg] alpha and [2l+1][seg] gamma of level l < kPcrLevels,
+                              // [2 kPcrLevels][seg] the reduced diagonal
     const uint32_t* pfib;     // fiber of each point
     uint32_t nseg;            // segments in total
     // full symmetric CSR of mB (rows = points) and mC (rows = dofs)
@@ -51,10 +58,68 @@ struct FiberLds {
 // thread, instead of by the thread that owns the point
 constexpr uint32_t kHubRow = 24;
 
+// Wavefront reductions (every lane of the wavefront must be active): DPP
+// moves inside each row of 16 lanes (the lane xor 1 and xor 2 quad
+// permutations, then the half-row mirror i <-> 7 - i and the row mirror
+// i <-> 15 - i), after which every lane of a row holds the row's result; the
+// four row results are read with readlane and combined in row order.  Every
+// lane returns the same value.  A butterfly of cross-lane shuffles took six
+// LDS-crossbar round trips; these moves are register operations.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)dpp_i32<CTRL>((int)(unsigned)(b & 0xffffffffull));
+    const unsigned hi = (unsigned)dpp_i32<CTRL>((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppRowMirror = 0x140;
+constexpr int kDppHalfMirror = 0x141;
+
+template <typename Op>
+__device__ __forceinline__ double wave_reduce(double v, const Op& op)
+{
+    v = op(v, dpp_f64<kDppXor1>(v));
+    v = op(v, dpp_f64<kDppXor2>(v));
+    v = op(v, dpp_f64<kDppHalfMirror>(v));
+    v = op(v, dpp_f64<kDppRowMirror>(v));
+    const double r0 = readlane_f64(v, 0), r1 = readlane_f64(v, 16);
+    const double r2 = readlane_f64(v, 32), r3 = readlane_f64(v, 48);
+    return op(op(r0, r1), op(r2, r3));
+}
+
+template <typename Op>
+__device__ __forceinline__ int wave_reduce_i32(int v, const Op& op)
+{
+    v = op(v, dpp_i32<kDppXor1>(v));
+    v = op(v, dpp_i32<kDppXor2>(v));
+    v = op(v, dpp_i32<kDppHalfMirror>(v));
+    v = op(v, dpp_i32<kDppRowMirror>(v));
+    const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return op(op(r0, r1), op(r2, r3));
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    return wave_reduce(v, [](double a, double b) { return a + b; });
 }
 
 __device__ __forceinline__ double nan_max(double a, double b)
@@ -66,8 +131,17 @@ __device__ __forceinline__ double nan_max(double a, double b)
 
 __device__ __forceinline__ double wave_max(double v)
 {
-    for (int o = 32; o > 0; o >>= 1) v = nan_max(v, __shfl_xor(v, o, kWave));
-    return v;
+    return wave_reduce(v, [](double a, double b) { return nan_max(a, b); });
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+    return wave_reduce(v, [](double a, double b) { return fmin(a, b); });
+}
+
+__device__ __forceinline__ int wave_max_i32(int v)
+{
+    return wave_reduce_i32(v, [](int a, int b) { return a > b ? a : b; });
 }
 
 // dptts2 with one right-hand side (LAPACK): solve L*D*L' x = b, sequential

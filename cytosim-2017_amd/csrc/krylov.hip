@@ -117,8 +117,7 @@ __device__ __forceinline__ double wmin(const double* part, uint32_t fb, uint32_t
 #pragma unroll
         for (int u = 0; u < kRed; ++u) v = fmin(v, t[u]);
     }
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
-    return v;
+    return wave_min(v);
 }
 
 // the 4 partials of D, stored together per entry ([entry][4])
@@ -921,6 +920,7 @@ struct TpLds {
     uint32_t fo[kTileMaxFibers + 1];
     uint32_t fseg[kTileMaxFibers];
     int fuse[kTileMaxFibers];
+    uint32_t fsl[kTileMaxFibers];   // first segment lane of each fiber (packed layout)
     int maxS;
     double bc[5];
     double wr[5 * TpCfg<NT>::waves];
@@ -948,6 +948,10 @@ struct Gath {
     Rec g[PRE];
 };
 
+// The first PRE entries of the share: addresses clamped to the share's last
+// entry and the weights beyond it zero, so the column and value loads are all
+// issued before any is used (a load under a per-entry branch made the compiler
+// wait for each in turn: one memory round trip per entry).
 template <int PRE, typename Rec>
 __device__ __forceinline__ void gath_entries(Gath<PRE, Rec>& G, const uint32_t* __restrict__ col,
                                              const double* __restrict__ val)
@@ -956,20 +960,28 @@ __device__ __forceinline__ void gath_entries(Gath<PRE, Rec>& G, const uint32_t* 
     for (int v = 0; v < PRE; ++v) {
         G.col[v] = 0;
         G.val[v] = 0.0;
-        if (G.e + v < G.e1) {
-            G.col[v] = col[G.e + v];
-            G.val[v] = val[G.e + v];
+    }
+    if (G.e < G.e1) {
+        const uint32_t el = G.e1 - 1;
+#pragma unroll
+        for (int v = 0; v < PRE; ++v) {
+            const uint32_t q = G.e + v < el ? G.e + v : el;
+            G.col[v] = col[q];
+            const double w = val[q];
+            G.val[v] = G.e + v < G.e1 ? w : 0.0;
         }
     }
 }
 
+// the records of the prefetched entries (clamped entries repeat the last one)
 template <int PRE, typename Rec>
 __device__ __forceinline__ void gath_values(Gath<PRE, Rec>& G, const Rec* __restrict__ rec, uint32_t mul,
                                             uint32_t add)
 {
+    if (G.e < G.e1) {
 #pragma unroll
-    for (int v = 0; v < PRE; ++v)
-        if (G.e + v < G.e1) G.g[v] = rec[G.col[v] * mul + add];
+        for (int v = 0; v < PRE; ++v) G.g[v] = rec[G.col[v] * mul + add];
+    }
 }
 
 // sum of val * X(record) over the share, in entry order
@@ -1046,6 +1058,10 @@ struct TpRow {
     // segment constants (thread s < ts), masked at the fiber ends after the tables
     double sdf[D], sjd, sja, sjc;
     int sk, sp;
+    // packed layout with precomputed reduction coefficients: the segment of the
+    // thread's lane (-1: none), its alpha and gamma per level, reduced diagonal
+    int psg;
+    double pal[kPcrLevels], pga[kPcrLevels], pbf;
 };
 
 // the thread's segment (s = threadIdx.x < ts): rfDiff and the JJ' entries of
@@ -1080,6 +1096,28 @@ __device__ __forceinline__ void tp_roles(const Tile& T, int& u, bool& act, bool&
     own = act && h == 0;
 }
 
+// The row's mB and mC extents, all four loaded before any is used (the
+// shares are computed after both loads), then the thread's share of each.
+template <int D, typename GBT, typename GCT>
+__device__ __forceinline__ void tp_extents(const FiberArgs& a, const Tile& T, int u, bool act, bool split, int h,
+                                           uint32_t g, GBT& GB, GCT& GC)
+{
+    uint32_t b0 = 0, b1 = 0, c0 = 0, c1 = 0;
+    if (act) {
+        if (a.Buse) {
+            const uint32_t r = T.P0 + (uint32_t)u / D;
+            b0 = a.Brow[r];
+            b1 = a.Brow[r + 1];
+        }
+        if (a.Cuse) {
+            c0 = a.Crow[g];
+            c1 = a.Crow[g + 1];
+        }
+    }
+    share(b0, b1, split, h, GB.e, GB.e1);
+    share(c0, c1, split, h, GC.e, GC.e1);
+}
+
 // P' and rfDiff of the segments after and before the owner's point, by
 // global index (masked later with the fiber's extent)
 template <int D, typename TB, int NB>
@@ -1104,7 +1142,7 @@ __device__ __forceinline__ void tp_row_constants(const FiberArgs& a, const Tile&
     }
 }
 
-// after the tables: the row's fiber, the block-inverse share, and the
+// after the tables: the row's fiber, its block-inverse share, and the
 // segment's fiber (masking the JJ' couplings at the fiber ends)
 template <int D, typename TB, int NB, typename LDS>
 __device__ __forceinline__ void tp_after_tables(const FiberArgs& a, const LDS& L, const Tile& T, const TB* B,
@@ -1112,38 +1150,13 @@ __device__ __forceinline__ void tp_after_tables(const FiberArgs& a, const LDS& L
 {
     R.j0 = R.j1 = 0;
     R.k = R.fo = R.np = R.p = 0;
-#pragma unroll
-    for (int v = 0; v < NB; ++v) R.blk[v] = TB(0);
     if (R.act) {
         const uint32_t pt = (uint32_t)R.u / D;
         R.k = tile_fiber_of(L.fo, (int)T.nft, pt);
         R.fo = (int)L.fo[R.k];
         R.np = (int)L.fo[R.k + 1] - R.fo;
         R.p = (int)pt - R.fo;
-        if (B != nullptr && L.fuse[R.k]) {
-            const int bs = D * R.np;
-            share(0, (uint32_t)bs, R.split, R.h, R.j0, R.j1);
-            const TB* col = B + L.fblk[R.k] + (R.u - D * R.fo);
-            if constexpr (std::is_same<TB, float>::value) {
-                // FP32: addresses clamped to the share's last column, entries
-                // beyond it zeroed by a select, so the loads are issued together
-                if (R.j1 > R.j0) {
-                    const uint32_t jl = R.j1 - 1;
-#pragma unroll
-                    for (int v = 0; v < NB; ++v) {
-                        const uint32_t j = R.j0 + v < jl ? R.j0 + v : jl;
-                        const TB e = __builtin_nontemporal_load(col + (size_t)bs * j);
-                        R.blk[v] = R.j0 + v < R.j1 ? e : TB(0);
-                    }
-                }
-            } else {
-                // FP64: predicated (the branch-free form holds twice the
-                // registers live and measures slower)
-#pragma unroll
-                for (int v = 0; v < NB; ++v)
-                    if (R.j0 + v < R.j1) R.blk[v] = __builtin_nontemporal_load(col + (size_t)bs * (R.j0 + v));
-            }
-        }
+        if (B != nullptr && L.fuse[R.k]) share(0, (uint32_t)(D * R.np), R.split, R.h, R.j0, R.j1);
     }
     R.sk = R.sp = 0;
     const int s = threadIdx.x;
@@ -1158,10 +1171,76 @@ __device__ __forceinline__ void tp_after_tables(const FiberArgs& a, const LDS& L
     }
 }
 
+// The first NB columns of the row's share of the block inverse, requested
+// after the coupling gathers of the launch: loads return in order, so a gather
+// issued after them would wait for the whole block stream before the operator
+// could start; issued here, the stream overlaps the operator's LDS phases.
+template <int D, typename TB, int NB, typename LDS>
+__device__ __forceinline__ void tp_block_prefetch(const LDS& L, const TB* B, TpRow<D, TB, NB>& R)
+{
+#pragma unroll
+    for (int v = 0; v < NB; ++v) R.blk[v] = TB(0);
+    if (R.j1 > R.j0) {
+        const int bs = D * R.np;
+        const TB* col = B + L.fblk[R.k] + (R.u - D * R.fo);
+        if constexpr (std::is_same<TB, float>::value) {
+            // FP32: addresses clamped to the share's last column, entries
+            // beyond it zeroed by a select, so the loads are issued together
+            const uint32_t jl = R.j1 - 1;
+#pragma unroll
+            for (int v = 0; v < NB; ++v) {
+                const uint32_t j = R.j0 + v < jl ? R.j0 + v : jl;
+                const TB e = col[(size_t)bs * j];
+                R.blk[v] = R.j0 + v < R.j1 ? e : TB(0);
+            }
+        } else {
+            // FP64: predicated (the branch-free form holds twice the
+            // registers live and measures slower)
+#pragma unroll
+            for (int v = 0; v < NB; ++v)
+                if (R.j0 + v < R.j1) R.blk[v] = col[(size_t)bs * (R.j0 + v)];
+        }
+    }
+}
+
+// The packed layout's lane of this thread: the segment it solves and the
+// precomputed reduction coefficients of that segment, requested right after
+// the tables so that they arrive during the gathers.  Levels past the tile's
+// longest fiber are not loaded (the solve stops before them).
+template <int D, typename TB, int NB, typename LDS>
+__device__ __forceinline__ void tp_pcr_prefetch(const FiberArgs& a, const LDS& L, const Tile& T,
+                                                TpRow<D, TB, NB>& R)
+{
+    R.psg = -1;
+#pragma unroll
+    for (int l = 0; l < kPcrLevels; ++l) R.pal[l] = R.pga[l] = 0.0;
+    R.pbf = 1.0;
+    const int s = threadIdx.x;
+    if (!T.ws || s >= (int)T.ws) return;
+    int lo = 0, hi = (int)T.nft;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (L.fsl[mid] <= (uint32_t)s) lo = mid;
+        else hi = mid;
+    }
+    const int off = s - (int)L.fsl[lo];
+    if (off >= (int)(L.fo[lo + 1] - L.fo[lo]) - 1) return;
+    R.psg = (int)L.fo[lo] - lo + off;
+    const size_t g = T.S0 + (uint32_t)R.psg, ns = a.nseg;
+#pragma unroll
+    for (int l = 0; l < kPcrLevels; ++l) {
+        if ((1 << l) < L.maxS) {
+            R.pal[l] = a.pcr[(size_t)(2 * l) * ns + g];
+            R.pga[l] = a.pcr[(size_t)(2 * l + 1) * ns + g];
+        }
+    }
+    R.pbf = a.pcr[(size_t)(2 * kPcrLevels) * ns + g];
+}
+
 // The operator over the tile once L.x holds the operand and the helpers'
 // coupling sums are in L.hb/L.hc: forces, multipliers, L.y.  bsum/csum: the
 // owner's own shares.  Ends synchronised.
-template <int D, typename TB, int NB, typename LDS>
+template <int D, typename TB, int NB, bool PK, typename LDS>
 __device__ __forceinline__ void tp_operator(LDS& L, const Tile& T, const TpRow<D, TB, NB>& R, double bsum,
                                             double csum)
 {
@@ -1210,7 +1289,60 @@ __device__ __forceinline__ void tp_operator(LDS& L, const Tile& T, const TpRow<D
 #pragma unroll
         for (int d = 0; d < D; ++d) t += R.sdf[d] * (L.f[D * (pt + 1) + d] - L.f[D * pt + d]);
     }
-    if (ts <= kWave) {
+    if (PK && T.ws && T.ws <= blockDim.x) {
+        // packed layout with precomputed coefficients: only the right-hand
+        // sides move, each level two shuffles and two products
+        if (s < ts) L.lam[s] = t;
+        __syncthreads();
+        if ((s & ~(kWave - 1)) < (int)T.ws) {
+            const int lane = s & (kWave - 1);
+            double dv = R.psg >= 0 ? L.lam[R.psg] : 0.0;
+#pragma unroll
+            for (int l = 0; l < kPcrLevels; ++l) {
+                const int k = 1 << l;
+                if (k >= L.maxS) break;
+                double dl = __shfl_up(dv, k, kWave), du = __shfl_down(dv, k, kWave);
+                if (lane < k) dl = 0.0;
+                if (lane + k >= kWave) du = 0.0;
+                dv = dv + R.pal[l] * dl + R.pga[l] * du;
+            }
+            const double lam = dv / R.pbf;
+            if (R.psg >= 0) L.lam[R.psg] = lam;
+        }
+    } else if (T.ws && T.ws <= blockDim.x) {
+        // packed layout: the segment rows are handed through LDS to the lanes
+        // of the packed layout, where every wavefront holds whole fibers and
+        // solves them with shuffles (one barrier instead of two per step)
+        if (s < ts) {
+            L.pa[s] = R.sja;
+            L.pb[s] = R.sjd;
+            L.pc[s] = R.sjc;
+            L.lam[s] = t;
+        }
+        __syncthreads();
+        if ((s & ~(kWave - 1)) < (int)T.ws) {
+            double ra = 0.0, rb = 1.0, rc = 0.0, rd = 0.0;
+            int sg = -1;
+            if (s < (int)T.ws) {
+                int lo = 0, hi = (int)T.nft;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (L.fsl[mid] <= (uint32_t)s) lo = mid;
+                    else hi = mid;
+                }
+                const int off = s - (int)L.fsl[lo];
+                if (off < (int)(L.fo[lo + 1] - L.fo[lo]) - 1) {
+                    sg = (int)L.fo[lo] - lo + off;
+                    ra = L.pa[sg];
+                    rb = L.pb[sg];
+                    rc = L.pc[sg];
+                    rd = L.lam[sg];
+                }
+            }
+            const double lam = pcr_solve_span(kWave, L.maxS, ra, rb, rc, rd);
+            if (sg >= 0) L.lam[sg] = lam;
+        }
+    } else if (ts <= kWave) {
         if (s < kWave) {
             const double lam = pcr_solve_span(ts, L.maxS, R.sja, R.sjd, R.sjc, t);
             if (s < ts) L.lam[s] = lam;
@@ -1269,13 +1401,13 @@ __device__ __forceinline__ double tp_block_apply(LDS& L, const Tile& T, const Tp
 #pragma unroll
                 for (int v = 0; v < CH; ++v) {
                     const uint32_t jj = j + v < jl ? j + v : jl;
-                    const TB e = __builtin_nontemporal_load(col + (size_t)bs * jj);
+                    const TB e = col[(size_t)bs * jj];
                     c[v] = j + v < R.j1 ? e : TB(0);
                 }
             } else {
 #pragma unroll
                 for (int v = 0; v < CH; ++v)
-                    c[v] = (j + v < R.j1) ? __builtin_nontemporal_load(col + (size_t)bs * (j + v)) : TB(0);
+                    c[v] = (j + v < R.j1) ? col[(size_t)bs * (j + v)] : TB(0);
             }
 #pragma unroll
             for (int v = 0; v < CH; ++v) {
@@ -1594,11 +1726,7 @@ __global__ __launch_bounds__(NT) void k_tp(KArgs k)
     Gath<TpCfg<NT>::prec, Rec> GC;
     GB.e = GB.e1 = GC.e = GC.e1 = 0;
     const int dd = R.u - D * (R.u / D);
-    if (R.act && a.Buse) {
-        const uint32_t r = T.P0 + (uint32_t)R.u / D;
-        share(a.Brow[r], a.Brow[r + 1], R.split, R.h, GB.e, GB.e1);
-    }
-    if (R.act && a.Cuse) share(a.Crow[g], a.Crow[g + 1], R.split, R.h, GC.e, GC.e1);
+    tp_extents<D>(a, T, R.u, R.act, R.split, R.h, g, GB, GC);
     gath_entries(GB, a.Bcol, a.Bval);
     gath_entries(GC, a.Ccol, a.Cval);
     gath_values(GB, rec, (uint32_t)D, (uint32_t)dd);
@@ -1607,6 +1735,7 @@ __global__ __launch_bounds__(NT) void k_tp(KArgs k)
     tp_seg_constants<D, TB>(a, T, R);
     SysScalars sc = state_load(k.st_in + T.sys);
     tile_tables<PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
+    if (T.ws && threadIdx.x < T.nft) L.fsl[threadIdx.x] = k.ta.fslot[T.fb + threadIdx.x];
     const uint32_t nt = k.ta.ntiles;
     tp_partials<FIRST>(k, T, L.bc);
     __syncthreads();
@@ -1619,6 +1748,7 @@ __global__ __launch_bounds__(NT) void k_tp(KArgs k)
     if (first && threadIdx.x == 0) state_store(k.st_out + T.sys, sc);
     if (sc.done) return;
     tp_after_tables<D, TB>(a, L, T, PcBlocks<PC>::get(k), R);
+    if constexpr (NT == kTpThreads) tp_pcr_prefetch<D, TB>(a, L, T, R);
     // vector updates and phat; q1 = (shat, w, phat_old, z_old) of the row
     double rmax = 0.0, yi = vy, phi = 0.0;
     if (R.own) {
@@ -1647,8 +1777,9 @@ __global__ __launch_bounds__(NT) void k_tp(KArgs k)
         L.hb[R.u] = bsum;
         L.hc[R.u] = csum;
     }
+    tp_block_prefetch<D, TB>(L, PcBlocks<PC>::get(k), R);
     __syncthreads();
-    tp_operator<D, TB>(L, T, R, bsum, csum);
+    tp_operator<D, TB, TpCfg<NT>::blk, NT == kTpThreads>(L, T, R, bsum, csum);
     double dl = 0.0;
     if (R.own) {
         k.v[g] = L.y[R.u];
@@ -1707,11 +1838,7 @@ __global__ __launch_bounds__(NT) void k_tq(KArgs k)
     Gath<TpCfg<NT>::prec, double2> GC;
     GB.e = GB.e1 = GC.e = GC.e1 = 0;
     const int dd = R.u - D * (R.u / D);
-    if (R.act && a.Buse) {
-        const uint32_t r = T.P0 + (uint32_t)R.u / D;
-        share(a.Brow[r], a.Brow[r + 1], R.split, R.h, GB.e, GB.e1);
-    }
-    if (R.act && a.Cuse) share(a.Crow[g], a.Crow[g + 1], R.split, R.h, GC.e, GC.e1);
+    tp_extents<D>(a, T, R.u, R.act, R.split, R.h, g, GB, GC);
     gath_entries(GB, a.Bcol, a.Bval);
     gath_entries(GC, a.Ccol, a.Cval);
     gath_values(GB, rec, (uint32_t)D, (uint32_t)dd);
@@ -1720,6 +1847,7 @@ __global__ __launch_bounds__(NT) void k_tq(KArgs k)
     tp_seg_constants<D, TB>(a, T, R);
     SysScalars sc = state_load(k.st_in + T.sys);
     tile_tables<PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
+    if (T.ws && threadIdx.x < T.nft) L.fsl[threadIdx.x] = k.ta.fslot[T.fb + threadIdx.x];
     const uint32_t nt = k.ta.ntiles;
     tq_partials(k, T, L.bc);
     __syncthreads();
@@ -1728,6 +1856,7 @@ __global__ __launch_bounds__(NT) void k_tq(KArgs k)
     if (first && threadIdx.x == 0) state_store(k.st_out + T.sys, sc);
     if (sc.done) return;
     tp_after_tables<D, TB>(a, L, T, PcBlocks<PC>::get(k), R);
+    if constexpr (NT == kTpThreads) tp_pcr_prefetch<D, TB>(a, L, T, R);
     double smax = 0.0, si = 0.0, shi = 0.0;
     if (R.own) {
         si = vr + (-sc.alpha) * vv;
@@ -1744,8 +1873,9 @@ __global__ __launch_bounds__(NT) void k_tq(KArgs k)
         L.hb[R.u] = bsum;
         L.hc[R.u] = csum;
     }
+    tp_block_prefetch<D, TB>(L, PcBlocks<PC>::get(k), R);
     __syncthreads();
-    tp_operator<D, TB>(L, T, R, bsum, csum);
+    tp_operator<D, TB, TpCfg<NT>::blk, NT == kTpThreads>(L, T, R, bsum, csum);
     double tt = 0, ts = 0, rts = 0, rtt = 0;
     if (R.own) {
         const double ti = L.y[R.u];
@@ -1809,6 +1939,7 @@ static KArgs make_kargs(Context& c)
     k.part4 = c.d_part.get() + (size_t)P_TT * c.nmec();
     k.fpart = c.d_part.get();
     k.ta.desc = c.d_tile_desc.get();
+    k.ta.fslot = c.d_fslot.get();
     k.ta.ntiles = c.ntiles;
     k.st_in = nullptr;
     k.st_out = nullptr;

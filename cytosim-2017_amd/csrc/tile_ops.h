@@ -21,6 +21,7 @@ constexpr int kTileMaxFibers = 64;
 // The tiling (TileDesc, one per tile, read with one scalar load).
 struct TileArgs {
     const TileDesc* desc;
+    const uint32_t* fslot;   // per fiber: first lane of its segments in the packed layout
     uint32_t ntiles;
 };
 
@@ -41,7 +42,7 @@ struct TileLds {
 };
 
 struct Tile {
-    uint32_t t, fb, nft, P0, tp, ts, U0, nu, sys, tb, te, S0;
+    uint32_t t, fb, nft, P0, tp, ts, U0, nu, sys, tb, te, S0, ws;
 };
 
 template <int D>
@@ -61,6 +62,7 @@ __device__ __forceinline__ Tile tile_begin(const TileArgs& ta, uint32_t t)
     T.tb = d.tb;
     T.te = d.te;
     T.S0 = d.S0;
+    T.ws = d.ws;
     return T;
 }
 
@@ -85,7 +87,7 @@ __device__ __forceinline__ void tile_tables(const FiberArgs& a, LDS& L, const Ti
         S = (int)a.np[f] - 1;
     }
     if (k == 0) L.fo[T.nft] = T.tp;
-    for (int o = 32; o > 0; o >>= 1) S = max(S, __shfl_xor(S, o, kWave));
+    S = wave_max_i32(S);
     if (k == 0) L.maxS = S;
 }
 
