@@ -63,7 +63,6 @@ struct Context {
     DevBuf<TileDesc> d_tile_desc;
     std::vector<uint32_t> h_fslot;   // each fiber's first segment lane in its tile's packed layout
     DevBuf<uint32_t> d_fslot;
-    DevBuf<uint4> d_text;            // per tile and thread of the iteration launch: its mB and mC entry ranges
     DevBuf<uint64_t> d_blkoff;
     DevBuf<double> d_cut, d_rig, d_mob, d_rfrig, d_drag;
     DevBuf<uint32_t> d_remap;          // new point -> point of the previous registration (or kInvalid32)

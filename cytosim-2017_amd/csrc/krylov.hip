@@ -55,7 +55,6 @@ struct KArgs {
     double* part4;      // [fiber or tile][4] partials of D
     const double* fpart;// per-fiber partials of the setup (noise, |b|_inf)
     TileArgs ta;
-    const uint4* text;  // per tile and thread: (mB first, end, mC first, end) of the thread's row share
     SysScalars* st_in;
     SysScalars* st_out;
     uint32_t nfib;      // Mecables: the stride of the partial sums (fibers, then Solids)
@@ -1692,29 +1691,6 @@ __device__ void solid_tile_q(const KArgs& k, const Tile& T, SolidTileLds& S)
     }
 }
 
-// The mB and mC entry ranges of every thread's row share (tp_extents), once
-// per solve: the iteration launches read them at an address known from the
-// workgroup and thread index alone, in parallel with the tile descriptor,
-// rather than after it (one dependent memory round trip less per launch).
-struct EntryRange {
-    uint32_t e, e1;
-};
-template <int D, int NT>
-__global__ __launch_bounds__(NT) void k_tile_extents(FiberArgs a, TileArgs ta, uint4* __restrict__ out)
-{
-    const Tile T = tile_begin<D>(ta, blockIdx.x);
-    uint4 ex = make_uint4(0, 0, 0, 0);
-    if (T.nft != 0) {
-        int u, h;
-        bool act, own, split;
-        tp_roles<D, NT>(T, u, act, own, split, h);
-        EntryRange GB, GC;
-        tp_extents<D>(a, T, u, act, split, h, T.U0 + (uint32_t)u, GB, GC);
-        ex = make_uint4(GB.e, GB.e1, GC.e, GC.e1);
-    }
-    out[(size_t)blockIdx.x * NT + threadIdx.x] = ex;
-}
-
 template <int D, int PC, bool FIRST, int NT>
 __global__ __launch_bounds__(NT) void k_tp(KArgs k)
 {
@@ -1723,7 +1699,6 @@ __global__ __launch_bounds__(NT) void k_tp(KArgs k)
     constexpr int NW = TpCfg<NT>::waves;
     __shared__ TpLds<D, NT> L;
     const FiberArgs& a = k.fa;
-    const uint4 ext = k.text[(size_t)blockIdx.x * NT + threadIdx.x];
     const Tile T = tile_begin<D>(k.ta, blockIdx.x);
     if constexpr (NT == kTpThreads) {
         if (T.nft == 0) {
@@ -1751,10 +1726,7 @@ __global__ __launch_bounds__(NT) void k_tp(KArgs k)
     Gath<TpCfg<NT>::prec, Rec> GC;
     GB.e = GB.e1 = GC.e = GC.e1 = 0;
     const int dd = R.u - D * (R.u / D);
-    GB.e = ext.x;
-    GB.e1 = ext.y;
-    GC.e = ext.z;
-    GC.e1 = ext.w;
+    tp_extents<D>(a, T, R.u, R.act, R.split, R.h, g, GB, GC);
     gath_entries(GB, a.Bcol, a.Bval);
     gath_entries(GC, a.Ccol, a.Cval);
     gath_values(GB, rec, (uint32_t)D, (uint32_t)dd);
@@ -1843,7 +1815,6 @@ __global__ __launch_bounds__(NT) void k_tq(KArgs k)
     constexpr int NW = TpCfg<NT>::waves;
     __shared__ TpLds<D, NT> L;
     const FiberArgs& a = k.fa;
-    const uint4 ext = k.text[(size_t)blockIdx.x * NT + threadIdx.x];
     const Tile T = tile_begin<D>(k.ta, blockIdx.x);
     if constexpr (NT == kTpThreads) {
         if (T.nft == 0) {
@@ -1867,10 +1838,7 @@ __global__ __launch_bounds__(NT) void k_tq(KArgs k)
     Gath<TpCfg<NT>::prec, double2> GC;
     GB.e = GB.e1 = GC.e = GC.e1 = 0;
     const int dd = R.u - D * (R.u / D);
-    GB.e = ext.x;
-    GB.e1 = ext.y;
-    GC.e = ext.z;
-    GC.e1 = ext.w;
+    tp_extents<D>(a, T, R.u, R.act, R.split, R.h, g, GB, GC);
     gath_entries(GB, a.Bcol, a.Bval);
     gath_entries(GC, a.Ccol, a.Cval);
     gath_values(GB, rec, (uint32_t)D, (uint32_t)dd);
@@ -1972,7 +1940,6 @@ static KArgs make_kargs(Context& c)
     k.fpart = c.d_part.get();
     k.ta.desc = c.d_tile_desc.get();
     k.ta.fslot = c.d_fslot.get();
-    k.text = c.d_text.get();
     k.ta.ntiles = c.ntiles;
     k.st_in = nullptr;
     k.st_out = nullptr;
@@ -2118,7 +2085,6 @@ static void run_tiled(Context& c, const double* guess, int max_iter, double tol,
     const KArgs kP0 = with(S1, S0, 0), kQ0 = with(S0, S1, 0);
     const KArgs kP1 = with(S1, S0, 1), kQ1 = with(S0, S1, 1);
     const dim3 g(c.ntiles), b(kTileThreads), bt(NT);
-    hipLaunchKernelGGL((k_tile_extents<D, NT>), g, bt, 0, c.stream, k.fa, k.ta, c.d_text.get());
     hipLaunchKernelGGL((k_tinit<D, PC>), g, b, 0, c.stream, kI, guess, tol, tol_mode, max_iter, c.prm.time_step);
     bool first_p = true;
     CYMG_HIP(hipGetLastError());
@@ -2226,7 +2192,6 @@ static double time_tiled(Context& c, int pairs)
     const KArgs kP0 = with(S1, S0, 0), kQ0 = with(S0, S1, 0);
     const KArgs kP1 = with(S1, S0, 1), kQ1 = with(S0, S1, 1);
     const dim3 g(c.ntiles), b(kTileThreads), bt(NT);
-    hipLaunchKernelGGL((k_tile_extents<D, NT>), g, bt, 0, c.stream, k.fa, k.ta, c.d_text.get());
     hipLaunchKernelGGL((k_tinit<D, PC>), g, b, 0, c.stream, kI, nullptr, 0.0, CYMG_TOL_RELATIVE, 1 << 30,
                        c.prm.time_step);
     pairs += pairs & 1;
@@ -2253,7 +2218,6 @@ static double time_tiled(Context& c, int pairs)
 static void tiled_buffers(Context& c)
 {
     const size_t n = c.n();
-    c.d_text.resize((size_t)(c.ntiles ? c.ntiles : 1) * (size_t)tile_threads_of(c));
     const bool fresh = c.k_g2.cap < n;
     c.d_tpart.resize((size_t)P_SLOTS * (c.ntiles ? c.ntiles : 1));
     c.k_r.resize(n); c.k_rt.resize(n); c.k_v.resize(n); c.k_t.resize(n);
