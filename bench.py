@@ -183,15 +183,17 @@ def fixed_tiles(pkg, m, args):
 
 
 def copy_calibration(args, torch):
-    """A plain device-to-device copy of known size, for scaling the HBM byte
-    counters (FETCH_SIZE, WRITE_SIZE) of a profiler run."""
+    """A plain device-side copy kernel of known size (an elementwise kernel that
+    reads and writes every element once: a memcpy may run on the DMA engine,
+    which the counters do not see), for scaling the HBM byte counters
+    (FETCH_SIZE, WRITE_SIZE) of a profiler run."""
     if args.copy_calibration_mb <= 0:
         return
     nb = args.copy_calibration_mb << 20
     src = torch.ones(nb // 8, dtype=torch.float64, device="cuda:0")
     dst = torch.empty_like(src)
     for _ in range(3):
-        dst.copy_(src)
+        torch.mul(src, 1.0, out=dst)
     torch.cuda.synchronize()
     del src, dst
 
