@@ -1,9 +1,10 @@
 """Parity at the benchmark workloads' full sizes: the 1000-fiber crosslinked
 scene (assembly bit-exact, one implicit step), the aster scene (10 Solids of
-201 points, 2000 fibers, 50000 motors; one step, then a motor step) and three
-systems of the 64-system batch, each against the oracle on the same inputs
+201 points, 2000 fibers, 50000 motors; one step, then a motor step) and every
+system of the 64-system batch, each against the oracle on the same inputs
 at a tight tolerance."""
 import dataclasses
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import pytest
@@ -63,16 +64,27 @@ def test_batch_of_64_systems_full_size(pkg, scenes, oracle):
     assert rep["converged"]
     reps = gpu.system_reports()
     pts = gpu.fiber_points()
+    gpu.close()
     offs = np.concatenate([[0], np.cumsum([sc.size for sc in batch.systems])])
     g = big.gauss(0)
-    for k in (0, 31, 63):
+
+    def oracle_step(k):
         sc = batch.systems[k]
-        print(f"system {k}: fibers={sc.nfib}", flush=True)
         orc = oracle_from_scene(oracle, sc)
         orep = orc.solve(1e-12, g[offs[k]:offs[k + 1]], tol_mode=1)
-        assert reps[k]["converged"] and orep["converged"]
-        assert rel_err(pts[offs[k]:offs[k + 1]], orc.fiber_points()) < 1e-9
-    gpu.close()
+        return k, orep["converged"], orc.fiber_points()
+
+    # every system against its own oracle step; the oracle's solves run in
+    # threads (one C call each, outside the interpreter lock), largest first
+    order = sorted(range(batch.nsys), key=lambda k: -batch.systems[k].nfib)
+    worst = 0.0
+    with ThreadPoolExecutor(8) as ex:
+        for k, conv, ref in ex.map(oracle_step, order):
+            err = rel_err(pts[offs[k]:offs[k + 1]], ref)
+            worst = max(worst, err)
+            assert reps[k]["converged"] and conv, k
+            assert err < 1e-9, (k, batch.systems[k].nfib, err)
+    print(f"64 systems against the oracle: largest relative position error {worst:.2e}", flush=True)
 
 
 def test_learned_fp32_on_the_1000_fiber_system(pkg, scenes, oracle):
