@@ -110,10 +110,15 @@ void same_system(const Context& c, std::initializer_list<uint32_t> pts)
 void build_tiles(Context& c)
 {
     const std::vector<uint32_t>& sb = c.h_sys_begin_used;
+    // one tile per Solid after the fiber tiles (a layout with Solids is one
+    // system), when every Solid fits a workgroup of the iteration launches
+    c.solid_tiles = c.nsol > 0;
+    for (uint32_t k = 0; k < c.nsol; ++k)
+        if (c.dim * c.h_sol_np[k] > 1024u) c.solid_tiles = false;
     // the tile limits of the iteration's workgroup: tiles of at most
     // threads/2 unknowns give every unknown two threads (the default), larger
     // ones one thread; a fiber longer than the target gets a tile of its own
-    const int threads = (c.tile_threads == 256 && c.nsol == 0 &&
+    const int threads = (c.tile_threads == 256 && (c.nsol == 0 || c.solid_tiles) &&
                          c.largest <= (uint32_t)tile_max_points(c, 256)) ? 256 : 512;
     const uint32_t maxp = (uint32_t)tile_max_points(c, threads);
     uint32_t target = c.tile_points;
@@ -137,11 +142,6 @@ void build_tiles(Context& c)
         }
     }
     const uint32_t nft = (uint32_t)c.h_tile_fib.size();
-    // one tile per Solid after the fiber tiles (a layout with Solids is one
-    // system), when every Solid fits a workgroup of the iteration launches
-    c.solid_tiles = c.nsol > 0;
-    for (uint32_t k = 0; k < c.nsol; ++k)
-        if (c.dim * c.h_sol_np[k] > 1024u) c.solid_tiles = false;
     c.ntiles = nft + (c.solid_tiles ? c.nsol : 0);
     c.h_sys_tile[c.nsys] = c.ntiles;
     c.h_tile_fib.push_back(c.nfib);
@@ -1673,7 +1673,7 @@ int cymg_autotune(cymg_context* ctx, int reps, int* tile_points, int* block_inve
         if ((!c.nsol || c.solid_tiles) && tiles_fit(c)) {
             const int cands[] = { 0, 16, 24, 32, 48, 64, 85, 96, 128, 170 };
             for (int nt : { 512, 256 }) {
-                if (nt == 256 && (c.nsol || c.largest > (uint32_t)tile_max_points(c, 256))) continue;
+                if (nt == 256 && c.largest > (uint32_t)tile_max_points(c, 256)) continue;
                 for (int t : cands) {
                     if (t > tile_max_points(c, nt)) continue;
                     c.tile_threads = (uint32_t)nt;

@@ -1509,20 +1509,165 @@ __device__ __forceinline__ double waves_sum(const double* w, int nw)
     return t;
 }
 
-constexpr int kSolidRowsPerThread = kSolidTileRows / kTpThreads;
+// Solid tiles of workgroups smaller than kTpThreads: the same arithmetic in
+// the same order, each row's vectors read inside its loop rather than held
+// for all of the thread's rows from the start (which would take the launch
+// past 128 VGPRs and a quarter of its resident workgroups on fiber tiles).
+template <int D, int PC, bool FIRST, int NT>
+__device__ void solid_tile_p_lean(const KArgs& k, const Tile& T, SolidTileLds& S)
+{
+    const uint32_t ks = T.fb - k.nfiber, U0 = T.U0, nu = T.nu, nt = k.ta.ntiles;
+    const int nw = blockDim.x / kWave;
+    SysScalars sc = state_load(k.st_in + T.sys);
+    tp_partials<FIRST>(k, T, S.bc);
+    __syncthreads();
+    if (sc.done) return;
+    tp_scalars(sc, S.bc);
+    if (sc.done) return;
+    double rmax = 0.0, dl = 0.0;
+    for (uint32_t u = threadIdx.x; u < nu; u += NT) {
+        const uint32_t g = U0 + u;
+        const double4 q1 = k.g1a[g];
+        double ri = k.r[g];
+        double yi = FIRST ? k.g2[g].x : 0.0;
+        if (sc.pending) {
+            k.x[g] = k.x[g] + sc.omega * q1.x;
+            ri = ri + (-sc.omega) * k.t[g];
+            k.r[g] = ri;
+            yi = q1.x + (-sc.omega) * q1.y;
+        }
+        rmax = nan_max(rmax, fabs(ri));
+        S.a[u] = sc.beta == 0.0 ? yi : yi + sc.beta * (q1.z + (-sc.omega) * q1.w);
+        S.e[u] = yi;
+    }
+    __syncthreads();
+    const auto O = [&](uint32_t i, double v) { S.y[i - U0] = v; };
+    if constexpr (FIRST) {
+        const PhatFirstComb C{ sc.beta };
+        const auto X = [&](uint32_t i) -> double { return i - U0 < nu ? S.a[i - U0] : C(k.g2[i]); };
+        solid_operator<D>(k.fa, k.sa, ks, X, O, S.red, S.lrow, kSolidTileRows + 1);
+    } else {
+        const PhatComb C{ sc.beta, sc.omega };
+        const auto X = [&](uint32_t i) -> double { return i - U0 < nu ? S.a[i - U0] : C(k.g1a[i]); };
+        solid_operator<D>(k.fa, k.sa, ks, X, O, S.red, S.lrow, kSolidTileRows + 1);
+    }
+    __syncthreads();
+    for (uint32_t u = threadIdx.x; u < nu; u += NT) {
+        const double v = S.y[u];
+        k.v[U0 + u] = v;
+        dl += k.rt[U0 + u] * v;
+    }
+    rmax = wave_max(rmax);
+    dl = wave_sum(dl);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        S.wr[threadIdx.x / kWave] = rmax;
+        S.wr[16 + threadIdx.x / kWave] = dl;
+    }
+    const auto Rec = [&](uint32_t i, double z) {
+        const uint32_t u = i - U0;
+        k.g2[i] = make_double2(S.e[u], z);
+        reinterpret_cast<double2*>(k.g1b + i)[1] = make_double2(S.a[u], z);
+    };
+    if (PC == CYMG_PRECOND_NONE) {
+        for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x) Rec(U0 + u, S.y[u]);
+        __syncthreads();
+    } else {
+        const auto In = [&](uint32_t i) -> double { return S.y[i - U0]; };
+        solid_precond<D>(k.sa, ks, In, Rec, S.red);
+    }
+    if (threadIdx.x == 0) {
+        k.part[P_RMAX * nt + T.t] = waves_max(S.wr, nw);
+        k.part[P_DELTA * nt + T.t] = waves_sum(S.wr + 16, nw);
+    }
+}
 
-// k_tp for a Solid tile
-template <int D, int PC, bool FIRST>
+template <int D, int PC, int NT>
+__device__ void solid_tile_q_lean(const KArgs& k, const Tile& T, SolidTileLds& S)
+{
+    const uint32_t ks = T.fb - k.nfiber, U0 = T.U0, nu = T.nu, nt = k.ta.ntiles;
+    const int nw = blockDim.x / kWave;
+    SysScalars sc = state_load(k.st_in + T.sys);
+    tq_partials(k, T, S.bc);
+    __syncthreads();
+    if (sc.done) return;
+    tq_scalars<PC>(sc, S.bc);
+    if (sc.done) return;
+    double smax = 0.0;
+    for (uint32_t u = threadIdx.x; u < nu; u += NT) {
+        const uint32_t g = U0 + u;
+        const double si = k.r[g] + (-sc.alpha) * k.v[g];
+        k.r[g] = si;
+        k.x[g] = k.x[g] + sc.alpha * k.g1b[g].z;
+        smax = nan_max(smax, fabs(si));
+        const double2 q2 = k.g2[g];
+        S.a[u] = q2.x + (-sc.alpha) * q2.y;
+    }
+    __syncthreads();
+    {
+        const ShatComb C{ sc.alpha };
+        const auto X = [&](uint32_t i) -> double { return i - U0 < nu ? S.a[i - U0] : C(k.g2[i]); };
+        const auto O = [&](uint32_t i, double v) { S.y[i - U0] = v; };
+        solid_operator<D>(k.fa, k.sa, ks, X, O, S.red, S.lrow, kSolidTileRows + 1);
+    }
+    __syncthreads();
+    double tt = 0, ts = 0, rts = 0, rtt = 0;
+    for (uint32_t u = threadIdx.x; u < nu; u += NT) {
+        const double ti = S.y[u];
+        const double si = k.r[U0 + u];
+        const double vrt = k.rt[U0 + u];
+        k.t[U0 + u] = ti;
+        tt += ti * ti;
+        ts += ti * si;
+        rts += vrt * si;
+        rtt += vrt * ti;
+    }
+    smax = wave_max(smax);
+    tt = wave_sum(tt);
+    ts = wave_sum(ts);
+    rts = wave_sum(rts);
+    rtt = wave_sum(rtt);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        const int w = threadIdx.x / kWave;
+        S.wr[w] = smax;
+        S.wr[16 + w] = tt;
+        S.wr[32 + w] = ts;
+        S.wr[48 + w] = rts;
+        S.wr[64 + w] = rtt;
+    }
+    const auto Rec = [&](uint32_t i, double w) {
+        reinterpret_cast<double2*>(k.g1b + i)[0] = make_double2(S.a[i - U0], w);
+    };
+    if (PC == CYMG_PRECOND_NONE) {
+        for (uint32_t u = threadIdx.x; u < nu; u += blockDim.x) Rec(U0 + u, S.y[u]);
+        __syncthreads();
+    } else {
+        const auto In = [&](uint32_t i) -> double { return S.y[i - U0]; };
+        solid_precond<D>(k.sa, ks, In, Rec, S.red);
+    }
+    if (threadIdx.x == 0) {
+        k.part[P_SMAX * nt + T.t] = waves_max(S.wr, nw);
+        *reinterpret_cast<double4*>(k.part4 + 4 * (size_t)T.t) =
+            make_double4(waves_sum(S.wr + 16, nw), waves_sum(S.wr + 32, nw), waves_sum(S.wr + 48, nw),
+                         waves_sum(S.wr + 64, nw));
+    }
+}
+
+// k_tp for a Solid tile (NT threads, kSolidTileRows / NT rows each)
+template <int D, int PC, bool FIRST, int NT>
 __device__ void solid_tile_p(const KArgs& k, const Tile& T, SolidTileLds& S)
 {
-    constexpr int RS = kSolidRowsPerThread;
+    if constexpr (NT < kTpThreads) {
+        solid_tile_p_lean<D, PC, FIRST, NT>(k, T, S);
+        return;
+    }
+    constexpr int RS = kSolidTileRows / NT;
     const uint32_t ks = T.fb - k.nfiber, U0 = T.U0, nu = T.nu, nt = k.ta.ntiles;
     const int nw = blockDim.x / kWave;
     double vr[RS], vt[RS], vx[RS], vrt[RS], vy[RS];
     double4 q1[RS];
 #pragma unroll
     for (int q = 0; q < RS; ++q) {
-        const uint32_t u = threadIdx.x + q * kTpThreads;
+        const uint32_t u = threadIdx.x + q * NT;
         vr[q] = vt[q] = vx[q] = vrt[q] = vy[q] = 0.0;
         q1[q] = make_double4(0, 0, 0, 0);
         if (u < nu) {
@@ -1541,7 +1686,7 @@ __device__ void solid_tile_p(const KArgs& k, const Tile& T, SolidTileLds& S)
     double rmax = 0.0, dl = 0.0;
 #pragma unroll
     for (int q = 0; q < RS; ++q) {
-        const uint32_t u = threadIdx.x + q * kTpThreads;
+        const uint32_t u = threadIdx.x + q * NT;
         if (u >= nu) continue;
         const uint32_t g = U0 + u;
         double ri = vr[q], yi = vy[q];
@@ -1569,7 +1714,7 @@ __device__ void solid_tile_p(const KArgs& k, const Tile& T, SolidTileLds& S)
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < RS; ++q) {
-        const uint32_t u = threadIdx.x + q * kTpThreads;
+        const uint32_t u = threadIdx.x + q * NT;
         if (u >= nu) continue;
         const double v = S.y[u];
         k.v[U0 + u] = v;
@@ -1601,17 +1746,21 @@ __device__ void solid_tile_p(const KArgs& k, const Tile& T, SolidTileLds& S)
 }
 
 // k_tq for a Solid tile
-template <int D, int PC>
+template <int D, int PC, int NT>
 __device__ void solid_tile_q(const KArgs& k, const Tile& T, SolidTileLds& S)
 {
-    constexpr int RS = kSolidRowsPerThread;
+    if constexpr (NT < kTpThreads) {
+        solid_tile_q_lean<D, PC, NT>(k, T, S);
+        return;
+    }
+    constexpr int RS = kSolidTileRows / NT;
     const uint32_t ks = T.fb - k.nfiber, U0 = T.U0, nu = T.nu, nt = k.ta.ntiles;
     const int nw = blockDim.x / kWave;
     double vr[RS], vv[RS], vx[RS], vph[RS], vrt[RS], si[RS];
     double2 q2[RS];
 #pragma unroll
     for (int q = 0; q < RS; ++q) {
-        const uint32_t u = threadIdx.x + q * kTpThreads;
+        const uint32_t u = threadIdx.x + q * NT;
         vr[q] = vv[q] = vx[q] = vph[q] = vrt[q] = si[q] = 0.0;
         q2[q] = make_double2(0, 0);
         if (u < nu) {
@@ -1630,7 +1779,7 @@ __device__ void solid_tile_q(const KArgs& k, const Tile& T, SolidTileLds& S)
     double smax = 0.0;
 #pragma unroll
     for (int q = 0; q < RS; ++q) {
-        const uint32_t u = threadIdx.x + q * kTpThreads;
+        const uint32_t u = threadIdx.x + q * NT;
         if (u >= nu) continue;
         const uint32_t g = U0 + u;
         si[q] = vr[q] + (-sc.alpha) * vv[q];
@@ -1650,7 +1799,7 @@ __device__ void solid_tile_q(const KArgs& k, const Tile& T, SolidTileLds& S)
     double tt = 0, ts = 0, rts = 0, rtt = 0;
 #pragma unroll
     for (int q = 0; q < RS; ++q) {
-        const uint32_t u = threadIdx.x + q * kTpThreads;
+        const uint32_t u = threadIdx.x + q * NT;
         if (u >= nu) continue;
         const double ti = S.y[u];
         k.t[U0 + u] = ti;
@@ -1691,21 +1840,27 @@ __device__ void solid_tile_q(const KArgs& k, const Tile& T, SolidTileLds& S)
     }
 }
 
+// at most 128 VGPRs on 256-thread tiles: four resident workgroups per compute unit
+template <int NT>
+constexpr int kTpWavesPerEu = NT == 256 ? 4 : 1;
+
 template <int D, int PC, bool FIRST, int NT>
-__global__ __launch_bounds__(NT) void k_tp(KArgs k)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kTpWavesPerEu<NT>))) void k_tp(KArgs k)
 {
     using TB = typename PcBlocks<PC>::T;
     using Rec = typename std::conditional<FIRST, double2, double4>::type;
     constexpr int NW = TpCfg<NT>::waves;
-    __shared__ TpLds<D, NT> L;
+    // a workgroup serves a fiber tile or a Solid tile: the two layouts share the LDS
+    __shared__ union {
+        TpLds<D, NT> t;
+        SolidTileLds s;
+    } sh;
+    TpLds<D, NT>& L = sh.t;
     const FiberArgs& a = k.fa;
     const Tile T = tile_begin<D>(k.ta, blockIdx.x);
-    if constexpr (NT == kTpThreads) {
-        if (T.nft == 0) {
-            __shared__ SolidTileLds SL;
-            solid_tile_p<D, PC, FIRST>(k, T, SL);
-            return;
-        }
+    if (T.nft == 0) {
+        solid_tile_p<D, PC, FIRST, NT>(k, T, sh.s);
+        return;
     }
     TpRow<D, TB, TpCfg<NT>::blk> R;
     tp_roles<D, NT>(T, R.u, R.act, R.own, R.split, R.h);
@@ -1809,19 +1964,20 @@ __global__ __launch_bounds__(NT) void k_tp(KArgs k)
 }
 
 template <int D, int PC, int NT>
-__global__ __launch_bounds__(NT) void k_tq(KArgs k)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kTpWavesPerEu<NT>))) void k_tq(KArgs k)
 {
     using TB = typename PcBlocks<PC>::T;
     constexpr int NW = TpCfg<NT>::waves;
-    __shared__ TpLds<D, NT> L;
+    __shared__ union {
+        TpLds<D, NT> t;
+        SolidTileLds s;
+    } sh;
+    TpLds<D, NT>& L = sh.t;
     const FiberArgs& a = k.fa;
     const Tile T = tile_begin<D>(k.ta, blockIdx.x);
-    if constexpr (NT == kTpThreads) {
-        if (T.nft == 0) {
-            __shared__ SolidTileLds SL;
-            solid_tile_q<D, PC>(k, T, SL);
-            return;
-        }
+    if (T.nft == 0) {
+        solid_tile_q<D, PC, NT>(k, T, sh.s);
+        return;
     }
     TpRow<D, TB, TpCfg<NT>::blk> R;
     tp_roles<D, NT>(T, R.u, R.act, R.own, R.split, R.h);
@@ -2236,7 +2392,7 @@ static void tiled_buffers(Context& c)
 // every tile fits (no Solid tile, every fiber within 256/DIM points), else 512
 int tile_threads_of(const Context& c)
 {
-    if (c.tile_threads == 256 && !c.solid_tiles && c.nsol == 0 &&
+    if (c.tile_threads == 256 && (c.nsol == 0 || c.solid_tiles) &&
         c.largest <= (uint32_t)TpCfg<256>::max_points(c.dim))
         return 256;
     return 512;

@@ -434,9 +434,10 @@ int         cymg_get_matrix(cymg_context* ctx, int which, uint32_t* nnz, uint32_
    CYMG_OPT_GRAPHS: 1 (default) replays chunks of 2 and 8 iterations of the
      two-launch iteration as captured HIP graphs, 0 launches every kernel;
    CYMG_OPT_TILE_THREADS: 512 (default) or 256 threads per tile of the
-     two-launch iteration; 256 (tiles of at most 256/DIM points, two or more
-     tiles resident per compute unit) applies to layouts without Solids whose
-     fibers all fit such a tile, others keep 512;
+     two-launch iteration; 256 (tiles of at most 256/DIM points, up to four
+     tiles resident per compute unit, a Solid's tile four rows per thread)
+     applies to layouts whose fibers all fit such a tile and whose Solids all
+     have tiles of their own, others keep 512;
    CYMG_OPT_GUESS_GUARD: 1 (default) replaces a learned guess whose residual
      |b - A x0|_inf is not below |b|_inf (that of the zero guess) by zero, on
      the device, and reports guess_used = CYMG_GUESS_ZERO; 0 keeps every guess */

@@ -74,9 +74,11 @@ def test_aster_operator_blocks_precondition(pkg, scenes, oracle, dim):
     gpu.close()
 
 
-def _aster_steps(pkg, oracle, sc, steps, tol, precondition, guess=0, init_guess=False, launches=2):
+def _aster_steps(pkg, oracle, sc, steps, tol, precondition, guess=0, init_guess=False, launches=2, threads=512):
     gpu = gpu_from_scene(pkg, sc)
     gpu.set_option(pkg.OPT_KRYLOV_LAUNCHES, launches)
+    gpu.set_option(pkg.OPT_TILE_THREADS, threads)
+    assert gpu.get_option(pkg.OPT_TILE_THREADS) == threads
     if init_guess:
         gpu.guess_init(4)
         gpu.set_option(pkg.OPT_GUESS_GUARD, 0)
@@ -97,13 +99,13 @@ def _aster_steps(pkg, oracle, sc, steps, tol, precondition, guess=0, init_guess=
     return gpu, orc, reps
 
 
-@pytest.mark.parametrize("dim,launches", [(2, 2), (3, 2), (3, 4)])
-def test_aster_steps_fp64(pkg, scenes, oracle, dim, launches):
+@pytest.mark.parametrize("dim,launches,threads", [(2, 2, 512), (3, 2, 512), (3, 4, 512), (2, 2, 256), (3, 2, 256)])
+def test_aster_steps_fp64(pkg, scenes, oracle, dim, launches, threads):
     """9 steps (every Solid gets at least one reshape: initial timers in
     [0, 7)) through the two-launch iteration (the Solids in tiles of their
-    own) and the four-launch one."""
+    own, in workgroups of 512 or 256 threads) and the four-launch one."""
     sc = scenes.mini_asters(dim=dim, seed=13)
-    gpu, orc, reps = _aster_steps(pkg, oracle, sc, 9, 1e-12, pkg.PRECOND_FP64, launches=launches)
+    gpu, orc, reps = _aster_steps(pkg, oracle, sc, 9, 1e-12, pkg.PRECOND_FP64, launches=launches, threads=threads)
     for r1, r2 in reps:
         assert r1["converged"] and r2["converged"]
     assert rel_err(gpu.vector(pkg.VEC_FOR), orc.vector(oracle.FOR)) < 1e-7
@@ -112,9 +114,10 @@ def test_aster_steps_fp64(pkg, scenes, oracle, dim, launches):
 
 
 @pytest.mark.parametrize("precond", [0, 2])
-def test_aster_step_other_preconditioners(pkg, scenes, oracle, precond):
+@pytest.mark.parametrize("threads", [512, 256])
+def test_aster_step_other_preconditioners(pkg, scenes, oracle, precond, threads):
     sc = scenes.mini_asters(dim=3, seed=14)
-    gpu, orc, reps = _aster_steps(pkg, oracle, sc, 2, 1e-12 if precond == 0 else 1e-9, precond)
+    gpu, orc, reps = _aster_steps(pkg, oracle, sc, 2, 1e-12 if precond == 0 else 1e-9, precond, threads=threads)
     assert all(r1["converged"] for r1, _ in reps)
     gpu.close()
 
@@ -127,11 +130,12 @@ def test_aster_step_learned_guess(pkg, scenes, oracle):
     gpu.close()
 
 
-def test_asters_reduced_config3(pkg, scenes, oracle):
+@pytest.mark.parametrize("threads", [512, 256])
+def test_asters_reduced_config3(pkg, scenes, oracle, threads):
     """The aster scene's shape (Solids of 1 + N points, fibers of 11-21 points,
     motors) at 3 asters x 40 fibers."""
     sc = scenes.asters(dim=3, seed=16, nasters=3, fibers_per_aster=40, nmotors=1500, spread=2.0, radius=8.0)
-    gpu, orc, reps = _aster_steps(pkg, oracle, sc, 2, 1e-11, pkg.PRECOND_FP64)
+    gpu, orc, reps = _aster_steps(pkg, oracle, sc, 2, 1e-11, pkg.PRECOND_FP64, threads=threads)
     assert all(r1["converged"] for r1, _ in reps)
     gpu.close()
 
