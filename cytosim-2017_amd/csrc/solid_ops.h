@@ -227,15 +227,41 @@ __device__ __forceinline__ void solid_coupling_moments(const FiberArgs& a, const
         if (np + 1 <= cap) {
             for (uint32_t i = threadIdx.x; i <= np; i += blockDim.x) lrow[i] = a.Brow[idx + i];
             __syncthreads();
-            const uint32_t q1 = lrow[np];
-            for (uint32_t q = lrow[0] + threadIdx.x; q < q1; q += blockDim.x) {
-                const uint32_t r = idx + row_of(lrow, np, q);
-                const double v = a.Bval[q];
-                const uint32_t c = a.Bcol[q];
-                double f[D];
+            // each thread a contiguous run of entries: one binary search for
+            // its first row, then the rows followed forward; the entries and
+            // their operands loaded four at a time (addresses clamped to the
+            // run's last entry, weights beyond it zero) before they are used
+            const uint32_t q0 = lrow[0], q1 = lrow[np];
+            const uint32_t per = (q1 - q0 + blockDim.x - 1) / blockDim.x;
+            const uint32_t qb = q0 + threadIdx.x * per;
+            const uint32_t qe = qb + per < q1 ? qb + per : q1;
+            if (qb < qe) {
+                uint32_t rr = row_of(lrow, np, qb);
+                const uint32_t ql = qe - 1;
+                for (uint32_t q = qb; q < qe; q += 4) {
+                    double v[4], x[4][D];
+                    uint32_t c[4];
 #pragma unroll
-                for (int d = 0; d < D; ++d) f[d] = v * X(D * c + d);
-                add_moments<D>(s.pos + D * r, f, acc);
+                    for (int j = 0; j < 4; ++j) {
+                        const uint32_t qq = q + j < ql ? q + j : ql;
+                        const double w = a.Bval[qq];
+                        v[j] = q + j < qe ? w : 0.0;
+                        c[j] = a.Bcol[qq];
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+#pragma unroll
+                        for (int d = 0; d < D; ++d) x[j][d] = X(D * c[j] + d);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (q + j >= qe) break;
+                        while (lrow[rr + 1] <= q + j) ++rr;
+                        double f[D];
+#pragma unroll
+                        for (int d = 0; d < D; ++d) f[d] = v[j] * x[j][d];
+                        add_moments<D>(s.pos + D * (idx + rr), f, acc);
+                    }
+                }
             }
             __syncthreads();
         } else {
