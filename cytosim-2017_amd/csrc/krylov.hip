@@ -996,24 +996,44 @@ __device__ __forceinline__ double gath_sum(const Gath<PRE, Rec>& G, const Rec* _
         if (G.e + v < G.e1) acc += G.val[v] * X(G.g[v]);
     // entries beyond the prefetched ones, a few at a time: the column and value
     // loads of a batch, then its record loads, are issued together (addresses
-    // clamped to the share's last entry, weights of the clamped ones zero) so
-    // a long row costs one memory round trip per batch instead of per entry
+    // clamped to the share's last entry, weights of the clamped ones zero),
+    // and the next batch's columns and values are requested before this
+    // batch's records are used, so a long row costs one memory round trip per
+    // batch instead of two (or one per entry)
     constexpr int CH = sizeof(Rec) > 16 ? 2 : 4;   // registers of the batch's records
     const uint32_t el = G.e1 - 1;
-    for (uint32_t e = G.e + PRE; e < G.e1; e += CH) {
-        uint32_t c[CH];
-        double w[CH];
+    uint32_t e = G.e + PRE;
+    if (e >= G.e1) return acc;
+    uint32_t c[CH];
+    double w[CH];
+    const auto entries = [&](uint32_t eb, uint32_t* cc, double* ww) {
 #pragma unroll
         for (int v = 0; v < CH; ++v) {
-            const uint32_t q = e + v < el ? e + v : el;
-            c[v] = col[q];
-            w[v] = e + v < G.e1 ? val[q] : 0.0;
+            const uint32_t q = eb + v < el ? eb + v : el;
+            cc[v] = col[q];
+            const double x = val[q];
+            ww[v] = eb + v < G.e1 ? x : 0.0;
         }
+    };
+    entries(e, c, w);
+    while (true) {
         Rec r[CH];
 #pragma unroll
         for (int v = 0; v < CH; ++v) r[v] = rec[c[v] * mul + add];
+        const uint32_t en = e + CH;
+        const bool more = en < G.e1;
+        uint32_t cn[CH];
+        double wn[CH];
+        if (more) entries(en, cn, wn);
 #pragma unroll
         for (int v = 0; v < CH; ++v) acc += w[v] * X(r[v]);
+        if (!more) break;
+        e = en;
+#pragma unroll
+        for (int v = 0; v < CH; ++v) {
+            c[v] = cn[v];
+            w[v] = wn[v];
+        }
     }
     return acc;
 }
