@@ -17,8 +17,10 @@ learned one is compared with), fp32 (FP32 blocks), learned-fp64 and
 learned-fp32 (seeded by the per-fiber network, trained on the device from
 (input, solution) pairs of an untimed rollout of zero-seeded FP64 steps with
 draws of their own; rollout and training times are reported beside the steps,
-not inside them).  Every mode's first step is compared with the oracle's FP64
-step solved to 1e-12 on the same input.
+not inside them), and on request fp32-arith (FP32 blocks applied in FP32
+arithmetic, the operand rounded and FP32 accumulation, which the four-launch
+iteration solves).  Every mode's first step is compared with the oracle's FP64 step
+solved to 1e-12 on the same input.
 
 Beside it: the 64-system batched workload (FP64, FP32 and learned + FP32), the
 aster workload (FP64 and FP32, first step against the FP64 oracle), the
@@ -46,7 +48,11 @@ MODES = {   # name: (preconditioner, initial guess)
     "fp32": ("PRECOND_FP32", "GUESS_ZERO"),
     "learned-fp64": ("PRECOND_FP64", "GUESS_LEARNED"),
     "learned-fp32": ("PRECOND_FP32", "GUESS_LEARNED"),
+    # FP32 blocks applied in FP32 arithmetic (operand rounded, FP32 accumulate):
+    # a nonlinear M^-1, solved by the four-launch iteration
+    "fp32-arith": ("PRECOND_FP32", "GUESS_ZERO"),
 }
+MODE_OPTIONS = {"fp32-arith": (("OPT_FP32_APPLY", 1),)}
 
 
 def log(msg):
@@ -89,7 +95,9 @@ def parse():
     ap.add_argument("--nlinks", type=int, default=20000)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-steps", type=int, default=24, help="oracle steps timed for cpu_baseline (0: skip)")
-    ap.add_argument("--modes", default=",".join(MODES), help="comma-separated subset of " + ", ".join(MODES))
+    ap.add_argument("--modes", default="fp64,fp64-previous,fp32,learned-fp64,learned-fp32",
+                    help="comma-separated subset of " + ", ".join(MODES) + " (fp32-arith: about 2 s per step on "
+                    "the 1000-fiber system, off by default)")
     ap.add_argument("--oracle-check", type=int, default=1,
                     help="compare each mode's first step with the oracle's FP64 step solved to 1e-12")
     ap.add_argument("--autotune", type=int, default=1,
@@ -204,6 +212,8 @@ def run_c2_mode(pkg, sc, mode, args, gauss_dev, torch, weights, ref):
     m = pkg.Meca(sc.dim, 0)
     pkg.load_scene(m, sc)
     fixed_tiles(pkg, m, args)
+    for opt, val in MODE_OPTIONS.get(mode, ()):
+        m.set_option(getattr(pkg, opt), val)
     tuned = None
     if args.autotune:
         # untimed: the autotuner picks the tile shape and block-inverse method on this system

@@ -1618,6 +1618,7 @@ int cymg_set_option(cymg_context* ctx, int option, int value)
             if (!c.layout_dirty && c.nfib) build_tiles(c);
         }
         else if (option == CYMG_OPT_GUESS_GUARD && (value == 0 || value == 1)) c.mlp.guard = value;
+        else if (option == CYMG_OPT_FP32_APPLY && (value == 0 || value == 1)) c.fp32_apply = value;
         else if (option == CYMG_OPT_TILE_THREADS && (value == 256 || value == 512)) {
             c.tile_threads = (uint32_t)value;
             if (!c.layout_dirty && c.nfib) build_tiles(c);
@@ -1637,6 +1638,7 @@ int cymg_get_option(cymg_context* ctx, int option, int* value)
         else if (option == CYMG_OPT_TILE_POINTS) *value = (int)c.tile_points;
         else if (option == CYMG_OPT_TILE_THREADS) *value = tile_threads_of(c);
         else if (option == CYMG_OPT_GUESS_GUARD) *value = c.mlp.guard;
+        else if (option == CYMG_OPT_FP32_APPLY) *value = c.fp32_apply;
         else throw Error(CYMG_ERR_INVALID, "unknown option");
     });
 }

@@ -60,6 +60,7 @@ struct KArgs {
     uint32_t nfib;      // Mecables: the stride of the partial sums (fibers, then Solids)
     uint32_t nsys;
     uint32_t nfiber;    // fibers: workgroups from nfiber on handle Solid blockIdx.x - nfiber
+    int f32apply;       // FP32 blocks applied in FP32 arithmetic (four-launch iteration only)
     SolidArgs sa;
 };
 
@@ -189,8 +190,10 @@ __device__ __forceinline__ void apply_block(const KArgs& k, uint32_t f, int bs, 
 {
     if constexpr (PC == CYMG_PRECOND_FP64)
         block_apply_mw<double, kApplyWaves>(k.B64 + k.blkoff[f], bs, k.useblk[f] != 0, xs, y, red);
-    else if constexpr (PC == CYMG_PRECOND_FP32)
-        block_apply_mw<float, kApplyWaves>(k.B32 + k.blkoff[f], bs, k.useblk[f] != 0, xs, y, red);
+    else if constexpr (PC == CYMG_PRECOND_FP32) {
+        if (k.f32apply) block_apply_mw_f32<kApplyWaves>(k.B32 + k.blkoff[f], bs, k.useblk[f] != 0, xs, y, red);
+        else block_apply_mw<float, kApplyWaves>(k.B32 + k.blkoff[f], bs, k.useblk[f] != 0, xs, y, red);
+    }
 }
 
 // row extents a Solid workgroup of the operator phases stages in the
@@ -2096,6 +2099,7 @@ static KArgs make_kargs(Context& c)
     KArgs k;
     std::memset(&k, 0, sizeof(k));
     k.fa = make_fiber_args(c);
+    k.f32apply = c.fp32_apply;
     k.blkoff = c.d_blkoff.get();
     k.useblk = c.d_useblk.get();
     k.B64 = c.d_binv.get();
@@ -2193,7 +2197,9 @@ static void run(Context& c, const double* guess, int max_iter, double tol, int t
     }
     *rep = c.sys_reports[worst];
     rep->converged = all_conv;
-    predicted = c.h_scal[worst].iter;
+    // the next solve's first chunk follows a converged solve only: an attempt
+    // that ran to its iteration limit would queue that many launches ahead
+    predicted = all_conv ? c.h_scal[worst].iter : 0;
 }
 
 // Captures (or reuses) the graphs of 2 and 8 iterations of the two-launch
@@ -2339,7 +2345,9 @@ static void run_tiled(Context& c, const double* guess, int max_iter, double tol,
     }
     *rep = c.sys_reports[worst];
     rep->converged = all_conv;
-    predicted = c.h_scal[worst].iter;
+    // the next solve's first chunk follows a converged solve only: an attempt
+    // that ran to its iteration limit would queue that many launches ahead
+    predicted = all_conv ? c.h_scal[worst].iter : 0;
 }
 
 // Average duration of one launch of the two-launch iteration (P and Q
@@ -2464,8 +2472,10 @@ void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter
                   cymg_solve_report* rep)
 {
     // tiles hold whole fibers or one Solid of at most kSolidTileRows unknowns;
-    // a larger Solid or a longer fiber runs the four-launch iteration
-    if (two_launch(c)) {
+    // a larger Solid or a longer fiber runs the four-launch iteration, and so
+    // do FP32 blocks applied in FP32 arithmetic (a nonlinear M^-1, which the
+    // two-launch recurrences cannot carry)
+    if (two_launch(c) && !(precond == CYMG_PRECOND_FP32 && c.fp32_apply)) {
         tiled_buffers(c);
         const bool small = tile_threads_of(c) == 256;
         auto go = [&](auto dim, auto nt) {

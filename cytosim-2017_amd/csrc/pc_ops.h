@@ -93,4 +93,48 @@ __device__ __forceinline__ void block_apply_mw(const T* __restrict__ B, int bs, 
     }
 }
 
+// The same product in FP32 arithmetic (CYMG_OPT_FP32_APPLY): the operand
+// rounded to FP32, one FP32 fused multiply-add per entry, the NW partial sums
+// added in FP32 in wave order (red holds them as floats), the result widened.
+// Rounding the operand makes the operator nonlinear; the four-launch
+// iteration, which applies M^-1 to each vector directly, is the one that uses it.
+template <int NW>
+__device__ __forceinline__ void block_apply_mw_f32(const float* __restrict__ B, int bs, bool use, const double* xs,
+                                                   double* y, double* red)
+{
+    constexpr int R = (kMaxFiberPoints * 3 + kWave - 1) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    if (!use) {
+        for (int i = threadIdx.x; i < bs; i += NW * kWave) y[i] = xs[i];
+        return;
+    }
+    float acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0f;
+    const int nr = (bs + kWave - 1) / kWave;
+    for (int j = w; j < bs; j += NW) {
+        const float x0 = (float)xs[j];
+        const float* c0 = B + (size_t)bs * j;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = lane + kWave * r;
+            if (r < nr && i < bs) acc[r] = fmaf(c0[i], x0, acc[r]);
+        }
+    }
+    float* rf = reinterpret_cast<float*>(red);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + kWave * r;
+        if (r < nr && i < bs) rf[w * bs + i] = acc[r];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < bs; i += NW * kWave) {
+        float v = rf[i];
+#pragma unroll
+        for (int q = 1; q < NW; ++q) v += rf[q * bs + i];
+        y[i] = (double)v;
+    }
+}
+
 }  // namespace cymg

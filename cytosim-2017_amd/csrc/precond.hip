@@ -8,6 +8,8 @@
 // inverts it in place (dgetri: inv(U), then X*L = inv(U), then column
 // interchanges) and streams the inverse to HBM, column-major, coalesced.
 // The FP32 variant builds the block in FP64 and factorises/inverts in FP32.
+#include <type_traits>
+
 #include "args.h"
 #include "pc_ops.h"
 
@@ -1084,7 +1086,8 @@ __global__ __launch_bounds__(4 * kWave) void k_precond_apply(int D, const uint32
                                                              const uint32_t* __restrict__ idxv,
                                                              const uint64_t* __restrict__ blkoff,
                                                              const int* __restrict__ useblk, const T* __restrict__ B,
-                                                             const double* __restrict__ X, double* __restrict__ Y)
+                                                             const double* __restrict__ X, double* __restrict__ Y,
+                                                             int f32apply)
 {
     __shared__ double xs[kMaxFiberPoints * 3];
     __shared__ double red[4 * kMaxFiberPoints * 3];
@@ -1093,6 +1096,12 @@ __global__ __launch_bounds__(4 * kWave) void k_precond_apply(int D, const uint32
     const uint32_t o = D * idxv[f];
     for (int j = threadIdx.x; j < bs; j += 4 * kWave) xs[j] = X[o + j];
     __syncthreads();
+    if constexpr (std::is_same<T, float>::value) {
+        if (f32apply) {
+            block_apply_mw_f32<4>(B + blkoff[f], bs, useblk[f] != 0, xs, Y + o, red);
+            return;
+        }
+    }
     block_apply_mw<T, 4>(B + blkoff[f], bs, useblk[f] != 0, xs, Y + o, red);
 }
 
@@ -1103,10 +1112,10 @@ void launch_precond_apply(Context& c, const double* X, double* Y)
     const dim3 g(c.nfib), b(4 * kWave);
     if (c.pc_precision == CYMG_PRECOND_FP32)
         hipLaunchKernelGGL(k_precond_apply<float>, g, b, 0, c.stream, c.dim, c.d_np.get(), c.d_idx.get(),
-                           c.d_blkoff.get(), c.d_useblk.get(), c.d_binv32.get(), X, Y);
+                           c.d_blkoff.get(), c.d_useblk.get(), c.d_binv32.get(), X, Y, c.fp32_apply);
     else
         hipLaunchKernelGGL(k_precond_apply<double>, g, b, 0, c.stream, c.dim, c.d_np.get(), c.d_idx.get(),
-                           c.d_blkoff.get(), c.d_useblk.get(), c.d_binv.get(), X, Y);
+                           c.d_blkoff.get(), c.d_useblk.get(), c.d_binv.get(), X, Y, 0);
     CYMG_HIP(hipGetLastError());
 }
 

@@ -440,9 +440,15 @@ int         cymg_get_matrix(cymg_context* ctx, int which, uint32_t* nnz, uint32_
      have tiles of their own, others keep 512;
    CYMG_OPT_GUESS_GUARD: 1 (default) replaces a learned guess whose residual
      |b - A x0|_inf is not below |b|_inf (that of the zero guess) by zero, on
-     the device, and reports guess_used = CYMG_GUESS_ZERO; 0 keeps every guess */
+     the device, and reports guess_used = CYMG_GUESS_ZERO; 0 keeps every guess;
+   CYMG_OPT_FP32_APPLY: 0 (default) applies the FP32 blocks of
+     CYMG_PRECOND_FP32 to the FP64 operand with FP64 accumulation (a fixed
+     linear operator, which the two-launch iteration's recurrences need);
+     1 applies them in FP32 arithmetic (operand rounded to FP32, FP32
+     accumulation), in the solve (which then runs the four-launch iteration,
+     applying M^-1 directly) and in cymg_precondition */
 enum { CYMG_OPT_KRYLOV_LAUNCHES = 1, CYMG_OPT_BLOCK_INVERSE = 2, CYMG_OPT_TILE_POINTS = 3, CYMG_OPT_GRAPHS = 4,
-       CYMG_OPT_TILE_THREADS = 5, CYMG_OPT_GUESS_GUARD = 6 };
+       CYMG_OPT_TILE_THREADS = 5, CYMG_OPT_GUESS_GUARD = 6, CYMG_OPT_FP32_APPLY = 7 };
 int         cymg_set_option(cymg_context* ctx, int option, int value);
 int         cymg_get_option(cymg_context* ctx, int option, int* value);   /* the value in use */
 /* autotuner (no counterpart in the reference): on the current system (after
