@@ -444,6 +444,7 @@ def asters_workload(args, pkg, scenes, torch):
         precond = getattr(pkg, MODES[mode][0])
         ma = pkg.Meca(3, 0)
         pkg.load_scene(ma, ast)
+        fixed_tiles(pkg, ma, args)
         its = []
         rec = {}
         t0 = None
