@@ -10,6 +10,8 @@ compared with it.
 * Two crossing fibers closer than twice their steric radius are pushed apart
   along the line of closest approach with force w (2r - d).
 * A steric_max_range too short for the segments is refused (point_grid.cc:86-95).
+* The calls come in the grid's cell order: the cell of each sphere contact's
+  first sphere, from the reference's grid dimensions, never decreases.
 """
 import numpy as np
 import pytest
@@ -240,3 +242,52 @@ def test_fiber_end_against_a_segment(oracle, dim, end, gap, first):
     want[1] = w * (2 * r - gap)
     assert np.allclose(F[tip * dim:tip * dim + dim], want, atol=1e-9 * push)
     assert np.allclose(F.reshape(-1, dim).sum(0), 0, atol=1e-9 * push)
+
+
+def _cells(pos, dim, ext, step):
+    """Cell index of each position in the reference's PointGrid: per axis
+    ceil(2 ext / step) + 2 cells of width `step` centred on the origin
+    (point_grid.cc:40-50), positions clamped to the edge cells, x fastest
+    (Grid::index, grid.h:445-453)."""
+    n = [int(np.ceil(2 * ext / step)) + 2 for _ in range(dim)]
+    inf = [-0.5 * n[d] * step for d in range(dim)]
+    idx = np.zeros(len(pos), np.int64)
+    ijk = np.zeros((len(pos), dim), np.int64)
+    for d in reversed(range(dim)):
+        c = np.clip(np.floor((pos[:, d] - inf[d]) / step).astype(np.int64), 0, n[d] - 1)
+        ijk[:, d] = c
+        idx = idx * n[d] + c
+    return idx, ijk
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("seed", [4, 5])
+def test_sphere_calls_follow_the_cell_order(oracle, dim, seed):
+    """The order of the calls (the order in which the reference assembles the
+    contributions): the grid visits its cells in index order, so the cell of
+    each contact's first sphere, computed here from the grid's dimensions,
+    never decreases along the calls, and the second sphere is in that cell or
+    a neighbour.  A grid without its two border cells makes the same calls in
+    another order and fails this."""
+    rng = np.random.default_rng(seed)
+    m = oracle.OracleMeca(dim, PARAMS)
+    pts = []
+    for k in range(60):
+        n = int(rng.integers(4, 9))
+        rad = rng.uniform(0.05, 0.14, n)
+        box = 1.9 if dim == 2 else 1.2
+        p = rng.uniform(-box, box, dim) + 0.25 * rng.standard_normal((n, dim))
+        m.add_solid(rad, p.ravel())
+        pts.append(p)
+    pos = np.concatenate(pts)
+    m.prepare(pos.ravel())
+    st = np.ones(60, np.int32)
+    ext, step = 2.0, 0.3
+    calls = m.steric([ext] * dim, step, 300.0, 100.0, st, np.zeros(60), np.zeros(60))
+    assert len(calls) > 100 and all(c[0] == 0 for c in calls)
+    first = np.array([c[1][0] for c in calls])
+    second = np.array([c[1][1] for c in calls])
+    ca, ija = _cells(pos[first], dim, ext, step)
+    cb, ijb = _cells(pos[second], dim, ext, step)
+    assert np.all(np.diff(ca) >= 0), "contacts out of the grid's cell order"
+    assert np.abs(ija - ijb).max() <= 1
