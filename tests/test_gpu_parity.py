@@ -258,6 +258,32 @@ def test_tiles_of_fibers(pkg, scenes, oracle, tile_points):
     assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-9
 
 
+# segments per fiber around the wavefront width: fibers that end on a
+# wavefront's last lane (63+1, 31+33, 32 after 32), fibers moved to the next
+# wavefront, two 64-segment fibers in one tile and a 65-segment fiber, whose
+# tile solves its multipliers through LDS instead of packed lanes
+_LANE_EDGE_SEGMENTS = [1, 63, 1, 31, 32, 33, 2, 64, 64, 65, 31, 33, 1, 1, 62, 2]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("threads,tile_points", [(512, 0), (512, 170), (256, 0), (256, 85)])
+def test_multiplier_lanes_at_the_wavefront_edge(pkg, scenes, oracle, dim, threads, tile_points):
+    """Tension multipliers of fibers whose segment lanes fill a wavefront
+    exactly, spill into the next one or exceed it, in tiles of 256 and 512
+    threads, against the oracle's positions and tensions."""
+    lengths = 0.5 * np.array(_LANE_EDGE_SEGMENTS, dtype=np.float64)
+    sc = scenes.mini(dim=dim, seed=31, nlinks=60, radius=8.0, lengths=lengths)
+    assert list(sc.nb_points - 1) == _LANE_EDGE_SEGMENTS
+    print(f"lane edge: dim={dim} threads={threads} tile_points={tile_points}", flush=True)
+    gpu, orc, reps = _step_pair(pkg, oracle, sc, 2, 1e-11, 1, tile_points=tile_points, threads=threads)
+    assert gpu.get_option(pkg.OPT_TILE_THREADS) == threads
+    for r1, r2 in reps:
+        assert r1["converged"] and r2["converged"]
+    assert rel_err(gpu.fiber_points(), orc.fiber_points()) < 1e-9
+    assert rel_err(gpu.tensions(), orc.tensions()) < 1e-7
+    gpu.close()
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("precondition", [0, 1, 2])
 @pytest.mark.parametrize("tile_points", [0, 16, 85])
