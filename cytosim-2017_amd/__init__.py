@@ -47,7 +47,7 @@ EXPORTED = [
     "cymg_guess_train", "cymg_guess_eval", "cymg_guess_check", "cymg_guess_features", "cymg_guess_predict",
 ]
 OPT_KRYLOV_LAUNCHES, OPT_BLOCK_INVERSE, OPT_TILE_POINTS, OPT_GRAPHS, OPT_TILE_THREADS, OPT_GUESS_GUARD, \
-    OPT_FP32_APPLY = range(1, 8)
+    OPT_FP32_APPLY, OPT_BLOCK_STAGE = range(1, 9)
 GUESS_INPUTS, GUESS_OUTPUTS, GUESS_POINTS, GUESS_WEIGHTS = 384, 96, 31, 254816
 
 

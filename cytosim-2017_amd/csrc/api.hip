@@ -1619,6 +1619,7 @@ int cymg_set_option(cymg_context* ctx, int option, int value)
         }
         else if (option == CYMG_OPT_GUESS_GUARD && (value == 0 || value == 1)) c.mlp.guard = value;
         else if (option == CYMG_OPT_FP32_APPLY && (value == 0 || value == 1)) c.fp32_apply = value;
+        else if (option == CYMG_OPT_BLOCK_STAGE && value >= 0 && value <= 2) c.block_stage = value;
         else if (option == CYMG_OPT_TILE_THREADS && (value == 256 || value == 512)) {
             c.tile_threads = (uint32_t)value;
             if (!c.layout_dirty && c.nfib) build_tiles(c);
@@ -1639,6 +1640,7 @@ int cymg_get_option(cymg_context* ctx, int option, int* value)
         else if (option == CYMG_OPT_TILE_THREADS) *value = tile_threads_of(c);
         else if (option == CYMG_OPT_GUESS_GUARD) *value = c.mlp.guard;
         else if (option == CYMG_OPT_FP32_APPLY) *value = c.fp32_apply;
+        else if (option == CYMG_OPT_BLOCK_STAGE) *value = c.block_stage;
         else throw Error(CYMG_ERR_INVALID, "unknown option");
     });
 }

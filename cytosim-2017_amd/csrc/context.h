@@ -166,7 +166,8 @@ struct Context {
     // captured iteration chunks of the two-launch iteration: [preconditioner][2 or 8 iterations],
     // valid while the launch arguments they were captured with stay the same
     int use_graphs = 1;
-    int fp32_apply = 0;               // FP32 blocks applied in FP32 arithmetic (four-launch iteration)
+    int fp32_apply = 0;
+    int block_stage = 1;     // staged launches (copying wavefronts per 512-thread tile): 0 off, 1 FP32 blocks, 2 all               // FP32 blocks applied in FP32 arithmetic (four-launch iteration)
     bool solid_tiles = false;          // the Solids have tiles of their own in the two-launch iteration
     hipGraphExec_t iter_graph[3][2]{};
     std::vector<unsigned char> iter_graph_key[3][2];

@@ -892,18 +892,35 @@ constexpr int kTpPreC = 3;      // mC entries per thread in registers
 // 595 to 394 us (FP32); 16/3/2 and 16/2/1 (3 tiles per compute unit), or
 // batches of 16 or 8 columns, measured slower.
 constexpr int kTp256Blk = 8, kTp256PreB = 2, kTp256PreC = 1;
+// Staged launches: the 512 threads of the tile plus kTpStageWaves wavefronts
+// that copy the tile's block inverses into LDS (global-to-LDS loads, no
+// registers) from the start of the launch, while the other wavefronts run the
+// prologue, the gathers and the operator; the block apply then reads the
+// staged columns from LDS.  Columns beyond the stage come from memory as in
+// the other launches, the first kTpStagedBlk of each thread's share
+// prefetched into registers.  Twelve wavefronts leave 168 VGPRs per thread
+// (three per SIMD): fewer coupling entries are prefetched than with 512
+// threads.  On the 1000-fiber system the FP32 blocks' launch takes 13.4
+// instead of 14.0-14.2 us; the FP64 blocks' 16.4 instead of 15.1 us (their
+// 112 KB per tile arrive too late), so only FP32 blocks are staged by default.
+constexpr int kTpStageWaves = 4;
+constexpr int kTpStaged = kTpThreads + kTpStageWaves * kWave;
+constexpr int kTpStagedBlk = 8, kTpStagedPreB = 5, kTpStagedPreC = 2;
+constexpr uint32_t kStageBytes = 128 * 1024;
 
 template <int NT>
 struct TpCfg {
-    static constexpr int waves = NT / kWave;
-    static constexpr int split = NT / 2;                  // unknowns a split tile may hold
-    static constexpr int blk = NT == 512 ? 48 : kTp256Blk;   // block-inverse columns per thread in registers
-    static constexpr int preb = NT == 512 ? kTpPreB : kTp256PreB;
-    static constexpr int prec = NT == 512 ? kTpPreC : kTp256PreC;
+    static constexpr int work = NT == kTpStaged ? kTpThreads : NT;   // threads that own unknowns
+    static constexpr int waves = work / kWave;
+    static constexpr int split = work / 2;                // unknowns a split tile may hold
+    // block-inverse columns per thread in registers
+    static constexpr int blk = NT == kTpStaged ? kTpStagedBlk : NT == 512 ? 48 : kTp256Blk;
+    static constexpr int preb = NT == kTpStaged ? kTpStagedPreB : NT == 512 ? kTpPreB : kTp256PreB;
+    static constexpr int prec = NT == kTpStaged ? kTpStagedPreC : NT == 512 ? kTpPreC : kTp256PreC;
     static constexpr int chunk = kApplyChunk;                 // block columns per load batch beyond blk
     static __host__ __device__ constexpr int max_points(int D)
     {
-        return NT / D < kTileMaxPoints ? NT / D : kTileMaxPoints;
+        return work / D < kTileMaxPoints ? work / D : kTileMaxPoints;
     }
 };
 
@@ -1075,8 +1092,9 @@ struct TpRow {
     // constants of the owner
     double pd0, pd1, df0, df1;
     uint32_t pt, f;
-    // block inverse share
-    uint32_t j0, j1;
+    // block inverse share: columns [j0, j1); in staged launches [j0, jg) come
+    // from the LDS stage, where the row's column 0 sits at soff
+    uint32_t j0, j1, jg, soff;
     TB blk[NB];
     // segment constants (thread s < ts), masked at the fiber ends after the tables
     double sdf[D], sjd, sja, sjc;
@@ -1115,7 +1133,7 @@ __device__ __forceinline__ void tp_roles(const Tile& T, int& u, bool& act, bool&
     const int tid = threadIdx.x;
     u = split ? (tid & (SP - 1)) : tid;
     h = split ? (tid / SP) : 0;
-    act = u < (int)T.nu;
+    act = u < (int)T.nu && tid < TpCfg<NT>::work;
     own = act && h == 0;
 }
 
@@ -1167,11 +1185,11 @@ __device__ __forceinline__ void tp_row_constants(const FiberArgs& a, const Tile&
 
 // after the tables: the row's fiber, its block-inverse share, and the
 // segment's fiber (masking the JJ' couplings at the fiber ends)
-template <int D, typename TB, int NB, typename LDS>
+template <int D, typename TB, int NB, bool SG = false, typename LDS>
 __device__ __forceinline__ void tp_after_tables(const FiberArgs& a, const LDS& L, const Tile& T, const TB* B,
                                                 TpRow<D, TB, NB>& R)
 {
-    R.j0 = R.j1 = 0;
+    R.j0 = R.j1 = R.jg = R.soff = 0;
     R.k = R.fo = R.np = R.p = 0;
     if (R.act) {
         const uint32_t pt = (uint32_t)R.u / D;
@@ -1180,6 +1198,19 @@ __device__ __forceinline__ void tp_after_tables(const FiberArgs& a, const LDS& L
         R.np = (int)L.fo[R.k + 1] - R.fo;
         R.p = (int)pt - R.fo;
         if (B != nullptr && L.fuse[R.k]) share(0, (uint32_t)(D * R.np), R.split, R.h, R.j0, R.j1);
+        R.jg = R.j0;
+        if constexpr (SG) {
+            // the stage holds the tile's blocks from the 16-byte boundary at or
+            // below the first one, up to kStageBytes: whole columns of the
+            // row's block that lie inside it are read from LDS
+            constexpr uint64_t al = 16 / sizeof(TB), cap = kStageBytes / sizeof(TB);
+            const uint64_t e0 = L.fblk[0] - L.fblk[0] % al;
+            const uint64_t off = L.fblk[R.k] - e0;
+            const uint32_t bs = (uint32_t)(D * R.np);
+            const uint32_t jst = off >= cap ? 0u : (uint32_t)((cap - off) / bs < bs ? (cap - off) / bs : bs);
+            R.jg = jst < R.j0 ? R.j0 : jst > R.j1 ? R.j1 : jst;
+            R.soff = (uint32_t)off + (uint32_t)(R.u - D * R.fo);
+        }
     }
     R.sk = R.sp = 0;
     const int s = threadIdx.x;
@@ -1198,12 +1229,14 @@ __device__ __forceinline__ void tp_after_tables(const FiberArgs& a, const LDS& L
 // after the coupling gathers of the launch: loads return in order, so a gather
 // issued after them would wait for the whole block stream before the operator
 // could start; issued here, the stream overlaps the operator's LDS phases.
+// (Staged launches: the first NB columns after the staged ones.)
 template <int D, typename TB, int NB, typename LDS>
 __device__ __forceinline__ void tp_block_prefetch(const LDS& L, const TB* B, TpRow<D, TB, NB>& R)
 {
 #pragma unroll
     for (int v = 0; v < NB; ++v) R.blk[v] = TB(0);
-    if (R.j1 > R.j0) {
+    const uint32_t jb = R.jg;
+    if (R.j1 > jb) {
         const int bs = D * R.np;
         const TB* col = B + L.fblk[R.k] + (R.u - D * R.fo);
         if constexpr (std::is_same<TB, float>::value) {
@@ -1212,18 +1245,67 @@ __device__ __forceinline__ void tp_block_prefetch(const LDS& L, const TB* B, TpR
             const uint32_t jl = R.j1 - 1;
 #pragma unroll
             for (int v = 0; v < NB; ++v) {
-                const uint32_t j = R.j0 + v < jl ? R.j0 + v : jl;
+                const uint32_t j = jb + v < jl ? jb + v : jl;
                 const TB e = col[(size_t)bs * j];
-                R.blk[v] = R.j0 + v < R.j1 ? e : TB(0);
+                R.blk[v] = jb + v < R.j1 ? e : TB(0);
             }
         } else {
             // FP64: predicated (the branch-free form holds twice the
             // registers live and measures slower)
 #pragma unroll
             for (int v = 0; v < NB; ++v)
-                if (R.j0 + v < R.j1) R.blk[v] = col[(size_t)bs * (R.j0 + v)];
+                if (jb + v < R.j1) R.blk[v] = col[(size_t)bs * (jb + v)];
         }
     }
+}
+
+// The staging wavefronts of a staged launch: the tile's block inverses, from
+// the 16-byte boundary at or below the first block, copied into the stage by
+// global-to-LDS loads of 16 bytes per lane (global_load_lds_dwordx4: M0 holds
+// the wavefront's LDS address, lane l writes 16 bytes at M0 + 16 l), each
+// wavefront every kTpStageWaves-th KB, up to kStageBytes.  A copy may read up
+// to 15 bytes past the tile's last block, inside the buffer's spare elements.
+// Nothing is copied for a system that has converged.  The loads are issued
+// from inline assembly so that the compiler does not make the workgroup's
+// barriers wait for them (a fence after a global-to-LDS load waits for it):
+// the copy runs through the prologue, the gathers and the operator, and the
+// staging wavefronts wait for it themselves (tp_stage_drain) before the
+// barrier ahead of the block apply and before any return.  They issue it last
+// before the first barrier, on a path of their own, so that no wait of the
+// prologue's (all loads count, in order) stands between it and that barrier.  Loads the compiler does not count
+// only make its own waits stricter (loads complete in order); the loads write
+// no register, and the one wait state between the M0 write and the load that
+// reads it is an s_nop.
+template <typename TB>
+__device__ __forceinline__ void tp_stage_blocks(const KArgs& k, const TB* B, const Tile& T, unsigned char* stage)
+{
+    const uint64_t b0 = k.blkoff[T.fb], b1 = k.blkoff[T.fb + T.nft];
+    if (k.st_in[T.sys].done) return;
+    constexpr uint64_t al = 16 / sizeof(TB);
+    const uint64_t e0 = b0 - b0 % al;
+    const uint64_t all = (b1 - e0) * sizeof(TB);
+    const uint32_t nb = (uint32_t)(all < kStageBytes ? all : kStageBytes);
+    const unsigned char* src = reinterpret_cast<const unsigned char*>(B + e0);
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t w = (threadIdx.x - kTpThreads) / kWave;
+    const uint32_t base = (uint32_t)(size_t)(__attribute__((address_space(3))) unsigned char*)stage;
+    for (uint32_t o = 16 * kWave * w; o < nb; o += 16 * kWave * kTpStageWaves) {
+        const uint32_t lo = o + 16 * lane;
+        if (lo < nb) {
+            const uint32_t m0 = __builtin_amdgcn_readfirstlane(base + o);
+            asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                         :: "s"(m0), "v"(src + lo) : "memory", "m0");
+        }
+    }
+}
+
+// The staging wavefronts wait for their copies (before the barrier that
+// precedes any read of the stage, and before they end).
+template <bool SG>
+__device__ __forceinline__ void tp_stage_drain()
+{
+    if constexpr (SG)
+        if (threadIdx.x >= kTpThreads) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 // The packed layout's lane of this thread: the segment it solves and the
@@ -1263,7 +1345,7 @@ __device__ __forceinline__ void tp_pcr_prefetch(const FiberArgs& a, const LDS& L
 // The operator over the tile once L.x holds the operand and the helpers'
 // coupling sums are in L.hb/L.hc: forces, multipliers, L.y.  bsum/csum: the
 // owner's own shares.  Ends synchronised.
-template <int D, typename TB, int NB, bool PK, typename LDS>
+template <int D, typename TB, int NB, bool PK, bool SG = false, typename LDS>
 __device__ __forceinline__ void tp_operator(LDS& L, const Tile& T, const TpRow<D, TB, NB>& R, double bsum,
                                             double csum)
 {
@@ -1388,33 +1470,58 @@ __device__ __forceinline__ void tp_operator(LDS& L, const Tile& T, const TpRow<D
         const double yv = L.fsca[k] * v;
         L.y[R.u] = yv + L.x[R.u];
     }
+    tp_stage_drain<SG>();   // the barrier before the block apply reads the stage
     __syncthreads();
 }
 
 // out[u] = Binv xin over the tile: each thread its column share, the helper's
 // partial through LDS; copies where there is no usable block.  Ends synchronised.
-template <int D, typename TB, int NB, int CH = kApplyChunk, typename LDS>
+template <int D, typename TB, int NB, int CH = kApplyChunk, bool SG = false, typename LDS>
 __device__ __forceinline__ double tp_block_apply(LDS& L, const Tile& T, const TpRow<D, TB, NB>& R, const TB* B,
-                                                 const double* xin)
+                                                 const double* xin, const TB* stage = nullptr)
 {
     double acc[4] = { 0, 0, 0, 0 };
     const int k = R.k, fo = R.fo, bs = D * R.np;
     const bool use = R.act && B != nullptr && L.fuse[k];
+    uint32_t rot = 0;
     if (use && R.j1 > R.j0) {
+        const double* xs = xin + D * fo;
+        if constexpr (SG) {
+            // staged columns [j0, jg) from LDS; column j accumulates in slot
+            // (j - j0) & 3 as in the other launches, so that the result is the
+            // same bit for bit: the slots are rotated for the columns after jg
+            // and rotated back at the end
+            const TB* sl = stage + R.soff;
+            uint32_t j = R.j0;
+#pragma unroll 2
+            for (; j + 4 <= R.jg; j += 4) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[v] += (double)sl[bs * (j + v)] * xs[j + v];
+            }
+            rot = R.jg - j;
+            if (rot > 0) acc[0] += (double)sl[bs * j] * xs[j];
+            if (rot > 1) acc[1] += (double)sl[bs * (j + 1)] * xs[j + 1];
+            if (rot > 2) acc[2] += (double)sl[bs * (j + 2)] * xs[j + 2];
+            const double a0 = acc[0], a1 = acc[1], a2 = acc[2], a3 = acc[3];
+            acc[0] = rot == 0 ? a0 : rot == 1 ? a1 : rot == 2 ? a2 : a3;
+            acc[1] = rot == 0 ? a1 : rot == 1 ? a2 : rot == 2 ? a3 : a0;
+            acc[2] = rot == 0 ? a2 : rot == 1 ? a3 : rot == 2 ? a0 : a1;
+            acc[3] = rot == 0 ? a3 : rot == 1 ? a0 : rot == 2 ? a1 : a2;
+        }
         // Branch-free: every operand read stays inside the
         // fiber's rows (the index is clamped to the last column of this
         // thread's share) and is multiplied by a zero entry beyond it, so the
         // LDS reads of the loop are in flight together instead of one per
         // branch (16.1 instead of 17.5 us per launch on the 1000-fiber system).
-        const double* xs = xin + D * fo;
+        const uint32_t jb = R.jg;
         const uint32_t jl = R.j1 - 1;
 #pragma unroll
         for (int v = 0; v < NB; ++v) {
-            const uint32_t j = R.j0 + v < jl ? R.j0 + v : jl;
+            const uint32_t j = jb + v < jl ? jb + v : jl;
             acc[v & 3] += (double)R.blk[v] * xs[j];
         }
         const TB* col = B + L.fblk[k] + (R.u - D * fo);
-        for (uint32_t j = R.j0 + NB; j < R.j1; j += CH) {
+        for (uint32_t j = jb + NB; j < R.j1; j += CH) {
             TB c[CH];
             if constexpr (std::is_same<TB, float>::value) {
                 // FP32: clamped addresses and a select (no branch per load), the
@@ -1438,6 +1545,13 @@ __device__ __forceinline__ double tp_block_apply(LDS& L, const Tile& T, const Tp
                 acc[v & 3] += (double)c[v] * xs[jj];
             }
         }
+    }
+    if constexpr (SG) {
+        const double a0 = acc[0], a1 = acc[1], a2 = acc[2], a3 = acc[3];
+        acc[0] = rot == 0 ? a0 : rot == 1 ? a3 : rot == 2 ? a2 : a1;
+        acc[1] = rot == 0 ? a1 : rot == 1 ? a0 : rot == 2 ? a3 : a2;
+        acc[2] = rot == 0 ? a2 : rot == 1 ? a1 : rot == 2 ? a0 : a3;
+        acc[3] = rot == 0 ? a3 : rot == 1 ? a2 : rot == 2 ? a1 : a0;
     }
     const double part = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     if (R.split && R.act && R.h == 1) L.hb[R.u] = part;
@@ -1867,6 +1981,12 @@ __device__ void solid_tile_q(const KArgs& k, const Tile& T, SolidTileLds& S)
 template <int NT>
 constexpr int kTpWavesPerEu = NT == 256 ? 4 : 1;
 
+// The LDS stage of a staged launch (a separate array, so that LDS accesses to
+// the tile tables are known not to touch it and wait for no copy); 16 bytes
+// elsewhere.
+template <int NT>
+constexpr uint32_t kStageOf = NT == kTpStaged ? kStageBytes : 16u;
+
 template <int D, int PC, bool FIRST, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kTpWavesPerEu<NT>))) void k_tp(KArgs k)
 {
@@ -1879,10 +1999,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kTpWavesPerE
         SolidTileLds s;
     } sh;
     TpLds<D, NT>& L = sh.t;
+    constexpr bool SG = NT == kTpStaged;
+    __shared__ __attribute__((aligned(16))) unsigned char stage[kStageOf<NT>];
     const FiberArgs& a = k.fa;
     const Tile T = tile_begin<D>(k.ta, blockIdx.x);
     if (T.nft == 0) {
-        solid_tile_p<D, PC, FIRST, NT>(k, T, sh.s);
+        if constexpr (!SG) solid_tile_p<D, PC, FIRST, NT>(k, T, sh.s);
         return;
     }
     TpRow<D, TB, TpCfg<NT>::blk> R;
@@ -1892,11 +2014,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kTpWavesPerE
     double vr = 0, vt = 0, vx = 0, vrt = 0;
     double4 q1 = make_double4(0, 0, 0, 0);
     double vy = 0;
-    if (R.own) {
-        vr = k.r[g]; vt = k.t[g]; vx = k.x[g]; vrt = k.rt[g];
-        q1 = k.g1a[g];
-        if (FIRST) vy = k.g2[g].x;
-    }
     const Rec* rec;
     if constexpr (FIRST) rec = k.g2;
     else rec = k.g1a;
@@ -1904,29 +2021,49 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kTpWavesPerE
     Gath<TpCfg<NT>::prec, Rec> GC;
     GB.e = GB.e1 = GC.e = GC.e1 = 0;
     const int dd = R.u - D * (R.u / D);
-    tp_extents<D>(a, T, R.u, R.act, R.split, R.h, g, GB, GC);
-    gath_entries(GB, a.Bcol, a.Bval);
-    gath_entries(GC, a.Ccol, a.Cval);
-    gath_values(GB, rec, (uint32_t)D, (uint32_t)dd);
-    gath_values(GC, rec, 1u, 0u);
-    tp_row_constants<D, TB>(a, T, R);
-    tp_seg_constants<D, TB>(a, T, R);
-    SysScalars sc = state_load(k.st_in + T.sys);
-    tile_tables<PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
-    if (T.ws && threadIdx.x < T.nft) L.fsl[threadIdx.x] = k.ta.fslot[T.fb + threadIdx.x];
+    SysScalars sc;
+    if (SG && threadIdx.x >= kTpThreads) {
+        // staging wavefronts: the copy, issued last before the first barrier
+        // so that no wait of the prologue's falls between the two
+        tp_row_constants<D, TB>(a, T, R);
+        tp_seg_constants<D, TB>(a, T, R);
+        sc = state_load(k.st_in + T.sys);
+        if constexpr (SG && PC != CYMG_PRECOND_NONE) tp_stage_blocks<TB>(k, PcBlocks<PC>::get(k), T, stage);
+        __syncthreads();
+    } else {
+        if (R.own) {
+            vr = k.r[g]; vt = k.t[g]; vx = k.x[g]; vrt = k.rt[g];
+            q1 = k.g1a[g];
+            if (FIRST) vy = k.g2[g].x;
+        }
+        tp_extents<D>(a, T, R.u, R.act, R.split, R.h, g, GB, GC);
+        gath_entries(GB, a.Bcol, a.Bval);
+        gath_entries(GC, a.Ccol, a.Cval);
+        gath_values(GB, rec, (uint32_t)D, (uint32_t)dd);
+        gath_values(GC, rec, 1u, 0u);
+        tp_row_constants<D, TB>(a, T, R);
+        tp_seg_constants<D, TB>(a, T, R);
+        sc = state_load(k.st_in + T.sys);
+        tile_tables<PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
+        if (T.ws && threadIdx.x < T.nft) L.fsl[threadIdx.x] = k.ta.fslot[T.fb + threadIdx.x];
+        tp_partials<FIRST>(k, T, L.bc);
+        __syncthreads();
+    }
     const uint32_t nt = k.ta.ntiles;
-    tp_partials<FIRST>(k, T, L.bc);
-    __syncthreads();
     const bool first = T.t == T.tb;
     if (sc.done) {
         if (first && threadIdx.x == 0) state_store(k.st_out + T.sys, sc);
+        tp_stage_drain<SG>();
         return;
     }
     tp_scalars(sc, L.bc);
     if (first && threadIdx.x == 0) state_store(k.st_out + T.sys, sc);
-    if (sc.done) return;
-    tp_after_tables<D, TB>(a, L, T, PcBlocks<PC>::get(k), R);
-    if constexpr (NT == kTpThreads) tp_pcr_prefetch<D, TB>(a, L, T, R);
+    if (sc.done) {
+        tp_stage_drain<SG>();
+        return;
+    }
+    tp_after_tables<D, TB, TpCfg<NT>::blk, SG>(a, L, T, PcBlocks<PC>::get(k), R);
+    if constexpr (TpCfg<NT>::work == kTpThreads) tp_pcr_prefetch<D, TB>(a, L, T, R);
     // vector updates and phat; q1 = (shat, w, phat_old, z_old) of the row
     double rmax = 0.0, yi = vy, phi = 0.0;
     if (R.own) {
@@ -1957,7 +2094,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kTpWavesPerE
     }
     tp_block_prefetch<D, TB>(L, PcBlocks<PC>::get(k), R);
     __syncthreads();
-    tp_operator<D, TB, TpCfg<NT>::blk, NT == kTpThreads>(L, T, R, bsum, csum);
+    tp_operator<D, TB, TpCfg<NT>::blk, TpCfg<NT>::work == kTpThreads, SG>(L, T, R, bsum, csum);
     double dl = 0.0;
     if (R.own) {
         k.v[g] = L.y[R.u];
@@ -1965,11 +2102,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kTpWavesPerE
     }
     rmax = wave_max(rmax);
     dl = wave_sum(dl);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
+    if ((threadIdx.x & (kWave - 1)) == 0 && threadIdx.x < TpCfg<NT>::work) {
         L.wr[threadIdx.x / kWave] = rmax;
         L.wr[NW + threadIdx.x / kWave] = dl;
     }
-    const double zi = tp_block_apply<D, TB, TpCfg<NT>::blk, TpCfg<NT>::chunk>(L, T, R, PcBlocks<PC>::get(k), L.y);
+    const double zi = tp_block_apply<D, TB, TpCfg<NT>::blk, TpCfg<NT>::chunk, SG>(
+        L, T, R, PcBlocks<PC>::get(k), L.y, reinterpret_cast<const TB*>(stage));
     if (R.own) {
         k.g2[g] = make_double2(yi, zi);
         reinterpret_cast<double2*>(k.g1b + g)[1] = make_double2(phi, zi);
@@ -1996,10 +2134,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kTpWavesPerE
         SolidTileLds s;
     } sh;
     TpLds<D, NT>& L = sh.t;
+    constexpr bool SG = NT == kTpStaged;
+    __shared__ __attribute__((aligned(16))) unsigned char stage[kStageOf<NT>];
     const FiberArgs& a = k.fa;
     const Tile T = tile_begin<D>(k.ta, blockIdx.x);
     if (T.nft == 0) {
-        solid_tile_q<D, PC, NT>(k, T, sh.s);
+        if constexpr (!SG) solid_tile_q<D, PC, NT>(k, T, sh.s);
         return;
     }
     TpRow<D, TB, TpCfg<NT>::blk> R;
@@ -2007,35 +2147,49 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kTpWavesPerE
     const uint32_t g = T.U0 + R.u;
     double vr = 0, vv = 0, vx = 0, vph = 0, vrt = 0;
     double2 q2 = make_double2(0, 0);
-    if (R.own) {
-        vr = k.r[g]; vv = k.v[g]; vx = k.x[g]; vrt = k.rt[g];
-        q2 = k.g2[g];
-        vph = k.g1b[g].z;
-    }
     const double2* rec = k.g2;
     Gath<TpCfg<NT>::preb, double2> GB;
     Gath<TpCfg<NT>::prec, double2> GC;
     GB.e = GB.e1 = GC.e = GC.e1 = 0;
     const int dd = R.u - D * (R.u / D);
-    tp_extents<D>(a, T, R.u, R.act, R.split, R.h, g, GB, GC);
-    gath_entries(GB, a.Bcol, a.Bval);
-    gath_entries(GC, a.Ccol, a.Cval);
-    gath_values(GB, rec, (uint32_t)D, (uint32_t)dd);
-    gath_values(GC, rec, 1u, 0u);
-    tp_row_constants<D, TB>(a, T, R);
-    tp_seg_constants<D, TB>(a, T, R);
-    SysScalars sc = state_load(k.st_in + T.sys);
-    tile_tables<PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
-    if (T.ws && threadIdx.x < T.nft) L.fsl[threadIdx.x] = k.ta.fslot[T.fb + threadIdx.x];
+    SysScalars sc;
+    if (SG && threadIdx.x >= kTpThreads) {
+        // staging wavefronts: the copy, issued last before the first barrier
+        // so that no wait of the prologue's falls between the two
+        tp_row_constants<D, TB>(a, T, R);
+        tp_seg_constants<D, TB>(a, T, R);
+        sc = state_load(k.st_in + T.sys);
+        if constexpr (SG && PC != CYMG_PRECOND_NONE) tp_stage_blocks<TB>(k, PcBlocks<PC>::get(k), T, stage);
+        __syncthreads();
+    } else {
+        if (R.own) {
+            vr = k.r[g]; vv = k.v[g]; vx = k.x[g]; vrt = k.rt[g];
+            q2 = k.g2[g];
+            vph = k.g1b[g].z;
+        }
+        tp_extents<D>(a, T, R.u, R.act, R.split, R.h, g, GB, GC);
+        gath_entries(GB, a.Bcol, a.Bval);
+        gath_entries(GC, a.Ccol, a.Cval);
+        gath_values(GB, rec, (uint32_t)D, (uint32_t)dd);
+        gath_values(GC, rec, 1u, 0u);
+        tp_row_constants<D, TB>(a, T, R);
+        tp_seg_constants<D, TB>(a, T, R);
+        sc = state_load(k.st_in + T.sys);
+        tile_tables<PC != CYMG_PRECOND_NONE>(a, L, T, k.blkoff, k.useblk);
+        if (T.ws && threadIdx.x < T.nft) L.fsl[threadIdx.x] = k.ta.fslot[T.fb + threadIdx.x];
+        tq_partials(k, T, L.bc);
+        __syncthreads();
+    }
     const uint32_t nt = k.ta.ntiles;
-    tq_partials(k, T, L.bc);
-    __syncthreads();
     const bool first = T.t == T.tb;
     if (!sc.done) tq_scalars<PC>(sc, L.bc);
     if (first && threadIdx.x == 0) state_store(k.st_out + T.sys, sc);
-    if (sc.done) return;
-    tp_after_tables<D, TB>(a, L, T, PcBlocks<PC>::get(k), R);
-    if constexpr (NT == kTpThreads) tp_pcr_prefetch<D, TB>(a, L, T, R);
+    if (sc.done) {
+        tp_stage_drain<SG>();
+        return;
+    }
+    tp_after_tables<D, TB, TpCfg<NT>::blk, SG>(a, L, T, PcBlocks<PC>::get(k), R);
+    if constexpr (TpCfg<NT>::work == kTpThreads) tp_pcr_prefetch<D, TB>(a, L, T, R);
     double smax = 0.0, si = 0.0, shi = 0.0;
     if (R.own) {
         si = vr + (-sc.alpha) * vv;
@@ -2054,7 +2208,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kTpWavesPerE
     }
     tp_block_prefetch<D, TB>(L, PcBlocks<PC>::get(k), R);
     __syncthreads();
-    tp_operator<D, TB, TpCfg<NT>::blk, NT == kTpThreads>(L, T, R, bsum, csum);
+    tp_operator<D, TB, TpCfg<NT>::blk, TpCfg<NT>::work == kTpThreads, SG>(L, T, R, bsum, csum);
     double tt = 0, ts = 0, rts = 0, rtt = 0;
     if (R.own) {
         const double ti = L.y[R.u];
@@ -2069,7 +2223,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kTpWavesPerE
     ts = wave_sum(ts);
     rts = wave_sum(rts);
     rtt = wave_sum(rtt);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
+    if ((threadIdx.x & (kWave - 1)) == 0 && threadIdx.x < TpCfg<NT>::work) {
         const int wv = threadIdx.x / kWave;
         L.wr[wv] = smax;
         L.wr[NW + wv] = tt;
@@ -2077,7 +2231,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kTpWavesPerE
         L.wr[3 * NW + wv] = rts;
         L.wr[4 * NW + wv] = rtt;
     }
-    const double wi = tp_block_apply<D, TB, TpCfg<NT>::blk, TpCfg<NT>::chunk>(L, T, R, PcBlocks<PC>::get(k), L.y);
+    const double wi = tp_block_apply<D, TB, TpCfg<NT>::blk, TpCfg<NT>::chunk, SG>(
+        L, T, R, PcBlocks<PC>::get(k), L.y, reinterpret_cast<const TB*>(stage));
     if (R.own) reinterpret_cast<double2*>(k.g1b + g)[0] = make_double2(shi, wi);
     if (threadIdx.x == 0) {
         double m = L.wr[0], s0 = L.wr[NW], s1 = L.wr[2 * NW], s2 = L.wr[3 * NW], s3 = L.wr[4 * NW];
@@ -2426,6 +2581,15 @@ int tile_threads_of(const Context& c)
     return 512;
 }
 
+// staged launches (kTpStaged threads) for the 512-thread tiles of a
+// fiber-only layout: with FP32 blocks when CYMG_OPT_BLOCK_STAGE is 1 (the
+// default), with FP32 or FP64 blocks when it is 2
+static bool staged(const Context& c, int precond)
+{
+    const bool want = c.block_stage == 2 || (c.block_stage == 1 && precond == CYMG_PRECOND_FP32);
+    return want && precond != CYMG_PRECOND_NONE && c.nsol == 0 && tile_threads_of(c) == kTpThreads;
+}
+
 // the two-launch iteration takes fibers of at most TpCfg<512>::max_points
 // points (a tile of one fiber, a thread per unknown); a system with a longer
 // fiber runs the four-launch iteration
@@ -2453,12 +2617,17 @@ double krylov_time_launch(Context& c, int precond, int launches)
     tiled_buffers(c);
     const int pairs = std::max(1, launches / 2);
     const bool small = tile_threads_of(c) == 256;
+    const bool sg = staged(c, precond);
     auto go = [&](auto dim) -> double {
         constexpr int D = decltype(dim)::value;
         if (small) {
             if (precond == CYMG_PRECOND_FP64) return time_tiled<D, CYMG_PRECOND_FP64, 256>(c, pairs);
             if (precond == CYMG_PRECOND_FP32) return time_tiled<D, CYMG_PRECOND_FP32, 256>(c, pairs);
             return time_tiled<D, CYMG_PRECOND_NONE, 256>(c, pairs);
+        }
+        if (sg) {
+            if (precond == CYMG_PRECOND_FP64) return time_tiled<D, CYMG_PRECOND_FP64, kTpStaged>(c, pairs);
+            return time_tiled<D, CYMG_PRECOND_FP32, kTpStaged>(c, pairs);
         }
         if (precond == CYMG_PRECOND_FP64) return time_tiled<D, CYMG_PRECOND_FP64, 512>(c, pairs);
         if (precond == CYMG_PRECOND_FP32) return time_tiled<D, CYMG_PRECOND_FP32, 512>(c, pairs);
@@ -2478,17 +2647,20 @@ void krylov_solve(Context& c, int precond, const double* guess_dev, int max_iter
     if (two_launch(c) && !(precond == CYMG_PRECOND_FP32 && c.fp32_apply)) {
         tiled_buffers(c);
         const bool small = tile_threads_of(c) == 256;
+        const bool sg = staged(c, precond);
         auto go = [&](auto dim, auto nt) {
             constexpr int D = decltype(dim)::value, NT = decltype(nt)::value;
             if (precond == CYMG_PRECOND_FP64) run_tiled<D, CYMG_PRECOND_FP64, NT>(c, guess_dev, max_iter, tol, tol_mode, rep);
             else if (precond == CYMG_PRECOND_FP32) run_tiled<D, CYMG_PRECOND_FP32, NT>(c, guess_dev, max_iter, tol, tol_mode, rep);
-            else run_tiled<D, CYMG_PRECOND_NONE, NT>(c, guess_dev, max_iter, tol, tol_mode, rep);
+            else if constexpr (NT != kTpStaged) run_tiled<D, CYMG_PRECOND_NONE, NT>(c, guess_dev, max_iter, tol, tol_mode, rep);
         };
         if (c.dim == 3) {
             if (small) go(std::integral_constant<int, 3>(), std::integral_constant<int, 256>());
+            else if (sg) go(std::integral_constant<int, 3>(), std::integral_constant<int, kTpStaged>());
             else go(std::integral_constant<int, 3>(), std::integral_constant<int, 512>());
         } else {
             if (small) go(std::integral_constant<int, 2>(), std::integral_constant<int, 256>());
+            else if (sg) go(std::integral_constant<int, 2>(), std::integral_constant<int, kTpStaged>());
             else go(std::integral_constant<int, 2>(), std::integral_constant<int, 512>());
         }
         return;

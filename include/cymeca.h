@@ -446,9 +446,15 @@ int         cymg_get_matrix(cymg_context* ctx, int which, uint32_t* nnz, uint32_
      linear operator, which the two-launch iteration's recurrences need);
      1 applies them in FP32 arithmetic (operand rounded to FP32, FP32
      accumulation), in the solve (which then runs the four-launch iteration,
-     applying M^-1 directly) and in cymg_precondition */
+     applying M^-1 directly) and in cymg_precondition;
+   CYMG_OPT_BLOCK_STAGE: staged launches of the two-launch iteration's
+     512-thread tiles (fiber-only layouts): four more wavefronts per tile copy
+     the tile's block inverses into LDS (up to 128 KB per tile) from the start
+     of each launch, for the block apply to read there; same results bit for
+     bit.  1 (default) stages FP32 blocks (CYMG_PRECOND_FP32), 2 FP32 and
+     FP64 blocks, 0 none (the blocks are read from memory) */
 enum { CYMG_OPT_KRYLOV_LAUNCHES = 1, CYMG_OPT_BLOCK_INVERSE = 2, CYMG_OPT_TILE_POINTS = 3, CYMG_OPT_GRAPHS = 4,
-       CYMG_OPT_TILE_THREADS = 5, CYMG_OPT_GUESS_GUARD = 6, CYMG_OPT_FP32_APPLY = 7 };
+       CYMG_OPT_TILE_THREADS = 5, CYMG_OPT_GUESS_GUARD = 6, CYMG_OPT_FP32_APPLY = 7, CYMG_OPT_BLOCK_STAGE = 8 };
 int         cymg_set_option(cymg_context* ctx, int option, int value);
 int         cymg_get_option(cymg_context* ctx, int option, int* value);   /* the value in use */
 /* autotuner (no counterpart in the reference): on the current system (after
