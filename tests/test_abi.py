@@ -32,3 +32,25 @@ def test_invalid_arguments_without_gpu(pkg):
     assert L.cymg_destroy(None) == pkg.ERR_INVALID
     assert L.cymg_clear(None) == pkg.ERR_INVALID
     assert L.cymg_last_error(None) == b"null context"
+
+
+def _header_enum(prefix):
+    """Name -> value of the header's enumerators that start with `prefix`."""
+    text = open(os.path.join(ROOT, "include", "cymeca.h")).read()
+    out = {}
+    for body in re.findall(r"enum\s*\{([^}]*)\}", text):
+        for name, value in re.findall(r"(\w+)\s*=\s*(-?\d+)", body):
+            if name.startswith(prefix):
+                out[name] = int(value)
+    return out
+
+
+def test_option_numbers_match_the_header(pkg):
+    """Every CYMG_OPT_* of the header has the package constant of the same
+    number (OPT_*), and the package defines no other option."""
+    opts = _header_enum("CYMG_OPT_")
+    assert "CYMG_OPT_BLOCK_STAGE" in opts and "CYMG_OPT_FP32_APPLY" in opts
+    for name, value in opts.items():
+        assert getattr(pkg, name[len("CYMG_"):]) == value, name
+    mine = {n for n in dir(pkg) if n.startswith("OPT_")}
+    assert mine == {n[len("CYMG_"):] for n in opts}
