@@ -91,3 +91,15 @@ def test_staged_launch_time(pkg, scenes):
     print(f"full-work launch: staged {1e3 * staged:.2f} us, unstaged {1e3 * plain:.2f} us", flush=True)
     assert staged > 0 and plain > 0
     gpu.close()
+
+
+def test_staged_fp32_full_size_same_bits(pkg, scenes):
+    """The 1000-fiber crosslinked system (240 tiles), one step with FP32 blocks
+    at relative residual 1e-6: the default (staged) launches and the unstaged
+    ones give the same points, tensions and iterations."""
+    sc = scenes.config2(seed=2)
+    print("staged FP32, 1000 fibers", flush=True)
+    ref, _ = _steps(pkg, sc, 0, 2, 1e-6, steps=1)
+    got, reps = _steps(pkg, sc, 1, 2, 1e-6, steps=1)
+    assert reps[0]["converged"]
+    _same_bits(ref, got)
