@@ -1272,10 +1272,10 @@ __device__ __forceinline__ void tp_block_prefetch(const LDS& L, const TB* B, TpR
 // staging wavefronts wait for it themselves (tp_stage_drain) before the
 // barrier ahead of the block apply and before any return.  They issue it last
 // before the first barrier, on a path of their own, so that no wait of the
-// prologue's (all loads count, in order) stands between it and that barrier.  Loads the compiler does not count
-// only make its own waits stricter (loads complete in order); the loads write
-// no register, and the one wait state between the M0 write and the load that
-// reads it is an s_nop.
+// prologue's (all loads count, in order) stands between it and that barrier.
+// Loads the compiler does not count only make its own waits stricter (loads
+// complete in order); the loads write no register, and the one wait state
+// between the M0 write and the load that reads it is an s_nop.
 template <typename TB>
 __device__ __forceinline__ void tp_stage_blocks(const KArgs& k, const TB* B, const Tile& T, unsigned char* stage)
 {
