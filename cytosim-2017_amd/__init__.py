@@ -712,12 +712,14 @@ class Meca:
         self._chk(self.L.cymg_set_option(self.h, option, value))
 
     def autotune(self, reps=20):
-        """Pick the block-inverse method and tile shape by timing them on the
-        current system (after setup); returns the choices."""
+        """Pick the block-inverse method, the tile shape and the staging of
+        the blocks by timing them on the current system (after setup);
+        returns the choices."""
         t, b, ms = C.c_int(), C.c_int(), C.c_double()
         self._chk(self.L.cymg_autotune(self.h, int(reps), C.byref(t), C.byref(b), C.byref(ms)))
         return {"tile_points": t.value, "tile_threads": self.get_option(OPT_TILE_THREADS),
-                "block_inverse": b.value, "launch_ms": ms.value}
+                "block_inverse": b.value, "block_stage": self.get_option(OPT_BLOCK_STAGE),
+                "launch_ms": ms.value}
 
     def get_option(self, option):
         v = C.c_int()

@@ -1690,6 +1690,20 @@ int cymg_autotune(cymg_context* ctx, int reps, int* tile_points, int* block_inve
             c.tile_points = (uint32_t)best_tile;
             c.tile_threads = (uint32_t)best_threads;
             build_tiles(c);
+            // staged launches of 512-thread tiles (fiber-only layouts): the
+            // current precision's blocks staged or not, whichever launch is
+            // shorter (the results are the same bit for bit); the other
+            // precision keeps the default (FP32 blocks staged)
+            if (precision != CYMG_PRECOND_NONE && c.nsol == 0 && tile_threads_of(c) == 512) {
+                const int keep = c.block_stage;
+                c.block_stage = 0;
+                const double plain = krylov_time_launch(c, precision, reps);
+                c.block_stage = 2;
+                const double staged = krylov_time_launch(c, precision, reps);
+                if (precision == CYMG_PRECOND_FP64) c.block_stage = staged < plain ? 2 : 1;
+                else c.block_stage = staged < plain ? (keep == 2 ? 2 : 1) : 0;
+                best = std::min(plain, staged);
+            }
         }
         c.last_iters[0] = c.last_iters[1] = c.last_iters[2] = 0;
         if (tile_points) *tile_points = best_tile;

@@ -461,7 +461,9 @@ int         cymg_get_option(cymg_context* ctx, int option, int* value);   /* the
    setup), times the block-inverse methods (Gauss-Jordan in registers / in
    LDS / blocked on the matrix cores) and the tile shapes of the two-launch iteration (target points per
    tile 16..170 and the layout default, for 512- and 256-thread tiles, `reps` launches each) and keeps the
-   fastest; reports the choices and the chosen launch time (ms) */
+   fastest; with 512-thread tiles of a fiber-only layout it then times the current precision's launch
+   staged and unstaged and sets CYMG_OPT_BLOCK_STAGE accordingly (FP64 blocks: 2 or 1; FP32
+   blocks: 1 (or 2) or 0); reports the choices and the chosen launch time (ms) */
 int         cymg_autotune(cymg_context* ctx, int reps, int* tile_points, int* block_inverse, double* launch_ms);
 
 /* device timings of the last solve, milliseconds */

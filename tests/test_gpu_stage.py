@@ -103,3 +103,29 @@ def test_staged_fp32_full_size_same_bits(pkg, scenes):
     got, reps = _steps(pkg, sc, 1, 2, 1e-6, steps=1)
     assert reps[0]["converged"]
     _same_bits(ref, got)
+
+
+def test_autotuned_staging_same_bits(pkg, scenes):
+    """cymg_autotune with FP32 blocks chooses the tile shape and whether to
+    stage them (1) or not (0); a solve on its choice gives the bits of the
+    unstaged launches on the same tiles."""
+    sc = scenes.config2(seed=49, nfib=200, nlinks=1500, radius=6.0)
+    gpu = gpu_from_scene(pkg, sc)
+    gpu.setup(sc.gauss(0))
+    gpu.compute_preconditioner(pkg.PRECOND_FP32)
+    t = gpu.autotune(reps=6)
+    print(f"autotune with FP32 blocks: {t}", flush=True)
+    assert t["block_stage"] in (0, 1) and t["launch_ms"] > 0
+    r = gpu.solve(1e-9, sc.gauss(0), tol_mode=1, precondition=2)
+    assert r["converged"]
+    tuned = gpu.fiber_points().copy()
+    gpu.close()
+    ref = gpu_from_scene(pkg, sc)
+    ref.set_option(pkg.OPT_TILE_THREADS, t["tile_threads"])
+    ref.set_option(pkg.OPT_TILE_POINTS, t["tile_points"])
+    ref.set_option(pkg.OPT_BLOCK_INVERSE, t["block_inverse"])
+    ref.set_option(pkg.OPT_BLOCK_STAGE, 0)
+    r0 = ref.solve(1e-9, sc.gauss(0), tol_mode=1, precondition=2)
+    assert r0["iterations"] == r["iterations"]
+    assert np.array_equal(ref.fiber_points(), tuned)
+    ref.close()
